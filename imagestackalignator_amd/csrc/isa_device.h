@@ -1,0 +1,103 @@
+// Device-side helpers shared by the gfx950 kernels of libisa_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ISA_CFA_RED 0
+#define ISA_CFA_GREEN 1
+#define ISA_CFA_BLUE 2
+
+// Bayer pattern passed by value (replaces the reference's __constant__ c_cfaPattern).
+struct Cfa {
+    int c[4];
+    __device__ __forceinline__ int at(int y, int x) const { return c[(y % 2) * 2 + (x % 2)]; }
+};
+
+template <class T> __device__ __forceinline__ T* row_ptr(T* base, int pitch, int y) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)pitch * (size_t)y);
+}
+template <class T> __device__ __forceinline__ const T* row_ptr(const T* base, int pitch, int y) {
+    return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)pitch * (size_t)y);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ---------------------------------------------------------------------------
+// Texture-unit emulation: normalised coordinates, x = u*W - 0.5, i = floor(x),
+// alpha = frac(x) (optionally rounded to 8 fractional bits). Mirror addressing
+// is the half-sample-symmetric reflection with period 2W.
+#define ISA_ADDR_CLAMP 0
+#define ISA_ADDR_MIRROR 1
+
+__device__ __forceinline__ int tex_addr(int i, int n, int mode) {
+    if (mode == ISA_ADDR_CLAMP) return clampi(i, 0, n - 1);
+    int p = 2 * n;
+    i %= p;
+    if (i < 0) i += p;
+    return i < n ? i : p - 1 - i;
+}
+
+__device__ __forceinline__ float tex_frac(float a, int quant) { return quant ? rintf(a * 256.0f) / 256.0f : a; }
+
+// Bilinear fetch of channel c of a `ch`-channel float image.
+template <int CH>
+__device__ __forceinline__ float tex_linear(const float* img, int pitch, int w, int h, int c, float u, float v, int mode,
+                                            int quant) {
+    float x = u * (float)w - 0.5f;
+    float y = v * (float)h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = tex_frac(x - fx, quant), b = tex_frac(y - fy, quant);
+    int x0 = (int)fx, y0 = (int)fy;
+    int xa = tex_addr(x0, w, mode), xb = tex_addr(x0 + 1, w, mode);
+    int ya = tex_addr(y0, h, mode), yb = tex_addr(y0 + 1, h, mode);
+    const float* ra = row_ptr(img, pitch, ya);
+    const float* rb = row_ptr(img, pitch, yb);
+    float t00 = ra[xa * CH + c], t10 = ra[xb * CH + c];
+    float t01 = rb[xa * CH + c], t11 = rb[xb * CH + c];
+    float oma = 1.0f - a, omb = 1.0f - b;
+    return oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11;
+}
+
+// Bilinear fetch of all CH channels at once (same weights for every channel).
+template <int CH>
+__device__ __forceinline__ void tex_linear_all(const float* img, int pitch, int w, int h, float u, float v, int mode,
+                                               int quant, float* out) {
+    float x = u * (float)w - 0.5f;
+    float y = v * (float)h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = tex_frac(x - fx, quant), b = tex_frac(y - fy, quant);
+    int x0 = (int)fx, y0 = (int)fy;
+    int xa = tex_addr(x0, w, mode), xb = tex_addr(x0 + 1, w, mode);
+    int ya = tex_addr(y0, h, mode), yb = tex_addr(y0 + 1, h, mode);
+    const float* ra = row_ptr(img, pitch, ya);
+    const float* rb = row_ptr(img, pitch, yb);
+    float oma = 1.0f - a, omb = 1.0f - b;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        float t00 = ra[xa * CH + c], t10 = ra[xb * CH + c];
+        float t01 = rb[xa * CH + c], t11 = rb[xb * CH + c];
+        out[c] = oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel formulas shared by the unfused (one-kernel-per-reference-kernel)
+// and fused entry points, so that both evaluate identical operations.
+
+// sRGB OETF after NaN->0 and clamp to [0,1] (R:Kernels/kernel.cu:271-313).
+__device__ __forceinline__ float srgb_prepared(float v) {
+    if (isnan(v)) v = 0.0f;
+    v = fmaxf(fminf(v, 1.0f), 0.0f);
+    if (v <= 0.0031308f) return 12.92f * v;
+    return (1.0f + 0.055f) * powf(v, 1.0f / 2.4f) - 0.055f;
+}
+
+// Accumulation weight of offset (px, py) for inverse covariance (k0, k1, k2)
+// = (Omega11, Omega22, Omega12); non-finite -> cross fallback
+// (R:Kernels/DeBayerKernels.cu:335-338).
+__device__ __forceinline__ float kernel_weight(int px, int py, float k0, float k1, float k2) {
+    float w = (float)(px * px) * k0 + (float)(2 * px * py) * k2 + (float)(py * py) * k1;
+    w = expf(-0.5f * w);
+    if (!isfinite(w)) w = (px * py == 0) ? 1.0f : 0.0f;
+    return w;
+}

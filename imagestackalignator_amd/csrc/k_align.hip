@@ -1,0 +1,838 @@
+// Tile alignment: tile extraction, SSD search, findMinimum, shift up-sampling,
+// the fused per-level tracking launch, and the FP64 ShiftMinimizer.
+#include <algorithm>
+#include <vector>
+
+#include "isa_device.h"
+#include "isa_host.h"
+
+namespace {
+
+constexpr int kMaxShift = 8;  // largest supported MaxShift (search window (2M+1)^2 <= 289)
+
+// Integer offset of a tile (R:Kernels/kernel.cu:183-202 and :238-261): round(preshift + R(theta)(-b) + R(theta)c - c).
+__device__ __forceinline__ void tile_offset(int tx, int ty, int tile, int w, int h, float bsx, float bsy, float rot,
+                                            float psx, float psy, int& ox, int& oy) {
+    float sf = sinf(rot), cf = cosf(rot);
+    float sx = psx + (cf * -bsx - sf * -bsy);
+    float sy = psy + (sf * -bsx + cf * -bsy);
+    float pcx = (float)(tx * tile + tile / 2 - w / 2);
+    float pcy = (float)(ty * tile + tile / 2 - h / 2);
+    sx += cf * pcx - sf * pcy - pcx;
+    sy += sf * pcx + cf * pcy - pcy;
+    ox = (int)roundf(sx);
+    oy = (int)roundf(sy);
+}
+
+// SSD core. r: T x T reference interior (LDS, row stride T); t: B x B track block (row stride B).
+// D(s) = (sum r^2 + sum_window t^2(s)) - 2 sum r t(.+s), in FP64 with a fixed order:
+// each row term is an fma chain over x, rows are then added in increasing y.
+// P: T * S * S doubles, TT: B * S doubles, R2: T doubles, D: S * S floats (all LDS).
+template <int M>
+__device__ void ssd_core(const float* r, const float* t, int T, double* P, double* TT, double* R2, float* D, int tid,
+                         int nthreads) {
+    constexpr int S = 2 * M + 1;
+    const int B = T + 2 * M;
+    // cross terms: task (y, sy)
+    for (int task = tid; task < T * S; task += nthreads) {
+        int y = task / S, iy = task - y * S;  // sy = iy - M
+        const float* trow = t + (y + iy) * B;
+        const float* rrow = r + y * T;
+        double acc[S], win[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) acc[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < S - 1; ++i) win[i] = (double)trow[i];
+        for (int x = 0; x < T; ++x) {
+            win[S - 1] = (double)trow[x + S - 1];
+            double rv = (double)rrow[x];
+#pragma unroll
+            for (int i = 0; i < S; ++i) acc[i] = fma(rv, win[i], acc[i]);
+#pragma unroll
+            for (int i = 0; i < S - 1; ++i) win[i] = win[i + 1];
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i) P[(y * S + iy) * S + i] = acc[i];
+    }
+    // window energies of the track rows: TT[j][i] = sum_x t[j][x+i]^2
+    for (int task = tid; task < B * S; task += nthreads) {
+        int j = task / S, i = task - j * S;
+        const float* trow = t + j * B + i;
+        double acc = 0.0;
+        for (int x = 0; x < T; ++x) {
+            double v = (double)trow[x];
+            acc = fma(v, v, acc);
+        }
+        TT[j * S + i] = acc;
+    }
+    for (int y = tid; y < T; y += nthreads) {
+        const float* rrow = r + y * T;
+        double acc = 0.0;
+        for (int x = 0; x < T; ++x) {
+            double v = (double)rrow[x];
+            acc = fma(v, v, acc);
+        }
+        R2[y] = acc;
+    }
+    __syncthreads();
+    for (int s = tid; s < S * S; s += nthreads) {
+        int iy = s / S, ix = s - iy * S;
+        double r2 = 0.0, box = 0.0, cc = 0.0;
+        for (int y = 0; y < T; ++y) {
+            r2 += R2[y];
+            box += TT[(y + iy) * S + ix];
+            cc += P[(y * S + iy) * S + ix];
+        }
+        D[s] = (float)((r2 + box) - 2.0 * cc);
+    }
+    __syncthreads();
+}
+
+// findMinimum (R:Kernels/kernel.cu:413-546) on an S x S map.
+__device__ void find_min_core(const float* v, int M, float threshold, float& cx, float& cy) {
+    const float FA11[9] = {1.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f, 2.0f / 4.0f, -4.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f,
+                           -2.0f / 4.0f, 1.0f / 4.0f};
+    const float FA22[9] = {1.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f, -2.0f / 4.0f, -4.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f,
+                           2.0f / 4.0f, 1.0f / 4.0f};
+    const float FA12[9] = {1.0f / 4.0f, 0.0f, -1.0f / 4.0f, 0.0f, 0.0f, 0.0f, -1.0f / 4.0f, 0.0f, 1.0f / 4.0f};
+    const float Fb1[9] = {-1.0f / 8.0f, 0.0f, 1.0f / 8.0f, -2.0f / 8.0f, 0.0f, 2.0f / 8.0f, -1.0f / 8.0f, 0.0f,
+                          1.0f / 8.0f};
+    const float Fb2[9] = {-1.0f / 8.0f, -2.0f / 8.0f, -1.0f / 8.0f, 0.0f, 0.0f, 0.0f, 1.0f / 8.0f, 2.0f / 8.0f,
+                          1.0f / 8.0f};
+    const int S = 2 * M + 1, n = S * S;
+    float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
+    int mi = -1;
+    for (int i = 0; i < n; ++i) {
+        float val = v[i];
+        mx = fmaxf(mx, val);
+        if (val < mn) { mn = val; mi = i; }
+    }
+    cy = (float)(mi / S);
+    cx = (float)mi - cy * (float)S;
+    if (cx < 1 || cy < 1 || cx >= (float)(2 * M) || cy >= (float)(2 * M)) {
+        cx = 0.0f;
+        cy = 0.0f;
+    } else {
+        float A11 = 0, A22 = 0, A12 = 0, b1 = 0, b2 = 0;
+        for (int i = 0; i < 9; ++i) {
+            int off = (i / 3 - 1) * S + (i % 3 - 1);
+            float img = v[mi + off];
+            A11 += FA11[i] * img;
+            A22 += FA22[i] * img;
+            A12 += FA12[i] * img;
+            b1 += Fb1[i] * img;
+            b2 += Fb2[i] * img;
+        }
+        A11 = fmaxf(A11, 0.0f);
+        A22 = fmaxf(A22, 0.0f);
+        float det = A11 * A22 - A12 * A12;
+        if (det < 0) { A12 = 0; det = A11 * A22; }
+        if (det != 0) {
+            float mux = (A22 * b1 - A12 * b2) / det;
+            float muy = (A11 * b2 - A12 * b1) / det;
+            if (fabsf(mux) > 1) mux = 0;
+            if (fabsf(muy) > 1) muy = 0;
+            cx -= mux;
+            cy -= muy;
+        }
+        cx -= (float)M;
+        cy -= (float)M;
+    }
+    if (threshold + mn > mx) { cx = 0.0f; cy = 0.0f; }
+}
+
+size_t ssd_smem(int T, int M) {
+    int S = 2 * M + 1, B = T + 2 * M;
+    size_t f = (size_t)T * T + (size_t)B * B + (size_t)S * S;  // floats
+    size_t d = (size_t)T * S * S + (size_t)B * S + (size_t)T;   // doubles
+    return ((f * 4 + 7) & ~(size_t)7) + d * 8;
+}
+
+struct SsdLds {
+    float *r, *t, *D;
+    double *P, *TT, *R2;
+};
+
+__device__ __forceinline__ SsdLds ssd_lds(char* smem, int T, int M) {
+    int S = 2 * M + 1, B = T + 2 * M;
+    SsdLds L;
+    L.r = (float*)smem;
+    L.t = L.r + T * T;
+    L.D = L.t + B * B;
+    size_t f = (size_t)T * T + (size_t)B * B + (size_t)S * S;
+    L.P = (double*)(smem + ((f * 4 + 7) & ~(size_t)7));
+    L.TT = L.P + T * S * S;
+    L.R2 = L.TT + B * S;
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// Fused PatchTracker.Track for every pair of a level at once: grid (tiles, pairs).
+// Level images: frames[f] (device array of pointers), row pitch img_pitch bytes.
+// shifts: [meas][tiles] float2 (meas_stride floats per measurement); pair p uses measurement index pk[p].
+struct TrackArgs {
+    const float* const* frames;
+    int img_pitch, w, h, tile, cx, cy;
+    const int* pa;  // reference frame of pair p
+    const int* pb;  // tracked frame of pair p
+    const int* pk;  // measurement index of pair p
+    const float* base;  // per frame: bsx, bsy, rot (already divided by the level)
+    float* shifts;
+    size_t meas_stride;
+    float threshold;
+    float* shift_images;  // optional: [pair][tile][S*S]
+};
+
+template <int M>
+__global__ void k_track_level(TrackArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    const int T = a.tile, B = T + 2 * M;
+    constexpr int S = 2 * M + 1;
+    SsdLds L = ssd_lds(smem, T, M);
+    const int tile_idx = blockIdx.x, p = blockIdx.y;
+    const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
+    const int fa = a.pa[p], fb = a.pb[p], k = a.pk[p];
+    float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
+    const float psx = sh[0], psy = sh[1];
+    int rox, roy, tox, toy;
+    tile_offset(tx, ty, T, a.w, a.h, a.base[fa * 3], a.base[fa * 3 + 1], a.base[fa * 3 + 2], 0.0f, 0.0f, rox, roy);
+    tile_offset(tx, ty, T, a.w, a.h, a.base[fb * 3], a.base[fb * 3 + 1], a.base[fb * 3 + 2], psx, psy, tox, toy);
+    const float* ia = a.frames[fa];
+    const float* ib = a.frames[fb];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < T * T; i += nt) {
+        int py = i / T, px = i - py * T;
+        int gx = clampi(tx * T + px + M + rox, 0, a.w - 1), gy = clampi(ty * T + py + M + roy, 0, a.h - 1);
+        L.r[i] = row_ptr(ia, a.img_pitch, gy)[gx];
+    }
+    for (int i = tid; i < B * B; i += nt) {
+        int py = i / B, px = i - py * B;
+        int gx = clampi(tx * T + px + tox, 0, a.w - 1), gy = clampi(ty * T + py + toy, 0, a.h - 1);
+        L.t[i] = row_ptr(ib, a.img_pitch, gy)[gx];
+    }
+    __syncthreads();
+    ssd_core<M>(L.r, L.t, T, L.P, L.TT, L.R2, L.D, tid, nt);
+    if (a.shift_images)
+        for (int s = tid; s < S * S; s += nt) a.shift_images[((size_t)p * a.cx * a.cy + tile_idx) * S * S + s] = L.D[s];
+    if (tid == 0) {
+        float cx, cy;
+        find_min_core(L.D, M, a.threshold, cx, cy);
+        sh[0] = psx + cx;
+        sh[1] = psy + cy;
+    }
+}
+
+// Stand-alone SSD over pre-extracted (T+2M)^2 tiles (isa_tile_ssd).
+template <int M>
+__global__ void k_tile_ssd(const float* __restrict__ rt, const float* __restrict__ tt, float* out, int T) {
+    extern __shared__ __align__(16) char smem[];
+    const int B = T + 2 * M;
+    constexpr int S = 2 * M + 1;
+    SsdLds L = ssd_lds(smem, T, M);
+    const int t = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const float* r = rt + (size_t)t * B * B;
+    const float* k = tt + (size_t)t * B * B;
+    for (int i = tid; i < T * T; i += nt) {
+        int py = i / T, px = i - py * T;
+        L.r[i] = r[(py + M) * B + px + M];
+    }
+    for (int i = tid; i < B * B; i += nt) L.t[i] = k[i];
+    __syncthreads();
+    ssd_core<M>(L.r, L.t, T, L.P, L.TT, L.R2, L.D, tid, nt);
+    for (int s = tid; s < S * S; s += nt) out[(size_t)t * S * S + s] = L.D[s];
+}
+
+int block_threads(int T, int M) {
+    int work = T * (2 * M + 1);
+    int bt = ((work + 63) / 64) * 64;
+    return std::max(64, std::min(bt, 512));
+}
+
+__global__ void k_tiles_border(const float* img, int pitch, float* tiles, int w, int h, int M, int T, int cx, int cy,
+                               float bsx, float bsy, float rot) {
+    int B = T + 2 * M;
+    int px = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y * blockDim.y + threadIdx.y;
+    int t = blockIdx.z * blockDim.z + threadIdx.z;
+    if (px >= B || py >= B || t >= cx * cy) return;
+    size_t o = (size_t)t * B * B + py * B + px;
+    if (px < M || py < M || px >= T + M || py >= T + M) {
+        tiles[o] = 0.0f;
+        return;
+    }
+    int ty = t / cx, tx = t - ty * cx, ox, oy;
+    tile_offset(tx, ty, T, w, h, bsx, bsy, rot, 0.0f, 0.0f, ox, oy);
+    tiles[o] = row_ptr(img, pitch, clampi(ty * T + py + oy, 0, h - 1))[clampi(tx * T + px + ox, 0, w - 1)];
+}
+
+__global__ void k_tiles_preshift(const float* img, int pitch, float* tiles, const float* pre, int pre_pitch, int w,
+                                 int h, int M, int T, int cx, int cy, float bsx, float bsy, float rot) {
+    int B = T + 2 * M;
+    int px = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y * blockDim.y + threadIdx.y;
+    int t = blockIdx.z * blockDim.z + threadIdx.z;
+    if (px >= B || py >= B || t >= cx * cy) return;
+    int ty = t / cx, tx = t - ty * cx, ox, oy;
+    const float* ps = row_ptr(pre, pre_pitch, ty) + tx * 2;
+    tile_offset(tx, ty, T, w, h, bsx, bsy, rot, ps[0], ps[1], ox, oy);
+    tiles[(size_t)t * B * B + py * B + px] =
+        row_ptr(img, pitch, clampi(ty * T + py + oy, 0, h - 1))[clampi(tx * T + px + ox, 0, w - 1)];
+}
+
+__global__ void k_find_minimum(const float* simg, float* coords, int pitch, int M, int cx, int cy, float thr) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cx * cy) return;
+    int S = 2 * M + 1;
+    float x, y;
+    find_min_core(simg + (size_t)t * S * S, M, thr, x, y);
+    int ty = t / cx, tx = t - ty * cx;
+    float* o = row_ptr(coords, pitch, ty) + tx * 2;
+    o[0] = x;
+    o[1] = y;
+}
+
+__global__ void k_add_shifts(float* pre, int pre_pitch, const float* c, int c_pitch, int cx, int cy) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= cx * 2 || y >= cy) return;
+    row_ptr(pre, pre_pitch, y)[x] += row_ptr(c, c_pitch, y)[x];
+}
+
+// UpSampleShifts (R:Kernels/kernel.cu:551-598).
+__global__ void k_upsample(const float* in, float* out, int in_pitch, int out_pitch, int old_level, int new_level,
+                           int ocx, int ocy, int ncx, int ncy, int old_tile, int new_tile) {
+    int bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y * blockDim.y + threadIdx.y;
+    if (bx >= ncx || by >= ncy) return;
+    float factor = (float)old_level * (float)old_tile / (float)(new_level * new_tile);
+    float ox = (float)bx / factor, oy = (float)by / factor;
+    int x0 = (int)floorf(ox), x1 = (int)ceilf(ox), y0 = (int)floorf(oy), y1 = (int)ceilf(oy);
+    x0 = min(x0, ocx - 1); x1 = min(x1, ocx - 1);
+    y0 = min(y0, ocy - 1); y1 = min(y1, ocy - 1);
+    const float* r0 = row_ptr(in, in_pitch, y0);
+    const float* r1 = row_ptr(in, in_pitch, y1);
+    float* o = row_ptr(out, out_pitch, by) + bx * 2;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        float mm = r0[x0 * 2 + c], pm = r0[x1 * 2 + c], mp = r1[x0 * 2 + c], pp = r1[x1 * 2 + c];
+        float t1 = mm + (pm - mm) * (1.0f - ((float)x1 - ox));
+        float t2 = mp + (pp - mp) * (1.0f - ((float)x1 - ox));
+        float v = t1 + (t2 - t1) * (1.0f - ((float)y1 - oy));
+        v *= (float)old_level / (float)new_level;
+        o[c] = v;
+    }
+}
+
+// All measurements at once: in/out are [meas][tiles] float2 with meas_stride floats per measurement
+// (dense rows of old / new count). Same arithmetic as k_upsample.
+__global__ void k_upsample_all(const float* in, float* out, size_t meas_stride, int old_level, int new_level, int ocx,
+                               int ocy, int ncx, int ncy, int old_tile, int new_tile) {
+    int bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (bx >= ncx || by >= ncy) return;
+    const float* i0 = in + (size_t)k * meas_stride;
+    float* o0 = out + (size_t)k * meas_stride;
+    float factor = (float)old_level * (float)old_tile / (float)(new_level * new_tile);
+    float ox = (float)bx / factor, oy = (float)by / factor;
+    int x0 = (int)floorf(ox), x1 = (int)ceilf(ox), y0 = (int)floorf(oy), y1 = (int)ceilf(oy);
+    x0 = min(x0, ocx - 1); x1 = min(x1, ocx - 1);
+    y0 = min(y0, ocy - 1); y1 = min(y1, ocy - 1);
+    const float* r0 = i0 + (size_t)y0 * ocx * 2;
+    const float* r1 = i0 + (size_t)y1 * ocx * 2;
+    float* o = o0 + ((size_t)by * ncx + bx) * 2;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        float mm = r0[x0 * 2 + c], pm = r0[x1 * 2 + c], mp = r1[x0 * 2 + c], pp = r1[x1 * 2 + c];
+        float t1 = mm + (pm - mm) * (1.0f - ((float)x1 - ox));
+        float t2 = mp + (pp - mp) * (1.0f - ((float)x1 - ox));
+        float v = t1 + (t2 - t1) * (1.0f - ((float)y1 - oy));
+        v *= (float)old_level / (float)new_level;
+        o[c] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ShiftMinimizer (R:PEFStudioDX/ShiftCollection.cs:738-822) in FP64, one 64-lane workgroup per tile.
+// Measurement row k estimates x_b - x_a (x_j = position of frame j, x_0 = 0). Minimising
+// sum_k |x_b - x_a - d_k|^2 over the active rows gives the same solution as the reference's
+// r = (A^T A)^-1 A^T d (x = prefix sums of r); the normal matrix is the reduced graph Laplacian,
+// solved by Cholesky. Singular <=> the active measurement graph is disconnected (status -1, r = 0).
+constexpr int kMinFrames = 64;
+constexpr int kMinThreads = 64;
+
+struct MinArgs {
+    float* meas;  // [m][tiles] float2, meas_stride floats per measurement
+    size_t meas_stride;
+    float* one_to_one;  // [tile][n1] float2
+    int* status;
+    int* iters;
+    const int* rows;  // [m][2] (a, b)
+    int tiles, n, m;
+};
+
+__global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    const int n = a.n, n1 = n - 1, m = a.m, tid = threadIdx.x;
+    double* d = (double*)smem;          // m x 2
+    double* L = d + 2 * m;              // n1 x n1
+    double* rhs = L + n1 * n1;          // n1 x 2
+    double* x = rhs + 2 * n1;           // n x 2
+    double* red_v = x + 2 * n;          // kMinThreads
+    int* red_i = (int*)(red_v + kMinThreads);  // kMinThreads
+    char* active = (char*)(red_i + kMinThreads);
+    __shared__ int s_parent[kMinFrames];
+    __shared__ int s_flag;
+    const int t = blockIdx.x;
+    for (int k = tid; k < m; k += kMinThreads) {
+        const float* v = a.meas + (size_t)k * a.meas_stride + (size_t)t * 2;
+        d[2 * k] = (double)v[0];
+        d[2 * k + 1] = (double)v[1];
+        active[k] = 1;
+    }
+    __syncthreads();
+    int status = 0, it;
+    for (it = 0; it < 10; ++it) {
+        // connectivity of the active measurement graph (union-find, one lane)
+        if (tid == 0) {
+            for (int i = 0; i < n; ++i) s_parent[i] = i;
+            for (int k = 0; k < m; ++k) {
+                if (!active[k]) continue;
+                int u = a.rows[2 * k], v = a.rows[2 * k + 1];
+                while (s_parent[u] != u) u = s_parent[u];
+                while (s_parent[v] != v) v = s_parent[v];
+                if (u != v) s_parent[u < v ? v : u] = u < v ? u : v;
+            }
+            int conn = 1;
+            for (int i = 0; i < n && conn; ++i) {
+                int u = i;
+                while (s_parent[u] != u) u = s_parent[u];
+                if (u != 0) conn = 0;
+            }
+            s_flag = conn;
+        }
+        __syncthreads();
+        if (!s_flag) {
+            status = -1;
+            for (int i = tid; i < 2 * n; i += kMinThreads) x[i] = 0.0;
+            __syncthreads();
+            break;
+        }
+        // reduced Laplacian (nodes 1..n-1) and right-hand side; lane i owns row i
+        for (int i = tid; i < n1; i += kMinThreads) {
+            for (int j = 0; j < n1; ++j) L[i * n1 + j] = 0.0;
+            double bx = 0.0, by = 0.0;
+            const int node = i + 1;
+            for (int k = 0; k < m; ++k) {
+                if (!active[k]) continue;
+                int ra = a.rows[2 * k], rb = a.rows[2 * k + 1];
+                if (rb == node) {
+                    L[i * n1 + i] += 1.0;
+                    if (ra != 0) L[i * n1 + ra - 1] -= 1.0;
+                    bx += d[2 * k];
+                    by += d[2 * k + 1];
+                } else if (ra == node) {
+                    L[i * n1 + i] += 1.0;
+                    if (rb != 0) L[i * n1 + rb - 1] -= 1.0;
+                    bx -= d[2 * k];
+                    by -= d[2 * k + 1];
+                }
+            }
+            rhs[2 * i] = bx;
+            rhs[2 * i + 1] = by;
+        }
+        __syncthreads();
+        // Cholesky, lower triangle in place
+        for (int j = 0; j < n1; ++j) {
+            if (tid == 0) L[j * n1 + j] = sqrt(L[j * n1 + j]);
+            __syncthreads();
+            const double djj = L[j * n1 + j];
+            for (int i = j + 1 + tid; i < n1; i += kMinThreads) L[i * n1 + j] /= djj;
+            __syncthreads();
+            for (int i = j + 1 + tid; i < n1; i += kMinThreads) {
+                const double lij = L[i * n1 + j];
+                for (int c = j + 1; c <= i; ++c) L[i * n1 + c] -= lij * L[c * n1 + j];
+            }
+            __syncthreads();
+        }
+        // forward / backward substitution, one lane per component
+        if (tid < 2) {
+            const int c = tid;
+            for (int i = 0; i < n1; ++i) {
+                double s = rhs[2 * i + c];
+                for (int k = 0; k < i; ++k) s -= L[i * n1 + k] * rhs[2 * k + c];
+                rhs[2 * i + c] = s / L[i * n1 + i];
+            }
+            for (int i = n1 - 1; i >= 0; --i) {
+                double s = rhs[2 * i + c];
+                for (int k = i + 1; k < n1; ++k) s -= L[k * n1 + i] * rhs[2 * k + c];
+                rhs[2 * i + c] = s / L[i * n1 + i];
+            }
+            x[c] = 0.0;
+            for (int i = 0; i < n1; ++i) x[2 * (i + 1) + c] = rhs[2 * i + c];
+        }
+        __syncthreads();
+        // largest residual above 1 px^2 (first index on ties, as the sequential scan)
+        double best = 1.0;
+        int bi = -1;
+        for (int k = tid; k < m; k += kMinThreads) {
+            int ra = a.rows[2 * k], rb = a.rows[2 * k + 1];
+            double ex = d[2 * k] - (x[2 * rb] - x[2 * ra]), ey = d[2 * k + 1] - (x[2 * rb + 1] - x[2 * ra + 1]);
+            double dist = ex * ex + ey * ey;
+            if (dist > best) { best = dist; bi = k; }
+        }
+        red_v[tid] = best;
+        red_i[tid] = bi;
+        __syncthreads();
+        if (tid == 0) {
+            double bv = 1.0;
+            int idx = -1;
+            for (int l = 0; l < kMinThreads; ++l) {
+                int li = red_i[l];
+                if (li < 0) continue;
+                if (red_v[l] > bv || (red_v[l] == bv && idx >= 0 && li < idx)) { bv = red_v[l]; idx = li; }
+            }
+            if (idx >= 0) {
+                d[2 * idx] = 0.0;
+                d[2 * idx + 1] = 0.0;
+                active[idx] = 0;
+            }
+            s_flag = idx;
+        }
+        __syncthreads();
+        status = s_flag;
+        if (status < 0) break;
+    }
+    // outputs: r_i = x_{i+1} - x_i; every measurement := x_b - x_a (A_safe r)
+    for (int i = tid; i < n1; i += kMinThreads) {
+        float* o = a.one_to_one + ((size_t)t * n1 + i) * 2;
+        o[0] = (float)(x[2 * (i + 1)] - x[2 * i]);
+        o[1] = (float)(x[2 * (i + 1) + 1] - x[2 * i + 1]);
+    }
+    for (int k = tid; k < m; k += kMinThreads) {
+        int ra = a.rows[2 * k], rb = a.rows[2 * k + 1];
+        float* v = a.meas + (size_t)k * a.meas_stride + (size_t)t * 2;
+        v[0] = (float)(x[2 * rb] - x[2 * ra]);
+        v[1] = (float)(x[2 * rb + 1] - x[2 * ra + 1]);
+    }
+    if (tid == 0) {
+        if (a.status) a.status[t] = status;
+        if (a.iters) a.iters[t] = it < 10 ? it + 1 : 10;
+    }
+}
+
+size_t minimize_smem(int n, int m) {
+    int n1 = n - 1;
+    return (size_t)(2 * m + n1 * n1 + 2 * n1 + 2 * n + kMinThreads) * 8 + kMinThreads * 4 + m + 16;
+}
+
+// getOptimalShifts (R:Kernels/ShiftMinimizerKernels.cu:178-218).
+__global__ void k_optimal_shifts(float* out, int pitch, const float* r, int n, int cx, int cy, int ref, int trk) {
+    int tx = blockIdx.x * blockDim.x + threadIdx.x, ty = blockIdx.y * blockDim.y + threadIdx.y;
+    if (tx >= cx || ty >= cy) return;
+    int n1 = n - 1;
+    const float* rr = r + (size_t)(tx + ty * cx) * n1 * 2;
+    float sx = 0.0f, sy = 0.0f;
+    if (ref < trk)
+        for (int i = ref; i < trk; ++i) { sx += rr[2 * i]; sy += rr[2 * i + 1]; }
+    else if (trk < ref)
+        for (int i = trk; i < ref; ++i) { sx -= rr[2 * i]; sy -= rr[2 * i + 1]; }
+    float* o = row_ptr(out, pitch, ty) + tx * 2;
+    o[0] = sx;
+    o[1] = sy;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+namespace isa {
+
+int shift_tables(int n, int ref, int strategy, int block_in, int* pairs, int* index, int* rows) {
+    if (n < 2 || n > kMinFrames || ref < 0 || ref >= n) return -ISA_ERR_ARG;
+    int block = block_in >= n ? n - 1 : block_in;
+    int n1 = n - 1, m;
+    switch (strategy) {
+        case ISA_STRATEGY_FULL: m = (int)((n - 1) * ((float)n / 2.0f)); break;
+        case ISA_STRATEGY_ONLY_ON_REFERENCE: m = n - 1; break;
+        case ISA_STRATEGY_ON_REFERENCE_BLOCK: m = (int)((block - 1) * ((float)block / 2.0f)) + block * (n - block); break;
+        case ISA_STRATEGY_BLOCKS: m = (int)(-0.5 * block * (block - 2 * n + 1)); break;
+        default: return -ISA_ERR_ARG;
+    }
+    if (block < 1 && strategy >= ISA_STRATEGY_ON_REFERENCE_BLOCK) return -ISA_ERR_ARG;
+    std::vector<int> pa, pb, idx((size_t)n * n, -1);
+    if (strategy == ISA_STRATEGY_FULL || strategy == ISA_STRATEGY_BLOCKS) {
+        int dmax = strategy == ISA_STRATEGY_FULL ? n1 : block;
+        if (strategy == ISA_STRATEGY_FULL) {
+            for (int a = 0; a < n; ++a)
+                for (int b = a + 1; b < n; ++b) { pa.push_back(a); pb.push_back(b); }
+        } else {
+            for (int s = 0; s < block; ++s)
+                for (int line = 0; line < n1 - s; ++line) { pa.push_back(line); pb.push_back(line + s + 1); }
+        }
+        int c = 0;
+        for (int d = 1; d <= dmax; ++d)
+            for (int f = 0; f + d < n; ++f) idx[(size_t)f * n + f + d] = c++;
+    } else if (strategy == ISA_STRATEGY_ONLY_ON_REFERENCE) {
+        int c = 0;
+        for (int k = 0; k < n; ++k)
+            if (k != ref) { pa.push_back(ref); pb.push_back(k); idx[(size_t)ref * n + k] = c++; }
+    } else {
+        int first, last;
+        if (ref < n / 2) { first = std::max(0, ref - block / 2); last = first + block - 1; }
+        else { last = std::min(n - 1, ref + block / 2); first = last - block + 1; }
+        if (first < 0 || last >= n) return -ISA_ERR_ARG;
+        int c = 0;
+        for (int s = 0; s < block - 1; ++s)
+            for (int line = 0; line < block - 1 - s; ++line) {
+                pa.push_back(line + first); pb.push_back(line + first + s + 1);
+                idx[(size_t)(line + first) * n + line + first + s + 1] = c++;
+            }
+        for (int i = 0; i < first; ++i)
+            for (int tb = first; tb <= last; ++tb) { pa.push_back(tb); pb.push_back(i); idx[(size_t)tb * n + i] = c++; }
+        for (int i = last + 1; i < n; ++i)
+            for (int tb = first; tb <= last; ++tb) { pa.push_back(tb); pb.push_back(i); idx[(size_t)tb * n + i] = c++; }
+    }
+    if ((int)pa.size() != m) {
+        set_error("shift tables: pair count %d does not match shift count %d", (int)pa.size(), m);
+        return -ISA_ERR_ARG;
+    }
+    for (int p = 0; p < m; ++p) {
+        int k = idx[(size_t)pa[p] * n + pb[p]];
+        if (k < 0 || k >= m) return -ISA_ERR_ARG;
+        if (pairs) { pairs[2 * p] = pa[p]; pairs[2 * p + 1] = pb[p]; }
+        if (rows) { rows[2 * k] = pa[p]; rows[2 * k + 1] = pb[p]; }
+    }
+    if (index) std::copy(idx.begin(), idx.end(), index);
+    return m;
+}
+
+template <int M>
+static int launch_track_level_m(const TrackArgs& a, int pairs, hipStream_t s) {
+    size_t smem = ssd_smem(a.tile, M);
+    k_track_level<M><<<dim3(a.cx * a.cy, pairs), block_threads(a.tile, M), smem, s>>>(a);
+    return check_launch("track_level");
+}
+
+int launch_track_level(const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
+                       const int* pa, const int* pb, const int* pk, int pairs, const float* base, float* shifts,
+                       size_t meas_stride, float threshold, float* shift_images, hipStream_t s) {
+    if (ssd_smem(tile, M) > 160 * 1024) {
+        set_error("track: tile %d with max shift %d needs too much LDS", tile, M);
+        return ISA_ERR_UNSUPPORTED;
+    }
+    TrackArgs a{frames, img_pitch, w, h, tile, cx, cy, pa, pb, pk, base, shifts, meas_stride, threshold,
+                shift_images};
+    switch (M) {
+        case 1: return launch_track_level_m<1>(a, pairs, s);
+        case 2: return launch_track_level_m<2>(a, pairs, s);
+        case 3: return launch_track_level_m<3>(a, pairs, s);
+        case 4: return launch_track_level_m<4>(a, pairs, s);
+        case 5: return launch_track_level_m<5>(a, pairs, s);
+        case 6: return launch_track_level_m<6>(a, pairs, s);
+        case 7: return launch_track_level_m<7>(a, pairs, s);
+        case 8: return launch_track_level_m<8>(a, pairs, s);
+    }
+    set_error("track: max shift %d outside [1, %d]", M, kMaxShift);
+    return ISA_ERR_UNSUPPORTED;
+}
+
+int launch_upsample(const float* in, float* out, int in_pitch, int out_pitch, int ol, int nl, int ocx, int ocy, int ncx,
+                    int ncy, int ot, int nt, hipStream_t s) {
+    k_upsample<<<dim3(div_up(ncx, 16), div_up(ncy, 16)), dim3(16, 16), 0, s>>>(in, out, in_pitch, out_pitch, ol, nl,
+                                                                               ocx, ocy, ncx, ncy, ot, nt);
+    return check_launch("upsample_shifts");
+}
+
+int launch_upsample_all(const float* in, float* out, size_t meas_stride, int m, int ol, int nl, int ocx, int ocy,
+                        int ncx, int ncy, int ot, int nt, hipStream_t s) {
+    k_upsample_all<<<dim3(div_up(ncx, 16), div_up(ncy, 16), m), dim3(16, 16), 0, s>>>(in, out, meas_stride, ol, nl, ocx,
+                                                                                      ocy, ncx, ncy, ot, nt);
+    return check_launch("upsample_shifts_all");
+}
+
+int launch_minimize(float* meas, size_t meas_stride, float* one_to_one, int* status, int* iters, const int* rows,
+                    int tiles, int n, int m, hipStream_t s) {
+    size_t smem = minimize_smem(n, m);
+    if (n > kMinFrames || smem > 160 * 1024) {
+        set_error("minimize: %d frames / %d measurements exceed the supported size", n, m);
+        return ISA_ERR_UNSUPPORTED;
+    }
+    MinArgs a{meas, meas_stride, one_to_one, status, iters, rows, tiles, n, m};
+    k_minimize<<<tiles, kMinThreads, smem, s>>>(a);
+    return check_launch("minimize_shifts");
+}
+
+int launch_optimal_shifts(float* out, int pitch, const float* r, int n, int cx, int cy, int ref, int trk,
+                          hipStream_t s) {
+    k_optimal_shifts<<<dim3(div_up(cx, 32), div_up(cy, 8)), dim3(32, 8), 0, s>>>(out, pitch, r, n, cx, cy, ref, trk);
+    return check_launch("get_optimal_shifts");
+}
+
+}  // namespace isa
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+int isa_convert_to_tiles_border(const float* img_dev, int img_pitch, float* tiles_dev, int width, int height,
+                                int max_shift, int tile_size, int tile_count_x, int tile_count_y,
+                                const float base_shift[2], float base_rotation, isa_stream_t stream) {
+    ISA_REQUIRE(img_dev && tiles_dev && width > 0 && height > 0 && max_shift >= 0 && tile_size > 0 &&
+                    tile_count_x > 0 && tile_count_y > 0,
+                "isa_convert_to_tiles_border: bad arguments");
+    int B = tile_size + 2 * max_shift;
+    k_tiles_border<<<dim3(isa::div_up(B, 24), B, isa::div_up(tile_count_x * tile_count_y, 8)), dim3(24, 1, 8), 0,
+                     isa::to_stream(stream)>>>(img_dev, img_pitch, tiles_dev, width, height, max_shift, tile_size,
+                                               tile_count_x, tile_count_y, base_shift[0], base_shift[1], base_rotation);
+    return isa::check_launch("isa_convert_to_tiles_border");
+}
+
+int isa_convert_to_tiles_preshift(const float* img_dev, int img_pitch, float* tiles_dev, const float* preshift2_dev,
+                                  int preshift_pitch, int width, int height, int max_shift, int tile_size,
+                                  int tile_count_x, int tile_count_y, const float base_shift[2], float base_rotation,
+                                  isa_stream_t stream) {
+    ISA_REQUIRE(img_dev && tiles_dev && preshift2_dev && width > 0 && height > 0 && max_shift >= 0 && tile_size > 0 &&
+                    tile_count_x > 0 && tile_count_y > 0,
+                "isa_convert_to_tiles_preshift: bad arguments");
+    int B = tile_size + 2 * max_shift;
+    k_tiles_preshift<<<dim3(isa::div_up(B, 24), B, isa::div_up(tile_count_x * tile_count_y, 8)), dim3(24, 1, 8), 0,
+                       isa::to_stream(stream)>>>(img_dev, img_pitch, tiles_dev, preshift2_dev, preshift_pitch, width,
+                                                 height, max_shift, tile_size, tile_count_x, tile_count_y,
+                                                 base_shift[0], base_shift[1], base_rotation);
+    return isa::check_launch("isa_convert_to_tiles_preshift");
+}
+
+int isa_tile_ssd(const float* ref_tiles_dev, const float* track_tiles_dev, float* shift_image_dev, int max_shift,
+                 int tile_size, int tile_count, isa_stream_t stream) {
+    ISA_REQUIRE(ref_tiles_dev && track_tiles_dev && shift_image_dev && tile_size > 0 && tile_count > 0,
+                "isa_tile_ssd: bad arguments");
+    ISA_REQUIRE(max_shift >= 1 && max_shift <= kMaxShift, "isa_tile_ssd: max_shift must be in [1, 8]");
+    size_t smem = ssd_smem(tile_size, max_shift);
+    if (smem > 160 * 1024) {
+        isa::set_error("isa_tile_ssd: tile too large for LDS");
+        return ISA_ERR_UNSUPPORTED;
+    }
+    hipStream_t s = isa::to_stream(stream);
+    int bt = block_threads(tile_size, max_shift);
+    switch (max_shift) {
+        case 1: k_tile_ssd<1><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+        case 2: k_tile_ssd<2><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+        case 3: k_tile_ssd<3><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+        case 4: k_tile_ssd<4><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+        case 5: k_tile_ssd<5><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+        case 6: k_tile_ssd<6><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+        case 7: k_tile_ssd<7><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+        default: k_tile_ssd<8><<<tile_count, bt, smem, s>>>(ref_tiles_dev, track_tiles_dev, shift_image_dev, tile_size); break;
+    }
+    return isa::check_launch("isa_tile_ssd");
+}
+
+int isa_find_minimum(const float* shift_image_dev, float* coords2_dev, int coords_pitch, int max_shift,
+                     int tile_count_x, int tile_count_y, float threshold, isa_stream_t stream) {
+    ISA_REQUIRE(shift_image_dev && coords2_dev && max_shift >= 0 && tile_count_x > 0 && tile_count_y > 0,
+                "isa_find_minimum: bad arguments");
+    int n = tile_count_x * tile_count_y;
+    k_find_minimum<<<isa::div_up(n, 128), 128, 0, isa::to_stream(stream)>>>(shift_image_dev, coords2_dev, coords_pitch,
+                                                                           max_shift, tile_count_x, tile_count_y,
+                                                                           threshold);
+    return isa::check_launch("isa_find_minimum");
+}
+
+int isa_add_shifts(float* preshift2_dev, int preshift_pitch, const float* coords2_dev, int coords_pitch,
+                   int tile_count_x, int tile_count_y, isa_stream_t stream) {
+    ISA_REQUIRE(preshift2_dev && coords2_dev && tile_count_x > 0 && tile_count_y > 0, "isa_add_shifts: bad arguments");
+    k_add_shifts<<<dim3(isa::div_up(2 * tile_count_x, 32), isa::div_up(tile_count_y, 8)), dim3(32, 8), 0,
+                   isa::to_stream(stream)>>>(preshift2_dev, preshift_pitch, coords2_dev, coords_pitch, tile_count_x,
+                                             tile_count_y);
+    return isa::check_launch("isa_add_shifts");
+}
+
+int isa_upsample_shifts(const float* in2_dev, float* out2_dev, int in_pitch, int out_pitch, int old_level,
+                        int new_level, int old_count_x, int old_count_y, int new_count_x, int new_count_y,
+                        int old_tile_size, int new_tile_size, isa_stream_t stream) {
+    ISA_REQUIRE(in2_dev && out2_dev && old_level > 0 && new_level > 0 && old_count_x > 0 && old_count_y > 0 &&
+                    new_count_x > 0 && new_count_y > 0 && old_tile_size > 0 && new_tile_size > 0,
+                "isa_upsample_shifts: bad arguments");
+    return isa::launch_upsample(in2_dev, out2_dev, in_pitch, out_pitch, old_level, new_level, old_count_x, old_count_y,
+                                new_count_x, new_count_y, old_tile_size, new_tile_size, isa::to_stream(stream));
+}
+
+int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+              int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
+              float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
+              isa_stream_t stream) {
+    // PatchTracker.Track through the fused level kernel with one pair (frame 0 = reference, frame 1 = tracked).
+    ISA_REQUIRE(img_track_dev && img_ref_dev && preshift2_dev && width > 0 && height > 0 && tile_size > 0,
+                "isa_track: bad arguments");
+    ISA_REQUIRE(max_shift >= 1 && max_shift <= kMaxShift, "isa_track: max_shift must be in [1, 8]");
+    int cx = (width - 2 * max_shift) / tile_size, cy = (height - 2 * max_shift) / tile_size;
+    ISA_REQUIRE(cx > 0 && cy > 0, "isa_track: image smaller than one tile");
+    hipStream_t s = isa::to_stream(stream);
+    struct Scratch {
+        const float* frames[2];
+        int pa, pb, pk, pad;
+        float base[6];
+    } h{{img_ref_dev, img_track_dev}, 0, 1, 0, 0,
+        {base_shift_ref[0], base_shift_ref[1], base_rotation_ref, base_shift_track[0], base_shift_track[1],
+         base_rotation_track}};
+    const size_t head = (sizeof(Scratch) + 255) & ~(size_t)255;
+    const size_t row = (size_t)cx * 2 * sizeof(float);
+    char* d = nullptr;
+    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d, head + row * cy, s), "isa_track: allocation"));
+    float* dn = (float*)(d + head);
+    const Scratch* sd = (const Scratch*)d;
+    int rc = isa::check_hip(hipMemcpyAsync(d, &h, sizeof(Scratch), hipMemcpyHostToDevice, s), "isa_track: upload");
+    if (rc == ISA_OK)
+        rc = isa::check_hip(hipMemcpy2DAsync(dn, row, preshift2_dev, preshift_pitch, row, cy, hipMemcpyDeviceToDevice, s),
+                            "isa_track: copy");
+    if (rc == ISA_OK)
+        rc = isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
+                                     &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, s);
+    if (rc == ISA_OK)
+        rc = isa::check_hip(hipMemcpy2DAsync(preshift2_dev, preshift_pitch, dn, row, row, cy, hipMemcpyDeviceToDevice, s),
+                            "isa_track: copy back");
+    // the host copy `h` must outlive the upload: synchronise before returning
+    int rs = isa::check_hip(hipStreamSynchronize(s), "isa_track: sync");
+    hipError_t fe = hipFreeAsync(d, s);
+    if (rc == ISA_OK) rc = rs;
+    if (rc == ISA_OK) rc = isa::check_hip(fe, "isa_track: free");
+    return rc;
+}
+
+int isa_shift_tables(int frame_count, int reference_index, int strategy, int block_size, int* pairs_out,
+                     int* index_out, int* rows_out) {
+    return isa::shift_tables(frame_count, reference_index, strategy, block_size, pairs_out, index_out, rows_out);
+}
+
+int isa_minimize_shifts(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev, int* iters_dev,
+                        int tile_count, int frame_count, int reference_index, int strategy, int block_size,
+                        isa_stream_t stream) {
+    ISA_REQUIRE(measured2_dev && one_to_one2_dev && tile_count > 0 && meas_pitch >= tile_count * 8,
+                "isa_minimize_shifts: bad arguments");
+    std::vector<int> rows(2 * 4096);
+    int m = isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, nullptr);
+    ISA_REQUIRE(m > 0 && m <= 4096, "isa_minimize_shifts: invalid strategy / frame count");
+    isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, rows.data());
+    hipStream_t s = isa::to_stream(stream);
+    int* d_rows = nullptr;
+    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d_rows, 2 * m * sizeof(int), s), "isa_minimize_shifts: allocation"));
+    int rc = isa::check_hip(hipMemcpyAsync(d_rows, rows.data(), 2 * m * sizeof(int), hipMemcpyHostToDevice, s),
+                            "isa_minimize_shifts: upload");
+    if (rc == ISA_OK)
+        rc = isa::launch_minimize(measured2_dev, meas_pitch / sizeof(float), one_to_one2_dev, status_dev, iters_dev,
+                                  d_rows, tile_count, frame_count, m, s);
+    int rs = isa::check_hip(hipStreamSynchronize(s), "isa_minimize_shifts: sync");  // `rows` must outlive the upload
+    hipError_t fe = hipFreeAsync(d_rows, s);
+    if (rc == ISA_OK) rc = rs;
+    if (rc == ISA_OK) rc = isa::check_hip(fe, "isa_minimize_shifts: free");
+    return rc;
+}
+
+int isa_get_optimal_shifts(float* out2_dev, int out_pitch, const float* one_to_one2_dev, int frame_count,
+                           int tile_count_x, int tile_count_y, int reference_image, int image_to_track,
+                           isa_stream_t stream) {
+    ISA_REQUIRE(out2_dev && one_to_one2_dev && frame_count >= 2 && tile_count_x > 0 && tile_count_y > 0 &&
+                    reference_image >= 0 && reference_image < frame_count && image_to_track >= 0 &&
+                    image_to_track < frame_count,
+                "isa_get_optimal_shifts: bad arguments");
+    return isa::launch_optimal_shifts(out2_dev, out_pitch, one_to_one2_dev, frame_count, tile_count_x, tile_count_y,
+                                      reference_image, image_to_track, isa::to_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,526 @@
+// Image-domain kernels: debayer, gamma/grey, Gaussian, box resize, structure
+// tensor, kernel parameters, erosion, weighting. gfx950, FP32.
+#include <vector>
+
+#include "isa_device.h"
+#include "isa_host.h"
+
+namespace {
+
+struct DebayerArgs {
+    Cfa cfa;
+    float black[3], scale[3], div[3];
+};
+
+// Green estimate at a red/blue or green site (R:Kernels/DeBayerKernels.cu:53-149).
+// RAW(x, y) returns the raw value as float.
+template <class RawF>
+__device__ __forceinline__ float green_at(int x, int y, const DebayerArgs& a, RawF raw) {
+    int site = a.cfa.at(y, x);
+    if (site == ISA_CFA_GREEN) return (raw(x, y) - a.black[1]) * a.scale[1];
+    if (site != ISA_CFA_RED && site != ISA_CFA_BLUE) return 0.0f;
+    int c = site == ISA_CFA_RED ? 0 : 2;
+    float bc = a.black[c], sc = a.scale[c], bg = a.black[1], sg = a.scale[1];
+    float p = (raw(x, y) - bc) * sc;
+    float xm2 = (raw(x - 2, y) - bc) * sc, xm1 = (raw(x - 1, y) - bg) * sg;
+    float xp1 = (raw(x + 1, y) - bg) * sg, xp2 = (raw(x + 2, y) - bc) * sc;
+    float ym2 = (raw(x, y - 2) - bc) * sc, ym1 = (raw(x, y - 1) - bg) * sg;
+    float yp1 = (raw(x, y + 1) - bg) * sg, yp2 = (raw(x, y + 2) - bc) * sc;
+    float gx = 0.5f * fabsf(xp1 - xm1);
+    float gy = 0.5f * fabsf(yp1 - ym1);
+    float lx = 0.25f * fabsf(2.0f * p - xm2 - xp2);
+    float ly = 0.25f * fabsf(2.0f * p - ym2 - yp2);
+    float ix = 0.125f * (-xm2 + 4.0f * xm1 + 2.0f * p + 4.0f * xp1 - xp2);
+    float iy = 0.125f * (-ym2 + 4.0f * ym1 + 2.0f * p + 4.0f * yp1 - yp2);
+    float w = (gy + ly) / (gx + gy + lx + ly + 0.000000001f);
+    return w * ix + (1.0f - w) * iy;
+}
+
+// Red and blue by colour-difference interpolation against green
+// (R:Kernels/DeBayerKernels.cu:151-231). G(x, y) returns the green plane.
+template <class RawF, class GreenF>
+__device__ __forceinline__ void redblue_at(int x, int y, const DebayerArgs& a, RawF raw, GreenF G, float& r, float& b) {
+    int site = a.cfa.at(y, x);
+    int row = a.cfa.at(y, x + 1);
+    float g = G(x, y);
+    auto RC = [&](int c, int xx, int yy) { return (raw(xx, yy) - a.black[c]) * a.scale[c]; };
+    r = 0.0f;
+    b = 0.0f;
+    if (site == ISA_CFA_GREEN) {
+        int ch = row == ISA_CFA_RED ? 0 : 2, cv = row == ISA_CFA_RED ? 2 : 0;
+        float hx = g + 0.5f * ((RC(ch, x - 1, y) - G(x - 1, y)) + (RC(ch, x + 1, y) - G(x + 1, y)));
+        float vy = g + 0.5f * ((RC(cv, x, y - 1) - G(x, y - 1)) + (RC(cv, x, y + 1) - G(x, y + 1)));
+        if (row == ISA_CFA_RED) { r = hx; b = vy; } else { b = hx; r = vy; }
+    } else if (site == ISA_CFA_RED || site == ISA_CFA_BLUE) {
+        int own = site == ISA_CFA_RED ? 0 : 2, other = site == ISA_CFA_RED ? 2 : 0;
+        float v = RC(own, x, y);
+        float d = g + 0.25f * ((RC(other, x - 1, y - 1) - G(x - 1, y - 1)) + (RC(other, x + 1, y - 1) - G(x + 1, y - 1)) +
+                               (RC(other, x + 1, y + 1) - G(x + 1, y + 1)) + (RC(other, x - 1, y + 1) - G(x - 1, y + 1)));
+        if (site == ISA_CFA_RED) { r = v; b = d; } else { b = v; r = d; }
+    }
+}
+
+__global__ void k_debayer_green(int w, int h, const float* __restrict__ in, int in_pitch, float* __restrict__ out,
+                                int out_pitch, DebayerArgs a) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w - 2 || x < 2 || y >= h - 2 || y < 2) return;
+    auto raw = [&](int xx, int yy) { return row_ptr(in, in_pitch, yy)[xx]; };
+    row_ptr(out, out_pitch, y)[x * 3 + 1] = green_at(x, y, a, raw);
+}
+
+__global__ void k_debayer_redblue(int w, int h, const float* __restrict__ in, int in_pitch, float* out, int out_pitch,
+                                  DebayerArgs a) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w - 2 || x < 2 || y >= h - 2 || y < 2) return;
+    auto raw = [&](int xx, int yy) { return row_ptr(in, in_pitch, yy)[xx]; };
+    auto G = [&](int xx, int yy) { return row_ptr(out, out_pitch, yy)[xx * 3 + 1]; };
+    float r, b;
+    redblue_at(x, y, a, raw, G, r, b);
+    float* o = row_ptr(out, out_pitch, y) + x * 3;
+    o[0] = r;
+    o[2] = b;
+}
+
+// Fused DeBayerFullRes / tracking-image front end on an LDS tile.
+// MODE 0: write the decoded float3 image (debayer / div, 0 on the 2-px border).
+// MODE 1: write grey = coef . sRGB(decoded) (gamma + ColorToGray fused).
+constexpr int DB_BX = 64, DB_BY = 8;
+template <int MODE>
+__global__ __launch_bounds__(DB_BX* DB_BY) void k_debayer_tile(int w, int h, const uint16_t* __restrict__ raw,
+                                                                int raw_pitch, float* __restrict__ out, int out_pitch,
+                                                                DebayerArgs a, float c0, float c1, float c2) {
+    __shared__ float s_raw[DB_BY + 6][DB_BX + 6];
+    __shared__ float s_g[DB_BY + 2][DB_BX + 2];
+    const int x0 = blockIdx.x * DB_BX, y0 = blockIdx.y * DB_BY;
+    const int tid = threadIdx.y * DB_BX + threadIdx.x;
+    for (int i = tid; i < (DB_BY + 6) * (DB_BX + 6); i += DB_BX * DB_BY) {
+        int ly = i / (DB_BX + 6), lx = i - ly * (DB_BX + 6);
+        int gx = clampi(x0 - 3 + lx, 0, w - 1), gy = clampi(y0 - 3 + ly, 0, h - 1);
+        s_raw[ly][lx] = (float)row_ptr(raw, raw_pitch, gy)[gx];
+    }
+    __syncthreads();
+    auto R = [&](int xx, int yy) { return s_raw[yy - (y0 - 3)][xx - (x0 - 3)]; };
+    for (int i = tid; i < (DB_BY + 2) * (DB_BX + 2); i += DB_BX * DB_BY) {
+        int ly = i / (DB_BX + 2), lx = i - ly * (DB_BX + 2);
+        int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+        float g = 0.0f;  // never-written border of the decoded image reads as the Set(0) value
+        if (gx >= 2 && gx < w - 2 && gy >= 2 && gy < h - 2) g = green_at(gx, gy, a, R);
+        s_g[ly][lx] = g;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2) {
+        auto G = [&](int xx, int yy) { return s_g[yy - (y0 - 1)][xx - (x0 - 1)]; };
+        float r, b;
+        redblue_at(x, y, a, R, G, r, b);
+        v[0] = r;
+        v[1] = G(x, y);
+        v[2] = b;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = v[c] / a.div[c];
+    if (MODE == 0) {
+        float* o = row_ptr(out, out_pitch, y) + x * 3;
+        o[0] = v[0];
+        o[1] = v[1];
+        o[2] = v[2];
+    } else {
+        float r = srgb_prepared(v[0]), g = srgb_prepared(v[1]), b = srgb_prepared(v[2]);
+        row_ptr(out, out_pitch, y)[x] = c0 * r + c1 * g + c2 * b;
+    }
+}
+
+__global__ void k_gamma(float* img, int w, int h, int pitch) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float* p = row_ptr(img, pitch, y) + x * 3;
+    p[0] = srgb_prepared(p[0]);
+    p[1] = srgb_prepared(p[1]);
+    p[2] = srgb_prepared(p[2]);
+}
+
+__global__ void k_color_to_gray(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, float c0,
+                                float c1, float c2) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const float* p = row_ptr(in, in_pitch, y) + x * 3;
+    row_ptr(out, out_pitch, y)[x] = c0 * p[0] + c1 * p[1] + c2 * p[2];
+}
+
+struct Taps {
+    float t[99];
+    int n;
+};
+
+// Separable Gaussian, replicate border: horizontal pass into tmp, vertical pass into out.
+template <int CH>
+__global__ void k_gauss_h(const float* in, int in_pitch, float* tmp, int w, int h, Taps taps) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const float* r = row_ptr(in, in_pitch, y);
+    int c0 = taps.n / 2;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        float s = 0.0f;
+        for (int i = 0; i < taps.n; ++i) s += taps.t[i] * r[clampi(x + i - c0, 0, w - 1) * CH + c];
+        tmp[((size_t)y * w + x) * CH + c] = s;
+    }
+}
+
+template <int CH>
+__global__ void k_gauss_v(const float* tmp, float* out, int out_pitch, int w, int h, Taps taps) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    int c0 = taps.n / 2;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        float s = 0.0f;
+        for (int i = 0; i < taps.n; ++i) s += taps.t[i] * tmp[((size_t)clampi(y + i - c0, 0, h - 1) * w + x) * CH + c];
+        row_ptr(out, out_pitch, y)[x * CH + c] = s;
+    }
+}
+
+// Single-channel Gaussian fused in LDS (taps <= 15): same operation order as the two-pass form.
+constexpr int GF_BX = 64, GF_BY = 16, GF_R = 7;
+__global__ __launch_bounds__(GF_BX* GF_BY) void k_gauss_fused1(const float* __restrict__ in, int in_pitch,
+                                                                float* __restrict__ out, int out_pitch, int w, int h,
+                                                                Taps taps) {
+    __shared__ float s_in[GF_BY + 2 * GF_R][GF_BX + 2 * GF_R];
+    __shared__ float s_h[GF_BY + 2 * GF_R][GF_BX];
+    const int c0 = taps.n / 2;
+    const int x0 = blockIdx.x * GF_BX, y0 = blockIdx.y * GF_BY;
+    const int tid = threadIdx.y * GF_BX + threadIdx.x;
+    const int rows = GF_BY + 2 * c0, cols = GF_BX + 2 * c0;
+    for (int i = tid; i < rows * cols; i += GF_BX * GF_BY) {
+        int ly = i / cols, lx = i - ly * cols;
+        int gx = clampi(x0 - c0 + lx, 0, w - 1), gy = clampi(y0 - c0 + ly, 0, h - 1);
+        s_in[ly][lx] = row_ptr(in, in_pitch, gy)[gx];
+    }
+    __syncthreads();
+    for (int i = tid; i < rows * GF_BX; i += GF_BX * GF_BY) {
+        int ly = i / GF_BX, lx = i - ly * GF_BX;
+        int gx = x0 + lx;
+        float s = 0.0f;
+        for (int k = 0; k < taps.n; ++k) {
+            int xx = clampi(gx + k - c0, 0, w - 1);  // replicate at the image edge
+            s += taps.t[k] * s_in[ly][xx - (x0 - c0)];
+        }
+        s_h[ly][lx] = s;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float s = 0.0f;
+    for (int k = 0; k < taps.n; ++k) {
+        int yy = clampi(y + k - c0, 0, h - 1);
+        s += taps.t[k] * s_h[yy - (y0 - c0)][threadIdx.x];
+    }
+    row_ptr(out, out_pitch, y)[x] = s;
+}
+
+__global__ void k_resize_box(const float* in, int in_pitch, float* out, int out_pitch, int ow, int oh, int l) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= ow || y >= oh) return;
+    float s = 0.0f;
+    for (int j = 0; j < l; ++j) {
+        const float* r = row_ptr(in, in_pitch, y * l + j) + x * l;
+        for (int i = 0; i < l; ++i) s += r[i];
+    }
+    row_ptr(out, out_pitch, y)[x] = s / (float)(l * l);
+}
+
+__global__ void k_structure_tensor(const float* dx, const float* dy, int dpitch, float* out, int out_pitch, int w,
+                                   int h) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float a = row_ptr(dx, dpitch, y)[x], b = row_ptr(dy, dpitch, y)[x];
+    float* o = row_ptr(out, out_pitch, y) + x * 3;
+    o[0] = a * a;
+    o[1] = b * b;
+    o[2] = a * b;
+}
+
+// ComputeKernelParam (R:Kernels/kernel.cu:627-700), in place; CH = 3 or 4 (the 4th channel is left alone).
+template <int CH>
+__global__ void k_kernel_param(float* img, int w, int h, int pitch, float Dth, float Dtr, float kDetail,
+                               float kDenoise, float kStretch, float kShrink) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float* p = row_ptr(img, pitch, y) + x * CH;
+    float a11 = p[0], a22 = p[1], a12 = p[2];
+    float help = sqrtf((a22 - a11) * (a22 - a11) + 4.0f * a12 * a12);
+    float c = 2.0f * a12, s = a22 - a11 + help;
+    float norm = sqrtf(c * c + s * s);
+    if (norm > 0) { c /= norm; s /= norm; }
+    else { c = 1.0f; s = 0.0f; }
+    float lam1 = (a11 + a22 + help) / 2.0f, lam2 = (a11 + a22 - help) / 2.0f;
+    float A = 1.0f + sqrtf((lam1 - lam2) * (lam1 - lam2) / ((lam1 + lam2) * (lam1 + lam2)));
+    float D = 1.0f - sqrtf(lam1) / Dtr + Dth;
+    D = fmaxf(fminf(1.0f, D), 0.0f);
+    float k1h = kDetail * kStretch * A, k2h = kDetail / kShrink * A;
+    float k1 = (1.0f - D) * k1h + D * kDetail * kDenoise;
+    float k2 = (1.0f - D) * k2h + D * kDetail * kDenoise;
+    k1 *= k1;
+    k2 *= k2;
+    float x2 = c, y2 = s, x1 = s, y1 = -c;
+    float b11 = k1 * x1 * x1 + x2 * x2 * k2;
+    float b12 = k1 * x1 * y1 + x2 * y2 * k2;
+    float b22 = k1 * y1 * y1 + y2 * y2 * k2;
+    float det = b11 * b22 - b12 * b12 + 0.0000000001f;
+    p[0] = b22 / det;
+    p[1] = b11 / det;
+    p[2] = -b12 / det;
+}
+
+__global__ void k_erode4(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int size) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    int r = size / 2;
+    float m[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    for (int j = -r; j <= size - 1 - r; ++j) {
+        const float* row = row_ptr(in, in_pitch, clampi(y + j, 0, h - 1));
+        for (int i = -r; i <= size - 1 - r; ++i) {
+            const float* p = row + clampi(x + i, 0, w - 1) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) m[c] = fminf(m[c], p[c]);
+        }
+    }
+    float* o = row_ptr(out, out_pitch, y) + x * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = m[c];
+}
+
+__global__ void k_apply_weighting(float* inout, const float* sum, const float* weight, int w, int h, int pitch,
+                                  float thr) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    float* io = row_ptr(inout, pitch, y) + x * 3;
+    const float* s = row_ptr(sum, pitch, y) + x * 3;
+    const float* wt = row_ptr(weight, pitch, y) + x * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float val = s[c], ww = wt[c], v = io[c];
+        if (ww < thr) { val += v; ww += 1.0f; }
+        v = 0.0f;
+        if (ww != 0.0f) v = val / ww;
+        io[c] = v;
+    }
+}
+
+DebayerArgs make_debayer_args(const float black[3], const float scale[3], const float div[3], const int cfa[4]) {
+    DebayerArgs a;
+    for (int i = 0; i < 4; ++i) a.cfa.c[i] = cfa[i];
+    for (int c = 0; c < 3; ++c) {
+        a.black[c] = black[c];
+        a.scale[c] = scale[c];
+        a.div[c] = div ? div[c] : 1.0f;
+    }
+    return a;
+}
+
+bool cfa_ok(const int cfa[4]) {
+    for (int i = 0; i < 4; ++i)
+        if (cfa[i] < 0 || cfa[i] > 2) return false;
+    return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers shared with the merger.
+namespace isa {
+
+int launch_raw_to_gray(const uint16_t* raw, int raw_pitch, float* gray, int gray_pitch, int w, int h,
+                       const float black[3], const float white[3], const float camwhite[3], const int cfa[4],
+                       int green_only, hipStream_t s) {
+    float scale[3], div[3];
+    for (int c = 0; c < 3; ++c) {
+        scale[c] = 1.0f / camwhite[c];
+        div[c] = white[c] / camwhite[c];
+    }
+    DebayerArgs a = make_debayer_args(black, scale, div, cfa);
+    float c0 = green_only ? 0.0f : 0.2989f, c1 = green_only ? 1.0f : 0.5870f, c2 = green_only ? 0.0f : 0.1141f;
+    dim3 grid(div_up(w, DB_BX), div_up(h, DB_BY));
+    k_debayer_tile<1><<<grid, dim3(DB_BX, DB_BY), 0, s>>>(w, h, raw, raw_pitch, gray, gray_pitch, a, c0, c1, c2);
+    return check_launch("raw_to_gray");
+}
+
+int launch_gauss1(const float* in, int in_pitch, float* out, int out_pitch, float* tmp, int w, int h,
+                  const float* taps, int n, hipStream_t s);
+
+int launch_resize_box(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int l, hipStream_t s) {
+    int ow = w / l, oh = h / l;
+    k_resize_box<<<dim3(div_up(ow, 32), div_up(oh, 8)), dim3(32, 8), 0, s>>>(in, in_pitch, out, out_pitch, ow, oh, l);
+    return check_launch("resize_box");
+}
+
+int launch_kernel_param4(float* img, int w, int h, int pitch, float dth, float dtr, float kd, float kn, float ks,
+                         float kh, hipStream_t s) {
+    k_kernel_param<4><<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(img, w, h, pitch, dth, dtr, kd, kn, ks,
+                                                                                kh);
+    return check_launch("kernel_param4");
+}
+
+}  // namespace isa
+
+static Taps make_taps(const float* t, int n) {
+    Taps tp;
+    tp.n = n;
+    for (int i = 0; i < 99; ++i) tp.t[i] = i < n ? t[i] : 0.0f;
+    return tp;
+}
+
+int isa::launch_gauss1(const float* in, int in_pitch, float* out, int out_pitch, float* tmp, int w, int h,
+                       const float* taps, int n, hipStream_t s) {
+    Taps tp = make_taps(taps, n);
+    if (n <= 2 * GF_R + 1) {
+        k_gauss_fused1<<<dim3(div_up(w, GF_BX), div_up(h, GF_BY)), dim3(GF_BX, GF_BY), 0, s>>>(in, in_pitch, out,
+                                                                                                out_pitch, w, h, tp);
+        return check_launch("gauss_fused1");
+    }
+    dim3 b(32, 8), g(div_up(w, 32), div_up(h, 8));
+    k_gauss_h<1><<<g, b, 0, s>>>(in, in_pitch, tmp, w, h, tp);
+    k_gauss_v<1><<<g, b, 0, s>>>(tmp, out, out_pitch, w, h, tp);
+    return check_launch("gauss1");
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+int isa_debayer_green(int width, int height, const float* in_dev, int in_pitch, float* out3_dev, int out_pitch,
+                      const float black[3], const float scale[3], const int cfa[4], isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && in_dev && out3_dev && cfa_ok(cfa), "isa_debayer_green: bad arguments");
+    DebayerArgs a = make_debayer_args(black, scale, nullptr, cfa);
+    k_debayer_green<<<dim3(isa::div_up(width, 32), isa::div_up(height, 16)), dim3(32, 16), 0, isa::to_stream(stream)>>>(
+        width, height, in_dev, in_pitch, out3_dev, out_pitch, a);
+    return isa::check_launch("isa_debayer_green");
+}
+
+int isa_debayer_redblue(int width, int height, const float* in_dev, int in_pitch, float* out3_dev, int out_pitch,
+                        const float black[3], const float scale[3], const int cfa[4], isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && in_dev && out3_dev && cfa_ok(cfa), "isa_debayer_redblue: bad arguments");
+    DebayerArgs a = make_debayer_args(black, scale, nullptr, cfa);
+    k_debayer_redblue<<<dim3(isa::div_up(width, 32), isa::div_up(height, 16)), dim3(32, 16), 0,
+                        isa::to_stream(stream)>>>(width, height, in_dev, in_pitch, out3_dev, out_pitch, a);
+    return isa::check_launch("isa_debayer_redblue");
+}
+
+int isa_debayer_full(int width, int height, const uint16_t* raw_dev, int raw_pitch, float* out3_dev, int out_pitch,
+                     const float black[3], const float white[3], const float camera_white[3], const int cfa[4],
+                     isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && raw_dev && out3_dev && cfa_ok(cfa), "isa_debayer_full: bad arguments");
+    float scale[3], div[3];
+    for (int c = 0; c < 3; ++c) {
+        scale[c] = 1.0f / camera_white[c];
+        div[c] = white[c] / camera_white[c];
+    }
+    DebayerArgs a = make_debayer_args(black, scale, div, cfa);
+    dim3 grid(isa::div_up(width, DB_BX), isa::div_up(height, DB_BY));
+    k_debayer_tile<0><<<grid, dim3(DB_BX, DB_BY), 0, isa::to_stream(stream)>>>(width, height, raw_dev, raw_pitch,
+                                                                              out3_dev, out_pitch, a, 0.f, 0.f, 0.f);
+    return isa::check_launch("isa_debayer_full");
+}
+
+int isa_gamma_srgb(float* img3_dev, int width, int height, int pitch, isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && img3_dev, "isa_gamma_srgb: bad arguments");
+    k_gamma<<<dim3(isa::div_up(width, 16), isa::div_up(height, 16)), dim3(16, 16), 0, isa::to_stream(stream)>>>(
+        img3_dev, width, height, pitch);
+    return isa::check_launch("isa_gamma_srgb");
+}
+
+int isa_color_to_gray(const float* in3_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height,
+                      const float coef[3], isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && in3_dev && out_dev, "isa_color_to_gray: bad arguments");
+    k_color_to_gray<<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0, isa::to_stream(stream)>>>(
+        in3_dev, in_pitch, out_dev, out_pitch, width, height, coef[0], coef[1], coef[2]);
+    return isa::check_launch("isa_color_to_gray");
+}
+
+int isa_gaussian_taps(float sigma, float* taps_out, int capacity) {
+    std::vector<float> t;
+    if (sigma <= 0) {
+        t = {0, 0, 0, 0, 1, 0, 0, 0, 0};
+    } else {
+        int size = (int)(sigma / 0.6f - 0.4f) * 2 + 1 + 2;
+        size = size < 99 ? size : 99;
+        t.resize(size);
+        int center = size / 2;
+        for (int i = 0; i < size; ++i) {
+            int x = i - center;
+            float e = (float)(-(x * x)) / (2.0f * sigma * sigma);  // exponent in float, exp in double
+            t[i] = (float)std::exp((double)e);
+        }
+        float sum = 0;
+        for (int i = 0; i < size; ++i) sum += t[i];
+        for (int i = 0; i < size; ++i) t[i] /= sum;
+    }
+    if ((int)t.size() > capacity) return -1;
+    for (size_t i = 0; i < t.size(); ++i) taps_out[i] = t[i];
+    return (int)t.size();
+}
+
+int isa_gauss_filter_border(const float* in_dev, int in_pitch, float* out_dev, int out_pitch, float* tmp_dev, int width,
+                            int height, int channels, const float* taps, int n_taps, isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && in_dev && out_dev && tmp_dev && taps && n_taps > 0 && n_taps <= 99,
+                "isa_gauss_filter_border: bad arguments");
+    ISA_REQUIRE(channels == 1 || channels == 3 || channels == 4, "isa_gauss_filter_border: channels must be 1, 3 or 4");
+    Taps tp = make_taps(taps, n_taps);
+    dim3 b(32, 8), g(isa::div_up(width, 32), isa::div_up(height, 8));
+    hipStream_t s = isa::to_stream(stream);
+    if (channels == 1) {
+        k_gauss_h<1><<<g, b, 0, s>>>(in_dev, in_pitch, tmp_dev, width, height, tp);
+        k_gauss_v<1><<<g, b, 0, s>>>(tmp_dev, out_dev, out_pitch, width, height, tp);
+    } else if (channels == 3) {
+        k_gauss_h<3><<<g, b, 0, s>>>(in_dev, in_pitch, tmp_dev, width, height, tp);
+        k_gauss_v<3><<<g, b, 0, s>>>(tmp_dev, out_dev, out_pitch, width, height, tp);
+    } else {
+        k_gauss_h<4><<<g, b, 0, s>>>(in_dev, in_pitch, tmp_dev, width, height, tp);
+        k_gauss_v<4><<<g, b, 0, s>>>(tmp_dev, out_dev, out_pitch, width, height, tp);
+    }
+    return isa::check_launch("isa_gauss_filter_border");
+}
+
+int isa_resize_box(const float* in_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height, int level,
+                   isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && level >= 1 && in_dev && out_dev && width / level > 0 && height / level > 0,
+                "isa_resize_box: bad arguments");
+    return isa::launch_resize_box(in_dev, in_pitch, out_dev, out_pitch, width, height, level, isa::to_stream(stream));
+}
+
+int isa_compute_structure_tensor(const float* dx_dev, const float* dy_dev, int dxdy_pitch, float* out3_dev,
+                                 int out_pitch, int width, int height, isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && dx_dev && dy_dev && out3_dev, "isa_compute_structure_tensor: bad arguments");
+    k_structure_tensor<<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0,
+                         isa::to_stream(stream)>>>(dx_dev, dy_dev, dxdy_pitch, out3_dev, out_pitch, width, height);
+    return isa::check_launch("isa_compute_structure_tensor");
+}
+
+int isa_compute_kernel_param(float* kernel3_dev, int width, int height, int pitch, float dth, float dtr,
+                             float k_detail, float k_denoise, float k_stretch, float k_shrink, isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && kernel3_dev, "isa_compute_kernel_param: bad arguments");
+    k_kernel_param<3><<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0, isa::to_stream(stream)>>>(
+        kernel3_dev, width, height, pitch, dth, dtr, k_detail, k_denoise, k_stretch, k_shrink);
+    return isa::check_launch("isa_compute_kernel_param");
+}
+
+int isa_erode_border4(const float* in4_dev, int in_pitch, float* out4_dev, int out_pitch, int width, int height,
+                      int size, isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && size >= 1 && size <= 41 && in4_dev && out4_dev,
+                "isa_erode_border4: bad arguments");
+    k_erode4<<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0, isa::to_stream(stream)>>>(
+        in4_dev, in_pitch, out4_dev, out_pitch, width, height, size);
+    return isa::check_launch("isa_erode_border4");
+}
+
+int isa_apply_weighting(float* inout3_dev, const float* sum3_dev, const float* weight3_dev, int width, int height,
+                        int pitch, float threshold, isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && inout3_dev && sum3_dev && weight3_dev, "isa_apply_weighting: bad arguments");
+    k_apply_weighting<<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0,
+                        isa::to_stream(stream)>>>(inout3_dev, sum3_dev, weight3_dev, width, height, pitch, threshold);
+    return isa::check_launch("isa_apply_weighting");
+}
+
+}  // extern "C"

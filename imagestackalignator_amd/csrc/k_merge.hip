@@ -1,0 +1,346 @@
+// Merge kernels: half-resolution debayer, robustness mask, kernel accumulation.
+#include "isa_device.h"
+#include "isa_host.h"
+
+namespace {
+
+// deBayersSubSample3 (R:Kernels/DeBayerKernels.cu:242-283); raw unpitched, 2*dim_x wide.
+__global__ void k_subsample3(const uint16_t* __restrict__ raw, float* out, float max_val, int hw, int hh,
+                             int out_pitch, Cfa cfa) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= hw || y >= hh) return;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    float factor = 1.0f / max_val;
+#pragma unroll
+    for (int ix = 0; ix < 2; ++ix)
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy) {
+            int site = cfa.c[iy * 2 + ix];
+            float v = (float)raw[(size_t)(2 * y + iy) * hw * 2 + 2 * x + ix];
+            if (site == ISA_CFA_GREEN) p[1] += v * factor * 0.5f;
+            else if (site == ISA_CFA_RED) p[0] = v * factor;
+            else if (site == ISA_CFA_BLUE) p[2] = v * factor;
+        }
+    float* o = row_ptr(out, out_pitch, y) + x * 3;
+    o[0] = p[0];
+    o[1] = p[1];
+    o[2] = p[2];
+}
+
+// ComputeRobustnessMask (R:Kernels/RobustnessModell.cu:28-158). Grid over the half-resolution
+// image (w x h); flow: full-resolution texture fw x fh (linear, mirror). Border not written.
+__global__ void k_robustness(const float* ref, const float* mov, int img_pitch, float* mask, int mask_pitch,
+                             const float* flow, int flow_pitch, int fw, int fh, int w, int h, float alpha, float beta,
+                             float thr_m, int quant) {
+    int px = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y * blockDim.y + threadIdx.y;
+    if (px >= w - 1 || py >= h - 1 || px < 1 || py < 1) return;
+    float u = ((float)px + 0.5f) / (float)w, v = ((float)py + 0.5f) / (float)h;
+    float f[2];
+    tex_linear_all<2>(flow, flow_pitch, fw, fh, u, v, ISA_ADDR_MIRROR, quant, f);
+    // The reference's 5x5 min/max loop overwrites its result each pass (R:Kernels/RobustnessModell.cu:62-72),
+    // so only the (+2, +2) sample determines M.
+    float su = ((float)px + 2.0f + 0.5f) / (float)w, sv = ((float)py + 2.0f + 0.5f) / (float)h;
+    float sm[2];
+    tex_linear_all<2>(flow, flow_pitch, fw, fh, su, sv, ISA_ADDR_MIRROR, quant, sm);
+    float maxx = fmaxf(sm[0], f[0]), maxy = fmaxf(sm[1], f[1]);
+    float minx = fminf(sm[0], f[0]), miny = fminf(sm[1], f[1]);
+    int shx = (int)roundf(f[0] * 0.5f), shy = (int)roundf(f[1] * 0.5f);
+    float mr[3] = {0, 0, 0}, mm[3] = {0, 0, 0};
+    float pr[9][3];
+#pragma unroll
+    for (int y = -1; y <= 1; ++y)
+#pragma unroll
+        for (int x = -1; x <= 1; ++x) {
+            const float* p = row_ptr(ref, img_pitch, py + y) + (px + x) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                pr[(y + 1) * 3 + x + 1][c] = p[c];
+                mr[c] += p[c];
+            }
+            int yy = clampi(py + shy + y, 0, h - 1), xx = clampi(px + shx + x, 0, w - 1);
+            const float* q = row_ptr(mov, img_pitch, yy) + xx * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mm[c] += q[c];
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        mr[c] /= 9.0f;
+        mm[c] /= 9.0f;
+    }
+    float meandist = fabsf(mr[0] - mm[0]) + fabsf(mr[1] - mm[1]) + fabsf(mr[2] - mm[2]);
+    meandist /= 3.0f;
+    maxx *= 0.5f * meandist;
+    maxy *= 0.5f * meandist;
+    minx *= 0.5f * meandist;
+    miny *= 0.5f * meandist;
+    float Mv = sqrtf((maxx - minx) * (maxx - minx) + (maxy - miny) * (maxy - miny));
+    float sr[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sr[c] += (pr[i][c] - mr[c]) * (pr[i][c] - mr[c]);
+    float s = 1.5f;
+    if (Mv > thr_m) s = 0.0f;
+    float out[4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float sd = sqrtf(sr[c] / 9.0f);
+        float smd = sqrtf(alpha * mr[c] + beta);
+        if (c == 1) smd = smd / sqrtf(2.0f);
+        float dist = fabsf(mr[c] - mm[c]);
+        float sig = fmaxf(smd, sd);
+        dist = dist * (sd * sd / (sd * sd + smd * smd));
+        out[c] = fmaxf(fminf(s * expf(-dist * dist / (sig * sig)) - 0.12f, 1.0f), 0.0f);
+    }
+    out[3] = Mv;
+    float* o = row_ptr(mask, mask_pitch, py) + px * 4;
+    o[0] = out[0];
+    o[1] = out[1];
+    o[2] = out[2];
+    o[3] = out[3];
+}
+
+struct AccArgs {
+    Cfa cfa;
+    float white[3], black[3];
+};
+
+// One 5x5 gather of one frame at output pixel (x, y): integer sample shift (sx, sy), weights w[25].
+__device__ __forceinline__ void gather1x(const uint16_t* __restrict__ raw, const float* __restrict__ mask,
+                                         int mask_pitch, int x, int y, int sx, int sy, int w, int h, const float* wt,
+                                         const AccArgs& a, float acc[3], float ws[3]) {
+#pragma unroll
+    for (int py = -2; py <= 2; ++py)
+#pragma unroll
+        for (int px = -2; px <= 2; ++px) {
+            int ppsx = clampi(x + px + sx, 0, w - 1), ppsy = clampi(y + py + sy, 0, h - 1);
+            int ppx = clampi(x + px, 0, w - 1), ppy = clampi(y + py, 0, h - 1);
+            int site = a.cfa.at(ppsy, ppsx);
+            int c = site == ISA_CFA_GREEN ? 1 : site == ISA_CFA_RED ? 0 : site == ISA_CFA_BLUE ? 2 : -1;
+            if (c < 0) continue;
+            float wgt = wt[(py + 2) * 5 + px + 2];
+            float v = (float)raw[(size_t)ppsy * w + ppsx];
+            v = (v - a.black[c]) / a.white[c];
+            float cert = row_ptr(mask, mask_pitch, ppy / 2)[(ppx / 2) * 4 + c];
+            if (!isfinite(cert)) cert = 0.0f;
+            acc[c] += v * wgt * cert;
+            ws[c] += wgt * cert;
+        }
+}
+
+// accumulateImages (R:Kernels/DeBayerKernels.cu:288-376).
+__global__ void k_accumulate(const uint16_t* __restrict__ raw, float* sum, float* wsum, const float* mask,
+                             const float* kern, const float* flow, int w, int h, int out_pitch, int mask_pitch,
+                             int flow_pitch, AccArgs a) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x < 1 || y < 1 || x >= w - 1 || y >= h - 1) return;
+    float* ps = row_ptr(sum, out_pitch, y) + x * 3;
+    float* pw = row_ptr(wsum, out_pitch, y) + x * 3;
+    const float* k = row_ptr(kern, out_pitch, y) + x * 3;
+    const float* f = row_ptr(flow, flow_pitch, y) + x * 2;
+    float acc[3] = {ps[0], ps[1], ps[2]}, ws[3] = {pw[0], pw[1], pw[2]};
+    float wt[25];
+#pragma unroll
+    for (int py = -2; py <= 2; ++py)
+#pragma unroll
+        for (int px = -2; px <= 2; ++px) wt[(py + 2) * 5 + px + 2] = kernel_weight(px, py, k[0], k[1], k[2]);
+    int sx = (int)roundf(f[0]), sy = (int)roundf(f[1]);
+    gather1x(raw, mask, mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps[c] = acc[c];
+        pw[c] = ws[c];
+    }
+}
+
+// Frame-fused accumulation: the 25 weights are computed once per pixel and the running
+// sums stay in registers across frames; frames are visited in the given order, so the
+// result equals running accumulateImages once per frame in that order.
+struct FramesArgs {
+    const uint16_t* const* raw;
+    const float* const* mask;
+    const float* const* flow;  // null entry = zero flow
+    int nframes;
+};
+
+__global__ void k_accumulate_frames(float* sum, float* wsum, const float* kern, int w, int h, int out_pitch,
+                                    int mask_pitch, int flow_pitch, FramesArgs fa, AccArgs a) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x < 1 || y < 1 || x >= w - 1 || y >= h - 1) return;
+    float* ps = row_ptr(sum, out_pitch, y) + x * 3;
+    float* pw = row_ptr(wsum, out_pitch, y) + x * 3;
+    const float* k = row_ptr(kern, out_pitch, y) + x * 3;
+    float acc[3] = {ps[0], ps[1], ps[2]}, ws[3] = {pw[0], pw[1], pw[2]};
+    float wt[25];
+#pragma unroll
+    for (int py = -2; py <= 2; ++py)
+#pragma unroll
+        for (int px = -2; px <= 2; ++px) wt[(py + 2) * 5 + px + 2] = kernel_weight(px, py, k[0], k[1], k[2]);
+    for (int f = 0; f < fa.nframes; ++f) {
+        int sx = 0, sy = 0;
+        if (fa.flow[f]) {
+            const float* fl = row_ptr(fa.flow[f], flow_pitch, y) + x * 2;
+            sx = (int)roundf(fl[0]);
+            sy = (int)roundf(fl[1]);
+        }
+        gather1x(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps[c] = acc[c];
+        pw[c] = ws[c];
+    }
+}
+
+// accumulateImagesSuperRes (R:Kernels/DeBayerKernels.cu:378-468): w x h output of the central half crop.
+__global__ void k_accumulate_sr(const uint16_t* __restrict__ raw, float* sum, float* wsum, const float* mask,
+                                const float* kern4, const float* flow, int w, int h, int out_pitch, int mask_pitch,
+                                int kern_pitch, int flow_pitch, AccArgs a, int quant) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x < 1 || y < 1 || x >= w - 1 || y >= h - 1) return;
+    float* ps = row_ptr(sum, out_pitch, y) + x * 3;
+    float* pw = row_ptr(wsum, out_pitch, y) + x * 3;
+    float acc[3] = {ps[0], ps[1], ps[2]}, ws[3] = {pw[0], pw[1], pw[2]};
+    float posx = ((float)x + 0.5f + (float)(w / 2)) / 2.0f / (float)w;
+    float posy = ((float)y + 0.5f + (float)(h / 2)) / 2.0f / (float)h;
+    float k[4], f[2];
+    tex_linear_all<4>(kern4, kern_pitch, w, h, posx, posy, ISA_ADDR_CLAMP, quant, k);
+    tex_linear_all<2>(flow, flow_pitch, w, h, posx, posy, ISA_ADDR_MIRROR, quant, f);
+    int sx = (int)roundf(f[0] * 2.0f), sy = (int)roundf(f[1] * 2.0f);
+    for (int py = -2; py <= 2; ++py)
+        for (int px = -2; px <= 2; ++px) {
+            int ppsx = x + px + sx + w / 2, ppsy = y + py + sy + h / 2;
+            int ppx = x + px + w / 2, ppy = y + py + h / 2;
+            ppsx = clampi(ppsx / 2, 0 + w / 4, w / 2 - 1 + w / 4);
+            ppsy = clampi(ppsy / 2, 0 + h / 4, h / 2 - 1 + h / 4);
+            ppx = clampi(ppx / 2, 0 + w / 4, w / 2 - 1 + w / 4);
+            ppy = clampi(ppy / 2, 0 + h / 4, h / 2 - 1 + h / 4);
+            int site = a.cfa.at(ppsy, ppsx);
+            int c = site == ISA_CFA_GREEN ? 1 : site == ISA_CFA_RED ? 0 : site == ISA_CFA_BLUE ? 2 : -1;
+            if (c < 0) continue;
+            float wgt = kernel_weight(px, py, k[0], k[1], k[2]);
+            float v = (float)raw[(size_t)ppsy * w + ppsx];
+            v = (v - a.black[c]) / a.white[c];
+            float cert = row_ptr(mask, mask_pitch, ppy / 2)[(ppx / 2) * 4 + c];
+            if (!isfinite(cert)) cert = 0.0f;
+            acc[c] += v * wgt * cert;
+            ws[c] += wgt * cert;
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps[c] = acc[c];
+        pw[c] = ws[c];
+    }
+}
+
+AccArgs make_acc(const float white[3], const float black[3], const int cfa[4]) {
+    AccArgs a;
+    for (int i = 0; i < 4; ++i) a.cfa.c[i] = cfa[i];
+    for (int c = 0; c < 3; ++c) {
+        a.white[c] = white[c];
+        a.black[c] = black[c];
+    }
+    return a;
+}
+
+}  // namespace
+
+namespace isa {
+
+int launch_subsample3(const uint16_t* raw, float* out, float max_val, int hw, int hh, int out_pitch, const int cfa[4],
+                      hipStream_t s) {
+    Cfa c;
+    for (int i = 0; i < 4; ++i) c.c[i] = cfa[i];
+    k_subsample3<<<dim3(div_up(hw, 16), div_up(hh, 16)), dim3(16, 16), 0, s>>>(raw, out, max_val, hw, hh, out_pitch, c);
+    return check_launch("subsample3");
+}
+
+int launch_robustness(const float* ref, const float* mov, int img_pitch, float* mask, int mask_pitch, const float* flow,
+                      int flow_pitch, int fw, int fh, int w, int h, float alpha, float beta, float thr, int quant,
+                      hipStream_t s) {
+    k_robustness<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(ref, mov, img_pitch, mask, mask_pitch, flow,
+                                                                            flow_pitch, fw, fh, w, h, alpha, beta, thr,
+                                                                            quant);
+    return check_launch("robustness_mask");
+}
+
+int launch_accumulate(const uint16_t* raw, float* sum, float* wsum, const float* mask, const float* kern,
+                      const float* flow, const float white[3], const float black[3], int w, int h, int out_pitch,
+                      int mask_pitch, int flow_pitch, const int cfa[4], hipStream_t s) {
+    k_accumulate<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(raw, sum, wsum, mask, kern, flow, w, h,
+                                                                           out_pitch, mask_pitch, flow_pitch,
+                                                                           make_acc(white, black, cfa));
+    return check_launch("accumulate");
+}
+
+int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, int h, int out_pitch, int mask_pitch,
+                             int flow_pitch, const uint16_t* const* raw, const float* const* mask,
+                             const float* const* flow, int nframes, const float white[3], const float black[3],
+                             const int cfa[4], hipStream_t s) {
+    FramesArgs fa{raw, mask, flow, nframes};
+    k_accumulate_frames<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(
+        sum, wsum, kern, w, h, out_pitch, mask_pitch, flow_pitch, fa, make_acc(white, black, cfa));
+    return check_launch("accumulate_frames");
+}
+
+int launch_accumulate_sr(const uint16_t* raw, float* sum, float* wsum, const float* mask, const float* kern4,
+                         const float* flow, const float white[3], const float black[3], int w, int h, int out_pitch,
+                         int mask_pitch, int kern_pitch, int flow_pitch, const int cfa[4], int quant, hipStream_t s) {
+    k_accumulate_sr<<<dim3(div_up(w, 16), div_up(h, 16)), dim3(16, 16), 0, s>>>(
+        raw, sum, wsum, mask, kern4, flow, w, h, out_pitch, mask_pitch, kern_pitch, flow_pitch,
+        make_acc(white, black, cfa), quant);
+    return check_launch("accumulate_superres");
+}
+
+}  // namespace isa
+
+static bool cfa_valid(const int cfa[4]) {
+    for (int i = 0; i < 4; ++i)
+        if (cfa[i] < 0 || cfa[i] > 2) return false;
+    return true;
+}
+
+extern "C" {
+
+int isa_debayer_subsample3(const uint16_t* raw_dev, float* out3_dev, float max_val, int dim_x, int dim_y, int out_pitch,
+                           const int cfa[4], isa_stream_t stream) {
+    ISA_REQUIRE(raw_dev && out3_dev && dim_x > 0 && dim_y > 0 && cfa_valid(cfa), "isa_debayer_subsample3: bad arguments");
+    return isa::launch_subsample3(raw_dev, out3_dev, max_val, dim_x, dim_y, out_pitch, cfa, isa::to_stream(stream));
+}
+
+int isa_compute_robustness_mask(const float* ref3_dev, const float* moved3_dev, int img_pitch, float* mask4_dev,
+                                int mask_pitch, const float* flow2_dev, int flow_pitch, int flow_w, int flow_h,
+                                int width, int height, float alpha, float beta, float threshold_m, int tex_quant,
+                                isa_stream_t stream) {
+    ISA_REQUIRE(ref3_dev && moved3_dev && mask4_dev && flow2_dev && width > 0 && height > 0 && flow_w > 0 &&
+                    flow_h > 0,
+                "isa_compute_robustness_mask: bad arguments");
+    return isa::launch_robustness(ref3_dev, moved3_dev, img_pitch, mask4_dev, mask_pitch, flow2_dev, flow_pitch, flow_w,
+                                  flow_h, width, height, alpha, beta, threshold_m, tex_quant, isa::to_stream(stream));
+}
+
+int isa_accumulate_images(const uint16_t* raw_dev, float* sum3_dev, float* weight3_dev, const float* mask4_dev,
+                          const float* kernel3_dev, const float* flow2_dev, const float white[3], const float black[3],
+                          int dim_x, int dim_y, int out_pitch, int mask_pitch, int flow_pitch, const int cfa[4],
+                          isa_stream_t stream) {
+    ISA_REQUIRE(raw_dev && sum3_dev && weight3_dev && mask4_dev && kernel3_dev && flow2_dev && dim_x > 0 && dim_y > 0 &&
+                    cfa_valid(cfa),
+                "isa_accumulate_images: bad arguments");
+    return isa::launch_accumulate(raw_dev, sum3_dev, weight3_dev, mask4_dev, kernel3_dev, flow2_dev, white, black,
+                                  dim_x, dim_y, out_pitch, mask_pitch, flow_pitch, cfa, isa::to_stream(stream));
+}
+
+int isa_accumulate_images_superres(const uint16_t* raw_dev, float* sum3_dev, float* weight3_dev, const float* mask4_dev,
+                                   const float* kernel4_dev, const float* flow2_dev, const float white[3],
+                                   const float black[3], int dim_x, int dim_y, int out_pitch, int mask_pitch,
+                                   int kernel_pitch, int flow_pitch, const int cfa[4], int tex_quant,
+                                   isa_stream_t stream) {
+    ISA_REQUIRE(raw_dev && sum3_dev && weight3_dev && mask4_dev && kernel4_dev && flow2_dev && dim_x > 0 && dim_y > 0 &&
+                    cfa_valid(cfa),
+                "isa_accumulate_images_superres: bad arguments");
+    return isa::launch_accumulate_sr(raw_dev, sum3_dev, weight3_dev, mask4_dev, kernel4_dev, flow2_dev, white, black,
+                                     dim_x, dim_y, out_pitch, mask_pitch, kernel_pitch, flow_pitch, cfa, tex_quant,
+                                     isa::to_stream(stream));
+}
+
+}  // extern "C"

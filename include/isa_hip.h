@@ -1,0 +1,301 @@
+/*
+ * isa_hip.h -- C ABI of the MI355X (gfx950) burst-merge library libisa_hip.so.
+ *
+ * It replaces the PTX-loading kernel wrappers of ImageStackAlignator
+ * (R:PEFStudioDX/BasicKernels.cs, DeBayerKernels.cs, OpticalFlowKernels.cs,
+ * RobustnessModellKernel.cs, ShiftMinimizerKernels.cs), the NPP / cuFFT /
+ * cuBLAS calls their host modules make on the merge path, and the host
+ * modules themselves (PatchTracker, ShiftCollection, OpticalFlow and the
+ * controller's pipeline methods) for callers that want the whole path.
+ *
+ * Conventions
+ *  - Every pointer argument named *_dev is device memory owned by the caller;
+ *    no per-kernel entry point allocates. Pitches are in BYTES. float2/3/4
+ *    images are interleaved (x, y[, z[, w]]) per pixel.
+ *  - `cfa` replaces the reference's module constant c_cfaPattern: four ints,
+ *    row-major 2x2, values 0 = red, 1 = green, 2 = blue
+ *    (R:Kernels/DeBayerKernels.cu:28-41; set via the BayerPattern setters,
+ *    R:PEFStudioDX/DeBayerKernels.cs:53-64).
+ *  - `stream` is a hipStream_t (0 = the null stream). Launches are
+ *    asynchronous; the reference launched synchronously.
+ *  - Return value: ISA_OK (0) or an error code; isa_last_error() gives the
+ *    message of the last failure on the calling thread. The reference threw
+ *    (ManagedCuda CudaException / ArgumentOutOfRangeException); a binding
+ *    turns a non-zero code into its exception.
+ *  - Texture fetches of the reference are evaluated in-kernel with the same
+ *    address and filter modes (normalised coordinates; linear filtering with
+ *    exact or, with tex_quant = 1, 8-bit fractional weights).
+ */
+#ifndef ISA_HIP_H
+#define ISA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* isa_stream_t;
+
+enum isa_status {
+    ISA_OK = 0,
+    ISA_ERR_ARG = 1,         /* invalid size / argument (reference: ArgumentOutOfRangeException) */
+    ISA_ERR_HIP = 2,         /* HIP runtime error (reference: CudaException) */
+    ISA_ERR_UNSUPPORTED = 3, /* combination not implemented by this library */
+    ISA_ERR_ALLOC = 4        /* device allocation failed */
+};
+
+enum isa_tracking_strategy { /* R:PEFStudioDX/ShiftCollection.cs:37-47 */
+    ISA_STRATEGY_FULL = 0,
+    ISA_STRATEGY_ONLY_ON_REFERENCE = 1,
+    ISA_STRATEGY_ON_REFERENCE_BLOCK = 2,
+    ISA_STRATEGY_BLOCKS = 3
+};
+
+#define ISA_MAX_LEVELS 8
+
+/* Parameters of the merge path: the controller's DependencyProperties with
+ * their defaults (R:PEFStudioDX/ImageStackAlignatorController.cs:443-675) plus
+ * the raw-file levels the controller reads from RawFile
+ * (R:PefFile/RawFile.cs:124-229). Level 0 is the finest pyramid level. */
+typedef struct isa_params {
+    float sigma_debayer_tracking;     /* 0.5 */
+    float sigma_debayer_accumulation; /* 0.5 */
+    float high_pass;                  /* 0.01 */
+    float high_pass_sigma;            /* 0.0025 */
+    int clear_axis;                   /* 0 (only 0 is supported by the high-pass) */
+    int use_high_pass;                /* 1: tracking images are high-passed (SkipPreAlignment/ComputePreAlignment path) */
+    float matching_threshold;         /* 1.0 */
+    int n_levels;                     /* 1 */
+    int resize_level[ISA_MAX_LEVELS]; /* {1} */
+    int tile_size[ISA_MAX_LEVELS];    /* {32} */
+    int max_shift[ISA_MAX_LEVELS];    /* {2} */
+    int tracking_strategy;            /* ISA_STRATEGY_FULL */
+    int tracking_block_size;          /* 5 */
+    float sigma_structure_tensor;     /* 1.0 */
+    float dth, dtr, k_detail, k_denoise, k_stretch, k_shrink; /* 0.001 0.006 0.25 3.0 4 2 */
+    int iterations_lk;                /* 3 */
+    int window_size_lk;               /* 11 */
+    float min_det_lk;                 /* 0.01 */
+    int erode_size;                   /* 5 */
+    float threshold_m;                /* 1e5 */
+    int super_resolution;             /* 0: 1x merge; 1: reference 2x merge of the central half */
+    int green_channel_only;           /* 0 */
+    float noise_alpha, noise_beta;    /* noise model sigma^2 = alpha x + beta on the [0,1] scale */
+    float black[3];                   /* black level per colour */
+    float white[3];                   /* white level per colour */
+    float camera_white[3];            /* as-shot white balance (1,1,1 = identity) */
+    int cfa[4];                       /* Bayer pattern, row-major 2x2 */
+    int reference_index;              /* -1: PreAlignmentStore rule (R:PEFStudioDX/PreAlignmentStore.cs:36-110) */
+    int tex_quant;                    /* 0 exact bilinear weights, 1 emulate 8-bit texture weights */
+} isa_params;
+
+void isa_default_params(isa_params* p);
+const char* isa_version(void);
+const char* isa_last_error(void);
+/* Writes the gcnArchName of the current device (e.g. "gfx950:sramecc+:xnack-"). */
+int isa_device_arch(char* buf, int len);
+
+/* ------------------------------------------------------------------------
+ * Per-kernel entry points (one per reference kernel / library call).
+ * --------------------------------------------------------------------- */
+
+/* deBayerGreenKernel, R:Kernels/DeBayerKernels.cu:53-149; wrapper R:PEFStudioDX/DeBayerKernels.cs:31-65. */
+int isa_debayer_green(int width, int height, const float* in_dev, int in_pitch, float* out3_dev, int out_pitch,
+                      const float black[3], const float scale[3], const int cfa[4], isa_stream_t stream);
+/* deBayerRedBlueKernel, R:Kernels/DeBayerKernels.cu:151-231; wrapper R:PEFStudioDX/DeBayerKernels.cs:67-101. */
+int isa_debayer_redblue(int width, int height, const float* in_dev, int in_pitch, float* out3_dev, int out_pitch,
+                        const float black[3], const float scale[3], const int cfa[4], isa_stream_t stream);
+/* DeBayerFullRes as one launch: u16 raw -> float3 decoded image: NPP Convert, green, red/blue, then
+ * NPP Div by white/camera_white (R:...Controller.cs:1268-1311). The 2-px border is written as 0. */
+int isa_debayer_full(int width, int height, const uint16_t* raw_dev, int raw_pitch, float* out3_dev, int out_pitch,
+                     const float black[3], const float white[3], const float camera_white[3], const int cfa[4],
+                     isa_stream_t stream);
+/* deBayersSubSample3, R:Kernels/DeBayerKernels.cu:242-283; wrapper R:PEFStudioDX/DeBayerKernels.cs:103-132.
+ * raw is unpitched (2*dim_x wide). */
+int isa_debayer_subsample3(const uint16_t* raw_dev, float* out3_dev, float max_val, int dim_x, int dim_y, int out_pitch,
+                           const int cfa[4], isa_stream_t stream);
+/* accumulateImages, R:Kernels/DeBayerKernels.cu:288-376; wrapper R:PEFStudioDX/DeBayerKernels.cs:134-175. */
+int isa_accumulate_images(const uint16_t* raw_dev, float* sum3_dev, float* weight3_dev, const float* mask4_dev,
+                          const float* kernel3_dev, const float* flow2_dev, const float white[3], const float black[3],
+                          int dim_x, int dim_y, int out_pitch, int mask_pitch, int flow_pitch, const int cfa[4],
+                          isa_stream_t stream);
+/* accumulateImagesSuperRes, R:Kernels/DeBayerKernels.cu:378-468; wrapper R:PEFStudioDX/DeBayerKernels.cs:177-230.
+ * kernel4 (clamp) and flow2 (mirror) are dim_x x dim_y textures sampled bilinearly. */
+int isa_accumulate_images_superres(const uint16_t* raw_dev, float* sum3_dev, float* weight3_dev, const float* mask4_dev,
+                                   const float* kernel4_dev, const float* flow2_dev, const float white[3],
+                                   const float black[3], int dim_x, int dim_y, int out_pitch, int mask_pitch,
+                                   int kernel_pitch, int flow_pitch, const int cfa[4], int tex_quant,
+                                   isa_stream_t stream);
+/* ApplyWeighting, R:Kernels/kernel.cu:316-372; wrapper R:PEFStudioDX/BasicKernels.cs:259-285. */
+int isa_apply_weighting(float* inout3_dev, const float* sum3_dev, const float* weight3_dev, int width, int height,
+                        int pitch, float threshold, isa_stream_t stream);
+/* GammasRGB, R:Kernels/kernel.cu:271-313; wrapper R:PEFStudioDX/BasicKernels.cs:243-258. */
+int isa_gamma_srgb(float* img3_dev, int width, int height, int pitch, isa_stream_t stream);
+/* NPP ColorToGray (R:...Controller.cs:2404-2408): gray = c0 R + c1 G + c2 B. */
+int isa_color_to_gray(const float* in3_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height,
+                      const float coef[3], isa_stream_t stream);
+/* gaussian_filter_1D (R:...Controller.cs:2351-2382): writes the taps to taps_out (host), returns their count
+ * (<= 99) or a negative value when capacity is too small. */
+int isa_gaussian_taps(float sigma, float* taps_out, int capacity);
+/* NPP FilterGaussBorder with replicate border (R:...Controller.cs:2414, 1855): separable, horizontal then
+ * vertical; channels = 1, 3 or 4; taps on the host (<= 99). tmp_dev: width*height*channels floats. */
+int isa_gauss_filter_border(const float* in_dev, int in_pitch, float* out_dev, int out_pitch, float* tmp_dev, int width,
+                            int height, int channels, const float* taps, int n_taps, isa_stream_t stream);
+/* PreAlignment.FourierFilter (R:PEFStudioDX/PreAlignment.cs:268-281 with fourierFilter R:Kernels/kernel.cu:702-779,
+ * lp = lps = 1): high-pass in place, then clamp to [0,1]. The low band where the transfer function differs from 1
+ * is handled by a partial DFT (see DESIGN). workspace_dev: isa_fourier_highpass_workspace() bytes. */
+size_t isa_fourier_highpass_workspace(int width, int height, float high_pass, float high_pass_sigma);
+int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int clear_axis, float high_pass,
+                         float high_pass_sigma, void* workspace_dev, isa_stream_t stream);
+/* NPP ResizeSqrPixel(1/level, SuperSampling) at an integer factor (R:...Controller.cs:1694-1703). */
+int isa_resize_box(const float* in_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height, int level,
+                   isa_stream_t stream);
+/* convertToTilesOverlapBorder, R:Kernels/kernel.cu:171-228; wrapper R:PEFStudioDX/BasicKernels.cs:48-63. */
+int isa_convert_to_tiles_border(const float* img_dev, int img_pitch, float* tiles_dev, int width, int height,
+                                int max_shift, int tile_size, int tile_count_x, int tile_count_y,
+                                const float base_shift[2], float base_rotation, isa_stream_t stream);
+/* convertToTilesOverlapPreShift, R:Kernels/kernel.cu:230-288; wrapper R:PEFStudioDX/BasicKernels.cs:31-46. */
+int isa_convert_to_tiles_preshift(const float* img_dev, int img_pitch, float* tiles_dev, const float* preshift2_dev,
+                                  int preshift_pitch, int width, int height, int max_shift, int tile_size,
+                                  int tile_count_x, int tile_count_y, const float base_shift[2], float base_rotation,
+                                  isa_stream_t stream);
+/* Tile SSD map: replaces cuFFT R2C x2 + conjugateComplexMulKernel + C2R + DivC + squaredSum + boxFilterWithBorderX/Y
+ * + normalizedCC (R:PEFStudioDX/PatchTracker.cs:328-348; R:Kernels/kernel.cu:27-169, 394-411). Writes
+ * D(s) = sum |r - t(.+s)|^2 over the tile interior for s in [-M,M]^2, [tile][(2M+1)^2], evaluated exactly in FP64. */
+int isa_tile_ssd(const float* ref_tiles_dev, const float* track_tiles_dev, float* shift_image_dev, int max_shift,
+                 int tile_size, int tile_count, isa_stream_t stream);
+/* findMinimum, R:Kernels/kernel.cu:413-546; wrapper R:PEFStudioDX/BasicKernels.cs:202-217. */
+int isa_find_minimum(const float* shift_image_dev, float* coords2_dev, int coords_pitch, int max_shift,
+                     int tile_count_x, int tile_count_y, float threshold, isa_stream_t stream);
+/* NPP Add of the float view (R:PEFStudioDX/PatchTracker.cs:355-358): preshift += coords. */
+int isa_add_shifts(float* preshift2_dev, int preshift_pitch, const float* coords2_dev, int coords_pitch,
+                   int tile_count_x, int tile_count_y, isa_stream_t stream);
+/* UpSampleShifts, R:Kernels/kernel.cu:551-598; wrapper R:PEFStudioDX/BasicKernels.cs:219-234. */
+int isa_upsample_shifts(const float* in2_dev, float* out2_dev, int in_pitch, int out_pitch, int old_level,
+                        int new_level, int old_count_x, int old_count_y, int new_count_x, int new_count_y,
+                        int old_tile_size, int new_tile_size, isa_stream_t stream);
+/* PatchTracker.Track for ONE pair (R:PEFStudioDX/PatchTracker.cs:317-360): tile extraction of both images, SSD map,
+ * findMinimum and preshift += result, fused; the images are the level images (width x height). */
+int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+              int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
+              float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
+              isa_stream_t stream);
+
+/* ShiftCollection tables (R:PEFStudioDX/ShiftCollection.cs:113-496). pairs_out: 2*m ints (reference, toTrack) in
+ * GetShiftPairs order; index_out: n*n ints (-1 where unmeasured); rows_out: 2*m ints (a, b): measurement row k
+ * measures x_b - x_a with x_j = sum_{i<j} r_i. Any output may be NULL. Returns m (>0) or -ISA_ERR_ARG. */
+int isa_shift_tables(int frame_count, int reference_index, int strategy, int block_size, int* pairs_out,
+                     int* index_out, int* rows_out);
+/* ShiftCollection.MinimizeCUBLAS (R:PEFStudioDX/ShiftCollection.cs:738-822) in FP64, on the device, all tiles at once
+ * (no host round trip per iteration). measured2_dev: [m][tile_count] float2 (measurement-major, pitch
+ * meas_pitch bytes per measurement) -- read, then overwritten with A_safe r as the reference's separateShifts does;
+ * one_to_one2_dev: [tile][n1] float2 (shiftsOneToOne_d); status_dev (optional): [tile] final status, iters_dev
+ * (optional): iterations used. */
+int isa_minimize_shifts(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev, int* iters_dev,
+                        int tile_count, int frame_count, int reference_index, int strategy, int block_size,
+                        isa_stream_t stream);
+/* getOptimalShifts, R:Kernels/ShiftMinimizerKernels.cu:178-218; wrapper R:PEFStudioDX/ShiftMinimizerKernels.cs:69-96. */
+int isa_get_optimal_shifts(float* out2_dev, int out_pitch, const float* one_to_one2_dev, int frame_count,
+                           int tile_count_x, int tile_count_y, int reference_image, int image_to_track,
+                           isa_stream_t stream);
+
+/* CreateFlowFieldFromTiles, R:Kernels/opticalFlow.cu:47-93; wrapper R:PEFStudioDX/OpticalFlowKernels.cs:58-85.
+ * tiles2 is a tex_w x tex_h texture (linear, clamp). */
+int isa_create_flow_field_from_tiles(float* flow2_dev, int flow_pitch, const float* tiles2_dev, int tiles_pitch,
+                                     int tex_w, int tex_h, int width, int height, const float base_shift[2],
+                                     float base_rotation, int tex_quant, isa_stream_t stream);
+/* WarpingKernel, R:Kernels/opticalFlow.cu:27-44; wrapper R:PEFStudioDX/OpticalFlowKernels.cs:32-57. */
+int isa_warp(const float* src_dev, int src_pitch, float* out_dev, int out_pitch, const float* flow2_dev,
+             int flow_pitch, int width, int height, int tex_quant, isa_stream_t stream);
+/* ComputeDerivativesKernel, R:Kernels/opticalFlow.cu:96-147; wrapper R:PEFStudioDX/OpticalFlowKernels.cs:86-119.
+ * Ix, Iy, Iz share `pitch`; source/target share `img_pitch`. */
+int isa_compute_derivatives(const float* src_dev, const float* tgt_dev, int img_pitch, float* ix_dev, float* iy_dev,
+                            float* iz_dev, int pitch, int width, int height, int tex_quant, isa_stream_t stream);
+/* ComputeDerivatives2Kernel, R:Kernels/opticalFlow.cu:150-185; wrapper R:PEFStudioDX/OpticalFlowKernels.cs:120-144. */
+int isa_compute_derivatives2(const float* img_dev, int img_pitch, float* ix_dev, float* iy_dev, int pitch, int width,
+                             int height, int tex_quant, isa_stream_t stream);
+/* lucasKanadeOptim, R:Kernels/opticalFlow.cu:189-325; wrapper R:PEFStudioDX/OpticalFlowKernels.cs:146-175.
+ * Window sums as in the reference (per-pixel (2h+1)^2 loops). */
+int isa_lucas_kanade_optim(float* flow2_dev, int flow_pitch, const float* ix_dev, const float* iy_dev,
+                           const float* it_dev, int img_pitch, int width, int height, int window_size, float min_det,
+                           isa_stream_t stream);
+/* OpticalFlow.LucasKanade (R:PEFStudioDX/OpticalFlow.cs:147-164): flow from tiles, `iterations` x {warp, derivatives,
+ * LK step}, final warp written over src. work_dev: 4*width*height floats (tmp, Ix, Iy, Iz). */
+int isa_lucas_kanade(float* src_dev, const float* tgt_dev, int img_pitch, const float* tiles2_dev, int tiles_pitch,
+                     int tex_w, int tex_h, float* flow2_dev, int flow_pitch, float* work_dev, int width, int height,
+                     int iterations, const float base_shift[2], float base_rotation, float min_det, int window_size,
+                     int tex_quant, isa_stream_t stream);
+/* ComputeStructureTensor, R:Kernels/kernel.cu:600-625; wrapper R:PEFStudioDX/BasicKernels.cs:287-302. */
+int isa_compute_structure_tensor(const float* dx_dev, const float* dy_dev, int dxdy_pitch, float* out3_dev,
+                                 int out_pitch, int width, int height, isa_stream_t stream);
+/* ComputeKernelParam, R:Kernels/kernel.cu:627-700; wrapper R:PEFStudioDX/BasicKernels.cs:304-319. */
+int isa_compute_kernel_param(float* kernel3_dev, int width, int height, int pitch, float dth, float dtr,
+                             float k_detail, float k_denoise, float k_stretch, float k_shrink, isa_stream_t stream);
+/* ComputeRobustnessMask, R:Kernels/RobustnessModell.cu:28-158; wrapper R:PEFStudioDX/RobustnessModellKernel.cs:30-67.
+ * width/height: half-resolution grid; flow2: flow_w x flow_h texture (linear, mirror). */
+int isa_compute_robustness_mask(const float* ref3_dev, const float* moved3_dev, int img_pitch, float* mask4_dev,
+                                int mask_pitch, const float* flow2_dev, int flow_pitch, int flow_w, int flow_h,
+                                int width, int height, float alpha, float beta, float threshold_m, int tex_quant,
+                                isa_stream_t stream);
+/* NPP ErodeBorder, size x size all-set mask, anchor size/2, replicate border (R:...Controller.cs:1976). */
+int isa_erode_border4(const float* in4_dev, int in_pitch, float* out4_dev, int out_pitch, int width, int height,
+                      int size, isa_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Whole-path entry points (device-resident; the fast path).
+ * --------------------------------------------------------------------- */
+typedef struct isa_merger isa_merger;
+
+/* Allocates every buffer the path needs for `frames` raw frames of width x height. */
+int isa_merger_create(isa_merger** out, int width, int height, int frames, const isa_params* params);
+int isa_merger_destroy(isa_merger* m);
+/* Runs SkipPreAlignment-style tracking-image preparation, TrackPatches, PrepareAccumulation, Accumulate and the
+ * final ApplyWeighting (R:...Controller.cs:1530-1569, 1601-1746, 1795-2002). raw_dev: frames x height x width u16,
+ * contiguous. base_shift (2*frames floats, host, may be NULL = 0) and base_rotation (frames floats, host, degrees,
+ * may be NULL) play the role of the PreAlignmentStore. out3_dev: width x height float3 (pitch out_pitch). */
+int isa_merger_run(isa_merger* m, const uint16_t* raw_dev, const float* base_shift, const float* base_rotation,
+                   float* out3_dev, int out_pitch, isa_stream_t stream);
+/* Device pointer and geometry of an internal buffer, for inspection:
+ *   0 tracking image of frame `index` (high-passed)   1 LK image of frame `index`
+ *   2 finest-level one-to-one shifts [tile][n1] float2 3 flow of frame `index` (float2)
+ *   4 eroded mask of frame `index` (float4, half res) 5 kernel parameters (float3 / float4 in SR)
+ *   6 accumulated sum (float3)                        7 accumulated weight (float3)
+ *   8 per-pair shifts of level `index` after minimisation, [m][tiles] float2
+ *   9 half-resolution debayer of frame `index` (float3)
+ * Returns ISA_ERR_ARG for an unknown buffer. */
+int isa_merger_buffer(isa_merger* m, int which, int index, void** ptr, int* pitch, int* width, int* height);
+/* Reference index chosen by the last run, and per-stage milliseconds of the last run (HIP events): stages are
+ * 0 tracking images, 1 track patches, 2 minimizer, 3 prepare accumulation, 4 LK, 5 robustness, 6 accumulate,
+ * 7 final weighting. Returns the number of stages written. */
+int isa_merger_reference_index(isa_merger* m);
+int isa_merger_stage_ms(isa_merger* m, float* ms, int capacity);
+
+/* ------------------------------------------------------------------------
+ * Synthetic bursts (benchmark / test input; not part of the reference).
+ * --------------------------------------------------------------------- */
+typedef struct isa_synth_config {
+    uint64_t seed;
+    int width, height, frames;
+    float global_shift_max;   /* global shift ~ U(-g, g) px per frame (frame 0 gets 0) */
+    float flow_max;           /* local flow amplitude (px) */
+    float flow_scale;         /* spatial scale of the local flow (px) */
+    float occluded_fraction;  /* fraction of 32x32 tiles with an occluding offset patch (frames != 0) */
+    float gain, read_sigma;   /* Poisson-Gaussian noise in DN: var = gain * signal + read_sigma^2 */
+    int n_blobs, n_edges, n_waves;
+    float white;              /* 16383 for 14-bit */
+} isa_synth_config;
+void isa_default_synth_config(isa_synth_config* c);
+/* Renders the burst (frames x height x width u16, RGGB) on the device. frame_shift_out (host, 2*frames, may be
+ * NULL) receives each frame's global shift. */
+int isa_synth_burst(const isa_synth_config* c, uint16_t* raw_dev, float* frame_shift_out, isa_stream_t stream);
+/* Noise-free RGB rendering of frame `frame`'s scene geometry on an out_w x out_h grid covering the frame with
+ * sample spacing `scale` (1 = pixel grid, 0.5 = 2x) and origin offset (ox, oy) in frame pixels. */
+int isa_synth_ground_truth(const isa_synth_config* c, int frame, float* out3_dev, int out_pitch, int out_w, int out_h,
+                           float scale, float ox, float oy, isa_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISA_HIP_H */

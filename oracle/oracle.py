@@ -1,0 +1,270 @@
+"""ctypes bindings of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker. The product (libisa_hip.so) never
+uses it. Arrays are numpy, C-contiguous; `dt` selects the instantiation:
+np.float32 = reference semantics (FP32, written operation order),
+np.float64 = FP64 oracle.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _declare(_lib)
+    return _lib
+
+
+P = C.c_void_p
+I = C.c_int
+
+
+def _declare(L):
+    spec = {
+        # name: (restype, [argtypes]) with T-typed scalars marked "T"
+        "debayer_full": (None, [I, I, P, P, P, P, P, P]),
+        "debayer_green": (None, [I, I, P, P, P, P, P]),
+        "debayer_redblue": (None, [I, I, P, P, P, P, P]),
+        "gamma_srgb": (None, [I, I, P]),
+        "color_to_gray": (None, [I, I, P, P, P]),
+        "fourier_highpass": (None, [I, I, P, "T", "T", I]),
+        "fourier_filter_value": ("T", [I, I, I, I, "T", "T", I]),
+        "gaussian_taps": (I, ["T", P, I]),
+        "gauss_filter": (None, [I, I, I, P, P, P, I]),
+        "tracking_image": (None, [I, I, P, P, P, I, "T"]),
+        "downscale_box": (None, [I, I, P, I, P]),
+        "tiles_border": (None, [I, I, P, P, I, I, I, I, "T", "T", "T"]),
+        "tiles_preshift": (None, [I, I, P, P, P, I, I, I, I, "T", "T", "T"]),
+        "tile_ssd": (None, [P, P, P, I, I, I]),
+        "find_minimum": (None, [P, P, I, I, "T"]),
+        "upsample_shifts": (None, [P, P, I, I, I, I, I, I, I, I]),
+        "get_optimal_shifts": (None, [P, P, I, I, I, I]),
+        "flow_from_tiles": (None, [P, I, I, P, I, I, "T", "T", "T", I]),
+        "warp": (None, [P, P, P, I, I, I]),
+        "derivatives": (None, [P, P, P, P, P, I, I, I]),
+        "derivatives2": (None, [P, P, P, I, I, I]),
+        "lucas_kanade_step": (None, [P, P, P, P, I, I, I, "T"]),
+        "lucas_kanade": (None, [P, P, P, I, I, P, I, I, I, "T", "T", "T", "T", I, I]),
+        "structure_tensor": (None, [P, P, P, I, I]),
+        "kernel_param": (None, [P, I, I, "T", "T", "T", "T", "T", "T"]),
+        "subsample3": (None, [P, P, "T", I, I, P]),
+        "robustness_mask": (None, [P, P, P, P, I, I, I, I, "T", "T", "T", I]),
+        "erode": (None, [P, P, I, I, I]),
+        "accumulate": (None, [P, P, P, P, P, P, P, P, I, I, P]),
+        "accumulate_sr": (None, [P, P, P, P, P, P, P, P, I, I, P, I]),
+        "apply_weighting": (None, [P, P, P, I, I, "T"]),
+        "merge": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    }
+    for sfx, T in (("f32", C.c_float), ("f64", C.c_double)):
+        for name, (rt, args) in spec.items():
+            fn = getattr(L, f"ora_{name}_{sfx}")
+            fn.restype = T if rt == "T" else rt
+            fn.argtypes = [T if a == "T" else a for a in args]
+    L.ora_set_threads.argtypes = [I]
+    L.ora_set_threads.restype = None
+    L.ora_get_threads.restype = I
+    L.ora_default_params.argtypes = [P]
+    L.ora_default_params.restype = None
+    L.ora_shift_tables.argtypes = [I, I, I, I, P, P, P, P]
+    L.ora_shift_tables.restype = I
+    L.ora_minimize_shifts.argtypes = [I, I, I, I, I, P, P, P, P, P]
+    L.ora_minimize_shifts.restype = I
+
+
+def set_threads(n: int) -> None:
+    lib().ora_set_threads(int(n))
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data_as(P)
+
+
+def fn(name: str, dt=np.float32):
+    sfx = "f32" if np.dtype(dt) == np.float32 else "f64"
+    return getattr(lib(), f"ora_{name}_{sfx}")
+
+
+def arr(values, dt):
+    return np.ascontiguousarray(np.asarray(values, dtype=dt))
+
+
+# ---------------------------------------------------------------------------
+# convenience wrappers (numpy in, numpy out)
+
+def debayer_full(raw: np.ndarray, black, white, camwhite, cfa, dt=np.float32):
+    h, w = raw.shape
+    out = np.zeros((h, w, 3), dt)
+    fn("debayer_full", dt)(w, h, ptr(np.ascontiguousarray(raw, np.uint16)), ptr(out), ptr(arr(black, dt)),
+                           ptr(arr(white, dt)), ptr(arr(camwhite, dt)), ptr(arr(cfa, np.int32)))
+    return out
+
+
+def tracking_image(raw: np.ndarray, params, use_hp: int, sigma: float, dt=np.float32):
+    h, w = raw.shape
+    out = np.zeros((h, w), dt)
+    fn("tracking_image", dt)(w, h, ptr(np.ascontiguousarray(raw, np.uint16)), ptr(out), C.addressof(params), use_hp,
+                             sigma)
+    return out
+
+
+def fourier_highpass(img: np.ndarray, hp=0.01, hps=0.0025, clear_axis=0, dt=np.float32):
+    out = np.ascontiguousarray(img, dt).copy()
+    h, w = out.shape
+    fn("fourier_highpass", dt)(w, h, ptr(out), hp, hps, clear_axis)
+    return out
+
+
+def gaussian_taps(sigma: float, dt=np.float32):
+    buf = np.zeros(128, dt)
+    n = fn("gaussian_taps", dt)(sigma, ptr(buf), 128)
+    return buf[:n].copy()
+
+
+def gauss_filter(img: np.ndarray, taps: np.ndarray, dt=np.float32):
+    a = np.ascontiguousarray(img, dt)
+    h, w = a.shape[:2]
+    ch = 1 if a.ndim == 2 else a.shape[2]
+    out = np.zeros_like(a)
+    t = np.ascontiguousarray(taps, dt)
+    fn("gauss_filter", dt)(w, h, ch, ptr(a), ptr(out), ptr(t), len(t))
+    return out
+
+
+def downscale_box(img: np.ndarray, level: int, dt=np.float32):
+    a = np.ascontiguousarray(img, dt)
+    h, w = a.shape
+    out = np.zeros((h // level, w // level), dt)
+    fn("downscale_box", dt)(w, h, ptr(a), level, ptr(out))
+    return out
+
+
+def tiles_border(img, M, tile, tcx, tcy, bs=(0, 0), rot=0.0, dt=np.float32):
+    a = np.ascontiguousarray(img, dt)
+    h, w = a.shape
+    B = tile + 2 * M
+    out = np.zeros((tcx * tcy, B, B), dt)
+    fn("tiles_border", dt)(w, h, ptr(a), ptr(out), M, tile, tcx, tcy, bs[0], bs[1], rot)
+    return out
+
+
+def tiles_preshift(img, pre, M, tile, tcx, tcy, bs=(0, 0), rot=0.0, dt=np.float32):
+    a = np.ascontiguousarray(img, dt)
+    h, w = a.shape
+    B = tile + 2 * M
+    out = np.zeros((tcx * tcy, B, B), dt)
+    p = np.ascontiguousarray(pre, dt)
+    fn("tiles_preshift", dt)(w, h, ptr(a), ptr(out), ptr(p), M, tile, tcx, tcy, bs[0], bs[1], rot)
+    return out
+
+
+def tile_ssd(rt, tt, M, tile, dt=np.float32):
+    rt = np.ascontiguousarray(rt, dt)
+    tt = np.ascontiguousarray(tt, dt)
+    n = rt.shape[0]
+    S = 2 * M + 1
+    out = np.zeros((n, S, S), dt)
+    fn("tile_ssd", dt)(ptr(rt), ptr(tt), ptr(out), M, tile, n)
+    return out
+
+
+def find_minimum(simg, M, thr=1.0, dt=np.float32):
+    s = np.ascontiguousarray(simg, dt)
+    n = s.shape[0]
+    out = np.zeros((n, 2), dt)
+    fn("find_minimum", dt)(ptr(s), ptr(out), M, n, thr)
+    return out
+
+
+def upsample_shifts(inp, ol, nl, ocx, ocy, ncx, ncy, ot, nt, dt=np.float32):
+    a = np.ascontiguousarray(inp, dt)
+    out = np.zeros((ncy, ncx, 2), dt)
+    fn("upsample_shifts", dt)(ptr(a), ptr(out), ol, nl, ocx, ocy, ncx, ncy, ot, nt)
+    return out
+
+
+def shift_tables(n, ref, strategy, block):
+    m = lib().ora_shift_tables(n, ref, strategy, block, None, None, None, None)
+    if m < 0:
+        raise ValueError("invalid strategy")
+    A = np.zeros(m * (n - 1), np.float32)
+    pairs = np.zeros((m, 2), np.int32)
+    index = np.zeros((n, n), np.int32)
+    rows = np.zeros((m, 2), np.int32)
+    lib().ora_shift_tables(n, ref, strategy, block, ptr(A), ptr(pairs), ptr(index), ptr(rows))
+    return A.reshape(n - 1, m).T.copy(), pairs, index, rows  # A as (m, n1)
+
+
+def minimize_shifts(d: np.ndarray, n, ref, strategy, block):
+    """d: (tiles, m, 2) float64 measurements. Returns (dhat, r, status, iters)."""
+    d = np.ascontiguousarray(d, np.float64)
+    tiles, m, _ = d.shape
+    dout = np.zeros_like(d)
+    r = np.zeros((tiles, n - 1, 2), np.float64)
+    st = np.zeros(tiles, np.int32)
+    it = np.zeros(tiles, np.int32)
+    rc = lib().ora_minimize_shifts(n, ref, strategy, block, tiles, ptr(d), ptr(dout), ptr(r), ptr(st), ptr(it))
+    if rc < 0:
+        raise ValueError("invalid strategy")
+    return dout, r, st, it
+
+
+def get_optimal_shifts(r, n, ref, trk, dt=np.float32):
+    r = np.ascontiguousarray(r, dt)
+    tiles = r.shape[0]
+    out = np.zeros((tiles, 2), dt)
+    fn("get_optimal_shifts", dt)(ptr(r), ptr(out), n, tiles, ref, trk)
+    return out
+
+
+def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32, want=()):
+    """Full oracle pipeline. raw: (n, h, w) uint16. Returns dict with 'out' and the requested intermediates."""
+    raw = np.ascontiguousarray(raw, np.uint16)
+    n, h, w = raw.shape
+    res = {"out": np.zeros((h, w, 3), dt)}
+    bufs = {
+        "one_to_one": None, "flows": (n, h, w, 2), "masks": (n, h // 2, w // 2, 4), "kern": (h, w, 3),
+        "track": (n, h, w), "lk": (n, h, w), "sum": (h, w, 3), "wsum": (h, w, 3),
+    }
+    for k in want:
+        if k == "one_to_one":
+            L = params.n_levels
+            cx = (w // params.resize_level[0] - 2 * params.max_shift[0]) // params.tile_size[0]
+            cy = (h // params.resize_level[0] - 2 * params.max_shift[0]) // params.tile_size[0]
+            res[k] = np.zeros((cy * cx, n - 1, 2), dt)
+            del L
+        else:
+            res[k] = np.zeros(bufs[k], dt)
+    bs = None if base_shift is None else np.ascontiguousarray(base_shift, np.float32)
+    br = None if base_rot is None else np.ascontiguousarray(base_rot, np.float32)
+    ref = C.c_int(-1)
+    g = res.get
+    rc = fn("merge", dt)(C.addressof(params), w, h, n, ptr(raw), ptr(bs), ptr(br), ptr(res["out"]), ptr(g("one_to_one")),
+                         ptr(g("flows")), ptr(g("masks")), ptr(g("kern")), ptr(g("track")), ptr(g("lk")),
+                         ptr(g("sum")), ptr(g("wsum")), C.byref(ref))
+    if rc != 0:
+        raise RuntimeError(f"oracle merge failed ({rc})")
+    res["ref"] = ref.value
+    return res

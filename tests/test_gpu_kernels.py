@@ -1,0 +1,338 @@
+"""GPU parity: every C-ABI kernel entry point of libisa_hip.so against the CPU
+oracle (reference semantics, FP32) on the same seeded inputs.
+
+Tolerances: integer / index / copy work is bit-exact; arithmetic built only
+from +,-,*,/,sqrt (IEEE-correct on both sides, FMA contraction off) is
+bit-exact; expressions that call expf/powf/atan2f/sinf/cosf are compared at
+a few ULP (device and host libm differ in the last bits)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from tests.gpu_util import dev, empty, host, p, stream, ulp_diff  # noqa: E402
+
+RGGB = [0, 1, 1, 2]
+GRBG = [1, 0, 2, 1]
+
+
+def _raw(rng, h, w, lo=200, hi=16000):
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 0.45 + 0.25 * np.sin(xx / 6.3 + 0.4) * np.cos(yy / 4.1) + 0.1 * np.sin((xx + yy) / 2.3)
+    img = base * hi + rng.normal(0, 60, (h, w))
+    return np.clip(img, lo, 16383).astype(np.uint16)
+
+
+@pytest.mark.parametrize("cfa", [RGGB, GRBG])
+def test_debayer_full(isa, oracle, gpu, cfa):
+    rng = np.random.default_rng(1)
+    h, w = 70, 94
+    raw = _raw(rng, h, w)
+    black, white, cw = [64.0, 60.0, 62.0], [16383.0, 16000.0, 15800.0], [1.9, 1.0, 1.4]
+    out = empty((h, w, 3), fill=7.0)
+    isa.call("isa_debayer_full", w, h, p(dev(raw)), w * 2, p(out), w * 12, isa.f32s(*black), isa.f32s(*white),
+             isa.f32s(*cw), isa.i32s(*cfa), stream())
+    ref = oracle.debayer_full(raw, black, white, cw, cfa)
+    np.testing.assert_array_equal(host(out), ref)
+
+
+def test_debayer_green_redblue(isa, oracle, gpu):
+    rng = np.random.default_rng(2)
+    h, w = 40, 52
+    raw = _raw(rng, h, w).astype(np.float32)
+    black, scale = [10.0, 12.0, 8.0], [0.5, 1.0, 0.7]
+    out = empty((h, w, 3), fill=0.0)
+    d_in = dev(raw)
+    isa.call("isa_debayer_green", w, h, p(d_in), w * 4, p(out), w * 12, isa.f32s(*black), isa.f32s(*scale),
+             isa.i32s(*RGGB), stream())
+    isa.call("isa_debayer_redblue", w, h, p(d_in), w * 4, p(out), w * 12, isa.f32s(*black), isa.f32s(*scale),
+             isa.i32s(*RGGB), stream())
+    ref = np.zeros((h, w, 3), np.float32)
+    fg, frb = oracle.fn("debayer_green"), oracle.fn("debayer_redblue")
+    b, s, c = oracle.arr(black, np.float32), oracle.arr(scale, np.float32), oracle.arr(RGGB, np.int32)
+    fg(w, h, oracle.ptr(raw), oracle.ptr(ref), oracle.ptr(b), oracle.ptr(s), oracle.ptr(c))
+    frb(w, h, oracle.ptr(raw), oracle.ptr(ref), oracle.ptr(b), oracle.ptr(s), oracle.ptr(c))
+    np.testing.assert_array_equal(host(out), ref)
+
+
+def test_gamma_gray(isa, oracle, gpu):
+    rng = np.random.default_rng(3)
+    h, w = 33, 47
+    img = rng.uniform(-0.2, 1.2, (h, w, 3)).astype(np.float32)
+    img[0, 0, 0] = np.nan
+    d = dev(img)
+    isa.call("isa_gamma_srgb", p(d), w, h, w * 12, stream())
+    ref = img.copy()
+    oracle.fn("gamma_srgb")(w, h, oracle.ptr(ref))
+    assert ulp_diff(host(d), ref) <= 2
+    g = empty((h, w))
+    coef = [0.2989, 0.5870, 0.1141]
+    isa.call("isa_color_to_gray", p(d), w * 12, p(g), w * 4, w, h, isa.f32s(*coef), stream())
+    gref = np.zeros((h, w), np.float32)
+    oracle.fn("color_to_gray")(w, h, oracle.ptr(host(d)), oracle.ptr(gref), oracle.ptr(oracle.arr(coef, np.float32)))
+    np.testing.assert_array_equal(host(g), gref)
+
+
+@pytest.mark.parametrize("sigma,ch", [(0.5, 1), (1.0, 3), (2.5, 1), (1.0, 4)])
+def test_gauss_filter(isa, oracle, gpu, sigma, ch):
+    rng = np.random.default_rng(4)
+    h, w = 29, 61
+    img = rng.uniform(0, 1, (h, w, ch) if ch > 1 else (h, w)).astype(np.float32)
+    taps = (C.c_float * 99)()
+    n = isa.lib().isa_gaussian_taps(sigma, taps, 99)
+    np.testing.assert_array_equal(np.array(taps[:n], np.float32), oracle.gaussian_taps(sigma))
+    out = empty(img.shape)
+    tmp = empty(img.shape)
+    isa.call("isa_gauss_filter_border", p(dev(img)), w * 4 * ch, p(out), w * 4 * ch, p(tmp), w, h, ch, taps, n,
+             stream())
+    ref = oracle.gauss_filter(img, np.array(taps[:n], np.float32))
+    np.testing.assert_array_equal(host(out), ref)
+
+
+@pytest.mark.parametrize("w,h", [(96, 64), (130, 90)])
+def test_fourier_highpass(isa, oracle, gpu, w, h):
+    rng = np.random.default_rng(5)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = (0.5 + 0.3 * np.sin(xx / 17.0) * np.cos(yy / 23.0) + 0.05 * rng.standard_normal((h, w))).astype(np.float32)
+    hp, hps = 0.01, 0.0025
+    ws_bytes = isa.lib().isa_fourier_highpass_workspace(w, h, hp, hps)
+    ws = empty((ws_bytes // 4 + 64,))
+    d = dev(img)
+    isa.call("isa_fourier_highpass", p(d), w * 4, w, h, 0, C.c_float(hp), C.c_float(hps), p(ws), stream())
+    ref = oracle.fourier_highpass(img, hp, hps, dt=np.float64)
+    err = np.abs(host(d).astype(np.float64) - ref).max()
+    assert err < 2e-5, err
+
+
+def test_resize_box(isa, oracle, gpu):
+    rng = np.random.default_rng(6)
+    img = rng.uniform(0, 1, (64, 96)).astype(np.float32)
+    for l in (2, 4, 8):
+        out = empty((64 // l, 96 // l))
+        isa.call("isa_resize_box", p(dev(img)), 96 * 4, p(out), (96 // l) * 4, 96, 64, l, stream())
+        np.testing.assert_array_equal(host(out), oracle.downscale_box(img, l))
+
+
+def _track_inputs(rng, h=80, w=112):
+    yy, xx = np.mgrid[0:h, 0:w]
+    a = (0.5 + 0.2 * np.sin(xx / 3.1) * np.cos(yy / 2.7) + 0.05 * rng.standard_normal((h, w))).astype(np.float32)
+    b = np.roll(a, (1, -2), axis=(0, 1)) + (0.01 * rng.standard_normal((h, w))).astype(np.float32)
+    return a, b.astype(np.float32)
+
+
+@pytest.mark.parametrize("tile,M", [(16, 2), (16, 4), (32, 3)])
+def test_tiles_ssd_findmin(isa, oracle, gpu, tile, M):
+    rng = np.random.default_rng(7 + M)
+    h, w = 80, 112
+    a, b = _track_inputs(rng, h, w)
+    cx, cy = (w - 2 * M) // tile, (h - 2 * M) // tile
+    B, S = tile + 2 * M, 2 * M + 1
+    pre = rng.uniform(-3, 3, (cy, cx, 2)).astype(np.float32)
+    bs = (1.25, -0.5)
+    rt, tt = empty((cx * cy, B, B)), empty((cx * cy, B, B))
+    isa.call("isa_convert_to_tiles_border", p(dev(a)), w * 4, p(rt), w, h, M, tile, cx, cy, isa.f32s(*bs),
+             C.c_float(0.0), stream())
+    isa.call("isa_convert_to_tiles_preshift", p(dev(b)), w * 4, p(tt), p(dev(pre)), cx * 8, w, h, M, tile, cx, cy,
+             isa.f32s(*bs), C.c_float(0.0), stream())
+    rt_ref = oracle.tiles_border(a, M, tile, cx, cy, bs)
+    tt_ref = oracle.tiles_preshift(b, pre, M, tile, cx, cy, bs)
+    np.testing.assert_array_equal(host(rt), rt_ref)
+    np.testing.assert_array_equal(host(tt), tt_ref)
+    simg = empty((cx * cy, S, S))
+    isa.call("isa_tile_ssd", p(rt), p(tt), p(simg), M, tile, cx * cy, stream())
+    s_ref = oracle.tile_ssd(rt_ref, tt_ref, M, tile)
+    np.testing.assert_array_equal(host(simg), s_ref)  # FP64 in the same order: bit-exact
+    coords = empty((cy, cx, 2))
+    isa.call("isa_find_minimum", p(simg), p(coords), cx * 8, M, cx, cy, C.c_float(1.0), stream())
+    np.testing.assert_array_equal(host(coords).reshape(-1, 2), oracle.find_minimum(s_ref, M, 1.0))
+    # fused PatchTracker.Track
+    d_pre = dev(pre)
+    isa.call("isa_track", p(dev(b)), p(dev(a)), w * 4, p(d_pre), cx * 8, w, h, tile, M, isa.f32s(*bs), C.c_float(0),
+             isa.f32s(*bs), C.c_float(0), C.c_float(1.0), stream())
+    np.testing.assert_array_equal(host(d_pre).reshape(-1, 2), pre.reshape(-1, 2) + oracle.find_minimum(s_ref, M, 1.0))
+
+
+def test_upsample_shifts(isa, oracle, gpu):
+    rng = np.random.default_rng(8)
+    ocx, ocy, ncx, ncy = 7, 5, 15, 11
+    inp = rng.uniform(-4, 4, (ocy, ocx, 2)).astype(np.float32)
+    out = empty((ncy, ncx, 2))
+    isa.call("isa_upsample_shifts", p(dev(inp)), p(out), ocx * 8, ncx * 8, 4, 2, ocx, ocy, ncx, ncy, 32, 32, stream())
+    np.testing.assert_array_equal(host(out), oracle.upsample_shifts(inp, 4, 2, ocx, ocy, ncx, ncy, 32, 32))
+
+
+@pytest.mark.parametrize("n,strategy,block", [(5, 0, 5), (8, 0, 5), (6, 1, 5), (9, 2, 4), (7, 3, 3)])
+def test_minimize_shifts(isa, oracle, gpu, n, strategy, block):
+    rng = np.random.default_rng(9 + n)
+    ref = 0 if strategy != 2 else 3
+    A, pairs, index, rows = oracle.shift_tables(n, ref, strategy, block)
+    m = A.shape[0]
+    tiles = 37
+    x = np.concatenate([np.zeros((tiles, 1, 2)), rng.uniform(-6, 6, (tiles, n - 1, 2))], axis=1)
+    d = x[:, rows[:, 1]] - x[:, rows[:, 0]] + rng.normal(0, 0.05, (tiles, m, 2))
+    # outliers in some tiles
+    for t in range(0, tiles, 3):
+        d[t, rng.integers(0, m)] += rng.uniform(3, 8, 2)
+    d = d.astype(np.float32).astype(np.float64)
+    dh_ref, r_ref, st_ref, it_ref = oracle.minimize_shifts(d, n, ref, strategy, block)
+    meas = dev(np.ascontiguousarray(np.transpose(d, (1, 0, 2))).astype(np.float32))  # [m][tile][2]
+    r = empty((tiles, n - 1, 2))
+    st = empty((tiles,), dtype=__import__("torch").int32)
+    it = empty((tiles,), dtype=__import__("torch").int32)
+    isa.call("isa_minimize_shifts", p(meas), tiles * 8, p(r), p(st), p(it), tiles, n, ref, strategy, block, stream())
+    np.testing.assert_array_equal(host(st), st_ref)
+    np.testing.assert_array_equal(host(it), it_ref)
+    np.testing.assert_allclose(host(r), r_ref, atol=2e-5)
+    np.testing.assert_allclose(np.transpose(host(meas), (1, 0, 2)), dh_ref, atol=2e-5)
+
+
+def test_get_optimal_shifts(isa, oracle, gpu):
+    rng = np.random.default_rng(10)
+    n, cx, cy = 6, 5, 4
+    r = rng.uniform(-3, 3, (cx * cy, n - 1, 2)).astype(np.float32)
+    for ref, trk in [(0, 4), (3, 1), (2, 2)]:
+        out = empty((cy, cx, 2))
+        isa.call("isa_get_optimal_shifts", p(out), cx * 8, p(dev(r)), n, cx, cy, ref, trk, stream())
+        np.testing.assert_array_equal(host(out).reshape(-1, 2), oracle.get_optimal_shifts(r, n, ref, trk))
+
+
+@pytest.mark.parametrize("quant", [0, 1])
+def test_flow_warp_derivatives(isa, oracle, gpu, quant):
+    rng = np.random.default_rng(11)
+    h, w = 48, 64
+    tiles = rng.uniform(-2, 2, (3, 4, 2)).astype(np.float32)
+    flow = empty((h, w, 2))
+    isa.call("isa_create_flow_field_from_tiles", p(flow), w * 8, p(dev(tiles)), 4 * 8, 4, 3, w, h,
+             isa.f32s(0.5, -0.25), C.c_float(0.0), quant, stream())
+    fref = np.zeros((h, w, 2), np.float32)
+    oracle.fn("flow_from_tiles")(oracle.ptr(tiles), 4, 3, oracle.ptr(fref), w, h, 0.5, -0.25, 0.0, quant)
+    np.testing.assert_array_equal(host(flow), fref)
+    src, tgt = _track_inputs(rng, h, w)
+    out = empty((h, w))
+    isa.call("isa_warp", p(dev(src)), w * 4, p(out), w * 4, p(flow), w * 8, w, h, quant, stream())
+    wref = np.zeros((h, w), np.float32)
+    oracle.fn("warp")(oracle.ptr(src), oracle.ptr(fref), oracle.ptr(wref), w, h, quant)
+    np.testing.assert_array_equal(host(out), wref)
+    ix, iy, iz = empty((h, w)), empty((h, w)), empty((h, w))
+    isa.call("isa_compute_derivatives", p(dev(wref)), p(dev(tgt)), w * 4, p(ix), p(iy), p(iz), w * 4, w, h, quant,
+             stream())
+    rx, ry, rz = (np.zeros((h, w), np.float32) for _ in range(3))
+    oracle.fn("derivatives")(oracle.ptr(wref), oracle.ptr(tgt), oracle.ptr(rx), oracle.ptr(ry), oracle.ptr(rz), w, h,
+                             quant)
+    for a, b in ((ix, rx), (iy, ry), (iz, rz)):
+        np.testing.assert_array_equal(host(a), b)
+    isa.call("isa_compute_derivatives2", p(dev(tgt)), w * 4, p(ix), p(iy), w * 4, w, h, quant, stream())
+    oracle.fn("derivatives2")(oracle.ptr(tgt), oracle.ptr(rx), oracle.ptr(ry), w, h, quant)
+    np.testing.assert_array_equal(host(ix), rx)
+    np.testing.assert_array_equal(host(iy), ry)
+
+
+def test_lucas_kanade_step(isa, oracle, gpu):
+    rng = np.random.default_rng(12)
+    h, w = 48, 64
+    src, tgt = _track_inputs(rng, h, w)
+    rx, ry, rz = (np.zeros((h, w), np.float32) for _ in range(3))
+    oracle.fn("derivatives")(oracle.ptr(src), oracle.ptr(tgt), oracle.ptr(rx), oracle.ptr(ry), oracle.ptr(rz), w, h, 0)
+    flow0 = rng.uniform(-0.5, 0.5, (h, w, 2)).astype(np.float32)
+    f = dev(flow0)
+    isa.call("isa_lucas_kanade_optim", p(f), w * 8, p(dev(rx)), p(dev(ry)), p(dev(rz)), w * 4, w, h, 11,
+             C.c_float(0.01), stream())
+    ref = flow0.copy()
+    oracle.fn("lucas_kanade_step")(oracle.ptr(ref), oracle.ptr(rx), oracle.ptr(ry), oracle.ptr(rz), w, h, 5, 0.01)
+    np.testing.assert_allclose(host(f), ref, rtol=1e-4, atol=2e-5)
+
+
+def test_structure_tensor_kernel_param(isa, oracle, gpu):
+    rng = np.random.default_rng(13)
+    h, w = 20, 30
+    dx, dy = (rng.normal(0, 0.05, (h, w)).astype(np.float32) for _ in range(2))
+    dx[0, :5] = 0
+    dy[0, :5] = 0  # zero gradient: NaN kernel (cross fallback later)
+    t = empty((h, w, 3))
+    isa.call("isa_compute_structure_tensor", p(dev(dx)), p(dev(dy)), w * 4, p(t), w * 12, w, h, stream())
+    tref = np.zeros((h, w, 3), np.float32)
+    oracle.fn("structure_tensor")(oracle.ptr(dx), oracle.ptr(dy), oracle.ptr(tref), w, h)
+    np.testing.assert_array_equal(host(t), tref)
+    args = (0.001, 0.006, 0.25, 3.0, 4.0, 2.0)
+    isa.call("isa_compute_kernel_param", p(t), w, h, w * 12, *[C.c_float(a) for a in args], stream())
+    oracle.fn("kernel_param")(oracle.ptr(tref), w, h, *args)
+    np.testing.assert_array_equal(host(t), tref)
+
+
+def test_subsample_robustness_erode(isa, oracle, gpu):
+    rng = np.random.default_rng(14)
+    h, w = 64, 80
+    hw, hh = w // 2, h // 2
+    ra, rb = _raw(rng, h, w), _raw(rng, h, w)
+    rb[20:40, 30:50] = np.clip(rb[20:40, 30:50].astype(int) + 3000, 0, 16383)
+    ha, hb = empty((hh, hw, 3)), empty((hh, hw, 3))
+    isa.call("isa_debayer_subsample3", p(dev(ra)), p(ha), C.c_float(16383.0), hw, hh, hw * 12, isa.i32s(*RGGB),
+             stream())
+    isa.call("isa_debayer_subsample3", p(dev(rb)), p(hb), C.c_float(16383.0), hw, hh, hw * 12, isa.i32s(*RGGB),
+             stream())
+    sa, sb = np.zeros((hh, hw, 3), np.float32), np.zeros((hh, hw, 3), np.float32)
+    c = oracle.arr(RGGB, np.int32)
+    oracle.fn("subsample3")(oracle.ptr(ra), oracle.ptr(sa), 16383.0, hw, hh, oracle.ptr(c))
+    oracle.fn("subsample3")(oracle.ptr(rb), oracle.ptr(sb), 16383.0, hw, hh, oracle.ptr(c))
+    np.testing.assert_array_equal(host(ha), sa)
+    np.testing.assert_array_equal(host(hb), sb)
+    flow = rng.uniform(-3, 3, (h, w, 2)).astype(np.float32)
+    for quant in (0, 1):
+        mask = empty((hh, hw, 4), fill=1.0)
+        alpha, beta = 0.5 / 16383, 9 / 16383 ** 2
+        isa.call("isa_compute_robustness_mask", p(ha), p(hb), hw * 12, p(mask), hw * 16, p(dev(flow)), w * 8, w, h,
+                 hw, hh, C.c_float(alpha), C.c_float(beta), C.c_float(1e5), quant, stream())
+        mref = np.ones((hh, hw, 4), np.float32)
+        oracle.fn("robustness_mask")(oracle.ptr(sa), oracle.ptr(sb), oracle.ptr(mref), oracle.ptr(flow), w, h, hw, hh,
+                                     alpha, beta, 1e5, quant)
+        assert ulp_diff(host(mask), mref) <= 8
+        er = empty((hh, hw, 4))
+        isa.call("isa_erode_border4", p(dev(mref)), hw * 16, p(er), hw * 16, hw, hh, 5, stream())
+        eref = np.zeros_like(mref)
+        oracle.fn("erode")(oracle.ptr(mref), oracle.ptr(eref), hw, hh, 5)
+        np.testing.assert_array_equal(host(er), eref)
+
+
+@pytest.mark.parametrize("sr", [0, 1])
+def test_accumulate_apply_weighting(isa, oracle, gpu, sr):
+    rng = np.random.default_rng(15 + sr)
+    h, w = 48, 64
+    hw, hh = w // 2, h // 2
+    raw = _raw(rng, h, w)
+    kch = 4 if sr else 3
+    kern = np.zeros((h, w, kch), np.float32)
+    kern[..., 0] = rng.uniform(0.2, 2.0, (h, w))
+    kern[..., 1] = rng.uniform(0.2, 2.0, (h, w))
+    kern[..., 2] = rng.uniform(-0.1, 0.1, (h, w))
+    kern[3, 3, :3] = np.nan
+    mask = rng.uniform(0, 1, (hh, hw, 4)).astype(np.float32)
+    flow = rng.uniform(-2.6, 2.6, (h, w, 2)).astype(np.float32)
+    s0 = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    w0 = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    white, black = [16383.0, 16000.0, 15000.0], [10.0, 20.0, 30.0]
+    ds, dw = dev(s0), dev(w0)
+    if sr:
+        isa.call("isa_accumulate_images_superres", p(dev(raw)), p(ds), p(dw), p(dev(mask)), p(dev(kern)),
+                 p(dev(flow)), isa.f32s(*white), isa.f32s(*black), w, h, w * 12, hw * 16, w * 16, w * 8,
+                 isa.i32s(*RGGB), 0, stream())
+    else:
+        isa.call("isa_accumulate_images", p(dev(raw)), p(ds), p(dw), p(dev(mask)), p(dev(kern)), p(dev(flow)),
+                 isa.f32s(*white), isa.f32s(*black), w, h, w * 12, hw * 16, w * 8, isa.i32s(*RGGB), stream())
+    sr_, wr_ = s0.copy(), w0.copy()
+    args = [oracle.ptr(raw), oracle.ptr(sr_), oracle.ptr(wr_), oracle.ptr(mask), oracle.ptr(kern), oracle.ptr(flow),
+            oracle.ptr(oracle.arr(white, np.float32)), oracle.ptr(oracle.arr(black, np.float32)), w, h,
+            oracle.ptr(oracle.arr(RGGB, np.int32))]
+    if sr:
+        oracle.fn("accumulate_sr")(*args, 0)
+    else:
+        oracle.fn("accumulate")(*args)
+    np.testing.assert_allclose(host(ds), sr_, rtol=2e-6, atol=1e-6)
+    np.testing.assert_allclose(host(dw), wr_, rtol=2e-6, atol=1e-6)
+    io = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    dio = dev(io)
+    isa.call("isa_apply_weighting", p(dio), p(dev(sr_)), p(dev(wr_)), w, h, w * 12, C.c_float(1.0), stream())
+    ioref = io.copy()
+    oracle.fn("apply_weighting")(oracle.ptr(ioref), oracle.ptr(sr_), oracle.ptr(wr_), w, h, 1.0)
+    np.testing.assert_array_equal(host(dio), ioref)

@@ -1,0 +1,74 @@
+"""GPU end-to-end: the device-resident merge (isa_merger_run) against the CPU
+oracle's pipeline (reference semantics, FP32) on seeded synthetic bursts.
+
+The tile search is exact (FP64, canonical order), so the per-tile shifts agree
+to float rounding. The LK step in the merge uses separable box sums for its
+window terms (a different summation order than the reference's per-pixel
+loops), so flows agree to ~1e-4 px and a few pixels may round to a different
+integer sample in robustness/accumulation; the merged image is therefore
+compared by PSNR and by the fraction of pixels within 1e-4."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from tests.gpu_util import host  # noqa: E402
+
+
+def _psnr(a, b, peak=1.0):
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return 99.0 if mse == 0 else 10 * np.log10(peak * peak / mse)
+
+
+CASES = {
+    "c0_small": dict(w=256, h=192, n=5, gmax=4.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0),
+    "three_levels": dict(w=320, h=256, n=4, gmax=10.0, levels=[(1, 32, 2), (2, 32, 4), (4, 16, 4)], sr=0, occ=0.05),
+    "superres": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=1, occ=0.0),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_merge_matches_oracle(isa, oracle, gpu, case):
+    c = CASES[case]
+    cfg = isa.default_synth(seed=2024 + len(case), width=c["w"], height=c["h"], frames=c["n"],
+                            global_shift_max=c["gmax"], occluded_fraction=c["occ"])
+    raw_t, _ = isa.synth_burst(cfg)
+    raw = host(raw_t).view(np.uint16)
+    assert raw.max() <= 16383
+    params = isa.default_params(levels=c["levels"], super_resolution=c["sr"])
+    m = isa.BurstMerger(c["w"], c["h"], c["n"], params)
+    out = host(m.run(raw_t))
+    ref = oracle.merge(params, raw, want=("one_to_one", "flows", "masks", "kern"))
+    assert m.reference_index == ref["ref"] == 0
+    # tile alignment: finest-level one-to-one shifts
+    r = host(m.buffer_tensor(2, 0, 2))  # (tiles, n1, 2)
+    np.testing.assert_allclose(r.reshape(ref["one_to_one"].shape), ref["one_to_one"], atol=1e-4)
+    # kernel parameters of the reference frame
+    kern = host(m.buffer_tensor(5, 0, 4 if c["sr"] else 3))[..., :3]
+    fin = np.isfinite(ref["kern"])
+    np.testing.assert_array_equal(np.isfinite(kern), fin)
+    np.testing.assert_allclose(kern[fin], ref["kern"][fin], rtol=1e-5, atol=1e-6)
+    # flows
+    for f in range(1, c["n"]):
+        fl = host(m.buffer_tensor(3, f, 2))
+        d = np.abs(fl - ref["flows"][f])
+        assert np.sqrt(np.mean(d ** 2)) < 5e-4, (f, np.sqrt(np.mean(d ** 2)))
+        assert np.quantile(d, 0.999) < 5e-3
+    # merged image
+    fin = np.isfinite(ref["out"]) & np.isfinite(out)
+    assert fin.mean() > 0.999
+    p = _psnr(out[fin], ref["out"][fin])
+    close = np.mean(np.abs(out[fin] - ref["out"][fin]) < 1e-4)
+    assert p > 60.0 and close > 0.99, (p, close)
+
+
+def test_merge_is_deterministic_and_reusable(isa, gpu):
+    cfg = isa.default_synth(seed=77, width=256, height=192, frames=4, global_shift_max=5.0)
+    raw_t, _ = isa.synth_burst(cfg)
+    params = isa.default_params(levels=[(1, 32, 2), (2, 32, 4)])
+    m = isa.BurstMerger(256, 192, 4, params)
+    a = host(m.run(raw_t)).copy()
+    b = host(m.run(raw_t)).copy()
+    np.testing.assert_array_equal(a, b)
+    st = m.stage_ms()
+    assert set(st) >= {"tracking_images", "track_patches", "accumulate"}
