@@ -81,6 +81,16 @@ __device__ __forceinline__ void tex_linear_all(const float* img, int pitch, int 
 }
 
 // ---------------------------------------------------------------------------
+// Transcendentals evaluated in double and rounded once to float. The CPU
+// oracle does the same, so both sides agree bit for bit except when the two
+// double results straddle a float rounding boundary (probability ~2^-28 per call).
+__device__ __forceinline__ float exp_r(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ float atan2_r(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ float sin_r(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float cos_r(float x) { return (float)cos((double)x); }
+
+// ---------------------------------------------------------------------------
 // Per-pixel formulas shared by the unfused (one-kernel-per-reference-kernel)
 // and fused entry points, so that both evaluate identical operations.
 
@@ -89,7 +99,7 @@ __device__ __forceinline__ float srgb_prepared(float v) {
     if (isnan(v)) v = 0.0f;
     v = fmaxf(fminf(v, 1.0f), 0.0f);
     if (v <= 0.0031308f) return 12.92f * v;
-    return (1.0f + 0.055f) * powf(v, 1.0f / 2.4f) - 0.055f;
+    return (1.0f + 0.055f) * pow_r(v, 1.0f / 2.4f) - 0.055f;
 }
 
 // Accumulation weight of offset (px, py) for inverse covariance (k0, k1, k2)
@@ -97,7 +107,7 @@ __device__ __forceinline__ float srgb_prepared(float v) {
 // (R:Kernels/DeBayerKernels.cu:335-338).
 __device__ __forceinline__ float kernel_weight(int px, int py, float k0, float k1, float k2) {
     float w = (float)(px * px) * k0 + (float)(2 * px * py) * k2 + (float)(py * py) * k1;
-    w = expf(-0.5f * w);
+    w = exp_r(-0.5f * w);
     if (!isfinite(w)) w = (px * py == 0) ? 1.0f : 0.0f;
     return w;
 }

@@ -13,7 +13,7 @@ constexpr int kMaxShift = 8;  // largest supported MaxShift (search window (2M+1
 // Integer offset of a tile (R:Kernels/kernel.cu:183-202 and :238-261): round(preshift + R(theta)(-b) + R(theta)c - c).
 __device__ __forceinline__ void tile_offset(int tx, int ty, int tile, int w, int h, float bsx, float bsy, float rot,
                                             float psx, float psy, int& ox, int& oy) {
-    float sf = sinf(rot), cf = cosf(rot);
+    float sf = sin_r(rot), cf = cos_r(rot);
     float sx = psx + (cf * -bsx - sf * -bsy);
     float sy = psy + (sf * -bsx + cf * -bsy);
     float pcx = (float)(tx * tile + tile / 2 - w / 2);
@@ -471,6 +471,7 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
         double best = 1.0;
         int bi = -1;
         for (int k = tid; k < m; k += kMinThreads) {
+            if (!active[k]) continue;  // a removed row has d = 0 and a zero row of A: residual 0
             int ra = a.rows[2 * k], rb = a.rows[2 * k + 1];
             double ex = d[2 * k] - (x[2 * rb] - x[2 * ra]), ey = d[2 * k + 1] - (x[2 * rb + 1] - x[2 * ra + 1]);
             double dist = ex * ex + ey * ey;
@@ -496,6 +497,7 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
         }
         __syncthreads();
         status = s_flag;
+        __syncthreads();  // every lane has read s_flag before lane 0 reuses it
         if (status < 0) break;
     }
     // outputs: r_i = x_{i+1} - x_i; every measurement := x_b - x_a (A_safe r)

@@ -10,7 +10,7 @@ __global__ void k_flow_from_tiles(float* flow, int flow_pitch, const float* tile
                                   int w, int h, float bsx, float bsy, float rot, int quant) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
-    float c = cosf(rot), s = sinf(rot);
+    float c = cos_r(rot), s = sin_r(rot);
     float fx = c * -bsx - s * -bsy;
     float fy = s * -bsx + c * -bsy;
     float pcx = (float)(x - w / 2), pcy = (float)(y - h / 2);
@@ -78,10 +78,14 @@ __global__ void k_derivatives2(const float* img, int img_pitch, float* Ix, float
 
 // Closed-form 2x2 SVD pseudo-inverse of [[a, b], [b, d]] (R:Kernels/opticalFlow.cu:236-292).
 // Returns false when the gate sigma1 < minDet holds (the reference's fminf(sigma1, sigma1)).
+template <bool EXACT>
 __device__ __forceinline__ bool lk_pinv(float a, float b, float d, float min_det, float inv[4]) {
+    auto at2 = [](float y, float x) { return EXACT ? atan2_r(y, x) : atan2f(y, x); };
+    auto cs = [](float x) { return EXACT ? cos_r(x) : cosf(x); };
+    auto sn = [](float x) { return EXACT ? sin_r(x) : sinf(x); };
     float c = b;
-    float theta = 0.5f * atan2f(2.0f * a * c + 2.0f * b * d, a * a + b * b - c * c - d * d);
-    float ct = cosf(theta), st = sinf(theta);
+    float theta = 0.5f * at2(2.0f * a * c + 2.0f * b * d, a * a + b * b - c * c - d * d);
+    float ct = cs(theta), st = sn(theta);
     float UT0 = ct, UT2 = -st, UT1 = st, UT3 = ct;
     float S1 = a * a + b * b + c * c + d * d;
     float S2 = sqrtf((a * a + b * b - c * c - d * d) * (a * a + b * b - c * c - d * d) +
@@ -93,8 +97,8 @@ __device__ __forceinline__ bool lk_pinv(float a, float b, float d, float min_det
     sigma1 = sigma1 != 0 ? 1.0f / sigma1 : 0.0f;
     sigma2 = sigma2 != 0 ? 1.0f / sigma2 : 0.0f;
     float S0 = sigma1, Sb = 0.0f, Sc = 0.0f, S3 = sigma2;
-    float eps = 0.5f * atan2f(2.0f * a * b + 2.0f * c * d, a * a - b * b + c * c - d * d);
-    float ce = cosf(eps), se = sinf(eps);
+    float eps = 0.5f * at2(2.0f * a * b + 2.0f * c * d, a * a - b * b + c * c - d * d);
+    float ce = cs(eps), se = sn(eps);
     float s11 = (a * ct + c * st) * ce + (b * ct + d * st) * se;
     float s22 = (a * st - c * ct) * se + (-b * st + d * ct) * ce;
     s11 = s11 > 0.0f ? 1.0f : s11 < 0 ? -1.0f : 0.0f;
@@ -126,7 +130,7 @@ __global__ void k_lk_step(float* flow, int flow_pitch, const float* Fx, const fl
         }
     }
     float inv[4];
-    if (!lk_pinv(g0, g1, g3, min_det, inv)) return;
+    if (!lk_pinv<true>(g0, g1, g3, min_det, inv)) return;
     float u0 = 0, u1 = 0;
     for (int i = 0; i < ws * ws; ++i) {
         int yy = i / ws, xx = i - yy * ws;
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
         g[q] = s;
     }
     float inv[4];
-    if (!lk_pinv(g[0], g[1], g[2], min_det, inv)) return;
+    if (!lk_pinv<false>(g[0], g[1], g[2], min_det, inv)) return;
     float u0 = inv[0] * g[3] + inv[1] * g[4];
     float u1 = inv[2] * g[3] + inv[3] * g[4];
     u0 = isnan(u0) ? 0.0f : u0;

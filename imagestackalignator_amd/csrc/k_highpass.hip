@@ -65,16 +65,16 @@ __device__ float filter_value(int kx, int ky, int w, int h, float lp, float hp, 
     if (lp > 0) { if (dist <= lp) fil = 1; }
     else { if (dist <= 1.0f) fil = 1; }
     if (lps > 0) {
-        float fil2 = (-fil + 1.0f) * expf(-((dist - lp) * (dist - lp) / (2.0f * lps * lps)));
+        float fil2 = (-fil + 1.0f) * exp_r(-((dist - lp) * (dist - lp) / (2.0f * lps * lps)));
         if (fil2 > 0.001f) fil = fil2;
     }
-    if (lps > 0 && lp == 0 && hp == 0 && hps == 0) fil = expf(-((dist - lp) * (dist - lp) / (2.0f * lps * lps)));
+    if (lps > 0 && lp == 0 && hp == 0 && hps == 0) fil = exp_r(-((dist - lp) * (dist - lp) / (2.0f * lps * lps)));
     if (hp > 0) {
         float fil2 = 0;
         if (dist >= hp) fil2 = 1;
         fil *= fil2;
         if (hps > 0) {
-            float fil3 = (-fil2 + 1.0f) * expf(-((dist - hp) * (dist - hp) / (2.0f * hps * hps)));
+            float fil3 = (-fil2 + 1.0f) * exp_r(-((dist - hp) * (dist - hp) / (2.0f * hps * hps)));
             if (fil3 > 0.001f) fil = fil3;
         }
     }

@@ -90,7 +90,7 @@ __global__ void k_robustness(const float* ref, const float* mov, int img_pitch, 
         float dist = fabsf(mr[c] - mm[c]);
         float sig = fmaxf(smd, sd);
         dist = dist * (sd * sd / (sd * sd + smd * smd));
-        out[c] = fmaxf(fminf(s * expf(-dist * dist / (sig * sig)) - 0.12f, 1.0f), 0.0f);
+        out[c] = fmaxf(fminf(s * exp_r(-dist * dist / (sig * sig)) - 0.12f, 1.0f), 0.0f);
     }
     out[3] = Mv;
     float* o = row_ptr(mask, mask_pitch, py) + px * 4;

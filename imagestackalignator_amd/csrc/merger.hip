@@ -117,7 +117,7 @@ struct isa_merger {
     size_t tab_bytes = 0;
     hipEvent_t upload_done = nullptr;
     hipEvent_t ev[ST_COUNT + 1]{};
-    hipEvent_t lev_ev[ISA_MAX_LEVELS][3]{};  // per level: track start, minimizer start, minimizer end
+    hipEvent_t lev_ev[ISA_MAX_LEVELS][4]{};  // per level: start, track launch, minimizer launch, end
     float stage_ms[ST_COUNT]{};
     bool have_times = false;
     int ref = 0;
@@ -293,7 +293,7 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
     if (rc == ISA_OK) rc = isa::check_hip(hipEventCreateWithFlags(&M->upload_done, hipEventDisableTiming), "event");
     for (int i = 0; i <= ST_COUNT && rc == ISA_OK; ++i) rc = isa::check_hip(hipEventCreate(&M->ev[i]), "event");
     for (int i = 0; i < M->L && rc == ISA_OK; ++i)
-        for (int j = 0; j < 3 && rc == ISA_OK; ++j) rc = isa::check_hip(hipEventCreate(&M->lev_ev[i][j]), "event");
+        for (int j = 0; j < 4 && rc == ISA_OK; ++j) rc = isa::check_hip(hipEventCreate(&M->lev_ev[i][j]), "event");
     if (rc == ISA_OK) rc = isa::check_hip(hipEventRecord(M->upload_done, 0), "event");
     if (rc != ISA_OK) {
         delete M;
@@ -413,14 +413,15 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             for (int f = 0; f < n; ++f)
                 ISA_TRY(isa::launch_resize_box(M->trk + (size_t)f * np, wp, M->lev[i] + (size_t)f * lw * lh,
                                                lw * (int)sizeof(float), w, h, l, s));
+        (void)hipEventRecord(M->lev_ev[i][1], s);
         ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
                                         P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk, m,
                                         d_base + (size_t)i * n * 3, M->shifts, mstride, P.matching_threshold, nullptr,
                                         s));
-        (void)hipEventRecord(M->lev_ev[i][1], s);
+        (void)hipEventRecord(M->lev_ev[i][2], s);
         ISA_TRY(isa::launch_minimize(M->shifts, mstride, M->one_to_one, nullptr, nullptr, d_rows, M->cx[i] * M->cy[i], n,
                                      m, s));
-        (void)hipEventRecord(M->lev_ev[i][2], s);
+        (void)hipEventRecord(M->lev_ev[i][3], s);
     }
     (void)hipEventRecord(M->ev[2], s);
     (void)hipEventRecord(M->ev[3], s);
@@ -519,8 +520,8 @@ int isa_merger_stage_ms(isa_merger* M, float* ms, int capacity) {
     float tr = 0, mn = 0;
     for (int i = 0; i < M->L; ++i) {
         float a = 0, b = 0;
-        (void)hipEventElapsedTime(&a, M->lev_ev[i][0], M->lev_ev[i][1]);
-        (void)hipEventElapsedTime(&b, M->lev_ev[i][1], M->lev_ev[i][2]);
+        (void)hipEventElapsedTime(&a, M->lev_ev[i][0], M->lev_ev[i][2]);
+        (void)hipEventElapsedTime(&b, M->lev_ev[i][2], M->lev_ev[i][3]);
         tr += a;
         mn += b;
     }
@@ -529,6 +530,17 @@ int isa_merger_stage_ms(isa_merger* M, float* ms, int capacity) {
     int k = 0;
     for (; k < ST_COUNT && k < capacity; ++k) ms[k] = t[k];
     return k;
+}
+
+int isa_merger_level_ms(isa_merger* M, int level, float ms[3]) {
+    ISA_REQUIRE(M && ms && level >= 0 && level < M->L, "isa_merger_level_ms: bad arguments");
+    if (!M->have_times) return ISA_ERR_ARG;
+    ISA_TRY(isa::check_hip(hipEventSynchronize(M->ev[ST_COUNT]), "isa_merger_level_ms: sync"));
+    for (int j = 0; j < 3; ++j) {
+        ms[j] = 0;
+        (void)hipEventElapsedTime(&ms[j], M->lev_ev[level][j], M->lev_ev[level][j + 1]);
+    }
+    return ISA_OK;
 }
 
 int isa_merger_buffer(isa_merger* M, int which, int index, void** ptr, int* pitch, int* width, int* height) {

@@ -271,6 +271,9 @@ int isa_merger_buffer(isa_merger* m, int which, int index, void** ptr, int* pitc
  * 7 final weighting. Returns the number of stages written. */
 int isa_merger_reference_index(isa_merger* m);
 int isa_merger_stage_ms(isa_merger* m, float* ms, int capacity);
+/* Milliseconds of the last run at pyramid level `level` (0 = finest): ms[0] shift up-sampling and level
+ * images, ms[1] the fused tile-search launch (all pairs), ms[2] the ShiftMinimizer launch. */
+int isa_merger_level_ms(isa_merger* m, int level, float ms[3]);
 
 /* ------------------------------------------------------------------------
  * Synthetic bursts (benchmark / test input; not part of the reference).

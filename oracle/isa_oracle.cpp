@@ -99,7 +99,7 @@ void debayer_full(int w, int h, const uint16_t* raw, T* out, const T black[3], c
 //     (R:...Controller.cs:2404-2408).
 template <class T> T srgb_oetf(T v) {
     if (v <= (T)0.0031308f) return (T)12.92f * v;
-    return ((T)1.0f + (T)0.055f) * std::pow(v, (T)1.0f / (T)2.4f) - (T)0.055f;
+    return ((T)1.0f + (T)0.055f) * pow_r(v, (T)1.0f / (T)2.4f) - (T)0.055f;
 }
 
 template <class T> void gamma_srgb(int w, int h, T* img3) {
@@ -137,16 +137,16 @@ template <class T> T fourier_filter_value(int kx, int ky, int w, int h, T lp, T 
     if (lp > 0) { if (dist <= lp) fil = 1; }
     else { if (dist <= (T)1.0f) fil = 1; }
     if (lps > 0) {
-        T fil2 = (-fil + (T)1.0f) * (T)std::exp(-((dist - lp) * (dist - lp) / ((T)2 * lps * lps)));
+        T fil2 = (-fil + (T)1.0f) * exp_r(-((dist - lp) * (dist - lp) / ((T)2 * lps * lps)));
         if (fil2 > (T)0.001f) fil = fil2;
     }
-    if (lps > 0 && lp == 0 && hp == 0 && hps == 0) fil = (T)std::exp(-((dist - lp) * (dist - lp) / ((T)2 * lps * lps)));
+    if (lps > 0 && lp == 0 && hp == 0 && hps == 0) fil = exp_r(-((dist - lp) * (dist - lp) / ((T)2 * lps * lps)));
     if (hp > 0) {
         T fil2 = 0;
         if (dist >= hp) fil2 = 1;
         fil *= fil2;
         if (hps > 0) {
-            T fil3 = (-fil2 + (T)1.0f) * (T)std::exp(-((dist - hp) * (dist - hp) / ((T)2 * hps * hps)));
+            T fil3 = (-fil2 + (T)1.0f) * exp_r(-((dist - hp) * (dist - hp) / ((T)2 * hps * hps)));
             if (fil3 > (T)0.001f) fil = fil3;
         }
     }
@@ -327,7 +327,7 @@ template <class T> void downscale_box(int w, int h, const T* in, int level, T* o
 //     :230-288 (track, pre-shifted). Tiles are (T+2M)^2, tile-major.
 template <class T>
 static void tile_offset(int tx, int ty, int tile, int w, int h, T bsx, T bsy, T rot, T psx, T psy, int& ox, int& oy) {
-    T sf = std::sin(rot), cf = std::cos(rot);
+    T sf = sin_r(rot), cf = cos_r(rot);
     T sx = psx + (cf * -bsx - sf * -bsy);
     T sy = psy + (sf * -bsx + cf * -bsy);
     T pcx = (T)(tx * tile + tile / 2 - w / 2);
@@ -780,7 +780,7 @@ void flow_from_tiles(const T* tiles, int tw, int th, T* flow, int w, int h, T bs
     parallel_for(h, [&](int y0, int y1) {
         for (int y = y0; y < y1; ++y)
             for (int x = 0; x < w; ++x) {
-                T c = std::cos(rot), s = std::sin(rot);
+                T c = cos_r(rot), s = sin_r(rot);
                 T fx = c * -bsx - s * -bsy;
                 T fy = s * -bsx + c * -bsy;
                 T pcx = (T)(x - w / 2), pcy = (T)(y - h / 2);
@@ -867,8 +867,8 @@ void lucas_kanade_step(T* flow, const T* Fx, const T* Fy, const T* Ft, int w, in
                         g3 += dy * dy;
                     }
                 T a = g0, b = g1, c = g1, d = g3;
-                T theta = (T)0.5f * std::atan2((T)2.0f * a * c + (T)2.0f * b * d, a * a + b * b - c * c - d * d);
-                T ct = std::cos(theta), st = std::sin(theta);
+                T theta = (T)0.5f * atan2_r((T)2.0f * a * c + (T)2.0f * b * d, a * a + b * b - c * c - d * d);
+                T ct = cos_r(theta), st = sin_r(theta);
                 T UT0 = ct, UT2 = -st, UT1 = st, UT3 = ct;
                 T S1 = a * a + b * b + c * c + d * d;
                 T S2 = std::sqrt((a * a + b * b - c * c - d * d) * (a * a + b * b - c * c - d * d) +
@@ -880,8 +880,8 @@ void lucas_kanade_step(T* flow, const T* Fx, const T* Fy, const T* Ft, int w, in
                 sigma1 = sigma1 != 0 ? (T)1.0f / sigma1 : (T)0;
                 sigma2 = sigma2 != 0 ? (T)1.0f / sigma2 : (T)0;
                 T S0 = sigma1, Sb = 0, Sc = 0, S3 = sigma2;
-                T eps = (T)0.5f * std::atan2((T)2.0f * a * b + (T)2.0f * c * d, a * a - b * b + c * c - d * d);
-                T ce = std::cos(eps), se = std::sin(eps);
+                T eps = (T)0.5f * atan2_r((T)2.0f * a * b + (T)2.0f * c * d, a * a - b * b + c * c - d * d);
+                T ce = cos_r(eps), se = sin_r(eps);
                 T s11 = (a * ct + c * st) * ce + (b * ct + d * st) * se;
                 T s22 = (a * st - c * ct) * se + (-b * st + d * ct) * ce;
                 s11 = s11 > (T)0.0f ? (T)1.0f : s11 < 0 ? (T)-1.0f : (T)0.0f;
@@ -1032,7 +1032,7 @@ void robustness_mask(const T* ref3, const T* mov3, T* mask4, const T* flow, int 
                     T dist = std::fabs(mr[c] - mm[c]);
                     T sig = std::fmax(smd, sd);
                     dist = dist * (sd * sd / (sd * sd + smd * smd));
-                    out[c] = std::fmax(std::fmin(s * std::exp(-dist * dist / (sig * sig)) - (T)0.12f, (T)1.0f), (T)0.0f);
+                    out[c] = std::fmax(std::fmin(s * exp_r(-dist * dist / (sig * sig)) - (T)0.12f, (T)1.0f), (T)0.0f);
                 }
                 out[3] = Mv;
                 for (int c = 0; c < 4; ++c) mask4[((size_t)py * hw + px) * 4 + c] = out[c];
@@ -1062,7 +1062,7 @@ template <class T> void erode(const T* in4, T* out4, int w, int h, int size) {
 // a20  accumulateImages / accumulateImagesSuperRes (R:Kernels/DeBayerKernels.cu:288-468).
 template <class T> static inline T kernel_weight(int px, int py, T k0, T k1, T k2) {
     T w = (T)(px * px) * k0 + (T)(2 * px * py) * k2 + (T)(py * py) * k1;
-    w = std::exp((T)-0.5f * w);
+    w = exp_r((T)-0.5f * w);
     if (!std::isfinite(w)) w = px * py == 0 ? (T)1 : (T)0;
     return w;
 }

@@ -48,6 +48,20 @@ inline void parallel_for(int n, const std::function<void(int, int)>& body) {
     for (auto& t : th) t.join();
 }
 
+// Transcendentals: evaluated in double and rounded once to T. For T = float this
+// is the correctly rounded result except in ~2^-28 of cases; the HIP kernels do
+// the same, so the two sides agree bit for bit.
+inline float exp_r(float x) { return (float)std::exp((double)x); }
+inline double exp_r(double x) { return std::exp(x); }
+inline float pow_r(float x, float y) { return (float)std::pow((double)x, (double)y); }
+inline double pow_r(double x, double y) { return std::pow(x, y); }
+inline float atan2_r(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+inline double atan2_r(double y, double x) { return std::atan2(y, x); }
+inline float sin_r(float x) { return (float)std::sin((double)x); }
+inline double sin_r(double x) { return std::sin(x); }
+inline float cos_r(float x) { return (float)std::cos((double)x); }
+inline double cos_r(double x) { return std::cos(x); }
+
 inline int cfa_at(const int cfa[4], int y, int x) { return cfa[(y % 2) * 2 + (x % 2)]; }
 
 // ---------------------------------------------------------------------------

@@ -36,3 +36,17 @@ def gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch
+
+
+@pytest.fixture(autouse=True)
+def _release_device_tensors():
+    yield
+    import sys as _sys
+
+    mod = _sys.modules.get("tests.gpu_util")
+    if mod is not None:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+        mod.KEEP.clear()

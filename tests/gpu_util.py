@@ -5,11 +5,20 @@ import numpy as np
 import torch
 
 
+# Device tensors handed to the library by raw pointer must outlive the launch that uses
+# them; the caching allocator would otherwise recycle their memory. They are kept here
+# until the end of the test (cleared by an autouse fixture in conftest).
+KEEP = []
+
+
 def dev(a: np.ndarray) -> torch.Tensor:
     a = np.ascontiguousarray(a)
     if a.dtype == np.uint16:
-        return torch.from_numpy(a.view(np.int16)).cuda()
-    return torch.from_numpy(a).cuda()
+        t = torch.from_numpy(a.view(np.int16)).cuda()
+    else:
+        t = torch.from_numpy(a).cuda()
+    KEEP.append(t)
+    return t
 
 
 def host(t: torch.Tensor, dtype=None) -> np.ndarray:
@@ -24,6 +33,7 @@ def empty(shape, dtype=torch.float32, fill=None) -> torch.Tensor:
     t = torch.empty(shape, dtype=dtype, device="cuda")
     if fill is not None:
         t.fill_(fill)
+    KEEP.append(t)
     return t
 
 
