@@ -75,7 +75,7 @@ void isa_default_params(isa_params* p) {
     p->cfa[2] = 1;
     p->cfa[3] = 2;
     p->reference_index = -1;
-    p->tex_quant = 0;
+    p->tex_quant = 1;
 }
 
 }  // extern "C"

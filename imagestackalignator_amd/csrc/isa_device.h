@@ -51,6 +51,10 @@ __device__ __forceinline__ float tex_linear(const float* img, int pitch, int w, 
     int xa = tex_addr(x0, w, mode), xb = tex_addr(x0 + 1, w, mode);
     int ya = tex_addr(y0, h, mode), yb = tex_addr(y0 + 1, h, mode);
     const float* ra = row_ptr(img, pitch, ya);
+    // With both weights 0 the bilinear sum below is t00 + (+-0) terms, i.e. exactly t00 for finite
+    // texels; the other three fetches are skipped (the common case of fetches at texel centres
+    // with quantised weights).
+    if (a == 0.0f && b == 0.0f) return ra[xa * CH + c];
     const float* rb = row_ptr(img, pitch, yb);
     float t00 = ra[xa * CH + c], t10 = ra[xb * CH + c];
     float t01 = rb[xa * CH + c], t11 = rb[xb * CH + c];

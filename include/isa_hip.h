@@ -88,7 +88,7 @@ typedef struct isa_params {
     float camera_white[3];            /* as-shot white balance (1,1,1 = identity) */
     int cfa[4];                       /* Bayer pattern, row-major 2x2 */
     int reference_index;              /* -1: PreAlignmentStore rule (R:PEFStudioDX/PreAlignmentStore.cs:36-110) */
-    int tex_quant;                    /* 0 exact bilinear weights, 1 emulate 8-bit texture weights */
+    int tex_quant;                    /* 1 (default): 8-bit texture weights as the texture unit; 0: exact weights */
 } isa_params;
 
 void isa_default_params(isa_params* p);
