@@ -56,7 +56,7 @@ def cpu_baseline(cfg, seconds_hint: float):
 
     threads = min(len(os.sched_getaffinity(0)), 16)
     O.set_threads(threads)
-    cw, ch = 768, 512
+    cw, ch = 1024, 768
     scfg = isa.default_synth(seed=7, width=cw, height=ch, frames=cfg["n"], global_shift_max=cfg["gmax"] / 8,
                              flow_max=cfg["flow"], occluded_fraction=cfg["occ"])
     raw_t, _ = isa.synth_burst(scfg)
