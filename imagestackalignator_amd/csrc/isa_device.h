@@ -30,6 +30,7 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 #define ISA_ADDR_MIRROR 1
 
 __device__ __forceinline__ int tex_addr(int i, int n, int mode) {
+    if ((unsigned)i < (unsigned)n) return i;  // in range: both modes are the identity
     if (mode == ISA_ADDR_CLAMP) return clampi(i, 0, n - 1);
     int p = 2 * n;
     i %= p;

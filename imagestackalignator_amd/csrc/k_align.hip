@@ -24,67 +24,89 @@ __device__ __forceinline__ void tile_offset(int tx, int ty, int tile, int w, int
     oy = (int)roundf(sy);
 }
 
-// SSD core. r: T x T reference interior (LDS, row stride T); t: B x B track block (row stride B).
-// D(s) = (sum r^2 + sum_window t^2(s)) - 2 sum r t(.+s), in FP64 with a fixed order:
-// each row term is an fma chain over x, rows are then added in increasing y.
-// P: T * S * S doubles, TT: B * S doubles, R2: T doubles, D: S * S floats (all LDS).
+// SSD core. r: T x T reference interior (LDS, row stride RS), t: B x B track block (row stride TS).
+// D(s) = sum_y R2[y] + sum_y Q[y][s],  R2[y] = sum_x r^2,  Q[y][s] = TT[y+sy+M][sx+M] - 2 C[y][s],
+// TT[j][i] = sum_x t[j][x+i]^2,  C[y][s] = sum_x r[y][x] t[y+sy+M][x+sx+M],
+// all in FP64: every row term is an fma chain in increasing x; the sums over y are pairwise trees
+// (stride 1, 2, 4, ...: Q[y] += Q[y + stride] for y a multiple of 2 stride). LDS: Q T*S*S doubles,
+// TT B*S doubles, R2 T doubles, D S*S floats.
+__device__ __forceinline__ int ssd_rs(int T) { return T + 1; }
+__device__ __forceinline__ int ssd_ts(int T, int M) { return T + 2 * M + 1; }
+
 template <int M>
-__device__ void ssd_core(const float* r, const float* t, int T, double* P, double* TT, double* R2, float* D, int tid,
+__device__ void ssd_core(const float* r, const float* t, int T, double* Q, double* TT, double* R2, float* D, int tid,
                          int nthreads) {
     constexpr int S = 2 * M + 1;
-    const int B = T + 2 * M;
-    // cross terms: task (y, sy)
+    const int B = T + 2 * M, RS = ssd_rs(T), TS = ssd_ts(T, M);
+    // 1. window energies of the track rows and the reference row energies
+    for (int task = tid; task < B * S + T; task += nthreads) {
+        double acc = 0.0;
+        if (task < B * S) {
+            int j = task / S, i = task - j * S;
+            const float* trow = t + j * TS + i;
+            for (int x = 0; x < T; ++x) {
+                double v = (double)trow[x];
+                acc = fma(v, v, acc);
+            }
+            TT[task] = acc;
+        } else {
+            const float* rrow = r + (task - B * S) * RS;
+            for (int x = 0; x < T; ++x) {
+                double v = (double)rrow[x];
+                acc = fma(v, v, acc);
+            }
+            R2[task - B * S] = acc;
+        }
+    }
+    __syncthreads();
+    // 2. cross terms, task (y, sy). The S-wide window of the track row stays in registers: element
+    // t[q] lives in slot q % S and the x loop is unrolled by S, so the window never moves.
     for (int task = tid; task < T * S; task += nthreads) {
         int y = task / S, iy = task - y * S;  // sy = iy - M
-        const float* trow = t + (y + iy) * B;
-        const float* rrow = r + y * T;
+        const float* trow = t + (y + iy) * TS;
+        const float* rrow = r + y * RS;
         double acc[S], win[S];
 #pragma unroll
         for (int i = 0; i < S; ++i) acc[i] = 0.0;
 #pragma unroll
         for (int i = 0; i < S - 1; ++i) win[i] = (double)trow[i];
-        for (int x = 0; x < T; ++x) {
-            win[S - 1] = (double)trow[x + S - 1];
-            double rv = (double)rrow[x];
+        int x = 0;
+        for (; x + S <= T; x += S) {
 #pragma unroll
-            for (int i = 0; i < S; ++i) acc[i] = fma(rv, win[i], acc[i]);
+            for (int k = 0; k < S; ++k) {
+                win[(k + S - 1) % S] = (double)trow[x + k + S - 1];
+                const double rv = (double)rrow[x + k];
 #pragma unroll
-            for (int i = 0; i < S - 1; ++i) win[i] = win[i + 1];
+                for (int i = 0; i < S; ++i) acc[i] = fma(rv, win[(k + i) % S], acc[i]);
+            }
+        }
+        const int rem = T - x;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            if (k < rem) {
+                win[(k + S - 1) % S] = (double)trow[x + k + S - 1];
+                const double rv = (double)rrow[x + k];
+#pragma unroll
+                for (int i = 0; i < S; ++i) acc[i] = fma(rv, win[(k + i) % S], acc[i]);
+            }
         }
 #pragma unroll
-        for (int i = 0; i < S; ++i) P[(y * S + iy) * S + i] = acc[i];
-    }
-    // window energies of the track rows: TT[j][i] = sum_x t[j][x+i]^2
-    for (int task = tid; task < B * S; task += nthreads) {
-        int j = task / S, i = task - j * S;
-        const float* trow = t + j * B + i;
-        double acc = 0.0;
-        for (int x = 0; x < T; ++x) {
-            double v = (double)trow[x];
-            acc = fma(v, v, acc);
-        }
-        TT[j * S + i] = acc;
-    }
-    for (int y = tid; y < T; y += nthreads) {
-        const float* rrow = r + y * T;
-        double acc = 0.0;
-        for (int x = 0; x < T; ++x) {
-            double v = (double)rrow[x];
-            acc = fma(v, v, acc);
-        }
-        R2[y] = acc;
+        for (int i = 0; i < S; ++i) Q[(y * S + iy) * S + i] = fma(-2.0, acc[i], TT[(y + iy) * S + i]);
     }
     __syncthreads();
-    for (int s = tid; s < S * S; s += nthreads) {
-        int iy = s / S, ix = s - iy * S;
-        double r2 = 0.0, box = 0.0, cc = 0.0;
-        for (int y = 0; y < T; ++y) {
-            r2 += R2[y];
-            box += TT[(y + iy) * S + ix];
-            cc += P[(y * S + iy) * S + ix];
+    // 3. pairwise trees over y (Q for every lag, R2 once)
+    for (int stride = 1; stride < T; stride *= 2) {
+        const int pairs = (T + 2 * stride - 1) / (2 * stride);
+        for (int task = tid; task < pairs * (S * S + 1); task += nthreads) {
+            int pr = task / (S * S + 1), s = task - pr * (S * S + 1);
+            int y = pr * 2 * stride;
+            if (y + stride >= T) continue;
+            if (s < S * S) Q[y * S * S + s] += Q[(y + stride) * S * S + s];
+            else R2[y] += R2[y + stride];
         }
-        D[s] = (float)((r2 + box) - 2.0 * cc);
+        __syncthreads();
     }
+    for (int s = tid; s < S * S; s += nthreads) D[s] = (float)(R2[0] + Q[s]);
     __syncthreads();
 }
 
@@ -143,25 +165,25 @@ __device__ void find_min_core(const float* v, int M, float threshold, float& cx,
 
 size_t ssd_smem(int T, int M) {
     int S = 2 * M + 1, B = T + 2 * M;
-    size_t f = (size_t)T * T + (size_t)B * B + (size_t)S * S;  // floats
-    size_t d = (size_t)T * S * S + (size_t)B * S + (size_t)T;   // doubles
+    size_t f = (size_t)T * (T + 1) + (size_t)B * (B + 1) + (size_t)S * S;  // floats
+    size_t d = (size_t)T * S * S + (size_t)B * S + (size_t)T;             // doubles
     return ((f * 4 + 7) & ~(size_t)7) + d * 8;
 }
 
 struct SsdLds {
     float *r, *t, *D;
-    double *P, *TT, *R2;
+    double *Q, *TT, *R2;
 };
 
 __device__ __forceinline__ SsdLds ssd_lds(char* smem, int T, int M) {
     int S = 2 * M + 1, B = T + 2 * M;
     SsdLds L;
     L.r = (float*)smem;
-    L.t = L.r + T * T;
-    L.D = L.t + B * B;
-    size_t f = (size_t)T * T + (size_t)B * B + (size_t)S * S;
-    L.P = (double*)(smem + ((f * 4 + 7) & ~(size_t)7));
-    L.TT = L.P + T * S * S;
+    L.t = L.r + T * (T + 1);
+    L.D = L.t + B * (B + 1);
+    size_t f = (size_t)T * (T + 1) + (size_t)B * (B + 1) + (size_t)S * S;
+    L.Q = (double*)(smem + ((f * 4 + 7) & ~(size_t)7));
+    L.TT = L.Q + T * S * S;
     L.R2 = L.TT + B * S;
     return L;
 }
@@ -189,7 +211,9 @@ __global__ void k_track_level(TrackArgs a) {
     const int T = a.tile, B = T + 2 * M;
     constexpr int S = 2 * M + 1;
     SsdLds L = ssd_lds(smem, T, M);
-    const int tile_idx = blockIdx.x, p = blockIdx.y;
+    // blockIdx.x = pair, blockIdx.y = tile: the pairs of one tile run together and share the tile
+    // regions of the frames through L2
+    const int p = blockIdx.x, tile_idx = blockIdx.y;
     const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
     const int fa = a.pa[p], fb = a.pb[p], k = a.pk[p];
     float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
@@ -203,15 +227,15 @@ __global__ void k_track_level(TrackArgs a) {
     for (int i = tid; i < T * T; i += nt) {
         int py = i / T, px = i - py * T;
         int gx = clampi(tx * T + px + M + rox, 0, a.w - 1), gy = clampi(ty * T + py + M + roy, 0, a.h - 1);
-        L.r[i] = row_ptr(ia, a.img_pitch, gy)[gx];
+        L.r[py * (T + 1) + px] = row_ptr(ia, a.img_pitch, gy)[gx];
     }
     for (int i = tid; i < B * B; i += nt) {
         int py = i / B, px = i - py * B;
         int gx = clampi(tx * T + px + tox, 0, a.w - 1), gy = clampi(ty * T + py + toy, 0, a.h - 1);
-        L.t[i] = row_ptr(ib, a.img_pitch, gy)[gx];
+        L.t[py * (B + 1) + px] = row_ptr(ib, a.img_pitch, gy)[gx];
     }
     __syncthreads();
-    ssd_core<M>(L.r, L.t, T, L.P, L.TT, L.R2, L.D, tid, nt);
+    ssd_core<M>(L.r, L.t, T, L.Q, L.TT, L.R2, L.D, tid, nt);
     if (a.shift_images)
         for (int s = tid; s < S * S; s += nt) a.shift_images[((size_t)p * a.cx * a.cy + tile_idx) * S * S + s] = L.D[s];
     if (tid == 0) {
@@ -234,11 +258,14 @@ __global__ void k_tile_ssd(const float* __restrict__ rt, const float* __restrict
     const float* k = tt + (size_t)t * B * B;
     for (int i = tid; i < T * T; i += nt) {
         int py = i / T, px = i - py * T;
-        L.r[i] = r[(py + M) * B + px + M];
+        L.r[py * (T + 1) + px] = r[(py + M) * B + px + M];
     }
-    for (int i = tid; i < B * B; i += nt) L.t[i] = k[i];
+    for (int i = tid; i < B * B; i += nt) {
+        int py = i / B, px = i - py * B;
+        L.t[py * (B + 1) + px] = k[i];
+    }
     __syncthreads();
-    ssd_core<M>(L.r, L.t, T, L.P, L.TT, L.R2, L.D, tid, nt);
+    ssd_core<M>(L.r, L.t, T, L.Q, L.TT, L.R2, L.D, tid, nt);
     for (int s = tid; s < S * S; s += nt) out[(size_t)t * S * S + s] = L.D[s];
 }
 
@@ -607,7 +634,7 @@ int shift_tables(int n, int ref, int strategy, int block_in, int* pairs, int* in
 template <int M>
 static int launch_track_level_m(const TrackArgs& a, int pairs, hipStream_t s) {
     size_t smem = ssd_smem(a.tile, M);
-    k_track_level<M><<<dim3(a.cx * a.cy, pairs), block_threads(a.tile, M), smem, s>>>(a);
+    k_track_level<M><<<dim3(pairs, a.cx * a.cy), block_threads(a.tile, M), smem, s>>>(a);
     return check_launch("track_level");
 }
 

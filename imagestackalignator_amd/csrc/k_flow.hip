@@ -48,20 +48,88 @@ __device__ __forceinline__ float stencil5(const float* img, int pitch, int w, in
     return t;
 }
 
-// ComputeDerivativesKernel (R:Kernels/opticalFlow.cu:96-147).
-__global__ void k_derivatives(const float* src, const float* tgt, int img_pitch, float* Ix, float* Iy, float* Iz,
-                              int pitch, int w, int h, int quant) {
+// Texture taps of the 5-point stencils, per column and per row. The stencil fetches at
+// x + k dx (k = -2..2) depend only on the column (and y + k dy only on the row), so their texel
+// indices (mirrored) and weights are tabulated once per image size; a fetch then combines one column
+// entry with one row entry. Same float operations as tex_linear, so the values are identical.
+struct Tap {
+    int i0, i1;
+    float a;
+};
+
+__global__ void k_tap_tables(Tap* col, Tap* row, int w, int h, int quant) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool is_col = i < w;
+    const int n = is_col ? w : h;
+    const int ix = is_col ? i : i - w;
+    if (ix >= n || i >= w + h) return;
+    float d = 1.0f / (float)n;
+    float x = ((float)ix + 0.5f) * d;
+    Tap* out = (is_col ? col : row) + (size_t)ix * 5;
+#pragma unroll
+    for (int k = -2; k <= 2; ++k) {
+        float u = k == 0 ? x : (k > 0 ? x + (float)k * d : x - (float)(-k) * d);
+        float xf = u * (float)n - 0.5f;
+        float fl = floorf(xf);
+        Tap t;
+        t.a = tex_frac(xf - fl, quant);
+        t.i0 = tex_addr((int)fl, n, ISA_ADDR_MIRROR);
+        t.i1 = tex_addr((int)fl + 1, n, ISA_ADDR_MIRROR);
+        out[k + 2] = t;
+    }
+}
+
+__device__ __forceinline__ float tap_fetch(const float* img, int pitch, const Tap& c, const Tap& r) {
+    const float* ra = row_ptr(img, pitch, r.i0);
+    const float a = c.a, b = r.a;
+    // weights at the corners reduce the bilinear sum to one texel exactly (finite data)
+    if (b == 0.0f) {
+        if (a == 0.0f) return ra[c.i0];
+        if (a == 1.0f) return ra[c.i1];
+    } else if (b == 1.0f) {
+        const float* rb = row_ptr(img, pitch, r.i1);
+        if (a == 0.0f) return rb[c.i0];
+        if (a == 1.0f) return rb[c.i1];
+    }
+    const float* rb = row_ptr(img, pitch, r.i1);
+    float t00 = ra[c.i0], t10 = ra[c.i1], t01 = rb[c.i0], t11 = rb[c.i1];
+    float oma = 1.0f - a, omb = 1.0f - b;
+    return oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11;
+}
+
+// ComputeDerivativesKernel (R:Kernels/opticalFlow.cu:96-147): 5-point stencils of both images along x
+// and y, averaged, and the temporal difference, with the stencil taps read from the tables.
+__global__ void k_derivatives_tab(const float* src, const float* tgt, int img_pitch, float* Ix, float* Iy, float* Iz,
+                                  int pitch, int w, int h, const Tap* __restrict__ col, const Tap* __restrict__ row) {
     int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y * blockDim.y + threadIdx.y;
     if (ix >= w || iy >= h) return;
-    float dx = 1.0f / (float)w, dy = 1.0f / (float)h;
-    float x = ((float)ix + 0.5f) * dx, y = ((float)iy + 0.5f) * dy;
-    float t0 = stencil5(src, img_pitch, w, h, x, y, dx, 0.0f, quant);
-    float t1 = stencil5(tgt, img_pitch, w, h, x, y, dx, 0.0f, quant);
+    Tap c[5], r[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        c[k] = col[(size_t)ix * 5 + k];
+        r[k] = row[(size_t)iy * 5 + k];
+    }
+    auto sx = [&](const float* I) {
+        float t = tap_fetch(I, img_pitch, c[4], r[2]);
+        t -= tap_fetch(I, img_pitch, c[3], r[2]) * 8.0f;
+        t += tap_fetch(I, img_pitch, c[1], r[2]) * 8.0f;
+        t -= tap_fetch(I, img_pitch, c[0], r[2]);
+        t /= 12.0f;
+        return t;
+    };
+    auto sy = [&](const float* I) {
+        float t = tap_fetch(I, img_pitch, c[2], r[4]);
+        t -= tap_fetch(I, img_pitch, c[2], r[3]) * 8.0f;
+        t += tap_fetch(I, img_pitch, c[2], r[1]) * 8.0f;
+        t -= tap_fetch(I, img_pitch, c[2], r[0]);
+        t /= 12.0f;
+        return t;
+    };
+    float t0 = sx(src), t1 = sx(tgt);
     row_ptr(Ix, pitch, iy)[ix] = (t0 + t1) * 0.5f;
-    row_ptr(Iz, pitch, iy)[ix] = tex_linear<1>(src, img_pitch, w, h, 0, x, y, ISA_ADDR_MIRROR, quant) -
-                                 tex_linear<1>(tgt, img_pitch, w, h, 0, x, y, ISA_ADDR_MIRROR, quant);
-    t0 = stencil5(src, img_pitch, w, h, x, y, 0.0f, dy, quant);
-    t1 = stencil5(tgt, img_pitch, w, h, x, y, 0.0f, dy, quant);
+    row_ptr(Iz, pitch, iy)[ix] = tap_fetch(src, img_pitch, c[2], r[2]) - tap_fetch(tgt, img_pitch, c[2], r[2]);
+    t0 = sy(src);
+    t1 = sy(tgt);
     row_ptr(Iy, pitch, iy)[ix] = (t0 + t1) * 0.5f;
 }
 
@@ -146,36 +214,58 @@ __global__ void k_lk_step(float* flow, int flow_pitch, const float* Fx, const fl
     f[1] += u1;
 }
 
-// Lucas-Kanade step with the five window sums (Ix^2, IxIy, Iy^2, IxIt, IyIt) formed by separable box
-// sums in LDS (O(1) per pixel instead of 2 (2h+1)^2 loads). The solve is the same closed form; the
-// update is pinv . (sum Ix It, sum Iy It), which equals the reference's sum of pinv . (Ix, Iy) It.
-constexpr int LK_BX = 32, LK_BY = 16, LK_HMAX = 10;
+// Lucas-Kanade step with the five window sums (Ix^2, IxIy, Iy^2, IxIt, IyIt) formed by separable
+// box sums in LDS instead of 2 (2h+1)^2 loads per pixel. Each horizontal / vertical pass starts a
+// segment of LK_SEG outputs with a plain (2h+1)-term sum and continues it as a running sum. The
+// solve is the same closed form; the update is pinv . (sum Ix It, sum Iy It), which equals the
+// reference's sum of pinv . (Ix, Iy) It.
+constexpr int LK_BX = 32, LK_BY = 16, LK_HMAX = 10, LK_SEG = 8;
 __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int flow_pitch, const float* Fx,
                                                                const float* Fy, const float* Ft, int pitch, int w,
                                                                int h, int hw, float min_det) {
-    __shared__ float s_p[5][LK_BY + 2 * LK_HMAX][LK_BX + 2 * LK_HMAX];
-    __shared__ float s_h[5][LK_BY + 2 * LK_HMAX][LK_BX];
+    constexpr int RW = LK_BX + 2 * LK_HMAX;  // region width (max)
+    __shared__ float s_p[LK_BY + 2 * LK_HMAX][RW][5];
+    __shared__ float s_h[LK_BY + 2 * LK_HMAX][LK_BX][5];
+    __shared__ float s_v[LK_BY][LK_BX][5];
     const int x0 = blockIdx.x * LK_BX, y0 = blockIdx.y * LK_BY;
-    const int tid = threadIdx.y * LK_BX + threadIdx.x;
-    const int rows = LK_BY + 2 * hw, cols = LK_BX + 2 * hw;
-    for (int i = tid; i < rows * cols; i += LK_BX * LK_BY) {
+    const int tid = threadIdx.y * LK_BX + threadIdx.x, nt = LK_BX * LK_BY;
+    const int rows = LK_BY + 2 * hw, cols = LK_BX + 2 * hw, win = 2 * hw + 1;
+    for (int i = tid; i < rows * cols; i += nt) {
         int ly = i / cols, lx = i - ly * cols;
         int gx = clampi(x0 - hw + lx, 0, w - 1), gy = clampi(y0 - hw + ly, 0, h - 1);
         float dx = row_ptr(Fx, pitch, gy)[gx], dy = row_ptr(Fy, pitch, gy)[gx], dt = row_ptr(Ft, pitch, gy)[gx];
-        s_p[0][ly][lx] = dx * dx;
-        s_p[1][ly][lx] = dx * dy;
-        s_p[2][ly][lx] = dy * dy;
-        s_p[3][ly][lx] = dx * dt;
-        s_p[4][ly][lx] = dy * dt;
+        s_p[ly][lx][0] = dx * dx;
+        s_p[ly][lx][1] = dx * dy;
+        s_p[ly][lx][2] = dy * dy;
+        s_p[ly][lx][3] = dx * dt;
+        s_p[ly][lx][4] = dy * dt;
     }
     __syncthreads();
-    for (int i = tid; i < rows * LK_BX; i += LK_BX * LK_BY) {
-        int ly = i / LK_BX, lx = i - ly * LK_BX;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            float s = 0.0f;
-            for (int k = 0; k <= 2 * hw; ++k) s += s_p[q][ly][lx + k];
-            s_h[q][ly][lx] = s;
+    // horizontal: task (row, segment, quantity)
+    constexpr int HSEG = LK_BX / LK_SEG;
+    for (int task = tid; task < rows * HSEG * 5; task += nt) {
+        int q = task % 5, rs = task / 5, seg = rs % HSEG, ly = rs / HSEG;
+        int xs = seg * LK_SEG;
+        float acc = 0.0f;
+        for (int k = 0; k < win; ++k) acc += s_p[ly][xs + k][q];
+        s_h[ly][xs][q] = acc;
+        for (int i = 1; i < LK_SEG; ++i) {
+            acc += s_p[ly][xs + i + win - 1][q] - s_p[ly][xs + i - 1][q];
+            s_h[ly][xs + i][q] = acc;
+        }
+    }
+    __syncthreads();
+    // vertical: task (column, segment, quantity)
+    constexpr int VSEG = LK_BY / LK_SEG;
+    for (int task = tid; task < LK_BX * VSEG * 5; task += nt) {
+        int q = task % 5, cs = task / 5, seg = cs % VSEG, lx = cs / VSEG;
+        int ys = seg * LK_SEG;
+        float acc = 0.0f;
+        for (int k = 0; k < win; ++k) acc += s_h[ys + k][lx][q];
+        s_v[ys][lx][q] = acc;
+        for (int i = 1; i < LK_SEG; ++i) {
+            acc += s_h[ys + i + win - 1][lx][q] - s_h[ys + i - 1][lx][q];
+            s_v[ys + i][lx][q] = acc;
         }
     }
     __syncthreads();
@@ -183,11 +273,7 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
     if (px < hw || px >= w - hw || py < hw || py >= h - hw) return;
     float g[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        float s = 0.0f;
-        for (int k = 0; k <= 2 * hw; ++k) s += s_h[q][threadIdx.y + k][threadIdx.x];
-        g[q] = s;
-    }
+    for (int q = 0; q < 5; ++q) g[q] = s_v[threadIdx.y][threadIdx.x][q];
     float inv[4];
     if (!lk_pinv<false>(g[0], g[1], g[2], min_det, inv)) return;
     float u0 = inv[0] * g[3] + inv[1] * g[4];
@@ -218,9 +304,10 @@ int launch_warp(const float* src, int src_pitch, float* out, int out_pitch, cons
 }
 
 int launch_derivatives(const float* src, const float* tgt, int img_pitch, float* ix, float* iy, float* iz, int pitch,
-                       int w, int h, int quant, hipStream_t s) {
-    k_derivatives<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(src, tgt, img_pitch, ix, iy, iz, pitch, w,
-                                                                             h, quant);
+                       int w, int h, const void* taps, hipStream_t s) {
+    const Tap* col = (const Tap*)taps;
+    k_derivatives_tab<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(src, tgt, img_pitch, ix, iy, iz, pitch,
+                                                                                 w, h, col, col + (size_t)w * 5);
     return check_launch("derivatives");
 }
 
@@ -245,9 +332,19 @@ int launch_lk_step(float* flow, int flow_pitch, const float* ix, const float* iy
 }
 
 // OpticalFlow.LucasKanade with an explicit work area (tmp, Ix, Iy, Iz: 4 w*h floats, pitch w*4).
+size_t tap_table_bytes(int w, int h) { return (size_t)(w + h) * 5 * sizeof(Tap); }
+
+int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s) {
+    Tap* col = (Tap*)tables;
+    Tap* row = col + (size_t)w * 5;
+    k_tap_tables<<<div_up(w + h, 256), 256, 0, s>>>(col, row, w, h, quant);
+    return check_launch("tap_tables");
+}
+
+// taps: tables from build_tap_tables for (w, h, quant).
 int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles, int tiles_pitch, int tw, int th,
                  float* flow, int flow_pitch, float* work, int w, int h, int iterations, float bsx, float bsy,
-                 float rot, float min_det, int window, int quant, bool box, hipStream_t s) {
+                 float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps) {
     const size_t np = (size_t)w * h;
     float* tmp = work;
     float* ix = work + np;
@@ -257,11 +354,25 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
     ISA_TRY(launch_flow_from_tiles(flow, flow_pitch, tiles, tiles_pitch, tw, th, w, h, bsx, bsy, rot, quant, s));
     for (int it = 0; it < iterations; ++it) {
         ISA_TRY(launch_warp(src, img_pitch, tmp, wp, flow, flow_pitch, w, h, quant, s));
-        ISA_TRY(launch_derivatives(tmp, tgt, wp, ix, iy, iz, wp, w, h, quant, s));
+        ISA_TRY(launch_derivatives(tmp, tgt, wp, ix, iy, iz, wp, w, h, taps, s));
         ISA_TRY(launch_lk_step(flow, flow_pitch, ix, iy, iz, wp, w, h, window, min_det, box, s));
     }
     return ISA_OK;
 }
+
+// Stream-ordered tap tables for a one-off call of the public entry points (the merger keeps its own).
+struct ScopedTaps {
+    void* p = nullptr;
+    hipStream_t s;
+    int rc = ISA_OK;
+    ScopedTaps(int w, int h, int quant, hipStream_t st) : s(st) {
+        rc = check_hip(hipMallocAsync(&p, tap_table_bytes(w, h), s), "tap tables");
+        if (rc == ISA_OK) rc = build_tap_tables(p, w, h, quant, s);
+    }
+    ~ScopedTaps() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
 
 }  // namespace isa
 
@@ -287,8 +398,10 @@ int isa_compute_derivatives(const float* src_dev, const float* tgt_dev, int img_
                             float* iz_dev, int pitch, int width, int height, int tex_quant, isa_stream_t stream) {
     ISA_REQUIRE(src_dev && tgt_dev && ix_dev && iy_dev && iz_dev && width > 0 && height > 0,
                 "isa_compute_derivatives: bad arguments");
-    return isa::launch_derivatives(src_dev, tgt_dev, img_pitch, ix_dev, iy_dev, iz_dev, pitch, width, height,
-                                   tex_quant, isa::to_stream(stream));
+    isa::ScopedTaps taps(width, height, tex_quant, isa::to_stream(stream));
+    ISA_TRY(taps.rc);
+    return isa::launch_derivatives(src_dev, tgt_dev, img_pitch, ix_dev, iy_dev, iz_dev, pitch, width, height, taps.p,
+                                   isa::to_stream(stream));
 }
 
 int isa_compute_derivatives2(const float* img_dev, int img_pitch, float* ix_dev, float* iy_dev, int pitch, int width,
@@ -315,9 +428,11 @@ int isa_lucas_kanade(float* src_dev, const float* tgt_dev, int img_pitch, const 
                     iterations >= 0 && window_size >= 1,
                 "isa_lucas_kanade: bad arguments");
     hipStream_t s = isa::to_stream(stream);
+    isa::ScopedTaps taps(width, height, tex_quant, s);
+    ISA_TRY(taps.rc);
     ISA_TRY(isa::lucas_kanade(src_dev, tgt_dev, img_pitch, tiles2_dev, tiles_pitch, tex_w, tex_h, flow2_dev, flow_pitch,
                               work_dev, width, height, iterations, base_shift[0], base_shift[1], base_rotation, min_det,
-                              window_size, tex_quant, false, s));
+                              window_size, tex_quant, false, s, taps.p));
     // final warp, copied over the source image (R:PEFStudioDX/OpticalFlow.cs:162-163)
     ISA_TRY(isa::launch_warp(src_dev, img_pitch, work_dev, width * (int)sizeof(float), flow2_dev, flow_pitch, width,
                              height, tex_quant, s));

@@ -86,23 +86,27 @@ __device__ float filter_value(int kx, int ky, int w, int h, float lp, float hp, 
 //   B(k, n) = BT ? B[n*ldb + k] : B[k*ldb + n]
 // EPI 0: C[m*ldc + n] = acc.   EPI 1: img(m, n) = clamp(img(m, n) - acc, 0, 1) in place (C = img, ldc = pitch).
 constexpr int TM = 64, TN = 64, TK = 16;
+// Split K: blockIdx.z covers k in [z kc, min(K, (z+1) kc)); EPI 0 writes its partial to C + z * c_zstride.
 template <bool BT, int EPI>
 __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, const float* __restrict__ A, int a_pitch,
-                                               const float* __restrict__ B, int ldb, float* C, int c_pitch) {
+                                               const float* __restrict__ B, int ldb, float* C, int c_pitch, int kc,
+                                               size_t c_zstride) {
     __shared__ float As[TK][TM + 4];
     __shared__ float Bs[TK][TN + 4];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
+    C += (size_t)blockIdx.z * c_zstride;
     float acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += TK) {
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
         for (int i = threadIdx.x; i < TM * TK; i += 256) {
             int mm = i / TK, kk = i - mm * TK;
             int gm = m0 + mm, gk = k0 + kk;
-            As[kk][mm] = (gm < M && gk < K) ? row_ptr(A, a_pitch, gm)[gk] : 0.0f;
+            As[kk][mm] = (gm < M && gk < kend) ? row_ptr(A, a_pitch, gm)[gk] : 0.0f;
         }
         for (int i = threadIdx.x; i < TN * TK; i += 256) {
             int nn, kk;
@@ -110,7 +114,7 @@ __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, const float*
             else { kk = i / TN; nn = i - kk * TN; }
             int gn = n0 + nn, gk = k0 + kk;
             float v = 0.0f;
-            if (gn < N && gk < K) v = BT ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+            if (gn < N && gk < kend) v = BT ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
             Bs[kk][nn] = v;
         }
         __syncthreads();
@@ -146,6 +150,17 @@ __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, const float*
         }
     }
 }
+
+// Sum of the split-K partials in a fixed order: out[i] = sum_z part[z][i].
+__global__ void k_sum_partials(const float* __restrict__ part, float* __restrict__ out, size_t n, int nz) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int z = 0; z < nz; ++z) s += part[(size_t)z * n + i];
+    out[i] = s;
+}
+
+constexpr int kRowSplit = 8;
 
 // Column twiddles for the band rows: ctab[y][j] = (cos, sin)(2 pi ky_j y / H), double.
 __global__ void k_col_twiddles(double* ctab, int h, int kyn, int ny) {
@@ -219,6 +234,7 @@ __global__ void k_col_idft(const double* __restrict__ Y, const double* __restric
 }
 
 struct HpLayout {
+    float* part;
     float* tw;
     double* ctab;
     float* U;
@@ -233,6 +249,8 @@ HpLayout hp_layout(void* ws, int w, int h, const Band& b) {
     char* base = (char*)ws;
     size_t off = 0;
     HpLayout L;
+    L.part = (float*)(base + off);
+    off += al((size_t)kRowSplit * h * 2 * b.kxn * sizeof(float));
     L.tw = (float*)(base + off);
     off += al((size_t)w * 2 * b.kxn * sizeof(float));
     L.ctab = (double*)(base + off);
@@ -268,16 +286,19 @@ int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws,
         k_col_twiddles<<<dim3(div_up(h, 256), b.ny), 256, 0, s>>>(L.ctab, h, b.kyn, b.ny);
     }
     // 1. U = img . Tw  (M = h, N = 2 kxn, K = w)
-    k_sgemm<false, 0><<<dim3(div_up(n2, TN), div_up(h, TM)), 256, 0, s>>>(h, n2, w, img, pitch, L.tw, n2, L.U,
-                                                                          n2 * (int)sizeof(float));
+    const int kc = div_up(div_up(w, kRowSplit), TK) * TK;
+    k_sgemm<false, 0><<<dim3(div_up(n2, TN), div_up(h, TM), div_up(w, kc)), 256, 0, s>>>(
+        h, n2, w, img, pitch, L.tw, n2, L.part, n2 * (int)sizeof(float), kc, (size_t)h * n2);
+    const size_t nu = (size_t)h * n2;
+    k_sum_partials<<<(unsigned)((nu + 255) / 256), 256, 0, s>>>(L.part, L.U, nu, div_up(w, kc));
     // 2. band column DFT with the transfer function
     k_col_dft_partial<<<dim3(div_up(b.kxn, 64), b.ny, kColChunks), 64, 0, s>>>(L.U, L.ctab, L.P, h, b.kxn, b.ny);
     k_col_dft_finish<<<dim3(div_up(b.kxn, 64), b.ny), 64, 0, s>>>(L.P, L.Y, w, h, b.kxn, b.kyn, b.ny, hp, hps);
     // 3. inverse column DFT
     k_col_idft<<<dim3(div_up(b.kxn, 64), h), 64, 0, s>>>(L.Y, L.ctab, L.V, h, b.kxn, b.ny);
     // 4. img = clamp(img - V' . Tw^T)  (M = h, N = w, K = 2 kxn)
-    k_sgemm<true, 1><<<dim3(div_up(w, TN), div_up(h, TM)), 256, 0, s>>>(h, w, n2, L.V, n2 * (int)sizeof(float), L.tw,
-                                                                        n2, img, pitch);
+    k_sgemm<true, 1><<<dim3(div_up(w, TN), div_up(h, TM), 1), 256, 0, s>>>(h, w, n2, L.V, n2 * (int)sizeof(float),
+                                                                           L.tw, n2, img, pitch, n2, 0);
     return check_launch("fourier_highpass");
 }
 

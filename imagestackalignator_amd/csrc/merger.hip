@@ -37,7 +37,9 @@ int launch_derivatives2(const float* img, int img_pitch, float* ix, float* iy, i
                         hipStream_t s);
 int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles, int tiles_pitch, int tw, int th,
                  float* flow, int flow_pitch, float* work, int w, int h, int iterations, float bsx, float bsy,
-                 float rot, float min_det, int window, int quant, bool box, hipStream_t s);
+                 float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps);
+size_t tap_table_bytes(int w, int h);
+int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s);
 int launch_subsample3(const uint16_t* raw, float* out, float max_val, int hw, int hh, int out_pitch, const int cfa[4],
                       hipStream_t s);
 int launch_robustness(const float* ref, const float* mov, int img_pitch, float* mask, int mask_pitch, const float* flow,
@@ -110,6 +112,7 @@ struct isa_merger {
     float* wsum = nullptr;
     void* hp_ws = nullptr;
     bool hp_tables = false;
+    char* taps = nullptr;  // derivative tap tables (w, h, tex_quant), built at create
     // device tables: pa, pb, pk (m each), rows (2m), base (L * n * 3 floats), frame pointers (L * n),
     // accumulate pointer arrays (raw n, mask n, flow n)
     char* tab = nullptr;
@@ -278,6 +281,7 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
     A(M->lk_work, 4 * np);
     A(M->sum, np * 3);
     A(M->wsum, np * 3);
+    A(M->taps, isa::tap_table_bytes(width, height));
     if (rc == ISA_OK && P.use_high_pass) {
         char* ws = nullptr;
         A(ws, isa::highpass_workspace(width, height, P.high_pass, P.high_pass_sigma));
@@ -294,6 +298,10 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
     for (int i = 0; i <= ST_COUNT && rc == ISA_OK; ++i) rc = isa::check_hip(hipEventCreate(&M->ev[i]), "event");
     for (int i = 0; i < M->L && rc == ISA_OK; ++i)
         for (int j = 0; j < 4 && rc == ISA_OK; ++j) rc = isa::check_hip(hipEventCreate(&M->lev_ev[i][j]), "event");
+    // derivative tap tables depend only on (width, height, tex_quant): built once, on the null stream,
+    // and complete before create returns
+    if (rc == ISA_OK) rc = isa::build_tap_tables(M->taps, width, height, P.tex_quant, 0);
+    if (rc == ISA_OK) rc = isa::check_hip(hipStreamSynchronize(0), "tap tables");
     if (rc == ISA_OK) rc = isa::check_hip(hipEventRecord(M->upload_done, 0), "event");
     if (rc != ISA_OK) {
         delete M;
@@ -460,7 +468,8 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         // the final warp of R:PEFStudioDX/OpticalFlow.cs:162-163 is skipped.
         ISA_TRY(isa::lucas_kanade(M->lk + (size_t)f * np, lk_ref, wp, M->tile_img, M->cx[0] * 8, M->cx[0], M->cy[0],
                                   M->flows + (size_t)f * np * 2, 2 * wp, M->lk_work, w, h, P.iterations_lk, bsx, bsy,
-                                  store_rotation(st, ref, f), P.min_det_lk, P.window_size_lk, P.tex_quant, true, s));
+                                  store_rotation(st, ref, f), P.min_det_lk, P.window_size_lk, P.tex_quant, true, s,
+                                  M->taps));
     }
     (void)hipEventRecord(M->ev[5], s);
     for (int f = 0; f < n; ++f) {

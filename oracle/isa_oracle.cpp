@@ -380,14 +380,24 @@ void tiles_preshift(int w, int h, const T* img, T* tiles, const T* preshift /*tc
 // through cuFFT (R:PEFStudioDX/PatchTracker.cs:327-348; R:Kernels/kernel.cu:27-169).
 // Because the reference tile is zero outside its T x T interior and |s| <= M,
 // the circular correlation never wraps, so every term is a plain sum over the
-// interior. The three sums are evaluated in double in a canonical order: each
-// row term is an fma chain over x, the row terms are then added in increasing y;
-// D = (R2 + box) - 2 cc. Output per tile: (2M+1)^2 values, index (sy+M)*(2M+1) + (sx+M).
+// interior. Evaluated in double in a canonical order:
+//   R2[y] = sum_x r^2, TT[j][i] = sum_x t[j][x+i]^2, C[y][s] = sum_x r t(.+s)
+//   (each an fma chain in increasing x), Q[y][s] = fma(-2, C, TT[y+sy+M][sx+M]),
+//   D(s) = tree_y(R2) + tree_y(Q[.][s]) with the pairwise tree
+//   for stride = 1, 2, 4, ...: v[y] += v[y + stride] for y a multiple of 2 stride.
+// Output per tile: (2M+1)^2 values, index (sy+M)*(2M+1) + (sx+M).
+static void tree_sum(std::vector<double>& v, int n, int stride_elems, int count) {
+    // v holds n rows of `count` values each (row stride stride_elems); sums rows into row 0
+    for (int st = 1; st < n; st *= 2)
+        for (int y = 0; y + st < n; y += 2 * st)
+            for (int c = 0; c < count; ++c) v[(size_t)y * stride_elems + c] += v[(size_t)(y + st) * stride_elems + c];
+}
+
 template <class T>
 void tile_ssd(const T* ref_tiles, const T* trk_tiles, T* out, int M, int tile, int tile_count) {
     int B = tile + 2 * M, S = 2 * M + 1;
     parallel_for(tile_count, [&](int t0, int t1) {
-        std::vector<double> R2(tile), TT((size_t)B * S), P((size_t)tile * S * S);
+        std::vector<double> R2(tile), TT((size_t)B * S), Q((size_t)tile * S * S);
         for (int t = t0; t < t1; ++t) {
             const T* r = ref_tiles + (size_t)t * B * B;
             const T* k = trk_tiles + (size_t)t * B * B;
@@ -409,18 +419,11 @@ void tile_ssd(const T* ref_tiles, const T* trk_tiles, T* out, int M, int tile, i
                     for (int ix = 0; ix < S; ++ix) {
                         double acc = 0;
                         for (int x = 0; x < tile; ++x) acc = std::fma(rv(y, x), tv(y + iy, x + ix), acc);
-                        P[((size_t)y * S + iy) * S + ix] = acc;
+                        Q[((size_t)y * S + iy) * S + ix] = std::fma(-2.0, acc, TT[(size_t)(y + iy) * S + ix]);
                     }
-            for (int iy = 0; iy < S; ++iy)
-                for (int ix = 0; ix < S; ++ix) {
-                    double r2 = 0, box = 0, cc = 0;
-                    for (int y = 0; y < tile; ++y) {
-                        r2 += R2[y];
-                        box += TT[(size_t)(y + iy) * S + ix];
-                        cc += P[((size_t)y * S + iy) * S + ix];
-                    }
-                    out[(size_t)t * S * S + iy * S + ix] = (T)((r2 + box) - 2.0 * cc);
-                }
+            tree_sum(R2, tile, 1, 1);
+            tree_sum(Q, tile, S * S, S * S);
+            for (int s = 0; s < S * S; ++s) out[(size_t)t * S * S + s] = (T)(R2[0] + Q[s]);
         }
     });
 }
