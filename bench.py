@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--config", default="full_fp32", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quality", action="store_true", help="also report PSNR vs the noiseless ground truth")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -97,6 +99,15 @@ def main():
     import imagestackalignator_amd as isa
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.calibrate_copy:
+        n_bytes = args.calibrate_copy << 20
+        a = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "copies": 3}), flush=True)
+        return
     cfg = CONFIGS[args.config]
     w, h, n = cfg["w"], cfg["h"], cfg["n"]
     scfg = isa.default_synth(seed=20240601, width=w, height=h, frames=n, global_shift_max=cfg["gmax"],
@@ -127,14 +138,17 @@ def main():
     for i in range(len(cfg["levels"])):
         buf = (C.c_float * 3)()
         isa.call("isa_merger_level_ms", merger._h, i, buf)
-        levels.append({"level": cfg["levels"][i][0], "prep_ms": round(buf[0], 4), "track_ms": round(buf[1], 4),
+        l, t, m = cfg["levels"][i]
+        tiles = ((w // l - 2 * m) // t) * ((h // l - 2 * m) // t)
+        levels.append({"level": l, "tiles": tiles, "prep_ms": round(buf[0], 4), "track_ms": round(buf[1], 4),
                        "minimize_ms": round(buf[2], 4), "track_gflop": round(track_flops(cfg, i, m_pairs) / 1e9, 2)})
     log(f"[bench] {ms:.2f} ms/merge, stages: " + ", ".join(f"{k}={v:.2f}" for k, v in stages.items()))
     # dominant kernel: the longest single tile-search launch
     dom = max(levels, key=lambda d: d["track_ms"])
     achieved = dom["track_gflop"] / dom["track_ms"]  # GFLOP/ms = TFLOP/s
     roofline = {
-        "kernel": f"k_track_level (level {dom['level']}, all {m_pairs} pairs x tiles in one launch)",
+        "kernel": f"tile search at level {dom['level']} (k_track_rows / k_track_level: all {m_pairs} pairs x "
+                  f"{dom['tiles']} tiles in one launch)",
         "bound": "mfma",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP64_TFLOPS,

@@ -22,6 +22,13 @@ template <class T> __device__ __forceinline__ const T* row_ptr(const T* base, in
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// a / (float)n for an integer 1 <= n <= 2^24, bit-identical to the IEEE float division but without
+// its ~10-instruction sequence. If n is a power of two the quotient is exact on both routes.
+// Otherwise n has an odd factor > 1, so a/n (a float with a 24-bit significand) is never a float
+// midpoint and lies at least 2^-25/n (relative) from one; the double product a * rd(1/n) is
+// within 2^-52 of a/n, so its rounding to float is the correctly rounded quotient.
+__device__ __forceinline__ float div_n(float a, double inv_n) { return (float)((double)a * inv_n); }
+
 // ---------------------------------------------------------------------------
 // Texture-unit emulation: normalised coordinates, x = u*W - 0.5, i = floor(x),
 // alpha = frac(x) (optionally rounded to 8 fractional bits). Mirror addressing

@@ -1,6 +1,7 @@
 // Tile alignment: tile extraction, SSD search, findMinimum, shift up-sampling,
 // the fused per-level tracking launch, and the FP64 ShiftMinimizer.
 #include <algorithm>
+#include <numeric>
 #include <vector>
 
 #include "isa_device.h"
@@ -24,12 +25,16 @@ __device__ __forceinline__ void tile_offset(int tx, int ty, int tile, int w, int
     oy = (int)roundf(sy);
 }
 
-// SSD core. r: T x T reference interior (LDS, row stride RS), t: B x B track block (row stride TS).
-// D(s) = sum_y R2[y] + sum_y Q[y][s],  R2[y] = sum_x r^2,  Q[y][s] = TT[y+sy+M][sx+M] - 2 C[y][s],
-// TT[j][i] = sum_x t[j][x+i]^2,  C[y][s] = sum_x r[y][x] t[y+sy+M][x+sx+M],
-// all in FP64: every row term is an fma chain in increasing x; the sums over y are pairwise trees
-// (stride 1, 2, 4, ...: Q[y] += Q[y + stride] for y a multiple of 2 stride). LDS: Q T*S*S doubles,
-// TT B*S doubles, R2 T doubles, D S*S floats.
+// SSD core. r: T x T reference interior (LDS, row stride RS), t: B x B track block (row stride TS),
+// B = T + 2M, S = 2M + 1, lag index i = s + M. Canonical column-wise FP64 order:
+//   R2[x]     = sum_y r(y,x)^2                                (fma chain in increasing y)
+//   TT[c][0]  = sum_{y<T} t[y][c]^2                          (fma chain in increasing y)
+//   TT[c][k+1] = (TT[c][k] - t[k][c]^2) + t[k+T][c]^2         (squares exact in double)
+//   C[x][i]   = sum_y r(y,x) t[y+iy][x+ix]                    (fma chain in increasing y)
+//   Q[x][i]   = fma(-2, C[x][i], TT[x+ix][iy])
+//   D(s)      = (float)(tree_x R2 + tree_x Q[.][i])
+// with the pairwise tree: for stride 1, 2, 4, ...: v[x] += v[x + stride] for x a multiple of
+// 2 stride. LDS: Q T*S*S doubles, TT B*S doubles, R2 T doubles, D S*S floats.
 __device__ __forceinline__ int ssd_rs(int T) { return T + 1; }
 __device__ __forceinline__ int ssd_ts(int T, int M) { return T + 2 * M + 1; }
 
@@ -38,71 +43,75 @@ __device__ void ssd_core(const float* r, const float* t, int T, double* Q, doubl
                          int nthreads) {
     constexpr int S = 2 * M + 1;
     const int B = T + 2 * M, RS = ssd_rs(T), TS = ssd_ts(T, M);
-    // 1. window energies of the track rows and the reference row energies
-    for (int task = tid; task < B * S + T; task += nthreads) {
+    // 1. column energies of the reference, window energies of the track columns
+    for (int task = tid; task < B + T; task += nthreads) {
         double acc = 0.0;
-        if (task < B * S) {
-            int j = task / S, i = task - j * S;
-            const float* trow = t + j * TS + i;
-            for (int x = 0; x < T; ++x) {
-                double v = (double)trow[x];
+        if (task < B) {
+            const int c = task;
+            for (int y = 0; y < T; ++y) {
+                double v = (double)t[y * TS + c];
                 acc = fma(v, v, acc);
             }
-            TT[task] = acc;
+            TT[c * S] = acc;
+            for (int k = 0; k + 1 < S; ++k) {
+                double a = (double)t[k * TS + c], b = (double)t[(k + T) * TS + c];
+                acc = (acc - a * a) + b * b;
+                TT[c * S + k + 1] = acc;
+            }
         } else {
-            const float* rrow = r + (task - B * S) * RS;
-            for (int x = 0; x < T; ++x) {
-                double v = (double)rrow[x];
+            const int x = task - B;
+            for (int y = 0; y < T; ++y) {
+                double v = (double)r[y * RS + x];
                 acc = fma(v, v, acc);
             }
-            R2[task - B * S] = acc;
+            R2[x] = acc;
         }
     }
     __syncthreads();
-    // 2. cross terms, task (y, sy). The S-wide window of the track row stays in registers: element
-    // t[q] lives in slot q % S and the x loop is unrolled by S, so the window never moves.
+    // 2. cross terms, task (x, ix). The S-tall window of track column x+ix stays in registers: row
+    // q lives in slot q % S and the y loop is unrolled by S, so the window never moves.
     for (int task = tid; task < T * S; task += nthreads) {
-        int y = task / S, iy = task - y * S;  // sy = iy - M
-        const float* trow = t + (y + iy) * TS;
-        const float* rrow = r + y * RS;
+        const int x = task / S, ix = task - x * S;
+        const float* tcol = t + x + ix;
+        const float* rcol = r + x;
         double acc[S], win[S];
 #pragma unroll
         for (int i = 0; i < S; ++i) acc[i] = 0.0;
 #pragma unroll
-        for (int i = 0; i < S - 1; ++i) win[i] = (double)trow[i];
-        int x = 0;
-        for (; x + S <= T; x += S) {
+        for (int i = 0; i < S - 1; ++i) win[i] = (double)tcol[i * TS];
+        int y = 0;
+        for (; y + S <= T; y += S) {
 #pragma unroll
             for (int k = 0; k < S; ++k) {
-                win[(k + S - 1) % S] = (double)trow[x + k + S - 1];
-                const double rv = (double)rrow[x + k];
+                win[(k + S - 1) % S] = (double)tcol[(y + k + S - 1) * TS];
+                const double rv = (double)rcol[(y + k) * RS];
 #pragma unroll
                 for (int i = 0; i < S; ++i) acc[i] = fma(rv, win[(k + i) % S], acc[i]);
             }
         }
-        const int rem = T - x;
+        const int rem = T - y;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             if (k < rem) {
-                win[(k + S - 1) % S] = (double)trow[x + k + S - 1];
-                const double rv = (double)rrow[x + k];
+                win[(k + S - 1) % S] = (double)tcol[(y + k + S - 1) * TS];
+                const double rv = (double)rcol[(y + k) * RS];
 #pragma unroll
                 for (int i = 0; i < S; ++i) acc[i] = fma(rv, win[(k + i) % S], acc[i]);
             }
         }
 #pragma unroll
-        for (int i = 0; i < S; ++i) Q[(y * S + iy) * S + i] = fma(-2.0, acc[i], TT[(y + iy) * S + i]);
+        for (int iy = 0; iy < S; ++iy) Q[(x * S + iy) * S + ix] = fma(-2.0, acc[iy], TT[(x + ix) * S + iy]);
     }
     __syncthreads();
-    // 3. pairwise trees over y (Q for every lag, R2 once)
+    // 3. pairwise trees over x (Q for every lag, R2 once)
     for (int stride = 1; stride < T; stride *= 2) {
         const int pairs = (T + 2 * stride - 1) / (2 * stride);
         for (int task = tid; task < pairs * (S * S + 1); task += nthreads) {
             int pr = task / (S * S + 1), s = task - pr * (S * S + 1);
-            int y = pr * 2 * stride;
-            if (y + stride >= T) continue;
-            if (s < S * S) Q[y * S * S + s] += Q[(y + stride) * S * S + s];
-            else R2[y] += R2[y + stride];
+            int x = pr * 2 * stride;
+            if (x + stride >= T) continue;
+            if (s < S * S) Q[x * S * S + s] += Q[(x + stride) * S * S + s];
+            else R2[x] += R2[x + stride];
         }
         __syncthreads();
     }
@@ -241,6 +250,204 @@ __global__ void k_track_level(TrackArgs a) {
     if (tid == 0) {
         float cx, cy;
         find_min_core(L.D, M, a.threshold, cx, cy);
+        sh[0] = psx + cx;
+        sh[1] = psy + cy;
+    }
+}
+
+// Column-parallel form of the same search for tiles with 64 % T == 0 (T >= 4) and M in {1, 2, 4}.
+// Lane (g, x) owns reference column x and the lags ix = g*RPG .. g*RPG+RPG-1 (all iy): it keeps
+// RPG x S cross-term accumulators and, per ix, the S-tall register window of track column x+ix,
+// so one load and one conversion feed S FMAs. Every operand is read straight from the level
+// image (a row of the tile is contiguous across lanes), the window energies TT go through LDS,
+// and the trees over x run on lane shuffles (a group's T lanes lie within one wave). The order of
+// every sum is that of ssd_core, so D is bit-identical to the block-parallel form.
+// One block holds PPB (pair, tile) items; items are spread so that each XCD works on a
+// contiguous range of tiles.
+template <int M> struct ColsPerGroup;
+template <> struct ColsPerGroup<1> { static constexpr int v = 3; };
+template <> struct ColsPerGroup<2> { static constexpr int v = 5; };
+template <> struct ColsPerGroup<4> { static constexpr int v = 3; };
+
+struct ColsShape {
+    int lanes_per_item, items_per_block, threads;
+};
+
+ColsShape cols_shape(int T, int M) {
+    const int S = 2 * M + 1, rpg = M == 2 ? 5 : 3;
+    ColsShape c;
+    c.lanes_per_item = (S / rpg) * T;
+    int g = std::gcd(c.lanes_per_item, 64);
+    c.items_per_block = 64 / g;
+    c.threads = c.items_per_block * c.lanes_per_item;
+    return c;
+}
+
+size_t cols_smem(int T, int M, int items) {
+    int S = 2 * M + 1, B = T + 2 * M;
+    return (size_t)items * ((size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7));
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_track_cols(TrackArgs a, int pairs, int items_total, int ppb, int lpi,
+                                                   int nblk) {
+    constexpr int S = 2 * M + 1, RPG = ColsPerGroup<M>::v, G = S / RPG;
+    static_assert(G * RPG == S, "groups must tile the lags");
+    extern __shared__ __align__(16) char smem[];
+    const int T = a.tile, B = T + 2 * M;
+    const int tid = threadIdx.x;
+    // XCD-aware block order: consecutive hardware block ids go round-robin over 8 XCDs; give each
+    // XCD a contiguous run of items so its L2 holds the tile regions its pairs share. The grid has
+    // 8 * per blocks, so (id % 8, id / 8) covers every slot of [0, 8 * per); slots >= nblk idle.
+    const int per = (int)(gridDim.x / 8);
+    const int blk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (blk >= nblk) return;
+    const size_t item_bytes = (size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7);
+    // per-item lane coordinates
+    const int pp = tid / lpi, lane = tid - pp * lpi;
+    const int g = lane / T, x = lane - g * T;
+    const int item = blk * ppb + pp;
+    const bool live = item < items_total;
+    const int it = live ? item : items_total - 1;
+    const int tile_idx = it / pairs, p = it - tile_idx * pairs;
+    const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
+    const int fa = a.pa[p], fb = a.pb[p], k = a.pk[p];
+    float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
+    const float psx = sh[0], psy = sh[1];
+    int rox, roy, tox, toy;
+    tile_offset(tx, ty, T, a.w, a.h, a.base[fa * 3], a.base[fa * 3 + 1], a.base[fa * 3 + 2], 0.0f, 0.0f, rox, roy);
+    tile_offset(tx, ty, T, a.w, a.h, a.base[fb * 3], a.base[fb * 3 + 1], a.base[fb * 3 + 2], psx, psy, tox, toy);
+    const float* ia = a.frames[fa];
+    const float* ib = a.frames[fb];
+    char* mine = smem + (size_t)pp * item_bytes;
+    double* TT = (double*)mine;
+    double* R2tot = TT + B * S;
+    float* Dm = (float*)(R2tot + 1);
+    // block origins in the level image
+    const int rx0 = tx * T + M + rox, ry0 = ty * T + M + roy;  // reference interior
+    const int tx0 = tx * T + tox, ty0 = ty * T + toy;          // track block
+    auto rowp = [&](const float* img, int gy) { return row_ptr(img, a.img_pitch, clampi(gy, 0, a.h - 1)); };
+    auto colc = [&](int gx) { return clampi(gx, 0, a.w - 1); };
+    // 1. window energies of the track columns, one column per task
+    for (int c = lane; c < B; c += lpi) {
+        const int gx = colc(tx0 + c);
+        double acc = 0.0;
+        for (int y = 0; y < T; ++y) {
+            double v = (double)rowp(ib, ty0 + y)[gx];
+            acc = fma(v, v, acc);
+        }
+        TT[c * S] = acc;
+        for (int q = 0; q + 1 < S; ++q) {
+            double u = (double)rowp(ib, ty0 + q)[gx], v = (double)rowp(ib, ty0 + q + T)[gx];
+            acc = (acc - u * u) + v * v;
+            TT[c * S + q + 1] = acc;
+        }
+    }
+    // 2. cross terms and reference column energy
+    double acc[RPG][S], win[RPG][S];
+    double r2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPG; ++q)
+#pragma unroll
+        for (int i = 0; i < S; ++i) acc[q][i] = 0.0;
+    const bool inside = rx0 >= 0 && rx0 + T <= a.w && ry0 >= 0 && ry0 + T <= a.h && tx0 >= 0 && tx0 + B <= a.w &&
+                        ty0 >= 0 && ty0 + B <= a.h;
+    if (inside) {
+        // interior tile: no clamping, row pointers advance by one pitch per step
+        typedef const __attribute__((address_space(1))) float* gptr;
+        const size_t pitch = (size_t)a.img_pitch / sizeof(float);
+        gptr rp = (gptr)(ia + (size_t)ry0 * pitch + rx0 + x);
+        gptr tp = (gptr)(ib + (size_t)ty0 * pitch + tx0 + x + g * RPG);
+#pragma unroll
+        for (int i = 0; i < S - 1; ++i)
+#pragma unroll
+            for (int q = 0; q < RPG; ++q) win[q][i] = (double)tp[i * pitch + q];
+        tp += (S - 1) * pitch;
+        auto step = [&](int kk) {
+            const double rv = (double)rp[0];
+            r2 = fma(rv, rv, r2);
+#pragma unroll
+            for (int q = 0; q < RPG; ++q) win[q][(kk + S - 1) % S] = (double)tp[q];
+            rp += pitch;
+            tp += pitch;
+#pragma unroll
+            for (int q = 0; q < RPG; ++q)
+#pragma unroll
+                for (int i = 0; i < S; ++i) acc[q][i] = fma(rv, win[q][(kk + i) % S], acc[q][i]);
+        };
+        int y = 0;
+        for (; y + S <= T; y += S) {
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) step(kk);
+        }
+#pragma unroll
+        for (int kk = 0; kk < S; ++kk)
+            if (kk < T - y) step(kk);
+    } else {
+        const int rgx = colc(rx0 + x);
+        int tgx[RPG];
+#pragma unroll
+        for (int q = 0; q < RPG; ++q) tgx[q] = colc(tx0 + x + g * RPG + q);
+#pragma unroll
+        for (int i = 0; i < S - 1; ++i) {
+            const float* tr = rowp(ib, ty0 + i);
+#pragma unroll
+            for (int q = 0; q < RPG; ++q) win[q][i] = (double)tr[tgx[q]];
+        }
+        auto step = [&](int y, int kk) {
+            const double rv = (double)rowp(ia, ry0 + y)[rgx];
+            const float* tr = rowp(ib, ty0 + y + S - 1);
+            r2 = fma(rv, rv, r2);
+#pragma unroll
+            for (int q = 0; q < RPG; ++q) win[q][(kk + S - 1) % S] = (double)tr[tgx[q]];
+#pragma unroll
+            for (int q = 0; q < RPG; ++q)
+#pragma unroll
+                for (int i = 0; i < S; ++i) acc[q][i] = fma(rv, win[q][(kk + i) % S], acc[q][i]);
+        };
+        int y = 0;
+        for (; y + S <= T; y += S) {
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) step(y + kk, kk);
+        }
+#pragma unroll
+        for (int kk = 0; kk < S; ++kk)
+            if (kk < T - y) step(y + kk, kk);
+    }
+    __syncthreads();  // TT complete
+#pragma unroll
+    for (int q = 0; q < RPG; ++q)
+#pragma unroll
+        for (int iy = 0; iy < S; ++iy) acc[q][iy] = fma(-2.0, acc[q][iy], TT[(x + g * RPG + q) * S + iy]);
+    // 3. pairwise trees over x on lane shuffles
+    for (int st = 1; st < T; st *= 2) {
+        const bool take = (x % (2 * st)) == 0 && x + st < T;
+        double o = __shfl_down(r2, st);
+        if (take) r2 += o;
+#pragma unroll
+        for (int q = 0; q < RPG; ++q)
+#pragma unroll
+            for (int iy = 0; iy < S; ++iy) {
+                o = __shfl_down(acc[q][iy], st);
+                if (take) acc[q][iy] += o;
+            }
+    }
+    if (x == 0 && g == 0) *R2tot = r2;
+    __syncthreads();
+    if (x == 0) {
+        const double r2t = *R2tot;
+#pragma unroll
+        for (int q = 0; q < RPG; ++q)
+#pragma unroll
+            for (int iy = 0; iy < S; ++iy) Dm[iy * S + g * RPG + q] = (float)(r2t + acc[q][iy]);
+    }
+    __syncthreads();
+    if (!live) return;
+    if (a.shift_images)
+        for (int s = lane; s < S * S; s += lpi) a.shift_images[((size_t)p * a.cx * a.cy + tile_idx) * S * S + s] = Dm[s];
+    if (lane == 0) {
+        float cx, cy;
+        find_min_core(Dm, M, a.threshold, cx, cy);
         sh[0] = psx + cx;
         sh[1] = psy + cy;
     }
@@ -633,6 +840,16 @@ int shift_tables(int n, int ref, int strategy, int block_in, int* pairs, int* in
 
 template <int M>
 static int launch_track_level_m(const TrackArgs& a, int pairs, hipStream_t s) {
+    if constexpr (M == 1 || M == 2 || M == 4) {
+        if (a.tile >= 4 && a.tile <= 64 && 64 % a.tile == 0) {
+            const ColsShape c = cols_shape(a.tile, M);
+            const int items = pairs * a.cx * a.cy;
+            const int blocks = div_up(items, c.items_per_block);
+            k_track_cols<M><<<8 * div_up(blocks, 8), c.threads, cols_smem(a.tile, M, c.items_per_block), s>>>(
+                a, pairs, items, c.items_per_block, c.lanes_per_item, blocks);
+            return check_launch("track_cols");
+        }
+    }
     size_t smem = ssd_smem(a.tile, M);
     k_track_level<M><<<dim3(pairs, a.cx * a.cy), block_threads(a.tile, M), smem, s>>>(a);
     return check_launch("track_level");

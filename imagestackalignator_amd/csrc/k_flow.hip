@@ -16,7 +16,8 @@ __global__ void k_flow_from_tiles(float* flow, int flow_pitch, const float* tile
     float pcx = (float)(x - w / 2), pcy = (float)(y - h / 2);
     fx += c * pcx - s * pcy - pcx;
     fy += s * pcx + c * pcy - pcy;
-    float u = ((float)x + 0.5f) / (float)w, v = ((float)y + 0.5f) / (float)h;
+    const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
+    float u = div_n((float)x + 0.5f, iw), v = div_n((float)y + 0.5f, ih);
     float t[2];
     tex_linear_all<2>(tiles, tiles_pitch, tw, th, u, v, ISA_ADDR_CLAMP, quant, t);
     float* o = row_ptr(flow, flow_pitch, y) + x * 2;
@@ -29,12 +30,13 @@ __global__ void k_warp(const float* src, int src_pitch, float* out, int out_pitc
                        int w, int h, int quant) {
     int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y * blockDim.y + threadIdx.y;
     if (ix >= w || iy >= h) return;
-    float u = ((float)ix + 0.5f) / (float)w, v = ((float)iy + 0.5f) / (float)h;
+    const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
+    float u = div_n((float)ix + 0.5f, iw), v = div_n((float)iy + 0.5f, ih);
     int fxi = tex_addr((int)floorf(u * (float)w), w, ISA_ADDR_CLAMP);
     int fyi = tex_addr((int)floorf(v * (float)h), h, ISA_ADDR_CLAMP);
     const float* f = row_ptr(flow, flow_pitch, fyi) + fxi * 2;
-    float x = ((float)ix + 0.5f + f[0]) / (float)w;
-    float y = ((float)iy + 0.5f + f[1]) / (float)h;
+    float x = div_n((float)ix + 0.5f + f[0], iw);
+    float y = div_n((float)iy + 0.5f + f[1], ih);
     row_ptr(out, out_pitch, iy)[ix] = tex_linear<1>(src, src_pitch, w, h, 0, x, y, ISA_ADDR_MIRROR, quant);
 }
 
@@ -44,7 +46,7 @@ __device__ __forceinline__ float stencil5(const float* img, int pitch, int w, in
     t -= tex_linear<1>(img, pitch, w, h, 0, x + 1.0f * dx, y + 1.0f * dy, ISA_ADDR_MIRROR, quant) * 8.0f;
     t += tex_linear<1>(img, pitch, w, h, 0, x - 1.0f * dx, y - 1.0f * dy, ISA_ADDR_MIRROR, quant) * 8.0f;
     t -= tex_linear<1>(img, pitch, w, h, 0, x - 2.0f * dx, y - 2.0f * dy, ISA_ADDR_MIRROR, quant);
-    t /= 12.0f;
+    t = div_n(t, 1.0 / 12.0);
     return t;
 }
 
@@ -79,57 +81,83 @@ __global__ void k_tap_tables(Tap* col, Tap* row, int w, int h, int quant) {
     }
 }
 
-__device__ __forceinline__ float tap_fetch(const float* img, int pitch, const Tap& c, const Tap& r) {
-    const float* ra = row_ptr(img, pitch, r.i0);
+// ComputeDerivativesKernel (R:Kernels/opticalFlow.cu:96-147): 5-point stencils of both images along x
+// and y, averaged, and the temporal difference, with the stencil taps read from the tables.
+// Every tap index of pixel (ix, iy) lies in [ix-3, ix+3] x [iy-3, iy+3] (mirroring at the edges
+// only moves an index towards the pixel), so both images are staged in LDS as the block plus a
+// 3-pixel halo and all fetches are served from there.
+constexpr int DV_BX = 64, DV_BY = 8, DV_H = 3, DV_LW = DV_BX + 2 * DV_H, DV_LH = DV_BY + 2 * DV_H;
+
+struct LTap {
+    int c0, c1;  // local texel columns (or rows)
+    float a;
+    int p;       // the single texel when a is 0 or 1 (then the bilinear sum is that texel exactly)
+};
+
+template <int LW>
+__device__ __forceinline__ float lds_fetch_w(const float (*L)[LW], const LTap& c, const LTap& r) {
     const float a = c.a, b = r.a;
     // weights at the corners reduce the bilinear sum to one texel exactly (finite data)
     if (b == 0.0f) {
-        if (a == 0.0f) return ra[c.i0];
-        if (a == 1.0f) return ra[c.i1];
+        if (a == 0.0f) return L[r.c0][c.c0];
+        if (a == 1.0f) return L[r.c0][c.c1];
     } else if (b == 1.0f) {
-        const float* rb = row_ptr(img, pitch, r.i1);
-        if (a == 0.0f) return rb[c.i0];
-        if (a == 1.0f) return rb[c.i1];
+        if (a == 0.0f) return L[r.c1][c.c0];
+        if (a == 1.0f) return L[r.c1][c.c1];
     }
-    const float* rb = row_ptr(img, pitch, r.i1);
-    float t00 = ra[c.i0], t10 = ra[c.i1], t01 = rb[c.i0], t11 = rb[c.i1];
+    float t00 = L[r.c0][c.c0], t10 = L[r.c0][c.c1], t01 = L[r.c1][c.c0], t11 = L[r.c1][c.c1];
     float oma = 1.0f - a, omb = 1.0f - b;
     return oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11;
 }
 
-// ComputeDerivativesKernel (R:Kernels/opticalFlow.cu:96-147): 5-point stencils of both images along x
-// and y, averaged, and the temporal difference, with the stencil taps read from the tables.
-__global__ void k_derivatives_tab(const float* src, const float* tgt, int img_pitch, float* Ix, float* Iy, float* Iz,
-                                  int pitch, int w, int h, const Tap* __restrict__ col, const Tap* __restrict__ row) {
-    int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y * blockDim.y + threadIdx.y;
+__device__ __forceinline__ float lds_fetch(const float (*L)[DV_LW], const LTap& c, const LTap& r) {
+    return lds_fetch_w<DV_LW>(L, c, r);
+}
+
+__global__ __launch_bounds__(DV_BX* DV_BY) void k_derivatives_tab(const float* src, const float* tgt, int img_pitch,
+                                                                   float* Ix, float* Iy, float* Iz, int pitch, int w,
+                                                                   int h, const Tap* __restrict__ col,
+                                                                   const Tap* __restrict__ row) {
+    __shared__ float Ls[DV_LH][DV_LW], Lt[DV_LH][DV_LW];
+    const int x0 = blockIdx.x * DV_BX - DV_H, y0 = blockIdx.y * DV_BY - DV_H;
+    const int tid = threadIdx.y * DV_BX + threadIdx.x;
+    for (int i = tid; i < DV_LH * DV_LW; i += DV_BX * DV_BY) {
+        int ly = i / DV_LW, lx = i - ly * DV_LW;
+        int gy = clampi(y0 + ly, 0, h - 1), gx = clampi(x0 + lx, 0, w - 1);
+        Ls[ly][lx] = row_ptr(src, img_pitch, gy)[gx];
+        Lt[ly][lx] = row_ptr(tgt, img_pitch, gy)[gx];
+    }
+    __syncthreads();
+    const int ix = blockIdx.x * DV_BX + threadIdx.x, iy = blockIdx.y * DV_BY + threadIdx.y;
     if (ix >= w || iy >= h) return;
-    Tap c[5], r[5];
+    LTap c[5], r[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        c[k] = col[(size_t)ix * 5 + k];
-        r[k] = row[(size_t)iy * 5 + k];
+        Tap tc = col[(size_t)ix * 5 + k], tr = row[(size_t)iy * 5 + k];
+        c[k] = {clampi(tc.i0 - x0, 0, DV_LW - 1), clampi(tc.i1 - x0, 0, DV_LW - 1), tc.a, 0};
+        r[k] = {clampi(tr.i0 - y0, 0, DV_LH - 1), clampi(tr.i1 - y0, 0, DV_LH - 1), tr.a, 0};
     }
-    auto sx = [&](const float* I) {
-        float t = tap_fetch(I, img_pitch, c[4], r[2]);
-        t -= tap_fetch(I, img_pitch, c[3], r[2]) * 8.0f;
-        t += tap_fetch(I, img_pitch, c[1], r[2]) * 8.0f;
-        t -= tap_fetch(I, img_pitch, c[0], r[2]);
-        t /= 12.0f;
+    auto sx = [&](const float (*L)[DV_LW]) {
+        float t = lds_fetch(L, c[4], r[2]);
+        t -= lds_fetch(L, c[3], r[2]) * 8.0f;
+        t += lds_fetch(L, c[1], r[2]) * 8.0f;
+        t -= lds_fetch(L, c[0], r[2]);
+        t = div_n(t, 1.0 / 12.0);
         return t;
     };
-    auto sy = [&](const float* I) {
-        float t = tap_fetch(I, img_pitch, c[2], r[4]);
-        t -= tap_fetch(I, img_pitch, c[2], r[3]) * 8.0f;
-        t += tap_fetch(I, img_pitch, c[2], r[1]) * 8.0f;
-        t -= tap_fetch(I, img_pitch, c[2], r[0]);
-        t /= 12.0f;
+    auto sy = [&](const float (*L)[DV_LW]) {
+        float t = lds_fetch(L, c[2], r[4]);
+        t -= lds_fetch(L, c[2], r[3]) * 8.0f;
+        t += lds_fetch(L, c[2], r[1]) * 8.0f;
+        t -= lds_fetch(L, c[2], r[0]);
+        t = div_n(t, 1.0 / 12.0);
         return t;
     };
-    float t0 = sx(src), t1 = sx(tgt);
+    float t0 = sx(Ls), t1 = sx(Lt);
     row_ptr(Ix, pitch, iy)[ix] = (t0 + t1) * 0.5f;
-    row_ptr(Iz, pitch, iy)[ix] = tap_fetch(src, img_pitch, c[2], r[2]) - tap_fetch(tgt, img_pitch, c[2], r[2]);
-    t0 = sy(src);
-    t1 = sy(tgt);
+    row_ptr(Iz, pitch, iy)[ix] = lds_fetch(Ls, c[2], r[2]) - lds_fetch(Lt, c[2], r[2]);
+    t0 = sy(Ls);
+    t1 = sy(Lt);
     row_ptr(Iy, pitch, iy)[ix] = (t0 + t1) * 0.5f;
 }
 
@@ -146,14 +174,40 @@ __global__ void k_derivatives2(const float* img, int img_pitch, float* Ix, float
 
 // Closed-form 2x2 SVD pseudo-inverse of [[a, b], [b, d]] (R:Kernels/opticalFlow.cu:236-292).
 // Returns false when the gate sigma1 < minDet holds (the reference's fminf(sigma1, sigma1)).
+// cos and sin of 0.5 * atan2(y, x) without trigonometric calls: with (X, Y) scaled to a unit vector
+// (cos 2t, sin 2t), cos t = sqrt((1 + cos 2t) / 2) and sin t = sin 2t / (2 cos t) for x >= 0, and the
+// mirrored pair for x < 0 (t in [-pi/2, pi/2], so cos t >= 0; the sign of sin t follows y, -0 included).
+__device__ __forceinline__ void half_angle(float y, float x, float& c, float& s) {
+    float m = fmaxf(fabsf(x), fabsf(y));
+    if (!(m > 0.0f)) {  // atan2(+-0, +-0) and NaN inputs: t = 0 (x = -0 gives pi; only the sign of
+        c = (signbit(x) && !isnan(x)) ? 0.0f : 1.0f;  // U matters there and no update survives the gate)
+        s = (signbit(x) && !isnan(x)) ? copysignf(1.0f, y) : 0.0f;
+        return;
+    }
+    const float rm = __builtin_amdgcn_rcpf(m);
+    float xs = x * rm, ys = y * rm;
+    float rr = rsqrtf(xs * xs + ys * ys);
+    float c2 = xs * rr, s2 = ys * rr;
+    if (c2 >= 0.0f) {
+        c = sqrtf(0.5f * (1.0f + c2));
+        s = s2 * 0.5f * __builtin_amdgcn_rcpf(c);
+    } else {
+        s = copysignf(sqrtf(0.5f * (1.0f - c2)), s2);
+        c = s2 * 0.5f * __builtin_amdgcn_rcpf(s);
+    }
+}
+
 template <bool EXACT>
 __device__ __forceinline__ bool lk_pinv(float a, float b, float d, float min_det, float inv[4]) {
-    auto at2 = [](float y, float x) { return EXACT ? atan2_r(y, x) : atan2f(y, x); };
-    auto cs = [](float x) { return EXACT ? cos_r(x) : cosf(x); };
-    auto sn = [](float x) { return EXACT ? sin_r(x) : sinf(x); };
     float c = b;
-    float theta = 0.5f * at2(2.0f * a * c + 2.0f * b * d, a * a + b * b - c * c - d * d);
-    float ct = cs(theta), st = sn(theta);
+    float ct, st;
+    if (EXACT) {
+        float theta = 0.5f * atan2_r(2.0f * a * c + 2.0f * b * d, a * a + b * b - c * c - d * d);
+        ct = cos_r(theta);
+        st = sin_r(theta);
+    } else {
+        half_angle(2.0f * a * c + 2.0f * b * d, a * a + b * b - c * c - d * d, ct, st);
+    }
     float UT0 = ct, UT2 = -st, UT1 = st, UT3 = ct;
     float S1 = a * a + b * b + c * c + d * d;
     float S2 = sqrtf((a * a + b * b - c * c - d * d) * (a * a + b * b - c * c - d * d) +
@@ -165,8 +219,14 @@ __device__ __forceinline__ bool lk_pinv(float a, float b, float d, float min_det
     sigma1 = sigma1 != 0 ? 1.0f / sigma1 : 0.0f;
     sigma2 = sigma2 != 0 ? 1.0f / sigma2 : 0.0f;
     float S0 = sigma1, Sb = 0.0f, Sc = 0.0f, S3 = sigma2;
-    float eps = 0.5f * at2(2.0f * a * b + 2.0f * c * d, a * a - b * b + c * c - d * d);
-    float ce = cs(eps), se = sn(eps);
+    float ce, se;
+    if (EXACT) {
+        float eps = 0.5f * atan2_r(2.0f * a * b + 2.0f * c * d, a * a - b * b + c * c - d * d);
+        ce = cos_r(eps);
+        se = sin_r(eps);
+    } else {
+        half_angle(2.0f * a * b + 2.0f * c * d, a * a - b * b + c * c - d * d, ce, se);
+    }
     float s11 = (a * ct + c * st) * ce + (b * ct + d * st) * se;
     float s22 = (a * st - c * ct) * se + (-b * st + d * ct) * ce;
     s11 = s11 > 0.0f ? 1.0f : s11 < 0 ? -1.0f : 0.0f;
@@ -285,6 +345,205 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
     f[1] += u1;
 }
 
+// One Lucas-Kanade iteration fused into one kernel (warp, derivatives, window sums, solve, update)
+// for half-windows HW <= 8. Per 64 x 16 block of output pixels:
+//  A. the warped source and the target over the block plus HW + 3 pixels of halo, into LDS
+//     (same arithmetic as k_warp);
+//  B. Ix, Iy, It over the block plus HW of halo from those tiles (same arithmetic as
+//     k_derivatives_tab; pixels outside the image are never inside the window of an updated
+//     pixel and are set to 0);
+//  C. per output pixel the five window sums: each window row is summed in increasing x, then
+//     the rows in increasing y;
+//  D. the closed-form solve and update, reading flow_in and writing flow_out (the halo of the
+//     warp reads neighbouring blocks' flow, so the flow is double-buffered across iterations).
+constexpr int LKF_BX = 64, LKF_BY = 16, LKF_RPT = 4;  // 256 threads, 4 output rows each
+
+template <int HW>
+__global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
+    const float* src, const float* tgt, int img_pitch, const float* flow_in, float* flow_out, int flow_pitch, int w,
+    int h, float min_det, int quant, const Tap* __restrict__ col, const Tap* __restrict__ row) {
+    constexpr int DH = LKF_BY + 2 * HW, DW = LKF_BX + 2 * HW;  // derivative region
+    constexpr int WR = DH + 6, WC = DW + 6;                     // warped region
+    constexpr int NT = LKF_BX * (LKF_BY / LKF_RPT);
+    __shared__ float Lw[WR][WC], Lt[WR][WC];
+    __shared__ float Ld[3][DH][DW];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * LKF_BX, y0 = blockIdx.y * LKF_BY;
+    const int wx0 = x0 - HW - 3, wy0 = y0 - HW - 3;
+    __shared__ LTap Tc[DW][5], Tr[DH][5];
+    __shared__ int s_mixed;  // some tap of the block has a weight strictly between 0 and 1
+    if (tid == 0) s_mixed = 0;
+    __syncthreads();
+    // tap tables of the block's derivative region, translated to warped-tile coordinates
+    for (int i = tid; i < (DW + DH) * 5; i += NT) {
+        const bool is_c = i < DW * 5;
+        const int e = is_c ? i : i - DW * 5, pix = e / 5, k = e - pix * 5;
+        const int g = (is_c ? x0 : y0) - HW + pix, n = is_c ? w : h, o = is_c ? wx0 : wy0, lim = is_c ? WC : WR;
+        const Tap t = (is_c ? col : row)[(size_t)clampi(g, 0, n - 1) * 5 + k];
+        LTap l = {clampi(t.i0 - o, 0, lim - 1), clampi(t.i1 - o, 0, lim - 1), t.a, 0};
+        l.p = t.a == 0.0f ? l.c0 : l.c1;
+        if (t.a != 0.0f && t.a != 1.0f) s_mixed = 1;
+        if (is_c) Tc[pix][k] = l;
+        else Tr[pix][k] = l;
+    }
+    // A. warped source and target: all flow loads first, then all source gathers (no branch between
+    // them, so every load of a thread is in flight at once)
+    constexpr int NA = (WR * WC + NT - 1) / NT;
+    const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
+    float fa[NA], fb[NA];
+    int gxs[NA], gys[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int i = min(tid + k * NT, WR * WC - 1);
+        const int ly = i / WC, lx = i - ly * WC;
+        const int gx = clampi(wx0 + lx, 0, w - 1), gy = clampi(wy0 + ly, 0, h - 1);
+        gxs[k] = gx;
+        gys[k] = gy;
+        float u = div_n((float)gx + 0.5f, iw), v = div_n((float)gy + 0.5f, ih);
+        int fxi = tex_addr((int)floorf(u * (float)w), w, ISA_ADDR_CLAMP);
+        int fyi = tex_addr((int)floorf(v * (float)h), h, ISA_ADDR_CLAMP);
+        const float* f = row_ptr(flow_in, flow_pitch, fyi) + fxi * 2;
+        fa[k] = f[0];
+        fb[k] = f[1];
+    }
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int i = tid + k * NT;
+        const int gx = gxs[k], gy = gys[k];
+        float x = div_n((float)gx + 0.5f + fa[k], iw);
+        float y = div_n((float)gy + 0.5f + fb[k], ih);
+        // tex_linear<1> (mirror) without its texel-centre shortcut: same value for finite data
+        float xf = x * (float)w - 0.5f, yf = y * (float)h - 0.5f;
+        float fx = floorf(xf), fy = floorf(yf);
+        float a = tex_frac(xf - fx, quant), b = tex_frac(yf - fy, quant);
+        int xa = tex_addr((int)fx, w, ISA_ADDR_MIRROR), xb = tex_addr((int)fx + 1, w, ISA_ADDR_MIRROR);
+        int ya = tex_addr((int)fy, h, ISA_ADDR_MIRROR), yb = tex_addr((int)fy + 1, h, ISA_ADDR_MIRROR);
+        const float* ra = row_ptr(src, img_pitch, ya);
+        const float* rb = row_ptr(src, img_pitch, yb);
+        float t00 = ra[xa], t10 = ra[xb], t01 = rb[xa], t11 = rb[xb];
+        float oma = 1.0f - a, omb = 1.0f - b;
+        float val = oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11;
+        float tv = row_ptr(tgt, img_pitch, gy)[gx];
+        if (i < WR * WC) {
+            const int ly = i / WC, lx = i - ly * WC;
+            Lw[ly][lx] = val;
+            Lt[ly][lx] = tv;
+        }
+    }
+    __syncthreads();
+    // B. derivatives
+    for (int i = tid; i < DH * DW; i += NT) {
+        int ly = i / DW, lx = i - ly * DW;
+        int gx = x0 - HW + lx, gy = y0 - HW + ly;
+        if (gx < 0 || gx >= w || gy < 0 || gy >= h) {
+            Ld[0][ly][lx] = 0.0f;
+            Ld[1][ly][lx] = 0.0f;
+            Ld[2][ly][lx] = 0.0f;
+            continue;
+        }
+        const LTap* c = Tc[lx];
+        const LTap* r = Tr[ly];
+        if (!s_mixed) {
+            // every tap lands on one texel: the stencils are plain texel differences
+            const int r2 = r[2].p, c2 = c[2].p;
+            auto px5 = [&](const float (*L)[WC]) {
+                float t = L[r2][c[4].p];
+                t -= L[r2][c[3].p] * 8.0f;
+                t += L[r2][c[1].p] * 8.0f;
+                t -= L[r2][c[0].p];
+                t = div_n(t, 1.0 / 12.0);
+                return t;
+            };
+            auto py5 = [&](const float (*L)[WC]) {
+                float t = L[r[4].p][c2];
+                t -= L[r[3].p][c2] * 8.0f;
+                t += L[r[1].p][c2] * 8.0f;
+                t -= L[r[0].p][c2];
+                t = div_n(t, 1.0 / 12.0);
+                return t;
+            };
+            float t0 = px5(Lw), t1 = px5(Lt);
+            Ld[0][ly][lx] = (t0 + t1) * 0.5f;
+            Ld[2][ly][lx] = Lw[r2][c2] - Lt[r2][c2];
+            t0 = py5(Lw);
+            t1 = py5(Lt);
+            Ld[1][ly][lx] = (t0 + t1) * 0.5f;
+            continue;
+        }
+        auto sx = [&](const float (*L)[WC]) {
+            float t = lds_fetch_w<WC>(L, c[4], r[2]);
+            t -= lds_fetch_w<WC>(L, c[3], r[2]) * 8.0f;
+            t += lds_fetch_w<WC>(L, c[1], r[2]) * 8.0f;
+            t -= lds_fetch_w<WC>(L, c[0], r[2]);
+            t = div_n(t, 1.0 / 12.0);
+            return t;
+        };
+        auto sy = [&](const float (*L)[WC]) {
+            float t = lds_fetch_w<WC>(L, c[2], r[4]);
+            t -= lds_fetch_w<WC>(L, c[2], r[3]) * 8.0f;
+            t += lds_fetch_w<WC>(L, c[2], r[1]) * 8.0f;
+            t -= lds_fetch_w<WC>(L, c[2], r[0]);
+            t = div_n(t, 1.0 / 12.0);
+            return t;
+        };
+        float t0 = sx(Lw), t1 = sx(Lt);
+        Ld[0][ly][lx] = (t0 + t1) * 0.5f;
+        Ld[2][ly][lx] = lds_fetch_w<WC>(Lw, c[2], r[2]) - lds_fetch_w<WC>(Lt, c[2], r[2]);
+        t0 = sy(Lw);
+        t1 = sy(Lt);
+        Ld[1][ly][lx] = (t0 + t1) * 0.5f;
+    }
+    __syncthreads();
+    // C. window sums for LKF_RPT output rows of one column
+    const int lx = tid % LKF_BX, oy0 = (tid / LKF_BX) * LKF_RPT;
+    float V[LKF_RPT][5];
+#pragma unroll
+    for (int o = 0; o < LKF_RPT; ++o)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) V[o][q] = 0.0f;
+#pragma unroll 1
+    for (int rr = 0; rr < LKF_RPT + 2 * HW; ++rr) {
+        const int ry = oy0 + rr;
+        float H[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int k = 0; k <= 2 * HW; ++k) {
+            float dx = Ld[0][ry][lx + k], dy = Ld[1][ry][lx + k], dt = Ld[2][ry][lx + k];
+            H[0] += dx * dx;
+            H[1] += dx * dy;
+            H[2] += dy * dy;
+            H[3] += dx * dt;
+            H[4] += dy * dt;
+        }
+#pragma unroll
+        for (int o = 0; o < LKF_RPT; ++o)
+            if (rr >= o && rr <= o + 2 * HW)
+#pragma unroll
+                for (int q = 0; q < 5; ++q) V[o][q] += H[q];
+    }
+    // D. solve and update
+    const int px = x0 + lx;
+    if (px >= w) return;
+#pragma unroll
+    for (int o = 0; o < LKF_RPT; ++o) {
+        const int py = y0 + oy0 + o;
+        if (py >= h) break;
+        const float* fi = row_ptr(flow_in, flow_pitch, py) + px * 2;
+        float f0 = fi[0], f1 = fi[1];
+        float inv[4];
+        if (px >= HW && px < w - HW && py >= HW && py < h - HW && lk_pinv<false>(V[o][0], V[o][1], V[o][2], min_det, inv)) {
+            float u0 = inv[0] * V[o][3] + inv[1] * V[o][4];
+            float u1 = inv[2] * V[o][3] + inv[3] * V[o][4];
+            u0 = isnan(u0) ? 0.0f : u0;
+            u1 = isnan(u1) ? 0.0f : u1;
+            f0 += u0;
+            f1 += u1;
+        }
+        float* fo = row_ptr(flow_out, flow_pitch, py) + px * 2;
+        fo[0] = f0;
+        fo[1] = f1;
+    }
+}
+
 }  // namespace
 
 namespace isa {
@@ -306,8 +565,8 @@ int launch_warp(const float* src, int src_pitch, float* out, int out_pitch, cons
 int launch_derivatives(const float* src, const float* tgt, int img_pitch, float* ix, float* iy, float* iz, int pitch,
                        int w, int h, const void* taps, hipStream_t s) {
     const Tap* col = (const Tap*)taps;
-    k_derivatives_tab<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(src, tgt, img_pitch, ix, iy, iz, pitch,
-                                                                                 w, h, col, col + (size_t)w * 5);
+    k_derivatives_tab<<<dim3(div_up(w, DV_BX), div_up(h, DV_BY)), dim3(DV_BX, DV_BY), 0, s>>>(
+        src, tgt, img_pitch, ix, iy, iz, pitch, w, h, col, col + (size_t)w * 5);
     return check_launch("derivatives");
 }
 
@@ -351,6 +610,30 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
     float* iy = work + 2 * np;
     float* iz = work + 3 * np;
     const int wp = w * (int)sizeof(float);
+    const int hw = window / 2;
+    if (box && hw >= 1 && hw <= 8 && iterations > 0 && flow_pitch == 2 * wp) {
+        // fused iterations, ping-ponging between `flow` and the work area so the last lands in `flow`
+        float* bufs[2] = {flow, work};
+        int cur = iterations % 2;  // buffer holding the initial field
+        ISA_TRY(launch_flow_from_tiles(bufs[cur], flow_pitch, tiles, tiles_pitch, tw, th, w, h, bsx, bsy, rot, quant,
+                                       s));
+        const Tap* col = (const Tap*)taps;
+        const dim3 grid(div_up(w, LKF_BX), div_up(h, LKF_BY)), block(LKF_BX * (LKF_BY / LKF_RPT));
+        for (int it = 0; it < iterations; ++it) {
+            const float* fi = bufs[cur];
+            float* fo = bufs[cur ^ 1];
+#define ISA_LKF(H)                                                                                                   \
+    case H:                                                                                                        \
+        k_lk_fused<H><<<grid, block, 0, s>>>(src, tgt, img_pitch, fi, fo, flow_pitch, w, h, min_det, quant, col,   \
+                                             col + (size_t)w * 5);                                                 \
+        break;
+            switch (hw) { ISA_LKF(1) ISA_LKF(2) ISA_LKF(3) ISA_LKF(4) ISA_LKF(5) ISA_LKF(6) ISA_LKF(7) ISA_LKF(8) }
+#undef ISA_LKF
+            ISA_TRY(check_launch("lucas_kanade_fused"));
+            cur ^= 1;
+        }
+        return ISA_OK;
+    }
     ISA_TRY(launch_flow_from_tiles(flow, flow_pitch, tiles, tiles_pitch, tw, th, w, h, bsx, bsy, rot, quant, s));
     for (int it = 0; it < iterations; ++it) {
         ISA_TRY(launch_warp(src, img_pitch, tmp, wp, flow, flow_pitch, w, h, quant, s));

@@ -34,12 +34,13 @@ __global__ void k_robustness(const float* ref, const float* mov, int img_pitch, 
                              float thr_m, int quant) {
     int px = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y * blockDim.y + threadIdx.y;
     if (px >= w - 1 || py >= h - 1 || px < 1 || py < 1) return;
-    float u = ((float)px + 0.5f) / (float)w, v = ((float)py + 0.5f) / (float)h;
+    const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
+    float u = div_n((float)px + 0.5f, iw), v = div_n((float)py + 0.5f, ih);
     float f[2];
     tex_linear_all<2>(flow, flow_pitch, fw, fh, u, v, ISA_ADDR_MIRROR, quant, f);
     // The reference's 5x5 min/max loop overwrites its result each pass (R:Kernels/RobustnessModell.cu:62-72),
     // so only the (+2, +2) sample determines M.
-    float su = ((float)px + 2.0f + 0.5f) / (float)w, sv = ((float)py + 2.0f + 0.5f) / (float)h;
+    float su = div_n((float)px + 2.0f + 0.5f, iw), sv = div_n((float)py + 2.0f + 0.5f, ih);
     float sm[2];
     tex_linear_all<2>(flow, flow_pitch, fw, fh, su, sv, ISA_ADDR_MIRROR, quant, sm);
     float maxx = fmaxf(sm[0], f[0]), maxy = fmaxf(sm[1], f[1]);
@@ -201,8 +202,8 @@ __global__ void k_accumulate_sr(const uint16_t* __restrict__ raw, float* sum, fl
     float* ps = row_ptr(sum, out_pitch, y) + x * 3;
     float* pw = row_ptr(wsum, out_pitch, y) + x * 3;
     float acc[3] = {ps[0], ps[1], ps[2]}, ws[3] = {pw[0], pw[1], pw[2]};
-    float posx = ((float)x + 0.5f + (float)(w / 2)) / 2.0f / (float)w;
-    float posy = ((float)y + 0.5f + (float)(h / 2)) / 2.0f / (float)h;
+    float posx = div_n(((float)x + 0.5f + (float)(w / 2)) / 2.0f, 1.0 / (double)w);
+    float posy = div_n(((float)y + 0.5f + (float)(h / 2)) / 2.0f, 1.0 / (double)h);
     float k[4], f[2];
     tex_linear_all<4>(kern4, kern_pitch, w, h, posx, posy, ISA_ADDR_CLAMP, quant, k);
     tex_linear_all<2>(flow, flow_pitch, w, h, posx, posy, ISA_ADDR_MIRROR, quant, f);

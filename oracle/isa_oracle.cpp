@@ -380,12 +380,16 @@ void tiles_preshift(int w, int h, const T* img, T* tiles, const T* preshift /*tc
 // through cuFFT (R:PEFStudioDX/PatchTracker.cs:327-348; R:Kernels/kernel.cu:27-169).
 // Because the reference tile is zero outside its T x T interior and |s| <= M,
 // the circular correlation never wraps, so every term is a plain sum over the
-// interior. Evaluated in double in a canonical order:
-//   R2[y] = sum_x r^2, TT[j][i] = sum_x t[j][x+i]^2, C[y][s] = sum_x r t(.+s)
-//   (each an fma chain in increasing x), Q[y][s] = fma(-2, C, TT[y+sy+M][sx+M]),
-//   D(s) = tree_y(R2) + tree_y(Q[.][s]) with the pairwise tree
-//   for stride = 1, 2, 4, ...: v[y] += v[y + stride] for y a multiple of 2 stride.
-// Output per tile: (2M+1)^2 values, index (sy+M)*(2M+1) + (sx+M).
+// interior. Evaluated in double in a canonical, column-wise order (r(y, x) is
+// the interior, t the full (T+2M)^2 block, i = s + M):
+//   R2[x]     = sum_y r(y,x)^2                       (fma chain in increasing y)
+//   TT[x'][0] = sum_{y<T} t[y][x']^2                 (fma chain in increasing y)
+//   TT[x'][k+1] = (TT[x'][k] - t[k][x']^2) + t[k+T][x']^2   (squares exact in double)
+//   C[x][i]   = sum_y r(y,x) t[y+iy][x+ix]            (fma chain in increasing y)
+//   Q[x][i]   = fma(-2, C[x][i], TT[x+ix][iy])
+//   D(s)      = tree_x(R2) + tree_x(Q[.][i]), with the pairwise tree
+//   for stride = 1, 2, 4, ...: v[x] += v[x + stride] for x a multiple of 2 stride.
+// Output per tile: (2M+1)^2 values, index iy*(2M+1) + ix.
 static void tree_sum(std::vector<double>& v, int n, int stride_elems, int count) {
     // v holds n rows of `count` values each (row stride stride_elems); sums rows into row 0
     for (int st = 1; st < n; st *= 2)
@@ -403,23 +407,26 @@ void tile_ssd(const T* ref_tiles, const T* trk_tiles, T* out, int M, int tile, i
             const T* k = trk_tiles + (size_t)t * B * B;
             auto rv = [&](int y, int x) { return (double)r[(M + y) * B + M + x]; };
             auto tv = [&](int j, int x) { return (double)k[j * B + x]; };
-            for (int y = 0; y < tile; ++y) {
+            for (int x = 0; x < tile; ++x) {
                 double acc = 0;
-                for (int x = 0; x < tile; ++x) acc = std::fma(rv(y, x), rv(y, x), acc);
-                R2[y] = acc;
+                for (int y = 0; y < tile; ++y) acc = std::fma(rv(y, x), rv(y, x), acc);
+                R2[x] = acc;
             }
-            for (int j = 0; j < B; ++j)
-                for (int i = 0; i < S; ++i) {
-                    double acc = 0;
-                    for (int x = 0; x < tile; ++x) acc = std::fma(tv(j, x + i), tv(j, x + i), acc);
-                    TT[(size_t)j * S + i] = acc;
+            for (int xc = 0; xc < B; ++xc) {
+                double acc = 0;
+                for (int y = 0; y < tile; ++y) acc = std::fma(tv(y, xc), tv(y, xc), acc);
+                TT[(size_t)xc * S] = acc;
+                for (int q = 0; q + 1 < S; ++q) {
+                    acc = (acc - tv(q, xc) * tv(q, xc)) + tv(q + tile, xc) * tv(q + tile, xc);
+                    TT[(size_t)xc * S + q + 1] = acc;
                 }
-            for (int y = 0; y < tile; ++y)
+            }
+            for (int x = 0; x < tile; ++x)
                 for (int iy = 0; iy < S; ++iy)
                     for (int ix = 0; ix < S; ++ix) {
                         double acc = 0;
-                        for (int x = 0; x < tile; ++x) acc = std::fma(rv(y, x), tv(y + iy, x + ix), acc);
-                        Q[((size_t)y * S + iy) * S + ix] = std::fma(-2.0, acc, TT[(size_t)(y + iy) * S + ix]);
+                        for (int y = 0; y < tile; ++y) acc = std::fma(rv(y, x), tv(y + iy, x + ix), acc);
+                        Q[((size_t)x * S + iy) * S + ix] = std::fma(-2.0, acc, TT[(size_t)(x + ix) * S + iy]);
                     }
             tree_sum(R2, tile, 1, 1);
             tree_sum(Q, tile, S * S, S * S);

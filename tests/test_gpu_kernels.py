@@ -122,22 +122,34 @@ def _track_inputs(rng, h=80, w=112):
     return a, b.astype(np.float32)
 
 
-@pytest.mark.parametrize("tile,M", [(16, 2), (16, 4), (32, 3)])
-def test_tiles_ssd_findmin(isa, oracle, gpu, tile, M):
-    rng = np.random.default_rng(7 + M)
-    h, w = 80, 112
+# (tile, M, base shift, rotation, image h, w). Tiles dividing 64 with M in {1, 2, 4} take the
+# row-parallel tile-search kernel, the others the block-parallel one; both must give the same bits.
+TRACK_CASES = [
+    (16, 2, (1.25, -0.5), 0.0, 80, 112),
+    (16, 4, (1.25, -0.5), 0.0, 80, 112),
+    (32, 3, (1.25, -0.5), 0.0, 80, 112),
+    (64, 2, (6.5, -3.0), 0.01, 150, 212),   # tiles cut by the image edge (clamped reference rows)
+    (32, 4, (-5.0, 4.5), -0.02, 120, 170),
+    (8, 1, (0.0, 0.0), 0.0, 60, 70),
+    (48, 2, (2.0, 1.0), 0.0, 110, 150),     # 64 % 48 != 0: block-parallel form
+]
+
+
+@pytest.mark.parametrize("tile,M,bs,rot,h,w", TRACK_CASES)
+def test_tiles_ssd_findmin(isa, oracle, gpu, tile, M, bs, rot, h, w):
+    rng = np.random.default_rng(7 + M + tile)
     a, b = _track_inputs(rng, h, w)
     cx, cy = (w - 2 * M) // tile, (h - 2 * M) // tile
     B, S = tile + 2 * M, 2 * M + 1
     pre = rng.uniform(-3, 3, (cy, cx, 2)).astype(np.float32)
-    bs = (1.25, -0.5)
+    print(f"track case tile={tile} M={M} tiles={cx}x{cy}", flush=True)
     rt, tt = empty((cx * cy, B, B)), empty((cx * cy, B, B))
     isa.call("isa_convert_to_tiles_border", p(dev(a)), w * 4, p(rt), w, h, M, tile, cx, cy, isa.f32s(*bs),
-             C.c_float(0.0), stream())
+             C.c_float(rot), stream())
     isa.call("isa_convert_to_tiles_preshift", p(dev(b)), w * 4, p(tt), p(dev(pre)), cx * 8, w, h, M, tile, cx, cy,
-             isa.f32s(*bs), C.c_float(0.0), stream())
-    rt_ref = oracle.tiles_border(a, M, tile, cx, cy, bs)
-    tt_ref = oracle.tiles_preshift(b, pre, M, tile, cx, cy, bs)
+             isa.f32s(*bs), C.c_float(rot), stream())
+    rt_ref = oracle.tiles_border(a, M, tile, cx, cy, bs, rot)
+    tt_ref = oracle.tiles_preshift(b, pre, M, tile, cx, cy, bs, rot)
     np.testing.assert_array_equal(host(rt), rt_ref)
     np.testing.assert_array_equal(host(tt), tt_ref)
     simg = empty((cx * cy, S, S))
@@ -149,8 +161,8 @@ def test_tiles_ssd_findmin(isa, oracle, gpu, tile, M):
     np.testing.assert_array_equal(host(coords).reshape(-1, 2), oracle.find_minimum(s_ref, M, 1.0))
     # fused PatchTracker.Track
     d_pre = dev(pre)
-    isa.call("isa_track", p(dev(b)), p(dev(a)), w * 4, p(d_pre), cx * 8, w, h, tile, M, isa.f32s(*bs), C.c_float(0),
-             isa.f32s(*bs), C.c_float(0), C.c_float(1.0), stream())
+    isa.call("isa_track", p(dev(b)), p(dev(a)), w * 4, p(d_pre), cx * 8, w, h, tile, M, isa.f32s(*bs),
+             C.c_float(rot), isa.f32s(*bs), C.c_float(rot), C.c_float(1.0), stream())
     np.testing.assert_array_equal(host(d_pre).reshape(-1, 2), pre.reshape(-1, 2) + oracle.find_minimum(s_ref, M, 1.0))
 
 
