@@ -328,91 +328,90 @@ __global__ __launch_bounds__(256) void k_track_cols(TrackArgs a, int pairs, int 
     const int tx0 = tx * T + tox, ty0 = ty * T + toy;          // track block
     auto rowp = [&](const float* img, int gy) { return row_ptr(img, a.img_pitch, clampi(gy, 0, a.h - 1)); };
     auto colc = [&](int gx) { return clampi(gx, 0, a.w - 1); };
-    // 1. window energies of the track columns, one column per task
-    for (int c = lane; c < B; c += lpi) {
-        const int gx = colc(tx0 + c);
-        double acc = 0.0;
-        for (int y = 0; y < T; ++y) {
-            double v = (double)rowp(ib, ty0 + y)[gx];
-            acc = fma(v, v, acc);
-        }
-        TT[c * S] = acc;
-        for (int q = 0; q + 1 < S; ++q) {
-            double u = (double)rowp(ib, ty0 + q)[gx], v = (double)rowp(ib, ty0 + q + T)[gx];
-            acc = (acc - u * u) + v * v;
-            TT[c * S + q + 1] = acc;
-        }
-    }
-    // 2. cross terms and reference column energy
+    // 1. cross terms, reference column energy and the track column energies, in one pass over y.
+    // Lane (g, x) keeps, besides its RPG x S cross terms, the energy chains of its first window
+    // column x + g*RPG and of its last one x + S - 1 (the latter is needed only from the lanes of
+    // the last group with x > T - RPG, which own the track columns no first column reaches).
+    const bool inside = rx0 >= 0 && rx0 + T <= a.w && ry0 >= 0 && ry0 + T <= a.h && tx0 >= 0 && tx0 + B <= a.w &&
+                        ty0 >= 0 && ty0 + B <= a.h;
+    const size_t pitchf = (size_t)a.img_pitch / sizeof(float);
+    const int rgx = colc(rx0 + x);
+    int tgx[RPG];
+#pragma unroll
+    for (int q = 0; q < RPG; ++q) tgx[q] = colc(tx0 + x + g * RPG + q);
+    // row pointers: interior tiles index directly, boundary tiles clamp the row
+    auto rrow = [&](int y) { return inside ? ia + (size_t)(ry0 + y) * pitchf : rowp(ia, ry0 + y); };
+    auto trow = [&](int j) { return inside ? ib + (size_t)(ty0 + j) * pitchf : rowp(ib, ty0 + j); };
     double acc[RPG][S], win[RPG][S];
-    double r2 = 0.0;
+    double r2 = 0.0, tt0 = 0.0, tt1 = 0.0;
 #pragma unroll
     for (int q = 0; q < RPG; ++q)
 #pragma unroll
         for (int i = 0; i < S; ++i) acc[q][i] = 0.0;
-    const bool inside = rx0 >= 0 && rx0 + T <= a.w && ry0 >= 0 && ry0 + T <= a.h && tx0 >= 0 && tx0 + B <= a.w &&
-                        ty0 >= 0 && ty0 + B <= a.h;
-    if (inside) {
-        // interior tile: no clamping, row pointers advance by one pitch per step
-        typedef const __attribute__((address_space(1))) float* gptr;
-        const size_t pitch = (size_t)a.img_pitch / sizeof(float);
-        gptr rp = (gptr)(ia + (size_t)ry0 * pitch + rx0 + x);
-        gptr tp = (gptr)(ib + (size_t)ty0 * pitch + tx0 + x + g * RPG);
 #pragma unroll
-        for (int i = 0; i < S - 1; ++i)
+    for (int i = 0; i < S - 1; ++i) {
+        const float* tr = trow(i);
 #pragma unroll
-            for (int q = 0; q < RPG; ++q) win[q][i] = (double)tp[i * pitch + q];
-        tp += (S - 1) * pitch;
-        auto step = [&](int kk) {
-            const double rv = (double)rp[0];
-            r2 = fma(rv, rv, r2);
+        for (int q = 0; q < RPG; ++q) win[q][i] = (double)tr[tgx[q]];
+    }
+    // operands of S steps are fetched as one block
+    float nr[S], nt[S][RPG];
+    auto fetch = [&](int y0s) {
 #pragma unroll
-            for (int q = 0; q < RPG; ++q) win[q][(kk + S - 1) % S] = (double)tp[q];
-            rp += pitch;
-            tp += pitch;
+        for (int kk = 0; kk < S; ++kk) {
+            const int y = min(y0s + kk, T - 1);  // past the end: a valid address, value unused
+            nr[kk] = rrow(y)[rgx];
+            const float* tr = trow(y + S - 1);
 #pragma unroll
-            for (int q = 0; q < RPG; ++q)
-#pragma unroll
-                for (int i = 0; i < S; ++i) acc[q][i] = fma(rv, win[q][(kk + i) % S], acc[q][i]);
-        };
-        int y = 0;
-        for (; y + S <= T; y += S) {
-#pragma unroll
-            for (int kk = 0; kk < S; ++kk) step(kk);
+            for (int q = 0; q < RPG; ++q) nt[kk][q] = tr[tgx[q]];
         }
+    };
+    // (one block ahead for M <= 2; for M = 4 the extra registers would halve the occupancy)
+    constexpr bool kAhead = M <= 2;
+    if (kAhead) fetch(0);
+    for (int y = 0; y < T; y += S) {
+        float cr[S], ct[S][RPG];
+        if (!kAhead) fetch(y);
 #pragma unroll
-        for (int kk = 0; kk < S; ++kk)
-            if (kk < T - y) step(kk);
-    } else {
-        const int rgx = colc(rx0 + x);
-        int tgx[RPG];
+        for (int kk = 0; kk < S; ++kk) {
+            cr[kk] = nr[kk];
 #pragma unroll
-        for (int q = 0; q < RPG; ++q) tgx[q] = colc(tx0 + x + g * RPG + q);
-#pragma unroll
-        for (int i = 0; i < S - 1; ++i) {
-            const float* tr = rowp(ib, ty0 + i);
-#pragma unroll
-            for (int q = 0; q < RPG; ++q) win[q][i] = (double)tr[tgx[q]];
+            for (int q = 0; q < RPG; ++q) ct[kk][q] = nt[kk][q];
         }
-        auto step = [&](int y, int kk) {
-            const double rv = (double)rowp(ia, ry0 + y)[rgx];
-            const float* tr = rowp(ib, ty0 + y + S - 1);
-            r2 = fma(rv, rv, r2);
+        if (kAhead && y + S < T) fetch(y + S);
 #pragma unroll
-            for (int q = 0; q < RPG; ++q) win[q][(kk + S - 1) % S] = (double)tr[tgx[q]];
+        for (int kk = 0; kk < S; ++kk) {
+            if (y + kk < T) {
+                const double rv = (double)cr[kk];
+                r2 = fma(rv, rv, r2);
 #pragma unroll
-            for (int q = 0; q < RPG; ++q)
+                for (int q = 0; q < RPG; ++q) win[q][(kk + S - 1) % S] = (double)ct[kk][q];
+                tt0 = fma(win[0][kk], win[0][kk], tt0);
+                tt1 = fma(win[RPG - 1][kk], win[RPG - 1][kk], tt1);
 #pragma unroll
-                for (int i = 0; i < S; ++i) acc[q][i] = fma(rv, win[q][(kk + i) % S], acc[q][i]);
-        };
-        int y = 0;
-        for (; y + S <= T; y += S) {
+                for (int q = 0; q < RPG; ++q)
 #pragma unroll
-            for (int kk = 0; kk < S; ++kk) step(y + kk, kk);
+                    for (int i = 0; i < S; ++i) acc[q][i] = fma(rv, win[q][(kk + i) % S], acc[q][i]);
+            }
         }
-#pragma unroll
-        for (int kk = 0; kk < S; ++kk)
-            if (kk < T - y) step(y + kk, kk);
+    }
+    // window energies of the other row offsets: TT[c][k+1] = (TT[c][k] - t[k][c]^2) + t[k+T][c]^2
+    {
+        const int c0 = x + g * RPG, c1 = x + S - 1;
+        const bool own1 = g == G - 1 && x > T - RPG;
+        const int gx0 = tgx[0], gx1 = tgx[RPG - 1];
+        TT[c0 * S] = tt0;
+        if (own1) TT[c1 * S] = tt1;
+        for (int q = 0; q + 1 < S; ++q) {
+            const float* ta = trow(q);
+            const float* tb = trow(q + T);
+            double u0 = (double)ta[gx0], v0 = (double)tb[gx0];
+            double u1 = (double)ta[gx1], v1 = (double)tb[gx1];
+            tt0 = (tt0 - u0 * u0) + v0 * v0;
+            tt1 = (tt1 - u1 * u1) + v1 * v1;
+            TT[c0 * S + q + 1] = tt0;
+            if (own1) TT[c1 * S + q + 1] = tt1;
+        }
     }
     __syncthreads();  // TT complete
 #pragma unroll

@@ -1,4 +1,6 @@
 // Merge kernels: half-resolution debayer, robustness mask, kernel accumulation.
+#include <cmath>
+
 #include "isa_device.h"
 #include "isa_host.h"
 
@@ -104,32 +106,101 @@ __global__ void k_robustness(const float* ref, const float* mov, int img_pitch, 
 struct AccArgs {
     Cfa cfa;
     float white[3], black[3];
+    double inv_white[3];
+    int white_int;  // every white level is an integer in [1, 2^24]: divide through div_n
 };
 
+__device__ __forceinline__ int site_channel(int site) { return (site >= 0 && site <= 2) ? site : -1; }
+
+// (v - black) / white; DIVN when every white level is an integer (div_n is then exact)
+template <bool DIVN> __device__ __forceinline__ float norm_raw(float v, int c, const AccArgs& a) {
+    float d = v - a.black[c];
+    return DIVN ? div_n(d, a.inv_white[c]) : d / a.white[c];
+}
+
 // One 5x5 gather of one frame at output pixel (x, y): integer sample shift (sx, sy), weights w[25].
-__device__ __forceinline__ void gather1x(const uint16_t* __restrict__ raw, const float* __restrict__ mask,
-                                         int mask_pitch, int x, int y, int sx, int sy, int w, int h, const float* wt,
-                                         const AccArgs& a, float acc[3], float ws[3]) {
+// Summation order: the taps are visited class by class, class = ((py & 1) << 1) | (px & 1) for
+// class 0..3, in raster order within a class; each tap adds to the sums of its (clamped, shifted)
+// site's channel. All taps of a class share the site parity, so away from the image edge a class
+// has one channel and its taps run on one accumulator pair with no per-tap channel selection.
+template <int CLS, bool DIVN>
+__device__ __forceinline__ void gather_class_inside(const uint16_t* __restrict__ rs, const float* __restrict__ mask,
+                                                    int mask_pitch, int x, int y, int w, const float* wt,
+                                                    const AccArgs& a, int ch, float acc[3], float ws[3]) {
+    if (ch < 0) return;
+    typedef const __attribute__((address_space(1))) uint16_t* graw;
+    typedef const __attribute__((address_space(1))) float* gmask;
+    float A = ch == 0 ? acc[0] : (ch == 1 ? acc[1] : acc[2]);
+    float W = ch == 0 ? ws[0] : (ch == 1 ? ws[1] : ws[2]);
 #pragma unroll
-    for (int py = -2; py <= 2; ++py)
+    for (int py = -2; py <= 2; ++py) {
+        if (((py & 1) << 1) != (CLS & 2)) continue;
+        graw rrow = (graw)(rs + (ptrdiff_t)py * w);
+        gmask mrow = (gmask)row_ptr(mask, mask_pitch, (y + py) >> 1);
 #pragma unroll
         for (int px = -2; px <= 2; ++px) {
+            if ((px & 1) != (CLS & 1)) continue;
+            const float wgt = wt[(py + 2) * 5 + px + 2];
+            const float v = norm_raw<DIVN>((float)rrow[px], ch, a);
+            float cert = mrow[((x + px) >> 1) * 4 + ch];
+            if (!isfinite(cert)) cert = 0.0f;
+            A += v * wgt * cert;
+            W += wgt * cert;
+        }
+    }
+    if (ch == 0) { acc[0] = A; ws[0] = W; }
+    else if (ch == 1) { acc[1] = A; ws[1] = W; }
+    else { acc[2] = A; ws[2] = W; }
+}
+
+template <int CLS, bool DIVN>
+__device__ __forceinline__ void gather_class_edge(const uint16_t* __restrict__ raw, const float* __restrict__ mask,
+                                                  int mask_pitch, int x, int y, int sx, int sy, int w, int h,
+                                                  const float* wt, const AccArgs& a, float acc[3], float ws[3]) {
+#pragma unroll
+    for (int py = -2; py <= 2; ++py) {
+        if (((py & 1) << 1) != (CLS & 2)) continue;
+#pragma unroll
+        for (int px = -2; px <= 2; ++px) {
+            if ((px & 1) != (CLS & 1)) continue;
             int ppsx = clampi(x + px + sx, 0, w - 1), ppsy = clampi(y + py + sy, 0, h - 1);
             int ppx = clampi(x + px, 0, w - 1), ppy = clampi(y + py, 0, h - 1);
-            int site = a.cfa.at(ppsy, ppsx);
-            int c = site == ISA_CFA_GREEN ? 1 : site == ISA_CFA_RED ? 0 : site == ISA_CFA_BLUE ? 2 : -1;
+            int c = site_channel(a.cfa.at(ppsy, ppsx));
             if (c < 0) continue;
             float wgt = wt[(py + 2) * 5 + px + 2];
-            float v = (float)raw[(size_t)ppsy * w + ppsx];
-            v = (v - a.black[c]) / a.white[c];
+            float v = norm_raw<DIVN>((float)raw[(size_t)ppsy * w + ppsx], c, a);
             float cert = row_ptr(mask, mask_pitch, ppy / 2)[(ppx / 2) * 4 + c];
             if (!isfinite(cert)) cert = 0.0f;
             acc[c] += v * wgt * cert;
             ws[c] += wgt * cert;
         }
+    }
+}
+
+template <bool DIVN>
+__device__ __forceinline__ void gather1x(const uint16_t* __restrict__ raw, const float* __restrict__ mask,
+                                         int mask_pitch, int x, int y, int sx, int sy, int w, int h, const float* wt,
+                                         const AccArgs& a, float acc[3], float ws[3]) {
+    const int qx = x + sx, qy = y + sy;
+    if (qx >= 2 && qx <= w - 3 && qy >= 2 && qy <= h - 3 && x >= 2 && x <= w - 3 && y >= 2 && y <= h - 3) {
+        const uint16_t* rs = raw + (size_t)qy * w + qx;
+        // channel of each class: the site of its tap nearest the centre (px, py in {0, 1})
+        const int c0 = site_channel(a.cfa.at(qy, qx)), c1 = site_channel(a.cfa.at(qy, qx + 1));
+        const int c2 = site_channel(a.cfa.at(qy + 1, qx)), c3 = site_channel(a.cfa.at(qy + 1, qx + 1));
+        gather_class_inside<0, DIVN>(rs, mask, mask_pitch, x, y, w, wt, a, c0, acc, ws);
+        gather_class_inside<1, DIVN>(rs, mask, mask_pitch, x, y, w, wt, a, c1, acc, ws);
+        gather_class_inside<2, DIVN>(rs, mask, mask_pitch, x, y, w, wt, a, c2, acc, ws);
+        gather_class_inside<3, DIVN>(rs, mask, mask_pitch, x, y, w, wt, a, c3, acc, ws);
+    } else {
+        gather_class_edge<0, DIVN>(raw, mask, mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+        gather_class_edge<1, DIVN>(raw, mask, mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+        gather_class_edge<2, DIVN>(raw, mask, mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+        gather_class_edge<3, DIVN>(raw, mask, mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+    }
 }
 
 // accumulateImages (R:Kernels/DeBayerKernels.cu:288-376).
+template <bool DIVN>
 __global__ void k_accumulate(const uint16_t* __restrict__ raw, float* sum, float* wsum, const float* mask,
                              const float* kern, const float* flow, int w, int h, int out_pitch, int mask_pitch,
                              int flow_pitch, AccArgs a) {
@@ -146,7 +217,7 @@ __global__ void k_accumulate(const uint16_t* __restrict__ raw, float* sum, float
 #pragma unroll
         for (int px = -2; px <= 2; ++px) wt[(py + 2) * 5 + px + 2] = kernel_weight(px, py, k[0], k[1], k[2]);
     int sx = (int)roundf(f[0]), sy = (int)roundf(f[1]);
-    gather1x(raw, mask, mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+    gather1x<DIVN>(raw, mask, mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         ps[c] = acc[c];
@@ -164,6 +235,7 @@ struct FramesArgs {
     int nframes;
 };
 
+template <bool DIVN>
 __global__ void k_accumulate_frames(float* sum, float* wsum, const float* kern, int w, int h, int out_pitch,
                                     int mask_pitch, int flow_pitch, FramesArgs fa, AccArgs a) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -184,7 +256,7 @@ __global__ void k_accumulate_frames(float* sum, float* wsum, const float* kern, 
             sx = (int)roundf(fl[0]);
             sy = (int)roundf(fl[1]);
         }
-        gather1x(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+        gather1x<DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -194,6 +266,7 @@ __global__ void k_accumulate_frames(float* sum, float* wsum, const float* kern, 
 }
 
 // accumulateImagesSuperRes (R:Kernels/DeBayerKernels.cu:378-468): w x h output of the central half crop.
+template <bool DIVN>
 __global__ void k_accumulate_sr(const uint16_t* __restrict__ raw, float* sum, float* wsum, const float* mask,
                                 const float* kern4, const float* flow, int w, int h, int out_pitch, int mask_pitch,
                                 int kern_pitch, int flow_pitch, AccArgs a, int quant) {
@@ -220,8 +293,7 @@ __global__ void k_accumulate_sr(const uint16_t* __restrict__ raw, float* sum, fl
             int c = site == ISA_CFA_GREEN ? 1 : site == ISA_CFA_RED ? 0 : site == ISA_CFA_BLUE ? 2 : -1;
             if (c < 0) continue;
             float wgt = kernel_weight(px, py, k[0], k[1], k[2]);
-            float v = (float)raw[(size_t)ppsy * w + ppsx];
-            v = (v - a.black[c]) / a.white[c];
+            float v = norm_raw<DIVN>((float)raw[(size_t)ppsy * w + ppsx], c, a);
             float cert = row_ptr(mask, mask_pitch, ppy / 2)[(ppx / 2) * 4 + c];
             if (!isfinite(cert)) cert = 0.0f;
             acc[c] += v * wgt * cert;
@@ -237,9 +309,12 @@ __global__ void k_accumulate_sr(const uint16_t* __restrict__ raw, float* sum, fl
 AccArgs make_acc(const float white[3], const float black[3], const int cfa[4]) {
     AccArgs a;
     for (int i = 0; i < 4; ++i) a.cfa.c[i] = cfa[i];
+    a.white_int = 1;
     for (int c = 0; c < 3; ++c) {
         a.white[c] = white[c];
         a.black[c] = black[c];
+        a.inv_white[c] = 1.0 / (double)white[c];
+        if (!(white[c] >= 1.0f && white[c] <= 16777216.0f && white[c] == std::floor(white[c]))) a.white_int = 0;
     }
     return a;
 }
@@ -268,9 +343,10 @@ int launch_robustness(const float* ref, const float* mov, int img_pitch, float* 
 int launch_accumulate(const uint16_t* raw, float* sum, float* wsum, const float* mask, const float* kern,
                       const float* flow, const float white[3], const float black[3], int w, int h, int out_pitch,
                       int mask_pitch, int flow_pitch, const int cfa[4], hipStream_t s) {
-    k_accumulate<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(raw, sum, wsum, mask, kern, flow, w, h,
-                                                                           out_pitch, mask_pitch, flow_pitch,
-                                                                           make_acc(white, black, cfa));
+    const AccArgs a = make_acc(white, black, cfa);
+    auto kern_fn = a.white_int ? k_accumulate<true> : k_accumulate<false>;
+    kern_fn<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(raw, sum, wsum, mask, kern, flow, w, h, out_pitch,
+                                                                      mask_pitch, flow_pitch, a);
     return check_launch("accumulate");
 }
 
@@ -279,17 +355,21 @@ int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, 
                              const float* const* flow, int nframes, const float white[3], const float black[3],
                              const int cfa[4], hipStream_t s) {
     FramesArgs fa{raw, mask, flow, nframes};
-    k_accumulate_frames<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(
-        sum, wsum, kern, w, h, out_pitch, mask_pitch, flow_pitch, fa, make_acc(white, black, cfa));
+    const AccArgs a = make_acc(white, black, cfa);
+    auto kern_fn = a.white_int ? k_accumulate_frames<true> : k_accumulate_frames<false>;
+    kern_fn<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(sum, wsum, kern, w, h, out_pitch, mask_pitch,
+                                                                      flow_pitch, fa, a);
     return check_launch("accumulate_frames");
 }
 
 int launch_accumulate_sr(const uint16_t* raw, float* sum, float* wsum, const float* mask, const float* kern4,
                          const float* flow, const float white[3], const float black[3], int w, int h, int out_pitch,
                          int mask_pitch, int kern_pitch, int flow_pitch, const int cfa[4], int quant, hipStream_t s) {
-    k_accumulate_sr<<<dim3(div_up(w, 16), div_up(h, 16)), dim3(16, 16), 0, s>>>(
-        raw, sum, wsum, mask, kern4, flow, w, h, out_pitch, mask_pitch, kern_pitch, flow_pitch,
-        make_acc(white, black, cfa), quant);
+    const AccArgs a = make_acc(white, black, cfa);
+    auto kern_fn = a.white_int ? k_accumulate_sr<true> : k_accumulate_sr<false>;
+    kern_fn<<<dim3(div_up(w, 16), div_up(h, 16)), dim3(16, 16), 0, s>>>(raw, sum, wsum, mask, kern4, flow, w, h,
+                                                                        out_pitch, mask_pitch, kern_pitch, flow_pitch, a,
+                                                                        quant);
     return check_launch("accumulate_superres");
 }
 

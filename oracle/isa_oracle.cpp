@@ -1077,10 +1077,29 @@ template <class T> static inline T kernel_weight(int px, int py, T k0, T k1, T k
     return w;
 }
 
+// The 1x gather visits the 25 taps class by class, class = ((py & 1) << 1) | (px & 1) = 0..3, in
+// raster order within a class (the reference visits them in raster order; the two orders differ
+// only in how the taps of the two green classes interleave in the green sums).
+struct TapOrder {
+    int pxy[50];
+    TapOrder() {
+        int n = 0;
+        for (int cls = 0; cls < 4; ++cls)
+            for (int py = -2; py <= 2; ++py)
+                for (int px = -2; px <= 2; ++px)
+                    if ((((py & 1) << 1) | (px & 1)) == cls) { pxy[2 * n] = px; pxy[2 * n + 1] = py; ++n; }
+    }
+};
+static const int* tap_order_by_class() {
+    static const TapOrder order;  // thread-safe initialisation
+    return order.pxy;
+}
+
 template <class T>
 void accumulate(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern3, const T* flow, const T white[3],
                 const T black[3], int w, int h, const int cfa[4]) {
     int mw = w / 2;
+    const int* order = tap_order_by_class();
     parallel_for(h, [&](int y0, int y1) {
         for (int y = std::max(y0, 1); y < std::min(y1, h - 1); ++y)
             for (int x = 1; x < w - 1; ++x) {
@@ -1089,8 +1108,8 @@ void accumulate(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T*
                 T wt[3] = {wsum3[o * 3], wsum3[o * 3 + 1], wsum3[o * 3 + 2]};
                 T k0 = kern3[o * 3], k1 = kern3[o * 3 + 1], k2 = kern3[o * 3 + 2];
                 int sx = (int)std::round(flow[o * 2]), sy = (int)std::round(flow[o * 2 + 1]);
-                for (int py = -2; py <= 2; ++py)
-                    for (int px = -2; px <= 2; ++px) {
+                for (int t = 0; t < 25; ++t) {
+                        const int px = order[2 * t], py = order[2 * t + 1];
                         int ppsx = std::min(std::max(x + px + sx, 0), w - 1), ppsy = std::min(std::max(y + py + sy, 0), h - 1);
                         int ppx = std::min(std::max(x + px, 0), w - 1), ppy = std::min(std::max(y + py, 0), h - 1);
                         int site = cfa_at(cfa, ppsy, ppsx);
@@ -1103,7 +1122,7 @@ void accumulate(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T*
                         if (!std::isfinite(cert)) cert = 0;
                         acc[c] += v * wgt * cert;
                         wt[c] += wgt * cert;
-                    }
+                }
                 for (int c = 0; c < 3; ++c) { sum3[o * 3 + c] = acc[c]; wsum3[o * 3 + c] = wt[c]; }
             }
     });
