@@ -1,6 +1,7 @@
 // Tile alignment: tile extraction, SSD search, findMinimum, shift up-sampling,
 // the fused per-level tracking launch, and the FP64 ShiftMinimizer.
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <vector>
 
@@ -12,6 +13,19 @@ namespace {
 constexpr int kMaxShift = 8;  // largest supported MaxShift (search window (2M+1)^2 <= 289)
 
 // Integer offset of a tile (R:Kernels/kernel.cu:183-202 and :238-261): round(preshift + R(theta)(-b) + R(theta)c - c).
+// Tile offset with the rotation given as its (sin, cos) pair, each rounded once to float.
+__device__ __forceinline__ void tile_offset_sc(int tx, int ty, int tile, int w, int h, float bsx, float bsy, float sf,
+                                               float cf, float psx, float psy, int& ox, int& oy) {
+    float sx = psx + (cf * -bsx - sf * -bsy);
+    float sy = psy + (sf * -bsx + cf * -bsy);
+    float pcx = (float)(tx * tile + tile / 2 - w / 2);
+    float pcy = (float)(ty * tile + tile / 2 - h / 2);
+    sx += cf * pcx - sf * pcy - pcx;
+    sy += sf * pcx + cf * pcy - pcy;
+    ox = (int)roundf(sx);
+    oy = (int)roundf(sy);
+}
+
 __device__ __forceinline__ void tile_offset(int tx, int ty, int tile, int w, int h, float bsx, float bsy, float rot,
                                             float psx, float psy, int& ox, int& oy) {
     float sf = sin_r(rot), cf = cos_r(rot);
@@ -207,7 +221,7 @@ struct TrackArgs {
     const int* pa;  // reference frame of pair p
     const int* pb;  // tracked frame of pair p
     const int* pk;  // measurement index of pair p
-    const float* base;  // per frame: bsx, bsy, rot (already divided by the level)
+    const float* base;  // per frame: bsx, bsy (already divided by the level), sin(rot), cos(rot)
     float* shifts;
     size_t meas_stride;
     float threshold;
@@ -228,8 +242,10 @@ __global__ void k_track_level(TrackArgs a) {
     float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
     const float psx = sh[0], psy = sh[1];
     int rox, roy, tox, toy;
-    tile_offset(tx, ty, T, a.w, a.h, a.base[fa * 3], a.base[fa * 3 + 1], a.base[fa * 3 + 2], 0.0f, 0.0f, rox, roy);
-    tile_offset(tx, ty, T, a.w, a.h, a.base[fb * 3], a.base[fb * 3 + 1], a.base[fb * 3 + 2], psx, psy, tox, toy);
+    tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fa * 4], a.base[fa * 4 + 1], a.base[fa * 4 + 2], a.base[fa * 4 + 3],
+                   0.0f, 0.0f, rox, roy);
+    tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2], a.base[fb * 4 + 3],
+                   psx, psy, tox, toy);
     const float* ia = a.frames[fa];
     const float* ib = a.frames[fb];
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -315,8 +331,10 @@ __global__ __launch_bounds__(256) void k_track_cols(TrackArgs a, int pairs, int 
     float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
     const float psx = sh[0], psy = sh[1];
     int rox, roy, tox, toy;
-    tile_offset(tx, ty, T, a.w, a.h, a.base[fa * 3], a.base[fa * 3 + 1], a.base[fa * 3 + 2], 0.0f, 0.0f, rox, roy);
-    tile_offset(tx, ty, T, a.w, a.h, a.base[fb * 3], a.base[fb * 3 + 1], a.base[fb * 3 + 2], psx, psy, tox, toy);
+    tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fa * 4], a.base[fa * 4 + 1], a.base[fa * 4 + 2], a.base[fa * 4 + 3],
+                   0.0f, 0.0f, rox, roy);
+    tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2], a.base[fb * 4 + 3],
+                   psx, psy, tox, toy);
     const float* ia = a.frames[fa];
     const float* ib = a.frames[fb];
     char* mine = smem + (size_t)pp * item_bytes;
@@ -1012,10 +1030,11 @@ int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitc
     struct Scratch {
         const float* frames[2];
         int pa, pb, pk, pad;
-        float base[6];
+        float base[8];
     } h{{img_ref_dev, img_track_dev}, 0, 1, 0, 0,
-        {base_shift_ref[0], base_shift_ref[1], base_rotation_ref, base_shift_track[0], base_shift_track[1],
-         base_rotation_track}};
+        {base_shift_ref[0], base_shift_ref[1], (float)std::sin((double)base_rotation_ref),
+         (float)std::cos((double)base_rotation_ref), base_shift_track[0], base_shift_track[1],
+         (float)std::sin((double)base_rotation_track), (float)std::cos((double)base_rotation_track)}};
     const size_t head = (sizeof(Scratch) + 255) & ~(size_t)255;
     const size_t row = (size_t)cx * 2 * sizeof(float);
     char* d = nullptr;

@@ -113,7 +113,7 @@ struct isa_merger {
     void* hp_ws = nullptr;
     bool hp_tables = false;
     char* taps = nullptr;  // derivative tap tables (w, h, tex_quant), built at create
-    // device tables: pa, pb, pk (m each), rows (2m), base (L * n * 3 floats), frame pointers (L * n),
+    // device tables: pa, pb, pk (m each), rows (2m), base (L * n * 4 floats), frame pointers (L * n),
     // accumulate pointer arrays (raw n, mask n, flow n)
     char* tab = nullptr;
     char* tab_host = nullptr;  // pinned staging copy
@@ -161,7 +161,7 @@ TabLayout tab_layout(int m, int n, int L) {
     t.pb = o; o += al(m * sizeof(int));
     t.pk = o; o += al(m * sizeof(int));
     t.rows = o; o += al(2 * m * sizeof(int));
-    t.base = o; o += al((size_t)L * n * 3 * sizeof(float));
+    t.base = o; o += al((size_t)L * n * 4 * sizeof(float));
     t.frames = o; o += al((size_t)L * n * sizeof(void*));
     t.acc_raw = o; o += al(n * sizeof(void*));
     t.acc_mask = o; o += al(n * sizeof(void*));
@@ -355,9 +355,11 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         for (int f = 0; f < n; ++f) {
             // GetShift(frame) / level and GetRotation(frame) (R:...Controller.cs:1705-1708)
             float gx = st.sx[f] - st.sx[ref], gy = st.sy[f] - st.sy[ref];
-            base[((size_t)i * n + f) * 3 + 0] = gx / (float)l;
-            base[((size_t)i * n + f) * 3 + 1] = gy / (float)l;
-            base[((size_t)i * n + f) * 3 + 2] = store_rotation(st, ref, f);
+            const float rot = store_rotation(st, ref, f);
+            base[((size_t)i * n + f) * 4 + 0] = gx / (float)l;
+            base[((size_t)i * n + f) * 4 + 1] = gy / (float)l;
+            base[((size_t)i * n + f) * 4 + 2] = (float)std::sin((double)rot);
+            base[((size_t)i * n + f) * 4 + 3] = (float)std::cos((double)rot);
             fr[(size_t)i * n + f] = l > 1 ? M->lev[i] + (size_t)f * lsz : M->trk + (size_t)f * np;
         }
     }
@@ -424,7 +426,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         (void)hipEventRecord(M->lev_ev[i][1], s);
         ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
                                         P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk, m,
-                                        d_base + (size_t)i * n * 3, M->shifts, mstride, P.matching_threshold, nullptr,
+                                        d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold, nullptr,
                                         s));
         (void)hipEventRecord(M->lev_ev[i][2], s);
         ISA_TRY(isa::launch_minimize(M->shifts, mstride, M->one_to_one, nullptr, nullptr, d_rows, M->cx[i] * M->cy[i], n,
