@@ -65,6 +65,7 @@ class IsaParams(C.Structure):
         ("cfa", C.c_int * 4),
         ("reference_index", C.c_int),
         ("tex_quant", C.c_int),
+        ("xc_precision", C.c_int),
     ]
 
     def set_levels(self, levels):

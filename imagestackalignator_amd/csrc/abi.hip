@@ -76,6 +76,7 @@ void isa_default_params(isa_params* p) {
     p->cfa[3] = 2;
     p->reference_index = -1;
     p->tex_quant = 1;
+    p->xc_precision = ISA_XC_FP64;
 }
 
 }  // extern "C"

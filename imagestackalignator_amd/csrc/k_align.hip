@@ -7,25 +7,13 @@
 
 #include "isa_device.h"
 #include "isa_host.h"
+#include "tile_search.h"
 
 namespace {
 
 constexpr int kMaxShift = 8;  // largest supported MaxShift (search window (2M+1)^2 <= 289)
 
 // Integer offset of a tile (R:Kernels/kernel.cu:183-202 and :238-261): round(preshift + R(theta)(-b) + R(theta)c - c).
-// Tile offset with the rotation given as its (sin, cos) pair, each rounded once to float.
-__device__ __forceinline__ void tile_offset_sc(int tx, int ty, int tile, int w, int h, float bsx, float bsy, float sf,
-                                               float cf, float psx, float psy, int& ox, int& oy) {
-    float sx = psx + (cf * -bsx - sf * -bsy);
-    float sy = psy + (sf * -bsx + cf * -bsy);
-    float pcx = (float)(tx * tile + tile / 2 - w / 2);
-    float pcy = (float)(ty * tile + tile / 2 - h / 2);
-    sx += cf * pcx - sf * pcy - pcx;
-    sy += sf * pcx + cf * pcy - pcy;
-    ox = (int)roundf(sx);
-    oy = (int)roundf(sy);
-}
-
 __device__ __forceinline__ void tile_offset(int tx, int ty, int tile, int w, int h, float bsx, float bsy, float rot,
                                             float psx, float psy, int& ox, int& oy) {
     float sf = sin_r(rot), cf = cos_r(rot);
@@ -133,59 +121,6 @@ __device__ void ssd_core(const float* r, const float* t, int T, double* Q, doubl
     __syncthreads();
 }
 
-// findMinimum (R:Kernels/kernel.cu:413-546) on an S x S map.
-__device__ void find_min_core(const float* v, int M, float threshold, float& cx, float& cy) {
-    const float FA11[9] = {1.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f, 2.0f / 4.0f, -4.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f,
-                           -2.0f / 4.0f, 1.0f / 4.0f};
-    const float FA22[9] = {1.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f, -2.0f / 4.0f, -4.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f,
-                           2.0f / 4.0f, 1.0f / 4.0f};
-    const float FA12[9] = {1.0f / 4.0f, 0.0f, -1.0f / 4.0f, 0.0f, 0.0f, 0.0f, -1.0f / 4.0f, 0.0f, 1.0f / 4.0f};
-    const float Fb1[9] = {-1.0f / 8.0f, 0.0f, 1.0f / 8.0f, -2.0f / 8.0f, 0.0f, 2.0f / 8.0f, -1.0f / 8.0f, 0.0f,
-                          1.0f / 8.0f};
-    const float Fb2[9] = {-1.0f / 8.0f, -2.0f / 8.0f, -1.0f / 8.0f, 0.0f, 0.0f, 0.0f, 1.0f / 8.0f, 2.0f / 8.0f,
-                          1.0f / 8.0f};
-    const int S = 2 * M + 1, n = S * S;
-    float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
-    int mi = -1;
-    for (int i = 0; i < n; ++i) {
-        float val = v[i];
-        mx = fmaxf(mx, val);
-        if (val < mn) { mn = val; mi = i; }
-    }
-    cy = (float)(mi / S);
-    cx = (float)mi - cy * (float)S;
-    if (cx < 1 || cy < 1 || cx >= (float)(2 * M) || cy >= (float)(2 * M)) {
-        cx = 0.0f;
-        cy = 0.0f;
-    } else {
-        float A11 = 0, A22 = 0, A12 = 0, b1 = 0, b2 = 0;
-        for (int i = 0; i < 9; ++i) {
-            int off = (i / 3 - 1) * S + (i % 3 - 1);
-            float img = v[mi + off];
-            A11 += FA11[i] * img;
-            A22 += FA22[i] * img;
-            A12 += FA12[i] * img;
-            b1 += Fb1[i] * img;
-            b2 += Fb2[i] * img;
-        }
-        A11 = fmaxf(A11, 0.0f);
-        A22 = fmaxf(A22, 0.0f);
-        float det = A11 * A22 - A12 * A12;
-        if (det < 0) { A12 = 0; det = A11 * A22; }
-        if (det != 0) {
-            float mux = (A22 * b1 - A12 * b2) / det;
-            float muy = (A11 * b2 - A12 * b1) / det;
-            if (fabsf(mux) > 1) mux = 0;
-            if (fabsf(muy) > 1) muy = 0;
-            cx -= mux;
-            cy -= muy;
-        }
-        cx -= (float)M;
-        cy -= (float)M;
-    }
-    if (threshold + mn > mx) { cx = 0.0f; cy = 0.0f; }
-}
-
 size_t ssd_smem(int T, int M) {
     int S = 2 * M + 1, B = T + 2 * M;
     size_t f = (size_t)T * (T + 1) + (size_t)B * (B + 1) + (size_t)S * S;  // floats
@@ -215,19 +150,6 @@ __device__ __forceinline__ SsdLds ssd_lds(char* smem, int T, int M) {
 // Fused PatchTracker.Track for every pair of a level at once: grid (tiles, pairs).
 // Level images: frames[f] (device array of pointers), row pitch img_pitch bytes.
 // shifts: [meas][tiles] float2 (meas_stride floats per measurement); pair p uses measurement index pk[p].
-struct TrackArgs {
-    const float* const* frames;
-    int img_pitch, w, h, tile, cx, cy;
-    const int* pa;  // reference frame of pair p
-    const int* pb;  // tracked frame of pair p
-    const int* pk;  // measurement index of pair p
-    const float* base;  // per frame: bsx, bsy (already divided by the level), sin(rot), cos(rot)
-    float* shifts;
-    size_t meas_stride;
-    float threshold;
-    float* shift_images;  // optional: [pair][tile][S*S]
-};
-
 template <int M>
 __global__ void k_track_level(TrackArgs a) {
     extern __shared__ __align__(16) char smem[];
@@ -795,6 +717,13 @@ __global__ void k_optimal_shifts(float* out, int pitch, const float* r, int n, i
 // ---------------------------------------------------------------------------
 // host side
 namespace isa {
+int launch_track_mfma(int precision, const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx,
+                      int cy, const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks,
+                      const float* base, float* shifts, size_t meas_stride, float threshold, float* shift_images,
+                      hipStream_t s);
+}  // namespace isa
+
+namespace isa {
 
 int shift_tables(int n, int ref, int strategy, int block_in, int* pairs, int* index, int* rows) {
     if (n < 2 || n > kMinFrames || ref < 0 || ref >= n) return -ISA_ERR_ARG;
@@ -1016,13 +945,14 @@ int isa_upsample_shifts(const float* in2_dev, float* out2_dev, int in_pitch, int
                                 new_count_x, new_count_y, old_tile_size, new_tile_size, isa::to_stream(stream));
 }
 
-int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
-              int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
-              float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
-              isa_stream_t stream) {
+int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+                 int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
+                 float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
+                 int xc_precision, isa_stream_t stream) {
     // PatchTracker.Track through the fused level kernel with one pair (frame 0 = reference, frame 1 = tracked).
     ISA_REQUIRE(img_track_dev && img_ref_dev && preshift2_dev && width > 0 && height > 0 && tile_size > 0,
                 "isa_track: bad arguments");
+    ISA_REQUIRE(xc_precision >= ISA_XC_FP64 && xc_precision <= ISA_XC_FP32, "isa_track: unknown xc_precision");
     ISA_REQUIRE(max_shift >= 1 && max_shift <= kMaxShift, "isa_track: max_shift must be in [1, 8]");
     int cx = (width - 2 * max_shift) / tile_size, cy = (height - 2 * max_shift) / tile_size;
     ISA_REQUIRE(cx > 0 && cy > 0, "isa_track: image smaller than one tile");
@@ -1030,8 +960,9 @@ int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitc
     struct Scratch {
         const float* frames[2];
         int pa, pb, pk, pad;
+        int chunk[16];
         float base[8];
-    } h{{img_ref_dev, img_track_dev}, 0, 1, 0, 0,
+    } h{{img_ref_dev, img_track_dev}, 0, 1, 0, 0, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
         {base_shift_ref[0], base_shift_ref[1], (float)std::sin((double)base_rotation_ref),
          (float)std::cos((double)base_rotation_ref), base_shift_track[0], base_shift_track[1],
          (float)std::sin((double)base_rotation_track), (float)std::cos((double)base_rotation_track)}};
@@ -1046,8 +977,12 @@ int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitc
         rc = isa::check_hip(hipMemcpy2DAsync(dn, row, preshift2_dev, preshift_pitch, row, cy, hipMemcpyDeviceToDevice, s),
                             "isa_track: copy");
     if (rc == ISA_OK)
-        rc = isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
-                                     &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, s);
+        rc = xc_precision == ISA_XC_FP64
+                 ? isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
+                                           &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, s)
+                 : isa::launch_track_mfma(xc_precision, sd->frames, img_pitch, width, height, tile_size, max_shift, cx,
+                                          cy, &sd->pa, &sd->pb, &sd->pk, sd->chunk, 1, sd->base, dn, 0, threshold,
+                                          nullptr, s);
     if (rc == ISA_OK)
         rc = isa::check_hip(hipMemcpy2DAsync(preshift2_dev, preshift_pitch, dn, row, row, cy, hipMemcpyDeviceToDevice, s),
                             "isa_track: copy back");
@@ -1057,6 +992,15 @@ int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitc
     if (rc == ISA_OK) rc = rs;
     if (rc == ISA_OK) rc = isa::check_hip(fe, "isa_track: free");
     return rc;
+}
+
+int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+              int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
+              float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
+              isa_stream_t stream) {
+    return isa_track_xc(img_track_dev, img_ref_dev, img_pitch, preshift2_dev, preshift_pitch, width, height, tile_size,
+                        max_shift, base_shift_ref, base_rotation_ref, base_shift_track, base_rotation_track, threshold,
+                        ISA_XC_FP64, stream);
 }
 
 int isa_shift_tables(int frame_count, int reference_index, int strategy, int block_size, int* pairs_out,

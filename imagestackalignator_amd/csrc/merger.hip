@@ -39,6 +39,12 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
                  float* flow, int flow_pitch, float* work, int w, int h, int iterations, float bsx, float bsy,
                  float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps);
 size_t tap_table_bytes(int w, int h);
+int track_chunks(const int* pa, int m, int n, int* out_chunks);
+int track_chunks_capacity(int m, int n);
+int launch_track_mfma(int precision, const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx,
+                      int cy, const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks,
+                      const float* base, float* shifts, size_t meas_stride, float threshold, float* shift_images,
+                      hipStream_t s);
 int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s);
 int launch_subsample3(const uint16_t* raw, float* out, float max_val, int hw, int hh, int out_pitch, const int cfa[4],
                       hipStream_t s);
@@ -150,7 +156,7 @@ struct isa_merger {
 namespace {
 
 struct TabLayout {
-    size_t pa, pb, pk, rows, base, frames, acc_raw, acc_mask, acc_flow, bytes;
+    size_t pa, pb, pk, rows, base, frames, acc_raw, acc_mask, acc_flow, chunks, bytes;
 };
 
 TabLayout tab_layout(int m, int n, int L) {
@@ -166,6 +172,7 @@ TabLayout tab_layout(int m, int n, int L) {
     t.acc_raw = o; o += al(n * sizeof(void*));
     t.acc_mask = o; o += al(n * sizeof(void*));
     t.acc_flow = o; o += al(n * sizeof(void*));
+    t.chunks = o; o += al((size_t)isa::track_chunks_capacity(m, n) * 16 * sizeof(int));
     t.bytes = o;
     return t;
 }
@@ -225,6 +232,7 @@ int validate(const isa_params& P, int w, int h, int n) {
     ISA_REQUIRE(P.iterations_lk >= 0 && P.window_size_lk >= 1 && P.erode_size >= 1 && P.erode_size <= 41,
                 "merger: LK / erode parameters out of range");
     for (int i = 0; i < 4; ++i) ISA_REQUIRE(P.cfa[i] >= 0 && P.cfa[i] <= 2, "merger: CFA entries must be 0, 1 or 2");
+    ISA_REQUIRE(P.xc_precision >= ISA_XC_FP64 && P.xc_precision <= ISA_XC_FP32, "merger: unknown xc_precision");
     if (P.use_high_pass && P.clear_axis != 0) {
         isa::set_error("merger: clear_axis != 0 is not supported");
         return ISA_ERR_UNSUPPORTED;
@@ -347,6 +355,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         pk[p] = index[(size_t)pa[p] * n + pb[p]];
     }
     std::memcpy(th + tl.rows, rows.data(), 2 * m * sizeof(int));
+    const int n_chunks = isa::track_chunks(pa, m, n, (int*)(th + tl.chunks));
     float* base = (float*)(th + tl.base);
     const float** fr = (const float**)(th + tl.frames);
     for (int i = 0; i < M->L; ++i) {
@@ -387,6 +396,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const int* d_pk = (const int*)(M->tab + tl.pk);
     const int* d_rows = (const int*)(M->tab + tl.rows);
     const float* d_base = (const float*)(M->tab + tl.base);
+    const int* d_chunks = (const int*)(M->tab + tl.chunks);
     const float* const* d_frames = (const float* const*)(M->tab + tl.frames);
 
     float taps_track[99], taps_acc[99], taps_st[99];
@@ -424,10 +434,16 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                 ISA_TRY(isa::launch_resize_box(M->trk + (size_t)f * np, wp, M->lev[i] + (size_t)f * lw * lh,
                                                lw * (int)sizeof(float), w, h, l, s));
         (void)hipEventRecord(M->lev_ev[i][1], s);
-        ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
-                                        P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk, m,
-                                        d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold, nullptr,
-                                        s));
+        if (P.xc_precision == ISA_XC_FP64)
+            ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
+                                            P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk, m,
+                                            d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold,
+                                            nullptr, s));
+        else
+            ISA_TRY(isa::launch_track_mfma(P.xc_precision, d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh,
+                                           P.tile_size[i], P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk,
+                                           d_chunks, n_chunks, d_base + (size_t)i * n * 4, M->shifts, mstride,
+                                           P.matching_threshold, nullptr, s));
         (void)hipEventRecord(M->lev_ev[i][2], s);
         ISA_TRY(isa::launch_minimize(M->shifts, mstride, M->one_to_one, nullptr, nullptr, d_rows, M->cx[i] * M->cy[i], n,
                                      m, s));

@@ -53,6 +53,16 @@ enum isa_tracking_strategy { /* R:PEFStudioDX/ShiftCollection.cs:37-47 */
     ISA_STRATEGY_BLOCKS = 3
 };
 
+/* Arithmetic of the tile cross terms sum r(x) t(x+s). No counterpart in the reference (which uses
+ * FP32 cuFFT); FP64 is the exact direct search, the others run the cross terms as a matrix product
+ * on the matrix cores with FP32 accumulation. */
+enum isa_xc_precision {
+    ISA_XC_FP64 = 0,   /* exact direct SSD in FP64 (default) */
+    ISA_XC_BF16X3 = 1, /* bf16 hi+lo split operands, three products */
+    ISA_XC_FP16X3 = 2, /* fp16 hi+lo split operands with per-tile power-of-two scaling, three products */
+    ISA_XC_FP32 = 3    /* fp32 operands */
+};
+
 #define ISA_MAX_LEVELS 8
 
 /* Parameters of the merge path: the controller's DependencyProperties with
@@ -89,6 +99,7 @@ typedef struct isa_params {
     int cfa[4];                       /* Bayer pattern, row-major 2x2 */
     int reference_index;              /* -1: PreAlignmentStore rule (R:PEFStudioDX/PreAlignmentStore.cs:36-110) */
     int tex_quant;                    /* 1 (default): 8-bit texture weights as the texture unit; 0: exact weights */
+    int xc_precision;                 /* tile cross terms: ISA_XC_FP64 (default, exact direct SSD) or an MFMA form */
 } isa_params;
 
 void isa_default_params(isa_params* p);
@@ -182,6 +193,12 @@ int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitc
               int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
               float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
               isa_stream_t stream);
+/* isa_track with the cross terms in the given isa_xc_precision (ISA_XC_FP64 is isa_track itself). No counterpart in
+ * the reference: the entry point of the mixed-precision alignment for one pair. */
+int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+                 int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
+                 float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
+                 int xc_precision, isa_stream_t stream);
 
 /* ShiftCollection tables (R:PEFStudioDX/ShiftCollection.cs:113-496). pairs_out: 2*m ints (reference, toTrack) in
  * GetShiftPairs order; index_out: n*n ints (-1 where unmeasured); rows_out: 2*m ints (a, b): measurement row k
