@@ -9,10 +9,14 @@
 // the columns, the track-block pixels as the reduction dimension. Operands:
 //   ISA_XC_BF16X3: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); hi.hi + hi.lo + lo.hi.
 //   ISA_XC_FP16X3: the same split in fp16 after scaling the tile by a power of two (the reference
-//                  interior's maximum is brought into [0.5, 1); the track blocks use the same scale).
+//                  interior's maximum is brought into [0.5, 1); the track blocks use the same scale,
+//                  and their squares are scaled once, not twice, so they stay far from fp16's range).
 //   ISA_XC_FP32:   fp32 operands (16x16x4 MFMA).
-// All accumulate in fp32. The window energies sum r^2 and sum t^2 are formed on the vector units;
-// D(s) = sum r^2 + sum t(. + s)^2 - 2 C(s), then findMinimum as in the exact search.
+// All accumulate in fp32. The window energies of the track blocks, E[s][j] = sum_u W[s][u] t_j(u)^2
+// with W[s][u] = 1 where u - s lies in the interior, are a second product on the same fragments
+// (W is exact in every format; t^2 is split like the operands, two products). The reference
+// energy sum r^2 is a block sum in FP64. D(s) = sum r^2 + E(s) - 2 C(s), then findMinimum as in
+// the exact search.
 #include <algorithm>
 #include <cmath>
 
@@ -28,6 +32,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPairsPerChunk = 16;  // MFMA columns
 constexpr int kWaves = 4;           // waves per block; wave w reduces over track rows uy = w mod 4
+constexpr int kMaxNB = 5;           // track-block width up to 80 (16-column blocks)
 
 __device__ __forceinline__ unsigned short bf16_rn(float x) {
     unsigned u = __float_as_uint(x);
@@ -53,9 +58,9 @@ struct MfmaLayout {
     int RW, RH;        // padded reference plane: RH rows x RW columns
     int Bp;            // track-block width rounded up to 16
     size_t off_plane;  // reference planes (hi, lo) or one fp32 plane, plus one zero row
-    size_t off_red;    // per-wave partial products [kWaves][MB][256]
-    size_t off_c;      // C [chunk][S*S]
-    size_t off_col;    // column energies [chunk][B][S] (float)
+    size_t off_red;    // per-wave partial products [2][kWaves][MB][256] (C, E)
+    size_t off_c;      // C, E [2][chunk][S*S]
+    size_t off_col;    // block partial sums of r^2 (double)
     size_t off_d;      // D [chunk][S*S]
     size_t off_misc;   // R2 (double), scale
     size_t bytes;
@@ -75,11 +80,11 @@ __host__ __device__ inline MfmaLayout mfma_layout(int T, int M, int prec) {
     L.off_plane = o;
     o += al((size_t)(L.RH + 1) * L.RW * elem);
     L.off_red = o;
-    o += al((size_t)kWaves * MB * 256 * 4);
+    o += al((size_t)kWaves * MB * 256 * 4 * 2);  // C and E partials
     L.off_c = o;
-    o += al((size_t)kPairsPerChunk * S * S * 4);
-    L.off_col = o;  // also the scratch of the block's 256 double partial sums of r^2
-    o += al(std::max((size_t)kPairsPerChunk * B * S * 4, (size_t)kWaves * 64 * 8));
+    o += al((size_t)kPairsPerChunk * S * S * 4 * 2);  // C and E
+    L.off_col = o;  // the block's 256 double partial sums of r^2
+    o += al((size_t)kWaves * 64 * 8);
     L.off_d = o;
     o += al((size_t)kPairsPerChunk * S * S * 4);
     L.off_misc = o;
@@ -117,11 +122,19 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, const i
     // ---- 0. power-of-two scale of the tile (fp16 only) and the reference energy sum r^2
     float rmax = 0.0f;
     double r2 = 0.0;
-    for (int i = tid; i < T * T; i += kWaves * 64) {
-        int y = i / T, x = i - y * T;
-        float v = row_ptr(ia, a.img_pitch, clampi(ry0 + y, 0, a.h - 1))[clampi(rx0 + x, 0, a.w - 1)];
-        rmax = fmaxf(rmax, fabsf(v));
-    }
+    if (PREC == ISA_XC_FP16X3)
+        for (int yb = wave; yb < T; yb += kWaves * 8) {
+            float v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int y = yb + (k >> 1) * kWaves, x = lane + 64 * (k & 1);
+                v[k] = (y < T && x < T)
+                           ? row_ptr(ia, a.img_pitch, clampi(ry0 + y, 0, a.h - 1))[clampi(rx0 + x, 0, a.w - 1)]
+                           : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) rmax = fmaxf(rmax, fabsf(v[k]));
+        }
     float* red = (float*)(smem + L.off_red);
     red[tid] = rmax;
     __syncthreads();
@@ -143,21 +156,36 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, const i
     unsigned short* ph = (unsigned short*)(smem + L.off_plane);  // hi plane (16-bit precisions)
     unsigned short* pl = ph + (size_t)(RH + 1) * RW;             // lo plane
     float* pf = (float*)(smem + L.off_plane);                    // fp32 plane
-    for (int i = tid; i < (RH + 1) * RW; i += kWaves * 64) {
-        int yy = i / RW, xx = i - yy * RW;
-        int y = yy - (S - 1), x = xx - (S - 1);
-        float v = 0.0f;
-        if (yy < RH && y >= 0 && y < T && x >= 0 && x < T)
-            v = row_ptr(ia, a.img_pitch, clampi(ry0 + y, 0, a.h - 1))[clampi(rx0 + x, 0, a.w - 1)];
-        if (PREC == ISA_XC_FP32) {
-            pf[i] = v;
-        } else {
-            unsigned short hi, lo;
-            split16<PREC>(v * sc, hi, lo);
-            ph[i] = hi;
-            pl[i] = lo;
+    // rows in batches of 8 per wave, all loads of a batch issued before any is used
+    constexpr int kFill = 8;
+    for (int yb = wave; yb < RH + 1; yb += kWaves * kFill) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int xx = lane + 64 * half;
+            const int x = xx - (S - 1);
+            float v[kFill];
+#pragma unroll
+            for (int k = 0; k < kFill; ++k) {
+                const int yy = yb + k * kWaves, y = yy - (S - 1);
+                const bool in = xx < RW && yy < RH && y >= 0 && y < T && x >= 0 && x < T;
+                v[k] = in ? row_ptr(ia, a.img_pitch, clampi(ry0 + y, 0, a.h - 1))[clampi(rx0 + x, 0, a.w - 1)] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < kFill; ++k) {
+                const int yy = yb + k * kWaves;
+                if (xx >= RW || yy >= RH + 1) continue;
+                const int i = yy * RW + xx;
+                if (PREC == ISA_XC_FP32) {
+                    pf[i] = v[k];
+                } else {
+                    unsigned short hi, lo;
+                    split16<PREC>(v[k] * sc, hi, lo);
+                    ph[i] = hi;
+                    pl[i] = lo;
+                }
+                r2 = fma((double)v[k], (double)v[k], r2);  // v = 0 outside the interior
+            }
         }
-        if (yy < RH && y >= 0 && y < T && x >= 0 && x < T) r2 = fma((double)v, (double)v, r2);
     }
     __syncthreads();
     // ---- 2. cross terms on MFMA: lane (col j = lane % 16, quad q = lane / 16)
@@ -185,75 +213,129 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, const i
         }
     }
     const int zero_row = RH * RW;
-    f32x4 acc[MB];
+    // interior window of each row's lag, for the energy product: u - s inside [0, T)^2
+    int lag_iy[MB], lag_ix[MB];
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) acc[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int mb = 0; mb < MB; ++mb) {
+        const int s = mb * 16 + j;
+        lag_iy[mb] = s < SS ? s / S : -1000000;  // padding rows: never inside
+        lag_ix[mb] = s < SS ? s - (s / S) * S : -1000000;
+    }
+    f32x4 acc[MB], accE[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        acc[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        accE[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
     const bool cols_in = tx0 >= 0 && tx0 + L.Bp <= a.w;
+    const int NB = L.Bp / 16;  // <= kMaxNB (checked by the launcher)
+    // B values of one track row: lane (j, q) needs t_j(uy, 16 nb + 4q + e) (16-bit forms) or
+    // t_j(uy, 16 nb + 4 kk + q) (fp32); the next row of the wave is fetched while this one computes
+    float cur[kMaxNB][4], nxt[kMaxNB][4];
+    auto fetch_row = [&](int uy, float (&dst)[kMaxNB][4]) {
+        const float* trow = row_ptr(ib, a.img_pitch, clampi(ty0 + min(uy, B - 1), 0, a.h - 1));
+#pragma unroll
+        for (int nb = 0; nb < kMaxNB; ++nb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int ux = 16 * nb + (PREC == ISA_XC_FP32 ? 4 * e + q : 4 * q + e);
+                const int gx = cols_in ? tx0 + ux : clampi(tx0 + ux, 0, a.w - 1);
+                dst[nb][e] = (pj >= 0 && nb < NB) ? trow[cols_in && nb < NB ? gx : clampi(gx, 0, a.w - 1)] : 0.0f;
+            }
+    };
+    fetch_row(wave, nxt);
     for (int uy = wave; uy < B; uy += kWaves) {
-        const float* trow = row_ptr(ib, a.img_pitch, clampi(ty0 + uy, 0, a.h - 1));
-        for (int ux0 = 0; ux0 < L.Bp; ux0 += 16) {
+#pragma unroll
+        for (int nb = 0; nb < kMaxNB; ++nb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cur[nb][e] = nxt[nb][e];
+        if (uy + kWaves < B) fetch_row(uy + kWaves, nxt);
+#pragma unroll
+        for (int nb = 0; nb < kMaxNB; ++nb) {
+            if (nb >= NB) break;
+            const int ux0 = 16 * nb;
             if (PREC == ISA_XC_FP32) {
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
                     const int ux = ux0 + 4 * kk + q;
-                    const float bv = pj >= 0 ? trow[cols_in ? tx0 + ux : clampi(tx0 + ux, 0, a.w - 1)] : 0.0f;
+                    const float bv = cur[nb][kk];
+                    const float bq = bv * bv;
 #pragma unroll
                     for (int mb = 0; mb < MB; ++mb) {
                         const float av = aoff[mb] >= 0 ? pf[aoff[mb] + uy * RW + ux] : pf[zero_row + ux];
+                        const int ry = uy - lag_iy[mb], rx = ux - lag_ix[mb];
+                        const float wv = (ry >= 0 && ry < T && rx >= 0 && rx < T) ? 1.0f : 0.0f;
                         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[mb], 0, 0, 0);
+                        accE[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(wv, bq, accE[mb], 0, 0, 0);
                     }
                 }
             } else {
-                // B operand: t_j(uy, ux0 + 4q + e), e = 0..3, split into hi / lo
-                unsigned short bh[4], bl[4];
+                unsigned short bh[4], bl[4], qh[4], ql[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int ux = ux0 + 4 * q + e;
-                    float bv = pj >= 0 ? trow[cols_in ? tx0 + ux : clampi(tx0 + ux, 0, a.w - 1)] : 0.0f;
-                    split16<PREC>(bv * sc, bh[e], bl[e]);
+                    const float bv = cur[nb][e] * sc;
+                    split16<PREC>(bv, bh[e], bl[e]);
+                    split16<PREC>(cur[nb][e] * bv, qh[e], ql[e]);  // t^2 scaled once: no fp16 overflow
                 }
+                constexpr unsigned short kOne = PREC == ISA_XC_BF16X3 ? 0x3F80 : 0x3C00;
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
                     const int base = (aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux0 + 4 * q;
-                    unsigned short ah[4], al[4];
+                    const int ry = uy - lag_iy[mb];
+                    const bool row_in = ry >= 0 && ry < T;
+                    unsigned short ah[4], al[4], wv[4];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         ah[e] = ph[base + e];
                         al[e] = pl[base + e];
+                        const int rx = ux0 + 4 * q + e - lag_ix[mb];
+                        wv[e] = (row_in && rx >= 0 && rx < T) ? kOne : (unsigned short)0;
                     }
                     if (PREC == ISA_XC_BF16X3) {
                         bf16x4 Ah = {(short)ah[0], (short)ah[1], (short)ah[2], (short)ah[3]};
                         bf16x4 Al = {(short)al[0], (short)al[1], (short)al[2], (short)al[3]};
                         bf16x4 Bh = {(short)bh[0], (short)bh[1], (short)bh[2], (short)bh[3]};
                         bf16x4 Bl = {(short)bl[0], (short)bl[1], (short)bl[2], (short)bl[3]};
+                        bf16x4 W = {(short)wv[0], (short)wv[1], (short)wv[2], (short)wv[3]};
+                        bf16x4 Qh = {(short)qh[0], (short)qh[1], (short)qh[2], (short)qh[3]};
+                        bf16x4 Ql = {(short)ql[0], (short)ql[1], (short)ql[2], (short)ql[3]};
                         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, acc[mb], 0, 0, 0);
                         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, acc[mb], 0, 0, 0);
                         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, acc[mb], 0, 0, 0);
+                        accE[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W, Ql, accE[mb], 0, 0, 0);
+                        accE[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W, Qh, accE[mb], 0, 0, 0);
                     } else {
-                        f16x4 Ah, Al, Bh, Bl;
+                        f16x4 Ah, Al, Bh, Bl, W, Qh, Ql;
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
                             Ah[e] = __builtin_bit_cast(_Float16, ah[e]);
                             Al[e] = __builtin_bit_cast(_Float16, al[e]);
                             Bh[e] = __builtin_bit_cast(_Float16, bh[e]);
                             Bl[e] = __builtin_bit_cast(_Float16, bl[e]);
+                            W[e] = __builtin_bit_cast(_Float16, wv[e]);
+                            Qh[e] = __builtin_bit_cast(_Float16, qh[e]);
+                            Ql[e] = __builtin_bit_cast(_Float16, ql[e]);
                         }
                         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16f16(Al, Bh, acc[mb], 0, 0, 0);
                         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bl, acc[mb], 0, 0, 0);
                         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bh, acc[mb], 0, 0, 0);
+                        accE[mb] = __builtin_amdgcn_mfma_f32_16x16x16f16(W, Ql, accE[mb], 0, 0, 0);
+                        accE[mb] = __builtin_amdgcn_mfma_f32_16x16x16f16(W, Qh, accE[mb], 0, 0, 0);
                     }
                 }
             }
         }
     }
     // ---- 3. combine the waves' partial products (fixed order w = 0..3); lane holds D[4q+i][j]
-    __syncthreads();  // the reduction buffer aliases nothing, but r-plane readers must be done before reuse
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) red[((size_t)wave * MB + mb) * 256 + (4 * q + i) * 16 + j] = acc[mb][i];
+        for (int i = 0; i < 4; ++i) {
+            red[((size_t)wave * MB + mb) * 256 + (4 * q + i) * 16 + j] = acc[mb][i];
+            red[((size_t)(kWaves + wave) * MB + mb) * 256 + (4 * q + i) * 16 + j] = accE[mb][i];
+        }
     // reference energy: block sum in a fixed order
-    double* r2buf = (double*)(smem + L.off_col);  // scratch until the column energies are formed
+    double* r2buf = (double*)(smem + L.off_col);
     r2buf[tid] = r2;
     __syncthreads();
     if (tid == 0) {
@@ -262,53 +344,27 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, const i
         *R2s = t;
     }
     float* Cm = (float*)(smem + L.off_c);
-    const float inv_sc2 = 1.0f / (sc * sc);
-    for (int i = tid; i < MB * 256; i += kWaves * 64) {
-        const int mb = i >> 8, e = i & 255, row = e >> 4, col = e & 15, s = mb * 16 + row;
+    float* Em = Cm + kPairsPerChunk * SS;
+    const float inv_sc = 1.0f / sc, inv_sc2 = inv_sc * inv_sc;
+    for (int i = tid; i < 2 * MB * 256; i += kWaves * 64) {
+        const int which = i / (MB * 256), r = i - which * MB * 256;
+        const int mb = r >> 8, e = r & 255, row = e >> 4, col = e & 15, s = mb * 16 + row;
         if (s >= SS) continue;
-        float v = red[((size_t)0 * MB + mb) * 256 + e];
+        const size_t w0 = (size_t)which * kWaves;
+        float v = red[(w0 * MB + mb) * 256 + e];
 #pragma unroll
-        for (int w = 1; w < kWaves; ++w) v += red[((size_t)w * MB + mb) * 256 + e];
-        Cm[col * SS + s] = v * inv_sc2;
+        for (int w = 1; w < kWaves; ++w) v += red[((w0 + w) * MB + mb) * 256 + e];
+        if (which == 0) Cm[col * SS + s] = v * inv_sc2;
+        else Em[col * SS + s] = v * inv_sc;
     }
     __syncthreads();
-    // ---- 4. window energies of each pair's track block: column sums over the S row windows, then
-    // the S column windows (fp32 column sums, double total)
-    float* colE = (float*)(smem + L.off_col);
-    for (int task = tid; task < kPairsPerChunk * B; task += kWaves * 64) {
-        const int jj = task / B, c = task - jj * B;
-        const int pp = cp[jj];
-        if (pp < 0) continue;
-        const int fb = a.pb[pp];
-        const float* sh = a.shifts + (size_t)a.pk[pp] * a.meas_stride + (size_t)tile_idx * 2;
-        int ox, oy;
-        tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2],
-                       a.base[fb * 4 + 3], sh[0], sh[1], ox, oy);
-        const float* img = a.frames[fb];
-        const int gx = clampi(tx * T + ox + c, 0, a.w - 1), gy0 = ty * T + oy;
-        float e = 0.0f;
-        for (int y = 0; y < T; ++y) {
-            float v = row_ptr(img, a.img_pitch, clampi(gy0 + y, 0, a.h - 1))[gx];
-            e = fmaf(v, v, e);
-        }
-        colE[((size_t)jj * B + c) * S] = e;
-        for (int k = 0; k + 1 < S; ++k) {
-            float u = row_ptr(img, a.img_pitch, clampi(gy0 + k, 0, a.h - 1))[gx];
-            float v = row_ptr(img, a.img_pitch, clampi(gy0 + k + T, 0, a.h - 1))[gx];
-            e = (e - u * u) + v * v;
-            colE[((size_t)jj * B + c) * S + k + 1] = e;
-        }
-    }
-    __syncthreads();
+    // ---- 4. D(s) = sum r^2 + E(s) - 2 C(s)
     float* Dm = (float*)(smem + L.off_d);
     const double R2 = *R2s;
     for (int task = tid; task < kPairsPerChunk * SS; task += kWaves * 64) {
         const int jj = task / SS, s = task - jj * SS;
         if (cp[jj] < 0) continue;
-        const int iy = s / S, ix = s - iy * S;
-        double tt = 0.0;
-        for (int x = 0; x < T; ++x) tt += (double)colE[((size_t)jj * B + x + ix) * S + iy];
-        Dm[jj * SS + s] = (float)(R2 + tt - 2.0 * (double)Cm[jj * SS + s]);
+        Dm[jj * SS + s] = (float)(R2 + (double)Em[jj * SS + s] - 2.0 * (double)Cm[jj * SS + s]);
     }
     __syncthreads();
     // ---- 5. findMinimum per pair
@@ -383,6 +439,10 @@ int launch_track_mfma(int precision, const float* const* frames, int img_pitch, 
                       int cy, const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks,
                       const float* base, float* shifts, size_t meas_stride, float threshold, float* shift_images,
                       hipStream_t s) {
+    if (mfma_layout(tile, M, precision).Bp / 16 > kMaxNB) {
+        set_error("track (mfma): tile %d with max shift %d is wider than 80 pixels", tile, M);
+        return ISA_ERR_UNSUPPORTED;
+    }
     if (mfma_layout(tile, M, precision).bytes > 160 * 1024) {
         set_error("track (mfma): tile %d with max shift %d needs too much LDS", tile, M);
         return ISA_ERR_UNSUPPORTED;
