@@ -41,10 +41,12 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
 size_t tap_table_bytes(int w, int h);
 int track_chunks(const int* pa, int m, int n, int* out_chunks);
 int track_chunks_capacity(int m, int n);
-int launch_track_mfma(int precision, const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx,
-                      int cy, const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks,
-                      const float* base, float* shifts, size_t meas_stride, float threshold, float* shift_images,
-                      hipStream_t s);
+int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes, int img_pitch, int w,
+                      int h, int tile, int M, int cx, int cy, const int* pa, const int* pb, const int* pk,
+                      const int* chunks, int n_chunks, const float* base, float* shifts, size_t meas_stride,
+                      float threshold, float* shift_images, hipStream_t s);
+int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s);
+bool track_mfma_supported(int tile, int M);
 int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s);
 int launch_subsample3(const uint16_t* raw, float* out, float max_val, int hw, int hh, int out_pitch, const int cfa[4],
                       hipStream_t s);
@@ -119,6 +121,8 @@ struct isa_merger {
     void* hp_ws = nullptr;
     bool hp_tables = false;
     char* taps = nullptr;  // derivative tap tables (w, h, tex_quant), built at create
+    unsigned* xplanes = nullptr;  // packed split planes of every frame at every level (16-bit cross terms)
+    size_t xplane_off[ISA_MAX_LEVELS]{};
     // device tables: pa, pb, pk (m each), rows (2m), base (L * n * 4 floats), frame pointers (L * n),
     // accumulate pointer arrays (raw n, mask n, flow n)
     char* tab = nullptr;
@@ -156,7 +160,7 @@ struct isa_merger {
 namespace {
 
 struct TabLayout {
-    size_t pa, pb, pk, rows, base, frames, acc_raw, acc_mask, acc_flow, chunks, bytes;
+    size_t pa, pb, pk, rows, base, frames, xframes, acc_raw, acc_mask, acc_flow, chunks, bytes;
 };
 
 TabLayout tab_layout(int m, int n, int L) {
@@ -169,6 +173,7 @@ TabLayout tab_layout(int m, int n, int L) {
     t.rows = o; o += al(2 * m * sizeof(int));
     t.base = o; o += al((size_t)L * n * 4 * sizeof(float));
     t.frames = o; o += al((size_t)L * n * sizeof(void*));
+    t.xframes = o; o += al((size_t)L * n * sizeof(void*));
     t.acc_raw = o; o += al(n * sizeof(void*));
     t.acc_mask = o; o += al(n * sizeof(void*));
     t.acc_flow = o; o += al(n * sizeof(void*));
@@ -233,6 +238,13 @@ int validate(const isa_params& P, int w, int h, int n) {
                 "merger: LK / erode parameters out of range");
     for (int i = 0; i < 4; ++i) ISA_REQUIRE(P.cfa[i] >= 0 && P.cfa[i] <= 2, "merger: CFA entries must be 0, 1 or 2");
     ISA_REQUIRE(P.xc_precision >= ISA_XC_FP64 && P.xc_precision <= ISA_XC_FP32, "merger: unknown xc_precision");
+    if (P.xc_precision != ISA_XC_FP64)
+        for (int i = 0; i < P.n_levels; ++i)
+            if (!isa::track_mfma_supported(P.tile_size[i], P.max_shift[i])) {
+                isa::set_error("merger: tile %d with max shift %d is too wide for the MFMA tile search",
+                               P.tile_size[i], P.max_shift[i]);
+                return ISA_ERR_UNSUPPORTED;
+            }
     if (P.use_high_pass && P.clear_axis != 0) {
         isa::set_error("merger: clear_axis != 0 is not supported");
         return ISA_ERR_UNSUPPORTED;
@@ -290,6 +302,14 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
     A(M->sum, np * 3);
     A(M->wsum, np * 3);
     A(M->taps, isa::tap_table_bytes(width, height));
+    if (P.xc_precision == ISA_XC_BF16X3 || P.xc_precision == ISA_XC_FP16X3) {
+        size_t tot = 0;
+        for (int i = 0; i < M->L; ++i) {
+            M->xplane_off[i] = tot;
+            tot += (size_t)frames * (width / P.resize_level[i]) * (height / P.resize_level[i]);
+        }
+        A(M->xplanes, tot);
+    }
     if (rc == ISA_OK && P.use_high_pass) {
         char* ws = nullptr;
         A(ws, isa::highpass_workspace(width, height, P.high_pass, P.high_pass_sigma));
@@ -358,6 +378,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const int n_chunks = isa::track_chunks(pa, m, n, (int*)(th + tl.chunks));
     float* base = (float*)(th + tl.base);
     const float** fr = (const float**)(th + tl.frames);
+    const unsigned** xfr = (const unsigned**)(th + tl.xframes);
     for (int i = 0; i < M->L; ++i) {
         const int l = P.resize_level[i];
         const size_t lsz = (size_t)(w / l) * (h / l);
@@ -370,6 +391,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             base[((size_t)i * n + f) * 4 + 2] = (float)std::sin((double)rot);
             base[((size_t)i * n + f) * 4 + 3] = (float)std::cos((double)rot);
             fr[(size_t)i * n + f] = l > 1 ? M->lev[i] + (size_t)f * lsz : M->trk + (size_t)f * np;
+            xfr[(size_t)i * n + f] = M->xplanes ? M->xplanes + M->xplane_off[i] + (size_t)f * lsz : nullptr;
         }
     }
     const uint16_t** acc_raw = (const uint16_t**)(th + tl.acc_raw);
@@ -397,6 +419,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const int* d_rows = (const int*)(M->tab + tl.rows);
     const float* d_base = (const float*)(M->tab + tl.base);
     const int* d_chunks = (const int*)(M->tab + tl.chunks);
+    const unsigned* const* d_xframes = (const unsigned* const*)(M->tab + tl.xframes);
     const float* const* d_frames = (const float* const*)(M->tab + tl.frames);
 
     float taps_track[99], taps_acc[99], taps_st[99];
@@ -433,6 +456,11 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             for (int f = 0; f < n; ++f)
                 ISA_TRY(isa::launch_resize_box(M->trk + (size_t)f * np, wp, M->lev[i] + (size_t)f * lw * lh,
                                                lw * (int)sizeof(float), w, h, l, s));
+        if (M->xplanes)
+            for (int f = 0; f < n; ++f)
+                ISA_TRY(isa::split_planes(P.xc_precision,
+                                          l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np,
+                                          M->xplanes + M->xplane_off[i] + (size_t)f * lw * lh, (size_t)lw * lh, s));
         (void)hipEventRecord(M->lev_ev[i][1], s);
         if (P.xc_precision == ISA_XC_FP64)
             ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
@@ -440,10 +468,10 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                             d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold,
                                             nullptr, s));
         else
-            ISA_TRY(isa::launch_track_mfma(P.xc_precision, d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh,
-                                           P.tile_size[i], P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk,
-                                           d_chunks, n_chunks, d_base + (size_t)i * n * 4, M->shifts, mstride,
-                                           P.matching_threshold, nullptr, s));
+            ISA_TRY(isa::launch_track_mfma(P.xc_precision, d_frames + (size_t)i * n, d_xframes + (size_t)i * n,
+                                           lw * (int)sizeof(float), lw, lh, P.tile_size[i], P.max_shift[i], M->cx[i],
+                                           M->cy[i], d_pa, d_pb, d_pk, d_chunks, n_chunks, d_base + (size_t)i * n * 4,
+                                           M->shifts, mstride, P.matching_threshold, nullptr, s));
         (void)hipEventRecord(M->lev_ev[i][2], s);
         ISA_TRY(isa::launch_minimize(M->shifts, mstride, M->one_to_one, nullptr, nullptr, d_rows, M->cx[i] * M->cy[i], n,
                                      m, s));
