@@ -717,10 +717,11 @@ __global__ void k_optimal_shifts(float* out, int pitch, const float* r, int n, i
 // ---------------------------------------------------------------------------
 // host side
 namespace isa {
-int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes, int img_pitch, int w,
-                      int h, int tile, int M, int cx, int cy, const int* pa, const int* pb, const int* pk,
-                      const int* chunks, int n_chunks, const float* base, float* shifts, size_t meas_stride,
-                      float threshold, float* shift_images, hipStream_t s);
+int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes,
+                      const float* const* vbox, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
+                      const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks, const float* base,
+                      float* shifts, size_t meas_stride, float threshold, float* shift_images, hipStream_t s);
+int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, hipStream_t s);
 int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s);
 }  // namespace isa
 
@@ -962,10 +963,11 @@ int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_p
     struct Scratch {
         const float* frames[2];
         const unsigned* xplanes[2];
+        const float* vbox[2];
         int pa, pb, pk, pad;
         int chunk[16];
         float base[8];
-    } h{{img_ref_dev, img_track_dev}, {nullptr, nullptr}, 0, 1, 0, 0,
+    } h{{img_ref_dev, img_track_dev}, {nullptr, nullptr}, {nullptr, nullptr}, 0, 1, 0, 0,
         {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
         {base_shift_ref[0], base_shift_ref[1], (float)std::sin((double)base_rotation_ref),
          (float)std::cos((double)base_rotation_ref), base_shift_track[0], base_shift_track[1],
@@ -975,14 +977,17 @@ int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_p
     const size_t plane = (((size_t)img_pitch / 4) * height * sizeof(unsigned) + 255) & ~(size_t)255;
     const bool split = xc_precision == ISA_XC_BF16X3 || xc_precision == ISA_XC_FP16X3;
     const size_t shifts_bytes = (row * cy + 255) & ~(size_t)255;
+    const bool mfma = xc_precision != ISA_XC_FP64;
+    const size_t vbytes = mfma ? (((size_t)(height - tile_size + 1) * width * sizeof(float) + 255) & ~(size_t)255) : 0;
     char* d = nullptr;
-    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d, head + shifts_bytes + (split ? 2 * plane : 0), s),
+    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d, head + shifts_bytes + (split ? 2 * plane : 0) + vbytes, s),
                            "isa_track: allocation"));
     float* dn = (float*)(d + head);
     if (split) {
         h.xplanes[0] = (const unsigned*)(d + head + shifts_bytes);
         h.xplanes[1] = (const unsigned*)(d + head + shifts_bytes + plane);
     }
+    if (mfma) h.vbox[1] = (const float*)(d + head + shifts_bytes + (split ? 2 * plane : 0));
     const Scratch* sd = (const Scratch*)d;
     int rc = isa::check_hip(hipMemcpyAsync(d, &h, sizeof(Scratch), hipMemcpyHostToDevice, s), "isa_track: upload");
     if (rc == ISA_OK)
@@ -991,13 +996,15 @@ int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_p
     for (int f = 0; f < 2 && split && rc == ISA_OK; ++f)
         rc = isa::split_planes(xc_precision, f ? img_track_dev : img_ref_dev, (unsigned*)h.xplanes[f],
                                ((size_t)img_pitch / 4) * height, s);
+    if (mfma && rc == ISA_OK)
+        rc = isa::vbox_energies(img_track_dev, img_pitch, width, height, tile_size, (float*)h.vbox[1], s);
     if (rc == ISA_OK)
         rc = xc_precision == ISA_XC_FP64
                  ? isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
                                            &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, s)
-                 : isa::launch_track_mfma(xc_precision, sd->frames, sd->xplanes, img_pitch, width, height, tile_size,
-                                          max_shift, cx, cy, &sd->pa, &sd->pb, &sd->pk, sd->chunk, 1, sd->base, dn, 0,
-                                          threshold, nullptr, s);
+                 : isa::launch_track_mfma(xc_precision, sd->frames, sd->xplanes, sd->vbox, img_pitch, width, height,
+                                          tile_size, max_shift, cx, cy, &sd->pa, &sd->pb, &sd->pk, sd->chunk, 1,
+                                          sd->base, dn, 0, threshold, nullptr, s);
     if (rc == ISA_OK)
         rc = isa::check_hip(hipMemcpy2DAsync(preshift2_dev, preshift_pitch, dn, row, row, cy, hipMemcpyDeviceToDevice, s),
                             "isa_track: copy back");

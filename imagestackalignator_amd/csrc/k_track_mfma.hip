@@ -53,10 +53,11 @@ template <int PREC> __device__ __forceinline__ unsigned split_pack(float x) {
     }
 }
 
-// four packed values -> (hi x4, lo x4) as two 64-bit operand halves
+// four packed values -> (hi x4, lo x4) as two 64-bit operand halves; one byte permute per dword
+// (v_perm_b32: bytes 0-3 of the 64-bit source are the second operand, 4-7 the first)
 __device__ __forceinline__ void unpack4(unsigned x0, unsigned x1, unsigned x2, unsigned x3, u32x2& hi, u32x2& lo) {
-    hi = u32x2{(x0 & 0xFFFFu) | (x1 << 16), (x2 & 0xFFFFu) | (x3 << 16)};
-    lo = u32x2{(x0 >> 16) | (x1 & 0xFFFF0000u), (x2 >> 16) | (x3 & 0xFFFF0000u)};
+    hi = u32x2{__builtin_amdgcn_perm(x1, x0, 0x05040100u), __builtin_amdgcn_perm(x3, x2, 0x05040100u)};
+    lo = u32x2{__builtin_amdgcn_perm(x1, x0, 0x07060302u), __builtin_amdgcn_perm(x3, x2, 0x07060302u)};
 }
 
 template <int PREC>
@@ -74,15 +75,51 @@ template <int PREC> __global__ void k_split_planes(const float* src, unsigned* d
     if (i < n) dst[i] = split_pack<PREC>(src[i]);
 }
 
+// Vertical window energies V(y0, x) = sum_{y = y0}^{y0 + T - 1} src(y, x)^2 for y0 in [0, h - T]:
+// one thread per column and run of kVRun window starts (a T-term sum, then slid in FP64).
+constexpr int kVRun = 256;
+__global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, float* dst) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y0 = blockIdx.y * kVRun, nv = h - T + 1;
+    if (x >= w || y0 >= nv) return;
+    auto at = [&](int y) { return (double)row_ptr(src, pitch, min(y, h - 1))[x]; };
+    double e = 0.0;
+    for (int yb = y0; yb < y0 + T; yb += 8) {  // loads of a batch issued together
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = yb + k < y0 + T ? at(yb + k) : 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) e += v[k] * v[k];
+    }
+    const int y1 = min(y0 + kVRun, nv);
+    for (int yb = y0; yb < y1; yb += 8) {
+        double a[8], b[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            a[k] = at(yb + k);
+            b[k] = at(yb + k + T);  // rows past the image are clamped and never used
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int y = yb + k;
+            if (y >= y1) break;
+            dst[(size_t)y * w + x] = (float)e;
+            if (y + T < h) e = (e - a[k] * a[k]) + b[k] * b[k];
+        }
+    }
+}
+
+// LDS: the reference plane is live until the product loop ends; the partial products, C, the
+// column energies and D reuse its bytes afterwards. misc (R2, scale, wave sums of r^2) is separate.
 struct MfmaLayout {
     int RW, RH;        // padded reference plane: RH rows x RW columns (+ one zero row)
     int Bp;            // track-block width rounded up to 16
     size_t off_plane;  // reference plane, 4 bytes per element (packed hi|lo, or fp32)
-    size_t off_red;    // per-wave partial products [kWaves][MB][256]
-    size_t off_c;      // C [chunk][S*S]
-    size_t off_col;    // column energies [chunk][B][S] (float); first the r^2 partial sums (double)
-    size_t off_d;      // D [chunk][S*S]
-    size_t off_misc;   // R2 (double), scale
+    size_t off_red;    // per-wave partial products [kWaves][MB][256]     (after the product loop)
+    size_t off_c;      // C [chunk][S*S]                                  (after the product loop)
+    size_t off_col;    // column energies [chunk][B][S] (float)           (after the product loop)
+    size_t off_d;      // D [chunk][S*S]                                  (after the product loop)
+    size_t off_misc;   // R2 (double), scale, per-wave sums of r^2 (double)
     size_t bytes;
 };
 
@@ -92,26 +129,26 @@ __host__ __device__ inline MfmaLayout mfma_layout(int T, int M) {
     L.Bp = (B + 15) / 16 * 16;
     L.RH = B + S - 1;
     L.RW = L.Bp + S - 1;  // reads reach column Bp - 1 + S - 1
+    auto al = [](size_t v) { return (v + 15) / 16 * 16; };
+    L.off_plane = 0;
+    const size_t plane = al((size_t)(L.RH + 1) * L.RW * 4);
     size_t o = 0;
-    L.off_plane = o;
-    o += ((size_t)(L.RH + 1) * L.RW * 4 + 15) / 16 * 16;
     L.off_red = o;
-    o += (size_t)kWaves * MB * 256 * 4;
+    o += al((size_t)kWaves * MB * 256 * 4);
     L.off_c = o;
-    o += ((size_t)kPairsPerChunk * S * S * 4 + 15) / 16 * 16;
+    o += al((size_t)kPairsPerChunk * S * S * 4);
     L.off_col = o;
-    size_t col = (size_t)kPairsPerChunk * B * S * 4, r2 = (size_t)kWaves * 64 * 8;
-    o += ((col > r2 ? col : r2) + 15) / 16 * 16;
+    o += al((size_t)kPairsPerChunk * B * S * 4);
     L.off_d = o;
-    o += ((size_t)kPairsPerChunk * S * S * 4 + 15) / 16 * 16;
-    L.off_misc = o;
-    o += 64;
-    L.bytes = o;
+    o += al((size_t)kPairsPerChunk * S * S * 4);
+    L.off_misc = o > plane ? o : plane;
+    L.bytes = L.off_misc + 64;
     return L;
 }
 
 struct XPlanes {
     const unsigned* const* frames;  // per frame: packed split plane, same pixel pitch as the level image
+    const float* const* vbox;       // per frame: vertical T-window energies, (h - T + 1) x w
 };
 
 // chunk c of the level: up to 16 pair indices sharing one reference frame (-1 = empty slot)
@@ -200,14 +237,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
             }
         }
     }
-    double* r2buf = (double*)(smem + L.off_col);
-    r2buf[tid] = r2;
+    // block sum of r^2: a butterfly within each wave, then the four wave sums in order
+    for (int off = 32; off >= 1; off >>= 1) r2 += __shfl_xor(r2, off);
+    double* r2buf = (double*)(smem + L.off_misc + 16);
+    if (lane == 0) r2buf[wave] = r2;
     __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int i = 0; i < NT; ++i) t += r2buf[i];
-        *R2s = t;
-    }
+    if (tid == 0) *R2s = ((r2buf[0] + r2buf[1]) + r2buf[2]) + r2buf[3];
     // ---- 2. cross terms on MFMA: lane (column j = lane % 16, quad q = lane / 16)
     const int j = lane & 15, q = lane >> 4;
     const int pj = cp[j];
@@ -284,11 +319,18 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
             } else {
                 u32x2 bh, bl;
                 unpack4(cur[nb][0], cur[nb][1], cur[nb][2], cur[nb][3], bh, bl);
+                // all A words of this column block first, so the LDS reads overlap the products
+                unsigned aw[MB][4];
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
                     const unsigned* ap = pu + (aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux0 + 4 * q;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) aw[mb][e] = ap[e];
+                }
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
                     u32x2 ah, al;
-                    unpack4(ap[0], ap[1], ap[2], ap[3], ah, al);
+                    unpack4(aw[mb][0], aw[mb][1], aw[mb][2], aw[mb][3], ah, al);
                     acc[mb] = mfma16<PREC>(al, bh, acc[mb]);
                     acc[mb] = mfma16<PREC>(ah, bl, acc[mb]);
                     acc[mb] = mfma16<PREC>(ah, bh, acc[mb]);
@@ -297,6 +339,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
         }
     }
     // ---- 3. combine the waves' partial products (fixed order w = 0..3); lane holds D[4q+i][j]
+    __syncthreads();  // every wave is done with the reference plane, whose bytes are reused below
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
@@ -314,7 +357,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
     }
     // ---- 4. window energies of each pair's track block: per column, the S row windows (a T-row
     // sum, then slid one row at a time), all loads of a column issued in batches of 8
-    float* colE = (float*)(smem + L.off_col);  // the r^2 partial sums were consumed above
+    float* colE = (float*)(smem + L.off_col);
     for (int task = tid; task < kPairsPerChunk * B; task += NT) {
         const int jj = task / B, c = task - jj * B;
         const int pp = cp[jj];
@@ -324,8 +367,18 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
         int ox, oy;
         tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2],
                        a.base[fb * 4 + 3], sh[0], sh[1], ox, oy);
+        const int bx0 = tx * T + ox, by0 = ty * T + oy;
+        float* ce = colE + ((size_t)jj * B + c) * S;
+        if (bx0 >= 0 && by0 >= 0 && bx0 + B <= a.w && by0 + B <= a.h) {
+            // block inside the image: the S row windows of this column are entries of the
+            // vertical window-energy image
+            const float* vb = xp.vbox[fb] + (size_t)by0 * a.w + bx0 + c;
+#pragma unroll
+            for (int k = 0; k < S; ++k) ce[k] = vb[(size_t)k * a.w];
+            continue;
+        }
         const float* img = a.frames[fb];
-        const int gx = clampi(tx * T + ox + c, 0, a.w - 1), gy0 = ty * T + oy;
+        const int gx = clampi(bx0 + c, 0, a.w - 1), gy0 = by0;
         auto px = [&](int y) { return row_ptr(img, a.img_pitch, clampi(gy0 + y, 0, a.h - 1))[gx]; };
         float e = 0.0f;
         for (int y0 = 0; y0 < T; y0 += 8) {
@@ -335,7 +388,6 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
 #pragma unroll
             for (int k = 0; k < 8; ++k) e = fmaf(v[k], v[k], e);
         }
-        float* ce = colE + ((size_t)jj * B + c) * S;
         ce[0] = e;
         float u[S - 1], w[S - 1];
 #pragma unroll
@@ -440,23 +492,31 @@ int split_planes(int precision, const float* src, unsigned* dst, size_t count, h
     return check_launch("split_planes");
 }
 
+// Vertical T-window energies of a level image (count of window starts: h - T + 1 rows of w).
+int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, hipStream_t s) {
+    if (h < T) return ISA_OK;
+    k_vbox_sq<<<dim3(div_up(w, 256), div_up(h - T + 1, kVRun)), 256, 0, s>>>(src, pitch, w, h, T, dst);
+    return check_launch("vbox_energies");
+}
+
 bool track_mfma_supported(int tile, int M) {
     const MfmaLayout L = mfma_layout(tile, M);
     return M >= 1 && M <= 8 && L.Bp / 16 <= kMaxNB && L.bytes <= 160 * 1024;
 }
 
-// xplanes: per frame, the packed split plane of the level image (same pixel pitch); unused for fp32.
-int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes, int img_pitch, int w,
-                      int h, int tile, int M, int cx, int cy, const int* pa, const int* pb, const int* pk,
-                      const int* chunks, int n_chunks, const float* base, float* shifts, size_t meas_stride,
-                      float threshold, float* shift_images, hipStream_t s) {
+// xplanes: per frame, the packed split plane of the level image (same pixel pitch; unused for fp32);
+// vbox: per frame, vbox_energies of the level image (dense, row length w).
+int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes,
+                      const float* const* vbox, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
+                      const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks, const float* base,
+                      float* shifts, size_t meas_stride, float threshold, float* shift_images, hipStream_t s) {
     if (!track_mfma_supported(tile, M)) {
         set_error("track (mfma): tile %d with max shift %d is not supported (block wider than 80 pixels)", tile, M);
         return ISA_ERR_UNSUPPORTED;
     }
     TrackArgs a{frames, img_pitch, w, h, tile, cx, cy, pa, pb, pk, base, shifts, meas_stride, threshold,
                 shift_images};
-    XPlanes xp{xplanes};
+    XPlanes xp{xplanes, vbox};
     switch (precision) {
         case ISA_XC_BF16X3: return launch_mfma_p<ISA_XC_BF16X3>(a, xp, M, chunks, n_chunks, s);
         case ISA_XC_FP16X3: return launch_mfma_p<ISA_XC_FP16X3>(a, xp, M, chunks, n_chunks, s);

@@ -41,10 +41,11 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
 size_t tap_table_bytes(int w, int h);
 int track_chunks(const int* pa, int m, int n, int* out_chunks);
 int track_chunks_capacity(int m, int n);
-int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes, int img_pitch, int w,
-                      int h, int tile, int M, int cx, int cy, const int* pa, const int* pb, const int* pk,
-                      const int* chunks, int n_chunks, const float* base, float* shifts, size_t meas_stride,
-                      float threshold, float* shift_images, hipStream_t s);
+int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes,
+                      const float* const* vbox, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
+                      const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks, const float* base,
+                      float* shifts, size_t meas_stride, float threshold, float* shift_images, hipStream_t s);
+int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, hipStream_t s);
 int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s);
 bool track_mfma_supported(int tile, int M);
 int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s);
@@ -123,6 +124,8 @@ struct isa_merger {
     char* taps = nullptr;  // derivative tap tables (w, h, tex_quant), built at create
     unsigned* xplanes = nullptr;  // packed split planes of every frame at every level (16-bit cross terms)
     size_t xplane_off[ISA_MAX_LEVELS]{};
+    float* vbox = nullptr;  // vertical window energies of every frame at every level (MFMA cross terms)
+    size_t vbox_off[ISA_MAX_LEVELS]{}, vbox_len[ISA_MAX_LEVELS]{};
     // device tables: pa, pb, pk (m each), rows (2m), base (L * n * 4 floats), frame pointers (L * n),
     // accumulate pointer arrays (raw n, mask n, flow n)
     char* tab = nullptr;
@@ -160,7 +163,7 @@ struct isa_merger {
 namespace {
 
 struct TabLayout {
-    size_t pa, pb, pk, rows, base, frames, xframes, acc_raw, acc_mask, acc_flow, chunks, bytes;
+    size_t pa, pb, pk, rows, base, frames, xframes, vframes, acc_raw, acc_mask, acc_flow, chunks, bytes;
 };
 
 TabLayout tab_layout(int m, int n, int L) {
@@ -174,6 +177,7 @@ TabLayout tab_layout(int m, int n, int L) {
     t.base = o; o += al((size_t)L * n * 4 * sizeof(float));
     t.frames = o; o += al((size_t)L * n * sizeof(void*));
     t.xframes = o; o += al((size_t)L * n * sizeof(void*));
+    t.vframes = o; o += al((size_t)L * n * sizeof(void*));
     t.acc_raw = o; o += al(n * sizeof(void*));
     t.acc_mask = o; o += al(n * sizeof(void*));
     t.acc_flow = o; o += al(n * sizeof(void*));
@@ -310,6 +314,16 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
         }
         A(M->xplanes, tot);
     }
+    if (P.xc_precision != ISA_XC_FP64) {
+        size_t tot = 0;
+        for (int i = 0; i < M->L; ++i) {
+            const int lw = width / P.resize_level[i], lh = height / P.resize_level[i];
+            M->vbox_off[i] = tot;
+            M->vbox_len[i] = (size_t)std::max(lh - P.tile_size[i] + 1, 1) * lw;
+            tot += (size_t)frames * M->vbox_len[i];
+        }
+        A(M->vbox, tot);
+    }
     if (rc == ISA_OK && P.use_high_pass) {
         char* ws = nullptr;
         A(ws, isa::highpass_workspace(width, height, P.high_pass, P.high_pass_sigma));
@@ -379,6 +393,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     float* base = (float*)(th + tl.base);
     const float** fr = (const float**)(th + tl.frames);
     const unsigned** xfr = (const unsigned**)(th + tl.xframes);
+    const float** vfr = (const float**)(th + tl.vframes);
     for (int i = 0; i < M->L; ++i) {
         const int l = P.resize_level[i];
         const size_t lsz = (size_t)(w / l) * (h / l);
@@ -392,6 +407,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             base[((size_t)i * n + f) * 4 + 3] = (float)std::cos((double)rot);
             fr[(size_t)i * n + f] = l > 1 ? M->lev[i] + (size_t)f * lsz : M->trk + (size_t)f * np;
             xfr[(size_t)i * n + f] = M->xplanes ? M->xplanes + M->xplane_off[i] + (size_t)f * lsz : nullptr;
+            vfr[(size_t)i * n + f] = M->vbox ? M->vbox + M->vbox_off[i] + (size_t)f * M->vbox_len[i] : nullptr;
         }
     }
     const uint16_t** acc_raw = (const uint16_t**)(th + tl.acc_raw);
@@ -420,6 +436,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const float* d_base = (const float*)(M->tab + tl.base);
     const int* d_chunks = (const int*)(M->tab + tl.chunks);
     const unsigned* const* d_xframes = (const unsigned* const*)(M->tab + tl.xframes);
+    const float* const* d_vframes = (const float* const*)(M->tab + tl.vframes);
     const float* const* d_frames = (const float* const*)(M->tab + tl.frames);
 
     float taps_track[99], taps_acc[99], taps_st[99];
@@ -461,6 +478,11 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                 ISA_TRY(isa::split_planes(P.xc_precision,
                                           l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np,
                                           M->xplanes + M->xplane_off[i] + (size_t)f * lw * lh, (size_t)lw * lh, s));
+        if (M->vbox)
+            for (int f = 0; f < n; ++f)
+                ISA_TRY(isa::vbox_energies(l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np,
+                                           lw * (int)sizeof(float), lw, lh, P.tile_size[i],
+                                           M->vbox + M->vbox_off[i] + (size_t)f * M->vbox_len[i], s));
         (void)hipEventRecord(M->lev_ev[i][1], s);
         if (P.xc_precision == ISA_XC_FP64)
             ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
@@ -469,7 +491,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                             nullptr, s));
         else
             ISA_TRY(isa::launch_track_mfma(P.xc_precision, d_frames + (size_t)i * n, d_xframes + (size_t)i * n,
-                                           lw * (int)sizeof(float), lw, lh, P.tile_size[i], P.max_shift[i], M->cx[i],
+                                           d_vframes + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i], P.max_shift[i], M->cx[i],
                                            M->cy[i], d_pa, d_pb, d_pk, d_chunks, n_chunks, d_base + (size_t)i * n * 4,
                                            M->shifts, mstride, P.matching_threshold, nullptr, s));
         (void)hipEventRecord(M->lev_ev[i][2], s);
