@@ -1,0 +1,182 @@
+"""Differentiable merge over device tensors: isa_merge_forward / isa_merge_backward.
+
+The reference's merge (robustness R:Kernels/RobustnessModell.cu:28-158, accumulation
+R:Kernels/DeBayerKernels.cu:288-468, kernel shape R:Kernels/kernel.cu:627-700) rounds the flow before
+it gathers, so its output is piecewise constant in the flow. The relaxed merge keeps the integer tap
+centres of the initial flow (the anchors) and lets the flow move the sample position inside them,
+which makes the merged image, and a squared loss against a target, differentiable in the flow and
+in the kernel parameters theta = (k_detail, k_denoise, D_th, D_tr).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import call, lib
+from .merge import _stream_ptr, _torch
+
+
+class IsaMergeDiff(C.Structure):
+    """Mirror of `isa_merge_diff` (include/isa_hip.h), field for field."""
+
+    _fields_ = [
+        ("width", C.c_int),
+        ("height", C.c_int),
+        ("frames", C.c_int),
+        ("reference", C.c_int),
+        ("raw_dev", C.c_void_p),
+        ("half_dev", C.c_void_p),
+        ("tensor_dev", C.c_void_p),
+        ("anchor_dev", C.c_void_p),
+        ("black", C.c_float * 3),
+        ("white", C.c_float * 3),
+        ("cfa", C.c_int * 4),
+        ("noise_alpha", C.c_float),
+        ("noise_beta", C.c_float),
+        ("k_stretch", C.c_float),
+        ("k_shrink", C.c_float),
+    ]
+
+
+def half_res(raw, params):
+    """isa_debayer_subsample3 of every frame: (frames, H/2, W/2, 3) float32 (R:Kernels/DeBayerKernels.cu:242-283)."""
+    torch = _torch()
+    n, h, w = raw.shape
+    out = torch.empty((n, h // 2, w // 2, 3), dtype=torch.float32, device=raw.device)
+    maxv = max(params.white[c] + params.black[c] for c in range(3))
+    for f in range(n):
+        call("isa_debayer_subsample3", C.c_void_p(raw[f].data_ptr()), C.c_void_p(out[f].data_ptr()), C.c_float(maxv),
+             w // 2, h // 2, (w // 2) * 12, params.cfa, _stream_ptr())
+    return out
+
+
+def smoothed_tensor(image, params):
+    """Structure tensor of a float image (H, W), Gaussian-smoothed with sigma_structure_tensor, as the
+    merge's PrepareAccumulation computes it before ComputeKernelParam: (H, W, 3) = (a11, a22, a12)."""
+    torch = _torch()
+    h, w = image.shape
+    dx = torch.empty((h, w), dtype=torch.float32, device=image.device)
+    dy = torch.empty_like(dx)
+    t = torch.empty((h, w, 3), dtype=torch.float32, device=image.device)
+    out = torch.empty_like(t)
+    tmp = torch.empty_like(t)
+    s = _stream_ptr()
+    call("isa_compute_derivatives2", C.c_void_p(image.data_ptr()), image.stride(0) * 4, C.c_void_p(dx.data_ptr()),
+         C.c_void_p(dy.data_ptr()), w * 4, w, h, params.tex_quant, s)
+    call("isa_compute_structure_tensor", C.c_void_p(dx.data_ptr()), C.c_void_p(dy.data_ptr()), w * 4,
+         C.c_void_p(t.data_ptr()), w * 12, w, h, s)
+    taps = (C.c_float * 99)()
+    nt = lib().isa_gaussian_taps(C.c_float(params.sigma_structure_tensor), taps, 99)
+    if nt <= 0:
+        raise ValueError("isa_gaussian_taps failed")
+    call("isa_gauss_filter_border", C.c_void_p(t.data_ptr()), w * 12, C.c_void_p(out.data_ptr()), w * 12,
+         C.c_void_p(tmp.data_ptr()), w, h, 3, taps, nt, s)
+    return out
+
+
+class MergeDiff:
+    """One relaxed-merge problem: raw frames, their half-resolution debayers, the reference's smoothed
+    structure tensor and the integer anchors, all device tensors kept alive by this object."""
+
+    def __init__(self, raw, half, tensor, anchor, reference: int, params):
+        torch = _torch()
+        n, h, w = raw.shape
+        assert half.shape == (n, h // 2, w // 2, 3) and tensor.shape == (h, w, 3) and anchor.shape == (n, h, w, 2)
+        for t in (raw, half, tensor, anchor):
+            assert t.is_contiguous() and t.is_cuda
+        assert half.dtype == tensor.dtype == anchor.dtype == torch.float32
+        self.raw, self.half, self.tensor, self.anchor = raw, half, tensor, anchor
+        self.width, self.height, self.frames = w, h, n
+        d = IsaMergeDiff()
+        d.width, d.height, d.frames, d.reference = w, h, n, reference
+        d.raw_dev, d.half_dev = raw.data_ptr(), half.data_ptr()
+        d.tensor_dev, d.anchor_dev = tensor.data_ptr(), anchor.data_ptr()
+        d.black[:] = list(params.black)
+        d.white[:] = list(params.white)
+        d.cfa[:] = list(params.cfa)
+        d.noise_alpha, d.noise_beta = params.noise_alpha, params.noise_beta
+        d.k_stretch, d.k_shrink = params.k_stretch, params.k_shrink
+        self.problem = d
+        nbytes = lib().isa_merge_diff_workspace(C.byref(d))
+        if nbytes == 0:
+            call("isa_merge_forward", C.byref(d), None, None, None, None, None, None, None)  # raises with the reason
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=raw.device)
+
+    @classmethod
+    def from_merger(cls, merger, raw):
+        """Problem around the last run of a BurstMerger: flows = the merger's LK flows, anchors = their
+        rounding, tensor from the reference's LK image. Returns (problem, initial flow)."""
+        torch = _torch()
+        n, h, w = raw.shape
+        ref = merger.reference_index
+        flow = torch.stack([merger.buffer_tensor(3, f, 2) for f in range(n)]).contiguous()
+        flow[ref].zero_()
+        # roundf (half away from zero), as the reference's gather rounds the flow
+        anchor = (torch.sign(flow) * torch.floor(flow.abs() + 0.5)).contiguous()
+        lk_ref = merger.buffer_tensor(1, ref, 1).contiguous()
+        tensor = smoothed_tensor(lk_ref, merger.params)
+        half = half_res(raw, merger.params)
+        return cls(raw, half, tensor, anchor, ref, merger.params), flow
+
+    def theta_of(self, params):
+        return [params.k_detail, params.k_denoise, params.dth, params.dtr]
+
+    def _flow(self, flow):
+        assert flow.shape == (self.frames, self.height, self.width, 2) and flow.is_contiguous()
+        assert flow.dtype == _torch().float32 and flow.is_cuda
+
+    def forward(self, flow, theta, gt=None, out=None):
+        """Merged (2H, 2W, 3) image and, with a target gt of that shape, the loss (sum of squared
+        differences over the interior, 4-pixel border excluded)."""
+        torch = _torch()
+        self._flow(flow)
+        if out is None:
+            out = torch.empty((2 * self.height, 2 * self.width, 3), dtype=torch.float32, device=flow.device)
+        loss = C.c_double(0.0)
+        th = (C.c_float * 4)(*[float(v) for v in theta])
+        call("isa_merge_forward", C.byref(self.problem), C.c_void_p(flow.data_ptr()), th, C.c_void_p(out.data_ptr()),
+             None if gt is None else C.c_void_p(gt.data_ptr()), C.byref(loss) if gt is not None else None,
+             C.c_void_p(self.workspace.data_ptr()), _stream_ptr())
+        return out, (loss.value if gt is not None else None)
+
+    def backward(self, flow, theta, gt, dflow=None):
+        """d loss / d flow (frames, H, W, 2) and d loss / d theta (4 floats)."""
+        torch = _torch()
+        self._flow(flow)
+        if dflow is None:
+            dflow = torch.empty_like(flow)
+        th = (C.c_float * 4)(*[float(v) for v in theta])
+        dth = (C.c_float * 4)()
+        call("isa_merge_backward", C.byref(self.problem), C.c_void_p(flow.data_ptr()), th, C.c_void_p(gt.data_ptr()),
+             C.c_void_p(dflow.data_ptr()), dth, C.c_void_p(self.workspace.data_ptr()), _stream_ptr())
+        return dflow, list(dth)
+
+    def refine(self, flow, theta, gt, steps: int, lr_flow: float, lr_theta=None, max_step: float = 0.25):
+        """Gradient descent with backtracking on the flow (each step clipped to max_step px, the flow kept
+        within 0.5 px of its anchor so the tap window stays centred) and, with lr_theta (4 step sizes), on
+        theta. A step that raises the loss is retried at half the step size (at most 8 times). Returns
+        (flow, theta, losses: the initial loss and the loss after each step)."""
+        torch = _torch()
+        flow = flow.clone()
+        theta = [float(v) for v in theta]
+        loss = self.forward(flow, theta, gt)[1]
+        losses = [loss]
+        scale = 1.0
+        for _ in range(steps):
+            g, gth = self.backward(flow, theta, gt)
+            g = g.clone()
+            for _try in range(9):
+                cand = flow - torch.clamp(scale * lr_flow * g, -max_step, max_step)
+                cand = torch.minimum(torch.maximum(cand, self.anchor - 0.5), self.anchor + 0.5).contiguous()
+                cth = list(theta)
+                if lr_theta is not None:
+                    cth = [t - scale * l * d for t, l, d in zip(theta, lr_theta, gth)]
+                new = self.forward(cand, cth, gt)[1]
+                if new <= loss:
+                    break
+                scale *= 0.5
+            else:
+                losses.append(loss)
+                continue
+            flow, theta, loss = cand, cth, new
+            losses.append(loss)
+        return flow, theta, losses

@@ -293,6 +293,39 @@ int isa_merger_stage_ms(isa_merger* m, float* ms, int capacity);
 int isa_merger_level_ms(isa_merger* m, int level, float ms[3]);
 
 /* ------------------------------------------------------------------------
+ * Differentiable merge (no counterpart in the reference, whose merge is piecewise constant in the
+ * flow). A continuous relaxation of robustness + kernel accumulation onto a 2W x 2H output of the
+ * whole frame: output pixel y at raw position y/2 - 1/4, base raw pixel b = floor(y/2); per frame
+ * the 5x5 taps around b + anchor_f(b) (fixed integers) weighted by exp(-d^T Omega d / 2) with
+ * d = tap - (y/2 - 1/4 + flow_f(b)), Omega from ComputeKernelParam on the reference's smoothed
+ * structure tensor and theta = {k_detail, k_denoise, D_th, D_tr}; certainty = the robustness mask
+ * of frame f at half-res pixel floor(b/2) with the moved 3x3 mean sampled bilinearly at
+ * h + flow_f(2h)/2 (no erosion). Loss: sum over the output interior (4-pixel border excluded) of
+ * (out - gt)^2. The backward pass recomputes the weights (recompute-in-backward) and is a gather.
+ * ------------------------------------------------------------------------ */
+typedef struct isa_merge_diff {
+    int width, height, frames;  /* raw frame size (even) and count */
+    int reference;              /* reference frame index */
+    const uint16_t* raw_dev;    /* frames x height x width */
+    const float* half_dev;      /* frames x (height/2) x (width/2) x 3: isa_debayer_subsample3 of each frame */
+    const float* tensor_dev;    /* height x width x 3: smoothed structure tensor of the reference (a11, a22, a12) */
+    const float* anchor_dev;    /* frames x height x width x 2: integer tap centres relative to b (held fixed) */
+    float black[3], white[3];   /* raw normalisation per colour */
+    int cfa[4];
+    float noise_alpha, noise_beta;
+    float k_stretch, k_shrink;  /* fixed kernel parameters */
+} isa_merge_diff;
+/* Device workspace bytes for isa_merge_forward / isa_merge_backward (0 on invalid input). */
+size_t isa_merge_diff_workspace(const isa_merge_diff* problem);
+/* Forward: out3 (2H x 2W x 3, may be NULL), loss (host, may be NULL; needs gt3). flow: frames x H x W x 2.
+ * theta: {k_detail, k_denoise, D_th, D_tr}. */
+int isa_merge_forward(const isa_merge_diff* problem, const float* flow_dev, const float theta[4], float* out3_dev,
+                      const float* gt3_dev, double* loss, void* workspace, isa_stream_t stream);
+/* Backward of the loss: dflow (frames x H x W x 2, device) and dtheta (host, 4 floats). */
+int isa_merge_backward(const isa_merge_diff* problem, const float* flow_dev, const float theta[4], const float* gt3_dev,
+                       float* dflow_dev, float dtheta[4], void* workspace, isa_stream_t stream);
+
+/* ------------------------------------------------------------------------
  * Synthetic bursts (benchmark / test input; not part of the reference).
  * --------------------------------------------------------------------- */
 typedef struct isa_synth_config {

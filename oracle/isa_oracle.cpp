@@ -1189,3 +1189,4 @@ template <class T> void apply_weighting(T* inout3, const T* sum3, const T* w3, i
 }  // namespace ora
 
 #include "isa_oracle_pipeline.inc"
+#include "isa_oracle_diff.inc"

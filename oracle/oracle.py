@@ -268,3 +268,48 @@ def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32
         raise RuntimeError(f"oracle merge failed ({rc})")
     res["ref"] = ref.value
     return res
+
+
+# ---------------------------------------------------------------------------
+# Differentiable (relaxed) merge in FP64: forward loss/output and its gradient.
+def _diff_args(prob):
+    L = lib()
+    for name in ("ora_merge_diff_forward", "ora_merge_diff_backward"):
+        f = getattr(L, name)
+        f.restype = C.c_double if name.endswith("forward") else None
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+    keep = dict(raw=np.ascontiguousarray(prob["raw"], np.uint16), half=f64(prob["half"]), tensor=f64(prob["tensor"]),
+                anchor=f64(prob["anchor"]), black=f64(prob["black"]), white=f64(prob["white"]),
+                cfa=np.ascontiguousarray(prob["cfa"], np.int32))
+    n, h, w = keep["raw"].shape
+    args = [C.c_int(w), C.c_int(h), C.c_int(n), C.c_int(prob["reference"]), ptr(keep["raw"]), ptr(keep["half"]),
+            ptr(keep["tensor"]), ptr(keep["anchor"]), ptr(keep["black"]), ptr(keep["white"]), ptr(keep["cfa"]),
+            C.c_double(prob["alpha"]), C.c_double(prob["beta"]), C.c_double(prob["k_stretch"]),
+            C.c_double(prob["k_shrink"])]
+    return args, keep, (n, h, w)
+
+
+def merge_diff_forward(prob, flow, theta, gt=None):
+    """Relaxed merge in FP64. prob: dict(raw, half, tensor, anchor, black, white, cfa, reference, alpha, beta,
+    k_stretch, k_shrink). Returns (loss, out (2h, 2w, 3), wsum, mask (n, h/2, w/2, 3))."""
+    args, keep, (n, h, w) = _diff_args(prob)
+    fl = np.ascontiguousarray(flow, np.float64)
+    th = np.ascontiguousarray(theta, np.float64)
+    out = np.zeros((2 * h, 2 * w, 3))
+    ws = np.zeros_like(out)
+    mask = np.zeros((n, h // 2, w // 2, 3))
+    g = None if gt is None else np.ascontiguousarray(gt, np.float64)
+    loss = lib().ora_merge_diff_forward(*args, ptr(fl), ptr(th), ptr(out), ptr(ws), ptr(mask), ptr(g))
+    return loss, out, ws, mask
+
+
+def merge_diff_backward(prob, flow, theta, gt):
+    """Gradient of the relaxed-merge loss: (dflow (n, h, w, 2), dtheta (4,))."""
+    args, keep, (n, h, w) = _diff_args(prob)
+    fl = np.ascontiguousarray(flow, np.float64)
+    th = np.ascontiguousarray(theta, np.float64)
+    g = np.ascontiguousarray(gt, np.float64)
+    dflow = np.zeros((n, h, w, 2))
+    dth = np.zeros(4)
+    lib().ora_merge_diff_backward(*args, ptr(fl), ptr(th), ptr(g), ptr(dflow), ptr(dth))
+    return dflow, dth
