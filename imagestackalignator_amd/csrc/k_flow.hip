@@ -346,33 +346,54 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
 }
 
 // One Lucas-Kanade iteration fused into one kernel (warp, derivatives, window sums, solve, update)
-// for half-windows HW <= 8. Per 64 x 16 block of output pixels:
+// for half-windows HW <= 8. Per 64 x 32 block of output pixels:
 //  A. the warped source and the target over the block plus HW + 3 pixels of halo, into LDS
-//     (same arithmetic as k_warp);
+//     (same arithmetic as k_warp; the flow texel of each column and row is tabulated per block);
 //  B. Ix, Iy, It over the block plus HW of halo from those tiles (same arithmetic as
 //     k_derivatives_tab; pixels outside the image are never inside the window of an updated
 //     pixel and are set to 0);
-//  C. per output pixel the five window sums: each window row is summed in increasing x, then
-//     the rows in increasing y;
+//  C. the five window sums as separable box sums: per region row, running sums along x over
+//     segments of 8 output columns (from a register window of the row's derivatives), then per
+//     output column running sums along y over 4 output rows;
 //  D. the closed-form solve and update, reading flow_in and writing flow_out (the halo of the
 //     warp reads neighbouring blocks' flow, so the flow is double-buffered across iterations).
-constexpr int LKF_BX = 64, LKF_BY = 16, LKF_RPT = 4;  // 256 threads, 4 output rows each
+constexpr int LKF_BX = 64, LKF_BY = 32, LKF_RPT = 4;  // 512 threads, 4 output rows each
+constexpr int LKF_SEG = 8;                            // output columns per horizontal running sum
+
+template <int HW>
+struct LkfShared {
+    static constexpr int DH = LKF_BY + 2 * HW, DW = LKF_BX + 2 * HW;  // derivative region
+    static constexpr int WR = DH + 6, WC = DW + 6;                     // warped region
+    static constexpr int LDW = DW | 1, HS = LKF_BX + 1;                // padded LDS row strides
+    struct AB {                                                       // phases A and B
+        float Lw[WR][WC], Lt[WR][WC];
+        LTap Tc[DW][5], Tr[DH][5];
+        int fx[WC], fy[WR];  // flow texel (point sampling, clamp) of each warped column and row
+        float Ld[3][DH][LDW];
+    };
+    union {
+        AB ab;
+        float H[5][DH][HS];  // phase C: horizontal window sums, written once every thread has read Ld
+    } u;
+    int mixed;  // some tap of the block has a weight strictly between 0 and 1
+};
 
 template <int HW>
 __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     const float* src, const float* tgt, int img_pitch, const float* flow_in, float* flow_out, int flow_pitch, int w,
     int h, float min_det, int quant, const Tap* __restrict__ col, const Tap* __restrict__ row) {
-    constexpr int DH = LKF_BY + 2 * HW, DW = LKF_BX + 2 * HW;  // derivative region
-    constexpr int WR = DH + 6, WC = DW + 6;                     // warped region
+    using S = LkfShared<HW>;
+    constexpr int DH = S::DH, DW = S::DW, WR = S::WR, WC = S::WC;
     constexpr int NT = LKF_BX * (LKF_BY / LKF_RPT);
-    __shared__ float Lw[WR][WC], Lt[WR][WC];
-    __shared__ float Ld[3][DH][DW];
+    __shared__ S sm;
+    auto& Lw = sm.u.ab.Lw;
+    auto& Lt = sm.u.ab.Lt;
+    auto& Ld = sm.u.ab.Ld;
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * LKF_BX, y0 = blockIdx.y * LKF_BY;
     const int wx0 = x0 - HW - 3, wy0 = y0 - HW - 3;
-    __shared__ LTap Tc[DW][5], Tr[DH][5];
-    __shared__ int s_mixed;  // some tap of the block has a weight strictly between 0 and 1
-    if (tid == 0) s_mixed = 0;
+    const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
+    if (tid == 0) sm.mixed = 0;
     __syncthreads();
     // tap tables of the block's derivative region, translated to warped-tile coordinates
     for (int i = tid; i < (DW + DH) * 5; i += NT) {
@@ -382,27 +403,33 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         const Tap t = (is_c ? col : row)[(size_t)clampi(g, 0, n - 1) * 5 + k];
         LTap l = {clampi(t.i0 - o, 0, lim - 1), clampi(t.i1 - o, 0, lim - 1), t.a, 0};
         l.p = t.a == 0.0f ? l.c0 : l.c1;
-        if (t.a != 0.0f && t.a != 1.0f) s_mixed = 1;
-        if (is_c) Tc[pix][k] = l;
-        else Tr[pix][k] = l;
+        if (t.a != 0.0f && t.a != 1.0f) sm.mixed = 1;
+        if (is_c) sm.u.ab.Tc[pix][k] = l;
+        else sm.u.ab.Tr[pix][k] = l;
     }
+    // flow texels: WarpingKernel reads the flow by point sampling at ((x + 0.5) / w, (y + 0.5) / h)
+    for (int i = tid; i < WC + WR; i += NT) {
+        const bool is_c = i < WC;
+        const int g = is_c ? clampi(wx0 + i, 0, w - 1) : clampi(wy0 + i - WC, 0, h - 1);
+        const int n = is_c ? w : h;
+        const float u = div_n((float)g + 0.5f, is_c ? iw : ih);
+        const int t = tex_addr((int)floorf(u * (float)n), n, ISA_ADDR_CLAMP);
+        if (is_c) sm.u.ab.fx[i] = t;
+        else sm.u.ab.fy[i - WC] = t;
+    }
+    __syncthreads();
     // A. warped source and target: all flow loads first, then all source gathers (no branch between
     // them, so every load of a thread is in flight at once)
     constexpr int NA = (WR * WC + NT - 1) / NT;
-    const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
     float fa[NA], fb[NA];
     int gxs[NA], gys[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) {
         const int i = min(tid + k * NT, WR * WC - 1);
         const int ly = i / WC, lx = i - ly * WC;
-        const int gx = clampi(wx0 + lx, 0, w - 1), gy = clampi(wy0 + ly, 0, h - 1);
-        gxs[k] = gx;
-        gys[k] = gy;
-        float u = div_n((float)gx + 0.5f, iw), v = div_n((float)gy + 0.5f, ih);
-        int fxi = tex_addr((int)floorf(u * (float)w), w, ISA_ADDR_CLAMP);
-        int fyi = tex_addr((int)floorf(v * (float)h), h, ISA_ADDR_CLAMP);
-        const float* f = row_ptr(flow_in, flow_pitch, fyi) + fxi * 2;
+        gxs[k] = clampi(wx0 + lx, 0, w - 1);
+        gys[k] = clampi(wy0 + ly, 0, h - 1);
+        const float* f = row_ptr(flow_in, flow_pitch, sm.u.ab.fy[ly]) + sm.u.ab.fx[lx] * 2;
         fa[k] = f[0];
         fb[k] = f[1];
     }
@@ -432,6 +459,7 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     }
     __syncthreads();
     // B. derivatives
+    const bool mixed = sm.mixed != 0;
     for (int i = tid; i < DH * DW; i += NT) {
         int ly = i / DW, lx = i - ly * DW;
         int gx = x0 - HW + lx, gy = y0 - HW + ly;
@@ -441,9 +469,9 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
             Ld[2][ly][lx] = 0.0f;
             continue;
         }
-        const LTap* c = Tc[lx];
-        const LTap* r = Tr[ly];
-        if (!s_mixed) {
+        const LTap* c = sm.u.ab.Tc[lx];
+        const LTap* r = sm.u.ab.Tr[ly];
+        if (!mixed) {
             // every tap lands on one texel: the stencils are plain texel differences
             const int r2 = r[2].p, c2 = c[2].p;
             auto px5 = [&](const float (*L)[WC]) {
@@ -494,31 +522,73 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         Ld[1][ly][lx] = (t0 + t1) * 0.5f;
     }
     __syncthreads();
-    // C. window sums for LKF_RPT output rows of one column
+    // C1. horizontal window sums of the five products for every region row: a task covers LKF_SEG
+    // output columns of one row, starting with a full (2HW+1)-term sum and continuing as a running
+    // sum (the derivative window of the task stays in registers). The sums overwrite the LDS of
+    // phases A and B, so they are stored after a barrier.
+    constexpr int NSEG = LKF_BX / LKF_SEG, NW = LKF_SEG + 2 * HW;
+    static_assert(DH * NSEG <= NT, "one horizontal task per thread");
+    static_assert(sizeof(sm.u.H) <= sizeof(sm.u.ab), "the window sums fit in the phase A/B storage");
+    const bool has_task = tid < DH * NSEG;
+    const int ry = tid % DH, xs = (tid / DH) * LKF_SEG;
+    float hs[5][LKF_SEG];
+    if (has_task) {
+        float ax[NW], ay[NW], at[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            ax[k] = Ld[0][ry][xs + k];
+            ay[k] = Ld[1][ry][xs + k];
+            at[k] = Ld[2][ry][xs + k];
+        }
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
+#pragma unroll
+        for (int k = 0; k <= 2 * HW; ++k) {
+            s0 += ax[k] * ax[k];
+            s1 += ax[k] * ay[k];
+            s2 += ay[k] * ay[k];
+            s3 += ax[k] * at[k];
+            s4 += ay[k] * at[k];
+        }
+#pragma unroll
+        for (int i = 0; i < LKF_SEG; ++i) {
+            if (i > 0) {
+                const int a = i + 2 * HW, b = i - 1;  // column entering and column leaving the window
+                s0 += ax[a] * ax[a] - ax[b] * ax[b];
+                s1 += ax[a] * ay[a] - ax[b] * ay[b];
+                s2 += ay[a] * ay[a] - ay[b] * ay[b];
+                s3 += ax[a] * at[a] - ax[b] * at[b];
+                s4 += ay[a] * at[a] - ay[b] * at[b];
+            }
+            hs[0][i] = s0;
+            hs[1][i] = s1;
+            hs[2][i] = s2;
+            hs[3][i] = s3;
+            hs[4][i] = s4;
+        }
+    }
+    __syncthreads();
+    auto& H = sm.u.H;
+    if (has_task)
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+#pragma unroll
+            for (int i = 0; i < LKF_SEG; ++i) H[q][ry][xs + i] = hs[q][i];
+    __syncthreads();
+    // C2. vertical window sums for LKF_RPT output rows of one column: a full sum for the first row,
+    // a running sum for the others
     const int lx = tid % LKF_BX, oy0 = (tid / LKF_BX) * LKF_RPT;
     float V[LKF_RPT][5];
 #pragma unroll
-    for (int o = 0; o < LKF_RPT; ++o)
+    for (int q = 0; q < 5; ++q) {
+        float v = 0.0f;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) V[o][q] = 0.0f;
-#pragma unroll 1
-    for (int rr = 0; rr < LKF_RPT + 2 * HW; ++rr) {
-        const int ry = oy0 + rr;
-        float H[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int r = 0; r <= 2 * HW; ++r) v += H[q][oy0 + r][lx];
+        V[0][q] = v;
 #pragma unroll
-        for (int k = 0; k <= 2 * HW; ++k) {
-            float dx = Ld[0][ry][lx + k], dy = Ld[1][ry][lx + k], dt = Ld[2][ry][lx + k];
-            H[0] += dx * dx;
-            H[1] += dx * dy;
-            H[2] += dy * dy;
-            H[3] += dx * dt;
-            H[4] += dy * dt;
+        for (int o = 1; o < LKF_RPT; ++o) {
+            v += H[q][oy0 + o + 2 * HW][lx] - H[q][oy0 + o - 1][lx];
+            V[o][q] = v;
         }
-#pragma unroll
-        for (int o = 0; o < LKF_RPT; ++o)
-            if (rr >= o && rr <= o + 2 * HW)
-#pragma unroll
-                for (int q = 0; q < 5; ++q) V[o][q] += H[q];
     }
     // D. solve and update
     const int px = x0 + lx;
