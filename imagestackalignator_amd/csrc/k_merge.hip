@@ -1,4 +1,6 @@
 // Merge kernels: half-resolution debayer, robustness mask, kernel accumulation.
+#include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "isa_device.h"
@@ -228,10 +230,11 @@ __global__ void k_accumulate(const uint16_t* __restrict__ raw, float* sum, float
 // Frame-fused accumulation: the 25 weights are computed once per pixel and the running
 // sums stay in registers across frames; frames are visited in the given order, so the
 // result equals running accumulateImages once per frame in that order.
+constexpr int ACC_MAXF = 32;  // frames per launch (passed by value in the kernel arguments)
 struct FramesArgs {
-    const uint16_t* const* raw;
-    const float* const* mask;
-    const float* const* flow;  // null entry = zero flow
+    const uint16_t* raw[ACC_MAXF];
+    const float* mask[ACC_MAXF];
+    const float* flow[ACC_MAXF];  // null entry = zero flow
     int nframes;
 };
 
@@ -354,12 +357,23 @@ int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, 
                              int flow_pitch, const uint16_t* const* raw, const float* const* mask,
                              const float* const* flow, int nframes, const float white[3], const float black[3],
                              const int cfa[4], hipStream_t s) {
-    FramesArgs fa{raw, mask, flow, nframes};
+    // raw, mask, flow: host arrays of device pointers, frames in accumulation order; launches of up to
+    // ACC_MAXF frames run in order on the stream, each continuing the sums of the previous one
     const AccArgs a = make_acc(white, black, cfa);
     auto kern_fn = a.white_int ? k_accumulate_frames<true> : k_accumulate_frames<false>;
-    kern_fn<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(sum, wsum, kern, w, h, out_pitch, mask_pitch,
-                                                                      flow_pitch, fa, a);
-    return check_launch("accumulate_frames");
+    for (int f0 = 0; f0 < nframes; f0 += ACC_MAXF) {
+        FramesArgs fa{};
+        fa.nframes = std::min(ACC_MAXF, nframes - f0);
+        for (int i = 0; i < fa.nframes; ++i) {
+            fa.raw[i] = raw[f0 + i];
+            fa.mask[i] = mask[f0 + i];
+            fa.flow[i] = flow[f0 + i];
+        }
+        kern_fn<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(sum, wsum, kern, w, h, out_pitch,
+                                                                          mask_pitch, flow_pitch, fa, a);
+        ISA_TRY(check_launch("accumulate_frames"));
+    }
+    return ISA_OK;
 }
 
 int launch_accumulate_sr(const uint16_t* raw, float* sum, float* wsum, const float* mask, const float* kern4,
@@ -409,6 +423,20 @@ int isa_accumulate_images(const uint16_t* raw_dev, float* sum3_dev, float* weigh
                 "isa_accumulate_images: bad arguments");
     return isa::launch_accumulate(raw_dev, sum3_dev, weight3_dev, mask4_dev, kernel3_dev, flow2_dev, white, black,
                                   dim_x, dim_y, out_pitch, mask_pitch, flow_pitch, cfa, isa::to_stream(stream));
+}
+
+int isa_accumulate_frames(const uint16_t* const* raw_devs, const float* const* mask4_devs,
+                          const float* const* flow2_devs, int frames, float* sum3_dev, float* weight3_dev,
+                          const float* kernel3_dev, const float white[3], const float black[3], int dim_x, int dim_y,
+                          int out_pitch, int mask_pitch, int flow_pitch, const int cfa[4], isa_stream_t stream) {
+    ISA_REQUIRE(raw_devs && mask4_devs && flow2_devs && frames >= 0 && sum3_dev && weight3_dev && kernel3_dev &&
+                    dim_x > 0 && dim_y > 0 && cfa_valid(cfa),
+                "isa_accumulate_frames: bad arguments");
+    for (int f = 0; f < frames; ++f)
+        ISA_REQUIRE(raw_devs[f] && mask4_devs[f], "isa_accumulate_frames: null frame pointer");
+    return isa::launch_accumulate_frames(sum3_dev, weight3_dev, kernel3_dev, dim_x, dim_y, out_pitch, mask_pitch,
+                                         flow_pitch, raw_devs, mask4_devs, flow2_devs, frames, white, black, cfa,
+                                         isa::to_stream(stream));
 }
 
 int isa_accumulate_images_superres(const uint16_t* raw_dev, float* sum3_dev, float* weight3_dev, const float* mask4_dev,

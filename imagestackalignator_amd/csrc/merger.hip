@@ -552,10 +552,10 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     // ---- 5. accumulation of every frame (reference first, then index order)
     if (!P.super_resolution) {
         ISA_TRY(isa::launch_accumulate_frames(M->sum, M->wsum, M->kern, w, h, 3 * wp, hw * 16, 2 * wp,
-                                              (const uint16_t* const*)(M->tab + tl.acc_raw),
-                                              (const float* const*)(M->tab + tl.acc_mask),
-                                              (const float* const*)(M->tab + tl.acc_flow), n, P.white, P.black, P.cfa,
-                                              s));
+                                              (const uint16_t* const*)(M->tab_host + tl.acc_raw),
+                                              (const float* const*)(M->tab_host + tl.acc_mask),
+                                              (const float* const*)(M->tab_host + tl.acc_flow), n, P.white, P.black,
+                                              P.cfa, s));
     } else {
         float* zero_flow = M->flows + (size_t)ref * np * 2;
         ISA_TRY(isa::check_hip(hipMemsetAsync(zero_flow, 0, np * 2 * sizeof(float), s), "memset"));

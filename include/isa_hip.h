@@ -132,6 +132,15 @@ int isa_accumulate_images(const uint16_t* raw_dev, float* sum3_dev, float* weigh
                           const float* kernel3_dev, const float* flow2_dev, const float white[3], const float black[3],
                           int dim_x, int dim_y, int out_pitch, int mask_pitch, int flow_pitch, const int cfa[4],
                           isa_stream_t stream);
+/* accumulateImages over several frames in one pass (the controller's calls for the reference frame,
+ * R:PEFStudioDX/ImageStackAlignatorController.cs:1901, and for every other frame, :1984; kernel
+ * R:Kernels/DeBayerKernels.cu:288-376):
+ * raw_devs, mask4_devs, flow2_devs are host arrays of `frames` device pointers in accumulation order
+ * (a NULL flow is the zero flow); the sums equal calling isa_accumulate_images once per frame in that order. */
+int isa_accumulate_frames(const uint16_t* const* raw_devs, const float* const* mask4_devs,
+                          const float* const* flow2_devs, int frames, float* sum3_dev, float* weight3_dev,
+                          const float* kernel3_dev, const float white[3], const float black[3], int dim_x, int dim_y,
+                          int out_pitch, int mask_pitch, int flow_pitch, const int cfa[4], isa_stream_t stream);
 /* accumulateImagesSuperRes, R:Kernels/DeBayerKernels.cu:378-468; wrapper R:PEFStudioDX/DeBayerKernels.cs:177-230.
  * kernel4 (clamp) and flow2 (mirror) are dim_x x dim_y textures sampled bilinearly. */
 int isa_accumulate_images_superres(const uint16_t* raw_dev, float* sum3_dev, float* weight3_dev, const float* mask4_dev,

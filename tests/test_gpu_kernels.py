@@ -307,6 +307,53 @@ def test_subsample_robustness_erode(isa, oracle, gpu):
         np.testing.assert_array_equal(host(er), eref)
 
 
+@pytest.mark.parametrize("cfa", [RGGB, GRBG])
+def test_accumulate_frames(isa, oracle, gpu, cfa):
+    """Several frames in one pass (LDS-staged windows, global-memory fallback for windows that leave the
+    image and for blocks whose shifted windows spread too far) against the oracle's accumulateImages
+    called once per frame in the same order."""
+    rng = np.random.default_rng(41 + cfa[0])
+    h, w, n = 72, 160, 6
+    hw, hh = w // 2, h // 2
+    raws = [_raw(rng, h, w) for _ in range(n)]
+    kern = np.zeros((h, w, 3), np.float32)
+    kern[..., 0] = rng.uniform(0.2, 2.0, (h, w))
+    kern[..., 1] = rng.uniform(0.2, 2.0, (h, w))
+    kern[..., 2] = rng.uniform(-0.1, 0.1, (h, w))
+    kern[5, 9] = np.nan
+    masks = [rng.uniform(0, 1, (hh, hw, 4)).astype(np.float32) for _ in range(n)]
+    masks[2][3, 4, 1] = np.nan
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    flows = [None]
+    for f in range(1, n):
+        g = rng.uniform(-6, 6, 2)
+        fl = np.stack([g[0] + 1.5 * np.sin(xx / 17.0 + f), g[1] + 1.5 * np.cos(yy / 13.0 - f)], -1)
+        fl += rng.uniform(-0.6, 0.6, fl.shape)
+        flows.append(fl.astype(np.float32))
+    flows[3][8:16, 40:72] = rng.uniform(-40, 40, (8, 32, 2))  # one block's windows spread past the LDS box
+    flows[4][:, :, 0] += 30.0                                  # windows leaving the image on the right
+    s0 = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    w0 = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    white, black = [16383.0, 16000.0, 15000.0], [10.0, 20.0, 30.0]
+    print(f"accumulate_frames {n} x {w}x{h} cfa={cfa}", flush=True)
+    d_raw = [dev(r) for r in raws]
+    d_mask = [dev(m) for m in masks]
+    d_flow = [None if fl is None else dev(fl) for fl in flows]
+    ptrs = lambda ts: (C.c_void_p * n)(*[None if t is None else t.data_ptr() for t in ts])  # noqa: E731
+    ds, dw = dev(s0), dev(w0)
+    isa.call("isa_accumulate_frames", ptrs(d_raw), ptrs(d_mask), ptrs(d_flow), n, p(ds), p(dw), p(dev(kern)),
+             isa.f32s(*white), isa.f32s(*black), w, h, w * 12, hw * 16, w * 8, isa.i32s(*cfa), stream())
+    sr_, wr_ = s0.copy(), w0.copy()
+    zero = np.zeros((h, w, 2), np.float32)
+    for f in range(n):
+        oracle.fn("accumulate")(oracle.ptr(raws[f]), oracle.ptr(sr_), oracle.ptr(wr_), oracle.ptr(masks[f]),
+                                oracle.ptr(kern), oracle.ptr(zero if flows[f] is None else flows[f]),
+                                oracle.ptr(oracle.arr(white, np.float32)), oracle.ptr(oracle.arr(black, np.float32)),
+                                w, h, oracle.ptr(oracle.arr(cfa, np.int32)))
+    np.testing.assert_array_equal(host(ds), sr_)
+    np.testing.assert_array_equal(host(dw), wr_)
+
+
 @pytest.mark.parametrize("sr", [0, 1])
 def test_accumulate_apply_weighting(isa, oracle, gpu, sr):
     rng = np.random.default_rng(15 + sr)
