@@ -217,54 +217,85 @@ __global__ void k_relaxed_mask(DiffArgs a, const float* mean, const float* var, 
 
 __device__ __forceinline__ float norm_raw(const DiffArgs& a, float v, int c) { return (v - a.black[c]) / a.white[c]; }
 
-// ---- forward: out (2W x 2H x 3), W (same), loss partials per block (double)
+// ---- forward: out (2W x 2H x 3), W (same), loss partials per block (double). One thread per raw
+// pixel b computes its four output pixels 2b + {0,1}^2, which share b's taps, flow and Omega.
 constexpr int kBX = 16, kBY = 16;
+
+// adds a to s[c] (c in 0..2) without a run-time array index
+__device__ __forceinline__ void add3(float& s0, float& s1, float& s2, int c, float a) {
+    s0 += c == 0 ? a : 0.0f;
+    s1 += c == 1 ? a : 0.0f;
+    s2 += c == 2 ? a : 0.0f;
+}
+__device__ __forceinline__ float pick3(const float v[3], int c) { return c == 0 ? v[0] : (c == 1 ? v[1] : v[2]); }
 
 __global__ __launch_bounds__(kBX* kBY) void k_relaxed_forward(DiffArgs a, const float* mask, float* out, float* wsum,
                                                               const float* gt, double* loss_part) {
-    const int ow = 2 * a.w, oh = 2 * a.h;
-    const int x = blockIdx.x * kBX + threadIdx.x, y = blockIdx.y * kBY + threadIdx.y;
+    const int ow = 2 * a.w, oh = 2 * a.h, hw = a.w / 2, hh = a.h / 2;
+    const int bx = blockIdx.x * kBX + threadIdx.x, by = blockIdx.y * kBY + threadIdx.y;
     const int tid = threadIdx.y * kBX + threadIdx.x;
     __shared__ double red[kBX * kBY];
     double l = 0.0;
-    if (x < ow && y < oh) {
-        const int bx = x >> 1, by = y >> 1;
-        const float px = 0.5f * x - 0.25f, py = 0.5f * y - 0.25f;
+    if (bx < a.w && by < a.h) {
         const size_t np = (size_t)a.w * a.h;
         const float* t = a.tensor + ((size_t)by * a.w + bx) * 3;
         float om[3];
         omega_val(t[0], t[1], t[2], a.th, a.kstretch, a.kshrink, om);
-        const int hw = a.w / 2, hh = a.h / 2;
         const int hx = min(bx >> 1, hw - 1), hy = min(by >> 1, hh - 1);
-        float S[3] = {0, 0, 0}, W[3] = {0, 0, 0};
+        float S[4][3], W[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) S[k][c] = W[k][c] = 0.0f;
         for (int f = 0; f < a.n; ++f) {
             const size_t o = (size_t)f * np + (size_t)by * a.w + bx;
             const float fx = a.flow[o * 2], fy = a.flow[o * 2 + 1];
             const int cx = bx + (int)a.anchor[o * 2], cy = by + (int)a.anchor[o * 2 + 1];
             const float* m = mask + ((size_t)f * hw * hh + (size_t)hy * hw + hx) * 3;
+            const float mk[3] = {m[0], m[1], m[2]};
             const uint16_t* raw = a.raw + (size_t)f * np;
-            const float zx = px + fx, zy = py + fy;
-            for (int qy = -2; qy <= 2; ++qy)
+            float zx[4], zy[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                zx[k] = (0.5f * (float)(2 * bx + (k & 1)) - 0.25f) + fx;
+                zy[k] = (0.5f * (float)(2 * by + (k >> 1)) - 0.25f) + fy;
+            }
+#pragma unroll 1
+            for (int qy = -2; qy <= 2; ++qy) {
+                const int sy = clampi(cy + qy, 0, a.h - 1);
+#pragma unroll
                 for (int qx = -2; qx <= 2; ++qx) {
-                    const int sx = clampi(cx + qx, 0, a.w - 1), sy = clampi(cy + qy, 0, a.h - 1);
+                    const int sx = clampi(cx + qx, 0, a.w - 1);
                     const int c = a.cfa.at(sy, sx);
-                    const float wt = tap_weight(om, (float)(cx + qx) - zx, (float)(cy + qy) - zy, qx, qy);
                     const float v = norm_raw(a, (float)raw[(size_t)sy * a.w + sx], c);
-                    S[c] += m[c] * wt * v;
-                    W[c] += m[c] * wt;
+                    const float mc = pick3(mk, c);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const float wt = tap_weight(om, (float)(cx + qx) - zx[k], (float)(cy + qy) - zy[k], qx, qy);
+                        const float mw = mc * wt;
+                        add3(S[k][0], S[k][1], S[k][2], c, mw * v);
+                        add3(W[k][0], W[k][1], W[k][2], c, mw);
+                    }
                 }
+            }
         }
-        float* op = out + ((size_t)y * ow + x) * 3;
-        float* wp = wsum + ((size_t)y * ow + x) * 3;
-        for (int c = 0; c < 3; ++c) {
-            op[c] = W[c] > 0.0f ? S[c] / W[c] : 0.0f;
-            wp[c] = W[c];
-        }
-        if (gt && x >= 4 && y >= 4 && x < ow - 4 && y < oh - 4) {
-            const float* g = gt + ((size_t)y * ow + x) * 3;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = 2 * bx + (k & 1), y = 2 * by + (k >> 1);
+            float* op = out + ((size_t)y * ow + x) * 3;
+            float* wp = wsum + ((size_t)y * ow + x) * 3;
+#pragma unroll
             for (int c = 0; c < 3; ++c) {
-                double d = (double)op[c] - (double)g[c];
-                l += d * d;
+                op[c] = W[k][c] > 0.0f ? S[k][c] / W[k][c] : 0.0f;
+                wp[c] = W[k][c];
+            }
+            if (gt && x >= 4 && y >= 4 && x < ow - 4 && y < oh - 4) {
+                const float* g = gt + ((size_t)y * ow + x) * 3;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double d = (double)op[c] - (double)g[c];
+                    l += d * d;
+                }
             }
         }
     }
@@ -290,15 +321,15 @@ __global__ __launch_bounds__(kBX* kBY) void k_relaxed_backward(DiffArgs a, const
     __shared__ double red[kBX * kBY][4];
     // upstream gradients of the four output pixels of b: dL/dS_c = g/W, dL/dW_c = -g out/W
     float gS[4][3], gW[4][3];
-    D4 om[3];
     float omv[3] = {0, 0, 0};
+    const float* tb = a.tensor + ((size_t)min(by, a.h - 1) * a.w + min(bx, a.w - 1)) * 3;
     if (live) {
-        const float* t = a.tensor + ((size_t)by * a.w + bx) * 3;
-        omega_of(t[0], t[1], t[2], a.th, a.kstretch, a.kshrink, om);
-        for (int k = 0; k < 3; ++k) omv[k] = om[k].v;
+        omega_val(tb[0], tb[1], tb[2], a.th, a.kstretch, a.kshrink, omv);
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int ox = 2 * bx + (k & 1), oy = 2 * by + (k >> 1);
             const bool in = gt && ox >= 4 && oy >= 4 && ox < ow - 4 && oy < oh - 4;
+#pragma unroll
             for (int c = 0; c < 3; ++c) {
                 gS[k][c] = gW[k][c] = 0.0f;
                 if (!in) continue;
@@ -311,61 +342,73 @@ __global__ __launch_bounds__(kBX* kBY) void k_relaxed_backward(DiffArgs a, const
             }
         }
     }
-    float dOm[3] = {0, 0, 0};
+    float dOm0 = 0.0f, dOm1 = 0.0f, dOm2 = 0.0f;
     // a non-finite Omega gives the constant cross-kernel weights: no gradient through w
     const bool om_ok = live && isfinite(omv[0]) && isfinite(omv[1]) && isfinite(omv[2]);
     const int hx = min(bx >> 1, hw - 1), hy = min(by >> 1, hh - 1);
     for (int f = 0; f < a.n; ++f) {
-        float gfx = 0.0f, gfy = 0.0f, dm[3] = {0, 0, 0};
+        float gfx = 0.0f, gfy = 0.0f, dm0 = 0.0f, dm1 = 0.0f, dm2 = 0.0f;
         if (live) {
             const size_t o = (size_t)f * np + (size_t)by * a.w + bx;
             const float fx = a.flow[o * 2], fy = a.flow[o * 2 + 1];
             const int cx = bx + (int)a.anchor[o * 2], cy = by + (int)a.anchor[o * 2 + 1];
             const float* m = mask + ((size_t)f * hw * hh + (size_t)hy * hw + hx) * 3;
+            const float mk[3] = {m[0], m[1], m[2]};
             const uint16_t* raw = a.raw + (size_t)f * np;
+            float zx[4], zy[4];
+#pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const float px = 0.5f * (2 * bx + (k & 1)) - 0.25f, py = 0.5f * (2 * by + (k >> 1)) - 0.25f;
-                const float zx = px + fx, zy = py + fy;
-                for (int qy = -2; qy <= 2; ++qy)
-                    for (int qx = -2; qx <= 2; ++qx) {
-                        const int sx = clampi(cx + qx, 0, a.w - 1), sy = clampi(cy + qy, 0, a.h - 1);
-                        const int c = a.cfa.at(sy, sx);
-                        const float dx = (float)(cx + qx) - zx, dy = (float)(cy + qy) - zy;
+                zx[k] = (0.5f * (float)(2 * bx + (k & 1)) - 0.25f) + fx;
+                zy[k] = (0.5f * (float)(2 * by + (k >> 1)) - 0.25f) + fy;
+            }
+#pragma unroll 1
+            for (int qy = -2; qy <= 2; ++qy) {
+                const int sy = clampi(cy + qy, 0, a.h - 1);
+#pragma unroll 1
+                for (int qx = -2; qx <= 2; ++qx) {
+                    const int sx = clampi(cx + qx, 0, a.w - 1);
+                    const int c = a.cfa.at(sy, sx);
+                    const float v = norm_raw(a, (float)raw[(size_t)sy * a.w + sx], c);
+                    const float mc = pick3(mk, c);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const float dx = (float)(cx + qx) - zx[k], dy = (float)(cy + qy) - zy[k];
                         const float wt = tap_weight(omv, dx, dy, qx, qy);
-                        const float v = norm_raw(a, (float)raw[(size_t)sy * a.w + sx], c);
-                        const float coef = gS[k][c] * v + gW[k][c];  // dL / d(m w)
-                        dm[c] += coef * wt;
+                        const float coef = pick3(gS[k], c) * v + pick3(gW[k], c);  // dL / d(m w)
+                        add3(dm0, dm1, dm2, c, coef * wt);
                         if (!om_ok) continue;
-                        const float t = coef * m[c] * wt;
+                        const float t = coef * mc * wt;
                         // d w / d flow = w Omega d
                         gfx += t * (omv[0] * dx + omv[2] * dy);
                         gfy += t * (omv[2] * dx + omv[1] * dy);
-                        dOm[0] += t * (-0.5f * dx * dx);
-                        dOm[1] += t * (-0.5f * dy * dy);
-                        dOm[2] += t * (-dx * dy);
+                        dOm0 += t * (-0.5f * dx * dx);
+                        dOm1 += t * (-0.5f * dy * dy);
+                        dOm2 += t * (-dx * dy);
                     }
+                }
             }
         }
         // the mask of h is read by the 2 x 2 raw pixels 2h + {0,1}^2: combine their terms at 2h
-        for (int c = 0; c < 3; ++c) s_dm[threadIdx.y][threadIdx.x][c] = dm[c];
+        s_dm[threadIdx.y][threadIdx.x][0] = dm0;
+        s_dm[threadIdx.y][threadIdx.x][1] = dm1;
+        s_dm[threadIdx.y][threadIdx.x][2] = dm2;
         __syncthreads();
         if (live && ((bx | by) & 1) == 0 && hx >= 1 && hy >= 1 && hx < hw - 1 && hy < hh - 1 && (bx >> 1) == hx &&
             (by >> 1) == hy) {
-            float tot[3];
-            for (int c = 0; c < 3; ++c)
-                tot[c] = (s_dm[threadIdx.y][threadIdx.x][c] + s_dm[threadIdx.y][threadIdx.x + 1][c]) +
-                         (s_dm[threadIdx.y + 1][threadIdx.x][c] + s_dm[threadIdx.y + 1][threadIdx.x + 1][c]);
             const size_t o = (size_t)f * np + (size_t)by * a.w + bx;
             const float fx = a.flow[o * 2], fy = a.flow[o * 2 + 1];
             float mm[3], gx[3], gy[3];
             bilin3(mean + (size_t)f * hw * hh * 3, hw, hh, hx + 0.5f * fx, hy + 0.5f * fy, mm, gx, gy);
             const float* mr = mean + ((size_t)a.ref * hw * hh + (size_t)hy * hw + hx) * 3;
             const float* vr = var + ((size_t)hy * hw + hx) * 3;
+#pragma unroll
             for (int c = 0; c < 3; ++c) {
+                const float tot = (s_dm[threadIdx.y][threadIdx.x][c] + s_dm[threadIdx.y][threadIdx.x + 1][c]) +
+                                  (s_dm[threadIdx.y + 1][threadIdx.x][c] + s_dm[threadIdx.y + 1][threadIdx.x + 1][c]);
                 float dmask;
                 mask_of(mr[c], vr[c], mm[c], a.alpha, a.beta, c, &dmask);
-                gfx += tot[c] * dmask * gx[c] * 0.5f;
-                gfy += tot[c] * dmask * gy[c] * 0.5f;
+                gfx += tot * dmask * gx[c] * 0.5f;
+                gfy += tot * dmask * gy[c] * 0.5f;
             }
         }
         __syncthreads();
@@ -375,13 +418,19 @@ __global__ __launch_bounds__(kBX* kBY) void k_relaxed_backward(DiffArgs a, const
             d[1] = gfy;
         }
     }
-    // d L / d theta at b: dL/dOmega . dOmega/dtheta, then a fixed-order block reduction
-    for (int i = 0; i < 4; ++i) {
-        double v = 0.0;
-        if (om_ok)
-            for (int k = 0; k < 3; ++k) v += (double)dOm[k] * (double)om[k].d[i];
-        red[tid][i] = v;
+    // d L / d theta at b: dL/dOmega . dOmega/dtheta (the duals are recomputed here), then a fixed-order
+    // block reduction
+    double v4[4] = {0.0, 0.0, 0.0, 0.0};
+    if (om_ok) {
+        D4 om[3];
+        omega_of(tb[0], tb[1], tb[2], a.th, a.kstretch, a.kshrink, om);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            v4[i] = (double)dOm0 * (double)om[0].d[i] + (double)dOm1 * (double)om[1].d[i] +
+                    (double)dOm2 * (double)om[2].d[i];
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[tid][i] = v4[i];
     __syncthreads();
     for (int st = kBX * kBY / 2; st > 0; st >>= 1) {
         if (tid < st)
@@ -401,8 +450,7 @@ struct Ws {
 Ws ws_layout(const isa_merge_diff* d, char* base) {
     auto al = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t hn = (size_t)(d->width / 2) * (d->height / 2) * 3, on = (size_t)4 * d->width * d->height * 3;
-    const size_t blocks = (size_t)isa::div_up(2 * d->width, kBX) * isa::div_up(2 * d->height, kBY) +
-                          (size_t)isa::div_up(d->width, kBX) * isa::div_up(d->height, kBY);
+    const size_t blocks = 2 * (size_t)isa::div_up(d->width, kBX) * isa::div_up(d->height, kBY);
     Ws w{};
     size_t o = 0;
     w.mean = (float*)(base + o);
@@ -483,7 +531,7 @@ int isa_merge_forward(const isa_merge_diff* d, const float* flow_dev, const floa
     const DiffArgs a = make_args(d, flow_dev, theta);
     const Ws w = ws_layout(d, (char*)workspace);
     ISA_TRY(masks(a, w, s));
-    const dim3 g(isa::div_up(2 * a.w, kBX), isa::div_up(2 * a.h, kBY)), b(kBX, kBY);
+    const dim3 g(isa::div_up(a.w, kBX), isa::div_up(a.h, kBY)), b(kBX, kBY);
     k_relaxed_forward<<<g, b, 0, s>>>(a, w.mask, w.out, w.wsum, gt3_dev, w.part);
     ISA_TRY(isa::check_launch("isa_merge_forward"));
     if (out3_dev)
@@ -511,11 +559,10 @@ int isa_merge_backward(const isa_merge_diff* d, const float* flow_dev, const flo
     const Ws w = ws_layout(d, (char*)workspace);
     // the forward state this gradient is taken at (masks, normalised output, weight sums)
     ISA_TRY(masks(a, w, s));
-    const dim3 g(isa::div_up(2 * a.w, kBX), isa::div_up(2 * a.h, kBY)), b(kBX, kBY);
-    k_relaxed_forward<<<g, b, 0, s>>>(a, w.mask, w.out, w.wsum, nullptr, w.part);
+    const dim3 gb(isa::div_up(a.w, kBX), isa::div_up(a.h, kBY)), b(kBX, kBY);
+    k_relaxed_forward<<<gb, b, 0, s>>>(a, w.mask, w.out, w.wsum, nullptr, w.part);
     ISA_TRY(isa::check_launch("isa_merge_backward: forward"));
-    const dim3 gb(isa::div_up(a.w, kBX), isa::div_up(a.h, kBY));
-    double* tpart = w.part + (size_t)g.x * g.y * 4;
+    double* tpart = w.part + (size_t)gb.x * gb.y;
     k_relaxed_backward<<<gb, b, 0, s>>>(a, w.mask, w.mean, w.var, w.out, w.wsum, gt3_dev, dflow_dev, tpart);
     ISA_TRY(isa::check_launch("isa_merge_backward"));
     if (dtheta) {
