@@ -226,8 +226,12 @@ size_t cols_smem(int T, int M, int items) {
     return (size_t)items * ((size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7));
 }
 
+// minimum waves per SIMD the register allocation must allow: M >= 2 runs faster at 3 waves (with a
+// few spills outside the row loop) than at 2 with more registers; M = 1 fits 4 as it is
+template <int M> struct TrackColsWaves { static constexpr int v = M >= 2 ? 3 : 1; };
+
 template <int M>
-__global__ __launch_bounds__(256) void k_track_cols(TrackArgs a, int pairs, int items_total, int ppb, int lpi,
+__global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackArgs a, int pairs, int items_total, int ppb, int lpi,
                                                    int nblk) {
     constexpr int S = 2 * M + 1, RPG = ColsPerGroup<M>::v, G = S / RPG;
     static_assert(G * RPG == S, "groups must tile the lags");
@@ -306,8 +310,8 @@ __global__ __launch_bounds__(256) void k_track_cols(TrackArgs a, int pairs, int 
             for (int q = 0; q < RPG; ++q) nt[kk][q] = tr[tgx[q]];
         }
     };
-    // (one block ahead for M <= 2; for M = 4 the extra registers would halve the occupancy)
-    constexpr bool kAhead = M <= 2;
+    // (one block ahead only for M = 1; for larger M the prefetch registers would cost occupancy)
+    constexpr bool kAhead = M <= 1;
     if (kAhead) fetch(0);
     for (int y = 0; y < T; y += S) {
         float cr[S], ct[S][RPG];
