@@ -209,15 +209,19 @@ __device__ __forceinline__ bool lk_pinv(float a, float b, float d, float min_det
         half_angle(2.0f * a * c + 2.0f * b * d, a * a + b * b - c * c - d * d, ct, st);
     }
     float UT0 = ct, UT2 = -st, UT1 = st, UT3 = ct;
+    // the fast form (fused LK) takes the hardware square root and reciprocal (about 1 ulp) where the
+    // exact form rounds correctly
+    auto sqrt_ = [](float v) { return EXACT ? sqrtf(v) : __builtin_amdgcn_sqrtf(v); };
+    auto rcp_ = [](float v) { return EXACT ? 1.0f / v : __builtin_amdgcn_rcpf(v); };
     float S1 = a * a + b * b + c * c + d * d;
-    float S2 = sqrtf((a * a + b * b - c * c - d * d) * (a * a + b * b - c * c - d * d) +
+    float S2 = sqrt_((a * a + b * b - c * c - d * d) * (a * a + b * b - c * c - d * d) +
                      4.0f * (a * c + b * d) * (a * c + b * d));
-    float sigma1 = sqrtf((S1 + S2) / 2.0f);
-    float sigma2 = sqrtf((S1 - S2) / 2.0f);
+    float sigma1 = sqrt_((S1 + S2) / 2.0f);
+    float sigma2 = sqrt_((S1 - S2) / 2.0f);
     float smin = fminf(sigma1, sigma1);
     if (smin < min_det) return false;
-    sigma1 = sigma1 != 0 ? 1.0f / sigma1 : 0.0f;
-    sigma2 = sigma2 != 0 ? 1.0f / sigma2 : 0.0f;
+    sigma1 = sigma1 != 0 ? rcp_(sigma1) : 0.0f;
+    sigma2 = sigma2 != 0 ? rcp_(sigma2) : 0.0f;
     float S0 = sigma1, Sb = 0.0f, Sc = 0.0f, S3 = sigma2;
     float ce, se;
     if (EXACT) {

@@ -220,6 +220,41 @@ __global__ __launch_bounds__(GF_BX* GF_BY) void k_gauss_fused1(const float* __re
     row_ptr(out, out_pitch, y)[x] = s;
 }
 
+// The same filter for a compile-time tap count: the block's rows and columns (plus the tap
+// radius, clamped at the image edge) are staged once, so no per-tap clamping or index division
+// remains. Same products and sums in the same order as k_gauss_fused1.
+template <int N>
+__global__ __launch_bounds__(GF_BX* GF_BY) void k_gauss_fixed(const float* __restrict__ in, int in_pitch,
+                                                              float* __restrict__ out, int out_pitch, int w, int h,
+                                                              Taps taps) {
+    constexpr int C = N / 2, RH = GF_BY + 2 * C, RW = GF_BX + 2 * C;
+    __shared__ float s_in[RH][RW + 1];
+    __shared__ float s_h[RH][GF_BX];
+    const int x0 = blockIdx.x * GF_BX, y0 = blockIdx.y * GF_BY;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int ly = ty; ly < RH; ly += GF_BY) {
+        const float* r = row_ptr(in, in_pitch, clampi(y0 - C + ly, 0, h - 1));
+        for (int lx = tx; lx < RW; lx += GF_BX) s_in[ly][lx] = r[clampi(x0 - C + lx, 0, w - 1)];
+    }
+    float t[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) t[k] = taps.t[k];
+    __syncthreads();
+    for (int ly = ty; ly < RH; ly += GF_BY) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc += t[k] * s_in[ly][tx + k];
+        s_h[ly][tx] = acc;
+    }
+    __syncthreads();
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h) return;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc += t[k] * s_h[ty + k][tx];
+    row_ptr(out, out_pitch, y)[x] = acc;
+}
+
 __global__ void k_resize_box(const float* in, int in_pitch, float* out, int out_pitch, int ow, int oh, int l) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= ow || y >= oh) return;
@@ -375,9 +410,15 @@ static Taps make_taps(const float* t, int n) {
 int isa::launch_gauss1(const float* in, int in_pitch, float* out, int out_pitch, float* tmp, int w, int h,
                        const float* taps, int n, hipStream_t s) {
     Taps tp = make_taps(taps, n);
+    const dim3 grid(div_up(w, GF_BX), div_up(h, GF_BY)), block(GF_BX, GF_BY);
+    switch (n) {
+        case 3: k_gauss_fixed<3><<<grid, block, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp); return check_launch("gauss3");
+        case 5: k_gauss_fixed<5><<<grid, block, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp); return check_launch("gauss5");
+        case 7: k_gauss_fixed<7><<<grid, block, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp); return check_launch("gauss7");
+        default: break;
+    }
     if (n <= 2 * GF_R + 1) {
-        k_gauss_fused1<<<dim3(div_up(w, GF_BX), div_up(h, GF_BY)), dim3(GF_BX, GF_BY), 0, s>>>(in, in_pitch, out,
-                                                                                                out_pitch, w, h, tp);
+        k_gauss_fused1<<<grid, block, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp);
         return check_launch("gauss_fused1");
     }
     dim3 b(32, 8), g(div_up(w, 32), div_up(h, 8));
