@@ -2,6 +2,7 @@
 // the fused per-level tracking launch, and the FP64 ShiftMinimizer.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -540,6 +541,7 @@ struct MinArgs {
     int* status;
     int* iters;
     const int* rows;  // [m][2] (a, b)
+    const double* lfull;  // Cholesky factor of the all-rows reduced Laplacian (n1 x n1), or null
     int tiles, n, m;
 };
 
@@ -553,8 +555,9 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
     double* red_v = x + 2 * n;          // kMinThreads
     int* red_i = (int*)(red_v + kMinThreads);  // kMinThreads
     char* active = (char*)(red_i + kMinThreads);
-    __shared__ int s_parent[kMinFrames];
+    __shared__ unsigned long long s_nb[kMinFrames];
     __shared__ int s_flag;
+    static_assert(kMinThreads == 64 && kMinFrames <= 64, "one wave, one lane per frame");
     const int t = blockIdx.x;
     for (int k = tid; k < m; k += kMinThreads) {
         const float* v = a.meas + (size_t)k * a.meas_stride + (size_t)t * 2;
@@ -565,23 +568,36 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
     __syncthreads();
     int status = 0, it;
     for (it = 0; it < 10; ++it) {
-        // connectivity of the active measurement graph (union-find, one lane)
-        if (tid == 0) {
-            for (int i = 0; i < n; ++i) s_parent[i] = i;
-            for (int k = 0; k < m; ++k) {
+        // every tile starts with all rows active: the graph, its Laplacian and the Laplacian's Cholesky
+        // factor are the same for all tiles and come precomputed (lfull is given only for a connected
+        // graph); the right-hand side is still the tile's own
+        const bool pre = it == 0 && a.lfull != nullptr;
+        // connectivity of the active measurement graph: per-node neighbour bit masks (LDS atomics over
+        // the rows), then a breadth-first sweep from node 0 with wave-wide ORs (lane i = node i)
+        if (pre) {
+            if (tid == 0) s_flag = 1;
+        } else {
+            if (tid < kMinFrames) s_nb[tid] = 0ull;
+            __syncthreads();
+            for (int k = tid; k < m; k += kMinThreads) {
                 if (!active[k]) continue;
-                int u = a.rows[2 * k], v = a.rows[2 * k + 1];
-                while (s_parent[u] != u) u = s_parent[u];
-                while (s_parent[v] != v) v = s_parent[v];
-                if (u != v) s_parent[u < v ? v : u] = u < v ? u : v;
+                const int u = a.rows[2 * k], v = a.rows[2 * k + 1];
+                atomicOr(&s_nb[u], 1ull << v);
+                atomicOr(&s_nb[v], 1ull << u);
             }
-            int conn = 1;
-            for (int i = 0; i < n && conn; ++i) {
-                int u = i;
-                while (s_parent[u] != u) u = s_parent[u];
-                if (u != 0) conn = 0;
+            __syncthreads();
+            const unsigned long long nb = tid < n ? s_nb[tid] : 0ull;
+            unsigned long long reached = 1ull;
+            for (int step = 0; step < n; ++step) {
+                unsigned long long add = ((reached >> tid) & 1ull) ? nb : 0ull;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) add |= __shfl_xor(add, off);
+                const unsigned long long nr = reached | add;
+                if (nr == reached) break;
+                reached = nr;
             }
-            s_flag = conn;
+            const unsigned long long all = n == 64 ? ~0ull : ((1ull << n) - 1ull);
+            if (tid == 0) s_flag = reached == all;
         }
         __syncthreads();
         if (!s_flag) {
@@ -591,21 +607,28 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
             break;
         }
         // reduced Laplacian (nodes 1..n-1) and right-hand side; lane i owns row i
+        if (pre)
+            for (int i = tid; i < n1 * n1; i += kMinThreads) L[i] = a.lfull[i];
         for (int i = tid; i < n1; i += kMinThreads) {
-            for (int j = 0; j < n1; ++j) L[i * n1 + j] = 0.0;
+            if (!pre)
+                for (int j = 0; j < n1; ++j) L[i * n1 + j] = 0.0;
             double bx = 0.0, by = 0.0;
             const int node = i + 1;
             for (int k = 0; k < m; ++k) {
                 if (!active[k]) continue;
                 int ra = a.rows[2 * k], rb = a.rows[2 * k + 1];
                 if (rb == node) {
-                    L[i * n1 + i] += 1.0;
-                    if (ra != 0) L[i * n1 + ra - 1] -= 1.0;
+                    if (!pre) {
+                        L[i * n1 + i] += 1.0;
+                        if (ra != 0) L[i * n1 + ra - 1] -= 1.0;
+                    }
                     bx += d[2 * k];
                     by += d[2 * k + 1];
                 } else if (ra == node) {
-                    L[i * n1 + i] += 1.0;
-                    if (rb != 0) L[i * n1 + rb - 1] -= 1.0;
+                    if (!pre) {
+                        L[i * n1 + i] += 1.0;
+                        if (rb != 0) L[i * n1 + rb - 1] -= 1.0;
+                    }
                     bx -= d[2 * k];
                     by -= d[2 * k + 1];
                 }
@@ -614,8 +637,8 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
             rhs[2 * i + 1] = by;
         }
         __syncthreads();
-        // Cholesky, lower triangle in place
-        for (int j = 0; j < n1; ++j) {
+        // Cholesky, lower triangle in place (the same steps as minimize_full_factor on the host)
+        for (int j = 0; j < n1 && !pre; ++j) {
             if (tid == 0) L[j * n1 + j] = sqrt(L[j * n1 + j]);
             __syncthreads();
             const double djj = L[j * n1 + j];
@@ -627,22 +650,29 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
             }
             __syncthreads();
         }
-        // forward / backward substitution, one lane per component
-        if (tid < 2) {
-            const int c = tid;
-            for (int i = 0; i < n1; ++i) {
-                double s = rhs[2 * i + c];
-                for (int k = 0; k < i; ++k) s -= L[i * n1 + k] * rhs[2 * k + c];
-                rhs[2 * i + c] = s / L[i * n1 + i];
+        // forward substitution L y = rhs, one column at a time: y_i, then every later row subtracts
+        // L_ri y_i (each row sees its subtractions in increasing i, as a row-by-row solve)
+        for (int i = 0; i < n1; ++i) {
+            if (tid < 2) rhs[2 * i + tid] = rhs[2 * i + tid] / L[i * n1 + i];
+            __syncthreads();
+            for (int e = tid; e < 2 * (n1 - 1 - i); e += kMinThreads) {
+                const int r = i + 1 + (e >> 1), c = e & 1;
+                rhs[2 * r + c] -= L[r * n1 + i] * rhs[2 * i + c];
             }
-            for (int i = n1 - 1; i >= 0; --i) {
-                double s = rhs[2 * i + c];
-                for (int k = i + 1; k < n1; ++k) s -= L[k * n1 + i] * rhs[2 * k + c];
-                rhs[2 * i + c] = s / L[i * n1 + i];
-            }
-            x[c] = 0.0;
-            for (int i = 0; i < n1; ++i) x[2 * (i + 1) + c] = rhs[2 * i + c];
+            __syncthreads();
         }
+        // backward substitution L^T x = y, one column at a time from the last
+        for (int i = n1 - 1; i >= 0; --i) {
+            if (tid < 2) rhs[2 * i + tid] = rhs[2 * i + tid] / L[i * n1 + i];
+            __syncthreads();
+            for (int e = tid; e < 2 * i; e += kMinThreads) {
+                const int r = e >> 1, c = e & 1;
+                rhs[2 * r + c] -= L[i * n1 + r] * rhs[2 * i + c];
+            }
+            __syncthreads();
+        }
+        if (tid < 2) x[tid] = 0.0;
+        for (int e = tid; e < 2 * n1; e += kMinThreads) x[2 + e] = rhs[e];
         __syncthreads();
         // largest residual above 1 px^2 (first index on ties, as the sequential scan)
         double best = 1.0;
@@ -654,23 +684,23 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
             double dist = ex * ex + ey * ey;
             if (dist > best) { best = dist; bi = k; }
         }
-        red_v[tid] = best;
-        red_i[tid] = bi;
-        __syncthreads();
+        // wave-wide arg-max (largest residual; the smallest row index among equal residuals)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (oi >= 0 && (bi < 0 || ob > best || (ob == best && oi < bi))) {
+                best = ob;
+                bi = oi;
+            }
+        }
         if (tid == 0) {
-            double bv = 1.0;
-            int idx = -1;
-            for (int l = 0; l < kMinThreads; ++l) {
-                int li = red_i[l];
-                if (li < 0) continue;
-                if (red_v[l] > bv || (red_v[l] == bv && idx >= 0 && li < idx)) { bv = red_v[l]; idx = li; }
+            if (bi >= 0) {
+                d[2 * bi] = 0.0;
+                d[2 * bi + 1] = 0.0;
+                active[bi] = 0;
             }
-            if (idx >= 0) {
-                d[2 * idx] = 0.0;
-                d[2 * idx + 1] = 0.0;
-                active[idx] = 0;
-            }
-            s_flag = idx;
+            s_flag = bi;
         }
         __syncthreads();
         status = s_flag;
@@ -844,14 +874,57 @@ int launch_upsample_all(const float* in, float* out, size_t meas_stride, int m, 
     return check_launch("upsample_shifts_all");
 }
 
+// The first iteration's normal matrix, factored on the host: the reduced Laplacian of all m rows
+// and its Cholesky factor, with the same operations in the same order as k_minimize (so the factor
+// is the one every tile would compute). Returns false when the graph is disconnected.
+bool minimize_full_factor(const int* rows, int n, int m, double* L) {
+    const int n1 = n - 1;
+    std::vector<int> parent(n);
+    std::iota(parent.begin(), parent.end(), 0);
+    auto root = [&](int u) {
+        while (parent[u] != u) u = parent[u];
+        return u;
+    };
+    for (int k = 0; k < m; ++k) {
+        int u = root(rows[2 * k]), v = root(rows[2 * k + 1]);
+        if (u != v) parent[std::max(u, v)] = std::min(u, v);
+    }
+    for (int i = 0; i < n; ++i)
+        if (root(i) != 0) return false;
+    for (int i = 0; i < n1 * n1; ++i) L[i] = 0.0;
+    for (int i = 0; i < n1; ++i) {
+        const int node = i + 1;
+        for (int k = 0; k < m; ++k) {
+            const int ra = rows[2 * k], rb = rows[2 * k + 1];
+            if (rb == node) {
+                L[i * n1 + i] += 1.0;
+                if (ra != 0) L[i * n1 + ra - 1] -= 1.0;
+            } else if (ra == node) {
+                L[i * n1 + i] += 1.0;
+                if (rb != 0) L[i * n1 + rb - 1] -= 1.0;
+            }
+        }
+    }
+    for (int j = 0; j < n1; ++j) {
+        L[j * n1 + j] = std::sqrt(L[j * n1 + j]);
+        const double djj = L[j * n1 + j];
+        for (int i = j + 1; i < n1; ++i) L[i * n1 + j] /= djj;
+        for (int i = j + 1; i < n1; ++i) {
+            const double lij = L[i * n1 + j];
+            for (int c = j + 1; c <= i; ++c) L[i * n1 + c] -= lij * L[c * n1 + j];
+        }
+    }
+    return true;
+}
+
 int launch_minimize(float* meas, size_t meas_stride, float* one_to_one, int* status, int* iters, const int* rows,
-                    int tiles, int n, int m, hipStream_t s) {
+                    const double* lfull, int tiles, int n, int m, hipStream_t s) {
     size_t smem = minimize_smem(n, m);
     if (n > kMinFrames || smem > 160 * 1024) {
         set_error("minimize: %d frames / %d measurements exceed the supported size", n, m);
         return ISA_ERR_UNSUPPORTED;
     }
-    MinArgs a{meas, meas_stride, one_to_one, status, iters, rows, tiles, n, m};
+    MinArgs a{meas, meas_stride, one_to_one, status, iters, rows, lfull, tiles, n, m};
     k_minimize<<<tiles, kMinThreads, smem, s>>>(a);
     return check_launch("minimize_shifts");
 }
@@ -1044,13 +1117,22 @@ int isa_minimize_shifts(float* measured2_dev, int meas_pitch, float* one_to_one2
     ISA_REQUIRE(m > 0 && m <= 4096, "isa_minimize_shifts: invalid strategy / frame count");
     isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, rows.data());
     hipStream_t s = isa::to_stream(stream);
+    // rows, then (for a connected graph) the all-rows Cholesky factor, in one upload
+    const int n1 = frame_count - 1;
+    const size_t rows_bytes = ((size_t)2 * m * sizeof(int) + 15) & ~(size_t)15;
+    std::vector<char> host(rows_bytes + (size_t)n1 * n1 * sizeof(double));
+    std::memcpy(host.data(), rows.data(), 2 * m * sizeof(int));
+    const bool have_factor =
+        frame_count <= kMinFrames &&
+        isa::minimize_full_factor(rows.data(), frame_count, m, (double*)(host.data() + rows_bytes));
     int* d_rows = nullptr;
-    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d_rows, 2 * m * sizeof(int), s), "isa_minimize_shifts: allocation"));
-    int rc = isa::check_hip(hipMemcpyAsync(d_rows, rows.data(), 2 * m * sizeof(int), hipMemcpyHostToDevice, s),
+    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d_rows, host.size(), s), "isa_minimize_shifts: allocation"));
+    int rc = isa::check_hip(hipMemcpyAsync(d_rows, host.data(), host.size(), hipMemcpyHostToDevice, s),
                             "isa_minimize_shifts: upload");
+    const double* d_factor = have_factor ? (const double*)((char*)d_rows + rows_bytes) : nullptr;
     if (rc == ISA_OK)
         rc = isa::launch_minimize(measured2_dev, meas_pitch / sizeof(float), one_to_one2_dev, status_dev, iters_dev,
-                                  d_rows, tile_count, frame_count, m, s);
+                                  d_rows, d_factor, tile_count, frame_count, m, s);
     int rs = isa::check_hip(hipStreamSynchronize(s), "isa_minimize_shifts: sync");  // `rows` must outlive the upload
     hipError_t fe = hipFreeAsync(d_rows, s);
     if (rc == ISA_OK) rc = rs;

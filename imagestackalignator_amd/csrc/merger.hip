@@ -29,8 +29,9 @@ int launch_track_level(const float* const* frames, int img_pitch, int w, int h, 
                        size_t meas_stride, float threshold, float* shift_images, hipStream_t s);
 int launch_upsample_all(const float* in, float* out, size_t meas_stride, int m, int ol, int nl, int ocx, int ocy,
                         int ncx, int ncy, int ot, int nt, hipStream_t s);
+bool minimize_full_factor(const int* rows, int n, int m, double* L);
 int launch_minimize(float* meas, size_t meas_stride, float* one_to_one, int* status, int* iters, const int* rows,
-                    int tiles, int n, int m, hipStream_t s);
+                    const double* lfull, int tiles, int n, int m, hipStream_t s);
 int launch_optimal_shifts(float* out, int pitch, const float* r, int n, int cx, int cy, int ref, int trk,
                           hipStream_t s);
 int launch_derivatives2(const float* img, int img_pitch, float* ix, float* iy, int pitch, int w, int h, int quant,
@@ -163,7 +164,7 @@ struct isa_merger {
 namespace {
 
 struct TabLayout {
-    size_t pa, pb, pk, rows, base, frames, xframes, vframes, acc_raw, acc_mask, acc_flow, chunks, bytes;
+    size_t pa, pb, pk, rows, lfull, base, frames, xframes, vframes, acc_raw, acc_mask, acc_flow, chunks, bytes;
 };
 
 TabLayout tab_layout(int m, int n, int L) {
@@ -174,6 +175,7 @@ TabLayout tab_layout(int m, int n, int L) {
     t.pb = o; o += al(m * sizeof(int));
     t.pk = o; o += al(m * sizeof(int));
     t.rows = o; o += al(2 * m * sizeof(int));
+    t.lfull = o; o += al((size_t)(n - 1) * (n - 1) * sizeof(double));
     t.base = o; o += al((size_t)L * n * 4 * sizeof(float));
     t.frames = o; o += al((size_t)L * n * sizeof(void*));
     t.xframes = o; o += al((size_t)L * n * sizeof(void*));
@@ -389,6 +391,8 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         pk[p] = index[(size_t)pa[p] * n + pb[p]];
     }
     std::memcpy(th + tl.rows, rows.data(), 2 * m * sizeof(int));
+    // the minimizer's first iteration (all rows active) is the same for every tile: factor it here
+    const bool have_factor = isa::minimize_full_factor(rows.data(), n, m, (double*)(th + tl.lfull));
     const int n_chunks = isa::track_chunks(pa, m, n, (int*)(th + tl.chunks));
     float* base = (float*)(th + tl.base);
     const float** fr = (const float**)(th + tl.frames);
@@ -433,6 +437,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const int* d_pb = (const int*)(M->tab + tl.pb);
     const int* d_pk = (const int*)(M->tab + tl.pk);
     const int* d_rows = (const int*)(M->tab + tl.rows);
+    const double* d_lfull = have_factor ? (const double*)(M->tab + tl.lfull) : nullptr;
     const float* d_base = (const float*)(M->tab + tl.base);
     const int* d_chunks = (const int*)(M->tab + tl.chunks);
     const unsigned* const* d_xframes = (const unsigned* const*)(M->tab + tl.xframes);
@@ -495,8 +500,8 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                            M->cy[i], d_pa, d_pb, d_pk, d_chunks, n_chunks, d_base + (size_t)i * n * 4,
                                            M->shifts, mstride, P.matching_threshold, nullptr, s));
         (void)hipEventRecord(M->lev_ev[i][2], s);
-        ISA_TRY(isa::launch_minimize(M->shifts, mstride, M->one_to_one, nullptr, nullptr, d_rows, M->cx[i] * M->cy[i], n,
-                                     m, s));
+        ISA_TRY(isa::launch_minimize(M->shifts, mstride, M->one_to_one, nullptr, nullptr, d_rows, d_lfull,
+                                     M->cx[i] * M->cy[i], n, m, s));
         (void)hipEventRecord(M->lev_ev[i][3], s);
     }
     (void)hipEventRecord(M->ev[2], s);
