@@ -327,6 +327,37 @@ __global__ void k_erode4(const float* in, int in_pitch, float* out, int out_pitc
     for (int c = 0; c < 4; ++c) o[c] = m[c];
 }
 
+// The same erosion (replicate border), separable (a min over the window's rows of mins over its
+// columns is the min over the window; fminf ignores NaN in any order) and staged in LDS.
+constexpr int ER_BX = 32, ER_BY = 8, ER_RMAX = 7;
+__global__ __launch_bounds__(ER_BX* ER_BY) void k_erode_staged(const float* in, int in_pitch, float* out, int out_pitch,
+                                                               int w, int h, int size) {
+    __shared__ float4 s_in[ER_BY + 2 * ER_RMAX][ER_BX + 2 * ER_RMAX];
+    __shared__ float4 s_h[ER_BY + 2 * ER_RMAX][ER_BX];
+    const int r = size / 2, lo = -r, hi = size - 1 - r;  // window offsets lo..hi (as k_erode4)
+    const int x0 = blockIdx.x * ER_BX, y0 = blockIdx.y * ER_BY, tx = threadIdx.x, ty = threadIdx.y;
+    const int rows = ER_BY + hi - lo, cols = ER_BX + hi - lo;
+    for (int ly = ty; ly < rows; ly += ER_BY) {
+        const float4* row = (const float4*)row_ptr(in, in_pitch, clampi(y0 + lo + ly, 0, h - 1));
+        for (int lx = tx; lx < cols; lx += ER_BX) s_in[ly][lx] = row[clampi(x0 + lo + lx, 0, w - 1)];
+    }
+    __syncthreads();
+    auto min4 = [](float4 m, float4 v) {
+        return make_float4(fminf(m.x, v.x), fminf(m.y, v.y), fminf(m.z, v.z), fminf(m.w, v.w));
+    };
+    for (int ly = ty; ly < rows; ly += ER_BY) {
+        float4 m = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+        for (int k = 0; k <= hi - lo; ++k) m = min4(m, s_in[ly][tx + k]);
+        s_h[ly][tx] = m;
+    }
+    __syncthreads();
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h) return;
+    float4 m = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+    for (int k = 0; k <= hi - lo; ++k) m = min4(m, s_h[ty + k][tx]);
+    ((float4*)row_ptr(out, out_pitch, y))[x] = m;
+}
+
 __global__ void k_apply_weighting(float* inout, const float* sum, const float* weight, int w, int h, int pitch,
                                   float thr) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -551,10 +582,16 @@ int isa_erode_border4(const float* in4_dev, int in_pitch, float* out4_dev, int o
                       int size, isa_stream_t stream) {
     ISA_REQUIRE(width > 0 && height > 0 && size >= 1 && size <= 41 && in4_dev && out4_dev,
                 "isa_erode_border4: bad arguments");
+    if (size / 2 <= ER_RMAX && size - 1 - size / 2 <= ER_RMAX) {
+        k_erode_staged<<<dim3(isa::div_up(width, ER_BX), isa::div_up(height, ER_BY)), dim3(ER_BX, ER_BY), 0,
+                         isa::to_stream(stream)>>>(in4_dev, in_pitch, out4_dev, out_pitch, width, height, size);
+        return isa::check_launch("isa_erode_border4");
+    }
     k_erode4<<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0, isa::to_stream(stream)>>>(
         in4_dev, in_pitch, out4_dev, out_pitch, width, height, size);
     return isa::check_launch("isa_erode_border4");
 }
+
 
 int isa_apply_weighting(float* inout3_dev, const float* sum3_dev, const float* weight3_dev, int width, int height,
                         int pitch, float threshold, isa_stream_t stream) {

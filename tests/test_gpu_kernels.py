@@ -305,6 +305,13 @@ def test_subsample_robustness_erode(isa, oracle, gpu):
         eref = np.zeros_like(mref)
         oracle.fn("erode")(oracle.ptr(mref), oracle.ptr(eref), hw, hh, 5)
         np.testing.assert_array_equal(host(er), eref)
+        # window sizes of the staged (separable) form and of the direct form
+        for size in (3, 7, 17):
+            er = empty((hh, hw, 4))
+            isa.call("isa_erode_border4", p(dev(mref)), hw * 16, p(er), hw * 16, hw, hh, size, stream())
+            eref = np.zeros_like(mref)
+            oracle.fn("erode")(oracle.ptr(mref), oracle.ptr(eref), hw, hh, size)
+            np.testing.assert_array_equal(host(er), eref)
 
 
 @pytest.mark.parametrize("cfa", [RGGB, GRBG])

@@ -89,3 +89,33 @@ def test_merge_is_deterministic_and_reusable(isa, gpu):
     np.testing.assert_array_equal(a, b)
     st = m.stage_ms()
     assert set(st) >= {"tracking_images", "track_patches", "accumulate"}
+
+
+def test_merger_accumulation_is_bit_exact_on_its_own_inputs(isa, oracle, gpu):
+    """The merger's frame accumulation (all frames in one pass) equals the
+    oracle's accumulateImages called frame by frame, reference first, on the merger's own flows,
+    eroded masks and kernel parameters."""
+    w, h, n = 256, 192, 5
+    cfg = isa.default_synth(seed=515, width=w, height=h, frames=n, global_shift_max=5.0, occluded_fraction=0.05)
+    raw_t, _ = isa.synth_burst(cfg)
+    raw = host(raw_t).view(np.uint16)
+    params = isa.default_params(levels=[(1, 32, 2), (2, 32, 4)])
+    m = isa.BurstMerger(w, h, n, params)
+    m.run(raw_t)
+    ref = m.reference_index
+    hw, hh = w // 2, h // 2
+    kern = np.ascontiguousarray(host(m.buffer_tensor(5, 0, 3)))
+    s = np.zeros((h, w, 3), np.float32)
+    wt = np.zeros((h, w, 3), np.float32)
+    zero = np.zeros((h, w, 2), np.float32)
+    order = [ref] + [f for f in range(n) if f != ref]
+    for f in order:
+        mask4 = np.ascontiguousarray(host(m.buffer_tensor(4, f, 4)))
+        flow = zero if f == ref else np.ascontiguousarray(host(m.buffer_tensor(3, f, 2)))
+        oracle.fn("accumulate")(oracle.ptr(np.ascontiguousarray(raw[f])), oracle.ptr(s), oracle.ptr(wt),
+                                oracle.ptr(mask4), oracle.ptr(kern), oracle.ptr(flow),
+                                oracle.ptr(oracle.arr(list(params.white), np.float32)),
+                                oracle.ptr(oracle.arr(list(params.black), np.float32)), w, h,
+                                oracle.ptr(oracle.arr(list(params.cfa), np.int32)))
+    np.testing.assert_array_equal(host(m.buffer_tensor(6, 0, 3)), s)
+    np.testing.assert_array_equal(host(m.buffer_tensor(7, 0, 3)), wt)
