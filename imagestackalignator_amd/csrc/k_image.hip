@@ -141,6 +141,12 @@ __global__ void k_gamma(float* img, int w, int h, int pitch) {
     p[2] = srgb_prepared(p[2]);
 }
 
+template <bool FAST>
+__global__ void k_srgb_1d(const float* in, float* out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = srgb_prepared<FAST>(in[i]);
+}
+
 __global__ void k_color_to_gray(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, float c0,
                                 float c1, float c2) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -501,6 +507,17 @@ int isa_gamma_srgb(float* img3_dev, int width, int height, int pitch, isa_stream
     k_gamma<<<dim3(isa::div_up(width, 16), isa::div_up(height, 16)), dim3(16, 16), 0, isa::to_stream(stream)>>>(
         img3_dev, width, height, pitch);
     return isa::check_launch("isa_gamma_srgb");
+}
+
+int isa_srgb_oetf(const float* in_dev, float* out_dev, size_t count, int exact, isa_stream_t stream) {
+    ISA_REQUIRE(in_dev && out_dev, "isa_srgb_oetf: bad arguments");
+    if (count == 0) return ISA_OK;
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    if (exact)
+        k_srgb_1d<false><<<blocks, 256, 0, isa::to_stream(stream)>>>(in_dev, out_dev, count);
+    else
+        k_srgb_1d<true><<<blocks, 256, 0, isa::to_stream(stream)>>>(in_dev, out_dev, count);
+    return isa::check_launch("isa_srgb_oetf");
 }
 
 int isa_color_to_gray(const float* in3_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height,

@@ -153,6 +153,10 @@ int isa_apply_weighting(float* inout3_dev, const float* sum3_dev, const float* w
                         int pitch, float threshold, isa_stream_t stream);
 /* GammasRGB, R:Kernels/kernel.cu:271-313; wrapper R:PEFStudioDX/BasicKernels.cs:243-258. */
 int isa_gamma_srgb(float* img3_dev, int width, int height, int pitch, isa_stream_t stream);
+/* The OETF of GammasRGB (NaN -> 0, clamp to [0, 1], sRGB curve) on `count` floats: exact = 1 evaluates the
+ * power as pow in double rounded once to float, exact = 0 through the certified fast form the debayer
+ * and gamma kernels use (the same floats). */
+int isa_srgb_oetf(const float* in_dev, float* out_dev, size_t count, int exact, isa_stream_t stream);
 /* NPP ColorToGray (R:...Controller.cs:2404-2408): gray = c0 R + c1 G + c2 B. */
 int isa_color_to_gray(const float* in3_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height,
                       const float coef[3], isa_stream_t stream);

@@ -402,3 +402,32 @@ def test_accumulate_apply_weighting(isa, oracle, gpu, sr):
     ioref = io.copy()
     oracle.fn("apply_weighting")(oracle.ptr(ioref), oracle.ptr(sr_), oracle.ptr(wr_), w, h, 1.0)
     np.testing.assert_array_equal(host(dio), ioref)
+
+
+def test_srgb_fast_power_is_exact(isa, oracle, gpu):
+    """The certified fast power of the sRGB OETF returns the float of the exact one (pow in double,
+    rounded once) for every float of the power branch (0.0031308, 1], and matches the oracle."""
+    import torch
+
+    lo = int(np.float32(0.0031308).view(np.int32)) - 4
+    hi = int(np.float32(1.0).view(np.int32)) + 4
+    print(f"srgb exhaustive over {hi - lo + 1} floats", flush=True)
+    n_all = hi - lo + 1
+    chunk = 1 << 24
+    for c0 in range(lo, hi + 1, chunk):
+        bits = torch.arange(c0, min(hi + 1, c0 + chunk), dtype=torch.int32, device="cuda")
+        x = bits.view(torch.float32)
+        fast, exact = torch.empty_like(x), torch.empty_like(x)
+        isa.call("isa_srgb_oetf", p(x), p(fast), C.c_size_t(x.numel()), 0, stream())
+        isa.call("isa_srgb_oetf", p(x), p(exact), C.c_size_t(x.numel()), 1, stream())
+        torch.cuda.synchronize()
+        assert torch.equal(fast.view(torch.int32), exact.view(torch.int32)), c0
+    assert n_all > 7 * 10**7
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(0, 1, 100000), rng.uniform(-0.5, 1.5, 1000),
+                         [np.nan, 0.0, -0.0, 1.0, 0.0031308, np.float32(0.0031308) * 1.0000001, 2.0, -1.0]]).astype(np.float32)
+    d_out = empty(xs.shape)
+    isa.call("isa_srgb_oetf", p(dev(xs)), p(d_out), C.c_size_t(xs.size), 0, stream())
+    img = np.ascontiguousarray(np.repeat(xs[:, None], 3, axis=1))
+    oracle.fn("gamma_srgb")(xs.size, 1, oracle.ptr(img))
+    np.testing.assert_array_equal(host(d_out), img[:, 0])
