@@ -106,70 +106,6 @@ __device__ __forceinline__ float cos_r(float x) { return (float)cos((double)x); 
 // Per-pixel formulas shared by the unfused (one-kernel-per-reference-kernel)
 // and fused entry points, so that both evaluate identical operations.
 
-// pow_r(x, y) for a normal x in (0, 1] and |y| <= 1, cheaper and returning the same float: the
-// power is formed in double as exp2(y log2 x), log by the atanh series in u = (m-1)/(m+1)
-// (m in [1/sqrt2, sqrt2), terms to u^15) and exp by its Taylor series to g^12 (|g| <= ln2/2), a
-// relative error below 2^-45. When that double lies farther than 2^-40 (relative) from every
-// rounding midpoint of its float, the true power and pow_r's double (within 2^-52 of it) both
-// round to that float; otherwise (about 2^-15 of calls) pow_r itself is evaluated.
-__device__ __forceinline__ float pow_cert(float x, float y) {
-    int e;
-    double m = frexp((double)x, &e);  // x = m 2^e, m in [0.5, 1)
-    if (m < 0.70710678118654752440) {
-        m *= 2.0;
-        e -= 1;
-    }
-    const double den = m + 1.0;
-    double r = __builtin_amdgcn_rcp(den);
-    r = fma(r, fma(-den, r, 1.0), r);
-    r = fma(r, fma(-den, r, 1.0), r);
-    const double u = (m - 1.0) * r, u2 = u * u;
-    double p = 1.0 / 15.0;
-    p = fma(p, u2, 1.0 / 13.0);
-    p = fma(p, u2, 1.0 / 11.0);
-    p = fma(p, u2, 1.0 / 9.0);
-    p = fma(p, u2, 1.0 / 7.0);
-    p = fma(p, u2, 1.0 / 5.0);
-    p = fma(p, u2, 1.0 / 3.0);
-    p = fma(p, u2, 1.0);
-    const double l2 = (double)e + (2.0 * u * p) * 1.44269504088896340736;  // log2 x
-    const double t = (double)y * l2;
-    const double k = rint(t);
-    const double g = (t - k) * 0.69314718055994530942;
-    double q = 1.0 / 479001600.0;  // 1 / 12!
-    q = fma(q, g, 1.0 / 39916800.0);
-    q = fma(q, g, 1.0 / 3628800.0);
-    q = fma(q, g, 1.0 / 362880.0);
-    q = fma(q, g, 1.0 / 40320.0);
-    q = fma(q, g, 1.0 / 5040.0);
-    q = fma(q, g, 1.0 / 720.0);
-    q = fma(q, g, 1.0 / 120.0);
-    q = fma(q, g, 1.0 / 24.0);
-    q = fma(q, g, 1.0 / 6.0);
-    q = fma(q, g, 0.5);
-    q = fma(q, g, 1.0);
-    q = fma(q, g, 1.0);
-    const double z = ldexp(q, (int)k);
-    const float f = (float)z;
-    int fe;
-    const float fm = frexpf(f, &fe);  // f = fm 2^fe, fm in [0.5, 1)
-    const double ulp = ldexp(1.0, fe - 24);
-    // half the gap to the neighbouring float on z's side (the gap below a power of two is halved)
-    const double half = (fm == 0.5f && z < (double)f) ? 0.25 * ulp : 0.5 * ulp;
-    if (fabs(z - (double)f) < half - 0x1p-40 * z) return f;
-    return pow_r(x, y);
-}
-
-// sRGB OETF after NaN->0 and clamp to [0,1] (R:Kernels/kernel.cu:271-313). The power is pow_r's
-// float (FAST evaluates it through pow_cert).
-template <bool FAST = true>
-__device__ __forceinline__ float srgb_prepared(float v) {
-    if (isnan(v)) v = 0.0f;
-    v = fmaxf(fminf(v, 1.0f), 0.0f);
-    if (v <= 0.0031308f) return 12.92f * v;
-    return (1.0f + 0.055f) * (FAST ? pow_cert(v, 1.0f / 2.4f) : pow_r(v, 1.0f / 2.4f)) - 0.055f;
-}
-
 // Accumulation weight of offset (px, py) for inverse covariance (k0, k1, k2)
 // = (Omega11, Omega22, Omega12); non-finite -> cross fallback
 // (R:Kernels/DeBayerKernels.cu:335-338).

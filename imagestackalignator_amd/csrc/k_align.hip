@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "isa_device.h"
@@ -271,7 +272,6 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
     // block origins in the level image
     const int rx0 = tx * T + M + rox, ry0 = ty * T + M + roy;  // reference interior
     const int tx0 = tx * T + tox, ty0 = ty * T + toy;          // track block
-    auto rowp = [&](const float* img, int gy) { return row_ptr(img, a.img_pitch, clampi(gy, 0, a.h - 1)); };
     auto colc = [&](int gx) { return clampi(gx, 0, a.w - 1); };
     // 1. cross terms, reference column energy and the track column energies, in one pass over y.
     // Lane (g, x) keeps, besides its RPG x S cross terms, the energy chains of its first window
@@ -279,85 +279,112 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
     // the last group with x > T - RPG, which own the track columns no first column reaches).
     const bool inside = rx0 >= 0 && rx0 + T <= a.w && ry0 >= 0 && ry0 + T <= a.h && tx0 >= 0 && tx0 + B <= a.w &&
                         ty0 >= 0 && ty0 + B <= a.h;
-    const size_t pitchf = (size_t)a.img_pitch / sizeof(float);
-    const int rgx = colc(rx0 + x);
-    int tgx[RPG];
-#pragma unroll
-    for (int q = 0; q < RPG; ++q) tgx[q] = colc(tx0 + x + g * RPG + q);
-    // row pointers: interior tiles index directly, boundary tiles clamp the row
-    auto rrow = [&](int y) { return inside ? ia + (size_t)(ry0 + y) * pitchf : rowp(ia, ry0 + y); };
-    auto trow = [&](int j) { return inside ? ib + (size_t)(ty0 + j) * pitchf : rowp(ib, ty0 + j); };
+    // Addresses: a lane's columns are held as global pointers and a row adds y * pitch elements for a
+    // uniform y (a scalar product). Waves whose items all lie inside the image use the unclamped
+    // addresses (the RPG track columns are then consecutive); a wave holding a boundary item clamps
+    // every column once and every row offset into [0, (h-1) pitch].
+    typedef const __attribute__((address_space(1))) float* gptr;
+    const int pf = a.img_pitch / (int)sizeof(float), maxoff = (a.h - 1) * pf;
+    int roff0 = ry0 * pf, toff0 = ty0 * pf;
+    // opaque to the optimiser: the row offset stays roff0 + y * pitch instead of being refactored
+    // into a per-lane (ry0 + y) * pitch multiply
+    asm("" : "+v"(roff0));
+    asm("" : "+v"(toff0));
     double acc[RPG][S], win[RPG][S];
     double r2 = 0.0, tt0 = 0.0, tt1 = 0.0;
 #pragma unroll
     for (int q = 0; q < RPG; ++q)
 #pragma unroll
         for (int i = 0; i < S; ++i) acc[q][i] = 0.0;
+    auto pass = [&](auto clamp_rows) {
+        constexpr bool CL = decltype(clamp_rows)::value;
+        // rv(y): reference value at row y; tv(j, q): track value at row j, column q of the lane
+        gptr rcol, tcol[RPG];
+        if (CL) {
+            rcol = (gptr)ia + colc(rx0 + x);
 #pragma unroll
-    for (int i = 0; i < S - 1; ++i) {
-        const float* tr = trow(i);
+            for (int q = 0; q < RPG; ++q) tcol[q] = (gptr)ib + colc(tx0 + x + g * RPG + q);
+        } else {
+            rcol = (gptr)ia + (rx0 + x) + roff0;
+            tcol[0] = (gptr)ib + (tx0 + x + g * RPG) + toff0;
 #pragma unroll
-        for (int q = 0; q < RPG; ++q) win[q][i] = (double)tr[tgx[q]];
-    }
-    // operands of S steps are fetched as one block
-    float nr[S], nt[S][RPG];
-    auto fetch = [&](int y0s) {
-#pragma unroll
-        for (int kk = 0; kk < S; ++kk) {
-            const int y = min(y0s + kk, T - 1);  // past the end: a valid address, value unused
-            nr[kk] = rrow(y)[rgx];
-            const float* tr = trow(y + S - 1);
-#pragma unroll
-            for (int q = 0; q < RPG; ++q) nt[kk][q] = tr[tgx[q]];
+            for (int q = 1; q < RPG; ++q) tcol[q] = tcol[0] + q;
         }
-    };
-    // (one block ahead only for M = 1; for larger M the prefetch registers would cost occupancy)
-    constexpr bool kAhead = M <= 1;
-    if (kAhead) fetch(0);
-    for (int y = 0; y < T; y += S) {
-        float cr[S], ct[S][RPG];
-        if (!kAhead) fetch(y);
+        auto rv = [&](int y) -> float {
+            if (CL) return rcol[(unsigned)min(max(roff0 + y * pf, 0), maxoff)];
+            return rcol[y * pf];
+        };
+        auto trow = [&](int j) -> int {  // element offset of track row j from tcol[q]
+            if (CL) return (int)(unsigned)min(max(toff0 + j * pf, 0), maxoff);
+            return j * pf;
+        };
+        auto tv = [&](int o, int q) -> float { return CL ? tcol[q][(unsigned)o] : (tcol[0] + o)[q]; };
 #pragma unroll
-        for (int kk = 0; kk < S; ++kk) {
-            cr[kk] = nr[kk];
+        for (int i = 0; i < S - 1; ++i) {
+            const int o = trow(i);
 #pragma unroll
-            for (int q = 0; q < RPG; ++q) ct[kk][q] = nt[kk][q];
+            for (int q = 0; q < RPG; ++q) win[q][i] = (double)tv(o, q);
         }
-        if (kAhead && y + S < T) fetch(y + S);
+        // operands of S steps are fetched as one block
+        float nr[S], nt[S][RPG];
+        auto fetch = [&](int y0s) {
 #pragma unroll
-        for (int kk = 0; kk < S; ++kk) {
-            if (y + kk < T) {
-                const double rv = (double)cr[kk];
-                r2 = fma(rv, rv, r2);
+            for (int kk = 0; kk < S; ++kk) {
+                const int y = min(y0s + kk, T - 1);  // past the end: a valid address, value unused
+                nr[kk] = rv(y);
+                const int o = trow(y + S - 1);
 #pragma unroll
-                for (int q = 0; q < RPG; ++q) win[q][(kk + S - 1) % S] = (double)ct[kk][q];
-                tt0 = fma(win[0][kk], win[0][kk], tt0);
-                tt1 = fma(win[RPG - 1][kk], win[RPG - 1][kk], tt1);
+                for (int q = 0; q < RPG; ++q) nt[kk][q] = tv(o, q);
+            }
+        };
+        // (one block ahead only for M = 1; for larger M the prefetch registers would cost occupancy)
+        constexpr bool kAhead = M <= 1;
+        if (kAhead) fetch(0);
+        for (int y = 0; y < T; y += S) {
+            float cr[S], ct[S][RPG];
+            if (!kAhead) fetch(y);
 #pragma unroll
-                for (int q = 0; q < RPG; ++q)
+            for (int kk = 0; kk < S; ++kk) {
+                cr[kk] = nr[kk];
 #pragma unroll
-                    for (int i = 0; i < S; ++i) acc[q][i] = fma(rv, win[q][(kk + i) % S], acc[q][i]);
+                for (int q = 0; q < RPG; ++q) ct[kk][q] = nt[kk][q];
+            }
+            if (kAhead && y + S < T) fetch(y + S);
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) {
+                if (y + kk < T) {
+                    const double r = (double)cr[kk];
+                    r2 = fma(r, r, r2);
+#pragma unroll
+                    for (int q = 0; q < RPG; ++q) win[q][(kk + S - 1) % S] = (double)ct[kk][q];
+                    tt0 = fma(win[0][kk], win[0][kk], tt0);
+                    tt1 = fma(win[RPG - 1][kk], win[RPG - 1][kk], tt1);
+#pragma unroll
+                    for (int q = 0; q < RPG; ++q)
+#pragma unroll
+                        for (int i = 0; i < S; ++i) acc[q][i] = fma(r, win[q][(kk + i) % S], acc[q][i]);
+                }
             }
         }
-    }
-    // window energies of the other row offsets: TT[c][k+1] = (TT[c][k] - t[k][c]^2) + t[k+T][c]^2
-    {
+        // window energies of the other row offsets: TT[c][k+1] = (TT[c][k] - t[k][c]^2) + t[k+T][c]^2
         const int c0 = x + g * RPG, c1 = x + S - 1;
         const bool own1 = g == G - 1 && x > T - RPG;
-        const int gx0 = tgx[0], gx1 = tgx[RPG - 1];
         TT[c0 * S] = tt0;
         if (own1) TT[c1 * S] = tt1;
         for (int q = 0; q + 1 < S; ++q) {
-            const float* ta = trow(q);
-            const float* tb = trow(q + T);
-            double u0 = (double)ta[gx0], v0 = (double)tb[gx0];
-            double u1 = (double)ta[gx1], v1 = (double)tb[gx1];
+            const int oa = trow(q), ob = trow(q + T);
+            double u0 = (double)tv(oa, 0), v0 = (double)tv(ob, 0);
+            double u1 = (double)tv(oa, RPG - 1), v1 = (double)tv(ob, RPG - 1);
             tt0 = (tt0 - u0 * u0) + v0 * v0;
             tt1 = (tt1 - u1 * u1) + v1 * v1;
             TT[c0 * S + q + 1] = tt0;
             if (own1) TT[c1 * S + q + 1] = tt1;
         }
-    }
+    };
+    if (__all(inside))
+        pass(std::false_type{});
+    else
+        pass(std::true_type{});
     __syncthreads();  // TT complete
 #pragma unroll
     for (int q = 0; q < RPG; ++q)

@@ -9,7 +9,11 @@
 //   2. column DFT at the band ky  Y = (1-H) c_kx/(WH) . sum_y U e^{-i..}
 //   3. inverse column DFT         V = sum_j Y e^{+i..}
 //   4. inverse row DFT, epilogue  img = clamp(img - V . Tw^T, 0, 1)   (GEMM, K = 2 kxn)
+// The two GEMMs run on the FP32 matrix cores (v_mfma_f32_16x16x4_f32) when the image rows are
+// 16-byte aligned, else as a tiled FP32 SIMT GEMM.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 
 #include "isa_device.h"
 #include "isa_host.h"
@@ -40,16 +44,19 @@ __device__ __forceinline__ int band_ky(int j, int kyn, int h) {
     return hi_start + (j - lo);
 }
 
-// tw[x][2k] = cos(2 pi k x / W), tw[x][2k+1] = sin(2 pi k x / W), exact argument reduction.
-__global__ void k_row_twiddles(float* tw, int w, int kxn) {
+// tw[x][2k] = cos(2 pi k x / W), tw[x][2k+1] = sin(2 pi k x / W) (row pitch n2p), exact argument
+// reduction; twT is the transpose (row pitch twp). Padding columns / rows stay 0.
+__global__ void k_row_twiddles(float* tw, int n2p, float* twT, int twp, int w, int kxn) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int k = blockIdx.y;
     if (x >= w) return;
     double a = kTwoPi * (double)(((long long)k * x) % w) / (double)w;
     double s, c;
     sincos(a, &s, &c);
-    tw[(size_t)x * 2 * kxn + 2 * k] = (float)c;
-    tw[(size_t)x * 2 * kxn + 2 * k + 1] = (float)s;
+    tw[(size_t)x * n2p + 2 * k] = (float)c;
+    tw[(size_t)x * n2p + 2 * k + 1] = (float)s;
+    twT[(size_t)(2 * k) * twp + x] = (float)c;
+    twT[(size_t)(2 * k + 1) * twp + x] = (float)s;
 }
 
 // Transfer function of R:Kernels/kernel.cu:702-779 (float, same operation order).
@@ -160,6 +167,163 @@ __global__ void k_sum_partials(const float* __restrict__ part, float* __restrict
     out[i] = s;
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Operand feed of both MFMA GEMMs: lane (j, q) = (lane & 15, lane >> 4) loads A[r + j][k + 4q .. 4q+3]
+// and B[k + 4q .. 4q+3][c + j] as float4 runs and issues four 16x16x4 products, e = 0..3, with
+// A[.][k + 4q + e] and B[k + 4q + e][.] in slot q of product e: one permutation of the K index
+// shared by both operands, so every product A[r][k] B[k][c] enters once. The result tile holds
+// D[4q + i][j] in element i.
+__device__ __forceinline__ f32x4 mfma4(const float4& a, const float4& b, f32x4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc, 0, 0, 0);
+}
+
+constexpr int HP_WAVES = 4;  // waves per block of both MFMA GEMMs
+constexpr int HP_NTG = 10;   // 16-column tiles per wave in the row DFT
+
+// Row DFT at the low band: part[z][y][c] = sum over x in K chunk z of img[y][x] twT[c][x]
+// (c < n2p, row pitch n2p). A block takes 32 rows, 16 HP_NTG columns and one K chunk of kc; each
+// wave sums a quarter of the chunk, and the four wave sums are added through LDS in the fixed order
+// ((w3 + w2) + w1) + w0.
+__global__ __launch_bounds__(HP_WAVES * 64, 2) void k_hp_row_dft(const float* __restrict__ img, int pitch, int h, int w,
+                                                              const float* __restrict__ twT, int twp, int n2p,
+                                                              float* __restrict__ part, int kc) {
+    __shared__ f32x4 red[2 * HP_NTG][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, q = lane >> 4;
+    const int y0 = blockIdx.x * 32;
+    const int t0 = blockIdx.y * HP_NTG, nt = min(HP_NTG, n2p / 16 - t0);
+    const int kq = kc / HP_WAVES;
+    const int kbeg = blockIdx.z * kc + wave * kq, kend = min(w, kbeg + kq);
+    f32x4 acc[2][HP_NTG];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int t = 0; t < HP_NTG; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const float* ar[2];
+    bool aok[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int y = y0 + 16 * m + j;
+        aok[m] = y < h;
+        ar[m] = row_ptr(img, pitch, aok[m] ? y : 0);
+    }
+    const float* br = twT + (size_t)(16 * t0 + j) * twp;
+    for (int k = kbeg; k < kend; k += 16) {
+        const int kk = k + 4 * q;
+        float4 a[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            if (aok[m] && kk + 4 <= kend) {
+                a[m] = *reinterpret_cast<const float4*>(ar[m] + kk);
+            } else {
+                a[m].x = aok[m] && kk < kend ? ar[m][kk] : 0.0f;
+                a[m].y = aok[m] && kk + 1 < kend ? ar[m][kk + 1] : 0.0f;
+                a[m].z = aok[m] && kk + 2 < kend ? ar[m][kk + 2] : 0.0f;
+                a[m].w = 0.0f;
+            }
+        }
+        float4 b[HP_NTG];
+#pragma unroll
+        for (int t = 0; t < HP_NTG; ++t)
+            if (t < nt) b[t] = *reinterpret_cast<const float4*>(br + (size_t)16 * t * twp + kk);
+#pragma unroll
+        for (int t = 0; t < HP_NTG; ++t)
+            if (t < nt) {
+                acc[0][t] = mfma4(a[0], b[t], acc[0][t]);
+                acc[1][t] = mfma4(a[1], b[t], acc[1][t]);
+            }
+    }
+    for (int src = HP_WAVES - 1; src >= 1; --src) {
+        if (wave == src) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int t = 0; t < HP_NTG; ++t) {
+                    f32x4 v = acc[m][t];
+                    if (src != HP_WAVES - 1) v = red[m * HP_NTG + t][lane] + v;
+                    red[m * HP_NTG + t][lane] = v;
+                }
+        }
+        __syncthreads();
+    }
+    if (wave != 0) return;
+    float* out = part + (size_t)blockIdx.z * h * n2p;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int t = 0; t < HP_NTG; ++t) {
+            if (t >= nt) continue;
+            const f32x4 v = red[m * HP_NTG + t][lane] + acc[m][t];
+            const int c = 16 * (t0 + t) + j;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int y = y0 + 16 * m + 4 * q + i;
+                if (y < h) out[(size_t)y * n2p + c] = v[i];
+            }
+        }
+}
+
+// U[y][c] (pitch n2) = sum over z of part[z][y][c] (pitch n2p), in z order.
+__global__ void k_hp_sum_parts(const float* __restrict__ part, float* __restrict__ U, int h, int n2, int n2p, int nz) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (c >= n2) return;
+    float s = 0.0f;
+    for (int z = 0; z < nz; ++z) s += part[((size_t)z * h + y) * n2p + c];
+    U[(size_t)y * n2 + c] = s;
+}
+
+// Inverse row DFT and epilogue: img[y][x] = clamp(img[y][x] - sum_c V[y][c] tw[x][c], 0, 1),
+// K = n2p (V and tw zero-padded, row pitch n2p). A wave takes 64 rows x 64 columns (4 x 4 tiles);
+// a block 2 x 2 waves.
+__global__ __launch_bounds__(HP_WAVES * 64, 2) void k_hp_row_idft(float* img, int pitch, int h, int w,
+                                                               const float* __restrict__ V,
+                                                               const float* __restrict__ tw, int n2p) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, q = lane >> 4;
+    const int y0 = blockIdx.y * 128 + (wave >> 1) * 64, x0 = blockIdx.x * 128 + (wave & 1) * 64;
+    if (y0 >= h || x0 >= w) return;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const float* ar[4];
+    const float* br[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) ar[m] = V + (size_t)min(y0 + 16 * m + j, h - 1) * n2p;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) br[t] = tw + (size_t)min(x0 + 16 * t + j, w - 1) * n2p;
+    for (int k = 0; k < n2p; k += 16) {
+        const int kk = k + 4 * q;
+        float4 a[4], b[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[m] = *reinterpret_cast<const float4*>(ar[m] + kk);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const float4*>(br[t] + kk);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[m][t] = mfma4(a[m], b[t], acc[m][t]);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int y = y0 + 16 * m + 4 * q + i;
+            if (y >= h) continue;
+            float* row = row_ptr(img, pitch, y);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int x = x0 + 16 * t + j;
+                if (x >= w) continue;
+                const float v = row[x] - acc[m][t][i];
+                row[x] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+            }
+        }
+}
+
 constexpr int kRowSplit = 8;
 
 // Column twiddles for the band rows: ctab[y][j] = (cos, sin)(2 pi ky_j y / H), double.
@@ -216,9 +380,10 @@ __global__ void k_col_dft_finish(const double* __restrict__ P, double* __restric
     Y[((size_t)j * kxn + k) * 2 + 1] = si * g;
 }
 
-// Inverse column DFT: V'[y][2k] = Re, V'[y][2k+1] = -Im (so that the inverse row pass is a plain GEMM with Tw^T).
+// Inverse column DFT: V'[y][2k] = Re, V'[y][2k+1] = -Im (so that the inverse row pass is a plain GEMM
+// with Tw^T); row pitch n2p.
 __global__ void k_col_idft(const double* __restrict__ Y, const double* __restrict__ ctab, float* __restrict__ V, int h,
-                           int kxn, int ny) {
+                           int kxn, int ny, int n2p) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
     if (k >= kxn || y >= h) return;
@@ -229,13 +394,34 @@ __global__ void k_col_idft(const double* __restrict__ Y, const double* __restric
         sr += yr * c - yi * s;
         si += yr * s + yi * c;
     }
-    V[(size_t)y * 2 * kxn + 2 * k] = (float)sr;
-    V[(size_t)y * 2 * kxn + 2 * k + 1] = (float)(-si);
+    V[(size_t)y * n2p + 2 * k] = (float)sr;
+    V[(size_t)y * n2p + 2 * k + 1] = (float)(-si);
+}
+
+struct HpPlan {
+    Band b;
+    int n2, n2p, twp;  // band columns 2 kxn, padded to 16; row pitch of twT (w padded to 16)
+    int kc, nz;        // K chunk and chunk count of the MFMA row DFT
+};
+
+HpPlan plan_of(int w, int h, float hp, float hps) {
+    HpPlan P;
+    P.b = band_of(w, h, hp, hps);
+    P.n2 = 2 * P.b.kxn;
+    P.n2p = isa::div_up(P.n2, 16) * 16;
+    P.twp = isa::div_up(w, 16) * 16;
+    // about 4096 waves in the row DFT
+    const int groups = isa::div_up(h, 32) * isa::div_up(P.n2p / 16, HP_NTG);
+    const int nz = std::max(1, std::min(isa::div_up(4096, HP_WAVES * groups), isa::div_up(w, 64)));
+    P.kc = isa::div_up(isa::div_up(w, nz), 64) * 64;
+    P.nz = isa::div_up(w, P.kc);
+    return P;
 }
 
 struct HpLayout {
     float* part;
     float* tw;
+    float* twT;
     double* ctab;
     float* U;
     float* V;
@@ -244,21 +430,24 @@ struct HpLayout {
     size_t bytes;
 };
 
-HpLayout hp_layout(void* ws, int w, int h, const Band& b) {
+HpLayout hp_layout(void* ws, int w, int h, const HpPlan& pl) {
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const Band& b = pl.b;
     char* base = (char*)ws;
     size_t off = 0;
     HpLayout L;
     L.part = (float*)(base + off);
-    off += al((size_t)kRowSplit * h * 2 * b.kxn * sizeof(float));
+    off += al(std::max((size_t)kRowSplit * h * pl.n2, (size_t)pl.nz * h * pl.n2p) * sizeof(float));
     L.tw = (float*)(base + off);
-    off += al((size_t)w * 2 * b.kxn * sizeof(float));
+    off += al((size_t)w * pl.n2p * sizeof(float));
+    L.twT = (float*)(base + off);
+    off += al((size_t)pl.n2p * pl.twp * sizeof(float));
     L.ctab = (double*)(base + off);
     off += al((size_t)h * b.ny * 2 * sizeof(double));
     L.U = (float*)(base + off);
-    off += al((size_t)h * 2 * b.kxn * sizeof(float));
+    off += al((size_t)h * pl.n2 * sizeof(float));
     L.V = (float*)(base + off);
-    off += al((size_t)h * 2 * b.kxn * sizeof(float));
+    off += al((size_t)h * pl.n2p * sizeof(float));
     L.P = (double*)(base + off);
     off += al((size_t)kColChunks * b.ny * b.kxn * 2 * sizeof(double));
     L.Y = (double*)(base + off);
@@ -267,38 +456,58 @@ HpLayout hp_layout(void* ws, int w, int h, const Band& b) {
     return L;
 }
 
+__global__ void k_hp_zero(float* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
 }  // namespace
 
 namespace isa {
 
 size_t highpass_workspace(int w, int h, float hp, float hps) {
-    Band b = band_of(w, h, hp, hps);
-    return hp_layout(nullptr, w, h, b).bytes;
+    return hp_layout(nullptr, w, h, plan_of(w, h, hp, hps)).bytes;
 }
 
-// If tables_ready, the twiddle tables at the start of the workspace are reused.
+// If tables_ready, the twiddle tables (and the zero padding of V) in the workspace are reused.
 int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, hipStream_t s) {
-    Band b = band_of(w, h, hp, hps);
-    HpLayout L = hp_layout(ws, w, h, b);
-    const int n2 = 2 * b.kxn;
+    const HpPlan pl = plan_of(w, h, hp, hps);
+    const Band& b = pl.b;
+    HpLayout L = hp_layout(ws, w, h, pl);
+    const int n2 = pl.n2, n2p = pl.n2p;
     if (!tables_ready) {
-        k_row_twiddles<<<dim3(div_up(w, 256), b.kxn), 256, 0, s>>>(L.tw, w, b.kxn);
+        // tw, twT and V are contiguous in the workspace only up to alignment: clear each
+        const size_t ntw = (size_t)w * n2p, ntwt = (size_t)n2p * pl.twp, nv = (size_t)h * n2p;
+        k_hp_zero<<<(unsigned)((ntw + 255) / 256), 256, 0, s>>>(L.tw, ntw);
+        k_hp_zero<<<(unsigned)((ntwt + 255) / 256), 256, 0, s>>>(L.twT, ntwt);
+        k_hp_zero<<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(L.V, nv);
+        k_row_twiddles<<<dim3(div_up(w, 256), b.kxn), 256, 0, s>>>(L.tw, n2p, L.twT, pl.twp, w, b.kxn);
         k_col_twiddles<<<dim3(div_up(h, 256), b.ny), 256, 0, s>>>(L.ctab, h, b.kyn, b.ny);
     }
+    const bool mfma = pitch % 16 == 0 && reinterpret_cast<uintptr_t>(img) % 16 == 0;
     // 1. U = img . Tw  (M = h, N = 2 kxn, K = w)
-    const int kc = div_up(div_up(w, kRowSplit), TK) * TK;
-    k_sgemm<false, 0><<<dim3(div_up(n2, TN), div_up(h, TM), div_up(w, kc)), 256, 0, s>>>(
-        h, n2, w, img, pitch, L.tw, n2, L.part, n2 * (int)sizeof(float), kc, (size_t)h * n2);
-    const size_t nu = (size_t)h * n2;
-    k_sum_partials<<<(unsigned)((nu + 255) / 256), 256, 0, s>>>(L.part, L.U, nu, div_up(w, kc));
+    if (mfma) {
+        k_hp_row_dft<<<dim3(div_up(h, 32), div_up(n2p / 16, HP_NTG), pl.nz), HP_WAVES * 64, 0, s>>>(
+            img, pitch, h, w, L.twT, pl.twp, n2p, L.part, pl.kc);
+        k_hp_sum_parts<<<dim3(div_up(n2, 64), h), 64, 0, s>>>(L.part, L.U, h, n2, n2p, pl.nz);
+    } else {
+        const int kc = div_up(div_up(w, kRowSplit), TK) * TK;
+        k_sgemm<false, 0><<<dim3(div_up(n2, TN), div_up(h, TM), div_up(w, kc)), 256, 0, s>>>(
+            h, n2, w, img, pitch, L.tw, n2p, L.part, n2 * (int)sizeof(float), kc, (size_t)h * n2);
+        const size_t nu = (size_t)h * n2;
+        k_sum_partials<<<(unsigned)((nu + 255) / 256), 256, 0, s>>>(L.part, L.U, nu, div_up(w, kc));
+    }
     // 2. band column DFT with the transfer function
     k_col_dft_partial<<<dim3(div_up(b.kxn, 64), b.ny, kColChunks), 64, 0, s>>>(L.U, L.ctab, L.P, h, b.kxn, b.ny);
     k_col_dft_finish<<<dim3(div_up(b.kxn, 64), b.ny), 64, 0, s>>>(L.P, L.Y, w, h, b.kxn, b.kyn, b.ny, hp, hps);
     // 3. inverse column DFT
-    k_col_idft<<<dim3(div_up(b.kxn, 64), h), 64, 0, s>>>(L.Y, L.ctab, L.V, h, b.kxn, b.ny);
+    k_col_idft<<<dim3(div_up(b.kxn, 64), h), 64, 0, s>>>(L.Y, L.ctab, L.V, h, b.kxn, b.ny, n2p);
     // 4. img = clamp(img - V' . Tw^T)  (M = h, N = w, K = 2 kxn)
-    k_sgemm<true, 1><<<dim3(div_up(w, TN), div_up(h, TM), 1), 256, 0, s>>>(h, w, n2, L.V, n2 * (int)sizeof(float),
-                                                                           L.tw, n2, img, pitch, n2, 0);
+    if (mfma)
+        k_hp_row_idft<<<dim3(div_up(w, 128), div_up(h, 128)), HP_WAVES * 64, 0, s>>>(img, pitch, h, w, L.V, L.tw, n2p);
+    else
+        k_sgemm<true, 1><<<dim3(div_up(w, TN), div_up(h, TM), 1), 256, 0, s>>>(
+            h, w, n2, L.V, n2p * (int)sizeof(float), L.tw, n2p, img, pitch, n2, 0);
     return check_launch("fourier_highpass");
 }
 

@@ -10,13 +10,26 @@ namespace {
 struct DebayerArgs {
     Cfa cfa;
     float black[3], scale[3], div[3];
+    double inv_div[3];  // rd(1 / div[c])
+    int fast_div;       // every div[c] in [2^-26, 2^26]: div_c may multiply by inv_div
 };
 
-// Green estimate at a red/blue or green site (R:Kernels/DeBayerKernels.cu:53-149).
+// v / div[c], bit-identical to the IEEE float division. Let x = v = X 2^p, d = div[c] = D 2^q (X, D
+// 24-bit integers) and m = M 2^k (M odd, 25 bits) a float rounding midpoint. x = d m is impossible
+// (X would hold the odd 25-bit factor M), so x - d m is a nonzero multiple of 2^min(p, q + k) and
+// |x/d - m| / m >= min(1/X, 1/(D M)) > 2^-49. The double product v * rd(1/d) is within 2^-52
+// (relative) of x/d, so it rounds to the same float. The bound needs a normal quotient: |v| >= 2^-100
+// (or v = 0) with d in [2^-26, 2^26] (fast_div); other v take the division.
+__device__ __forceinline__ float div_c(float v, const DebayerArgs& a, int c) {
+    const float av = fabsf(v);
+    if (!a.fast_div || (av != 0.0f && av < 0x1p-100f)) return v / a.div[c];
+    return (float)((double)v * a.inv_div[c]);
+}
+
+// Green estimate at a pixel of CFA colour `site` (R:Kernels/DeBayerKernels.cu:53-149).
 // RAW(x, y) returns the raw value as float.
 template <class RawF>
-__device__ __forceinline__ float green_at(int x, int y, const DebayerArgs& a, RawF raw) {
-    int site = a.cfa.at(y, x);
+__device__ __forceinline__ float green_at_site(int site, int x, int y, const DebayerArgs& a, RawF raw) {
     if (site == ISA_CFA_GREEN) return (raw(x, y) - a.black[1]) * a.scale[1];
     if (site != ISA_CFA_RED && site != ISA_CFA_BLUE) return 0.0f;
     int c = site == ISA_CFA_RED ? 0 : 2;
@@ -36,12 +49,17 @@ __device__ __forceinline__ float green_at(int x, int y, const DebayerArgs& a, Ra
     return w * ix + (1.0f - w) * iy;
 }
 
+template <class RawF>
+__device__ __forceinline__ float green_at(int x, int y, const DebayerArgs& a, RawF raw) {
+    return green_at_site(a.cfa.at(y, x), x, y, a, raw);
+}
+
 // Red and blue by colour-difference interpolation against green
-// (R:Kernels/DeBayerKernels.cu:151-231). G(x, y) returns the green plane.
+// (R:Kernels/DeBayerKernels.cu:151-231) at a pixel of CFA colour `site` whose right-hand
+// neighbour has colour `row`. G(x, y) returns the green plane.
 template <class RawF, class GreenF>
-__device__ __forceinline__ void redblue_at(int x, int y, const DebayerArgs& a, RawF raw, GreenF G, float& r, float& b) {
-    int site = a.cfa.at(y, x);
-    int row = a.cfa.at(y, x + 1);
+__device__ __forceinline__ void redblue_at_site(int site, int row, int x, int y, const DebayerArgs& a, RawF raw,
+                                                GreenF G, float& r, float& b) {
     float g = G(x, y);
     auto RC = [&](int c, int xx, int yy) { return (raw(xx, yy) - a.black[c]) * a.scale[c]; };
     r = 0.0f;
@@ -58,6 +76,11 @@ __device__ __forceinline__ void redblue_at(int x, int y, const DebayerArgs& a, R
                                (RC(other, x + 1, y + 1) - G(x + 1, y + 1)) + (RC(other, x - 1, y + 1) - G(x - 1, y + 1)));
         if (site == ISA_CFA_RED) { r = v; b = d; } else { b = v; r = d; }
     }
+}
+
+template <class RawF, class GreenF>
+__device__ __forceinline__ void redblue_at(int x, int y, const DebayerArgs& a, RawF raw, GreenF G, float& r, float& b) {
+    redblue_at_site(a.cfa.at(y, x), a.cfa.at(y, x + 1), x, y, a, raw, G, r, b);
 }
 
 __global__ void k_debayer_green(int w, int h, const float* __restrict__ in, int in_pitch, float* __restrict__ out,
@@ -81,54 +104,168 @@ __global__ void k_debayer_redblue(int w, int h, const float* __restrict__ in, in
     o[2] = b;
 }
 
-// Fused DeBayerFullRes / tracking-image front end on an LDS tile.
+// ---------------------------------------------------------------------------
+// sRGB OETF power x^y, y = float(1/2.4), for x in (2^-9, 1] (the OETF's power branch starts at
+// 0.0031308 > 2^-9). With x = 2^-e m, m in [1, 2), and c = 1 + (j + 1/2)/128 the centre of m's
+// 1/128 interval: x^y = 2^(-e y) c^y (1 + r)^y with r = (m - c)/c, |r| <= 2^-8. The first two
+// factors come from tables (double, within a few ulp), the last from its binomial series to r^5
+// (remainder below 2^-53), so the double z is within 2^-48 (relative) of x^y. When z lies farther
+// than 2^-40 from every rounding midpoint of its float, x^y and pow_r's double (within 2^-52 of
+// x^y) both round to that float; otherwise pow_r itself is evaluated.
+namespace srgb_pow {
+constexpr float kY = 1.0f / 2.4f;
+constexpr int kJ = 128, kE = 10;
+
+constexpr double ln_1to2(double c) {  // ln c = 2 atanh((c - 1)/(c + 1)) for c in [1, 2]
+    const double u = (c - 1.0) / (c + 1.0), u2 = u * u;
+    double s = 0.0, t = u;
+    for (int k = 0; k < 40; ++k) {
+        s += t / (2 * k + 1);
+        t *= u2;
+    }
+    return 2.0 * s;
+}
+
+constexpr double exp_small(double t) {  // e^t for |t| <= 1, Taylor series
+    double s = 1.0, term = 1.0;
+    for (int k = 1; k < 30; ++k) {
+        term = term * t / k;
+        s += term;
+    }
+    return s;
+}
+
+struct Tables {
+    double mant[kJ][2];  // (c_j^y, 1 / c_j)
+    double expo[kE];     // 2^(-e y)
+};
+
+constexpr Tables make_tables() {
+    Tables T{};
+    const double y = (double)kY;
+    for (int j = 0; j < kJ; ++j) {
+        const double c = 1.0 + (j + 0.5) / kJ;
+        T.mant[j][0] = exp_small(y * ln_1to2(c));
+        T.mant[j][1] = 1.0 / c;
+    }
+    for (int e = 0; e < kE; ++e) {  // 2^(-e y) = 2^-n 2^(n - e y), n = ceil(e y)
+        const double t = e * y;
+        int n = (int)t;
+        if (n < t) ++n;
+        double v = exp_small((n - t) * 0.69314718055994530942);
+        for (int i = 0; i < n; ++i) v *= 0.5;
+        T.expo[e] = v;
+    }
+    return T;
+}
+
+// binomial coefficients of (1 + r)^y
+constexpr double kB1 = (double)kY, kB2 = kB1 * ((double)kY - 1.0) / 2.0, kB3 = kB2 * ((double)kY - 2.0) / 3.0,
+                 kB4 = kB3 * ((double)kY - 3.0) / 4.0, kB5 = kB4 * ((double)kY - 4.0) / 5.0;
+}  // namespace srgb_pow
+
+__constant__ srgb_pow::Tables kSrgbPow = srgb_pow::make_tables();
+
+__device__ __forceinline__ float pow_srgb(float x) {
+    using namespace srgb_pow;
+    const unsigned ux = __float_as_uint(x);
+    const int e = 127 - (int)(ux >> 23);
+    if (e < 0 || e >= kE) return pow_r(x, kY);
+    const unsigned mb = ux & 0x7fffffu;
+    const int j = (int)(mb >> 16);
+    const double d = (double)((int)(mb & 0xffffu) - 0x8000) * 0x1p-23;  // m - c_j, exact
+    const double r = d * kSrgbPow.mant[j][1];
+    double q = fma(kB5, r, kB4);
+    q = fma(q, r, kB3);
+    q = fma(q, r, kB2);
+    q = fma(q, r, kB1);
+    q = fma(q, r, 1.0);
+    const double z = kSrgbPow.expo[e] * kSrgbPow.mant[j][0] * q;
+    const float f = (float)z;
+    const unsigned uf = __float_as_uint(f);
+    double half = __longlong_as_double((long long)((uf >> 23) + 872u) << 52);  // half an ulp of f (normal)
+    if ((uf & 0x7fffffu) == 0 && z < (double)f) half *= 0.5;                   // the gap below 2^k is halved
+    if (fabs(z - (double)f) < half - 0x1p-40 * z) return f;
+    return pow_r(x, kY);
+}
+
+// sRGB OETF after NaN->0 and clamp to [0,1] (R:Kernels/kernel.cu:271-313). The power is pow_r's
+// float (FAST evaluates it through pow_srgb).
+template <bool FAST = true>
+__device__ __forceinline__ float srgb_prepared(float v) {
+    if (isnan(v)) v = 0.0f;
+    v = fmaxf(fminf(v, 1.0f), 0.0f);
+    if (v <= 0.0031308f) return 12.92f * v;
+    return (1.0f + 0.055f) * (FAST ? pow_srgb(v) : pow_r(v, 1.0f / 2.4f)) - 0.055f;
+}
+
+// Fused DeBayerFullRes / tracking-image front end, one thread per 2x2 Bayer quad: the four CFA
+// sites of a quad are the same in every thread, so the site branches of the debayer are uniform.
+// A block covers DQ_X x DQ_Y quads; the green plane is formed in LDS for those quads and one ring
+// of quads around them (the red/blue step reads green one pixel out).
 // MODE 0: write the decoded float3 image (debayer / div, 0 on the 2-px border).
 // MODE 1: write grey = coef . sRGB(decoded) (gamma + ColorToGray fused).
-constexpr int DB_BX = 64, DB_BY = 8;
+constexpr int DQ_X = 64, DQ_Y = 8;
+constexpr int DQ_RW = 2 * DQ_X + 8, DQ_RH = 2 * DQ_Y + 8;  // raw tile: pixels x0-4 .. x0+2 DQ_X+3
+constexpr int DQ_GW = 2 * DQ_X + 4, DQ_GH = 2 * DQ_Y + 4;  // green tile: pixels x0-2 .. x0+2 DQ_X+1
 template <int MODE>
-__global__ __launch_bounds__(DB_BX* DB_BY) void k_debayer_tile(int w, int h, const uint16_t* __restrict__ raw,
-                                                                int raw_pitch, float* __restrict__ out, int out_pitch,
-                                                                DebayerArgs a, float c0, float c1, float c2) {
-    __shared__ float s_raw[DB_BY + 6][DB_BX + 6];
-    __shared__ float s_g[DB_BY + 2][DB_BX + 2];
-    const int x0 = blockIdx.x * DB_BX, y0 = blockIdx.y * DB_BY;
-    const int tid = threadIdx.y * DB_BX + threadIdx.x;
-    for (int i = tid; i < (DB_BY + 6) * (DB_BX + 6); i += DB_BX * DB_BY) {
-        int ly = i / (DB_BX + 6), lx = i - ly * (DB_BX + 6);
-        int gx = clampi(x0 - 3 + lx, 0, w - 1), gy = clampi(y0 - 3 + ly, 0, h - 1);
-        s_raw[ly][lx] = (float)row_ptr(raw, raw_pitch, gy)[gx];
+__global__ __launch_bounds__(DQ_X* DQ_Y) void k_debayer_quad(int w, int h, const uint16_t* __restrict__ raw,
+                                                              int raw_pitch, float* __restrict__ out, int out_pitch,
+                                                              DebayerArgs a, float c0, float c1, float c2) {
+    __shared__ float s_raw[DQ_RH][DQ_RW + 1];
+    __shared__ float s_g[DQ_GH][DQ_GW + 1];
+    const int x0 = blockIdx.x * 2 * DQ_X, y0 = blockIdx.y * 2 * DQ_Y;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * DQ_X + tx;
+    for (int ly = ty; ly < DQ_RH; ly += DQ_Y) {
+        const uint16_t* r = row_ptr(raw, raw_pitch, clampi(y0 - 4 + ly, 0, h - 1));
+        for (int lx = tx; lx < DQ_RW; lx += DQ_X) s_raw[ly][lx] = (float)r[clampi(x0 - 4 + lx, 0, w - 1)];
     }
     __syncthreads();
-    auto R = [&](int xx, int yy) { return s_raw[yy - (y0 - 3)][xx - (x0 - 3)]; };
-    for (int i = tid; i < (DB_BY + 2) * (DB_BX + 2); i += DB_BX * DB_BY) {
-        int ly = i / (DB_BX + 2), lx = i - ly * (DB_BX + 2);
-        int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-        float g = 0.0f;  // never-written border of the decoded image reads as the Set(0) value
-        if (gx >= 2 && gx < w - 2 && gy >= 2 && gy < h - 2) g = green_at(gx, gy, a, R);
-        s_g[ly][lx] = g;
-    }
-    __syncthreads();
-    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-    if (x >= w || y >= h) return;
-    float v[3] = {0.0f, 0.0f, 0.0f};
-    if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2) {
-        auto G = [&](int xx, int yy) { return s_g[yy - (y0 - 1)][xx - (x0 - 1)]; };
-        float r, b;
-        redblue_at(x, y, a, R, G, r, b);
-        v[0] = r;
-        v[1] = G(x, y);
-        v[2] = b;
-    }
+    auto R = [&](int xx, int yy) { return s_raw[yy - (y0 - 4)][xx - (x0 - 4)]; };
+    for (int q = tid; q < (DQ_X + 2) * (DQ_Y + 2); q += DQ_X * DQ_Y) {
+        const int gqy = q / (DQ_X + 2), gqx = q - gqy * (DQ_X + 2);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = v[c] / a.div[c];
-    if (MODE == 0) {
-        float* o = row_ptr(out, out_pitch, y) + x * 3;
-        o[0] = v[0];
-        o[1] = v[1];
-        o[2] = v[2];
-    } else {
-        float r = srgb_prepared(v[0]), g = srgb_prepared(v[1]), b = srgb_prepared(v[2]);
-        row_ptr(out, out_pitch, y)[x] = c0 * r + c1 * g + c2 * b;
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int gx = x0 - 2 + 2 * gqx + dx, gy = y0 - 2 + 2 * gqy + dy;
+                float g = 0.0f;  // never-written border of the decoded image reads as the Set(0) value
+                if (gx >= 2 && gx < w - 2 && gy >= 2 && gy < h - 2) g = green_at_site(a.cfa.c[dy * 2 + dx], gx, gy, a, R);
+                s_g[2 * gqy + dy][2 * gqx + dx] = g;
+            }
+    }
+    __syncthreads();
+    auto G = [&](int xx, int yy) { return s_g[yy - (y0 - 2)][xx - (x0 - 2)]; };
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy) {
+        const int y = y0 + 2 * ty + dy;
+        float res[2][3];
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int x = x0 + 2 * tx + dx;
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2) {
+                redblue_at_site(a.cfa.c[dy * 2 + dx], a.cfa.c[dy * 2 + (dx ^ 1)], x, y, a, R, G, v[0], v[2]);
+                v[1] = G(x, y);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) res[dx][c] = div_c(v[c], a, c);
+        }
+        if (y >= h) continue;
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int x = x0 + 2 * tx + dx;
+            if (x >= w) continue;
+            if (MODE == 0) {
+                float* o = row_ptr(out, out_pitch, y) + x * 3;
+                o[0] = res[dx][0];
+                o[1] = res[dx][1];
+                o[2] = res[dx][2];
+            } else {
+                float r = srgb_prepared(res[dx][0]), g = srgb_prepared(res[dx][1]), b = srgb_prepared(res[dx][2]);
+                row_ptr(out, out_pitch, y)[x] = c0 * r + c1 * g + c2 * b;
+            }
+        }
     }
 }
 
@@ -384,10 +521,13 @@ __global__ void k_apply_weighting(float* inout, const float* sum, const float* w
 DebayerArgs make_debayer_args(const float black[3], const float scale[3], const float div[3], const int cfa[4]) {
     DebayerArgs a;
     for (int i = 0; i < 4; ++i) a.cfa.c[i] = cfa[i];
+    a.fast_div = 1;
     for (int c = 0; c < 3; ++c) {
         a.black[c] = black[c];
         a.scale[c] = scale[c];
         a.div[c] = div ? div[c] : 1.0f;
+        a.inv_div[c] = 1.0 / (double)a.div[c];
+        if (!(a.div[c] >= 0x1p-26f && a.div[c] <= 0x1p26f)) a.fast_div = 0;
     }
     return a;
 }
@@ -414,8 +554,8 @@ int launch_raw_to_gray(const uint16_t* raw, int raw_pitch, float* gray, int gray
     }
     DebayerArgs a = make_debayer_args(black, scale, div, cfa);
     float c0 = green_only ? 0.0f : 0.2989f, c1 = green_only ? 1.0f : 0.5870f, c2 = green_only ? 0.0f : 0.1141f;
-    dim3 grid(div_up(w, DB_BX), div_up(h, DB_BY));
-    k_debayer_tile<1><<<grid, dim3(DB_BX, DB_BY), 0, s>>>(w, h, raw, raw_pitch, gray, gray_pitch, a, c0, c1, c2);
+    dim3 grid(div_up(w, 2 * DQ_X), div_up(h, 2 * DQ_Y));
+    k_debayer_quad<1><<<grid, dim3(DQ_X, DQ_Y), 0, s>>>(w, h, raw, raw_pitch, gray, gray_pitch, a, c0, c1, c2);
     return check_launch("raw_to_gray");
 }
 
@@ -496,8 +636,8 @@ int isa_debayer_full(int width, int height, const uint16_t* raw_dev, int raw_pit
         div[c] = white[c] / camera_white[c];
     }
     DebayerArgs a = make_debayer_args(black, scale, div, cfa);
-    dim3 grid(isa::div_up(width, DB_BX), isa::div_up(height, DB_BY));
-    k_debayer_tile<0><<<grid, dim3(DB_BX, DB_BY), 0, isa::to_stream(stream)>>>(width, height, raw_dev, raw_pitch,
+    dim3 grid(isa::div_up(width, 2 * DQ_X), isa::div_up(height, 2 * DQ_Y));
+    k_debayer_quad<0><<<grid, dim3(DQ_X, DQ_Y), 0, isa::to_stream(stream)>>>(width, height, raw_dev, raw_pitch,
                                                                               out3_dev, out_pitch, a, 0.f, 0.f, 0.f);
     return isa::check_launch("isa_debayer_full");
 }

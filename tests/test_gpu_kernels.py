@@ -16,6 +16,8 @@ from tests.gpu_util import dev, empty, host, p, stream, ulp_diff  # noqa: E402
 
 RGGB = [0, 1, 1, 2]
 GRBG = [1, 0, 2, 1]
+GBRG = [1, 2, 0, 1]
+BGGR = [2, 1, 1, 0]
 
 
 def _raw(rng, h, w, lo=200, hi=16000):
@@ -25,10 +27,10 @@ def _raw(rng, h, w, lo=200, hi=16000):
     return np.clip(img, lo, 16383).astype(np.uint16)
 
 
-@pytest.mark.parametrize("cfa", [RGGB, GRBG])
-def test_debayer_full(isa, oracle, gpu, cfa):
+@pytest.mark.parametrize("cfa", [RGGB, GRBG, GBRG, BGGR])
+@pytest.mark.parametrize("h,w", [(70, 94), (37, 301)])  # one block; odd sizes over 3 x 3 blocks
+def test_debayer_full(isa, oracle, gpu, cfa, h, w):
     rng = np.random.default_rng(1)
-    h, w = 70, 94
     raw = _raw(rng, h, w)
     black, white, cw = [64.0, 60.0, 62.0], [16383.0, 16000.0, 15800.0], [1.9, 1.0, 1.4]
     out = empty((h, w, 3), fill=7.0)
@@ -36,6 +38,22 @@ def test_debayer_full(isa, oracle, gpu, cfa):
              isa.f32s(*cw), isa.i32s(*cfa), stream())
     ref = oracle.debayer_full(raw, black, white, cw, cfa)
     np.testing.assert_array_equal(host(out), ref)
+
+
+@pytest.mark.parametrize("cfa", [GBRG, BGGR])
+def test_merger_grey_front_end(isa, oracle, gpu, cfa):
+    """The merger's fused debayer + sRGB + grey + Gaussian (its Lucas-Kanade images), bit-exact
+    against the oracle for CFA layouts other than the pipeline tests' RGGB."""
+    w, h, n = 290, 166, 2
+    cfg = isa.default_synth(seed=5, width=w, height=h, frames=n)
+    raw_t, _ = isa.synth_burst(cfg)
+    params = isa.default_params(levels=[(1, 32, 2)], cfa=cfa)
+    m = isa.BurstMerger(w, h, n, params)
+    m.run(raw_t)
+    raw = host(raw_t).view(np.uint16)
+    for f in range(n):
+        ref = oracle.tracking_image(raw[f], params, 0, params.sigma_debayer_accumulation)
+        np.testing.assert_array_equal(host(m.buffer_tensor(1, f, 1)), ref)
 
 
 def test_debayer_green_redblue(isa, oracle, gpu):
@@ -91,12 +109,14 @@ def test_gauss_filter(isa, oracle, gpu, sigma, ch):
     np.testing.assert_array_equal(host(out), ref)
 
 
-@pytest.mark.parametrize("w,h", [(96, 64), (130, 90)])
-def test_fourier_highpass(isa, oracle, gpu, w, h):
+# Rows of 4 w bytes: 16-byte aligned for w = 96 and 700 (matrix-core GEMMs; at 700 x 300 with
+# hp + hps = 0.12 the band has 11 column tiles in 2 groups and the K sum 11 chunks, the last ragged),
+# 8-byte aligned for w = 130 (SIMT GEMMs).
+@pytest.mark.parametrize("w,h,hp,hps", [(96, 64, 0.01, 0.0025), (130, 90, 0.01, 0.0025), (700, 300, 0.1, 0.02)])
+def test_fourier_highpass(isa, oracle, gpu, w, h, hp, hps):
     rng = np.random.default_rng(5)
     yy, xx = np.mgrid[0:h, 0:w]
     img = (0.5 + 0.3 * np.sin(xx / 17.0) * np.cos(yy / 23.0) + 0.05 * rng.standard_normal((h, w))).astype(np.float32)
-    hp, hps = 0.01, 0.0025
     ws_bytes = isa.lib().isa_fourier_highpass_workspace(w, h, hp, hps)
     ws = empty((ws_bytes // 4 + 64,))
     d = dev(img)
