@@ -223,9 +223,35 @@ ColsShape cols_shape(int T, int M) {
     return c;
 }
 
-size_t cols_smem(int T, int M, int items) {
-    int S = 2 * M + 1, B = T + 2 * M;
-    return (size_t)items * ((size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7));
+// per item: TT (B S doubles), the reference energy (1 double), D (S S floats, padded to 8 bytes),
+// the argmin key (8 bytes) and the max key (4 bytes, padded to 8)
+__host__ __device__ inline size_t cols_item_bytes(int T, int M) {
+    const int S = 2 * M + 1, B = T + 2 * M;
+    return (size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7) + 16;
+}
+
+size_t cols_smem(int T, int M, int items) { return (size_t)items * cols_item_bytes(T, M); }
+
+// One step of the pairwise tree over x: lane x adds lane x + ST (ST = 1, 2, 4, ... < T). Only the
+// lanes with x % (2 ST) == 0 keep a meaningful sum, and they only ever read lanes that kept one at
+// every earlier step, so every lane adds (no branch) and the kept sums are those of the branchy
+// form. Strides below 16 stay within a 16-lane row (DPP row shift), 16 swaps 32-lane halves' rows
+// (ds_swizzle), 32 crosses them (bpermute).
+template <int ST>
+__device__ __forceinline__ double tree_step(double v) {
+    const long long b = __double_as_longlong(v);
+    int lo = (int)b, hi = (int)(b >> 32);
+    if constexpr (ST < 16) {
+        lo = __builtin_amdgcn_mov_dpp(lo, 0x100 + ST, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_mov_dpp(hi, 0x100 + ST, 0xf, 0xf, false);
+    } else if constexpr (ST == 16) {
+        lo = __builtin_amdgcn_ds_swizzle(lo, 0x401f);
+        hi = __builtin_amdgcn_ds_swizzle(hi, 0x401f);
+    } else {
+        lo = __shfl_down(lo, ST);
+        hi = __shfl_down(hi, ST);
+    }
+    return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // minimum waves per SIMD the register allocation must allow: M >= 2 runs faster at 3 waves (with a
@@ -246,7 +272,7 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
     const int per = (int)(gridDim.x / 8);
     const int blk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (blk >= nblk) return;
-    const size_t item_bytes = (size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7);
+    const size_t item_bytes = cols_item_bytes(T, M);
     // per-item lane coordinates
     const int pp = tid / lpi, lane = tid - pp * lpi;
     const int g = lane / T, x = lane - g * T;
@@ -269,6 +295,8 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
     double* TT = (double*)mine;
     double* R2tot = TT + B * S;
     float* Dm = (float*)(R2tot + 1);
+    unsigned long long* kmin = (unsigned long long*)(mine + item_bytes - 16);
+    unsigned* kmax = (unsigned*)(kmin + 1);
     // block origins in the level image
     const int rx0 = tx * T + M + rox, ry0 = ty * T + M + roy;  // reference interior
     const int tx0 = tx * T + tox, ty0 = ty * T + toy;          // track block
@@ -390,20 +418,27 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
     for (int q = 0; q < RPG; ++q)
 #pragma unroll
         for (int iy = 0; iy < S; ++iy) acc[q][iy] = fma(-2.0, acc[q][iy], TT[(x + g * RPG + q) * S + iy]);
-    // 3. pairwise trees over x on lane shuffles
-    for (int st = 1; st < T; st *= 2) {
-        const bool take = (x % (2 * st)) == 0 && x + st < T;
-        double o = __shfl_down(r2, st);
-        if (take) r2 += o;
+    // 3. pairwise trees over x (the tree of ssd_core: stride 1, 2, 4, ... < T)
+    auto level = [&](auto st) {
+        constexpr int ST = decltype(st)::value;
+        if (ST >= T) return;
+        r2 = tree_step<ST>(r2);
 #pragma unroll
         for (int q = 0; q < RPG; ++q)
 #pragma unroll
-            for (int iy = 0; iy < S; ++iy) {
-                o = __shfl_down(acc[q][iy], st);
-                if (take) acc[q][iy] += o;
-            }
+            for (int iy = 0; iy < S; ++iy) acc[q][iy] = tree_step<ST>(acc[q][iy]);
+    };
+    level(std::integral_constant<int, 1>{});
+    level(std::integral_constant<int, 2>{});
+    level(std::integral_constant<int, 4>{});
+    level(std::integral_constant<int, 8>{});
+    level(std::integral_constant<int, 16>{});
+    level(std::integral_constant<int, 32>{});
+    if (x == 0 && g == 0) {
+        *R2tot = r2;
+        *kmin = ~0ull;
+        *kmax = ordered_bits(-3.402823466e+38f);
     }
-    if (x == 0 && g == 0) *R2tot = r2;
     __syncthreads();
     if (x == 0) {
         const double r2t = *R2tot;
@@ -413,12 +448,22 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
             for (int iy = 0; iy < S; ++iy) Dm[iy * S + g * RPG + q] = (float)(r2t + acc[q][iy]);
     }
     __syncthreads();
+    // findMinimum's scan over the S x S values, spread over the item's lanes
+    for (int i = lane; i < S * S; i += lpi) {
+        const float v = Dm[i];
+        atomicMin(kmin, argmin_key(v, i));
+        if (!isnan(v)) atomicMax(kmax, ordered_bits(v));
+    }
+    __syncthreads();
     if (!live) return;
     if (a.shift_images)
         for (int s = lane; s < S * S; s += lpi) a.shift_images[((size_t)p * a.cx * a.cy + tile_idx) * S * S + s] = Dm[s];
     if (lane == 0) {
+        const unsigned long long km = *kmin;
+        const int mi = km == ~0ull ? -1 : (int)(unsigned)km;
+        const float mn = mi < 0 ? 3.402823466e+38f : Dm[mi];
         float cx, cy;
-        find_min_core(Dm, M, a.threshold, cx, cy);
+        find_min_fit(Dm, M, mi, mn, from_ordered_bits(*kmax), a.threshold, cx, cy);
         sh[0] = psx + cx;
         sh[1] = psy + cy;
     }

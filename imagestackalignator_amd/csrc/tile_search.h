@@ -19,7 +19,10 @@ __device__ __forceinline__ void tile_offset_sc(int tx, int ty, int tile, int w, 
 }
 
 // findMinimum (R:Kernels/kernel.cu:413-546) on an S x S map.
-__device__ void find_min_core(const float* v, int M, float threshold, float& cx, float& cy) {
+// Sub-pixel position of the minimum mi (value mn; mx the largest value) of an S x S shift image:
+// the 3x3 quadratic fit of findMinimum, or 0 at the window edge / for a flat image.
+__device__ void find_min_fit(const float* v, int M, int mi, float mn, float mx, float threshold, float& cx,
+                             float& cy) {
     const float FA11[9] = {1.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f, 2.0f / 4.0f, -4.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f,
                            -2.0f / 4.0f, 1.0f / 4.0f};
     const float FA22[9] = {1.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f, -2.0f / 4.0f, -4.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f,
@@ -29,14 +32,7 @@ __device__ void find_min_core(const float* v, int M, float threshold, float& cx,
                           1.0f / 8.0f};
     const float Fb2[9] = {-1.0f / 8.0f, -2.0f / 8.0f, -1.0f / 8.0f, 0.0f, 0.0f, 0.0f, 1.0f / 8.0f, 2.0f / 8.0f,
                           1.0f / 8.0f};
-    const int S = 2 * M + 1, n = S * S;
-    float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
-    int mi = -1;
-    for (int i = 0; i < n; ++i) {
-        float val = v[i];
-        mx = fmaxf(mx, val);
-        if (val < mn) { mn = val; mi = i; }
-    }
+    const int S = 2 * M + 1;
     cy = (float)(mi / S);
     cx = (float)mi - cy * (float)S;
     if (cx < 1 || cy < 1 || cx >= (float)(2 * M) || cy >= (float)(2 * M)) {
@@ -69,6 +65,35 @@ __device__ void find_min_core(const float* v, int M, float threshold, float& cx,
         cy -= (float)M;
     }
     if (threshold + mn > mx) { cx = 0.0f; cy = 0.0f; }
+}
+
+// findMinimum: the first index of the smallest value (values >= FLT_MAX and NaN never qualify),
+// the largest value, then find_min_fit.
+__device__ void find_min_core(const float* v, int M, float threshold, float& cx, float& cy) {
+    const int S = 2 * M + 1, n = S * S;
+    float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
+    int mi = -1;
+    for (int i = 0; i < n; ++i) {
+        float val = v[i];
+        mx = fmaxf(mx, val);
+        if (val < mn) { mn = val; mi = i; }
+    }
+    find_min_fit(v, M, mi, mn, mx, threshold, cx, cy);
+}
+
+// Order-preserving map of a non-NaN float to unsigned (for integer min / max).
+__device__ __forceinline__ unsigned ordered_bits(float v) {
+    const unsigned u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered_bits(unsigned k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+// Argmin key of value v at index i: the smallest key is the first index of the smallest value, as
+// in find_min_core's scan (-0 ties with +0; values >= FLT_MAX and NaN never qualify).
+__device__ __forceinline__ unsigned long long argmin_key(float v, int i) {
+    if (!(v < 3.402823466e+38f)) return ~0ull;
+    return ((unsigned long long)ordered_bits(v + 0.0f) << 32) | (unsigned)i;
 }
 
 struct TrackArgs {

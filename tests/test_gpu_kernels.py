@@ -186,6 +186,31 @@ def test_tiles_ssd_findmin(isa, oracle, gpu, tile, M, bs, rot, h, w):
     np.testing.assert_array_equal(host(d_pre).reshape(-1, 2), pre.reshape(-1, 2) + oracle.find_minimum(s_ref, M, 1.0))
 
 
+@pytest.mark.parametrize("tile,M", [(16, 1), (16, 2), (32, 4)])
+def test_track_ties_take_the_first_minimum(isa, oracle, gpu, tile, M):
+    """On an image of period 2 in x, shifts 2 px apart give exactly equal FP64 SSD values, so the
+    shift images hold tied minima; the fused search must take the first one in scan order, as
+    findMinimum does."""
+    rng = np.random.default_rng(31 + M)
+    h, w = 96, 128
+    col = rng.uniform(0.2, 0.8, (h, 2)).astype(np.float32)
+    a = np.ascontiguousarray(np.tile(col, (1, w // 2)))
+    b = np.ascontiguousarray(np.roll(a, 1, axis=0))
+    cx, cy = (w - 2 * M) // tile, (h - 2 * M) // tile
+    bs, rot = (0.0, 0.0), 0.0
+    pre = rng.integers(-1, 2, (cy, cx, 2)).astype(np.float32)
+    print(f"track ties tile={tile} M={M}", flush=True)
+    rt_ref = oracle.tiles_border(a, M, tile, cx, cy, bs, rot)
+    tt_ref = oracle.tiles_preshift(b, pre, M, tile, cx, cy, bs, rot)
+    s_ref = oracle.tile_ssd(rt_ref, tt_ref, M, tile)
+    mins = s_ref.reshape(len(s_ref), -1)
+    assert np.mean((mins == mins.min(axis=1, keepdims=True)).sum(axis=1) > 1) > 0.5  # ties are common
+    d_pre = dev(pre)
+    isa.call("isa_track", p(dev(b)), p(dev(a)), w * 4, p(d_pre), cx * 8, w, h, tile, M, isa.f32s(*bs),
+             C.c_float(rot), isa.f32s(*bs), C.c_float(rot), C.c_float(1.0), stream())
+    np.testing.assert_array_equal(host(d_pre).reshape(-1, 2), pre.reshape(-1, 2) + oracle.find_minimum(s_ref, M, 1.0))
+
+
 def test_upsample_shifts(isa, oracle, gpu):
     rng = np.random.default_rng(8)
     ocx, ocy, ncx, ncy = 7, 5, 15, 11
