@@ -3,6 +3,8 @@
 #include <climits>
 #include <cmath>
 
+#include <cstdint>
+
 #include "isa_device.h"
 #include "isa_host.h"
 
@@ -173,8 +175,11 @@ __device__ __forceinline__ void gather_class_edge(const uint16_t* __restrict__ r
             float v = norm_raw<DIVN>((float)raw[(size_t)ppsy * w + ppsx], c, a);
             float cert = row_ptr(mask, mask_pitch, ppy / 2)[(ppx / 2) * 4 + c];
             if (!isfinite(cert)) cert = 0.0f;
-            acc[c] += v * wgt * cert;
-            ws[c] += wgt * cert;
+            const float t = v * wgt * cert, u = wgt * cert;
+            // constant indices only: the sums stay in registers
+            if (c == 0) { acc[0] += t; ws[0] += u; }
+            else if (c == 1) { acc[1] += t; ws[1] += u; }
+            else { acc[2] += t; ws[2] += u; }
         }
     }
 }
@@ -260,6 +265,129 @@ __global__ void k_accumulate_frames(float* sum, float* wsum, const float* kern, 
             sy = (int)roundf(fl[1]);
         }
         gather1x<DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps[c] = acc[c];
+        pw[c] = ws[c];
+    }
+}
+
+// The same accumulation with each wave on pixels of one (x, y) parity: the half-resolution mask
+// pixels under a 5 x 5 window are then the same 3 x 3 block for every lane, loaded as nine float4,
+// and each window row of raw samples is one aligned 12-byte load. Needs 4-byte aligned raw frames
+// of even width and 16-byte aligned mask rows. Taps, classes and every sum as in gather1x.
+constexpr int ACCW_X = 64, ACCW_Y = 4;  // block: 4 waves, one parity each, of 32 x 2 pixels
+
+// raw samples at columns qx-2 .. qx+2 of a row, packed two per word: the three 32-bit words of the
+// pair-aligned run qx-2-o .. qx+3-o (o = qx & 1), shifted by o samples
+struct RawRow {
+    unsigned w0, w1, w2;
+    __device__ __forceinline__ float at(int j) const {  // sample at column qx - 2 + j
+        const unsigned v = j < 2 ? w0 : (j < 4 ? w1 : w2);
+        return (float)((j & 1) ? (v >> 16) : (v & 0xffffu));
+    }
+};
+
+__device__ __forceinline__ RawRow raw_row5(const uint16_t* rowp, int qx) {
+    typedef const __attribute__((address_space(1))) unsigned* gword;
+    const int o = qx & 1;
+    const gword p = (gword)(rowp + (qx - 2 - o));
+    const unsigned a = p[0], b = p[1], c = p[2], sh = 16u * (unsigned)o;
+    return RawRow{__builtin_amdgcn_alignbit(b, a, sh), __builtin_amdgcn_alignbit(c, b, sh), c >> sh};
+}
+
+__device__ __forceinline__ float chan(const float4& v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
+
+// class CLS of the window for a pixel of parity (PX, PY): r the 5 raw rows, m the 3 x 3 mask pixels
+// from ((x >> 1) - 1, (y >> 1) - 1)
+template <int CLS, int PX, int PY, bool DIVN>
+__device__ __forceinline__ void gather_class_wide(const RawRow (&r)[5], const float4 (&m)[3][3], const float* wt,
+                                                  const AccArgs& a, int ch, float acc[3], float ws[3]) {
+    if (ch < 0) return;
+    float A = ch == 0 ? acc[0] : (ch == 1 ? acc[1] : acc[2]);
+    float W = ch == 0 ? ws[0] : (ch == 1 ? ws[1] : ws[2]);
+#pragma unroll
+    for (int py = -2; py <= 2; ++py) {
+        if (((py & 1) << 1) != (CLS & 2)) continue;
+#pragma unroll
+        for (int px = -2; px <= 2; ++px) {
+            if ((px & 1) != (CLS & 1)) continue;
+            const float wgt = wt[(py + 2) * 5 + px + 2];
+            const float v = norm_raw<DIVN>(r[py + 2].at(px + 2), ch, a);
+            float cert = chan(m[((PY + py) >> 1) + 1][((PX + px) >> 1) + 1], ch);
+            if (!isfinite(cert)) cert = 0.0f;
+            A += v * wgt * cert;
+            W += wgt * cert;
+        }
+    }
+    if (ch == 0) { acc[0] = A; ws[0] = W; }
+    else if (ch == 1) { acc[1] = A; ws[1] = W; }
+    else { acc[2] = A; ws[2] = W; }
+}
+
+// one frame, window and sample shift inside the image, pixel of parity (PX, PY)
+template <int PX, int PY, bool DIVN>
+__device__ __forceinline__ void frame_wide(const uint16_t* raw, const float* mask, int mask_pitch, int x, int y, int qx,
+                                           int qy, int w, const float* wt, const AccArgs& a, float acc[3], float ws[3]) {
+    typedef const __attribute__((address_space(1))) float* gmask;
+    float4 m[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const gmask mr = (gmask)row_ptr(mask, mask_pitch, (y >> 1) - 1 + i) + ((x >> 1) - 1) * 4;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) m[i][j] = make_float4(mr[4 * j], mr[4 * j + 1], mr[4 * j + 2], 0.0f);
+    }
+    RawRow r[5];
+    const uint16_t* rs = raw + (size_t)qy * w;
+#pragma unroll
+    for (int py = -2; py <= 2; ++py) r[py + 2] = raw_row5(rs + (ptrdiff_t)py * w, qx);
+    const int c0 = site_channel(a.cfa.at(qy, qx)), c1 = site_channel(a.cfa.at(qy, qx + 1));
+    const int c2 = site_channel(a.cfa.at(qy + 1, qx)), c3 = site_channel(a.cfa.at(qy + 1, qx + 1));
+    gather_class_wide<0, PX, PY, DIVN>(r, m, wt, a, c0, acc, ws);
+    gather_class_wide<1, PX, PY, DIVN>(r, m, wt, a, c1, acc, ws);
+    gather_class_wide<2, PX, PY, DIVN>(r, m, wt, a, c2, acc, ws);
+    gather_class_wide<3, PX, PY, DIVN>(r, m, wt, a, c3, acc, ws);
+}
+
+template <bool DIVN>
+__global__ __launch_bounds__(256) void k_accumulate_frames_wide(float* sum, float* wsum, const float* kern, int w,
+                                                                int h, int out_pitch, int mask_pitch, int flow_pitch,
+                                                                FramesArgs fa, AccArgs a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x = blockIdx.x * ACCW_X + 2 * (lane & 31) + (wave & 1), y = blockIdx.y * ACCW_Y + 2 * (lane >> 5) + (wave >> 1);
+    if (x < 1 || y < 1 || x >= w - 1 || y >= h - 1) return;
+    float* ps = row_ptr(sum, out_pitch, y) + x * 3;
+    float* pw = row_ptr(wsum, out_pitch, y) + x * 3;
+    const float* k = row_ptr(kern, out_pitch, y) + x * 3;
+    float acc[3] = {ps[0], ps[1], ps[2]}, ws[3] = {pw[0], pw[1], pw[2]};
+    float wt[25];
+#pragma unroll
+    for (int py = -2; py <= 2; ++py)
+#pragma unroll
+        for (int px = -2; px <= 2; ++px) wt[(py + 2) * 5 + px + 2] = kernel_weight(px, py, k[0], k[1], k[2]);
+    const bool in_xy = x >= 2 && x <= w - 3 && y >= 2 && y <= h - 3;
+    for (int f = 0; f < fa.nframes; ++f) {
+        int sx = 0, sy = 0;
+        if (fa.flow[f]) {
+            const float* fl = row_ptr(fa.flow[f], flow_pitch, y) + x * 2;
+            sx = (int)roundf(fl[0]);
+            sy = (int)roundf(fl[1]);
+        }
+        const int qx = x + sx, qy = y + sy;
+        if (in_xy && qx >= 2 && qx <= w - 3 && qy >= 2 && qy <= h - 3) {
+            switch (wave) {  // the parity of every pixel of the wave
+                case 0: frame_wide<0, 0, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, qx, qy, w, wt, a, acc, ws); break;
+                case 1: frame_wide<1, 0, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, qx, qy, w, wt, a, acc, ws); break;
+                case 2: frame_wide<0, 1, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, qx, qy, w, wt, a, acc, ws); break;
+                default: frame_wide<1, 1, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, qx, qy, w, wt, a, acc, ws); break;
+            }
+        } else {
+            gather_class_edge<0, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+            gather_class_edge<1, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+            gather_class_edge<2, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+            gather_class_edge<3, DIVN>(fa.raw[f], fa.mask[f], mask_pitch, x, y, sx, sy, w, h, wt, a, acc, ws);
+        }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -360,7 +488,12 @@ int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, 
     // raw, mask, flow: host arrays of device pointers, frames in accumulation order; launches of up to
     // ACC_MAXF frames run in order on the stream, each continuing the sums of the previous one
     const AccArgs a = make_acc(white, black, cfa);
+    // wide form: raw frames 4-byte aligned with an even width, mask rows 16-byte aligned
+    bool wide = w % 2 == 0 && mask_pitch % 16 == 0;
+    for (int f = 0; f < nframes && wide; ++f)
+        wide = reinterpret_cast<uintptr_t>(raw[f]) % 4 == 0 && reinterpret_cast<uintptr_t>(mask[f]) % 16 == 0;
     auto kern_fn = a.white_int ? k_accumulate_frames<true> : k_accumulate_frames<false>;
+    auto wide_fn = a.white_int ? k_accumulate_frames_wide<true> : k_accumulate_frames_wide<false>;
     for (int f0 = 0; f0 < nframes; f0 += ACC_MAXF) {
         FramesArgs fa{};
         fa.nframes = std::min(ACC_MAXF, nframes - f0);
@@ -369,8 +502,12 @@ int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, 
             fa.mask[i] = mask[f0 + i];
             fa.flow[i] = flow[f0 + i];
         }
-        kern_fn<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(sum, wsum, kern, w, h, out_pitch,
-                                                                          mask_pitch, flow_pitch, fa, a);
+        if (wide)
+            wide_fn<<<dim3(div_up(w, ACCW_X), div_up(h, ACCW_Y)), 256, 0, s>>>(sum, wsum, kern, w, h, out_pitch,
+                                                                            mask_pitch, flow_pitch, fa, a);
+        else
+            kern_fn<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(sum, wsum, kern, w, h, out_pitch,
+                                                                              mask_pitch, flow_pitch, fa, a);
         ISA_TRY(check_launch("accumulate_frames"));
     }
     return ISA_OK;

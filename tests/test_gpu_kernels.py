@@ -360,10 +360,12 @@ def test_subsample_robustness_erode(isa, oracle, gpu):
 
 
 @pytest.mark.parametrize("cfa", [RGGB, GRBG])
-def test_accumulate_frames(isa, oracle, gpu, cfa):
-    """Several frames in one pass (LDS-staged windows, global-memory fallback for windows that leave the
-    image and for blocks whose shifted windows spread too far) against the oracle's accumulateImages
-    called once per frame in the same order."""
+@pytest.mark.parametrize("layout", ["aligned", "offset"])
+def test_accumulate_frames(isa, oracle, gpu, cfa, layout):
+    """Several frames in one pass against the oracle's accumulateImages called once per frame in the
+    same order: the parity-grouped wide-load form (aligned frames) and the general form (one raw
+    frame at an odd 2-byte offset), with NaN kernel and mask values, wild flows and windows that
+    leave the image."""
     rng = np.random.default_rng(41 + cfa[0])
     h, w, n = 72, 160, 6
     hw, hh = w // 2, h // 2
@@ -389,6 +391,10 @@ def test_accumulate_frames(isa, oracle, gpu, cfa):
     white, black = [16383.0, 16000.0, 15000.0], [10.0, 20.0, 30.0]
     print(f"accumulate_frames {n} x {w}x{h} cfa={cfa}", flush=True)
     d_raw = [dev(r) for r in raws]
+    if layout == "offset":  # frame 1 one sample into a larger buffer: not 4-byte aligned
+        buf = dev(np.concatenate([np.zeros(1, np.uint16), raws[1].ravel()]))
+        d_raw[1] = buf[1:]
+        assert d_raw[1].data_ptr() % 4 == 2
     d_mask = [dev(m) for m in masks]
     d_flow = [None if fl is None else dev(fl) for fl in flows]
     ptrs = lambda ts: (C.c_void_p * n)(*[None if t is None else t.data_ptr() for t in ts])  # noqa: E731
