@@ -20,6 +20,14 @@ template <class T> __device__ __forceinline__ const T* row_ptr(const T* base, in
     return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)pitch * (size_t)y);
 }
 
+// Global load at a 32-bit byte offset from a kernel-argument base: the address is the uniform base
+// plus a zero-extended 32-bit offset, so it needs no per-lane 64-bit arithmetic. The caller keeps
+// every offset below 2^32.
+template <class T> __device__ __forceinline__ T ld_off(const T* base, unsigned byte_off) {
+    typedef const __attribute__((address_space(1))) char* gchar;
+    return *(const __attribute__((address_space(1))) T*)((gchar)base + byte_off);
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // a / (float)n for an integer 1 <= n <= 2^24, bit-identical to the IEEE float division but without

@@ -380,6 +380,7 @@ struct LkfShared {
         float H[5][DH][HS];  // phase C: horizontal window sums, written once every thread has read Ld
     } u;
     int mixed;  // some tap of the block has a weight strictly between 0 and 1
+    int ident;  // every tap of the block is the neighbouring texel itself (cleared otherwise)
 };
 
 template <int HW>
@@ -397,7 +398,10 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     const int x0 = blockIdx.x * LKF_BX, y0 = blockIdx.y * LKF_BY;
     const int wx0 = x0 - HW - 3, wy0 = y0 - HW - 3;
     const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
-    if (tid == 0) sm.mixed = 0;
+    if (tid == 0) {
+        sm.mixed = 0;
+        sm.ident = 1;
+    }
     __syncthreads();
     // tap tables of the block's derivative region, translated to warped-tile coordinates
     for (int i = tid; i < (DW + DH) * 5; i += NT) {
@@ -408,6 +412,7 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         LTap l = {clampi(t.i0 - o, 0, lim - 1), clampi(t.i1 - o, 0, lim - 1), t.a, 0};
         l.p = t.a == 0.0f ? l.c0 : l.c1;
         if (t.a != 0.0f && t.a != 1.0f) sm.mixed = 1;
+        if (l.p != pix + k + 1) sm.ident = 0;  // tap k of region pixel pix is warped-tile index pix + 1 + k
         if (is_c) sm.u.ab.Tc[pix][k] = l;
         else sm.u.ab.Tr[pix][k] = l;
     }
@@ -422,9 +427,12 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         else sm.u.ab.fy[i - WC] = t;
     }
     __syncthreads();
-    // A. warped source and target: all flow loads first, then all source gathers (no branch between
-    // them, so every load of a thread is in flight at once)
+    // A. warped source and target. All flow loads first, then the texel coordinates of every element.
+    // When the 2x2 footprint of every element of the wave lies inside the image (the common case)
+    // the texel indices are the coordinates themselves and all gathers are issued together from
+    // 32-bit element offsets; otherwise each index is mirrored as the texture unit does.
     constexpr int NA = (WR * WC + NT - 1) / NT;
+    const unsigned pf = (unsigned)img_pitch / 4u, fpf = (unsigned)flow_pitch / 4u;
     float fa[NA], fb[NA];
     int gxs[NA], gys[NA];
 #pragma unroll
@@ -433,37 +441,68 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         const int ly = i / WC, lx = i - ly * WC;
         gxs[k] = clampi(wx0 + lx, 0, w - 1);
         gys[k] = clampi(wy0 + ly, 0, h - 1);
-        const float* f = row_ptr(flow_in, flow_pitch, sm.u.ab.fy[ly]) + sm.u.ab.fx[lx] * 2;
-        fa[k] = f[0];
-        fb[k] = f[1];
+        const unsigned fo = ((unsigned)sm.u.ab.fy[ly] * fpf + (unsigned)sm.u.ab.fx[lx] * 2u) * 4u;
+        fa[k] = ld_off(flow_in, fo);
+        fb[k] = ld_off(flow_in, fo + 4u);
+    }
+    float wa[NA], wb[NA];
+    int ix0[NA], iy0[NA];
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        float x = div_n((float)gxs[k] + 0.5f + fa[k], iw);
+        float y = div_n((float)gys[k] + 0.5f + fb[k], ih);
+        // tex_linear<1> (mirror) without its texel-centre shortcut: same value for finite data
+        float xf = x * (float)w - 0.5f, yf = y * (float)h - 0.5f;
+        float fx = floorf(xf), fy = floorf(yf);
+        wa[k] = tex_frac(xf - fx, quant);
+        wb[k] = tex_frac(yf - fy, quant);
+        ix0[k] = (int)fx;
+        iy0[k] = (int)fy;
+        in = in & ((unsigned)ix0[k] < (unsigned)(w - 1)) & ((unsigned)iy0[k] < (unsigned)(h - 1));
+    }
+    float val[NA], tv[NA];
+    if (__all(in)) {
+        float t00[NA], t10[NA], t01[NA], t11[NA];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            const unsigned o = ((unsigned)iy0[k] * pf + (unsigned)ix0[k]) * 4u, o1 = o + pf * 4u;
+            t00[k] = ld_off(src, o);
+            t10[k] = ld_off(src, o + 4u);
+            t01[k] = ld_off(src, o1);
+            t11[k] = ld_off(src, o1 + 4u);
+            tv[k] = ld_off(tgt, ((unsigned)gys[k] * pf + (unsigned)gxs[k]) * 4u);
+        }
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            const float a = wa[k], b = wb[k], oma = 1.0f - a, omb = 1.0f - b;
+            val[k] = oma * omb * t00[k] + a * omb * t10[k] + oma * b * t01[k] + a * b * t11[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            int xa = tex_addr(ix0[k], w, ISA_ADDR_MIRROR), xb = tex_addr(ix0[k] + 1, w, ISA_ADDR_MIRROR);
+            int ya = tex_addr(iy0[k], h, ISA_ADDR_MIRROR), yb = tex_addr(iy0[k] + 1, h, ISA_ADDR_MIRROR);
+            const unsigned oa = (unsigned)ya * pf, ob = (unsigned)yb * pf;
+            const float t00 = ld_off(src, (oa + xa) * 4u), t10 = ld_off(src, (oa + xb) * 4u);
+            const float t01 = ld_off(src, (ob + xa) * 4u), t11 = ld_off(src, (ob + xb) * 4u);
+            const float a = wa[k], b = wb[k], oma = 1.0f - a, omb = 1.0f - b;
+            val[k] = oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11;
+            tv[k] = ld_off(tgt, ((unsigned)gys[k] * pf + (unsigned)gxs[k]) * 4u);
+        }
     }
 #pragma unroll
     for (int k = 0; k < NA; ++k) {
         const int i = tid + k * NT;
-        const int gx = gxs[k], gy = gys[k];
-        float x = div_n((float)gx + 0.5f + fa[k], iw);
-        float y = div_n((float)gy + 0.5f + fb[k], ih);
-        // tex_linear<1> (mirror) without its texel-centre shortcut: same value for finite data
-        float xf = x * (float)w - 0.5f, yf = y * (float)h - 0.5f;
-        float fx = floorf(xf), fy = floorf(yf);
-        float a = tex_frac(xf - fx, quant), b = tex_frac(yf - fy, quant);
-        int xa = tex_addr((int)fx, w, ISA_ADDR_MIRROR), xb = tex_addr((int)fx + 1, w, ISA_ADDR_MIRROR);
-        int ya = tex_addr((int)fy, h, ISA_ADDR_MIRROR), yb = tex_addr((int)fy + 1, h, ISA_ADDR_MIRROR);
-        const float* ra = row_ptr(src, img_pitch, ya);
-        const float* rb = row_ptr(src, img_pitch, yb);
-        float t00 = ra[xa], t10 = ra[xb], t01 = rb[xa], t11 = rb[xb];
-        float oma = 1.0f - a, omb = 1.0f - b;
-        float val = oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11;
-        float tv = row_ptr(tgt, img_pitch, gy)[gx];
         if (i < WR * WC) {
             const int ly = i / WC, lx = i - ly * WC;
-            Lw[ly][lx] = val;
-            Lt[ly][lx] = tv;
+            Lw[ly][lx] = val[k];
+            Lt[ly][lx] = tv[k];
         }
     }
     __syncthreads();
     // B. derivatives
-    const bool mixed = sm.mixed != 0;
+    const bool mixed = sm.mixed != 0, ident = !mixed && sm.ident != 0;
     for (int i = tid; i < DH * DW; i += NT) {
         int ly = i / DW, lx = i - ly * DW;
         int gx = x0 - HW + lx, gy = y0 - HW + ly;
@@ -471,6 +510,34 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
             Ld[0][ly][lx] = 0.0f;
             Ld[1][ly][lx] = 0.0f;
             Ld[2][ly][lx] = 0.0f;
+            continue;
+        }
+        if (ident) {
+            // every tap is the neighbouring texel itself: tap k of derivative pixel (ly, lx) is
+            // warped-tile row ly + 1 + k (column lx + 1 + k)
+            const int r2 = ly + 3, c2 = lx + 3;
+            auto px5 = [&](const float (*L)[WC]) {
+                float t = L[r2][c2 + 2];
+                t -= L[r2][c2 + 1] * 8.0f;
+                t += L[r2][c2 - 1] * 8.0f;
+                t -= L[r2][c2 - 2];
+                t = div_n(t, 1.0 / 12.0);
+                return t;
+            };
+            auto py5 = [&](const float (*L)[WC]) {
+                float t = L[r2 + 2][c2];
+                t -= L[r2 + 1][c2] * 8.0f;
+                t += L[r2 - 1][c2] * 8.0f;
+                t -= L[r2 - 2][c2];
+                t = div_n(t, 1.0 / 12.0);
+                return t;
+            };
+            float t0 = px5(Lw), t1 = px5(Lt);
+            Ld[0][ly][lx] = (t0 + t1) * 0.5f;
+            Ld[2][ly][lx] = Lw[r2][c2] - Lt[r2][c2];
+            t0 = py5(Lw);
+            t1 = py5(Lt);
+            Ld[1][ly][lx] = (t0 + t1) * 0.5f;
             continue;
         }
         const LTap* c = sm.u.ab.Tc[lx];
@@ -685,7 +752,10 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
     float* iz = work + 3 * np;
     const int wp = w * (int)sizeof(float);
     const int hw = window / 2;
-    if (box && hw >= 1 && hw <= 8 && iterations > 0 && flow_pitch == 2 * wp) {
+    // the fused kernel addresses the images and the flow with 32-bit byte offsets
+    const bool off32 = img_pitch % 4 == 0 && flow_pitch % 4 == 0 && (size_t)img_pitch * h < ((size_t)1 << 32) &&
+                       (size_t)flow_pitch * h < ((size_t)1 << 32);
+    if (box && hw >= 1 && hw <= 8 && iterations > 0 && flow_pitch == 2 * wp && off32) {
         // fused iterations, ping-ponging between `flow` and the work area so the last lands in `flow`
         float* bufs[2] = {flow, work};
         int cur = iterations % 2;  // buffer holding the initial field
