@@ -315,12 +315,15 @@ def main():
         torch.cuda.synchronize()
         ms2 = (time.perf_counter() - t0) * 1000.0 / max(1, args.steps)
         st2 = mm.stage_ms()
+        ex_items = mm.exact_items()
         p_gt = psnr(o2, gt)
         mixed[name] = {"value": round(n * (w * h / 1e6) / (ms2 / 1000.0), 3), "ms_per_step": round(ms2, 3),
                        "track_patches_ms": round(st2["track_patches"], 3),
                        "psnr_vs_exact_db": psnr(o2, out), "psnr_vs_gt_db": p_gt,
-                       "psnr_loss_vs_exact_db": round(result["psnr_vs_gt_db"] - p_gt, 4)}
-        log(f"[bench] xc {name}: {ms2:.2f} ms/merge, track {st2['track_patches']:.2f} ms")
+                       "psnr_loss_vs_exact_db": round(result["psnr_vs_gt_db"] - p_gt, 4),
+                       "exact_search_fraction": [round(a / max(1, b), 5) for a, b in ex_items]}
+        log(f"[bench] xc {name}: {ms2:.2f} ms/merge, track {st2['track_patches']:.2f} ms, items searched exactly "
+            f"per level {ex_items}")
         del mm
     if mixed:
         result["mixed"] = mixed

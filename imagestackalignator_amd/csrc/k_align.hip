@@ -149,18 +149,15 @@ __device__ __forceinline__ SsdLds ssd_lds(char* smem, int T, int M) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused PatchTracker.Track for every pair of a level at once: grid (tiles, pairs).
+// Fused PatchTracker.Track for every pair of a level at once.
 // Level images: frames[f] (device array of pointers), row pitch img_pitch bytes.
 // shifts: [meas][tiles] float2 (meas_stride floats per measurement); pair p uses measurement index pk[p].
+// One (pair, tile) item searched by a whole block (LDS from ssd_lds).
 template <int M>
-__global__ void k_track_level(TrackArgs a) {
-    extern __shared__ __align__(16) char smem[];
+__device__ void track_item(const TrackArgs& a, int p, int tile_idx, char* smem) {
     const int T = a.tile, B = T + 2 * M;
     constexpr int S = 2 * M + 1;
     SsdLds L = ssd_lds(smem, T, M);
-    // blockIdx.x = pair, blockIdx.y = tile: the pairs of one tile run together and share the tile
-    // regions of the frames through L2
-    const int p = blockIdx.x, tile_idx = blockIdx.y;
     const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
     const int fa = a.pa[p], fb = a.pb[p], k = a.pk[p];
     float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
@@ -189,9 +186,30 @@ __global__ void k_track_level(TrackArgs a) {
         for (int s = tid; s < S * S; s += nt) a.shift_images[((size_t)p * a.cx * a.cy + tile_idx) * S * S + s] = L.D[s];
     if (tid == 0) {
         float cx, cy;
-        find_min_core(L.D, M, a.threshold, cx, cy);
+        const int dec = find_min_core(L.D, M, a.threshold, cx, cy);
+        if (a.decisions) a.decisions[(size_t)p * a.cx * a.cy + tile_idx] = dec;
         sh[0] = psx + cx;
         sh[1] = psy + cy;
+    }
+}
+
+// grid (pairs, tiles): the pairs of one tile run together and share the tile regions of the frames
+// through L2
+template <int M>
+__global__ void k_track_level(TrackArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    track_item<M>(a, blockIdx.x, blockIdx.y, smem);
+}
+
+// The (pair, tile) items a matrix-core search could not certify (queue[2 i] = pair, queue[2 i + 1] =
+// tile, *count items), searched exactly; each block takes items count-strided.
+template <int M>
+__global__ void k_track_queue(TrackArgs a, const int* __restrict__ queue, const int* __restrict__ count) {
+    extern __shared__ __align__(16) char smem[];
+    const int n = *count;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        track_item<M>(a, queue[2 * i], queue[2 * i + 1], smem);
+        __syncthreads();  // the next item reuses the LDS
     }
 }
 
@@ -258,20 +276,15 @@ __device__ __forceinline__ double tree_step(double v) {
 // few spills outside the row loop) than at 2 with more registers; M = 1 fits 4 as it is
 template <int M> struct TrackColsWaves { static constexpr int v = M >= 2 ? 3 : 1; };
 
-template <int M>
-__global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackArgs a, int pairs, int items_total, int ppb, int lpi,
-                                                   int nblk) {
+// Group blk of the items: item i is (pair i % pairs, tile i / pairs), or, with a queue, (queue[2 i],
+// queue[2 i + 1]).
+template <int M, bool DEC>
+__device__ __forceinline__ void track_cols_group(const TrackArgs& a, int pairs, int items_total, int ppb, int lpi,
+                                                 int blk, const int* __restrict__ queue, char* smem) {
     constexpr int S = 2 * M + 1, RPG = ColsPerGroup<M>::v, G = S / RPG;
     static_assert(G * RPG == S, "groups must tile the lags");
-    extern __shared__ __align__(16) char smem[];
     const int T = a.tile, B = T + 2 * M;
     const int tid = threadIdx.x;
-    // XCD-aware block order: consecutive hardware block ids go round-robin over 8 XCDs; give each
-    // XCD a contiguous run of items so its L2 holds the tile regions its pairs share. The grid has
-    // 8 * per blocks, so (id % 8, id / 8) covers every slot of [0, 8 * per); slots >= nblk idle.
-    const int per = (int)(gridDim.x / 8);
-    const int blk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (blk >= nblk) return;
     const size_t item_bytes = cols_item_bytes(T, M);
     // per-item lane coordinates
     const int pp = tid / lpi, lane = tid - pp * lpi;
@@ -279,7 +292,14 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
     const int item = blk * ppb + pp;
     const bool live = item < items_total;
     const int it = live ? item : items_total - 1;
-    const int tile_idx = it / pairs, p = it - tile_idx * pairs;
+    int tile_idx, p;
+    if (queue) {
+        p = queue[2 * it];
+        tile_idx = queue[2 * it + 1];
+    } else {
+        tile_idx = it / pairs;
+        p = it - tile_idx * pairs;
+    }
     const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
     const int fa = a.pa[p], fb = a.pb[p], k = a.pk[p];
     float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
@@ -463,9 +483,37 @@ __global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackA
         const int mi = km == ~0ull ? -1 : (int)(unsigned)km;
         const float mn = mi < 0 ? 3.402823466e+38f : Dm[mi];
         float cx, cy;
-        find_min_fit(Dm, M, mi, mn, from_ordered_bits(*kmax), a.threshold, cx, cy);
+        const int dec = find_min_fit(Dm, M, mi, mn, from_ordered_bits(*kmax), a.threshold, cx, cy);
+        if (DEC) a.decisions[(size_t)p * a.cx * a.cy + tile_idx] = dec;
         sh[0] = psx + cx;
         sh[1] = psy + cy;
+    }
+}
+
+// DEC: write a.decisions (instantiated separately, so the plain search carries no extra registers)
+template <int M, bool DEC>
+__global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols(TrackArgs a, int pairs, int items_total,
+                                                                         int ppb, int lpi, int nblk) {
+    extern __shared__ __align__(16) char smem[];
+    // XCD-aware block order: consecutive hardware block ids go round-robin over 8 XCDs; give each
+    // XCD a contiguous run of items so its L2 holds the tile regions its pairs share. The grid has
+    // 8 * per blocks, so (id % 8, id / 8) covers every slot of [0, 8 * per); slots >= nblk idle.
+    const int per = (int)(gridDim.x / 8);
+    const int blk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (blk >= nblk) return;
+    track_cols_group<M, DEC>(a, pairs, items_total, ppb, lpi, blk, nullptr, smem);
+}
+
+// The queued items (queue, *count) in groups of ppb; each block takes groups count-strided.
+template <int M, bool DEC>
+__global__ __launch_bounds__(256, TrackColsWaves<M>::v) void k_track_cols_queue(TrackArgs a, int ppb, int lpi,
+                                                                               const int* __restrict__ queue,
+                                                                               const int* __restrict__ count) {
+    extern __shared__ __align__(16) char smem[];
+    const int n = *count, nblk = (n + ppb - 1) / ppb;
+    for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        track_cols_group<M, DEC>(a, 0, n, ppb, lpi, blk, queue, smem);
+        __syncthreads();  // the next group reuses the LDS
     }
 }
 
@@ -824,10 +872,12 @@ __global__ void k_optimal_shifts(float* out, int pitch, const float* r, int n, i
 // host side
 namespace isa {
 int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes,
-                      const float* const* vbox, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
-                      const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks, const float* base,
-                      float* shifts, size_t meas_stride, float threshold, float* shift_images, hipStream_t s);
-int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, hipStream_t s);
+                      const float* const* vbox, const float* const* vrun, int img_pitch, int w, int h, int tile, int M,
+                      int cx, int cy, const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks,
+                      const float* base, float* shifts, size_t meas_stride, float threshold, float* shift_images,
+                      int* decisions, int* queue, int* count, hipStream_t s);
+int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax, hipStream_t s);
+int vbox_run_rows(int h, int T);
 int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s);
 }  // namespace isa
 
@@ -899,8 +949,14 @@ static int launch_track_level_m(const TrackArgs& a, int pairs, hipStream_t s) {
             const ColsShape c = cols_shape(a.tile, M);
             const int items = pairs * a.cx * a.cy;
             const int blocks = div_up(items, c.items_per_block);
-            k_track_cols<M><<<8 * div_up(blocks, 8), c.threads, cols_smem(a.tile, M, c.items_per_block), s>>>(
-                a, pairs, items, c.items_per_block, c.lanes_per_item, blocks);
+            const size_t csm = cols_smem(a.tile, M, c.items_per_block);
+            const unsigned grid = 8 * div_up(blocks, 8);
+            if (a.decisions)
+                k_track_cols<M, true><<<grid, c.threads, csm, s>>>(a, pairs, items, c.items_per_block,
+                                                                   c.lanes_per_item, blocks);
+            else
+                k_track_cols<M, false><<<grid, c.threads, csm, s>>>(a, pairs, items, c.items_per_block,
+                                                                    c.lanes_per_item, blocks);
             return check_launch("track_cols");
         }
     }
@@ -911,13 +967,13 @@ static int launch_track_level_m(const TrackArgs& a, int pairs, hipStream_t s) {
 
 int launch_track_level(const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
                        const int* pa, const int* pb, const int* pk, int pairs, const float* base, float* shifts,
-                       size_t meas_stride, float threshold, float* shift_images, hipStream_t s) {
+                       size_t meas_stride, float threshold, float* shift_images, int* decisions, hipStream_t s) {
     if (ssd_smem(tile, M) > 160 * 1024) {
         set_error("track: tile %d with max shift %d needs too much LDS", tile, M);
         return ISA_ERR_UNSUPPORTED;
     }
     TrackArgs a{frames, img_pitch, w, h, tile, cx, cy, pa, pb, pk, base, shifts, meas_stride, threshold,
-                shift_images};
+                shift_images, decisions};
     switch (M) {
         case 1: return launch_track_level_m<1>(a, pairs, s);
         case 2: return launch_track_level_m<2>(a, pairs, s);
@@ -928,6 +984,43 @@ int launch_track_level(const float* const* frames, int img_pitch, int w, int h, 
         case 7: return launch_track_level_m<7>(a, pairs, s);
         case 8: return launch_track_level_m<8>(a, pairs, s);
     }
+    set_error("track: max shift %d outside [1, %d]", M, kMaxShift);
+    return ISA_ERR_UNSUPPORTED;
+}
+
+// Exact search of the queued items (queue, *count: from launch_track_mfma). The grid is fixed; its
+// blocks take the items count-strided, so the count stays on the device.
+int launch_track_queue(const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
+                       const int* pa, const int* pb, const int* pk, const float* base, float* shifts,
+                       size_t meas_stride, float threshold, int* decisions, const int* queue, const int* count,
+                       hipStream_t s) {
+    const size_t smem = ssd_smem(tile, M);
+    if (smem > 160 * 1024) {
+        set_error("track: tile %d with max shift %d needs too much LDS", tile, M);
+        return ISA_ERR_UNSUPPORTED;
+    }
+    TrackArgs a{frames, img_pitch, w, h, tile, cx, cy, pa, pb, pk, base, shifts, meas_stride, threshold,
+                nullptr, decisions};
+    if ((M == 1 || M == 2 || M == 4) && tile >= 4 && tile <= 64 && 64 % tile == 0) {
+        const ColsShape c = cols_shape(tile, M);
+        const size_t csm = cols_smem(tile, M, c.items_per_block);
+        const int ppb = c.items_per_block, lpi = c.lanes_per_item;
+#define ISA_TCQ(MM)                                                                                          \
+    if (M == MM) {                                                                                         \
+        if (decisions) k_track_cols_queue<MM, true><<<4096, c.threads, csm, s>>>(a, ppb, lpi, queue, count);  \
+        else k_track_cols_queue<MM, false><<<4096, c.threads, csm, s>>>(a, ppb, lpi, queue, count);           \
+    }
+        ISA_TCQ(1) ISA_TCQ(2) ISA_TCQ(4)
+#undef ISA_TCQ
+        return check_launch("track_cols_queue");
+    }
+    const int grid = 2048, threads = block_threads(tile, M);
+#define ISA_TQ(MM)                                                                   \
+    case MM:                                                                       \
+        k_track_queue<MM><<<grid, threads, smem, s>>>(a, queue, count);            \
+        return check_launch("track_queue");
+    switch (M) { ISA_TQ(1) ISA_TQ(2) ISA_TQ(3) ISA_TQ(4) ISA_TQ(5) ISA_TQ(6) ISA_TQ(7) ISA_TQ(8) }
+#undef ISA_TQ
     set_error("track: max shift %d outside [1, %d]", M, kMaxShift);
     return ISA_ERR_UNSUPPORTED;
 }
@@ -1096,10 +1189,11 @@ int isa_upsample_shifts(const float* in2_dev, float* out2_dev, int in_pitch, int
                                 new_count_x, new_count_y, old_tile_size, new_tile_size, isa::to_stream(stream));
 }
 
-int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
-                 int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
-                 float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
-                 int xc_precision, isa_stream_t stream) {
+int isa_track_xc_ex(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+                    int preshift_pitch, int width, int height, int tile_size, int max_shift,
+                    const float base_shift_ref[2], float base_rotation_ref, const float base_shift_track[2],
+                    float base_rotation_track, float threshold, int xc_precision, int* decisions_dev,
+                    int* exact_items, isa_stream_t stream) {
     // PatchTracker.Track through the fused level kernel with one pair (frame 0 = reference, frame 1 = tracked).
     ISA_REQUIRE(img_track_dev && img_ref_dev && preshift2_dev && width > 0 && height > 0 && tile_size > 0,
                 "isa_track: bad arguments");
@@ -1109,34 +1203,44 @@ int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_p
     ISA_REQUIRE(cx > 0 && cy > 0, "isa_track: image smaller than one tile");
     hipStream_t s = isa::to_stream(stream);
     ISA_REQUIRE(img_pitch % 4 == 0 && img_pitch >= width * 4, "isa_track: bad image pitch");
+    const bool split = xc_precision == ISA_XC_BF16X3 || xc_precision == ISA_XC_FP16X3;
+    const bool mfma = xc_precision != ISA_XC_FP64;
     struct Scratch {
         const float* frames[2];
         const unsigned* xplanes[2];
         const float* vbox[2];
-        int pa, pb, pk, pad;
+        const float* vrun[2];
+        int pa, pb, pk, count;
         int chunk[16];
         float base[8];
-    } h{{img_ref_dev, img_track_dev}, {nullptr, nullptr}, {nullptr, nullptr}, 0, 1, 0, 0,
+    } h{{img_ref_dev, img_track_dev}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, 0, 1, 0, 0,
         {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
         {base_shift_ref[0], base_shift_ref[1], (float)std::sin((double)base_rotation_ref),
          (float)std::cos((double)base_rotation_ref), base_shift_track[0], base_shift_track[1],
          (float)std::sin((double)base_rotation_track), (float)std::cos((double)base_rotation_track)}};
-    const size_t head = (sizeof(Scratch) + 255) & ~(size_t)255;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t head = al(sizeof(Scratch));
     const size_t row = (size_t)cx * 2 * sizeof(float);
-    const size_t plane = (((size_t)img_pitch / 4) * height * sizeof(unsigned) + 255) & ~(size_t)255;
-    const bool split = xc_precision == ISA_XC_BF16X3 || xc_precision == ISA_XC_FP16X3;
-    const size_t shifts_bytes = (row * cy + 255) & ~(size_t)255;
-    const bool mfma = xc_precision != ISA_XC_FP64;
-    const size_t vbytes = mfma ? (((size_t)(height - tile_size + 1) * width * sizeof(float) + 255) & ~(size_t)255) : 0;
+    const size_t shifts_bytes = al(row * cy);
+    const size_t plane = split ? al(((size_t)img_pitch / 4) * height * sizeof(unsigned)) : 0;
+    const size_t vbytes = mfma ? al((size_t)(height - tile_size + 1) * width * sizeof(float)) : 0;
+    const size_t rbytes = mfma ? al((size_t)isa::vbox_run_rows(height, tile_size) * width * sizeof(float)) : 0;
+    const size_t qbytes = mfma ? al((size_t)cx * cy * 2 * sizeof(int)) : 0;
     char* d = nullptr;
-    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d, head + shifts_bytes + (split ? 2 * plane : 0) + vbytes, s),
+    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d, head + shifts_bytes + 2 * plane + vbytes + rbytes + qbytes, s),
                            "isa_track: allocation"));
     float* dn = (float*)(d + head);
+    char* o = d + head + shifts_bytes;
     if (split) {
-        h.xplanes[0] = (const unsigned*)(d + head + shifts_bytes);
-        h.xplanes[1] = (const unsigned*)(d + head + shifts_bytes + plane);
+        h.xplanes[0] = (const unsigned*)o;
+        h.xplanes[1] = (const unsigned*)(o + plane);
     }
-    if (mfma) h.vbox[1] = (const float*)(d + head + shifts_bytes + (split ? 2 * plane : 0));
+    o += 2 * plane;
+    if (mfma) {
+        h.vbox[1] = (const float*)o;
+        h.vrun[1] = (const float*)(o + vbytes);
+    }
+    int* queue = (int*)(o + vbytes + rbytes);
     const Scratch* sd = (const Scratch*)d;
     int rc = isa::check_hip(hipMemcpyAsync(d, &h, sizeof(Scratch), hipMemcpyHostToDevice, s), "isa_track: upload");
     if (rc == ISA_OK)
@@ -1146,23 +1250,41 @@ int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_p
         rc = isa::split_planes(xc_precision, f ? img_track_dev : img_ref_dev, (unsigned*)h.xplanes[f],
                                ((size_t)img_pitch / 4) * height, s);
     if (mfma && rc == ISA_OK)
-        rc = isa::vbox_energies(img_track_dev, img_pitch, width, height, tile_size, (float*)h.vbox[1], s);
+        rc = isa::vbox_energies(img_track_dev, img_pitch, width, height, tile_size, (float*)h.vbox[1],
+                                (float*)h.vrun[1], s);
     if (rc == ISA_OK)
-        rc = xc_precision == ISA_XC_FP64
-                 ? isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
-                                           &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, s)
-                 : isa::launch_track_mfma(xc_precision, sd->frames, sd->xplanes, sd->vbox, img_pitch, width, height,
-                                          tile_size, max_shift, cx, cy, &sd->pa, &sd->pb, &sd->pk, sd->chunk, 1,
-                                          sd->base, dn, 0, threshold, nullptr, s);
+        rc = !mfma ? isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
+                                             &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, decisions_dev, s)
+                   : isa::launch_track_mfma(xc_precision, sd->frames, sd->xplanes, sd->vbox, sd->vrun, img_pitch, width,
+                                            height, tile_size, max_shift, cx, cy, &sd->pa, &sd->pb, &sd->pk, sd->chunk, 1,
+                                            sd->base, dn, 0, threshold, nullptr, decisions_dev, queue,
+                                            (int*)&sd->count, s);
+    if (mfma && rc == ISA_OK)
+        rc = isa::launch_track_queue(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
+                                     &sd->pb, &sd->pk, sd->base, dn, 0, threshold, decisions_dev, queue, &sd->count, s);
     if (rc == ISA_OK)
         rc = isa::check_hip(hipMemcpy2DAsync(preshift2_dev, preshift_pitch, dn, row, row, cy, hipMemcpyDeviceToDevice, s),
                             "isa_track: copy back");
-    // the host copy `h` must outlive the upload: synchronise before returning
+    int count = 0;
+    if (rc == ISA_OK && exact_items)
+        rc = isa::check_hip(hipMemcpyAsync(&count, &sd->count, sizeof(int), hipMemcpyDeviceToHost, s),
+                            "isa_track: count");
+    // the host copies `h` and `count` must outlive the transfers: synchronise before returning
     int rs = isa::check_hip(hipStreamSynchronize(s), "isa_track: sync");
     hipError_t fe = hipFreeAsync(d, s);
     if (rc == ISA_OK) rc = rs;
     if (rc == ISA_OK) rc = isa::check_hip(fe, "isa_track: free");
+    if (rc == ISA_OK && exact_items) *exact_items = mfma ? count : cx * cy;
     return rc;
+}
+
+int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+                 int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
+                 float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
+                 int xc_precision, isa_stream_t stream) {
+    return isa_track_xc_ex(img_track_dev, img_ref_dev, img_pitch, preshift2_dev, preshift_pitch, width, height,
+                           tile_size, max_shift, base_shift_ref, base_rotation_ref, base_shift_track,
+                           base_rotation_track, threshold, xc_precision, nullptr, nullptr, stream);
 }
 
 int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,

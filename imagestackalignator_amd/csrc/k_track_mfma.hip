@@ -76,14 +76,16 @@ template <int PREC> __global__ void k_split_planes(const float* src, unsigned* d
 }
 
 // Vertical window energies V(y0, x) = sum_{y = y0}^{y0 + T - 1} src(y, x)^2 for y0 in [0, h - T]:
-// one thread per column and run of kVRun window starts (a T-term sum, then slid in FP64).
+// one thread per column and run of kVRun window starts (a T-term sum, then slid in FP64). runmax gets
+// the largest energy of each (run, column), rounded up: the squares are exact in FP64, so a window's
+// error before its rounding to float is at most (T + 2 kVRun) 2^-53 times that maximum.
 constexpr int kVRun = 256;
-__global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, float* dst) {
+__global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y0 = blockIdx.y * kVRun, nv = h - T + 1;
     if (x >= w || y0 >= nv) return;
     auto at = [&](int y) { return (double)row_ptr(src, pitch, min(y, h - 1))[x]; };
-    double e = 0.0;
+    double e = 0.0, emax = 0.0;
     for (int yb = y0; yb < y0 + T; yb += 8) {  // loads of a batch issued together
         double v[8];
 #pragma unroll
@@ -104,9 +106,11 @@ __global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, floa
             const int y = yb + k;
             if (y >= y1) break;
             dst[(size_t)y * w + x] = (float)e;
+            emax = fmax(emax, e);
             if (y + T < h) e = (e - a[k] * a[k]) + b[k] * b[k];
         }
     }
+    runmax[(size_t)blockIdx.y * w + x] = (float)(emax * (1.0 + 0x1p-20));
 }
 
 // LDS: the reference plane is live until the product loop ends; the partial products, C, the
@@ -115,11 +119,12 @@ struct MfmaLayout {
     int RW, RH;        // padded reference plane: RH rows x RW columns (+ one zero row)
     int Bp;            // track-block width rounded up to 16
     size_t off_plane;  // reference plane, 4 bytes per element (packed hi|lo, or fp32)
-    size_t off_red;    // per-wave partial products [kWaves][MB][256]     (after the product loop)
+    size_t off_red;    // running sum of the waves' partial products [MB][256] (after the product loop)
     size_t off_c;      // C [chunk][S*S]                                  (after the product loop)
-    size_t off_col;    // column energies [chunk][B][S] (float)           (after the product loop)
+    size_t off_col;    // column energies [chunk][B][S] (float); aliases red (after C is formed)
     size_t off_d;      // D [chunk][S*S]                                  (after the product loop)
-    size_t off_misc;   // R2 (double), scale, per-wave sums of r^2 (double)
+    size_t off_w;      // certificate half-widths W [chunk][S*S] (float)   (after the product loop)
+    size_t off_misc;   // R2 (double), scale, per-wave sums of r^2 (double), per-pair energy maxima
     size_t bytes;
 };
 
@@ -133,27 +138,35 @@ __host__ __device__ inline MfmaLayout mfma_layout(int T, int M) {
     L.off_plane = 0;
     const size_t plane = al((size_t)(L.RH + 1) * L.RW * 4);
     size_t o = 0;
+    // the running sum of the waves' partial products and, after it, the column energies share bytes
     L.off_red = o;
-    o += al((size_t)kWaves * MB * 256 * 4);
+    L.off_col = o;
+    o += std::max(al((size_t)MB * 256 * 4), al((size_t)kPairsPerChunk * B * S * 4));
     L.off_c = o;
     o += al((size_t)kPairsPerChunk * S * S * 4);
-    L.off_col = o;
-    o += al((size_t)kPairsPerChunk * B * S * 4);
     L.off_d = o;
     o += al((size_t)kPairsPerChunk * S * S * 4);
+    L.off_w = o;
+    o += al((size_t)kPairsPerChunk * S * S * 4);
     L.off_misc = o > plane ? o : plane;
-    L.bytes = L.off_misc + 64;
+    L.bytes = L.off_misc + 128;
     return L;
 }
 
 struct XPlanes {
     const unsigned* const* frames;  // per frame: packed split plane, same pixel pitch as the level image
     const float* const* vbox;       // per frame: vertical T-window energies, (h - T + 1) x w
+    const float* const* vrun;       // per frame: run maxima of vbox, ceil((h - T + 1) / kVRun) x w
+    int* queue;                     // items left to the exact search: (pair, tile) int pairs
+    int* count;                     // number of queued items
 };
 
 // chunk c of the level: up to 16 pair indices sharing one reference frame (-1 = empty slot)
+// waves per SIMD the register allocation must allow: 4 for the 25-lag search, 3 above
+template <int M> struct MfmaWaves { static constexpr int v = M <= 2 ? 4 : 3; };
+
 template <int PREC, int M>
-__global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes xp, const int* __restrict__ chunk_pairs,
+__global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(TrackArgs a, XPlanes xp, const int* __restrict__ chunk_pairs,
                                                             int n_chunks, int nblk) {
     constexpr int S = 2 * M + 1, SS = S * S, MB = (SS + 15) / 16;
     constexpr int NT = kWaves * 64;
@@ -165,6 +178,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
     const int per = (int)(gridDim.x / 8);
     const int blk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (blk >= nblk) return;
+    if (threadIdx.x < kPairsPerChunk) ((unsigned*)(smem + mfma_layout(a.tile, M).off_misc + 64))[threadIdx.x] = 0u;
     const int tile_idx = blk / n_chunks, chunk = blk - tile_idx * n_chunks;
     const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
     const int* cp = chunk_pairs + (size_t)chunk * kPairsPerChunk;
@@ -338,26 +352,30 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
             }
         }
     }
-    // ---- 3. combine the waves' partial products (fixed order w = 0..3); lane holds D[4q+i][j]
-    __syncthreads();  // every wave is done with the reference plane, whose bytes are reused below
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) red[((size_t)wave * MB + mb) * 256 + (4 * q + i) * 16 + j] = acc[mb][i];
-    __syncthreads();
+    // ---- 3. combine the waves' partial products in the fixed order ((w0 + w1) + w2) + w3: wave w adds
+    // its own to the running sum in LDS, one wave after the other; lane holds C[4q+i][j]
     float* Cm = (float*)(smem + L.off_c);
     const float inv_sc = 1.0f / sc;
-    for (int i = tid; i < MB * 256; i += NT) {
-        const int mb = i >> 8, e = i & 255, row = e >> 4, col = e & 15, s = mb * 16 + row;
-        if (s >= SS) continue;
-        float v = red[(size_t)mb * 256 + e];
+    __syncthreads();  // every wave is done with the reference plane, whose bytes are reused below
+    for (int w = 0; w < kWaves; ++w) {
+        if (wave == w) {
 #pragma unroll
-        for (int w = 1; w < kWaves; ++w) v += red[((size_t)w * MB + mb) * 256 + e];
-        Cm[col * SS + s] = v * inv_sc;
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int e = mb * 256 + (4 * q + i) * 16 + j, s = mb * 16 + 4 * q + i;
+                    const float v = w == 0 ? acc[mb][i] : red[e] + acc[mb][i];
+                    if (w + 1 < kWaves) red[e] = v;
+                    else if (s < SS) Cm[j * SS + s] = v * inv_sc;
+                }
+        }
+        __syncthreads();  // the last one also ends the reads of red, whose bytes the column energies reuse
     }
     // ---- 4. window energies of each pair's track block: per column, the S row windows (a T-row
-    // sum, then slid one row at a time), all loads of a column issued in batches of 8
+    // sum, then slid one row at a time), all loads of a column issued in batches of 8. rmax[jj] gets
+    // the largest window energy any entry of pair jj was slid through (for the certificate).
     float* colE = (float*)(smem + L.off_col);
+    unsigned* rmax = (unsigned*)(smem + L.off_misc + 64);
     for (int task = tid; task < kPairsPerChunk * B; task += NT) {
         const int jj = task / B, c = task - jj * B;
         const int pp = cp[jj];
@@ -369,42 +387,73 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
                        a.base[fb * 4 + 3], sh[0], sh[1], ox, oy);
         const int bx0 = tx * T + ox, by0 = ty * T + oy;
         float* ce = colE + ((size_t)jj * B + c) * S;
+        float rm;
         if (bx0 >= 0 && by0 >= 0 && bx0 + B <= a.w && by0 + B <= a.h) {
             // block inside the image: the S row windows of this column are entries of the
             // vertical window-energy image
             const float* vb = xp.vbox[fb] + (size_t)by0 * a.w + bx0 + c;
 #pragma unroll
             for (int k = 0; k < S; ++k) ce[k] = vb[(size_t)k * a.w];
-            continue;
-        }
-        const float* img = a.frames[fb];
-        const int gx = clampi(bx0 + c, 0, a.w - 1), gy0 = by0;
-        auto px = [&](int y) { return row_ptr(img, a.img_pitch, clampi(gy0 + y, 0, a.h - 1))[gx]; };
-        float e = 0.0f;
-        for (int y0 = 0; y0 < T; y0 += 8) {
-            float v[8];
+            const float* vr = xp.vrun[fb] + bx0 + c;
+            rm = fmaxf(vr[(size_t)(by0 / kVRun) * a.w], vr[(size_t)((by0 + S - 1) / kVRun) * a.w]);
+        } else {
+            // the exact search's column energies (FP64 chain, then slid), rows and columns clamped
+            const float* img = a.frames[fb];
+            const int gx = clampi(bx0 + c, 0, a.w - 1), gy0 = by0;
+            auto px = [&](int y) { return (double)row_ptr(img, a.img_pitch, clampi(gy0 + y, 0, a.h - 1))[gx]; };
+            double e = 0.0, eb = 0.0;
+            for (int y0 = 0; y0 < T; y0 += 8) {
+                double v[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) v[k] = y0 + k < T ? px(y0 + k) : 0.0f;
+                for (int k = 0; k < 8; ++k) v[k] = y0 + k < T ? px(y0 + k) : 0.0;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) e = fmaf(v[k], v[k], e);
-        }
-        ce[0] = e;
-        float u[S - 1], w[S - 1];
+                for (int k = 0; k < 8; ++k) e = fma(v[k], v[k], e);
+            }
+            ce[0] = (float)e;
+            eb = e;
+            double u[S - 1], v[S - 1];
 #pragma unroll
-        for (int k = 0; k + 1 < S; ++k) {
-            u[k] = px(k);
-            w[k] = px(k + T);
-        }
+            for (int k = 0; k + 1 < S; ++k) {
+                u[k] = px(k);
+                v[k] = px(k + T);
+            }
 #pragma unroll
-        for (int k = 0; k + 1 < S; ++k) {
-            e = (e - u[k] * u[k]) + w[k] * w[k];
-            ce[k + 1] = e;
+            for (int k = 0; k + 1 < S; ++k) {
+                e = (e - u[k] * u[k]) + v[k] * v[k];
+                eb += v[k] * v[k];  // energy of the whole column block bounds every window
+                ce[k + 1] = (float)e;
+            }
+            rm = (float)(eb * (1.0 + 0x1p-20));
         }
+        atomicMax(&rmax[jj], __float_as_uint(rm));  // non-negative floats order as their bits
     }
     __syncthreads();
-    // ---- 5. D(s) = sum r^2 + sum_x colE[x + ix][iy] - 2 C(s)
+    // ---- 5. D(s) = sum r^2 + sum_x colE[x + ix][iy] - 2 C(s), and the half-width W(s) of an interval
+    // holding both D(s) and the exact search's value of it:
+    //  - cross terms: each product of the split operands differs from r t by at most c_split |r t|
+    //    (plus, in fp16, terms from the subnormal spacing); a wave's accumulator receives hgt - 3
+    //    products (or fewer, whatever order and grouping the matrix cores use inside an
+    //    instruction) and the four waves' sums are added in fp32, so any summation tree has height
+    //    <= hgt and the fp32 error is at most gamma(hgt) = hgt u / (1 - hgt u) (u = 2^-23: every
+    //    rounding faithful) times the sum of |products|; sum |r(u - s) t(u)| <= sqrt(R2 TT(s));
+    //  - window energies: rounding to float (2^-24), the FP64 slide (T + 2 kVRun) 2^-53 rmax per
+    //    column and the FP64 sum over the T columns;
+    //  - the FP64 evaluation of R2 + TT - 2C on both sides and the rounding of D to float on both.
     float* Dm = (float*)(smem + L.off_d);
+    float* Wm = (float*)(smem + L.off_w);
     const double R2 = *R2s;
+    constexpr int kProd = PREC == ISA_XC_FP32 ? 4 : 16;   // products per matrix instruction
+    constexpr int kPerBlk = PREC == ISA_XC_FP32 ? 4 : 3;  // instructions per 16-column block and row
+    const int hgt = kProd * ((B + kWaves - 1) / kWaves) * NB * kPerBlk + (kWaves - 1) + (PREC == ISA_XC_FP32 ? 1 : 0);
+    const float gam = (float)(hgt * 0x1p-23 / (1.0 - hgt * 0x1p-23));
+    const float csplit = PREC == ISA_XC_BF16X3 ? 3.1f * 0x1p-16f : (PREC == ISA_XC_FP16X3 ? 3.1f * 0x1p-22f : 0.0f);
+    const float eta = 0x1p-24f;  // fp16: bound on the absolute error of a split from the subnormal spacing
+    // W is evaluated in fp32: every quantity below is rounded up by a relative 2^-20 or more, which
+    // covers the roundings of the fp32 evaluation (a few 2^-24 each)
+    const float sr2 = fmaxf((float)(sqrt(R2) * (1.0 + 0x1p-20)), R2 > 0.0 ? 1.17549435e-38f : 0.0f);
+    const float r2f = (float)(R2 * (1.0 + 0x1p-20));
+    const float kslide = (float)((double)T * (T + 2 * kVRun) * 0x1p-53 * 1.01);
+    const float crel = (csplit + 1.03f * gam) * (1.0f + 0x1p-18f);
     for (int task = tid; task < kPairsPerChunk * SS; task += NT) {
         const int jj = task / SS, s = task - jj * SS;
         if (cp[jj] < 0) continue;
@@ -412,22 +461,82 @@ __global__ __launch_bounds__(kWaves * 64) void k_track_mfma(TrackArgs a, XPlanes
         const float* ce = colE + ((size_t)jj * B + ix) * S + iy;
         double tt = 0.0;
         for (int x = 0; x < T; ++x) tt += (double)ce[(size_t)x * S];
-        Dm[jj * SS + s] = (float)(R2 + tt - 2.0 * (double)Cm[jj * SS + s]);
+        const double dd = R2 + tt - 2.0 * (double)Cm[jj * SS + s];
+        Dm[jj * SS + s] = (float)dd;
+        const float ttf = (float)tt * (1.0f + 0x1p-20f), rm = __uint_as_float(rmax[jj]);
+        const float ett = 0x1p-23f * ttf + kslide * rm + (float)T * 0x1p-52f * ttf;
+        const float ttu = ttf + ett;
+        const float P = sr2 * sqrtf(ttu) * (1.0f + 0x1p-20f);
+        float ec = crel * P;
+        if (PREC == ISA_XC_FP16X3) ec += (1.1f * eta * T * (sc * sr2 + sqrtf(ttu)) + 3.0f * eta * eta * T * T) / sc;
+        const float wd = 1.06f * (2.0f * ec + ett + 0x1p-44f * (r2f + ttf + 2.0f * P) + 0x1p-22f * fabsf((float)dd)) + 1e-30f;
+        Wm[jj * SS + s] = wd;
     }
     __syncthreads();
-    // ---- 6. findMinimum per pair
-    if (tid < kPairsPerChunk) {
-        const int pp = cp[tid];
-        if (pp >= 0) {
+    // ---- 6. findMinimum per pair, when the intervals decide the integer minimum and the threshold
+    // test as the exact search would; otherwise the item is queued for the exact search. Sixteen lanes
+    // per pair (aligned groups within a wave) scan the lags and combine by shuffles.
+    {
+        static_assert(kPairsPerChunk * 16 == NT, "sixteen lanes per pair");
+        const int jj = tid >> 4, l = tid & 15;
+        const int pp = cp[jj];
+        const float* D = Dm + jj * SS;
+        const float* W = Wm + jj * SS;
+        unsigned long long key = ~0ull;
+        float mx = -3.402823466e+38f;
+        int fin = 1;
+        if (pp >= 0)
+            for (int s = l; s < SS; s += 16) {
+                const float v = D[s];
+                key = min(key, argmin_key(v, s));  // first index of the smallest value, as find_min_core
+                mx = fmaxf(mx, v);
+                fin &= isfinite(v) && isfinite(W[s]);
+            }
+        for (int o = 8; o >= 1; o >>= 1) {
+            key = min(key, (unsigned long long)__shfl_xor((long long)key, o, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, o, 16));
+            fin &= __shfl_xor(fin, o, 16);
+        }
+        const int mi = key == ~0ull ? -1 : (int)(unsigned)key;
+        const float mn = mi >= 0 ? D[mi] : 3.402823466e+38f;
+        const double hi = mi >= 0 ? (double)mn + W[mi] : 0.0, lo = mi >= 0 ? (double)mn - W[mi] : 0.0;
+        // every other lag lies strictly above the minimum's interval
+        double lother = 1e300, lmax = -1e300, hmax = -1e300;
+        if (pp >= 0 && mi >= 0)
+            for (int s = l; s < SS; s += 16) {
+                const double lv = (double)D[s] - W[s];
+                lmax = fmax(lmax, lv);
+                hmax = fmax(hmax, (double)D[s] + W[s]);
+                if (s != mi) lother = fmin(lother, lv);
+            }
+        for (int o = 8; o >= 1; o >>= 1) {
+            lother = fmin(lother, __shfl_xor(lother, o, 16));
+            lmax = fmax(lmax, __shfl_xor(lmax, o, 16));
+            hmax = fmax(hmax, __shfl_xor(hmax, o, 16));
+        }
+        if (pp >= 0 && l == 0) {
             float* sh = a.shifts + (size_t)a.pk[pp] * a.meas_stride + (size_t)tile_idx * 2;
             if (a.shift_images)
-                for (int s = 0; s < SS; ++s)
-                    a.shift_images[((size_t)pp * a.cx * a.cy + tile_idx) * SS + s] = Dm[tid * SS + s];
-            float cx, cy;
-            find_min_core(Dm + tid * SS, M, a.threshold, cx, cy);
-            const float psx = sh[0], psy = sh[1];
-            sh[0] = psx + cx;
-            sh[1] = psy + cy;
+#pragma unroll 1
+                for (int s = 0; s < SS; ++s) a.shift_images[((size_t)pp * a.cx * a.cy + tile_idx) * SS + s] = D[s];
+            // threshold + min > max decided the same way for every value in the intervals
+            const double thr = a.threshold;
+            const bool zero_sure = (thr + lo) - hmax > 0x1p-20 * (fabs(thr) + fabs(lo) + fabs(hmax));
+            const bool keep_sure = lmax - (thr + hi) > 0x1p-20 * (fabs(thr) + fabs(hi) + fabs(lmax));
+            const bool ok = fin && mi >= 0 && lother > hi && (zero_sure || keep_sure);
+            if (ok) {
+                float cx, cy;
+                const int dec = find_min_fit(D, M, mi, mn, mx, a.threshold, cx, cy);
+                if (a.decisions) a.decisions[(size_t)pp * a.cx * a.cy + tile_idx] = dec;
+                const float psx = sh[0], psy = sh[1];
+                sh[0] = psx + cx;
+                sh[1] = psy + cy;
+            } else {
+                // the queue holds one slot per (pair, tile) of the level, so q stays in range
+                const int q = atomicAdd(xp.count, 1);
+                xp.queue[2 * q] = pp;
+                xp.queue[2 * q + 1] = tile_idx;
+            }
         }
     }
 }
@@ -493,11 +602,14 @@ int split_planes(int precision, const float* src, unsigned* dst, size_t count, h
 }
 
 // Vertical T-window energies of a level image (count of window starts: h - T + 1 rows of w).
-int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, hipStream_t s) {
+// runmax: vbox_run_rows(h, T) x w floats.
+int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax, hipStream_t s) {
     if (h < T) return ISA_OK;
-    k_vbox_sq<<<dim3(div_up(w, 256), div_up(h - T + 1, kVRun)), 256, 0, s>>>(src, pitch, w, h, T, dst);
+    k_vbox_sq<<<dim3(div_up(w, 256), div_up(h - T + 1, kVRun)), 256, 0, s>>>(src, pitch, w, h, T, dst, runmax);
     return check_launch("vbox_energies");
 }
+
+int vbox_run_rows(int h, int T) { return h < T ? 0 : div_up(h - T + 1, kVRun); }
 
 bool track_mfma_supported(int tile, int M) {
     const MfmaLayout L = mfma_layout(tile, M);
@@ -505,18 +617,22 @@ bool track_mfma_supported(int tile, int M) {
 }
 
 // xplanes: per frame, the packed split plane of the level image (same pixel pitch; unused for fp32);
-// vbox: per frame, vbox_energies of the level image (dense, row length w).
+// vbox, vrun: per frame, vbox_energies of the level image (dense, row length w) and its run maxima.
+// Items whose integer minimum and threshold test the error bounds cannot decide are appended to
+// queue (2 ints each; room for every (pair, tile) of the level) and counted in *count, which the
+// caller clears; launch_track_queue then searches them exactly.
 int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes,
-                      const float* const* vbox, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
-                      const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks, const float* base,
-                      float* shifts, size_t meas_stride, float threshold, float* shift_images, hipStream_t s) {
+                      const float* const* vbox, const float* const* vrun, int img_pitch, int w, int h, int tile, int M,
+                      int cx, int cy, const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks,
+                      const float* base, float* shifts, size_t meas_stride, float threshold, float* shift_images,
+                      int* decisions, int* queue, int* count, hipStream_t s) {
     if (!track_mfma_supported(tile, M)) {
         set_error("track (mfma): tile %d with max shift %d is not supported (block wider than 80 pixels)", tile, M);
         return ISA_ERR_UNSUPPORTED;
     }
     TrackArgs a{frames, img_pitch, w, h, tile, cx, cy, pa, pb, pk, base, shifts, meas_stride, threshold,
-                shift_images};
-    XPlanes xp{xplanes, vbox};
+                shift_images, decisions};
+    XPlanes xp{xplanes, vbox, vrun, queue, count};
     switch (precision) {
         case ISA_XC_BF16X3: return launch_mfma_p<ISA_XC_BF16X3>(a, xp, M, chunks, n_chunks, s);
         case ISA_XC_FP16X3: return launch_mfma_p<ISA_XC_FP16X3>(a, xp, M, chunks, n_chunks, s);

@@ -26,7 +26,11 @@ size_t highpass_workspace(int w, int h, float hp, float hps);
 int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, hipStream_t s);
 int launch_track_level(const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
                        const int* pa, const int* pb, const int* pk, int pairs, const float* base, float* shifts,
-                       size_t meas_stride, float threshold, float* shift_images, hipStream_t s);
+                       size_t meas_stride, float threshold, float* shift_images, int* decisions, hipStream_t s);
+int launch_track_queue(const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
+                       const int* pa, const int* pb, const int* pk, const float* base, float* shifts,
+                       size_t meas_stride, float threshold, int* decisions, const int* queue, const int* count,
+                       hipStream_t s);
 int launch_upsample_all(const float* in, float* out, size_t meas_stride, int m, int ol, int nl, int ocx, int ocy,
                         int ncx, int ncy, int ot, int nt, hipStream_t s);
 bool minimize_full_factor(const int* rows, int n, int m, double* L);
@@ -43,10 +47,12 @@ size_t tap_table_bytes(int w, int h);
 int track_chunks(const int* pa, int m, int n, int* out_chunks);
 int track_chunks_capacity(int m, int n);
 int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes,
-                      const float* const* vbox, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
-                      const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks, const float* base,
-                      float* shifts, size_t meas_stride, float threshold, float* shift_images, hipStream_t s);
-int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, hipStream_t s);
+                      const float* const* vbox, const float* const* vrun, int img_pitch, int w, int h, int tile, int M,
+                      int cx, int cy, const int* pa, const int* pb, const int* pk, const int* chunks, int n_chunks,
+                      const float* base, float* shifts, size_t meas_stride, float threshold, float* shift_images,
+                      int* decisions, int* queue, int* count, hipStream_t s);
+int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax, hipStream_t s);
+int vbox_run_rows(int h, int T);
 int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s);
 bool track_mfma_supported(int tile, int M);
 int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s);
@@ -127,8 +133,12 @@ struct isa_merger {
     size_t xplane_off[ISA_MAX_LEVELS]{};
     float* vbox = nullptr;  // vertical window energies of every frame at every level (MFMA cross terms)
     size_t vbox_off[ISA_MAX_LEVELS]{}, vbox_len[ISA_MAX_LEVELS]{};
+    float* vrun = nullptr;  // their run maxima (error bounds of the MFMA search)
+    size_t vrun_off[ISA_MAX_LEVELS]{}, vrun_len[ISA_MAX_LEVELS]{};
+    int* xc_queue = nullptr;  // (pair, tile) items the MFMA search leaves to the exact search
+    int* xc_count = nullptr;  // per level: number of such items
     // device tables: pa, pb, pk (m each), rows (2m), base (L * n * 4 floats), frame pointers (L * n),
-    // accumulate pointer arrays (raw n, mask n, flow n)
+    // accumulate pointer arrays (raw n, mask n, flow n), MFMA pair chunks, vbox and run-maximum pointers
     char* tab = nullptr;
     char* tab_host = nullptr;  // pinned staging copy
     size_t tab_bytes = 0;
@@ -138,6 +148,7 @@ struct isa_merger {
     float stage_ms[ST_COUNT]{};
     bool have_times = false;
     int ref = 0;
+    int m_last = 0;  // measurements (pairs) of the last run
 
     ~isa_merger() {
         for (void* p : allocs) (void)hipFree(p);
@@ -164,7 +175,8 @@ struct isa_merger {
 namespace {
 
 struct TabLayout {
-    size_t pa, pb, pk, rows, lfull, base, frames, xframes, vframes, acc_raw, acc_mask, acc_flow, chunks, bytes;
+    size_t pa, pb, pk, rows, lfull, base, frames, xframes, vframes, rframes, acc_raw, acc_mask, acc_flow, chunks,
+        bytes;
 };
 
 TabLayout tab_layout(int m, int n, int L) {
@@ -180,6 +192,7 @@ TabLayout tab_layout(int m, int n, int L) {
     t.frames = o; o += al((size_t)L * n * sizeof(void*));
     t.xframes = o; o += al((size_t)L * n * sizeof(void*));
     t.vframes = o; o += al((size_t)L * n * sizeof(void*));
+    t.rframes = o; o += al((size_t)L * n * sizeof(void*));
     t.acc_raw = o; o += al(n * sizeof(void*));
     t.acc_mask = o; o += al(n * sizeof(void*));
     t.acc_flow = o; o += al(n * sizeof(void*));
@@ -325,6 +338,16 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
             tot += (size_t)frames * M->vbox_len[i];
         }
         A(M->vbox, tot);
+        tot = 0;
+        for (int i = 0; i < M->L; ++i) {
+            const int lw = width / P.resize_level[i], lh = height / P.resize_level[i];
+            M->vrun_off[i] = tot;
+            M->vrun_len[i] = (size_t)std::max(isa::vbox_run_rows(lh, P.tile_size[i]), 1) * lw;
+            tot += (size_t)frames * M->vrun_len[i];
+        }
+        A(M->vrun, tot);
+        A(M->xc_queue, (size_t)M->m_alloc * mtiles * 2);
+        A(M->xc_count, ISA_MAX_LEVELS);
     }
     if (rc == ISA_OK && P.use_high_pass) {
         char* ws = nullptr;
@@ -379,6 +402,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     int m = isa::shift_tables(n, ref, P.tracking_strategy, P.tracking_block_size, pairs.data(), index.data(),
                               rows.data());
     ISA_REQUIRE(m > 0 && m <= M->m_alloc, "isa_merger_run: invalid tracking strategy for %d frames", n);
+    M->m_last = m;
     TabLayout tl = tab_layout(M->m_alloc, n, M->L);
     ISA_TRY(isa::check_hip(hipEventSynchronize(M->upload_done), "isa_merger_run: wait"));
     char* th = M->tab_host;
@@ -398,6 +422,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const float** fr = (const float**)(th + tl.frames);
     const unsigned** xfr = (const unsigned**)(th + tl.xframes);
     const float** vfr = (const float**)(th + tl.vframes);
+    const float** rfr = (const float**)(th + tl.rframes);
     for (int i = 0; i < M->L; ++i) {
         const int l = P.resize_level[i];
         const size_t lsz = (size_t)(w / l) * (h / l);
@@ -412,6 +437,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             fr[(size_t)i * n + f] = l > 1 ? M->lev[i] + (size_t)f * lsz : M->trk + (size_t)f * np;
             xfr[(size_t)i * n + f] = M->xplanes ? M->xplanes + M->xplane_off[i] + (size_t)f * lsz : nullptr;
             vfr[(size_t)i * n + f] = M->vbox ? M->vbox + M->vbox_off[i] + (size_t)f * M->vbox_len[i] : nullptr;
+            rfr[(size_t)i * n + f] = M->vrun ? M->vrun + M->vrun_off[i] + (size_t)f * M->vrun_len[i] : nullptr;
         }
     }
     const uint16_t** acc_raw = (const uint16_t**)(th + tl.acc_raw);
@@ -442,6 +468,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const int* d_chunks = (const int*)(M->tab + tl.chunks);
     const unsigned* const* d_xframes = (const unsigned* const*)(M->tab + tl.xframes);
     const float* const* d_vframes = (const float* const*)(M->tab + tl.vframes);
+    const float* const* d_rframes = (const float* const*)(M->tab + tl.rframes);
     const float* const* d_frames = (const float* const*)(M->tab + tl.frames);
 
     float taps_track[99], taps_acc[99], taps_st[99];
@@ -466,6 +493,8 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     // ---- 2. TrackPatches: levels coarse -> fine
     const size_t mtiles = (size_t)M->mcx * M->mcy, mstride = mtiles * 2;
     ISA_TRY(isa::check_hip(hipMemsetAsync(M->shifts, 0, (size_t)m * mstride * sizeof(float), s), "memset"));
+    if (M->xc_count)
+        ISA_TRY(isa::check_hip(hipMemsetAsync(M->xc_count, 0, ISA_MAX_LEVELS * sizeof(int), s), "memset"));
     for (int i = M->L - 1; i >= 0; --i) {
         const int l = P.resize_level[i], lw = w / l, lh = h / l;
         (void)hipEventRecord(M->lev_ev[i][0], s);
@@ -487,18 +516,26 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             for (int f = 0; f < n; ++f)
                 ISA_TRY(isa::vbox_energies(l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np,
                                            lw * (int)sizeof(float), lw, lh, P.tile_size[i],
-                                           M->vbox + M->vbox_off[i] + (size_t)f * M->vbox_len[i], s));
+                                           M->vbox + M->vbox_off[i] + (size_t)f * M->vbox_len[i],
+                                           M->vrun + M->vrun_off[i] + (size_t)f * M->vrun_len[i], s));
         (void)hipEventRecord(M->lev_ev[i][1], s);
-        if (P.xc_precision == ISA_XC_FP64)
+        if (P.xc_precision == ISA_XC_FP64) {
             ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
                                             P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk, m,
                                             d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold,
-                                            nullptr, s));
-        else
+                                            nullptr, nullptr, s));
+        } else {
             ISA_TRY(isa::launch_track_mfma(P.xc_precision, d_frames + (size_t)i * n, d_xframes + (size_t)i * n,
-                                           d_vframes + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i], P.max_shift[i], M->cx[i],
+                                           d_vframes + (size_t)i * n, d_rframes + (size_t)i * n,
+                                           lw * (int)sizeof(float), lw, lh, P.tile_size[i], P.max_shift[i], M->cx[i],
                                            M->cy[i], d_pa, d_pb, d_pk, d_chunks, n_chunks, d_base + (size_t)i * n * 4,
-                                           M->shifts, mstride, P.matching_threshold, nullptr, s));
+                                           M->shifts, mstride, P.matching_threshold, nullptr, nullptr, M->xc_queue,
+                                           M->xc_count + i, s));
+            ISA_TRY(isa::launch_track_queue(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
+                                            P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk,
+                                            d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold,
+                                            nullptr, M->xc_queue, M->xc_count + i, s));
+        }
         (void)hipEventRecord(M->lev_ev[i][2], s);
         ISA_TRY(isa::launch_minimize(M->shifts, mstride, M->one_to_one, nullptr, nullptr, d_rows, d_lfull,
                                      M->cx[i] * M->cy[i], n, m, s));
@@ -623,6 +660,19 @@ int isa_merger_level_ms(isa_merger* M, int level, float ms[3]) {
         (void)hipEventElapsedTime(&ms[j], M->lev_ev[level][j], M->lev_ev[level][j + 1]);
     }
     return ISA_OK;
+}
+
+int isa_merger_exact_items(isa_merger* M, int level, int* exact_items, int* items) {
+    ISA_REQUIRE(M && exact_items && items && level >= 0 && level < M->L, "isa_merger_exact_items: bad arguments");
+    if (!M->have_times) return ISA_ERR_ARG;
+    *items = M->m_last * M->cx[level] * M->cy[level];
+    if (!M->xc_count) {
+        *exact_items = *items;  // the exact search ran on every item
+        return ISA_OK;
+    }
+    ISA_TRY(isa::check_hip(hipEventSynchronize(M->ev[ST_COUNT]), "isa_merger_exact_items: sync"));
+    return isa::check_hip(hipMemcpy(exact_items, M->xc_count + level, sizeof(int), hipMemcpyDeviceToHost),
+                          "isa_merger_exact_items: copy");
 }
 
 int isa_merger_buffer(isa_merger* M, int which, int index, void** ptr, int* pitch, int* width, int* height) {

@@ -21,8 +21,10 @@ __device__ __forceinline__ void tile_offset_sc(int tx, int ty, int tile, int w, 
 // findMinimum (R:Kernels/kernel.cu:413-546) on an S x S map.
 // Sub-pixel position of the minimum mi (value mn; mx the largest value) of an S x S shift image:
 // the 3x3 quadratic fit of findMinimum, or 0 at the window edge / for a flat image.
-__device__ void find_min_fit(const float* v, int M, int mi, float mn, float mx, float threshold, float& cx,
-                             float& cy) {
+// Returns the decision code 2 mi + z (z = 1 when the threshold test zeroes the shift), -1 without a
+// minimum: the integer part of the search, which the matrix-core variants certify.
+__device__ int find_min_fit(const float* v, int M, int mi, float mn, float mx, float threshold, float& cx,
+                            float& cy) {
     const float FA11[9] = {1.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f, 2.0f / 4.0f, -4.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f,
                            -2.0f / 4.0f, 1.0f / 4.0f};
     const float FA22[9] = {1.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f, -2.0f / 4.0f, -4.0f / 4.0f, -2.0f / 4.0f, 1.0f / 4.0f,
@@ -64,12 +66,14 @@ __device__ void find_min_fit(const float* v, int M, int mi, float mn, float mx, 
         cx -= (float)M;
         cy -= (float)M;
     }
-    if (threshold + mn > mx) { cx = 0.0f; cy = 0.0f; }
+    const bool zeroed = threshold + mn > mx;
+    if (zeroed) { cx = 0.0f; cy = 0.0f; }
+    return mi < 0 ? -1 : 2 * mi + (zeroed ? 1 : 0);
 }
 
 // findMinimum: the first index of the smallest value (values >= FLT_MAX and NaN never qualify),
 // the largest value, then find_min_fit.
-__device__ void find_min_core(const float* v, int M, float threshold, float& cx, float& cy) {
+__device__ int find_min_core(const float* v, int M, float threshold, float& cx, float& cy) {
     const int S = 2 * M + 1, n = S * S;
     float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
     int mi = -1;
@@ -78,7 +82,7 @@ __device__ void find_min_core(const float* v, int M, float threshold, float& cx,
         mx = fmaxf(mx, val);
         if (val < mn) { mn = val; mi = i; }
     }
-    find_min_fit(v, M, mi, mn, mx, threshold, cx, cy);
+    return find_min_fit(v, M, mi, mn, mx, threshold, cx, cy);
 }
 
 // Order-preserving map of a non-NaN float to unsigned (for integer min / max).
@@ -107,6 +111,7 @@ struct TrackArgs {
     size_t meas_stride;
     float threshold;
     float* shift_images;  // optional: [pair][tile][S*S]
+    int* decisions;       // optional: [pair][tile] decision code of find_min_fit
 };
 
 

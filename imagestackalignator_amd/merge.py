@@ -65,6 +65,17 @@ class BurstMerger:
         n = lib().isa_merger_stage_ms(self._h, buf, 16)
         return {STAGES[i]: buf[i] for i in range(n)}
 
+    def exact_items(self):
+        """Per pyramid level (finest first) of the last run: ((pair, tile) items the exact FP64 search
+        decided, all items). With a matrix-core precision, the exact items are those whose error bounds
+        left the integer minimum or the threshold test open."""
+        out = []
+        for level in range(self.params.n_levels):
+            n_exact, items = C.c_int(), C.c_int()
+            call("isa_merger_exact_items", self._h, level, C.byref(n_exact), C.byref(items))
+            out.append((n_exact.value, items.value))
+        return out
+
     def buffer(self, which: int, index: int = 0):
         """(device pointer, pitch bytes, width, height) of an internal buffer."""
         p, pitch, w, h = C.c_void_p(), C.c_int(), C.c_int(), C.c_int()

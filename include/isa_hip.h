@@ -212,6 +212,16 @@ int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_p
                  int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
                  float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
                  int xc_precision, isa_stream_t stream);
+/* isa_track_xc with two outputs for checking the mixed-precision search. No counterpart in the reference.
+ * decisions_dev (device, tiles ints, may be NULL): per tile the integer part of findMinimum, 2 * argmin + 1 when the
+ * threshold test zeroes the shift, -1 without a minimum. The matrix-core forms bound the error of every lag's value;
+ * where the bounds cannot decide the argmin and the threshold test as the exact search would, the tile is searched
+ * exactly, so the decisions equal isa_track's. exact_items (host, may be NULL): tiles searched exactly. */
+int isa_track_xc_ex(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
+                    int preshift_pitch, int width, int height, int tile_size, int max_shift,
+                    const float base_shift_ref[2], float base_rotation_ref, const float base_shift_track[2],
+                    float base_rotation_track, float threshold, int xc_precision, int* decisions_dev,
+                    int* exact_items, isa_stream_t stream);
 
 /* ShiftCollection tables (R:PEFStudioDX/ShiftCollection.cs:113-496). pairs_out: 2*m ints (reference, toTrack) in
  * GetShiftPairs order; index_out: n*n ints (-1 where unmeasured); rows_out: 2*m ints (a, b): measurement row k
@@ -304,6 +314,10 @@ int isa_merger_stage_ms(isa_merger* m, float* ms, int capacity);
 /* Milliseconds of the last run at pyramid level `level` (0 = finest): ms[0] shift up-sampling and level
  * images, ms[1] the fused tile-search launch (all pairs), ms[2] the ShiftMinimizer launch. */
 int isa_merger_level_ms(isa_merger* m, int level, float ms[3]);
+/* (pair, tile) items of the last run at pyramid level `level`, and how many of them the exact FP64 search decided
+ * (all of them for ISA_XC_FP64; with a matrix-core precision, those whose error bounds left the integer minimum or
+ * the threshold test open). No counterpart in the reference. */
+int isa_merger_exact_items(isa_merger* m, int level, int* exact_items, int* items);
 
 /* ------------------------------------------------------------------------
  * Differentiable merge (no counterpart in the reference, whose merge is piecewise constant in the
