@@ -1,16 +1,18 @@
 // Optical flow: flow field from tile shifts, warping, derivatives and the
 // Lucas-Kanade step (R:Kernels/opticalFlow.cu; driver R:PEFStudioDX/OpticalFlow.cs:147-164).
+#include <cmath>
+
 #include "isa_device.h"
 #include "isa_host.h"
 
 namespace {
 
-// CreateFlowFieldFromTiles (R:Kernels/opticalFlow.cu:47-93).
+// CreateFlowFieldFromTiles (R:Kernels/opticalFlow.cu:47-93). c, s: cos and sin of the rotation, each
+// evaluated in double and rounded once to float (by the launcher, once per field).
 __global__ void k_flow_from_tiles(float* flow, int flow_pitch, const float* tiles, int tiles_pitch, int tw, int th,
-                                  int w, int h, float bsx, float bsy, float rot, int quant) {
+                                  int w, int h, float bsx, float bsy, float c, float s, int quant) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
-    float c = cos_r(rot), s = sin_r(rot);
     float fx = c * -bsx - s * -bsy;
     float fy = s * -bsx + c * -bsy;
     float pcx = (float)(x - w / 2), pcy = (float)(y - h / 2);
@@ -691,8 +693,9 @@ namespace isa {
 
 int launch_flow_from_tiles(float* flow, int flow_pitch, const float* tiles, int tiles_pitch, int tw, int th, int w,
                            int h, float bsx, float bsy, float rot, int quant, hipStream_t s) {
+    const float c = (float)std::cos((double)rot), sn = (float)std::sin((double)rot);
     k_flow_from_tiles<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(flow, flow_pitch, tiles, tiles_pitch,
-                                                                                 tw, th, w, h, bsx, bsy, rot, quant);
+                                                                                 tw, th, w, h, bsx, bsy, c, sn, quant);
     return check_launch("flow_from_tiles");
 }
 

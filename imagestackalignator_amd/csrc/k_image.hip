@@ -398,6 +398,82 @@ __global__ __launch_bounds__(GF_BX* GF_BY) void k_gauss_fixed(const float* __res
     row_ptr(out, out_pitch, y)[x] = acc;
 }
 
+// The same filter in registers for widths that are a multiple of 4 (16-byte aligned rows): lane L of a
+// wave holds columns x..x+3 (x = 4 (64 blockIdx.x + L)) of one row as a float4 and takes its C
+// horizontal neighbours on each side from lanes L - 1 and L + 1 (the wave's edge lanes load one more
+// float4); a wave walks GR_ROWS output rows, keeping the horizontal results of the last N input rows.
+// Rows and columns are clamped as in k_gauss_fixed, and the products and sums are the same in the
+// same order, so the output is identical.
+constexpr int GR_ROWS = 16, GR_WAVES = 4;
+
+template <int N>
+__global__ __launch_bounds__(64 * GR_WAVES) void k_gauss_rows(const float* __restrict__ in, int in_pitch,
+                                                             float* __restrict__ out, int out_pitch, int w, int h,
+                                                             Taps taps) {
+    constexpr int C = N / 2;
+    static_assert(C <= 4, "neighbours come from the adjacent lanes only");
+    const int lane = threadIdx.x, x = 4 * (blockIdx.x * 64 + lane);
+    const int y0 = (blockIdx.y * GR_WAVES + threadIdx.y) * GR_ROWS;
+    if (y0 >= h) return;  // whole wave
+    const bool live = x < w;
+    const int xl = live ? x : w - 4;  // lanes past the row end load a valid vector, write nothing
+    float t[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) t[k] = taps.t[k];
+    const unsigned pf = (unsigned)in_pitch / 4u;
+    auto row_h = [&](int ry, float hv[4]) {
+        const unsigned ro = (unsigned)clampi(ry, 0, h - 1) * pf;
+        const float4 v = *(const float4*)(in + ro + xl);
+        float4 lv = v, rv = v;  // neighbour vectors x-4..x-1 and x+4..x+7
+        if (lane == 0 && xl >= 4) lv = *(const float4*)(in + ro + xl - 4);
+        if (lane == 63 && xl + 4 < w) rv = *(const float4*)(in + ro + xl + 4);
+        float c[4 + 2 * C];
+        c[C + 0] = v.x;
+        c[C + 1] = v.y;
+        c[C + 2] = v.z;
+        c[C + 3] = v.w;
+        const float u[4] = {__shfl_up(v.x, 1), __shfl_up(v.y, 1), __shfl_up(v.z, 1), __shfl_up(v.w, 1)};
+        const float d[4] = {__shfl_down(v.x, 1), __shfl_down(v.y, 1), __shfl_down(v.z, 1), __shfl_down(v.w, 1)};
+        const float lvv[4] = {lv.x, lv.y, lv.z, lv.w}, rvv[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+        for (int k = 1; k <= C; ++k) {
+            // column x - k (clamped to 0) and x + 3 + k (clamped to w - 1)
+            c[C - k] = xl - k < 0 ? v.x : (lane == 0 ? lvv[4 - k] : u[4 - k]);
+            c[C + 3 + k] = xl + 3 + k > w - 1 ? v.w : (lane == 63 ? rvv[k - 1] : d[k - 1]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc += t[k] * c[j + k];
+            hv[j] = acc;
+        }
+    };
+    float ring[N][4];
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) row_h(y0 - C + k, ring[k]);
+    const int y1 = min(y0 + GR_ROWS, h);
+    for (int y = y0; y < y1; ++y) {
+        // ring[N - 1] <- input row y + C; ring[k] holds input row y - C + k
+        row_h(y + C, ring[N - 1]);
+        if (live) {
+            float o[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int k = 0; k < N; ++k) acc += t[k] * ring[k][j];
+                o[j] = acc;
+            }
+            *(float4*)(out + (unsigned)y * ((unsigned)out_pitch / 4u) + x) = make_float4(o[0], o[1], o[2], o[3]);
+        }
+#pragma unroll
+        for (int k = 0; k + 1 < N; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) ring[k][j] = ring[k + 1][j];
+    }
+}
+
 __global__ void k_resize_box(const float* in, int in_pitch, float* out, int out_pitch, int ow, int oh, int l) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= ow || y >= oh) return;
@@ -587,6 +663,16 @@ static Taps make_taps(const float* t, int n) {
 int isa::launch_gauss1(const float* in, int in_pitch, float* out, int out_pitch, float* tmp, int w, int h,
                        const float* taps, int n, hipStream_t s) {
     Taps tp = make_taps(taps, n);
+    const bool vec = w % 4 == 0 && w >= 8 && in_pitch % 16 == 0 && out_pitch % 16 == 0 && ((size_t)in & 15) == 0 &&
+                     ((size_t)out & 15) == 0 && (size_t)in_pitch * h < ((size_t)1 << 32) &&
+                     (size_t)out_pitch * h < ((size_t)1 << 32);
+    if (vec && (n == 3 || n == 5 || n == 7)) {
+        const dim3 g(div_up(w / 4, 64), div_up(h, GR_ROWS * GR_WAVES)), b(64, GR_WAVES);
+        if (n == 3) k_gauss_rows<3><<<g, b, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp);
+        if (n == 5) k_gauss_rows<5><<<g, b, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp);
+        if (n == 7) k_gauss_rows<7><<<g, b, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp);
+        return check_launch("gauss_rows");
+    }
     const dim3 grid(div_up(w, GF_BX), div_up(h, GF_BY)), block(GF_BX, GF_BY);
     switch (n) {
         case 3: k_gauss_fixed<3><<<grid, block, 0, s>>>(in, in_pitch, out, out_pitch, w, h, tp); return check_launch("gauss3");
@@ -696,13 +782,13 @@ int isa_gauss_filter_border(const float* in_dev, int in_pitch, float* out_dev, i
     ISA_REQUIRE(width > 0 && height > 0 && in_dev && out_dev && tmp_dev && taps && n_taps > 0 && n_taps <= 99,
                 "isa_gauss_filter_border: bad arguments");
     ISA_REQUIRE(channels == 1 || channels == 3 || channels == 4, "isa_gauss_filter_border: channels must be 1, 3 or 4");
+    hipStream_t s = isa::to_stream(stream);
+    // one channel: the merger's filter (every form gives the same values)
+    if (channels == 1) return isa::launch_gauss1(in_dev, in_pitch, out_dev, out_pitch, tmp_dev, width, height, taps,
+                                                 n_taps, s);
     Taps tp = make_taps(taps, n_taps);
     dim3 b(32, 8), g(isa::div_up(width, 32), isa::div_up(height, 8));
-    hipStream_t s = isa::to_stream(stream);
-    if (channels == 1) {
-        k_gauss_h<1><<<g, b, 0, s>>>(in_dev, in_pitch, tmp_dev, width, height, tp);
-        k_gauss_v<1><<<g, b, 0, s>>>(tmp_dev, out_dev, out_pitch, width, height, tp);
-    } else if (channels == 3) {
+    if (channels == 3) {
         k_gauss_h<3><<<g, b, 0, s>>>(in_dev, in_pitch, tmp_dev, width, height, tp);
         k_gauss_v<3><<<g, b, 0, s>>>(tmp_dev, out_dev, out_pitch, width, height, tp);
     } else {

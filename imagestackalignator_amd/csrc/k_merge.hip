@@ -69,13 +69,14 @@ __global__ void k_robustness(const float* ref, const float* mov, int img_pitch, 
 #pragma unroll
             for (int c = 0; c < 3; ++c) mm[c] += q[c];
         }
+    // divisions by 9 and 3 through div_n: the IEEE quotients, without the division sequence
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        mr[c] /= 9.0f;
-        mm[c] /= 9.0f;
+        mr[c] = div_n(mr[c], 1.0 / 9.0);
+        mm[c] = div_n(mm[c], 1.0 / 9.0);
     }
     float meandist = fabsf(mr[0] - mm[0]) + fabsf(mr[1] - mm[1]) + fabsf(mr[2] - mm[2]);
-    meandist /= 3.0f;
+    meandist = div_n(meandist, 1.0 / 3.0);
     maxx *= 0.5f * meandist;
     maxy *= 0.5f * meandist;
     minx *= 0.5f * meandist;
@@ -91,7 +92,7 @@ __global__ void k_robustness(const float* ref, const float* mov, int img_pitch, 
     float out[4];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        float sd = sqrtf(sr[c] / 9.0f);
+        float sd = sqrtf(div_n(sr[c], 1.0 / 9.0));
         float smd = sqrtf(alpha * mr[c] + beta);
         if (c == 1) smd = smd / sqrtf(2.0f);
         float dist = fabsf(mr[c] - mm[c]);

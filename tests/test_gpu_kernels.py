@@ -93,10 +93,14 @@ def test_gamma_gray(isa, oracle, gpu):
     np.testing.assert_array_equal(host(g), gref)
 
 
-@pytest.mark.parametrize("sigma,ch", [(0.5, 1), (1.0, 3), (2.5, 1), (1.0, 4)])
-def test_gauss_filter(isa, oracle, gpu, sigma, ch):
+# one channel with w % 4 == 0 takes the register form (float4 lanes, neighbours by shuffles): widths
+# of one vector, of a partial wave, of several waves (wave-edge lanes load their neighbours) and
+# ragged heights; other widths the LDS form
+@pytest.mark.parametrize("sigma,ch,h,w", [(0.5, 1, 29, 61), (1.0, 3, 29, 61), (2.5, 1, 29, 61), (1.0, 4, 29, 61),
+                                          (0.5, 1, 29, 8), (0.5, 1, 70, 260), (0.5, 1, 33, 1028),
+                                          (1.0, 1, 70, 516), (0.7, 1, 65, 772), (1.5, 1, 40, 516)])
+def test_gauss_filter(isa, oracle, gpu, sigma, ch, h, w):
     rng = np.random.default_rng(4)
-    h, w = 29, 61
     img = rng.uniform(0, 1, (h, w, ch) if ch > 1 else (h, w)).astype(np.float32)
     taps = (C.c_float * 99)()
     n = isa.lib().isa_gaussian_taps(sigma, taps, 99)
