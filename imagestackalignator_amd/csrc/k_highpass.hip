@@ -339,7 +339,7 @@ __global__ void k_col_twiddles(double* ctab, int h, int kyn, int ny) {
     ctab[((size_t)y * ny + j) * 2 + 1] = s;
 }
 
-constexpr int kColChunks = 16;
+constexpr int kColChunks = 64;
 
 // Column DFT at the band rows, partial over a chunk of rows. U: [h][2 kxn] with
 // U[.][2k] = sum x cos, U[.][2k+1] = sum x sin (= -Im of the row DFT).
@@ -351,6 +351,7 @@ __global__ void k_col_dft_partial(const float* __restrict__ U, const double* __r
     if (k >= kxn) return;
     int y0 = (int)((long long)h * ch / kColChunks), y1 = (int)((long long)h * (ch + 1) / kColChunks);
     double sr = 0.0, si = 0.0;
+#pragma unroll 4
     for (int y = y0; y < y1; ++y) {
         double c = ctab[((size_t)y * ny + j) * 2], s = ctab[((size_t)y * ny + j) * 2 + 1];
         double ur = U[(size_t)y * 2 * kxn + 2 * k], ui = -(double)U[(size_t)y * 2 * kxn + 2 * k + 1];
@@ -388,6 +389,7 @@ __global__ void k_col_idft(const double* __restrict__ Y, const double* __restric
     int y = blockIdx.y;
     if (k >= kxn || y >= h) return;
     double sr = 0.0, si = 0.0;
+#pragma unroll 4
     for (int j = 0; j < ny; ++j) {
         double c = ctab[((size_t)y * ny + j) * 2], s = ctab[((size_t)y * ny + j) * 2 + 1];
         double yr = Y[((size_t)j * kxn + k) * 2], yi = Y[((size_t)j * kxn + k) * 2 + 1];

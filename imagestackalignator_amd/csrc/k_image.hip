@@ -485,6 +485,61 @@ __global__ void k_resize_box(const float* in, int in_pitch, float* out, int out_
     row_ptr(out, out_pitch, y)[x] = s / (float)(l * l);
 }
 
+// Box means of 2x2, 4x4 and 8x8 blocks in one pass (pyramid levels 2, 4 and 8; any output may be
+// null): one thread per 8x8 block, loaded as 16 float4; every mean sums its pixels row by row in
+// increasing x and divides by l^2, as k_resize_box, so the values are identical.
+__global__ void k_resize_box248(const float* __restrict__ in, int in_pitch, float* o2, int p2, float* o4, int p4,
+                                float* o8, int p8, int bw, int bh) {
+    const int bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y * blockDim.y + threadIdx.y;
+    if (bx >= bw || by >= bh) return;
+    float v[8][8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float4* r = (const float4*)(row_ptr(in, in_pitch, by * 8 + j) + bx * 8);
+        const float4 a = r[0], b = r[1];
+        v[j][0] = a.x; v[j][1] = a.y; v[j][2] = a.z; v[j][3] = a.w;
+        v[j][4] = b.x; v[j][5] = b.y; v[j][6] = b.z; v[j][7] = b.w;
+    }
+    if (o8) {
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s += v[j][i];
+        row_ptr(o8, p8, by)[bx] = s / 64.0f;
+    }
+    if (o4)
+#pragma unroll
+        for (int qy = 0; qy < 2; ++qy) {
+            float q[2];
+#pragma unroll
+            for (int qx = 0; qx < 2; ++qx) {
+                float s = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) s += v[qy * 4 + j][qx * 4 + i];
+                q[qx] = s / 16.0f;
+            }
+            *(float2*)(row_ptr(o4, p4, by * 2 + qy) + bx * 2) = make_float2(q[0], q[1]);
+        }
+    if (o2)
+#pragma unroll
+        for (int qy = 0; qy < 4; ++qy) {
+            float q[4];
+#pragma unroll
+            for (int qx = 0; qx < 4; ++qx) {
+                float s = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) s += v[qy * 2 + j][qx * 2 + i];
+                q[qx] = s / 4.0f;
+            }
+            *(float4*)(row_ptr(o2, p2, by * 4 + qy) + bx * 4) = make_float4(q[0], q[1], q[2], q[3]);
+        }
+}
+
 __global__ void k_structure_tensor(const float* dx, const float* dy, int dpitch, float* out, int out_pitch, int w,
                                    int h) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -637,6 +692,9 @@ int launch_raw_to_gray(const uint16_t* raw, int raw_pitch, float* gray, int gray
 
 int launch_gauss1(const float* in, int in_pitch, float* out, int out_pitch, float* tmp, int w, int h,
                   const float* taps, int n, hipStream_t s);
+bool resize_box248_ok(const float* in, int in_pitch, int w, int h);
+int launch_resize_box248(const float* in, int in_pitch, float* o2, int p2, float* o4, int p4, float* o8, int p8, int w,
+                         int h, hipStream_t s);
 
 int launch_resize_box(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int l, hipStream_t s) {
     int ow = w / l, oh = h / l;
@@ -658,6 +716,20 @@ static Taps make_taps(const float* t, int n) {
     tp.n = n;
     for (int i = 0; i < 99; ++i) tp.t[i] = i < n ? t[i] : 0.0f;
     return tp;
+}
+
+bool isa::resize_box248_ok(const float* in, int in_pitch, int w, int h) {
+    return w % 8 == 0 && h % 8 == 0 && in_pitch % 16 == 0 && ((size_t)in & 15) == 0;
+}
+
+// Levels 2, 4 and 8 of a w x h image in one pass (resize_box248_ok; outputs may be null, their pitches
+// multiples of 16, 8 and 4 bytes).
+int isa::launch_resize_box248(const float* in, int in_pitch, float* o2, int p2, float* o4, int p4, float* o8, int p8,
+                              int w, int h, hipStream_t s) {
+    const int bw = w / 8, bh = h / 8;
+    k_resize_box248<<<dim3(div_up(bw, 64), div_up(bh, 4)), dim3(64, 4), 0, s>>>(in, in_pitch, o2, p2, o4, p4, o8, p8,
+                                                                                bw, bh);
+    return check_launch("resize_box248");
 }
 
 int isa::launch_gauss1(const float* in, int in_pitch, float* out, int out_pitch, float* tmp, int w, int h,
@@ -803,6 +875,18 @@ int isa_resize_box(const float* in_dev, int in_pitch, float* out_dev, int out_pi
     ISA_REQUIRE(width > 0 && height > 0 && level >= 1 && in_dev && out_dev && width / level > 0 && height / level > 0,
                 "isa_resize_box: bad arguments");
     return isa::launch_resize_box(in_dev, in_pitch, out_dev, out_pitch, width, height, level, isa::to_stream(stream));
+}
+
+int isa_resize_box_levels(const float* in_dev, int in_pitch, int width, int height, float* out2_dev, int pitch2,
+                          float* out4_dev, int pitch4, float* out8_dev, int pitch8, isa_stream_t stream) {
+    ISA_REQUIRE(in_dev && width > 0 && height > 0, "isa_resize_box_levels: bad arguments");
+    ISA_REQUIRE(isa::resize_box248_ok(in_dev, in_pitch, width, height),
+                "isa_resize_box_levels: width and height must be multiples of 8, the input 16-byte aligned");
+    ISA_REQUIRE((!out2_dev || (pitch2 % 16 == 0 && ((size_t)out2_dev & 15) == 0)) &&
+                    (!out4_dev || (pitch4 % 8 == 0 && ((size_t)out4_dev & 7) == 0)),
+                "isa_resize_box_levels: level-2 rows must be 16-byte and level-4 rows 8-byte aligned");
+    return isa::launch_resize_box248(in_dev, in_pitch, out2_dev, pitch2, out4_dev, pitch4, out8_dev, pitch8, width,
+                                     height, isa::to_stream(stream));
 }
 
 int isa_compute_structure_tensor(const float* dx_dev, const float* dy_dev, int dxdy_pitch, float* out3_dev,

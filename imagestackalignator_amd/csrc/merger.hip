@@ -20,6 +20,9 @@ int launch_raw_to_gray(const uint16_t* raw, int raw_pitch, float* gray, int gray
 int launch_gauss1(const float* in, int in_pitch, float* out, int out_pitch, float* tmp, int w, int h,
                   const float* taps, int n, hipStream_t s);
 int launch_resize_box(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int l, hipStream_t s);
+bool resize_box248_ok(const float* in, int in_pitch, int w, int h);
+int launch_resize_box248(const float* in, int in_pitch, float* o2, int p2, float* o4, int p4, float* o8, int p8, int w,
+                         int h, hipStream_t s);
 int launch_kernel_param4(float* img, int w, int h, int pitch, float dth, float dtr, float kd, float kn, float ks,
                          float kh, hipStream_t s);
 size_t highpass_workspace(int w, int h, float hp, float hps);
@@ -495,6 +498,24 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     ISA_TRY(isa::check_hip(hipMemsetAsync(M->shifts, 0, (size_t)m * mstride * sizeof(float), s), "memset"));
     if (M->xc_count)
         ISA_TRY(isa::check_hip(hipMemsetAsync(M->xc_count, 0, ISA_MAX_LEVELS * sizeof(int), s), "memset"));
+    // pyramid levels 2, 4 and 8 of every frame in one pass over its tracking image, when they are
+    // the only levels above 1 (each at most once) and the frame tiles into 8 x 8 blocks
+    float* lev248[3] = {nullptr, nullptr, nullptr};
+    bool pyr248 = isa::resize_box248_ok(M->trk, wp, w, h) && (np * sizeof(float)) % 16 == 0;
+    for (int i = 0; i < M->L && pyr248; ++i) {
+        const int l = P.resize_level[i];
+        if (l == 1) continue;
+        const int k = l == 2 ? 0 : (l == 4 ? 1 : (l == 8 ? 2 : -1));
+        if (k < 0 || lev248[k]) pyr248 = false;
+        else lev248[k] = M->lev[i];
+    }
+    if (pyr248 && (lev248[0] || lev248[1] || lev248[2])) {
+        for (int f = 0; f < n; ++f) {
+            auto at = [&](int k, int l) { return lev248[k] ? lev248[k] + (size_t)f * (w / l) * (h / l) : nullptr; };
+            ISA_TRY(isa::launch_resize_box248(M->trk + (size_t)f * np, wp, at(0, 2), (w / 2) * 4, at(1, 4), (w / 4) * 4,
+                                              at(2, 8), (w / 8) * 4, w, h, s));
+        }
+    }
     for (int i = M->L - 1; i >= 0; --i) {
         const int l = P.resize_level[i], lw = w / l, lh = h / l;
         (void)hipEventRecord(M->lev_ev[i][0], s);
@@ -503,7 +524,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                              M->cy[i + 1], M->cx[i], M->cy[i], P.tile_size[i + 1], P.tile_size[i], s));
             std::swap(M->shifts, M->shifts_tmp);
         }
-        if (l > 1)
+        if (l > 1 && !pyr248)
             for (int f = 0; f < n; ++f)
                 ISA_TRY(isa::launch_resize_box(M->trk + (size_t)f * np, wp, M->lev[i] + (size_t)f * lw * lh,
                                                lw * (int)sizeof(float), w, h, l, s));

@@ -176,6 +176,11 @@ int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int c
 /* NPP ResizeSqrPixel(1/level, SuperSampling) at an integer factor (R:...Controller.cs:1694-1703). */
 int isa_resize_box(const float* in_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height, int level,
                    isa_stream_t stream);
+/* The levels 2, 4 and 8 of isa_resize_box in one pass over the input (each output may be NULL). No counterpart in
+ * the reference: the merger's pyramid. width and height multiples of 8; input 16-byte aligned; level-2 rows 16-byte
+ * and level-4 rows 8-byte aligned. Values identical to isa_resize_box. */
+int isa_resize_box_levels(const float* in_dev, int in_pitch, int width, int height, float* out2_dev, int pitch2,
+                          float* out4_dev, int pitch4, float* out8_dev, int pitch8, isa_stream_t stream);
 /* convertToTilesOverlapBorder, R:Kernels/kernel.cu:171-228; wrapper R:PEFStudioDX/BasicKernels.cs:48-63. */
 int isa_convert_to_tiles_border(const float* img_dev, int img_pitch, float* tiles_dev, int width, int height,
                                 int max_shift, int tile_size, int tile_count_x, int tile_count_y,

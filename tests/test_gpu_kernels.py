@@ -130,6 +130,19 @@ def test_fourier_highpass(isa, oracle, gpu, w, h, hp, hps):
     assert err < 2e-5, err
 
 
+@pytest.mark.parametrize("h,w,which", [(64, 96, (2, 4, 8)), (40, 520, (2, 4, 8)), (16, 8, (2, 8)), (48, 1032, (4,))])
+def test_resize_box_levels(isa, oracle, gpu, h, w, which):
+    """The one-pass pyramid equals isa_resize_box (the oracle's box means) bit for bit, for any subset of levels."""
+    rng = np.random.default_rng(16)
+    img = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    outs = {l: (empty((h // l, w // l), fill=-1.0) if l in which else None) for l in (2, 4, 8)}
+    ptr = {l: (p(outs[l]) if outs[l] is not None else None) for l in (2, 4, 8)}
+    isa.call("isa_resize_box_levels", p(dev(img)), w * 4, w, h, ptr[2], (w // 2) * 4, ptr[4], (w // 4) * 4, ptr[8],
+             (w // 8) * 4, stream())
+    for l in which:
+        np.testing.assert_array_equal(host(outs[l]), oracle.downscale_box(img, l))
+
+
 def test_resize_box(isa, oracle, gpu):
     rng = np.random.default_rng(6)
     img = rng.uniform(0, 1, (64, 96)).astype(np.float32)
