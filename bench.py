@@ -78,6 +78,112 @@ def cpu_baseline(cfg, seconds_hint: float):
     }
 
 
+# Compulsory HBM bytes per pixel of each merger stage (each input read once, each output written once;
+# u16 raw, f32 intermediates): per frame (f), per non-reference frame (g) and once per merge (o).
+STAGE_BYTES = {
+    "tracking_images": dict(f=30.0),  # debayer + grey 6, high-pass 8, two Gaussians 8 + 8
+    "prepare_accumulation": dict(o=16.0),  # structure tensor, smoothing and kernel parameters
+    "lucas_kanade": dict(g=80.0),  # flow from tiles 8, three LK iterations 24 each
+    "robustness": dict(f=31.0),  # half-res debayer 5, mask + erosion 26
+    "accumulate": dict(f=14.0, o=36.0),
+    "final_weighting": dict(o=48.0),
+}
+
+
+def stage_rooflines(stages, n, w, h, levels):
+    """Achieved rate of every stage against the MI355X peak that bounds it: HBM bytes for the image stages,
+    FP64 flops of the tile cross terms for track_patches (the minimizer is latency-bound: time only)."""
+    P = w * h
+    out = {}
+    for k, ms in stages.items():
+        if k in STAGE_BYTES:
+            c = STAGE_BYTES[k]
+            b = P * (c.get("f", 0.0) * n + c.get("g", 0.0) * (n - 1) + c.get("o", 0.0))
+            gbs = b / 1e6 / ms if ms > 0 else 0.0
+            out[k] = {"ms": round(ms, 3), "bound": "hbm", "algorithmic_gb": round(b / 1e9, 3),
+                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4)}
+        elif k == "track_patches":
+            gf = sum(l["track_gflop"] for l in levels)
+            tf = gf / ms if ms > 0 else 0.0
+            out[k] = {"ms": round(ms, 3), "bound": "fp64", "algorithmic_gflop": round(gf, 1),
+                      "achieved_tflops": round(tf, 2), "frac": round(tf / PEAK_FP64_TFLOPS, 4)}
+        else:
+            out[k] = {"ms": round(ms, 3), "bound": "latency"}
+    return out
+
+
+def precision_vs_fp64(isa, torch):
+    """Errors of the GPU paths against the FP64 oracle on the 8-frame 1024x1024 configuration: finest-level
+    one-to-one shifts (max, RMS px), flow (RMS px over the interior), eroded mask (max abs), merged image (PSNR
+    and max relative error over the interior) and the PSNR loss against the noiseless ground truth."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    cfg = CONFIGS["small"]
+    w, h, n = cfg["w"], cfg["h"], cfg["n"]
+    scfg = isa.default_synth(seed=20240602, width=w, height=h, frames=n, global_shift_max=cfg["gmax"],
+                             flow_max=cfg["flow"], occluded_fraction=0.02)
+    raw_t, _ = isa.synth_burst(scfg)
+    torch.cuda.synchronize()
+    raw = raw_t.cpu().numpy().view(np.uint16)
+    O.set_threads(min(len(os.sched_getaffinity(0)), 16))
+    t0 = time.perf_counter()
+    params = isa.default_params(levels=cfg["levels"])
+    ref = O.merge(params, raw, dt=np.float64, want=("one_to_one", "flows", "masks"))
+    t_oracle = time.perf_counter() - t0
+    gt = isa.ground_truth(scfg, ref["ref"], w, h).cpu().numpy().astype(np.float64)
+    b = 8  # interior: the reference's borders (Q4-Q6) and the LK window are excluded
+    sl = (slice(b, -b), slice(b, -b))
+
+    def psnr(x, y):
+        d = (x - y)[sl]
+        d = d[np.isfinite(d)]
+        mse = float(np.mean(d ** 2))
+        return 99.0 if mse == 0 else 10 * np.log10(1.0 / mse)
+
+    p_gt_ref = psnr(ref["out"], gt)
+    res = {"workload": f"{n}-frame {w}x{h} RGGB 14-bit burst, levels "
+                       + "/".join(str(l[0]) for l in cfg["levels"]) + ", 1x merge",
+           "oracle_s": round(t_oracle, 2), "psnr_vs_gt_fp64_db": round(p_gt_ref, 3)}
+    # the FP32 oracle (the reference's arithmetic) against the FP64 one: the errors the reference itself has
+    r32 = O.merge(params, raw, dt=np.float32, want=("one_to_one", "flows", "masks"))
+    paths = [("oracle_fp32", None), ("plain", 0), ("bf16x3", 1), ("fp16x3", 2), ("fp32", 3)]
+    for name, code in paths:
+        if code is None:
+            out, o2o, ref_index = r32["out"].astype(np.float64), r32["one_to_one"], r32["ref"]
+            flow_of = lambda f: r32["flows"][f].astype(np.float64)  # noqa: E731
+            mask_of = lambda f: r32["masks"][f].astype(np.float64)[..., :3]  # noqa: E731
+        else:
+            m = isa.BurstMerger(w, h, n, isa.default_params(levels=cfg["levels"], xc_precision=code))
+            out = m.run(raw_t).cpu().numpy().astype(np.float64)
+            ref_index = m.reference_index
+            o2o = m.buffer_tensor(2, 0, 2).cpu().numpy().reshape(ref["one_to_one"].shape)
+            flow_of = lambda f: m.buffer_tensor(3, f, 2).cpu().numpy().astype(np.float64)  # noqa: E731
+            mask_of = lambda f: m.buffer_tensor(4, f, 4).cpu().numpy().astype(np.float64)[..., :3]  # noqa: E731
+        if ref_index != ref["ref"]:
+            res[name] = {"error": "reference frame differs from the oracle's"}
+            continue
+        ds = np.abs(o2o - ref["one_to_one"])
+        fl_err, mk_err = [], 0.0
+        for f in range(n):
+            if f == ref["ref"]:
+                continue
+            d = (flow_of(f) - ref["flows"][f])[sl]
+            fl_err.append(d[np.isfinite(d)] ** 2)
+            dm = np.abs(mask_of(f) - ref["masks"][f][..., :3])[b // 2:-b // 2, b // 2:-b // 2]
+            mk_err = max(mk_err, float(np.nanmax(dm)))
+        fl_err = np.concatenate(fl_err)
+        rel = np.abs(out - ref["out"])[sl] / np.maximum(np.abs(ref["out"][sl]), 1e-3)
+        res[name] = {"shift_max_px": float(f"{ds.max():.3g}"), "shift_rms_px": float(f"{np.sqrt(np.mean(ds ** 2)):.3g}"),
+                     "flow_rms_px": float(f"{np.sqrt(np.mean(fl_err)):.3g}"), "mask_max_abs": float(f"{mk_err:.3g}"),
+                     "psnr_vs_fp64_db": round(psnr(out, ref["out"]), 2),
+                     "max_rel_vs_fp64": float(f"{np.nanmax(rel):.3g}"),
+                     "flow_frac_over_0.01px": round(float(np.mean(fl_err > 1e-4)), 5),
+                     "psnr_loss_vs_gt_db": round(p_gt_ref - psnr(out, gt), 4)}
+    return res
+
+
 def adjoint_bench(isa, torch, steps: int, warmup: int):
     """Differentiable merge on a 16-frame 2048x2048 burst at 2x output: forward (relaxed robustness +
     accumulation, loss against the noiseless 2x rendering) and backward (flow and kernel parameters)
@@ -177,6 +283,7 @@ def main():
     ap.add_argument("--xc", default="bf16x3,fp16x3,fp32",
                     help="mixed-precision (MFMA) tile-search variants timed beside the exact path; '' for none")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-merge section")
+    ap.add_argument("--no-precision", action="store_true", help="skip the errors against the FP64 oracle")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the differentiable-merge section (prints its JSON object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -284,6 +391,7 @@ def main():
                                + "/".join(str(l[1]) for l in cfg["levels"]) + ", 1x merge",
                    "frames": n, "width": w, "height": h, "pairs": m_pairs},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+        "stage_roofline": stage_rooflines(stages, n, w, h, levels),
         "levels": levels,
         "roofline": roofline,
     }
@@ -332,6 +440,10 @@ def main():
         log("[bench] differentiable merge")
         result["adjoint"] = adjoint_bench(isa, torch, args.steps, args.warmup)
         log(f"[bench] adjoint: {json.dumps(result['adjoint'])}")
+    if not args.no_precision:
+        log("[bench] errors against the FP64 oracle")
+        result["precision_vs_fp64"] = precision_vs_fp64(isa, torch)
+        log(f"[bench] precision: {json.dumps(result['precision_vs_fp64'])}")
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(cfg, 20.0)
