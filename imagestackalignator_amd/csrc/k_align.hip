@@ -845,6 +845,216 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
     }
 }
 
+// The same search with the reduced Laplacian in registers, for n - 1 <= N1MAX: lane i owns row i of
+// L (Lr[j] = L[i][j]) and the part of column i below the diagonal (Lc[r] = L[r][i]); rows and columns
+// are exchanged by lane broadcasts, so the factorisation and the two substitutions have no LDS round
+// trips. The Laplacian comes from the neighbour bit masks of the connectivity test (the rows are
+// distinct frame pairs, so an off-diagonal entry is -1 or 0 and a diagonal one the node's degree), and
+// each node's right-hand side from its own rows in increasing order. Every floating-point operation
+// is that of k_minimize, in the same order, so the results are identical.
+__device__ __forceinline__ double bcast_lane(double v, int lane) { return __shfl(v, lane); }
+
+template <int N1MAX>
+__global__ __launch_bounds__(kMinThreads) void k_minimize_reg(MinArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    const int n = a.n, n1 = n - 1, m = a.m, tid = threadIdx.x;
+    double* d = (double*)smem;          // m x 2
+    double* xs = d + 2 * m;             // n x 2
+    int* rows = (int*)(xs + 2 * n);     // m x 2
+    int* list = rows + 2 * m;           // n1 x (n - 1)
+    char* active = (char*)(list + n1 * (n - 1));
+    __shared__ unsigned long long s_nb[kMinFrames];
+    __shared__ int s_flag;
+    static_assert(kMinThreads == 64 && N1MAX < 64, "one wave, one lane per node");
+    const int t = blockIdx.x;
+    for (int k = tid; k < m; k += kMinThreads) {
+        const float* v = a.meas + (size_t)k * a.meas_stride + (size_t)t * 2;
+        d[2 * k] = (double)v[0];
+        d[2 * k + 1] = (double)v[1];
+        active[k] = 1;
+        rows[2 * k] = a.rows[2 * k];
+        rows[2 * k + 1] = a.rows[2 * k + 1];
+    }
+    __syncthreads();
+    // rows of node tid + 1 in increasing k: 2 k when it is the row's second frame, 2 k + 1 when the first
+    int cnt = 0;
+    if (tid < n1)
+        for (int k = 0; k < m; ++k) {
+            const int ra = rows[2 * k], rb = rows[2 * k + 1];
+            const int side = rb == tid + 1 ? 0 : (ra == tid + 1 ? 1 : -1);
+            if (side >= 0 && cnt < n - 1) list[tid * (n - 1) + cnt++] = 2 * k + side;
+        }
+    double Lr[N1MAX], Lc[N1MAX];
+    double bx = 0.0, by = 0.0;
+    int status = 0, it;
+    for (it = 0; it < 10; ++it) {
+        const bool pre = it == 0 && a.lfull != nullptr;
+        if (pre) {
+            if (tid == 0) s_flag = 1;
+        } else {
+            if (tid < kMinFrames) s_nb[tid] = 0ull;
+            __syncthreads();
+            for (int k = tid; k < m; k += kMinThreads) {
+                if (!active[k]) continue;
+                const int u = rows[2 * k], v = rows[2 * k + 1];
+                atomicOr(&s_nb[u], 1ull << v);
+                atomicOr(&s_nb[v], 1ull << u);
+            }
+            __syncthreads();
+            const unsigned long long nb = tid < n ? s_nb[tid] : 0ull;
+            unsigned long long reached = 1ull;
+            for (int step = 0; step < n; ++step) {
+                unsigned long long add = ((reached >> tid) & 1ull) ? nb : 0ull;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) add |= __shfl_xor(add, off);
+                const unsigned long long nr = reached | add;
+                if (nr == reached) break;
+                reached = nr;
+            }
+            const unsigned long long all = n == 64 ? ~0ull : ((1ull << n) - 1ull);
+            if (tid == 0) s_flag = reached == all;
+        }
+        __syncthreads();
+        if (!s_flag) {
+            status = -1;
+            for (int i = tid; i < 2 * n; i += kMinThreads) xs[i] = 0.0;
+            __syncthreads();
+            break;
+        }
+        // L (lane tid: reduced node tid, frame tid + 1) and the right-hand side
+        if (pre) {
+#pragma unroll
+            for (int j = 0; j < N1MAX; ++j) {
+                Lr[j] = (tid < n1 && j <= tid) ? a.lfull[tid * n1 + j] : 0.0;
+                Lc[j] = (tid < n1 && j > tid && j < n1) ? a.lfull[j * n1 + tid] : 0.0;
+            }
+        } else {
+            const unsigned long long nbi = tid < n1 ? s_nb[tid + 1] : 0ull;
+#pragma unroll
+            for (int j = 0; j < N1MAX; ++j)
+                Lr[j] = j == tid ? (double)__popcll(nbi) : -(double)((nbi >> (j + 1)) & 1ull);
+        }
+        bx = 0.0;
+        by = 0.0;
+        for (int e = 0; e < cnt; ++e) {
+            const int v = list[tid * (n - 1) + e], k = v >> 1;
+            if (!active[k]) continue;
+            if (v & 1) {
+                bx -= d[2 * k];
+                by -= d[2 * k + 1];
+            } else {
+                bx += d[2 * k];
+                by += d[2 * k + 1];
+            }
+        }
+        // Cholesky, lower triangle (step j: the diagonal, column j divided, then the trailing rows); the
+        // loops run to N1MAX with their bodies guarded, so every register index is a constant
+        if (!pre)
+#pragma unroll
+            for (int j = 0; j < N1MAX; ++j) {
+                if (j < n1) {
+                    if (tid == j) Lr[j] = sqrt(Lr[j]);
+                    const double djj = bcast_lane(Lr[j], j);
+                    if (tid > j && tid < n1) Lr[j] /= djj;
+#pragma unroll
+                    for (int c = j + 1; c < N1MAX; ++c) {
+                        if (c < n1) {
+                            const double lcj = bcast_lane(Lr[j], c);  // L[c][j]
+                            if (tid == j) Lc[c] = lcj;
+                            if (tid >= c && tid < n1) Lr[c] -= Lr[j] * lcj;
+                        }
+                    }
+                }
+            }
+        // forward substitution L y = rhs
+#pragma unroll
+        for (int i = 0; i < N1MAX; ++i) {
+            if (i < n1) {
+                if (tid == i) {
+                    bx = bx / Lr[i];
+                    by = by / Lr[i];
+                }
+                const double yx = bcast_lane(bx, i), yy = bcast_lane(by, i);
+                if (tid > i && tid < n1) {
+                    bx -= Lr[i] * yx;
+                    by -= Lr[i] * yy;
+                }
+            }
+        }
+        // backward substitution L^T x = y
+#pragma unroll
+        for (int i = N1MAX - 1; i >= 0; --i) {
+            if (i < n1) {
+                if (tid == i) {
+                    bx = bx / Lr[i];
+                    by = by / Lr[i];
+                }
+                const double xx = bcast_lane(bx, i), xy = bcast_lane(by, i);
+                if (tid < i) {
+                    bx -= Lc[i] * xx;
+                    by -= Lc[i] * xy;
+                }
+            }
+        }
+        if (tid < 2) xs[tid] = 0.0;
+        if (tid < n1) {
+            xs[2 + 2 * tid] = bx;
+            xs[3 + 2 * tid] = by;
+        }
+        __syncthreads();
+        // largest residual above 1 px^2 (first index on ties, as the sequential scan)
+        double best = 1.0;
+        int bi = -1;
+        for (int k = tid; k < m; k += kMinThreads) {
+            if (!active[k]) continue;
+            const int ra = rows[2 * k], rb = rows[2 * k + 1];
+            double ex = d[2 * k] - (xs[2 * rb] - xs[2 * ra]), ey = d[2 * k + 1] - (xs[2 * rb + 1] - xs[2 * ra + 1]);
+            double dist = ex * ex + ey * ey;
+            if (dist > best) { best = dist; bi = k; }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (oi >= 0 && (bi < 0 || ob > best || (ob == best && oi < bi))) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (tid == 0) {
+            if (bi >= 0) {
+                d[2 * bi] = 0.0;
+                d[2 * bi + 1] = 0.0;
+                active[bi] = 0;
+            }
+            s_flag = bi;
+        }
+        __syncthreads();
+        status = s_flag;
+        __syncthreads();
+        if (status < 0) break;
+    }
+    for (int i = tid; i < n1; i += kMinThreads) {
+        float* o = a.one_to_one + ((size_t)t * n1 + i) * 2;
+        o[0] = (float)(xs[2 * (i + 1)] - xs[2 * i]);
+        o[1] = (float)(xs[2 * (i + 1) + 1] - xs[2 * i + 1]);
+    }
+    for (int k = tid; k < m; k += kMinThreads) {
+        int ra = rows[2 * k], rb = rows[2 * k + 1];
+        float* v = a.meas + (size_t)k * a.meas_stride + (size_t)t * 2;
+        v[0] = (float)(xs[2 * rb] - xs[2 * ra]);
+        v[1] = (float)(xs[2 * rb + 1] - xs[2 * ra + 1]);
+    }
+    if (tid == 0) {
+        if (a.status) a.status[t] = status;
+        if (a.iters) a.iters[t] = it < 10 ? it + 1 : 10;
+    }
+}
+
+size_t minimize_reg_smem(int n, int m) {
+    return (size_t)(2 * m + 2 * n) * 8 + (size_t)(2 * m + (n - 1) * (n - 1)) * 4 + m + 16;
+}
+
 size_t minimize_smem(int n, int m) {
     int n1 = n - 1;
     return (size_t)(2 * m + n1 * n1 + 2 * n1 + 2 * n + kMinThreads) * 8 + kMinThreads * 4 + m + 16;
@@ -1090,7 +1300,17 @@ int launch_minimize(float* meas, size_t meas_stride, float* one_to_one, int* sta
         return ISA_ERR_UNSUPPORTED;
     }
     MinArgs a{meas, meas_stride, one_to_one, status, iters, rows, lfull, tiles, n, m};
-    k_minimize<<<tiles, kMinThreads, smem, s>>>(a);
+    if (n <= 8) {
+        k_minimize_reg<7><<<tiles, kMinThreads, minimize_reg_smem(n, m), s>>>(a);
+    } else if (n <= 16) {
+        k_minimize_reg<15><<<tiles, kMinThreads, minimize_reg_smem(n, m), s>>>(a);
+    } else if (n <= 24) {
+        k_minimize_reg<23><<<tiles, kMinThreads, minimize_reg_smem(n, m), s>>>(a);
+    } else if (n <= 32) {
+        k_minimize_reg<31><<<tiles, kMinThreads, minimize_reg_smem(n, m), s>>>(a);
+    } else {
+        k_minimize<<<tiles, kMinThreads, smem, s>>>(a);
+    }
     return check_launch("minimize_shifts");
 }
 

@@ -237,7 +237,10 @@ def test_upsample_shifts(isa, oracle, gpu):
     np.testing.assert_array_equal(host(out), oracle.upsample_shifts(inp, 4, 2, ocx, ocy, ncx, ncy, 32, 32))
 
 
-@pytest.mark.parametrize("n,strategy,block", [(5, 0, 5), (8, 0, 5), (6, 1, 5), (9, 2, 4), (7, 3, 3)])
+# frame counts up to 8, 16, 24 and 32 take the register forms of the minimizer (lane per frame), 40 the
+# LDS form; the larger cases carry extra outliers so that tiles iterate several times
+@pytest.mark.parametrize("n,strategy,block", [(5, 0, 5), (8, 0, 5), (6, 1, 5), (9, 2, 4), (7, 3, 3), (20, 0, 5),
+                                              (17, 2, 6), (30, 3, 4), (40, 0, 5)])
 def test_minimize_shifts(isa, oracle, gpu, n, strategy, block):
     rng = np.random.default_rng(9 + n)
     ref = 0 if strategy != 2 else 3
@@ -249,6 +252,10 @@ def test_minimize_shifts(isa, oracle, gpu, n, strategy, block):
     # outliers in some tiles
     for t in range(0, tiles, 3):
         d[t, rng.integers(0, m)] += rng.uniform(3, 8, 2)
+    if n > 9:
+        for t in range(1, tiles, 4):
+            for k in rng.choice(m, 4, replace=False):
+                d[t, k] += rng.uniform(2, 9, 2)
     d = d.astype(np.float32).astype(np.float64)
     dh_ref, r_ref, st_ref, it_ref = oracle.minimize_shifts(d, n, ref, strategy, block)
     meas = dev(np.ascontiguousarray(np.transpose(d, (1, 0, 2))).astype(np.float32))  # [m][tile][2]
