@@ -28,6 +28,7 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));  // 16 bytes at a 4-byte aligned address
 
 constexpr int kPairsPerChunk = 16;  // MFMA columns
 constexpr int kWaves = 4;           // waves per block; wave w reduces over track rows uy = w mod 4
@@ -290,16 +291,28 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
     for (int nb = 0; nb < kMaxNB; ++nb)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int ux = 16 * nb + (PREC == ISA_XC_FP32 ? 4 * e + q : 4 * q + e);
+            const int ux = 16 * nb + 4 * q + e;  // a lane's four operand words: consecutive columns
             colx[nb][e] = cols_in ? tx0 + ux : clampi(tx0 + ux, 0, a.w - 1);
         }
     // operand words of one track row (packed split or fp32 bits); the wave's next row is fetched
     // while the current one is multiplied
     unsigned cur[kMaxNB][4], nxt[kMaxNB][4];
+    // (block inside the image: a lane's four words are consecutive columns, one 16-byte load at a
+    // 4-byte aligned address)
+    const unsigned* bw = PREC == ISA_XC_FP32 ? (const unsigned*)bimg : bx;
     auto fetch_row = [&](int uy) {
         const size_t ro = (size_t)clampi(ty0 + min(uy, B - 1), 0, a.h - 1) * ipitch;
 #pragma unroll
-        for (int nb = 0; nb < kMaxNB; ++nb)
+        for (int nb = 0; nb < kMaxNB; ++nb) {
+            if (cols_in) {
+                u32x4a4 v = {0u, 0u, 0u, 0u};
+                if (pj >= 0 && nb < NB) v = *(const u32x4a4*)(bw + ro + colx[nb][0]);
+                nxt[nb][0] = v.x;
+                nxt[nb][1] = v.y;
+                nxt[nb][2] = v.z;
+                nxt[nb][3] = v.w;
+                continue;
+            }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 unsigned v = 0;
@@ -307,6 +320,7 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
                     v = PREC == ISA_XC_FP32 ? __float_as_uint(bimg[ro + colx[nb][e]]) : bx[ro + colx[nb][e]];
                 nxt[nb][e] = v;
             }
+        }
     };
     fetch_row(wave);
     for (int uy = wave; uy < B; uy += kWaves) {
@@ -320,27 +334,27 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
             if (nb >= NB) break;
             const int ux0 = 16 * nb;
             if (PREC == ISA_XC_FP32) {
+                // product kk takes column ux0 + 4 q + kk of every lane quad q (the K slices of the four
+                // products cover the 16 columns once)
+                u32x4a4 av[MB];
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+                    av[mb] = *(const u32x4a4*)(pu + (aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux0 + 4 * q);
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
                     const float bv = __uint_as_float(cur[nb][kk]);
-                    const int ux = ux0 + 4 * kk + q;
 #pragma unroll
-                    for (int mb = 0; mb < MB; ++mb) {
-                        const float av = pf[(aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux];
-                        acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[mb], 0, 0, 0);
-                    }
+                    for (int mb = 0; mb < MB; ++mb)
+                        acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(av[mb][kk]), bv, acc[mb], 0, 0, 0);
                 }
             } else {
                 u32x2 bh, bl;
                 unpack4(cur[nb][0], cur[nb][1], cur[nb][2], cur[nb][3], bh, bl);
                 // all A words of this column block first, so the LDS reads overlap the products
-                unsigned aw[MB][4];
+                u32x4a4 aw[MB];
 #pragma unroll
-                for (int mb = 0; mb < MB; ++mb) {
-                    const unsigned* ap = pu + (aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux0 + 4 * q;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) aw[mb][e] = ap[e];
-                }
+                for (int mb = 0; mb < MB; ++mb)
+                    aw[mb] = *(const u32x4a4*)(pu + (aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux0 + 4 * q);
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
                     u32x2 ah, al;
