@@ -71,16 +71,23 @@ __device__ __forceinline__ f32x4 mfma16(u32x2 a, u32x2 b, f32x4 c) {
                                                      0, 0);
 }
 
-template <int PREC> __global__ void k_split_planes(const float* src, unsigned* dst, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = split_pack<PREC>(src[i]);
+// four elements per thread (16-byte accesses when both arrays are 16-byte aligned, else one by one)
+template <int PREC> __global__ void k_split_planes(const float* src, unsigned* dst, size_t n, bool vec) {
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (vec && i + 4 <= n) {
+        const float4 v = *(const float4*)(src + i);
+        *(uint4*)(dst + i) = make_uint4(split_pack<PREC>(v.x), split_pack<PREC>(v.y), split_pack<PREC>(v.z),
+                                        split_pack<PREC>(v.w));
+        return;
+    }
+    for (size_t k = i; k < i + 4 && k < n; ++k) dst[k] = split_pack<PREC>(src[k]);
 }
 
 // Vertical window energies V(y0, x) = sum_{y = y0}^{y0 + T - 1} src(y, x)^2 for y0 in [0, h - T]:
 // one thread per column and run of kVRun window starts (a T-term sum, then slid in FP64). runmax gets
 // the largest energy of each (run, column), rounded up: the squares are exact in FP64, so a window's
 // error before its rounding to float is at most (T + 2 kVRun) 2^-53 times that maximum.
-constexpr int kVRun = 256;
+constexpr int kVRun = 64;
 __global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y0 = blockIdx.y * kVRun, nv = h - T + 1;
@@ -608,9 +615,10 @@ static int launch_mfma_p(const TrackArgs& a, XPlanes xp, int M, const int* chunk
 
 // Packed (hi | lo << 16) split planes of count pixels (nothing to do for ISA_XC_FP32).
 int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s) {
-    const unsigned blocks = (unsigned)((count + 255) / 256);
-    if (precision == ISA_XC_BF16X3) k_split_planes<ISA_XC_BF16X3><<<blocks, 256, 0, s>>>(src, dst, count);
-    else if (precision == ISA_XC_FP16X3) k_split_planes<ISA_XC_FP16X3><<<blocks, 256, 0, s>>>(src, dst, count);
+    const unsigned blocks = (unsigned)((count + 1023) / 1024);  // four elements per thread
+    const bool vec = ((size_t)src & 15) == 0 && ((size_t)dst & 15) == 0;
+    if (precision == ISA_XC_BF16X3) k_split_planes<ISA_XC_BF16X3><<<blocks, 256, 0, s>>>(src, dst, count, vec);
+    else if (precision == ISA_XC_FP16X3) k_split_planes<ISA_XC_FP16X3><<<blocks, 256, 0, s>>>(src, dst, count, vec);
     else return ISA_OK;
     return check_launch("split_planes");
 }
