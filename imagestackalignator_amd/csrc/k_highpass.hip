@@ -9,8 +9,11 @@
 //   2. column DFT at the band ky  Y = (1-H) c_kx/(WH) . sum_y U e^{-i..}
 //   3. inverse column DFT         V = sum_j Y e^{+i..}
 //   4. inverse row DFT, epilogue  img = clamp(img - V . Tw^T, 0, 1)   (GEMM, K = 2 kxn)
-// The two GEMMs run on the FP32 matrix cores (v_mfma_f32_16x16x4_f32) when the image rows are
-// 16-byte aligned, else as a tiled FP32 SIMT GEMM.
+// The two GEMMs run on the matrix cores when the image rows are 16-byte aligned, else as a tiled
+// FP32 SIMT GEMM. On the matrix cores they use FP32 products (v_mfma_f32_16x16x4_f32), or, when the
+// caller bounds the image values, three fp16 products of two-term fp16 splits of both operands
+// (v_mfma_f32_16x16x32_f16: hi.hi + hi.lo + lo.hi), each within about 2^-21 of the FP32 product,
+// at a fifth of the matrix-core time.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -44,19 +47,44 @@ __device__ __forceinline__ int band_ky(int j, int kyn, int h) {
     return hi_start + (j - lo);
 }
 
+typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+
+// Two-term fp16 split v = hi + lo, hi = RN16(v), lo = RN16(v - hi) (v - hi is exact in float): for
+// |v| < 2^15 the remainder |v - hi - lo| is at most max(2^-22 |v|, 2^-25).
+struct Split16 {
+    _Float16 hi, lo;
+};
+__device__ __forceinline__ Split16 split16(float v) {
+    const _Float16 hi = (_Float16)v;
+    return Split16{hi, (_Float16)(v - (float)hi)};
+}
+
 // tw[x][2k] = cos(2 pi k x / W), tw[x][2k+1] = sin(2 pi k x / W) (row pitch n2p), exact argument
-// reduction; twT is the transpose (row pitch twp). Padding columns / rows stay 0.
-__global__ void k_row_twiddles(float* tw, int n2p, float* twT, int twp, int w, int kxn) {
+// reduction; twT is the transpose (row pitch twp). twh / twl and twTh / twTl hold the fp16 splits of
+// the same floats. Padding columns / rows stay 0.
+__global__ void k_row_twiddles(float* tw, int n2p, float* twT, int twp, _Float16* twh, _Float16* twl, _Float16* twTh,
+                               _Float16* twTl, int w, int kxn) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int k = blockIdx.y;
     if (x >= w) return;
     double a = kTwoPi * (double)(((long long)k * x) % w) / (double)w;
     double s, c;
     sincos(a, &s, &c);
-    tw[(size_t)x * n2p + 2 * k] = (float)c;
-    tw[(size_t)x * n2p + 2 * k + 1] = (float)s;
-    twT[(size_t)(2 * k) * twp + x] = (float)c;
-    twT[(size_t)(2 * k + 1) * twp + x] = (float)s;
+    const float cf = (float)c, sf = (float)s;
+    const Split16 cs = split16(cf), ss = split16(sf);
+    const size_t r = (size_t)x * n2p + 2 * k, tc = (size_t)(2 * k) * twp + x, ts = (size_t)(2 * k + 1) * twp + x;
+    tw[r] = cf;
+    tw[r + 1] = sf;
+    twT[tc] = cf;
+    twT[ts] = sf;
+    twh[r] = cs.hi;
+    twl[r] = cs.lo;
+    twh[r + 1] = ss.hi;
+    twl[r + 1] = ss.lo;
+    twTh[tc] = cs.hi;
+    twTl[tc] = cs.lo;
+    twTh[ts] = ss.hi;
+    twTl[ts] = ss.lo;
 }
 
 // Transfer function of R:Kernels/kernel.cu:702-779 (float, same operation order).
@@ -181,6 +209,27 @@ __device__ __forceinline__ f32x4 mfma4(const float4& a, const float4& b, f32x4 a
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc, 0, 0, 0);
 }
 
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+
+// One 16 x 16 x 32 step on fp16 splits (v_mfma_f32_16x16x32_f16: lane (j, q) holds row or column j
+// at K indices 8q .. 8q+7 of both operands; the result tile as above): hi.hi + hi.lo + lo.hi,
+// accumulated in FP32. Each fp16 product is exact in FP32; the dropped lo.lo term and the split
+// remainders leave each product within about 2^-21 |a b| + 2^-25 (|a| + |b|) of a b.
+__device__ __forceinline__ f32x4 mfma3(const h16x8& ah, const h16x8& al, const h16x8& bh, const h16x8& bl, f32x4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+}
+
+__device__ __forceinline__ void split8(const float (&v)[8], h16x8& hi, h16x8& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const Split16 p = split16(v[e]);
+        hi[e] = p.hi;
+        lo[e] = p.lo;
+    }
+}
+
 constexpr int HP_WAVES = 4;  // waves per block of both MFMA GEMMs
 constexpr int HP_NTG = 10;   // 16-column tiles per wave in the row DFT
 
@@ -188,8 +237,12 @@ constexpr int HP_NTG = 10;   // 16-column tiles per wave in the row DFT
 // (c < n2p, row pitch n2p). A block takes 32 rows, 16 HP_NTG columns and one K chunk of kc; each
 // wave sums a quarter of the chunk, and the four wave sums are added through LDS in the fixed order
 // ((w3 + w2) + w1) + w0.
+// SPLIT: the products on the fp16 splits of img (formed here) and of twT (twTh, twTl).
+template <bool SPLIT>
 __global__ __launch_bounds__(HP_WAVES * 64, 2) void k_hp_row_dft(const float* __restrict__ img, int pitch, int h, int w,
-                                                              const float* __restrict__ twT, int twp, int n2p,
+                                                              const float* __restrict__ twT,
+                                                              const _Float16* __restrict__ twTh,
+                                                              const _Float16* __restrict__ twTl, int twp, int n2p,
                                                               float* __restrict__ part, int kc) {
     __shared__ f32x4 red[2 * HP_NTG][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, q = lane >> 4;
@@ -210,31 +263,64 @@ __global__ __launch_bounds__(HP_WAVES * 64, 2) void k_hp_row_dft(const float* __
         aok[m] = y < h;
         ar[m] = row_ptr(img, pitch, aok[m] ? y : 0);
     }
-    const float* br = twT + (size_t)(16 * t0 + j) * twp;
-    for (int k = kbeg; k < kend; k += 16) {
-        const int kk = k + 4 * q;
-        float4 a[2];
+    const size_t boff = (size_t)(16 * t0 + j) * twp;
+    const float* br = twT + boff;
+    // K steps of 32 (SPLIT) or 16; kbeg is a multiple of 32 and twT rows are padded to 32 with zeros
+    for (int k = kbeg; k < kend; k += SPLIT ? 32 : 16) {
+        if (SPLIT) {
+            const int kk = k + 8 * q;
+            h16x8 ah[2], al[2], bh[HP_NTG], bl[HP_NTG];
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            if (aok[m] && kk + 4 <= kend) {
-                a[m] = *reinterpret_cast<const float4*>(ar[m] + kk);
-            } else {
-                a[m].x = aok[m] && kk < kend ? ar[m][kk] : 0.0f;
-                a[m].y = aok[m] && kk + 1 < kend ? ar[m][kk + 1] : 0.0f;
-                a[m].z = aok[m] && kk + 2 < kend ? ar[m][kk + 2] : 0.0f;
-                a[m].w = 0.0f;
+            for (int m = 0; m < 2; ++m) {
+                float a[8];
+                if (aok[m] && kk + 8 <= kend) {
+                    const float4 u = *reinterpret_cast<const float4*>(ar[m] + kk);
+                    const float4 v = *reinterpret_cast<const float4*>(ar[m] + kk + 4);
+                    a[0] = u.x, a[1] = u.y, a[2] = u.z, a[3] = u.w, a[4] = v.x, a[5] = v.y, a[6] = v.z, a[7] = v.w;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) a[e] = aok[m] && kk + e < kend ? ar[m][kk + e] : 0.0f;
+                }
+                split8(a, ah[m], al[m]);
             }
+#pragma unroll
+            for (int t = 0; t < HP_NTG; ++t)
+                if (t < nt) {
+                    const size_t o = boff + (size_t)16 * t * twp + kk;
+                    bh[t] = *reinterpret_cast<const h16x8*>(twTh + o);
+                    bl[t] = *reinterpret_cast<const h16x8*>(twTl + o);
+                }
+#pragma unroll
+            for (int t = 0; t < HP_NTG; ++t)
+                if (t < nt) {
+                    acc[0][t] = mfma3(ah[0], al[0], bh[t], bl[t], acc[0][t]);
+                    acc[1][t] = mfma3(ah[1], al[1], bh[t], bl[t], acc[1][t]);
+                }
+        } else {
+            const int kk = k + 4 * q;
+            float4 a[2];
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                if (aok[m] && kk + 4 <= kend) {
+                    a[m] = *reinterpret_cast<const float4*>(ar[m] + kk);
+                } else {
+                    a[m].x = aok[m] && kk < kend ? ar[m][kk] : 0.0f;
+                    a[m].y = aok[m] && kk + 1 < kend ? ar[m][kk + 1] : 0.0f;
+                    a[m].z = aok[m] && kk + 2 < kend ? ar[m][kk + 2] : 0.0f;
+                    a[m].w = 0.0f;
+                }
+            }
+            float4 b[HP_NTG];
+#pragma unroll
+            for (int t = 0; t < HP_NTG; ++t)
+                if (t < nt) b[t] = *reinterpret_cast<const float4*>(br + (size_t)16 * t * twp + kk);
+#pragma unroll
+            for (int t = 0; t < HP_NTG; ++t)
+                if (t < nt) {
+                    acc[0][t] = mfma4(a[0], b[t], acc[0][t]);
+                    acc[1][t] = mfma4(a[1], b[t], acc[1][t]);
+                }
         }
-        float4 b[HP_NTG];
-#pragma unroll
-        for (int t = 0; t < HP_NTG; ++t)
-            if (t < nt) b[t] = *reinterpret_cast<const float4*>(br + (size_t)16 * t * twp + kk);
-#pragma unroll
-        for (int t = 0; t < HP_NTG; ++t)
-            if (t < nt) {
-                acc[0][t] = mfma4(a[0], b[t], acc[0][t]);
-                acc[1][t] = mfma4(a[1], b[t], acc[1][t]);
-            }
     }
     for (int src = HP_WAVES - 1; src >= 1; --src) {
         if (wave == src) {
@@ -278,9 +364,15 @@ __global__ void k_hp_sum_parts(const float* __restrict__ part, float* __restrict
 // Inverse row DFT and epilogue: img[y][x] = clamp(img[y][x] - sum_c V[y][c] tw[x][c], 0, 1),
 // K = n2p (V and tw zero-padded, row pitch n2p). A wave takes 64 rows x 64 columns (4 x 4 tiles);
 // a block 2 x 2 waves.
+// SPLIT: the products on the fp16 splits of V (Vh, Vl) and of tw (twh, twl).
+template <bool SPLIT>
 __global__ __launch_bounds__(HP_WAVES * 64, 2) void k_hp_row_idft(float* img, int pitch, int h, int w,
                                                                const float* __restrict__ V,
-                                                               const float* __restrict__ tw, int n2p) {
+                                                               const float* __restrict__ tw,
+                                                               const _Float16* __restrict__ Vh,
+                                                               const _Float16* __restrict__ Vl,
+                                                               const _Float16* __restrict__ twh,
+                                                               const _Float16* __restrict__ twl, int n2p) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, q = lane >> 4;
     const int y0 = blockIdx.y * 128 + (wave >> 1) * 64, x0 = blockIdx.x * 128 + (wave & 1) * 64;
     if (y0 >= h || x0 >= w) return;
@@ -289,23 +381,69 @@ __global__ __launch_bounds__(HP_WAVES * 64, 2) void k_hp_row_idft(float* img, in
     for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const float* ar[4];
-    const float* br[4];
+    size_t ao[4], bo[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) ar[m] = V + (size_t)min(y0 + 16 * m + j, h - 1) * n2p;
+    for (int m = 0; m < 4; ++m) ao[m] = (size_t)min(y0 + 16 * m + j, h - 1) * n2p;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) br[t] = tw + (size_t)min(x0 + 16 * t + j, w - 1) * n2p;
-    for (int k = 0; k < n2p; k += 16) {
-        const int kk = k + 4 * q;
-        float4 a[4], b[4];
+    for (int t = 0; t < 4; ++t) bo[t] = (size_t)min(x0 + 16 * t + j, w - 1) * n2p;
+    for (int k = 0; k < n2p; k += SPLIT ? 32 : 16) {  // n2p is a multiple of 32
+        if (SPLIT) {
+            // operands swapped: the tile is C[x][y], so lane (j, q) ends with row y = .. + j at the four
+            // consecutive columns x = .. + 4q + i
+            const int kk = k + 8 * q;
+            h16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) a[m] = *reinterpret_cast<const float4*>(ar[m] + kk);
+            for (int m = 0; m < 4; ++m) {
+                ah[m] = *reinterpret_cast<const h16x8*>(Vh + ao[m] + kk);
+                al[m] = *reinterpret_cast<const h16x8*>(Vl + ao[m] + kk);
+            }
 #pragma unroll
-        for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const float4*>(br[t] + kk);
+            for (int t = 0; t < 4; ++t) {
+                bh[t] = *reinterpret_cast<const h16x8*>(twh + bo[t] + kk);
+                bl[t] = *reinterpret_cast<const h16x8*>(twl + bo[t] + kk);
+            }
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+            for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[m][t] = mfma4(a[m], b[t], acc[m][t]);
+                for (int t = 0; t < 4; ++t) acc[m][t] = mfma3(bh[t], bl[t], ah[m], al[m], acc[m][t]);
+        } else {
+            const int kk = k + 4 * q;
+            float4 a[4], b[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a[m] = *reinterpret_cast<const float4*>(V + ao[m] + kk);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const float4*>(tw + bo[t] + kk);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[m][t] = mfma4(a[m], b[t], acc[m][t]);
+        }
+    }
+    auto clamp01 = [](float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); };
+    if (SPLIT) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int y = y0 + 16 * m + j;
+            if (y >= h) continue;
+            float* row = row_ptr(img, pitch, y);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int x = x0 + 16 * t + 4 * q;
+                if (x + 4 <= w) {
+                    float4 v = *reinterpret_cast<const float4*>(row + x);
+                    v.x = clamp01(v.x - acc[m][t][0]);
+                    v.y = clamp01(v.y - acc[m][t][1]);
+                    v.z = clamp01(v.z - acc[m][t][2]);
+                    v.w = clamp01(v.w - acc[m][t][3]);
+                    *reinterpret_cast<float4*>(row + x) = v;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (x + i < w) row[x + i] = clamp01(row[x + i] - acc[m][t][i]);
+                }
+            }
+        }
+        return;
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -318,8 +456,7 @@ __global__ __launch_bounds__(HP_WAVES * 64, 2) void k_hp_row_idft(float* img, in
             for (int t = 0; t < 4; ++t) {
                 const int x = x0 + 16 * t + j;
                 if (x >= w) continue;
-                const float v = row[x] - acc[m][t][i];
-                row[x] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+                row[x] = clamp01(row[x] - acc[m][t][i]);
             }
         }
 }
@@ -383,8 +520,9 @@ __global__ void k_col_dft_finish(const double* __restrict__ P, double* __restric
 
 // Inverse column DFT: V'[y][2k] = Re, V'[y][2k+1] = -Im (so that the inverse row pass is a plain GEMM
 // with Tw^T); row pitch n2p.
-__global__ void k_col_idft(const double* __restrict__ Y, const double* __restrict__ ctab, float* __restrict__ V, int h,
-                           int kxn, int ny, int n2p) {
+// Vh / Vl (when not null): the fp16 splits of the same floats.
+__global__ void k_col_idft(const double* __restrict__ Y, const double* __restrict__ ctab, float* __restrict__ V,
+                           _Float16* __restrict__ Vh, _Float16* __restrict__ Vl, int h, int kxn, int ny, int n2p) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
     if (k >= kxn || y >= h) return;
@@ -396,13 +534,22 @@ __global__ void k_col_idft(const double* __restrict__ Y, const double* __restric
         sr += yr * c - yi * s;
         si += yr * s + yi * c;
     }
-    V[(size_t)y * n2p + 2 * k] = (float)sr;
-    V[(size_t)y * n2p + 2 * k + 1] = (float)(-si);
+    const size_t o = (size_t)y * n2p + 2 * k;
+    const float re = (float)sr, im = (float)(-si);
+    V[o] = re;
+    V[o + 1] = im;
+    if (Vh) {
+        const Split16 a = split16(re), b = split16(im);
+        Vh[o] = a.hi;
+        Vl[o] = a.lo;
+        Vh[o + 1] = b.hi;
+        Vl[o + 1] = b.lo;
+    }
 }
 
 struct HpPlan {
     Band b;
-    int n2, n2p, twp;  // band columns 2 kxn, padded to 16; row pitch of twT (w padded to 16)
+    int n2, n2p, twp;  // band columns 2 kxn, padded to 32; row pitch of twT (w padded to 32)
     int kc, nz;        // K chunk and chunk count of the MFMA row DFT
 };
 
@@ -410,12 +557,12 @@ HpPlan plan_of(int w, int h, float hp, float hps) {
     HpPlan P;
     P.b = band_of(w, h, hp, hps);
     P.n2 = 2 * P.b.kxn;
-    P.n2p = isa::div_up(P.n2, 16) * 16;
-    P.twp = isa::div_up(w, 16) * 16;
+    P.n2p = isa::div_up(P.n2, 32) * 32;
+    P.twp = isa::div_up(w, 32) * 32;
     // about 4096 waves in the row DFT
     const int groups = isa::div_up(h, 32) * isa::div_up(P.n2p / 16, HP_NTG);
     const int nz = std::max(1, std::min(isa::div_up(4096, HP_WAVES * groups), isa::div_up(w, 64)));
-    P.kc = isa::div_up(isa::div_up(w, nz), 64) * 64;
+    P.kc = isa::div_up(isa::div_up(w, nz), 128) * 128;  // wave quarters of 32-multiples
     P.nz = isa::div_up(w, P.kc);
     return P;
 }
@@ -429,6 +576,7 @@ struct HpLayout {
     float* V;
     double* P;
     double* Y;
+    _Float16 *twh, *twl, *twTh, *twTl, *Vh, *Vl;  // fp16 splits of tw, twT and V (same pitches)
     size_t bytes;
 };
 
@@ -454,6 +602,13 @@ HpLayout hp_layout(void* ws, int w, int h, const HpPlan& pl) {
     off += al((size_t)kColChunks * b.ny * b.kxn * 2 * sizeof(double));
     L.Y = (double*)(base + off);
     off += al((size_t)b.ny * b.kxn * 2 * sizeof(double));
+    _Float16** halves[6] = {&L.twh, &L.twl, &L.twTh, &L.twTl, &L.Vh, &L.Vl};
+    const size_t counts[6] = {(size_t)w * pl.n2p, (size_t)w * pl.n2p, (size_t)pl.n2p * pl.twp,
+                              (size_t)pl.n2p * pl.twp, (size_t)h * pl.n2p, (size_t)h * pl.n2p};
+    for (int i = 0; i < 6; ++i) {
+        *halves[i] = (_Float16*)(base + off);
+        off += al(counts[i] * sizeof(_Float16));
+    }
     L.bytes = off;
     return L;
 }
@@ -472,25 +627,34 @@ size_t highpass_workspace(int w, int h, float hp, float hps) {
 }
 
 // If tables_ready, the twiddle tables (and the zero padding of V) in the workspace are reused.
-int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, hipStream_t s) {
+// value_bound > 0 promises |img| <= value_bound: the two GEMMs then run on fp16 splits of their
+// operands (three fp16 products per product) when every operand stays well inside the fp16 range:
+// |img| <= B, |twiddles| <= 1 and |V| <= 4 ny B (|U| <= sqrt(2) w B, |Y| <= 2 sqrt(2) B).
+int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, float value_bound,
+             hipStream_t s) {
     const HpPlan pl = plan_of(w, h, hp, hps);
     const Band& b = pl.b;
     HpLayout L = hp_layout(ws, w, h, pl);
     const int n2 = pl.n2, n2p = pl.n2p;
     if (!tables_ready) {
-        // tw, twT and V are contiguous in the workspace only up to alignment: clear each
+        // the tables and V are contiguous in the workspace only up to alignment: clear each (the
+        // fp16 arrays hold an even element count: n2p is a multiple of 16)
         const size_t ntw = (size_t)w * n2p, ntwt = (size_t)n2p * pl.twp, nv = (size_t)h * n2p;
-        k_hp_zero<<<(unsigned)((ntw + 255) / 256), 256, 0, s>>>(L.tw, ntw);
-        k_hp_zero<<<(unsigned)((ntwt + 255) / 256), 256, 0, s>>>(L.twT, ntwt);
-        k_hp_zero<<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(L.V, nv);
-        k_row_twiddles<<<dim3(div_up(w, 256), b.kxn), 256, 0, s>>>(L.tw, n2p, L.twT, pl.twp, w, b.kxn);
+        float* const zf[9] = {L.tw, L.twT, L.V, (float*)L.twh, (float*)L.twl, (float*)L.twTh, (float*)L.twTl,
+                              (float*)L.Vh, (float*)L.Vl};
+        const size_t zn[9] = {ntw, ntwt, nv, ntw / 2, ntw / 2, ntwt / 2, ntwt / 2, nv / 2, nv / 2};
+        for (int i = 0; i < 9; ++i) k_hp_zero<<<(unsigned)((zn[i] + 255) / 256), 256, 0, s>>>(zf[i], zn[i]);
+        k_row_twiddles<<<dim3(div_up(w, 256), b.kxn), 256, 0, s>>>(L.tw, n2p, L.twT, pl.twp, L.twh, L.twl, L.twTh,
+                                                                    L.twTl, w, b.kxn);
         k_col_twiddles<<<dim3(div_up(h, 256), b.ny), 256, 0, s>>>(L.ctab, h, b.kyn, b.ny);
     }
     const bool mfma = pitch % 16 == 0 && reinterpret_cast<uintptr_t>(img) % 16 == 0;
+    const bool split = mfma && value_bound > 0.0f && 4.0 * b.ny * (double)value_bound <= 16384.0;
     // 1. U = img . Tw  (M = h, N = 2 kxn, K = w)
     if (mfma) {
-        k_hp_row_dft<<<dim3(div_up(h, 32), div_up(n2p / 16, HP_NTG), pl.nz), HP_WAVES * 64, 0, s>>>(
-            img, pitch, h, w, L.twT, pl.twp, n2p, L.part, pl.kc);
+        auto row_dft = split ? k_hp_row_dft<true> : k_hp_row_dft<false>;
+        row_dft<<<dim3(div_up(h, 32), div_up(n2p / 16, HP_NTG), pl.nz), HP_WAVES * 64, 0, s>>>(
+            img, pitch, h, w, L.twT, L.twTh, L.twTl, pl.twp, n2p, L.part, pl.kc);
         k_hp_sum_parts<<<dim3(div_up(n2, 64), h), 64, 0, s>>>(L.part, L.U, h, n2, n2p, pl.nz);
     } else {
         const int kc = div_up(div_up(w, kRowSplit), TK) * TK;
@@ -503,11 +667,14 @@ int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws,
     k_col_dft_partial<<<dim3(div_up(b.kxn, 64), b.ny, kColChunks), 64, 0, s>>>(L.U, L.ctab, L.P, h, b.kxn, b.ny);
     k_col_dft_finish<<<dim3(div_up(b.kxn, 64), b.ny), 64, 0, s>>>(L.P, L.Y, w, h, b.kxn, b.kyn, b.ny, hp, hps);
     // 3. inverse column DFT
-    k_col_idft<<<dim3(div_up(b.kxn, 64), h), 64, 0, s>>>(L.Y, L.ctab, L.V, h, b.kxn, b.ny, n2p);
+    k_col_idft<<<dim3(div_up(b.kxn, 64), h), 64, 0, s>>>(L.Y, L.ctab, L.V, split ? L.Vh : nullptr,
+                                                         split ? L.Vl : nullptr, h, b.kxn, b.ny, n2p);
     // 4. img = clamp(img - V' . Tw^T)  (M = h, N = w, K = 2 kxn)
-    if (mfma)
-        k_hp_row_idft<<<dim3(div_up(w, 128), div_up(h, 128)), HP_WAVES * 64, 0, s>>>(img, pitch, h, w, L.V, L.tw, n2p);
-    else
+    if (mfma) {
+        auto row_idft = split ? k_hp_row_idft<true> : k_hp_row_idft<false>;
+        row_idft<<<dim3(div_up(w, 128), div_up(h, 128)), HP_WAVES * 64, 0, s>>>(img, pitch, h, w, L.V, L.tw, L.Vh,
+                                                                                L.Vl, L.twh, L.twl, n2p);
+    } else
         k_sgemm<true, 1><<<dim3(div_up(w, TN), div_up(h, TM), 1), 256, 0, s>>>(
             h, w, n2, L.V, n2p * (int)sizeof(float), L.tw, n2p, img, pitch, n2, 0);
     return check_launch("fourier_highpass");
@@ -531,7 +698,7 @@ int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int c
         isa::set_error("isa_fourier_highpass: clear_axis != 0 is not supported");
         return ISA_ERR_UNSUPPORTED;
     }
-    return isa::highpass(img_dev, pitch, width, height, high_pass, high_pass_sigma, workspace_dev, false,
+    return isa::highpass(img_dev, pitch, width, height, high_pass, high_pass_sigma, workspace_dev, false, 0.0f,
                          isa::to_stream(stream));
 }
 

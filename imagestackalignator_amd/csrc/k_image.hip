@@ -166,7 +166,9 @@ constexpr double kB1 = (double)kY, kB2 = kB1 * ((double)kY - 1.0) / 2.0, kB3 = k
 
 __constant__ srgb_pow::Tables kSrgbPow = srgb_pow::make_tables();
 
-__device__ __forceinline__ float pow_srgb(float x) {
+// T: the tables (a per-block LDS copy where the power is evaluated per pixel, so that the
+// per-lane lookups are LDS reads rather than vector-memory gathers).
+__device__ __forceinline__ float pow_srgb(float x, const srgb_pow::Tables& T) {
     using namespace srgb_pow;
     const unsigned ux = __float_as_uint(x);
     const int e = 127 - (int)(ux >> 23);
@@ -174,13 +176,13 @@ __device__ __forceinline__ float pow_srgb(float x) {
     const unsigned mb = ux & 0x7fffffu;
     const int j = (int)(mb >> 16);
     const double d = (double)((int)(mb & 0xffffu) - 0x8000) * 0x1p-23;  // m - c_j, exact
-    const double r = d * kSrgbPow.mant[j][1];
+    const double r = d * T.mant[j][1];
     double q = fma(kB5, r, kB4);
     q = fma(q, r, kB3);
     q = fma(q, r, kB2);
     q = fma(q, r, kB1);
     q = fma(q, r, 1.0);
-    const double z = kSrgbPow.expo[e] * kSrgbPow.mant[j][0] * q;
+    const double z = T.expo[e] * T.mant[j][0] * q;
     const float f = (float)z;
     const unsigned uf = __float_as_uint(f);
     double half = __longlong_as_double((long long)((uf >> 23) + 872u) << 52);  // half an ulp of f (normal)
@@ -192,11 +194,11 @@ __device__ __forceinline__ float pow_srgb(float x) {
 // sRGB OETF after NaN->0 and clamp to [0,1] (R:Kernels/kernel.cu:271-313). The power is pow_r's
 // float (FAST evaluates it through pow_srgb).
 template <bool FAST = true>
-__device__ __forceinline__ float srgb_prepared(float v) {
+__device__ __forceinline__ float srgb_prepared(float v, const srgb_pow::Tables& T = kSrgbPow) {
     if (isnan(v)) v = 0.0f;
     v = fmaxf(fminf(v, 1.0f), 0.0f);
     if (v <= 0.0031308f) return 12.92f * v;
-    return (1.0f + 0.055f) * (FAST ? pow_srgb(v) : pow_r(v, 1.0f / 2.4f)) - 0.055f;
+    return (1.0f + 0.055f) * (FAST ? pow_srgb(v, T) : pow_r(v, 1.0f / 2.4f)) - 0.055f;
 }
 
 // Fused DeBayerFullRes / tracking-image front end, one thread per 2x2 Bayer quad: the four CFA
@@ -214,8 +216,14 @@ __global__ __launch_bounds__(DQ_X* DQ_Y) void k_debayer_quad(int w, int h, const
                                                               DebayerArgs a, float c0, float c1, float c2) {
     __shared__ float s_raw[DQ_RH][DQ_RW + 1];
     __shared__ float s_g[DQ_GH][DQ_GW + 1];
+    __shared__ srgb_pow::Tables s_pow;
     const int x0 = blockIdx.x * 2 * DQ_X, y0 = blockIdx.y * 2 * DQ_Y;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * DQ_X + tx;
+    if (MODE == 1) {
+        constexpr int nd = (int)(sizeof(srgb_pow::Tables) / sizeof(double));
+        static_assert(nd <= DQ_X * DQ_Y, "one table word per thread");
+        if (tid < nd) reinterpret_cast<double*>(&s_pow)[tid] = reinterpret_cast<const double*>(&kSrgbPow)[tid];
+    }
     for (int ly = ty; ly < DQ_RH; ly += DQ_Y) {
         const uint16_t* r = row_ptr(raw, raw_pitch, clampi(y0 - 4 + ly, 0, h - 1));
         for (int lx = tx; lx < DQ_RW; lx += DQ_X) s_raw[ly][lx] = (float)r[clampi(x0 - 4 + lx, 0, w - 1)];
@@ -262,7 +270,8 @@ __global__ __launch_bounds__(DQ_X* DQ_Y) void k_debayer_quad(int w, int h, const
                 o[1] = res[dx][1];
                 o[2] = res[dx][2];
             } else {
-                float r = srgb_prepared(res[dx][0]), g = srgb_prepared(res[dx][1]), b = srgb_prepared(res[dx][2]);
+                float r = srgb_prepared(res[dx][0], s_pow), g = srgb_prepared(res[dx][1], s_pow),
+                      b = srgb_prepared(res[dx][2], s_pow);
                 row_ptr(out, out_pitch, y)[x] = c0 * r + c1 * g + c2 * b;
             }
         }

@@ -26,7 +26,8 @@ int launch_resize_box248(const float* in, int in_pitch, float* o2, int p2, float
 int launch_kernel_param4(float* img, int w, int h, int pitch, float dth, float dtr, float kd, float kn, float ks,
                          float kh, hipStream_t s);
 size_t highpass_workspace(int w, int h, float hp, float hps);
-int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, hipStream_t s);
+int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, float value_bound,
+             hipStream_t s);
 int launch_track_level(const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
                        const int* pa, const int* pb, const int* pk, int pairs, const float* base, float* shifts,
                        size_t meas_stride, float threshold, float* shift_images, int* decisions, hipStream_t s);
@@ -487,7 +488,10 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                         P.camera_white, P.cfa, P.green_channel_only, s));
         ISA_TRY(isa::launch_gauss1(M->gray, wp, M->lk + (size_t)f * np, wp, M->gtmp, w, h, taps_acc, nt_acc, s));
         if (P.use_high_pass) {
-            ISA_TRY(isa::highpass(M->gray, wp, w, h, P.high_pass, P.high_pass_sigma, M->hp_ws, M->hp_tables, s));
+            // the grey image is a convex combination of clamped sRGB values: within [0, 1] up to
+            // rounding, so 2 bounds it
+            ISA_TRY(isa::highpass(M->gray, wp, w, h, P.high_pass, P.high_pass_sigma, M->hp_ws, M->hp_tables, 2.0f,
+                                  s));
             M->hp_tables = true;
         }
         ISA_TRY(isa::launch_gauss1(M->gray, wp, M->trk + (size_t)f * np, wp, M->gtmp, w, h, taps_track, nt_track, s));

@@ -385,13 +385,14 @@ def test_subsample_robustness_erode(isa, oracle, gpu):
 
 @pytest.mark.parametrize("cfa", [RGGB, GRBG])
 @pytest.mark.parametrize("layout", ["aligned", "offset"])
-def test_accumulate_frames(isa, oracle, gpu, cfa, layout):
+@pytest.mark.parametrize("h,w", [(72, 160), (38, 130)])
+def test_accumulate_frames(isa, oracle, gpu, cfa, layout, h, w):
     """Several frames in one pass against the oracle's accumulateImages called once per frame in the
     same order: the parity-grouped wide-load form (aligned frames) and the general form (one raw
     frame at an odd 2-byte offset), with NaN kernel and mask values, wild flows and windows that
     leave the image."""
     rng = np.random.default_rng(41 + cfa[0])
-    h, w, n = 72, 160, 6
+    n = 6
     hw, hh = w // 2, h // 2
     raws = [_raw(rng, h, w) for _ in range(n)]
     kern = np.zeros((h, w, 3), np.float32)
