@@ -43,7 +43,8 @@ def test_merge_matches_oracle(isa, oracle, gpu, case):
     # LK images (debayer, gamma, grey, Gaussian): identical operations on both sides
     for f in range(c["n"]):
         np.testing.assert_array_equal(host(m.buffer_tensor(1, f, 1)), ref["lk"][f])
-    # tracking images: the high-pass low band is an FP32 partial DFT on the GPU, FP64 in the oracle
+    # tracking images: the high-pass low band is a partial DFT on the GPU (products of fp16 hi + lo
+    # splits on the matrix cores for 16-byte aligned rows, else FP32), FP64 in the oracle
     for f in range(c["n"]):
         assert np.abs(host(m.buffer_tensor(0, f, 1)) - ref["track"][f]).max() < 2e-5
     # tile alignment: finest-level one-to-one shifts (tile search exact in FP64; inputs differ by <2e-5)
