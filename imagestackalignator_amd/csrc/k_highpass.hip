@@ -702,4 +702,18 @@ int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int c
                          isa::to_stream(stream));
 }
 
+int isa_fourier_highpass_bounded(float* img_dev, int pitch, int width, int height, int clear_axis, float high_pass,
+                                 float high_pass_sigma, float value_bound, void* workspace_dev, isa_stream_t stream) {
+    ISA_REQUIRE(width > 0 && height > 0 && img_dev && workspace_dev, "isa_fourier_highpass_bounded: bad arguments");
+    ISA_REQUIRE(high_pass >= 0 && high_pass_sigma >= 0 && (double)high_pass + high_pass_sigma < 0.25,
+                "isa_fourier_highpass_bounded: high_pass + high_pass_sigma must lie in [0, 0.25)");
+    ISA_REQUIRE(!(value_bound != value_bound), "isa_fourier_highpass_bounded: value_bound is NaN");
+    if (clear_axis != 0) {
+        isa::set_error("isa_fourier_highpass_bounded: clear_axis != 0 is not supported");
+        return ISA_ERR_UNSUPPORTED;
+    }
+    return isa::highpass(img_dev, pitch, width, height, high_pass, high_pass_sigma, workspace_dev, false, value_bound,
+                         isa::to_stream(stream));
+}
+
 }  // extern "C"

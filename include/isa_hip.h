@@ -168,11 +168,19 @@ int isa_gaussian_taps(float sigma, float* taps_out, int capacity);
 int isa_gauss_filter_border(const float* in_dev, int in_pitch, float* out_dev, int out_pitch, float* tmp_dev, int width,
                             int height, int channels, const float* taps, int n_taps, isa_stream_t stream);
 /* PreAlignment.FourierFilter (R:PEFStudioDX/PreAlignment.cs:268-281 with fourierFilter R:Kernels/kernel.cu:702-779,
- * lp = lps = 1): high-pass in place, then clamp to [0,1]. The low band where the transfer function differs from 1
- * is handled by a partial DFT (see DESIGN). workspace_dev: isa_fourier_highpass_workspace() bytes. */
+ * lp = lps = 1): high-pass in place, then clamp to [0,1]. The transfer function equals 1 outside a low band, so the
+ * filter subtracts the low band's part, computed as a partial DFT (row DFT at the band's columns, column DFT at its
+ * rows, and their inverses) in FP32 products. workspace_dev: isa_fourier_highpass_workspace() bytes. */
 size_t isa_fourier_highpass_workspace(int width, int height, float high_pass, float high_pass_sigma);
 int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int clear_axis, float high_pass,
                          float high_pass_sigma, void* workspace_dev, isa_stream_t stream);
+/* The same filter (R:PEFStudioDX/PreAlignment.cs:268-281) for an image the caller bounds: value_bound > 0 promises
+ * |img| <= value_bound (a grey image in [0,1] passes 1). With 16-byte aligned rows the two GEMMs of the partial DFT
+ * then run on the matrix cores on fp16 hi + lo splits of both operands (three fp16 products per FP32 product, each
+ * within about 2^-21 of it); value_bound <= 0, or a bound that could take an operand near the fp16 range, gives
+ * isa_fourier_highpass. The merger filters its tracking images this way. Same workspace. */
+int isa_fourier_highpass_bounded(float* img_dev, int pitch, int width, int height, int clear_axis, float high_pass,
+                                 float high_pass_sigma, float value_bound, void* workspace_dev, isa_stream_t stream);
 /* NPP ResizeSqrPixel(1/level, SuperSampling) at an integer factor (R:...Controller.cs:1694-1703). */
 int isa_resize_box(const float* in_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height, int level,
                    isa_stream_t stream);

@@ -130,6 +130,32 @@ def test_fourier_highpass(isa, oracle, gpu, w, h, hp, hps):
     assert err < 2e-5, err
 
 
+# At 1024 x 512 the row DFT sums 8 K chunks of 128 columns; at 700 x 300 the band has 4 column tiles.
+@pytest.mark.parametrize("w,h,hp,hps", [(96, 64, 0.01, 0.0025), (700, 300, 0.1, 0.02), (1024, 512, 0.01, 0.0025)])
+def test_fourier_highpass_bounded(isa, oracle, gpu, w, h, hp, hps):
+    """The fp16-split matrix-core form for an image in [0, 1]: within 2e-5 of the FP64 oracle (the bar of the
+    FP32-product form) and within 2e-6 of the FP32-product form; a bound too large for fp16 operands gives the
+    FP32-product form bit for bit."""
+    rng = np.random.default_rng(6)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = 0.5 + 0.3 * np.sin(xx / 17.0) * np.cos(yy / 23.0) + 0.05 * rng.standard_normal((h, w))
+    img = np.clip(img, 0.0, 1.0).astype(np.float32)
+    ws = empty((isa.lib().isa_fourier_highpass_workspace(w, h, hp, hps) // 4 + 64,))
+    d32, d16, dbig = dev(img), dev(img), dev(img)
+    args = (w * 4, w, h, 0, C.c_float(hp), C.c_float(hps))
+    isa.call("isa_fourier_highpass", p(d32), *args, p(ws), stream())
+    isa.call("isa_fourier_highpass_bounded", p(d16), *args, C.c_float(1.0), p(ws), stream())
+    isa.call("isa_fourier_highpass_bounded", p(dbig), *args, C.c_float(1e9), p(ws), stream())
+    ref = oracle.fourier_highpass(img, hp, hps, dt=np.float64)
+    e64 = np.abs(host(d16).astype(np.float64) - ref).max()
+    e32 = np.abs(host(d16).astype(np.float64) - host(d32)).max()
+    print(f"highpass bounded {w}x{h}: max |fp16 split - FP64| {e64:.3g}, max |fp16 split - FP32 products| {e32:.3g}")
+    assert e64 < 2e-5 and e32 < 2e-6
+    np.testing.assert_array_equal(host(dbig), host(d32))
+    with pytest.raises(isa.IsaError):
+        isa.call("isa_fourier_highpass_bounded", p(d16), *args, C.c_float(float("nan")), p(ws), stream())
+
+
 @pytest.mark.parametrize("h,w,which", [(64, 96, (2, 4, 8)), (40, 520, (2, 4, 8)), (16, 8, (2, 8)), (48, 1032, (4,))])
 def test_resize_box_levels(isa, oracle, gpu, h, w, which):
     """The one-pass pyramid equals isa_resize_box (the oracle's box means) bit for bit, for any subset of levels."""
