@@ -11,20 +11,46 @@ namespace {
 // evaluated in double and rounded once to float (by the launcher, once per field).
 __global__ void k_flow_from_tiles(float* flow, int flow_pitch, const float* tiles, int tiles_pitch, int tw, int th,
                                   int w, int h, float bsx, float bsy, float c, float s, int quant) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= w || y >= h) return;
-    float fx = c * -bsx - s * -bsy;
-    float fy = s * -bsx + c * -bsy;
-    float pcx = (float)(x - w / 2), pcy = (float)(y - h / 2);
-    fx += c * pcx - s * pcy - pcx;
-    fy += s * pcx + c * pcy - pcy;
+    // four consecutive pixels of one row per thread: the row's texture coordinate, weights and tile
+    // rows are shared; every pixel's arithmetic is tex_linear_all<2>'s, in its order
+    const int x0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x), y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x0 >= w || y >= h) return;
     const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
-    float u = div_n((float)x + 0.5f, iw), v = div_n((float)y + 0.5f, ih);
-    float t[2];
-    tex_linear_all<2>(tiles, tiles_pitch, tw, th, u, v, ISA_ADDR_CLAMP, quant, t);
-    float* o = row_ptr(flow, flow_pitch, y) + x * 2;
-    o[0] = fx + t[0];
-    o[1] = fy + t[1];
+    const float gx = c * -bsx - s * -bsy, gy = s * -bsx + c * -bsy;
+    const float pcy = (float)(y - h / 2);
+    const float v = div_n((float)y + 0.5f, ih);
+    const float yf = v * (float)th - 0.5f, fy = floorf(yf);
+    const float b = tex_frac(yf - fy, quant), omb = 1.0f - b;
+    const int ty0 = (int)fy;
+    const float* ra = row_ptr(tiles, tiles_pitch, tex_addr(ty0, th, ISA_ADDR_CLAMP));
+    const float* rb = row_ptr(tiles, tiles_pitch, tex_addr(ty0 + 1, th, ISA_ADDR_CLAMP));
+    float out[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int x = min(x0 + i, w - 1);
+        const float pcx = (float)(x - w / 2);
+        const float fx = gx + (c * pcx - s * pcy - pcx), fyv = gy + (s * pcx + c * pcy - pcy);
+        const float u = div_n((float)x + 0.5f, iw);
+        const float xf = u * (float)tw - 0.5f, fxf = floorf(xf);
+        const float a = tex_frac(xf - fxf, quant), oma = 1.0f - a;
+        const int tx0 = (int)fxf;
+        const int xa = tex_addr(tx0, tw, ISA_ADDR_CLAMP), xb = tex_addr(tx0 + 1, tw, ISA_ADDR_CLAMP);
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+            const float t00 = ra[xa * 2 + ch], t10 = ra[xb * 2 + ch], t01 = rb[xa * 2 + ch], t11 = rb[xb * 2 + ch];
+            out[2 * i + ch] = (ch == 0 ? fx : fyv) + (oma * omb * t00 + a * omb * t10 + oma * b * t01 + a * b * t11);
+        }
+    }
+    float* o = row_ptr(flow, flow_pitch, y) + x0 * 2;
+    if (x0 + 4 <= w && (reinterpret_cast<uintptr_t>(o) & 15) == 0) {
+        reinterpret_cast<float4*>(o)[0] = make_float4(out[0], out[1], out[2], out[3]);
+        reinterpret_cast<float4*>(o)[1] = make_float4(out[4], out[5], out[6], out[7]);
+    } else {
+        for (int i = 0; i < 4 && x0 + i < w; ++i) {
+            o[2 * i] = out[2 * i];
+            o[2 * i + 1] = out[2 * i + 1];
+        }
+    }
 }
 
 // WarpingKernel (R:Kernels/opticalFlow.cu:27-44): flow texture point/clamp, image linear/mirror.
@@ -694,8 +720,9 @@ namespace isa {
 int launch_flow_from_tiles(float* flow, int flow_pitch, const float* tiles, int tiles_pitch, int tw, int th, int w,
                            int h, float bsx, float bsy, float rot, int quant, hipStream_t s) {
     const float c = (float)std::cos((double)rot), sn = (float)std::sin((double)rot);
-    k_flow_from_tiles<<<dim3(div_up(w, 32), div_up(h, 8)), dim3(32, 8), 0, s>>>(flow, flow_pitch, tiles, tiles_pitch,
-                                                                                 tw, th, w, h, bsx, bsy, c, sn, quant);
+    k_flow_from_tiles<<<dim3(div_up(w, 4 * 64), div_up(h, 4)), dim3(64, 4), 0, s>>>(flow, flow_pitch, tiles,
+                                                                                     tiles_pitch, tw, th, w, h, bsx,
+                                                                                     bsy, c, sn, quant);
     return check_launch("flow_from_tiles");
 }
 
