@@ -368,6 +368,21 @@ def test_structure_tensor_kernel_param(isa, oracle, gpu):
     np.testing.assert_array_equal(host(t), tref)
 
 
+@pytest.mark.parametrize("cfa", [RGGB, GRBG])
+@pytest.mark.parametrize("hw,hh", [(36, 10), (38, 9), (260, 7)])
+def test_subsample3_forms(isa, oracle, gpu, cfa, hw, hh):
+    """deBayersSubSample3 bit for bit: four output pixels per thread (half width a multiple of 4, aligned rows)
+    and the per-pixel form (38)."""
+    rng = np.random.default_rng(hw + hh)
+    raw = _raw(rng, 2 * hh, 2 * hw)
+    out = empty((hh, hw, 3), fill=-1.0)
+    isa.call("isa_debayer_subsample3", p(dev(raw)), p(out), C.c_float(16383.0), hw, hh, hw * 12, isa.i32s(*cfa),
+             stream())
+    ref = np.zeros((hh, hw, 3), np.float32)
+    oracle.fn("subsample3")(oracle.ptr(raw), oracle.ptr(ref), 16383.0, hw, hh, oracle.ptr(oracle.arr(cfa, np.int32)))
+    np.testing.assert_array_equal(host(out), ref)
+
+
 def test_subsample_robustness_erode(isa, oracle, gpu):
     rng = np.random.default_rng(14)
     h, w = 64, 80
