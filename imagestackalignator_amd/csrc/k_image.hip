@@ -612,7 +612,7 @@ __global__ void k_erode4(const float* in, int in_pitch, float* out, int out_pitc
 
 // The same erosion (replicate border), separable (a min over the window's rows of mins over its
 // columns is the min over the window; fminf ignores NaN in any order) and staged in LDS.
-constexpr int ER_BX = 32, ER_BY = 8, ER_RMAX = 7;
+constexpr int ER_BX = 32, ER_BY = 16, ER_RMAX = 7;
 __global__ __launch_bounds__(ER_BX* ER_BY) void k_erode_staged(const float* in, int in_pitch, float* out, int out_pitch,
                                                                int w, int h, int size) {
     __shared__ float4 s_in[ER_BY + 2 * ER_RMAX][ER_BX + 2 * ER_RMAX];
