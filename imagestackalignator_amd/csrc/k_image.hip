@@ -313,12 +313,18 @@ __global__ void k_gauss_h(const float* in, int in_pitch, float* tmp, int w, int 
     if (x >= w || y >= h) return;
     const float* r = row_ptr(in, in_pitch, y);
     int c0 = taps.n / 2;
+    // every channel's sum in tap order; the channels of a tap are read together
+    float s[CH];
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-        float s = 0.0f;
-        for (int i = 0; i < taps.n; ++i) s += taps.t[i] * r[clampi(x + i - c0, 0, w - 1) * CH + c];
-        tmp[((size_t)y * w + x) * CH + c] = s;
+    for (int c = 0; c < CH; ++c) s[c] = 0.0f;
+    for (int i = 0; i < taps.n; ++i) {
+        const float* q = r + clampi(x + i - c0, 0, w - 1) * CH;
+        const float t = taps.t[i];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) s[c] += t * q[c];
     }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) tmp[((size_t)y * w + x) * CH + c] = s[c];
 }
 
 template <int CH>
@@ -326,12 +332,17 @@ __global__ void k_gauss_v(const float* tmp, float* out, int out_pitch, int w, in
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
     int c0 = taps.n / 2;
+    float s[CH];
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-        float s = 0.0f;
-        for (int i = 0; i < taps.n; ++i) s += taps.t[i] * tmp[((size_t)clampi(y + i - c0, 0, h - 1) * w + x) * CH + c];
-        row_ptr(out, out_pitch, y)[x * CH + c] = s;
+    for (int c = 0; c < CH; ++c) s[c] = 0.0f;
+    for (int i = 0; i < taps.n; ++i) {
+        const float* q = tmp + ((size_t)clampi(y + i - c0, 0, h - 1) * w + x) * CH;
+        const float t = taps.t[i];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) s[c] += t * q[c];
     }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) row_ptr(out, out_pitch, y)[x * CH + c] = s[c];
 }
 
 // Single-channel Gaussian fused in LDS (taps <= 15): same operation order as the two-pass form.
