@@ -669,6 +669,29 @@ __global__ void k_apply_weighting(float* inout, const float* sum, const float* w
     }
 }
 
+// The same, elementwise (each channel value is weighted on its own): one float4 of a row's 3 w floats
+// per thread, so a wave reads and writes 1 KB runs. Needs w % 4 == 0, 16-byte aligned rows and pointers.
+__global__ void k_apply_weighting_x4(float* inout, const float* sum, const float* weight, int w, int h, int pitch,
+                                     float thr) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (4 * j >= 3 * w || y >= h) return;
+    float4* io = reinterpret_cast<float4*>(row_ptr(inout, pitch, y)) + j;
+    const float4 s4 = reinterpret_cast<const float4*>(row_ptr(sum, pitch, y))[j];
+    const float4 w4 = reinterpret_cast<const float4*>(row_ptr(weight, pitch, y))[j];
+    const float4 v4 = *io;
+    const float sv[4] = {s4.x, s4.y, s4.z, s4.w}, wv[4] = {w4.x, w4.y, w4.z, w4.w};
+    float ov[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {  // the per-element operations of k_apply_weighting
+        float val = sv[e], ww = wv[e], v = ov[e];
+        if (ww < thr) { val += v; ww += 1.0f; }
+        v = 0.0f;
+        if (ww != 0.0f) v = val / ww;
+        ov[e] = v;
+    }
+    *io = make_float4(ov[0], ov[1], ov[2], ov[3]);
+}
+
 DebayerArgs make_debayer_args(const float black[3], const float scale[3], const float div[3], const int cfa[4]) {
     DebayerArgs a;
     for (int i = 0; i < 4; ++i) a.cfa.c[i] = cfa[i];
@@ -943,8 +966,14 @@ int isa_erode_border4(const float* in4_dev, int in_pitch, float* out4_dev, int o
 int isa_apply_weighting(float* inout3_dev, const float* sum3_dev, const float* weight3_dev, int width, int height,
                         int pitch, float threshold, isa_stream_t stream) {
     ISA_REQUIRE(width > 0 && height > 0 && inout3_dev && sum3_dev && weight3_dev, "isa_apply_weighting: bad arguments");
-    k_apply_weighting<<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0,
-                        isa::to_stream(stream)>>>(inout3_dev, sum3_dev, weight3_dev, width, height, pitch, threshold);
+    auto al16 = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+    if (width % 4 == 0 && pitch % 16 == 0 && al16(inout3_dev) && al16(sum3_dev) && al16(weight3_dev))
+        k_apply_weighting_x4<<<dim3(isa::div_up(3 * width / 4, 256), height), 256, 0, isa::to_stream(stream)>>>(
+            inout3_dev, sum3_dev, weight3_dev, width, height, pitch, threshold);
+    else
+        k_apply_weighting<<<dim3(isa::div_up(width, 32), isa::div_up(height, 8)), dim3(32, 8), 0,
+                            isa::to_stream(stream)>>>(inout3_dev, sum3_dev, weight3_dev, width, height, pitch,
+                                                      threshold);
     return isa::check_launch("isa_apply_weighting");
 }
 

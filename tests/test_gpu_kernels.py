@@ -478,10 +478,10 @@ def test_accumulate_frames(isa, oracle, gpu, cfa, layout, h, w):
     np.testing.assert_array_equal(host(dw), wr_)
 
 
-@pytest.mark.parametrize("sr", [0, 1])
-def test_accumulate_apply_weighting(isa, oracle, gpu, sr):
+@pytest.mark.parametrize("sr,w", [(0, 64), (1, 64), (0, 62)])
+def test_accumulate_apply_weighting(isa, oracle, gpu, sr, w):
     rng = np.random.default_rng(15 + sr)
-    h, w = 48, 64
+    h = 48
     hw, hh = w // 2, h // 2
     raw = _raw(rng, h, w)
     kch = 4 if sr else 3
