@@ -1,12 +1,39 @@
 // Library-wide C ABI: version, defaults, error text, device query.
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
+#include <cstdint>
 #include <cstring>
 
 #include "isa_host.h"
 
 namespace {
 thread_local char g_err[512] = "";
+
+// One piece of a host table carried to the device in the kernel's argument block.
+constexpr int kPutWords = 480;  // 1920 bytes of payload per launch
+struct PutChunk {
+    unsigned w[kPutWords];
+};
+
+__global__ void k_put_words(PutChunk c, unsigned* dst, int nwords) {
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) dst[i] = c.w[i];
+}
+}  // namespace
+
+int isa::upload_by_args(void* dst_dev, const void* src, size_t bytes, hipStream_t s) {
+    ISA_REQUIRE(reinterpret_cast<uintptr_t>(dst_dev) % 4 == 0 && bytes % 4 == 0, "upload: unaligned table");
+    const unsigned* in = static_cast<const unsigned*>(src);
+    unsigned* out = static_cast<unsigned*>(dst_dev);
+    const size_t words = bytes / 4;
+    for (size_t o = 0; o < words; o += kPutWords) {
+        const int nw = (int)std::min<size_t>(kPutWords, words - o);
+        PutChunk c;
+        std::memcpy(c.w, in + o, (size_t)nw * 4);
+        k_put_words<<<1, 256, 0, s>>>(c, out + o, nw);
+        ISA_TRY(check_launch("upload"));
+    }
+    return ISA_OK;
 }
 
 void isa::set_error(const char* fmt, ...) {

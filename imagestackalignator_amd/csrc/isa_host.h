@@ -47,6 +47,11 @@ inline int check_hip(hipError_t e, const char* what) {
         if (_rc != ISA_OK) return _rc; \
     } while (0)
 
+// Copies a host table (bytes a multiple of 4, dst 4-byte aligned) into device memory in stream
+// order through kernel arguments: the host buffer may be reused as soon as this returns, and
+// nothing waits for the device. For small tables of the per-call entry points.
+int upload_by_args(void* dst_dev, const void* src, size_t bytes, hipStream_t s);
+
 // Shift-collection tables (R:PEFStudioDX/ShiftCollection.cs:113-496); host integers.
 // Fills pairs (reference, toTrack) in GetShiftPairs order, the n x n index
 // table and the (a, b) endpoints of every measurement row (row k measures

@@ -1409,111 +1409,147 @@ int isa_upsample_shifts(const float* in2_dev, float* out2_dev, int in_pitch, int
                                 new_count_x, new_count_y, old_tile_size, new_tile_size, isa::to_stream(stream));
 }
 
+}  // extern "C"
+
+namespace {
+
+// Workspace of the per-call tile search (one pair): the argument block the level kernels read
+// (frame pointers, pair indices, base shifts, the exact-search counter), the tracked shifts in
+// dense rows, and for the matrix-core forms the split planes, window energies and queue.
+struct TrackHead {
+    const float* frames[2];
+    const unsigned* xplanes[2];
+    const float* vbox[2];
+    const float* vrun[2];
+    int pa, pb, pk, count;
+    int chunk[16];
+    float base[8];
+};
+
+struct TrackLayout {
+    size_t head, shifts, planes, vbox, vrun, queue, bytes;
+};
+
+size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+TrackLayout track_layout(int width, int height, int img_pitch, int tile_size, int max_shift, int precision) {
+    const int cx = (width - 2 * max_shift) / tile_size, cy = (height - 2 * max_shift) / tile_size;
+    const bool split = precision == ISA_XC_BF16X3 || precision == ISA_XC_FP16X3;
+    const bool mfma = precision != ISA_XC_FP64;
+    TrackLayout t;
+    size_t o = 0;
+    t.head = o; o += al256(sizeof(TrackHead));
+    t.shifts = o; o += al256((size_t)cx * cy * 2 * sizeof(float));
+    t.planes = o; o += split ? 2 * al256(((size_t)img_pitch / 4) * height * sizeof(unsigned)) : 0;
+    t.vbox = o; o += mfma ? al256((size_t)(height - tile_size + 1) * width * sizeof(float)) : 0;
+    t.vrun = o; o += mfma ? al256((size_t)isa::vbox_run_rows(height, tile_size) * width * sizeof(float)) : 0;
+    t.queue = o; o += mfma ? al256((size_t)cx * cy * 2 * sizeof(int)) : 0;
+    t.bytes = o;
+    return t;
+}
+
+__global__ void k_put_track_head(TrackHead h, TrackHead* dst) { *dst = h; }
+
+__global__ void k_set_int(int* dst, const int* src, int value) { *dst = src ? *src : value; }
+
+}  // namespace
+
+extern "C" {
+
+size_t isa_track_workspace(int width, int height, int img_pitch, int tile_size, int max_shift, int xc_precision) {
+    if (width <= 0 || height <= 0 || tile_size <= 0 || max_shift < 1 || max_shift > kMaxShift ||
+        img_pitch < width * 4 || xc_precision < ISA_XC_FP64 || xc_precision > ISA_XC_FP32)
+        return 0;
+    if ((width - 2 * max_shift) / tile_size <= 0 || (height - 2 * max_shift) / tile_size <= 0) return 0;
+    return track_layout(width, height, img_pitch, tile_size, max_shift, xc_precision).bytes;
+}
+
 int isa_track_xc_ex(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
                     int preshift_pitch, int width, int height, int tile_size, int max_shift,
                     const float base_shift_ref[2], float base_rotation_ref, const float base_shift_track[2],
                     float base_rotation_track, float threshold, int xc_precision, int* decisions_dev,
-                    int* exact_items, isa_stream_t stream) {
+                    int* exact_count_dev, void* workspace_dev, isa_stream_t stream) {
     // PatchTracker.Track through the fused level kernel with one pair (frame 0 = reference, frame 1 = tracked).
-    ISA_REQUIRE(img_track_dev && img_ref_dev && preshift2_dev && width > 0 && height > 0 && tile_size > 0,
+    ISA_REQUIRE(img_track_dev && img_ref_dev && preshift2_dev && workspace_dev && width > 0 && height > 0 &&
+                    tile_size > 0,
                 "isa_track: bad arguments");
+    ISA_REQUIRE(reinterpret_cast<uintptr_t>(workspace_dev) % 256 == 0, "isa_track: workspace must be 256-byte aligned");
     ISA_REQUIRE(xc_precision >= ISA_XC_FP64 && xc_precision <= ISA_XC_FP32, "isa_track: unknown xc_precision");
     ISA_REQUIRE(max_shift >= 1 && max_shift <= kMaxShift, "isa_track: max_shift must be in [1, 8]");
-    int cx = (width - 2 * max_shift) / tile_size, cy = (height - 2 * max_shift) / tile_size;
+    const int cx = (width - 2 * max_shift) / tile_size, cy = (height - 2 * max_shift) / tile_size;
     ISA_REQUIRE(cx > 0 && cy > 0, "isa_track: image smaller than one tile");
-    hipStream_t s = isa::to_stream(stream);
     ISA_REQUIRE(img_pitch % 4 == 0 && img_pitch >= width * 4, "isa_track: bad image pitch");
+    ISA_REQUIRE(preshift_pitch >= cx * 8, "isa_track: bad shift pitch");
+    hipStream_t s = isa::to_stream(stream);
     const bool split = xc_precision == ISA_XC_BF16X3 || xc_precision == ISA_XC_FP16X3;
     const bool mfma = xc_precision != ISA_XC_FP64;
-    struct Scratch {
-        const float* frames[2];
-        const unsigned* xplanes[2];
-        const float* vbox[2];
-        const float* vrun[2];
-        int pa, pb, pk, count;
-        int chunk[16];
-        float base[8];
-    } h{{img_ref_dev, img_track_dev}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, 0, 1, 0, 0,
-        {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
-        {base_shift_ref[0], base_shift_ref[1], (float)std::sin((double)base_rotation_ref),
-         (float)std::cos((double)base_rotation_ref), base_shift_track[0], base_shift_track[1],
-         (float)std::sin((double)base_rotation_track), (float)std::cos((double)base_rotation_track)}};
-    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    const size_t head = al(sizeof(Scratch));
-    const size_t row = (size_t)cx * 2 * sizeof(float);
-    const size_t shifts_bytes = al(row * cy);
-    const size_t plane = split ? al(((size_t)img_pitch / 4) * height * sizeof(unsigned)) : 0;
-    const size_t vbytes = mfma ? al((size_t)(height - tile_size + 1) * width * sizeof(float)) : 0;
-    const size_t rbytes = mfma ? al((size_t)isa::vbox_run_rows(height, tile_size) * width * sizeof(float)) : 0;
-    const size_t qbytes = mfma ? al((size_t)cx * cy * 2 * sizeof(int)) : 0;
-    char* d = nullptr;
-    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d, head + shifts_bytes + 2 * plane + vbytes + rbytes + qbytes, s),
-                           "isa_track: allocation"));
-    float* dn = (float*)(d + head);
-    char* o = d + head + shifts_bytes;
+    const TrackLayout tl = track_layout(width, height, img_pitch, tile_size, max_shift, xc_precision);
+    char* d = static_cast<char*>(workspace_dev);
+    TrackHead h{{img_ref_dev, img_track_dev}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, 0, 1, 0, 0,
+                {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
+                {base_shift_ref[0], base_shift_ref[1], (float)std::sin((double)base_rotation_ref),
+                 (float)std::cos((double)base_rotation_ref), base_shift_track[0], base_shift_track[1],
+                 (float)std::sin((double)base_rotation_track), (float)std::cos((double)base_rotation_track)}};
+    const size_t plane = split ? (tl.vbox - tl.planes) / 2 : 0;
     if (split) {
-        h.xplanes[0] = (const unsigned*)o;
-        h.xplanes[1] = (const unsigned*)(o + plane);
+        h.xplanes[0] = (const unsigned*)(d + tl.planes);
+        h.xplanes[1] = (const unsigned*)(d + tl.planes + plane);
     }
-    o += 2 * plane;
     if (mfma) {
-        h.vbox[1] = (const float*)o;
-        h.vrun[1] = (const float*)(o + vbytes);
+        h.vbox[1] = (const float*)(d + tl.vbox);
+        h.vrun[1] = (const float*)(d + tl.vrun);
     }
-    int* queue = (int*)(o + vbytes + rbytes);
-    const Scratch* sd = (const Scratch*)d;
-    int rc = isa::check_hip(hipMemcpyAsync(d, &h, sizeof(Scratch), hipMemcpyHostToDevice, s), "isa_track: upload");
-    if (rc == ISA_OK)
-        rc = isa::check_hip(hipMemcpy2DAsync(dn, row, preshift2_dev, preshift_pitch, row, cy, hipMemcpyDeviceToDevice, s),
-                            "isa_track: copy");
-    for (int f = 0; f < 2 && split && rc == ISA_OK; ++f)
-        rc = isa::split_planes(xc_precision, f ? img_track_dev : img_ref_dev, (unsigned*)h.xplanes[f],
-                               ((size_t)img_pitch / 4) * height, s);
-    if (mfma && rc == ISA_OK)
-        rc = isa::vbox_energies(img_track_dev, img_pitch, width, height, tile_size, (float*)h.vbox[1],
-                                (float*)h.vrun[1], s);
-    if (rc == ISA_OK)
-        rc = !mfma ? isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
-                                             &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, decisions_dev, s)
-                   : isa::launch_track_mfma(xc_precision, sd->frames, sd->xplanes, sd->vbox, sd->vrun, img_pitch, width,
-                                            height, tile_size, max_shift, cx, cy, &sd->pa, &sd->pb, &sd->pk, sd->chunk, 1,
-                                            sd->base, dn, 0, threshold, nullptr, decisions_dev, queue,
-                                            (int*)&sd->count, s);
-    if (mfma && rc == ISA_OK)
-        rc = isa::launch_track_queue(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
-                                     &sd->pb, &sd->pk, sd->base, dn, 0, threshold, decisions_dev, queue, &sd->count, s);
-    if (rc == ISA_OK)
-        rc = isa::check_hip(hipMemcpy2DAsync(preshift2_dev, preshift_pitch, dn, row, row, cy, hipMemcpyDeviceToDevice, s),
-                            "isa_track: copy back");
-    int count = 0;
-    if (rc == ISA_OK && exact_items)
-        rc = isa::check_hip(hipMemcpyAsync(&count, &sd->count, sizeof(int), hipMemcpyDeviceToHost, s),
-                            "isa_track: count");
-    // the host copies `h` and `count` must outlive the transfers: synchronise before returning
-    int rs = isa::check_hip(hipStreamSynchronize(s), "isa_track: sync");
-    hipError_t fe = hipFreeAsync(d, s);
-    if (rc == ISA_OK) rc = rs;
-    if (rc == ISA_OK) rc = isa::check_hip(fe, "isa_track: free");
-    if (rc == ISA_OK && exact_items) *exact_items = mfma ? count : cx * cy;
-    return rc;
+    TrackHead* sd = (TrackHead*)(d + tl.head);
+    float* dn = (float*)(d + tl.shifts);
+    int* queue = (int*)(d + tl.queue);
+    const size_t row = (size_t)cx * 2 * sizeof(float);
+    // the argument block travels in the kernel's own arguments: nothing on the host has to outlive a copy
+    k_put_track_head<<<1, 1, 0, s>>>(h, sd);
+    ISA_TRY(isa::check_launch("isa_track: arguments"));
+    ISA_TRY(isa::check_hip(hipMemcpy2DAsync(dn, row, preshift2_dev, preshift_pitch, row, cy, hipMemcpyDeviceToDevice, s),
+                           "isa_track: copy"));
+    for (int f = 0; f < 2 && split; ++f)
+        ISA_TRY(isa::split_planes(xc_precision, f ? img_track_dev : img_ref_dev, (unsigned*)h.xplanes[f],
+                                  ((size_t)img_pitch / 4) * height, s));
+    if (mfma)
+        ISA_TRY(isa::vbox_energies(img_track_dev, img_pitch, width, height, tile_size, (float*)h.vbox[1],
+                                   (float*)h.vrun[1], s));
+    if (!mfma)
+        ISA_TRY(isa::launch_track_level(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
+                                        &sd->pb, &sd->pk, 1, sd->base, dn, 0, threshold, nullptr, decisions_dev, s));
+    else {
+        ISA_TRY(isa::launch_track_mfma(xc_precision, sd->frames, sd->xplanes, sd->vbox, sd->vrun, img_pitch, width,
+                                       height, tile_size, max_shift, cx, cy, &sd->pa, &sd->pb, &sd->pk, sd->chunk, 1,
+                                       sd->base, dn, 0, threshold, nullptr, decisions_dev, queue, &sd->count, s));
+        ISA_TRY(isa::launch_track_queue(sd->frames, img_pitch, width, height, tile_size, max_shift, cx, cy, &sd->pa,
+                                        &sd->pb, &sd->pk, sd->base, dn, 0, threshold, decisions_dev, queue,
+                                        &sd->count, s));
+    }
+    ISA_TRY(isa::check_hip(hipMemcpy2DAsync(preshift2_dev, preshift_pitch, dn, row, row, cy, hipMemcpyDeviceToDevice, s),
+                           "isa_track: copy back"));
+    if (exact_count_dev) {
+        k_set_int<<<1, 1, 0, s>>>(exact_count_dev, mfma ? &sd->count : nullptr, cx * cy);
+        ISA_TRY(isa::check_launch("isa_track: count"));
+    }
+    return ISA_OK;
 }
 
 int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
                  int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
                  float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
-                 int xc_precision, isa_stream_t stream) {
+                 int xc_precision, void* workspace_dev, isa_stream_t stream) {
     return isa_track_xc_ex(img_track_dev, img_ref_dev, img_pitch, preshift2_dev, preshift_pitch, width, height,
                            tile_size, max_shift, base_shift_ref, base_rotation_ref, base_shift_track,
-                           base_rotation_track, threshold, xc_precision, nullptr, nullptr, stream);
+                           base_rotation_track, threshold, xc_precision, nullptr, nullptr, workspace_dev, stream);
 }
 
 int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
               int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
               float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
-              isa_stream_t stream) {
+              void* workspace_dev, isa_stream_t stream) {
     return isa_track_xc(img_track_dev, img_ref_dev, img_pitch, preshift2_dev, preshift_pitch, width, height, tile_size,
                         max_shift, base_shift_ref, base_rotation_ref, base_shift_track, base_rotation_track, threshold,
-                        ISA_XC_FP64, stream);
+                        ISA_XC_FP64, workspace_dev, stream);
 }
 
 int isa_shift_tables(int frame_count, int reference_index, int strategy, int block_size, int* pairs_out,
@@ -1521,37 +1557,53 @@ int isa_shift_tables(int frame_count, int reference_index, int strategy, int blo
     return isa::shift_tables(frame_count, reference_index, strategy, block_size, pairs_out, index_out, rows_out);
 }
 
+// Workspace of isa_minimize_shifts: the measurement rows, then (for a connected graph of at most
+// kMinFrames frames) the Cholesky factor of the all-rows normal equations.
+static bool minimize_layout(int frame_count, int reference_index, int strategy, int block_size, int* m_out,
+                            size_t* rows_bytes, size_t* total) {
+    int m = isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, nullptr);
+    if (m <= 0 || m > 4096) return false;
+    const int n1 = frame_count - 1;
+    *m_out = m;
+    *rows_bytes = ((size_t)2 * m * sizeof(int) + 15) & ~(size_t)15;
+    *total = *rows_bytes + (size_t)n1 * n1 * sizeof(double);
+    return true;
+}
+
+size_t isa_minimize_shifts_workspace(int frame_count, int reference_index, int strategy, int block_size) {
+    int m = 0;
+    size_t rb = 0, tot = 0;
+    return minimize_layout(frame_count, reference_index, strategy, block_size, &m, &rb, &tot) ? tot : 0;
+}
+
 int isa_minimize_shifts(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev, int* iters_dev,
                         int tile_count, int frame_count, int reference_index, int strategy, int block_size,
-                        isa_stream_t stream) {
-    ISA_REQUIRE(measured2_dev && one_to_one2_dev && tile_count > 0 && meas_pitch >= tile_count * 8,
+                        void* workspace_dev, isa_stream_t stream) {
+    ISA_REQUIRE(measured2_dev && one_to_one2_dev && workspace_dev && tile_count > 0 && meas_pitch >= tile_count * 8 &&
+                    meas_pitch % 8 == 0,
                 "isa_minimize_shifts: bad arguments");
-    std::vector<int> rows(2 * 4096);
-    int m = isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, nullptr);
-    ISA_REQUIRE(m > 0 && m <= 4096, "isa_minimize_shifts: invalid strategy / frame count");
+    ISA_REQUIRE(reinterpret_cast<uintptr_t>(workspace_dev) % 16 == 0, "isa_minimize_shifts: workspace must be 16-byte aligned");
+    int m = 0;
+    size_t rows_bytes = 0, total = 0;
+    ISA_REQUIRE(minimize_layout(frame_count, reference_index, strategy, block_size, &m, &rows_bytes, &total),
+                "isa_minimize_shifts: invalid strategy / frame count");
+    std::vector<int> rows(2 * m);
     isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, rows.data());
     hipStream_t s = isa::to_stream(stream);
-    // rows, then (for a connected graph) the all-rows Cholesky factor, in one upload
     const int n1 = frame_count - 1;
-    const size_t rows_bytes = ((size_t)2 * m * sizeof(int) + 15) & ~(size_t)15;
-    std::vector<char> host(rows_bytes + (size_t)n1 * n1 * sizeof(double));
+    std::vector<char> host(total, 0);
     std::memcpy(host.data(), rows.data(), 2 * m * sizeof(int));
     const bool have_factor =
         frame_count <= kMinFrames &&
         isa::minimize_full_factor(rows.data(), frame_count, m, (double*)(host.data() + rows_bytes));
-    int* d_rows = nullptr;
-    ISA_TRY(isa::check_hip(hipMallocAsync((void**)&d_rows, host.size(), s), "isa_minimize_shifts: allocation"));
-    int rc = isa::check_hip(hipMemcpyAsync(d_rows, host.data(), host.size(), hipMemcpyHostToDevice, s),
-                            "isa_minimize_shifts: upload");
-    const double* d_factor = have_factor ? (const double*)((char*)d_rows + rows_bytes) : nullptr;
-    if (rc == ISA_OK)
-        rc = isa::launch_minimize(measured2_dev, meas_pitch / sizeof(float), one_to_one2_dev, status_dev, iters_dev,
-                                  d_rows, d_factor, tile_count, frame_count, m, s);
-    int rs = isa::check_hip(hipStreamSynchronize(s), "isa_minimize_shifts: sync");  // `rows` must outlive the upload
-    hipError_t fe = hipFreeAsync(d_rows, s);
-    if (rc == ISA_OK) rc = rs;
-    if (rc == ISA_OK) rc = isa::check_hip(fe, "isa_minimize_shifts: free");
-    return rc;
+    // rows and factor travel in kernel arguments: no host buffer outlives this call, nothing waits
+    const size_t up = have_factor ? total : rows_bytes;
+    ISA_TRY(isa::upload_by_args(workspace_dev, host.data(), up, s));
+    const int* d_rows = (const int*)workspace_dev;
+    const double* d_factor = have_factor ? (const double*)((char*)workspace_dev + rows_bytes) : nullptr;
+    (void)n1;
+    return isa::launch_minimize(measured2_dev, meas_pitch / sizeof(float), one_to_one2_dev, status_dev, iters_dev,
+                                d_rows, d_factor, tile_count, frame_count, m, s);
 }
 
 int isa_get_optimal_shifts(float* out2_dev, int out_pitch, const float* one_to_one2_dev, int frame_count,

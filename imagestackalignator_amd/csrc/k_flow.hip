@@ -87,26 +87,29 @@ struct Tap {
     float a;
 };
 
+// Stencil tap k (-2..2) of column (or row) ix of an n-texel axis: the fetch at (ix + 0.5) / n + k / n.
+__device__ __forceinline__ Tap stencil_tap(int ix, int n, int k, int quant) {
+    float d = 1.0f / (float)n;
+    float x = ((float)ix + 0.5f) * d;
+    float u = k == 0 ? x : (k > 0 ? x + (float)k * d : x - (float)(-k) * d);
+    float xf = u * (float)n - 0.5f;
+    float fl = floorf(xf);
+    Tap t;
+    t.a = tex_frac(xf - fl, quant);
+    t.i0 = tex_addr((int)fl, n, ISA_ADDR_MIRROR);
+    t.i1 = tex_addr((int)fl + 1, n, ISA_ADDR_MIRROR);
+    return t;
+}
+
 __global__ void k_tap_tables(Tap* col, Tap* row, int w, int h, int quant) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool is_col = i < w;
     const int n = is_col ? w : h;
     const int ix = is_col ? i : i - w;
     if (ix >= n || i >= w + h) return;
-    float d = 1.0f / (float)n;
-    float x = ((float)ix + 0.5f) * d;
     Tap* out = (is_col ? col : row) + (size_t)ix * 5;
 #pragma unroll
-    for (int k = -2; k <= 2; ++k) {
-        float u = k == 0 ? x : (k > 0 ? x + (float)k * d : x - (float)(-k) * d);
-        float xf = u * (float)n - 0.5f;
-        float fl = floorf(xf);
-        Tap t;
-        t.a = tex_frac(xf - fl, quant);
-        t.i0 = tex_addr((int)fl, n, ISA_ADDR_MIRROR);
-        t.i1 = tex_addr((int)fl + 1, n, ISA_ADDR_MIRROR);
-        out[k + 2] = t;
-    }
+    for (int k = -2; k <= 2; ++k) out[k + 2] = stencil_tap(ix, n, k, quant);
 }
 
 // ComputeDerivativesKernel (R:Kernels/opticalFlow.cu:96-147): 5-point stencils of both images along x
@@ -142,10 +145,12 @@ __device__ __forceinline__ float lds_fetch(const float (*L)[DV_LW], const LTap& 
     return lds_fetch_w<DV_LW>(L, c, r);
 }
 
+// TABLE: taps read from the tables of k_tap_tables; otherwise evaluated in place (the same floats).
+template <bool TABLE>
 __global__ __launch_bounds__(DV_BX* DV_BY) void k_derivatives_tab(const float* src, const float* tgt, int img_pitch,
                                                                    float* Ix, float* Iy, float* Iz, int pitch, int w,
                                                                    int h, const Tap* __restrict__ col,
-                                                                   const Tap* __restrict__ row) {
+                                                                   const Tap* __restrict__ row, int quant) {
     __shared__ float Ls[DV_LH][DV_LW], Lt[DV_LH][DV_LW];
     const int x0 = blockIdx.x * DV_BX - DV_H, y0 = blockIdx.y * DV_BY - DV_H;
     const int tid = threadIdx.y * DV_BX + threadIdx.x;
@@ -161,7 +166,8 @@ __global__ __launch_bounds__(DV_BX* DV_BY) void k_derivatives_tab(const float* s
     LTap c[5], r[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        Tap tc = col[(size_t)ix * 5 + k], tr = row[(size_t)iy * 5 + k];
+        Tap tc = TABLE ? col[(size_t)ix * 5 + k] : stencil_tap(ix, w, k - 2, quant);
+        Tap tr = TABLE ? row[(size_t)iy * 5 + k] : stencil_tap(iy, h, k - 2, quant);
         c[k] = {clampi(tc.i0 - x0, 0, DV_LW - 1), clampi(tc.i1 - x0, 0, DV_LW - 1), tc.a, 0};
         r[k] = {clampi(tr.i0 - y0, 0, DV_LH - 1), clampi(tr.i1 - y0, 0, DV_LH - 1), tr.a, 0};
     }
@@ -733,11 +739,17 @@ int launch_warp(const float* src, int src_pitch, float* out, int out_pitch, cons
     return check_launch("warp");
 }
 
+// taps: tables from build_tap_tables for (w, h, quant), or null to evaluate the taps in place.
 int launch_derivatives(const float* src, const float* tgt, int img_pitch, float* ix, float* iy, float* iz, int pitch,
-                       int w, int h, const void* taps, hipStream_t s) {
+                       int w, int h, int quant, const void* taps, hipStream_t s) {
     const Tap* col = (const Tap*)taps;
-    k_derivatives_tab<<<dim3(div_up(w, DV_BX), div_up(h, DV_BY)), dim3(DV_BX, DV_BY), 0, s>>>(
-        src, tgt, img_pitch, ix, iy, iz, pitch, w, h, col, col + (size_t)w * 5);
+    const dim3 grid(div_up(w, DV_BX), div_up(h, DV_BY)), block(DV_BX, DV_BY);
+    if (col)
+        k_derivatives_tab<true><<<grid, block, 0, s>>>(src, tgt, img_pitch, ix, iy, iz, pitch, w, h, col,
+                                                       col + (size_t)w * 5, quant);
+    else
+        k_derivatives_tab<false><<<grid, block, 0, s>>>(src, tgt, img_pitch, ix, iy, iz, pitch, w, h, nullptr, nullptr,
+                                                        quant);
     return check_launch("derivatives");
 }
 
@@ -811,25 +823,11 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
     ISA_TRY(launch_flow_from_tiles(flow, flow_pitch, tiles, tiles_pitch, tw, th, w, h, bsx, bsy, rot, quant, s));
     for (int it = 0; it < iterations; ++it) {
         ISA_TRY(launch_warp(src, img_pitch, tmp, wp, flow, flow_pitch, w, h, quant, s));
-        ISA_TRY(launch_derivatives(tmp, tgt, wp, ix, iy, iz, wp, w, h, taps, s));
+        ISA_TRY(launch_derivatives(tmp, tgt, wp, ix, iy, iz, wp, w, h, quant, taps, s));
         ISA_TRY(launch_lk_step(flow, flow_pitch, ix, iy, iz, wp, w, h, window, min_det, box, s));
     }
     return ISA_OK;
 }
-
-// Stream-ordered tap tables for a one-off call of the public entry points (the merger keeps its own).
-struct ScopedTaps {
-    void* p = nullptr;
-    hipStream_t s;
-    int rc = ISA_OK;
-    ScopedTaps(int w, int h, int quant, hipStream_t st) : s(st) {
-        rc = check_hip(hipMallocAsync(&p, tap_table_bytes(w, h), s), "tap tables");
-        if (rc == ISA_OK) rc = build_tap_tables(p, w, h, quant, s);
-    }
-    ~ScopedTaps() {
-        if (p) (void)hipFreeAsync(p, s);
-    }
-};
 
 }  // namespace isa
 
@@ -855,10 +853,8 @@ int isa_compute_derivatives(const float* src_dev, const float* tgt_dev, int img_
                             float* iz_dev, int pitch, int width, int height, int tex_quant, isa_stream_t stream) {
     ISA_REQUIRE(src_dev && tgt_dev && ix_dev && iy_dev && iz_dev && width > 0 && height > 0,
                 "isa_compute_derivatives: bad arguments");
-    isa::ScopedTaps taps(width, height, tex_quant, isa::to_stream(stream));
-    ISA_TRY(taps.rc);
-    return isa::launch_derivatives(src_dev, tgt_dev, img_pitch, ix_dev, iy_dev, iz_dev, pitch, width, height, taps.p,
-                                   isa::to_stream(stream));
+    return isa::launch_derivatives(src_dev, tgt_dev, img_pitch, ix_dev, iy_dev, iz_dev, pitch, width, height,
+                                   tex_quant, nullptr, isa::to_stream(stream));
 }
 
 int isa_compute_derivatives2(const float* img_dev, int img_pitch, float* ix_dev, float* iy_dev, int pitch, int width,
@@ -885,11 +881,9 @@ int isa_lucas_kanade(float* src_dev, const float* tgt_dev, int img_pitch, const 
                     iterations >= 0 && window_size >= 1,
                 "isa_lucas_kanade: bad arguments");
     hipStream_t s = isa::to_stream(stream);
-    isa::ScopedTaps taps(width, height, tex_quant, s);
-    ISA_TRY(taps.rc);
     ISA_TRY(isa::lucas_kanade(src_dev, tgt_dev, img_pitch, tiles2_dev, tiles_pitch, tex_w, tex_h, flow2_dev, flow_pitch,
                               work_dev, width, height, iterations, base_shift[0], base_shift[1], base_rotation, min_det,
-                              window_size, tex_quant, false, s, taps.p));
+                              window_size, tex_quant, false, s, nullptr));
     // final warp, copied over the source image (R:PEFStudioDX/OpticalFlow.cs:162-163)
     ISA_TRY(isa::launch_warp(src_dev, img_pitch, work_dev, width * (int)sizeof(float), flow2_dev, flow_pitch, width,
                              height, tex_quant, s));

@@ -23,6 +23,28 @@ def _stream_ptr():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def check_tensor(name, t, shape, dtypes):
+    """Raises ValueError unless t is a contiguous GPU tensor of the given shape and one of the dtypes: the
+    library reads and writes through raw pointers, so a wrong shape, a view or a host tensor would make a
+    kernel access memory outside the tensor."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: must be a GPU tensor")
+    if t.dtype not in dtypes:
+        raise ValueError(f"{name}: dtype {t.dtype} not in {tuple(str(d) for d in dtypes)}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+
+
+def raw_dtypes():
+    torch = _torch()
+    return tuple(d for d in (torch.int16, getattr(torch, "uint16", None)) if d is not None)
+
+
 class BurstMerger:
     """Mirror of the controller's pipeline (TrackPatches, PrepareAccumulation,
     Accumulate) on one GPU; all buffers are allocated once at construction."""
@@ -48,8 +70,14 @@ class BurstMerger:
         """raw: uint16-valued torch tensor (frames, H, W) on the GPU (torch.int16/uint16 storage).
         Returns the merged (H, W, 3) float32 image."""
         torch = _torch()
+        check_tensor("raw", raw, (self.frames, self.height, self.width), raw_dtypes())
         if out is None:
             out = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=raw.device)
+        check_tensor("out", out, (self.height, self.width, 3), (torch.float32,))
+        if base_shift is not None and len(base_shift) != 2 * self.frames:
+            raise ValueError(f"base_shift: expected {2 * self.frames} values (x, y per frame)")
+        if base_rotation is not None and len(base_rotation) != self.frames:
+            raise ValueError(f"base_rotation: expected {self.frames} values")
         bs = None if base_shift is None else (C.c_float * (2 * self.frames))(*[float(v) for v in base_shift])
         br = None if base_rotation is None else (C.c_float * self.frames)(*[float(v) for v in base_rotation])
         call("isa_merger_run", self._h, C.c_void_p(raw.data_ptr()), bs, br, C.c_void_p(out.data_ptr()),

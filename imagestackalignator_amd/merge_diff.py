@@ -12,7 +12,7 @@ from __future__ import annotations
 import ctypes as C
 
 from . import call, lib
-from .merge import _stream_ptr, _torch
+from .merge import _stream_ptr, _torch, check_tensor, raw_dtypes
 
 
 class IsaMergeDiff(C.Structure):
@@ -40,7 +40,12 @@ class IsaMergeDiff(C.Structure):
 def half_res(raw, params):
     """isa_debayer_subsample3 of every frame: (frames, H/2, W/2, 3) float32 (R:Kernels/DeBayerKernels.cu:242-283)."""
     torch = _torch()
+    if raw.dim() != 3:
+        raise ValueError("raw: expected (frames, H, W)")
     n, h, w = raw.shape
+    check_tensor("raw", raw, (n, h, w), raw_dtypes())
+    if h % 2 or w % 2:
+        raise ValueError("raw: width and height must be even")
     out = torch.empty((n, h // 2, w // 2, 3), dtype=torch.float32, device=raw.device)
     maxv = max(params.white[c] + params.black[c] for c in range(3))
     for f in range(n):
@@ -53,7 +58,10 @@ def smoothed_tensor(image, params):
     """Structure tensor of a float image (H, W), Gaussian-smoothed with sigma_structure_tensor, as the
     merge's PrepareAccumulation computes it before ComputeKernelParam: (H, W, 3) = (a11, a22, a12)."""
     torch = _torch()
+    if image.dim() != 2:
+        raise ValueError("image: expected (H, W)")
     h, w = image.shape
+    check_tensor("image", image, (h, w), (torch.float32,))
     dx = torch.empty((h, w), dtype=torch.float32, device=image.device)
     dy = torch.empty_like(dx)
     t = torch.empty((h, w, 3), dtype=torch.float32, device=image.device)
@@ -79,11 +87,13 @@ class MergeDiff:
 
     def __init__(self, raw, half, tensor, anchor, reference: int, params):
         torch = _torch()
+        if raw.dim() != 3:
+            raise ValueError("raw: expected (frames, H, W)")
         n, h, w = raw.shape
-        assert half.shape == (n, h // 2, w // 2, 3) and tensor.shape == (h, w, 3) and anchor.shape == (n, h, w, 2)
-        for t in (raw, half, tensor, anchor):
-            assert t.is_contiguous() and t.is_cuda
-        assert half.dtype == tensor.dtype == anchor.dtype == torch.float32
+        check_tensor("raw", raw, (n, h, w), raw_dtypes())
+        check_tensor("half", half, (n, h // 2, w // 2, 3), (torch.float32,))
+        check_tensor("tensor", tensor, (h, w, 3), (torch.float32,))
+        check_tensor("anchor", anchor, (n, h, w, 2), (torch.float32,))
         self.raw, self.half, self.tensor, self.anchor = raw, half, tensor, anchor
         self.width, self.height, self.frames = w, h, n
         d = IsaMergeDiff()
@@ -120,17 +130,22 @@ class MergeDiff:
     def theta_of(self, params):
         return [params.k_detail, params.k_denoise, params.dth, params.dtr]
 
-    def _flow(self, flow):
-        assert flow.shape == (self.frames, self.height, self.width, 2) and flow.is_contiguous()
-        assert flow.dtype == _torch().float32 and flow.is_cuda
+    def _flow(self, flow, name="flow"):
+        check_tensor(name, flow, (self.frames, self.height, self.width, 2), (_torch().float32,))
+
+    def _gt(self, gt):
+        if gt is not None:
+            check_tensor("gt", gt, (2 * self.height, 2 * self.width, 3), (_torch().float32,))
 
     def forward(self, flow, theta, gt=None, out=None):
         """Merged (2H, 2W, 3) image and, with a target gt of that shape, the loss (sum of squared
         differences over the interior, 4-pixel border excluded)."""
         torch = _torch()
         self._flow(flow)
+        self._gt(gt)
         if out is None:
             out = torch.empty((2 * self.height, 2 * self.width, 3), dtype=torch.float32, device=flow.device)
+        check_tensor("out", out, (2 * self.height, 2 * self.width, 3), (torch.float32,))
         loss = C.c_double(0.0)
         th = (C.c_float * 4)(*[float(v) for v in theta])
         call("isa_merge_forward", C.byref(self.problem), C.c_void_p(flow.data_ptr()), th, C.c_void_p(out.data_ptr()),
@@ -142,8 +157,12 @@ class MergeDiff:
         """d loss / d flow (frames, H, W, 2) and d loss / d theta (4 floats)."""
         torch = _torch()
         self._flow(flow)
+        if gt is None:
+            raise ValueError("gt: the loss needs a target")
+        self._gt(gt)
         if dflow is None:
             dflow = torch.empty_like(flow)
+        self._flow(dflow, "dflow")
         th = (C.c_float * 4)(*[float(v) for v in theta])
         dth = (C.c_float * 4)()
         call("isa_merge_backward", C.byref(self.problem), C.c_void_p(flow.data_ptr()), th, C.c_void_p(gt.data_ptr()),

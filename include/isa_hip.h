@@ -9,9 +9,18 @@
  * controller's pipeline methods) for callers that want the whole path.
  *
  * Conventions
- *  - Every pointer argument named *_dev is device memory owned by the caller;
- *    no per-kernel entry point allocates. Pitches are in BYTES. float2/3/4
- *    images are interleaved (x, y[, z[, w]]) per pixel.
+ *  - Every pointer argument named *_dev is device memory owned by the caller.
+ *    No per-kernel entry point allocates device memory or waits for the
+ *    device: scratch comes in a caller-owned workspace (with a *_workspace()
+ *    size query) and host tables travel in kernel arguments. The exceptions
+ *    are the calls that return values in host memory, which synchronise the
+ *    stream: isa_merge_forward (loss), isa_merge_backward (dtheta),
+ *    isa_merge_refine_step, isa_merger_stage_ms / _level_ms / _exact_items,
+ *    isa_synth_burst and isa_synth_ground_truth (which also stage their scene
+ *    tables in memory of their own). isa_merger_create allocates every buffer
+ *    of the whole-path merger once, as the reference's controller did.
+ *    Pitches are in BYTES. float2/3/4 images are interleaved (x, y[, z[, w]])
+ *    per pixel.
  *  - `cfa` replaces the reference's module constant c_cfaPattern: four ints,
  *    row-major 2x2, values 0 = red, 1 = green, 2 = blue
  *    (R:Kernels/DeBayerKernels.cu:28-41; set via the BayerPattern setters,
@@ -213,42 +222,55 @@ int isa_add_shifts(float* preshift2_dev, int preshift_pitch, const float* coords
 int isa_upsample_shifts(const float* in2_dev, float* out2_dev, int in_pitch, int out_pitch, int old_level,
                         int new_level, int old_count_x, int old_count_y, int new_count_x, int new_count_y,
                         int old_tile_size, int new_tile_size, isa_stream_t stream);
+/* Device workspace bytes of isa_track, isa_track_xc and isa_track_xc_ex for one level image size, pitch, tile size,
+ * maximum shift and precision (0 on invalid input). The workspace is the caller's, 256-byte aligned, and may be reused
+ * by the next call on the same stream. Replaces the buffers PatchTracker allocates in InitForSize / AllocateDeviceMemory
+ * (R:PEFStudioDX/PatchTracker.cs:211-253). */
+size_t isa_track_workspace(int width, int height, int img_pitch, int tile_size, int max_shift, int xc_precision);
 /* PatchTracker.Track for ONE pair (R:PEFStudioDX/PatchTracker.cs:317-360): tile extraction of both images, SSD map,
- * findMinimum and preshift += result, fused; the images are the level images (width x height). */
+ * findMinimum and preshift += result, fused; the images are the level images (width x height). Stream-ordered: no
+ * allocation, no host synchronisation. workspace_dev: isa_track_workspace(..., ISA_XC_FP64) bytes. */
 int isa_track(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
               int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
               float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
-              isa_stream_t stream);
+              void* workspace_dev, isa_stream_t stream);
 /* isa_track with the cross terms in the given isa_xc_precision (ISA_XC_FP64 is isa_track itself). No counterpart in
- * the reference: the entry point of the mixed-precision alignment for one pair. */
+ * the reference: the entry point of the mixed-precision alignment for one pair. workspace_dev:
+ * isa_track_workspace(..., xc_precision) bytes. */
 int isa_track_xc(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
                  int preshift_pitch, int width, int height, int tile_size, int max_shift, const float base_shift_ref[2],
                  float base_rotation_ref, const float base_shift_track[2], float base_rotation_track, float threshold,
-                 int xc_precision, isa_stream_t stream);
+                 int xc_precision, void* workspace_dev, isa_stream_t stream);
 /* isa_track_xc with two outputs for checking the mixed-precision search. No counterpart in the reference.
  * decisions_dev (device, tiles ints, may be NULL): per tile the integer part of findMinimum, 2 * argmin + 1 when the
  * threshold test zeroes the shift, -1 without a minimum. The matrix-core forms bound the error of every lag's value;
  * where the bounds cannot decide the argmin and the threshold test as the exact search would, the tile is searched
- * exactly, so the decisions equal isa_track's. exact_items (host, may be NULL): tiles searched exactly. */
+ * exactly, so the decisions equal isa_track's. exact_count_dev (device, one int, may be NULL): tiles searched
+ * exactly. */
 int isa_track_xc_ex(const float* img_track_dev, const float* img_ref_dev, int img_pitch, float* preshift2_dev,
                     int preshift_pitch, int width, int height, int tile_size, int max_shift,
                     const float base_shift_ref[2], float base_rotation_ref, const float base_shift_track[2],
                     float base_rotation_track, float threshold, int xc_precision, int* decisions_dev,
-                    int* exact_items, isa_stream_t stream);
+                    int* exact_count_dev, void* workspace_dev, isa_stream_t stream);
 
 /* ShiftCollection tables (R:PEFStudioDX/ShiftCollection.cs:113-496). pairs_out: 2*m ints (reference, toTrack) in
  * GetShiftPairs order; index_out: n*n ints (-1 where unmeasured); rows_out: 2*m ints (a, b): measurement row k
  * measures x_b - x_a with x_j = sum_{i<j} r_i. Any output may be NULL. Returns m (>0) or -ISA_ERR_ARG. */
 int isa_shift_tables(int frame_count, int reference_index, int strategy, int block_size, int* pairs_out,
                      int* index_out, int* rows_out);
+/* Device workspace bytes of isa_minimize_shifts for a frame count and tracking strategy (0 on invalid input): the
+ * measurement rows and the all-rows factor, which ShiftCollection's constructor keeps on the device
+ * (R:PEFStudioDX/ShiftCollection.cs:498-586). */
+size_t isa_minimize_shifts_workspace(int frame_count, int reference_index, int strategy, int block_size);
 /* ShiftCollection.MinimizeCUBLAS (R:PEFStudioDX/ShiftCollection.cs:738-822) in FP64, on the device, all tiles at once
  * (no host round trip per iteration). measured2_dev: [m][tile_count] float2 (measurement-major, pitch
  * meas_pitch bytes per measurement) -- read, then overwritten with A_safe r as the reference's separateShifts does;
  * one_to_one2_dev: [tile][n1] float2 (shiftsOneToOne_d); status_dev (optional): [tile] final status, iters_dev
- * (optional): iterations used. */
+ * (optional): iterations used. workspace_dev: isa_minimize_shifts_workspace() bytes, 16-byte aligned; its tables are
+ * written in stream order on each call (no allocation, no host synchronisation). */
 int isa_minimize_shifts(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev, int* iters_dev,
                         int tile_count, int frame_count, int reference_index, int strategy, int block_size,
-                        isa_stream_t stream);
+                        void* workspace_dev, isa_stream_t stream);
 /* getOptimalShifts, R:Kernels/ShiftMinimizerKernels.cu:178-218; wrapper R:PEFStudioDX/ShiftMinimizerKernels.cs:69-96. */
 int isa_get_optimal_shifts(float* out2_dev, int out_pitch, const float* one_to_one2_dev, int frame_count,
                            int tile_count_x, int tile_count_y, int reference_image, int image_to_track,
@@ -275,7 +297,8 @@ int isa_lucas_kanade_optim(float* flow2_dev, int flow_pitch, const float* ix_dev
                            const float* it_dev, int img_pitch, int width, int height, int window_size, float min_det,
                            isa_stream_t stream);
 /* OpticalFlow.LucasKanade (R:PEFStudioDX/OpticalFlow.cs:147-164): flow from tiles, `iterations` x {warp, derivatives,
- * LK step}, final warp written over src. work_dev: 4*width*height floats (tmp, Ix, Iy, Iz). */
+ * LK step}, final warp written over src. work_dev: 4*width*height floats (tmp, Ix, Iy, Iz), the buffers OpticalFlow's
+ * constructor allocates (R:PEFStudioDX/OpticalFlow.cs:115-145). No allocation, no host synchronisation. */
 int isa_lucas_kanade(float* src_dev, const float* tgt_dev, int img_pitch, const float* tiles2_dev, int tiles_pitch,
                      int tex_w, int tex_h, float* flow2_dev, int flow_pitch, float* work_dev, int width, int height,
                      int iterations, const float base_shift[2], float base_rotation, float min_det, int window_size,

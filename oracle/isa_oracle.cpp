@@ -1077,29 +1077,52 @@ template <class T> static inline T kernel_weight(int px, int py, T k0, T k1, T k
     return w;
 }
 
-// The 1x gather visits the 25 taps class by class, class = ((py & 1) << 1) | (px & 1) = 0..3, in
-// raster order within a class (the reference visits them in raster order; the two orders differ
-// only in how the taps of the two green classes interleave in the green sums).
+// One 1x tap of accumulateImages (R:Kernels/DeBayerKernels.cu:314-372): the sample at the clamped,
+// shifted site, its CFA colour, (raw - black) / white, the certainty at the clamped unshifted
+// half-resolution position (Q15), and the two running sums of that colour.
+template <class T>
+static inline void accumulate_tap(const uint16_t* raw, const T* mask4, int mw, int w, int h, int x, int y, int px,
+                                  int py, int sx, int sy, T k0, T k1, T k2, const T white[3], const T black[3],
+                                  const int cfa[4], T acc[3], T wt[3]) {
+    int ppsx = std::min(std::max(x + px + sx, 0), w - 1), ppsy = std::min(std::max(y + py + sy, 0), h - 1);
+    int ppx = std::min(std::max(x + px, 0), w - 1), ppy = std::min(std::max(y + py, 0), h - 1);
+    int site = cfa_at(cfa, ppsy, ppsx);
+    T wgt = kernel_weight(px, py, k0, k1, k2);
+    T v = (T)raw[(size_t)ppsy * w + ppsx];
+    int c = site == CFA_GREEN ? 1 : site == CFA_RED ? 0 : site == CFA_BLUE ? 2 : -1;
+    if (c < 0) return;
+    v = (v - black[c]) / white[c];
+    T cert = mask4[((size_t)(ppy / 2) * mw + (ppx / 2)) * 4 + c];
+    if (!std::isfinite(cert)) cert = 0;
+    acc[c] += v * wgt * cert;
+    wt[c] += wgt * cert;
+}
+
+// Tap orders of the 1x gather. Raster: py = -2..2, px = -2..2, the reference's order. By class:
+// class = ((py & 1) << 1) | (px & 1) = 0..3, raster order within a class -- the order of the HIP
+// kernels, which keep one colour per class away from the image edge. The two orders add the same
+// terms to each colour's sums; they differ in how the two green classes interleave and, where
+// clamping maps taps of several classes to one colour, at the image edge.
 struct TapOrder {
     int pxy[50];
-    TapOrder() {
+    explicit TapOrder(bool by_class) {
         int n = 0;
-        for (int cls = 0; cls < 4; ++cls)
+        for (int cls = 0; cls < (by_class ? 4 : 1); ++cls)
             for (int py = -2; py <= 2; ++py)
                 for (int px = -2; px <= 2; ++px)
-                    if ((((py & 1) << 1) | (px & 1)) == cls) { pxy[2 * n] = px; pxy[2 * n + 1] = py; ++n; }
+                    if (!by_class || (((py & 1) << 1) | (px & 1)) == cls) { pxy[2 * n] = px; pxy[2 * n + 1] = py; ++n; }
     }
 };
-static const int* tap_order_by_class() {
-    static const TapOrder order;  // thread-safe initialisation
-    return order.pxy;
+static const int* tap_order(bool by_class) {
+    static const TapOrder raster(false), classes(true);  // thread-safe initialisation
+    return by_class ? classes.pxy : raster.pxy;
 }
 
 template <class T>
-void accumulate(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern3, const T* flow, const T white[3],
-                const T black[3], int w, int h, const int cfa[4]) {
+static void accumulate_ordered(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern3, const T* flow,
+                               const T white[3], const T black[3], int w, int h, const int cfa[4], bool by_class) {
     int mw = w / 2;
-    const int* order = tap_order_by_class();
+    const int* order = tap_order(by_class);
     parallel_for(h, [&](int y0, int y1) {
         for (int y = std::max(y0, 1); y < std::min(y1, h - 1); ++y)
             for (int x = 1; x < w - 1; ++x) {
@@ -1108,24 +1131,26 @@ void accumulate(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T*
                 T wt[3] = {wsum3[o * 3], wsum3[o * 3 + 1], wsum3[o * 3 + 2]};
                 T k0 = kern3[o * 3], k1 = kern3[o * 3 + 1], k2 = kern3[o * 3 + 2];
                 int sx = (int)std::round(flow[o * 2]), sy = (int)std::round(flow[o * 2 + 1]);
-                for (int t = 0; t < 25; ++t) {
-                        const int px = order[2 * t], py = order[2 * t + 1];
-                        int ppsx = std::min(std::max(x + px + sx, 0), w - 1), ppsy = std::min(std::max(y + py + sy, 0), h - 1);
-                        int ppx = std::min(std::max(x + px, 0), w - 1), ppy = std::min(std::max(y + py, 0), h - 1);
-                        int site = cfa_at(cfa, ppsy, ppsx);
-                        T wgt = kernel_weight(px, py, k0, k1, k2);
-                        T v = (T)raw[(size_t)ppsy * w + ppsx];
-                        int c = site == CFA_GREEN ? 1 : site == CFA_RED ? 0 : site == CFA_BLUE ? 2 : -1;
-                        if (c < 0) continue;
-                        v = (v - black[c]) / white[c];
-                        T cert = mask4[((size_t)(ppy / 2) * mw + (ppx / 2)) * 4 + c];
-                        if (!std::isfinite(cert)) cert = 0;
-                        acc[c] += v * wgt * cert;
-                        wt[c] += wgt * cert;
-                }
+                for (int t = 0; t < 25; ++t)
+                    accumulate_tap(raw, mask4, mw, w, h, x, y, order[2 * t], order[2 * t + 1], sx, sy, k0, k1, k2, white,
+                                   black, cfa, acc, wt);
                 for (int c = 0; c < 3; ++c) { sum3[o * 3 + c] = acc[c]; wsum3[o * 3 + c] = wt[c]; }
             }
     });
+}
+
+// accumulateImages, the reference's raster tap order (R:Kernels/DeBayerKernels.cu:314-372).
+template <class T>
+void accumulate(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern3, const T* flow, const T white[3],
+                const T black[3], int w, int h, const int cfa[4]) {
+    accumulate_ordered(raw, sum3, wsum3, mask4, kern3, flow, white, black, w, h, cfa, false);
+}
+
+// The same sums with the taps visited class by class (the HIP kernels' order).
+template <class T>
+void accumulate_class_order(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern3, const T* flow,
+                            const T white[3], const T black[3], int w, int h, const int cfa[4]) {
+    accumulate_ordered(raw, sum3, wsum3, mask4, kern3, flow, white, black, w, h, cfa, true);
 }
 
 // kern4, flow: full-resolution W x H textures (clamp / mirror, linear).

@@ -71,6 +71,7 @@ def _declare(L):
         "robustness_mask": (None, [P, P, P, P, I, I, I, I, "T", "T", "T", I]),
         "erode": (None, [P, P, I, I, I]),
         "accumulate": (None, [P, P, P, P, P, P, P, P, I, I, P]),
+        "accumulate_class_order": (None, [P, P, P, P, P, P, P, P, I, I, P]),
         "accumulate_sr": (None, [P, P, P, P, P, P, P, P, I, I, P, I]),
         "apply_weighting": (None, [P, P, P, I, I, "T"]),
         "merge": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P]),

@@ -51,3 +51,26 @@ def ulp_diff(a: np.ndarray, b: np.ndarray) -> int:
     a = np.where(a < 0, np.int64(-2**31) - a, a)
     b = np.where(b < 0, np.int64(-2**31) - b, b)
     return int(np.abs(a - b).max()) if a.size else 0
+
+
+def within_reorder_bound(s_got, w_got, s_ref, w_ref, s0, w0, additions, vmax=1.0):
+    """Whether two accumulations of the same terms in different orders (value sums s, weight sums w,
+    started from s0, w0) are within the reordering bound of a floating-point sum: 2 gamma_k times
+    |s0| + sum |terms|, k additions, gamma_k = k u / (1 - k u), u = 2^-24. Weight terms are
+    non-negative; value terms are at most vmax (the largest |normalised raw|) times their weight term."""
+    u = 2.0 ** -24
+    g = additions * u / (1 - additions * u)
+    wref = np.asarray(w_ref, np.float64)
+    bound_w = 2 * g * np.abs(wref)
+    bound_s = 2 * g * (np.abs(np.asarray(s0, np.float64)) + vmax * np.abs(wref - np.asarray(w0, np.float64)))
+    return bool(np.all(np.abs(np.asarray(w_got, np.float64) - wref) <= bound_w) and
+                np.all(np.abs(np.asarray(s_got, np.float64) - np.asarray(s_ref, np.float64)) <= bound_s))
+
+
+def workspace(nbytes: int) -> torch.Tensor:
+    """A device workspace of nbytes (256-byte aligned, as the caching allocator returns)."""
+    assert nbytes > 0, "workspace query returned 0 (invalid arguments)"
+    t = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
+    assert t.data_ptr() % 256 == 0
+    KEEP.append(t)
+    return t

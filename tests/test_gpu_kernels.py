@@ -12,7 +12,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-from tests.gpu_util import dev, empty, host, p, stream, ulp_diff  # noqa: E402
+from tests.gpu_util import dev, empty, host, p, stream, ulp_diff, within_reorder_bound, workspace  # noqa: E402
 
 RGGB = [0, 1, 1, 2]
 GRBG = [1, 0, 2, 1]
@@ -224,8 +224,9 @@ def test_tiles_ssd_findmin(isa, oracle, gpu, tile, M, bs, rot, h, w):
     np.testing.assert_array_equal(host(coords).reshape(-1, 2), oracle.find_minimum(s_ref, M, 1.0))
     # fused PatchTracker.Track
     d_pre = dev(pre)
+    ws = workspace(isa.lib().isa_track_workspace(w, h, w * 4, tile, M, 0))
     isa.call("isa_track", p(dev(b)), p(dev(a)), w * 4, p(d_pre), cx * 8, w, h, tile, M, isa.f32s(*bs),
-             C.c_float(rot), isa.f32s(*bs), C.c_float(rot), C.c_float(1.0), stream())
+             C.c_float(rot), isa.f32s(*bs), C.c_float(rot), C.c_float(1.0), p(ws), stream())
     np.testing.assert_array_equal(host(d_pre).reshape(-1, 2), pre.reshape(-1, 2) + oracle.find_minimum(s_ref, M, 1.0))
 
 
@@ -249,8 +250,9 @@ def test_track_ties_take_the_first_minimum(isa, oracle, gpu, tile, M):
     mins = s_ref.reshape(len(s_ref), -1)
     assert np.mean((mins == mins.min(axis=1, keepdims=True)).sum(axis=1) > 1) > 0.5  # ties are common
     d_pre = dev(pre)
+    ws = workspace(isa.lib().isa_track_workspace(w, h, w * 4, tile, M, 0))
     isa.call("isa_track", p(dev(b)), p(dev(a)), w * 4, p(d_pre), cx * 8, w, h, tile, M, isa.f32s(*bs),
-             C.c_float(rot), isa.f32s(*bs), C.c_float(rot), C.c_float(1.0), stream())
+             C.c_float(rot), isa.f32s(*bs), C.c_float(rot), C.c_float(1.0), p(ws), stream())
     np.testing.assert_array_equal(host(d_pre).reshape(-1, 2), pre.reshape(-1, 2) + oracle.find_minimum(s_ref, M, 1.0))
 
 
@@ -288,7 +290,9 @@ def test_minimize_shifts(isa, oracle, gpu, n, strategy, block):
     r = empty((tiles, n - 1, 2))
     st = empty((tiles,), dtype=__import__("torch").int32)
     it = empty((tiles,), dtype=__import__("torch").int32)
-    isa.call("isa_minimize_shifts", p(meas), tiles * 8, p(r), p(st), p(it), tiles, n, ref, strategy, block, stream())
+    ws = workspace(isa.lib().isa_minimize_shifts_workspace(n, ref, strategy, block))
+    isa.call("isa_minimize_shifts", p(meas), tiles * 8, p(r), p(st), p(it), tiles, n, ref, strategy, block, p(ws),
+             stream())
     np.testing.assert_array_equal(host(st), st_ref)
     np.testing.assert_array_equal(host(it), it_ref)
     np.testing.assert_allclose(host(r), r_ref, atol=2e-5)
@@ -349,6 +353,41 @@ def test_lucas_kanade_step(isa, oracle, gpu):
     ref = flow0.copy()
     oracle.fn("lucas_kanade_step")(oracle.ptr(ref), oracle.ptr(rx), oracle.ptr(ry), oracle.ptr(rz), w, h, 5, 0.01)
     np.testing.assert_array_equal(host(f), ref)
+
+
+@pytest.mark.parametrize("quant,iters,window,rot", [(1, 3, 11, 0.0), (0, 2, 7, 0.004), (1, 0, 11, 0.0)])
+def test_lucas_kanade_drop_in(isa, oracle, gpu, quant, iters, window, rot):
+    """OpticalFlow.LucasKanade (R:PEFStudioDX/OpticalFlow.cs:147-164) as one call: flow from the tile
+    shifts, `iters` x {warp, derivatives, LK step} with the reference's per-pixel window loops, then the
+    final warp written over the source. Bit for bit against the oracle's lucas_kanade."""
+    rng = np.random.default_rng(40 + iters + window)
+    h, w = 56, 72
+    src, tgt = _track_inputs(rng, h, w)
+    tw, th = 5, 4
+    tiles = rng.uniform(-1.5, 1.5, (th, tw, 2)).astype(np.float32)
+    bs = (0.75, -0.5)
+    print(f"lucas_kanade quant={quant} iters={iters} window={window}", flush=True)
+    d_src, flow, work = dev(src), empty((h, w, 2)), empty((4, h, w))
+    isa.call("isa_lucas_kanade", p(d_src), p(dev(tgt)), w * 4, p(dev(tiles)), tw * 8, tw, th, p(flow), w * 8, p(work),
+             w, h, iters, isa.f32s(*bs), C.c_float(rot), C.c_float(0.01), window, quant, stream())
+    s_ref, f_ref = src.copy(), np.zeros((h, w, 2), np.float32)
+    oracle.fn("lucas_kanade")(oracle.ptr(s_ref), oracle.ptr(tgt), oracle.ptr(tiles), tw, th, oracle.ptr(f_ref), w, h,
+                              iters, bs[0], bs[1], rot, 0.01, window, quant)
+    np.testing.assert_array_equal(host(flow), f_ref)
+    np.testing.assert_array_equal(host(d_src), s_ref)
+
+
+def test_add_shifts(isa, gpu):
+    """NPP Add of the float view of the shift images (R:PEFStudioDX/PatchTracker.cs:355-358), pitched."""
+    rng = np.random.default_rng(44)
+    cx, cy = 13, 7
+    pre = rng.uniform(-5, 5, (cy, cx + 3, 2)).astype(np.float32)  # 3 columns of row padding
+    co = rng.uniform(-1, 1, (cy, cx + 1, 2)).astype(np.float32)
+    d_pre = dev(pre)
+    isa.call("isa_add_shifts", p(d_pre), (cx + 3) * 8, p(dev(co)), (cx + 1) * 8, cx, cy, stream())
+    want = pre.copy()
+    want[:, :cx] += co[:, :cx]
+    np.testing.assert_array_equal(host(d_pre), want)
 
 
 def test_structure_tensor_kernel_param(isa, oracle, gpu):
@@ -467,15 +506,22 @@ def test_accumulate_frames(isa, oracle, gpu, cfa, layout, h, w):
     ds, dw = dev(s0), dev(w0)
     isa.call("isa_accumulate_frames", ptrs(d_raw), ptrs(d_mask), ptrs(d_flow), n, p(ds), p(dw), p(dev(kern)),
              isa.f32s(*white), isa.f32s(*black), w, h, w * 12, hw * 16, w * 8, isa.i32s(*cfa), stream())
-    sr_, wr_ = s0.copy(), w0.copy()
     zero = np.zeros((h, w, 2), np.float32)
-    for f in range(n):
-        oracle.fn("accumulate")(oracle.ptr(raws[f]), oracle.ptr(sr_), oracle.ptr(wr_), oracle.ptr(masks[f]),
-                                oracle.ptr(kern), oracle.ptr(zero if flows[f] is None else flows[f]),
-                                oracle.ptr(oracle.arr(white, np.float32)), oracle.ptr(oracle.arr(black, np.float32)),
-                                w, h, oracle.ptr(oracle.arr(cfa, np.int32)))
-    np.testing.assert_array_equal(host(ds), sr_)
-    np.testing.assert_array_equal(host(dw), wr_)
+    res = {}
+    for order in ("accumulate_class_order", "accumulate"):
+        sr_, wr_ = s0.copy(), w0.copy()
+        for f in range(n):
+            oracle.fn(order)(oracle.ptr(raws[f]), oracle.ptr(sr_), oracle.ptr(wr_), oracle.ptr(masks[f]),
+                             oracle.ptr(kern), oracle.ptr(zero if flows[f] is None else flows[f]),
+                             oracle.ptr(oracle.arr(white, np.float32)), oracle.ptr(oracle.arr(black, np.float32)),
+                             w, h, oracle.ptr(oracle.arr(cfa, np.int32)))
+        res[order] = (sr_, wr_)
+    # bit for bit against the same tap order (class by class); within the reordering bound of the
+    # reference's raster order
+    got_s, got_w = host(ds), host(dw)
+    np.testing.assert_array_equal(got_s, res["accumulate_class_order"][0])
+    np.testing.assert_array_equal(got_w, res["accumulate_class_order"][1])
+    assert within_reorder_bound(got_s, got_w, *res["accumulate"], s0, w0, 25 * n, vmax=16383 / 15000)
 
 
 @pytest.mark.parametrize("sr,w", [(0, 64), (1, 64), (0, 62)])
@@ -508,9 +554,12 @@ def test_accumulate_apply_weighting(isa, oracle, gpu, sr, w):
             oracle.ptr(oracle.arr(white, np.float32)), oracle.ptr(oracle.arr(black, np.float32)), w, h,
             oracle.ptr(oracle.arr(RGGB, np.int32))]
     if sr:
-        oracle.fn("accumulate_sr")(*args, 0)
+        oracle.fn("accumulate_sr")(*args, 0)  # raster order on both sides
     else:
-        oracle.fn("accumulate")(*args)
+        oracle.fn("accumulate_class_order")(*args)
+        s_r, w_r = s0.copy(), w0.copy()
+        oracle.fn("accumulate")(*(args[:1] + [oracle.ptr(s_r), oracle.ptr(w_r)] + args[3:]))
+        assert within_reorder_bound(host(ds), host(dw), s_r, w_r, s0, w0, 25, vmax=16383 / 15000)
     np.testing.assert_array_equal(host(ds), sr_)
     np.testing.assert_array_equal(host(dw), wr_)
     io = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)

@@ -12,7 +12,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-from tests.gpu_util import host  # noqa: E402
+from tests.gpu_util import host, within_reorder_bound  # noqa: E402
 
 
 def _psnr(a, b, peak=1.0):
@@ -24,6 +24,11 @@ CASES = {
     "c0_small": dict(w=256, h=192, n=5, gmax=4.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0),
     "three_levels": dict(w=320, h=256, n=4, gmax=10.0, levels=[(1, 32, 2), (2, 32, 4), (4, 16, 4)], sr=0, occ=0.05),
     "superres": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=1, occ=0.0),
+    # non-zero, unequal black and white levels: the merge normalises with the levels as given (the
+    # reference's in-place WhiteLevel += black of each PrepareAccumulation / Accumulate call is not
+    # reproduced, on either side)
+    "black_level": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0,
+                        extra=dict(black=(64.0, 60.0, 72.0), white=(16000.0, 16200.0, 15800.0))),
 }
 
 
@@ -35,7 +40,7 @@ def test_merge_matches_oracle(isa, oracle, gpu, case):
     raw_t, _ = isa.synth_burst(cfg)
     raw = host(raw_t).view(np.uint16)
     assert raw.max() <= 16383
-    params = isa.default_params(levels=c["levels"], super_resolution=c["sr"])
+    params = isa.default_params(levels=c["levels"], super_resolution=c["sr"], **c.get("extra", {}))
     m = isa.BurstMerger(c["w"], c["h"], c["n"], params)
     out = host(m.run(raw_t))
     ref = oracle.merge(params, raw, want=("one_to_one", "flows", "masks", "kern", "track", "lk"))
@@ -94,8 +99,9 @@ def test_merge_is_deterministic_and_reusable(isa, gpu):
 
 def test_merger_accumulation_is_bit_exact_on_its_own_inputs(isa, oracle, gpu):
     """The merger's frame accumulation (all frames in one pass) equals the
-    oracle's accumulateImages called frame by frame, reference first, on the merger's own flows,
-    eroded masks and kernel parameters."""
+    oracle's accumulateImages in the same tap order (class by class) called frame by frame, reference
+    first, on the merger's own flows, eroded masks and kernel parameters, and is within the
+    reordering bound of the reference's raster tap order."""
     w, h, n = 256, 192, 5
     cfg = isa.default_synth(seed=515, width=w, height=h, frames=n, global_shift_max=5.0, occluded_fraction=0.05)
     raw_t, _ = isa.synth_burst(cfg)
@@ -106,17 +112,22 @@ def test_merger_accumulation_is_bit_exact_on_its_own_inputs(isa, oracle, gpu):
     ref = m.reference_index
     hw, hh = w // 2, h // 2
     kern = np.ascontiguousarray(host(m.buffer_tensor(5, 0, 3)))
-    s = np.zeros((h, w, 3), np.float32)
-    wt = np.zeros((h, w, 3), np.float32)
     zero = np.zeros((h, w, 2), np.float32)
     order = [ref] + [f for f in range(n) if f != ref]
-    for f in order:
-        mask4 = np.ascontiguousarray(host(m.buffer_tensor(4, f, 4)))
-        flow = zero if f == ref else np.ascontiguousarray(host(m.buffer_tensor(3, f, 2)))
-        oracle.fn("accumulate")(oracle.ptr(np.ascontiguousarray(raw[f])), oracle.ptr(s), oracle.ptr(wt),
-                                oracle.ptr(mask4), oracle.ptr(kern), oracle.ptr(flow),
-                                oracle.ptr(oracle.arr(list(params.white), np.float32)),
-                                oracle.ptr(oracle.arr(list(params.black), np.float32)), w, h,
-                                oracle.ptr(oracle.arr(list(params.cfa), np.int32)))
-    np.testing.assert_array_equal(host(m.buffer_tensor(6, 0, 3)), s)
-    np.testing.assert_array_equal(host(m.buffer_tensor(7, 0, 3)), wt)
+    res = {}
+    for taps in ("accumulate_class_order", "accumulate"):
+        s = np.zeros((h, w, 3), np.float32)
+        wt = np.zeros((h, w, 3), np.float32)
+        for f in order:
+            mask4 = np.ascontiguousarray(host(m.buffer_tensor(4, f, 4)))
+            flow = zero if f == ref else np.ascontiguousarray(host(m.buffer_tensor(3, f, 2)))
+            oracle.fn(taps)(oracle.ptr(np.ascontiguousarray(raw[f])), oracle.ptr(s), oracle.ptr(wt),
+                            oracle.ptr(mask4), oracle.ptr(kern), oracle.ptr(flow),
+                            oracle.ptr(oracle.arr(list(params.white), np.float32)),
+                            oracle.ptr(oracle.arr(list(params.black), np.float32)), w, h,
+                            oracle.ptr(oracle.arr(list(params.cfa), np.int32)))
+        res[taps] = (s, wt)
+    got_s, got_w = host(m.buffer_tensor(6, 0, 3)), host(m.buffer_tensor(7, 0, 3))
+    np.testing.assert_array_equal(got_s, res["accumulate_class_order"][0])
+    np.testing.assert_array_equal(got_w, res["accumulate_class_order"][1])
+    assert within_reorder_bound(got_s, got_w, *res["accumulate"], 0.0, 0.0, 25 * n)

@@ -19,7 +19,7 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
-from tests.gpu_util import dev, empty, host, p, stream  # noqa: E402
+from tests.gpu_util import dev, empty, host, p, stream, workspace  # noqa: E402
 
 XC = {"bf16x3": 1, "fp16x3": 2, "fp32": 3}
 # fraction of tiles within TOL px of the exact shift, and the tolerance
@@ -45,11 +45,13 @@ def _track(isa, a, b, pre, w, h, tile, M, code, thr=1.0):
     cx, cy = (w - 2 * M) // tile, (h - 2 * M) // tile
     d_pre = dev(pre)
     dec = empty((cy * cx,), dtype=torch.int32, fill=-7)
-    n_exact = C.c_int(-1)
+    n_exact = empty((1,), dtype=torch.int32, fill=-1)
     bs = (0.75, -0.25)
+    ws = workspace(isa.lib().isa_track_workspace(w, h, w * 4, tile, M, code))
     isa.call("isa_track_xc_ex", p(dev(b)), p(dev(a)), w * 4, p(d_pre), cx * 8, w, h, tile, M, isa.f32s(*bs),
-             C.c_float(0.0), isa.f32s(*bs), C.c_float(0.0), C.c_float(thr), code, p(dec), C.byref(n_exact), stream())
-    return host(d_pre).reshape(-1, 2) - pre.reshape(-1, 2), host(dec), n_exact.value
+             C.c_float(0.0), isa.f32s(*bs), C.c_float(0.0), C.c_float(thr), code, p(dec), p(n_exact), p(ws),
+             stream())
+    return host(d_pre).reshape(-1, 2) - pre.reshape(-1, 2), host(dec), int(host(n_exact)[0])
 
 
 CASES = [(32, 2, 200, 260), (64, 2, 210, 280), (32, 4, 200, 240), (16, 4, 120, 150), (48, 3, 160, 200)]
@@ -98,9 +100,11 @@ def test_track_xc_decisions_on_near_ties(isa, gpu, prec, kind, thr):
 def test_track_xc_rejects_unknown_precision(isa, gpu):
     a = np.zeros((64, 64), np.float32)
     d_pre = dev(np.zeros((1, 1, 2), np.float32))
+    ws = workspace(isa.lib().isa_track_workspace(64, 64, 64 * 4, 32, 2, 0))
     with pytest.raises(isa.IsaError):
         isa.call("isa_track_xc", p(dev(a)), p(dev(a)), 64 * 4, p(d_pre), 8, 64, 64, 32, 2, isa.f32s(0, 0),
-                 C.c_float(0.0), isa.f32s(0, 0), C.c_float(0.0), C.c_float(1.0), 7, stream())
+                 C.c_float(0.0), isa.f32s(0, 0), C.c_float(0.0), C.c_float(1.0), 7, p(ws), stream())
+    assert isa.lib().isa_track_workspace(64, 64, 64 * 4, 32, 2, 7) == 0
 
 
 def _psnr(x, y):
