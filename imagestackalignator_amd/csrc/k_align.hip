@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <vector>
@@ -1557,18 +1558,79 @@ int isa_shift_tables(int frame_count, int reference_index, int strategy, int blo
     return isa::shift_tables(frame_count, reference_index, strategy, block_size, pairs_out, index_out, rows_out);
 }
 
+}  // extern "C"
+
+namespace {
+
 // Workspace of isa_minimize_shifts: the measurement rows, then (for a connected graph of at most
-// kMinFrames frames) the Cholesky factor of the all-rows normal equations.
-static bool minimize_layout(int frame_count, int reference_index, int strategy, int block_size, int* m_out,
-                            size_t* rows_bytes, size_t* total) {
+// kMinFrames frames) the Cholesky factor of the all-rows normal equations. At most kMinFrames = 64
+// frames: m <= 64 * 63 / 2 = 2016 rows (16,128 bytes) and a 63 x 63 factor (31,752 bytes).
+constexpr size_t kMinimizeTablesMax = 16128 + 31752;
+
+bool minimize_layout(int frame_count, int reference_index, int strategy, int block_size, int* m_out,
+                     size_t* rows_bytes, size_t* total) {
+    if (frame_count < 2 || frame_count > kMinFrames) return false;
     int m = isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, nullptr);
-    if (m <= 0 || m > 4096) return false;
+    if (m <= 0) return false;
     const int n1 = frame_count - 1;
     *m_out = m;
     *rows_bytes = ((size_t)2 * m * sizeof(int) + 15) & ~(size_t)15;
     *total = *rows_bytes + (size_t)n1 * n1 * sizeof(double);
-    return true;
+    return *total <= kMinimizeTablesMax;
 }
+
+// The minimizer on the tables in `tables_dev` (at least the layout's total bytes, 16-byte aligned):
+// the rows and factor are built in a host stack buffer and written in stream order through kernel
+// arguments. No device allocation, no host synchronisation.
+int minimize_with_tables(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev,
+                         int* iters_dev, int tile_count, int frame_count, int reference_index, int strategy,
+                         int block_size, void* tables_dev, const char* who, hipStream_t s) {
+    ISA_REQUIRE(measured2_dev && one_to_one2_dev && tile_count > 0 && meas_pitch >= tile_count * 8 &&
+                    meas_pitch % 8 == 0,
+                "%s: bad arguments", who);
+    int m = 0;
+    size_t rows_bytes = 0, total = 0;
+    ISA_REQUIRE(minimize_layout(frame_count, reference_index, strategy, block_size, &m, &rows_bytes, &total),
+                "%s: invalid strategy / frame count (2..%d frames)", who, kMinFrames);
+    alignas(16) unsigned char host[kMinimizeTablesMax];  // 47,880 bytes of stack
+    int* rows = reinterpret_cast<int*>(host);
+    isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, rows);
+    double* factor = reinterpret_cast<double*>(host + rows_bytes);
+    const bool have_factor = isa::minimize_full_factor(rows, frame_count, m, factor);
+    ISA_TRY(isa::upload_by_args(tables_dev, host, have_factor ? total : rows_bytes, s));
+    const int* d_rows = (const int*)tables_dev;
+    const double* d_factor = have_factor ? (const double*)((char*)tables_dev + rows_bytes) : nullptr;
+    return isa::launch_minimize(measured2_dev, meas_pitch / sizeof(float), one_to_one2_dev, status_dev, iters_dev,
+                                d_rows, d_factor, tile_count, frame_count, m, s);
+}
+
+// Device-resident tables of isa_minimize_shifts_noalloc: one copy per GPU, held until the process ends.
+static __device__ __align__(256) unsigned char scratch[49152];
+static_assert(sizeof(scratch) >= kMinimizeTablesMax, "scratch holds the largest tables");
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch` on each GPU, looked up once
+
+int scratch_address(void** out) {
+    int dev = 0;
+    ISA_TRY(isa::check_hip(hipGetDevice(&dev), "isa_minimize_shifts_noalloc: hipGetDevice"));
+    ISA_REQUIRE(dev >= 0 && dev < kMaxDevices, "isa_minimize_shifts_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        ISA_TRY(isa::check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                               "isa_minimize_shifts_noalloc: scratch address"));
+        ISA_REQUIRE(p != nullptr, "isa_minimize_shifts_noalloc: no scratch address");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return ISA_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 size_t isa_minimize_shifts_workspace(int frame_count, int reference_index, int strategy, int block_size) {
     int m = 0;
@@ -1579,31 +1641,26 @@ size_t isa_minimize_shifts_workspace(int frame_count, int reference_index, int s
 int isa_minimize_shifts(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev, int* iters_dev,
                         int tile_count, int frame_count, int reference_index, int strategy, int block_size,
                         void* workspace_dev, isa_stream_t stream) {
-    ISA_REQUIRE(measured2_dev && one_to_one2_dev && workspace_dev && tile_count > 0 && meas_pitch >= tile_count * 8 &&
-                    meas_pitch % 8 == 0,
-                "isa_minimize_shifts: bad arguments");
-    ISA_REQUIRE(reinterpret_cast<uintptr_t>(workspace_dev) % 16 == 0, "isa_minimize_shifts: workspace must be 16-byte aligned");
+    ISA_REQUIRE(workspace_dev && reinterpret_cast<uintptr_t>(workspace_dev) % 16 == 0,
+                "isa_minimize_shifts: workspace must be a 16-byte aligned device buffer");
+    return minimize_with_tables(measured2_dev, meas_pitch, one_to_one2_dev, status_dev, iters_dev, tile_count,
+                                frame_count, reference_index, strategy, block_size, workspace_dev,
+                                "isa_minimize_shifts", isa::to_stream(stream));
+}
+
+int isa_minimize_shifts_noalloc(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev,
+                                int* iters_dev, int tile_count, int frame_count, int reference_index, int strategy,
+                                int block_size, isa_stream_t stream) {
     int m = 0;
-    size_t rows_bytes = 0, total = 0;
-    ISA_REQUIRE(minimize_layout(frame_count, reference_index, strategy, block_size, &m, &rows_bytes, &total),
-                "isa_minimize_shifts: invalid strategy / frame count");
-    std::vector<int> rows(2 * m);
-    isa::shift_tables(frame_count, reference_index, strategy, block_size, nullptr, nullptr, rows.data());
-    hipStream_t s = isa::to_stream(stream);
-    const int n1 = frame_count - 1;
-    std::vector<char> host(total, 0);
-    std::memcpy(host.data(), rows.data(), 2 * m * sizeof(int));
-    const bool have_factor =
-        frame_count <= kMinFrames &&
-        isa::minimize_full_factor(rows.data(), frame_count, m, (double*)(host.data() + rows_bytes));
-    // rows and factor travel in kernel arguments: no host buffer outlives this call, nothing waits
-    const size_t up = have_factor ? total : rows_bytes;
-    ISA_TRY(isa::upload_by_args(workspace_dev, host.data(), up, s));
-    const int* d_rows = (const int*)workspace_dev;
-    const double* d_factor = have_factor ? (const double*)((char*)workspace_dev + rows_bytes) : nullptr;
-    (void)n1;
-    return isa::launch_minimize(measured2_dev, meas_pitch / sizeof(float), one_to_one2_dev, status_dev, iters_dev,
-                                d_rows, d_factor, tile_count, frame_count, m, s);
+    size_t rb = 0, total = 0;
+    ISA_REQUIRE(minimize_layout(frame_count, reference_index, strategy, block_size, &m, &rb, &total),
+                "isa_minimize_shifts_noalloc: tables of %d frames exceed the %zu-byte scratch (2..%d frames)",
+                frame_count, sizeof(scratch), kMinFrames);
+    void* tables = nullptr;
+    ISA_TRY(scratch_address(&tables));
+    return minimize_with_tables(measured2_dev, meas_pitch, one_to_one2_dev, status_dev, iters_dev, tile_count,
+                                frame_count, reference_index, strategy, block_size, tables,
+                                "isa_minimize_shifts_noalloc", isa::to_stream(stream));
 }
 
 int isa_get_optimal_shifts(float* out2_dev, int out_pitch, const float* one_to_one2_dev, int frame_count,

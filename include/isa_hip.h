@@ -271,6 +271,14 @@ size_t isa_minimize_shifts_workspace(int frame_count, int reference_index, int s
 int isa_minimize_shifts(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev, int* iters_dev,
                         int tile_count, int frame_count, int reference_index, int strategy, int block_size,
                         void* workspace_dev, isa_stream_t stream);
+/* isa_minimize_shifts with its tables in a fixed device array of the library (48 KiB per GPU, enough for every
+ * supported frame count, 2..64) instead of a caller workspace. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * It allocates no device memory and does not synchronise. Every call on one GPU rewrites that GPU's one copy of the
+ * tables, so calls that overlap on one GPU (on different streams or threads) race; serialise them. */
+int isa_minimize_shifts_noalloc(float* measured2_dev, int meas_pitch, float* one_to_one2_dev, int* status_dev,
+                                int* iters_dev, int tile_count, int frame_count, int reference_index, int strategy,
+                                int block_size, isa_stream_t stream);
 /* getOptimalShifts, R:Kernels/ShiftMinimizerKernels.cu:178-218; wrapper R:PEFStudioDX/ShiftMinimizerKernels.cs:69-96. */
 int isa_get_optimal_shifts(float* out2_dev, int out_pitch, const float* one_to_one2_dev, int frame_count,
                            int tile_count_x, int tile_count_y, int reference_image, int image_to_track,
