@@ -66,6 +66,7 @@ class IsaParams(C.Structure):
         ("reference_index", C.c_int),
         ("tex_quant", C.c_int),
         ("xc_precision", C.c_int),
+        ("lk_ill_ratio", C.c_float),
     ]
 
     def set_levels(self, levels):

@@ -104,6 +104,7 @@ void isa_default_params(isa_params* p) {
     p->reference_index = -1;
     p->tex_quant = 1;
     p->xc_precision = ISA_XC_FP64;
+    p->lk_ill_ratio = 0.01f;
 }
 
 }  // extern "C"

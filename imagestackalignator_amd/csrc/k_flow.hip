@@ -231,6 +231,37 @@ __device__ __forceinline__ void half_angle(float y, float x, float& c, float& s)
     }
 }
 
+// Ill-conditioned window: lambda2 / lambda1 below ill_ratio, as det / trace^2 of the FP32 sums in double
+// (exact products of floats); also true for det <= 0 and NaN. The FP32 closed form loses lambda2 there
+// (S1 - S2 = 2 lambda2^2 cancels next to S1 ~ lambda1^2).
+__device__ __forceinline__ bool lk_ill(float fa, float fb, float fd, double ill_ratio) {
+    const double a = fa, b = fb, d = fd, tr = a + d;
+    return !(a * d - b * b > ill_ratio * tr * tr);
+}
+
+// FP64 solve of an ill-conditioned window from its FP32 sums: the reference's gate (lambda1 = sigma1 <
+// min_det skips), u = G^-1 (b0, b1) by the adjugate for det > 0 (the reference's closed-form
+// pseudo-inverse is the inverse there), the rank-1 pseudo-inverse G / trace^2 for det <= 0; NaN -> 0.
+__device__ __forceinline__ bool lk_solve_fp64(float fa, float fb, float fd, float fb0, float fb1, float min_det,
+                                              float& u0, float& u1) {
+    const double a = fa, b = fb, d = fd, b0 = fb0, b1 = fb1, tr = a + d;
+    const double lam1 = 0.5 * (tr + sqrt((a - d) * (a - d) + 4.0 * b * b));
+    if (lam1 < (double)min_det) return false;
+    const double det = a * d - b * b;
+    double v0, v1;
+    if (det > 0) {
+        v0 = (d * b0 - b * b1) / det;
+        v1 = (a * b1 - b * b0) / det;
+    } else {
+        const double t2 = tr * tr;
+        v0 = (a * b0 + b * b1) / t2;
+        v1 = (b * b0 + d * b1) / t2;
+    }
+    u0 = isnan(v0) ? 0.0f : (float)v0;
+    u1 = isnan(v1) ? 0.0f : (float)v1;
+    return true;
+}
+
 template <bool EXACT>
 __device__ __forceinline__ bool lk_pinv(float a, float b, float d, float min_det, float inv[4]) {
     float c = b;
@@ -313,14 +344,14 @@ __global__ void k_lk_step(float* flow, int flow_pitch, const float* Fx, const fl
 }
 
 // Lucas-Kanade step with the five window sums (Ix^2, IxIy, Iy^2, IxIt, IyIt) formed by separable
-// box sums in LDS instead of 2 (2h+1)^2 loads per pixel. Each horizontal / vertical pass starts a
-// segment of LK_SEG outputs with a plain (2h+1)-term sum and continues it as a running sum. The
-// solve is the same closed form; the update is pinv . (sum Ix It, sum Iy It), which equals the
-// reference's sum of pinv . (Ix, Iy) It.
+// box sums in LDS instead of 2 (2h+1)^2 loads per pixel: a plain (2h+1)-term sum per output of each
+// horizontal and vertical pass. The solve is the same closed form (windows below ill_ratio in FP64,
+// as in the fused kernel); the update is pinv . (sum Ix It, sum Iy It), which equals the reference's
+// sum of pinv . (Ix, Iy) It.
 constexpr int LK_BX = 32, LK_BY = 16, LK_HMAX = 10, LK_SEG = 8;
 __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int flow_pitch, const float* Fx,
                                                                const float* Fy, const float* Ft, int pitch, int w,
-                                                               int h, int hw, float min_det) {
+                                                               int h, int hw, float min_det, double ill_ratio) {
     constexpr int RW = LK_BX + 2 * LK_HMAX;  // region width (max)
     __shared__ float s_p[LK_BY + 2 * LK_HMAX][RW][5];
     __shared__ float s_h[LK_BY + 2 * LK_HMAX][LK_BX][5];
@@ -344,11 +375,9 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
     for (int task = tid; task < rows * HSEG * 5; task += nt) {
         int q = task % 5, rs = task / 5, seg = rs % HSEG, ly = rs / HSEG;
         int xs = seg * LK_SEG;
-        float acc = 0.0f;
-        for (int k = 0; k < win; ++k) acc += s_p[ly][xs + k][q];
-        s_h[ly][xs][q] = acc;
-        for (int i = 1; i < LK_SEG; ++i) {
-            acc += s_p[ly][xs + i + win - 1][q] - s_p[ly][xs + i - 1][q];
+        for (int i = 0; i < LK_SEG; ++i) {
+            float acc = 0.0f;
+            for (int k = 0; k < win; ++k) acc += s_p[ly][xs + i + k][q];
             s_h[ly][xs + i][q] = acc;
         }
     }
@@ -358,11 +387,9 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
     for (int task = tid; task < LK_BX * VSEG * 5; task += nt) {
         int q = task % 5, cs = task / 5, seg = cs % VSEG, lx = cs / VSEG;
         int ys = seg * LK_SEG;
-        float acc = 0.0f;
-        for (int k = 0; k < win; ++k) acc += s_h[ys + k][lx][q];
-        s_v[ys][lx][q] = acc;
-        for (int i = 1; i < LK_SEG; ++i) {
-            acc += s_h[ys + i + win - 1][lx][q] - s_h[ys + i - 1][lx][q];
+        for (int i = 0; i < LK_SEG; ++i) {
+            float acc = 0.0f;
+            for (int k = 0; k < win; ++k) acc += s_h[ys + i + k][lx][q];
             s_v[ys + i][lx][q] = acc;
         }
     }
@@ -372,12 +399,17 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
     float g[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) g[q] = s_v[threadIdx.y][threadIdx.x][q];
-    float inv[4];
-    if (!lk_pinv<false>(g[0], g[1], g[2], min_det, inv)) return;
-    float u0 = inv[0] * g[3] + inv[1] * g[4];
-    float u1 = inv[2] * g[3] + inv[3] * g[4];
-    u0 = isnan(u0) ? 0.0f : u0;
-    u1 = isnan(u1) ? 0.0f : u1;
+    float u0, u1;
+    if (ill_ratio > 0.0 && lk_ill(g[0], g[1], g[2], ill_ratio)) {
+        if (!lk_solve_fp64(g[0], g[1], g[2], g[3], g[4], min_det, u0, u1)) return;
+    } else {
+        float inv[4];
+        if (!lk_pinv<false>(g[0], g[1], g[2], min_det, inv)) return;
+        u0 = inv[0] * g[3] + inv[1] * g[4];
+        u1 = inv[2] * g[3] + inv[3] * g[4];
+        u0 = isnan(u0) ? 0.0f : u0;
+        u1 = isnan(u1) ? 0.0f : u1;
+    }
     float* f = row_ptr(flow, flow_pitch, py) + px * 2;
     f[0] += u0;
     f[1] += u1;
@@ -390,13 +422,46 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
 //  B. Ix, Iy, It over the block plus HW of halo from those tiles (same arithmetic as
 //     k_derivatives_tab; pixels outside the image are never inside the window of an updated
 //     pixel and are set to 0);
-//  C. the five window sums as separable box sums: per region row, running sums along x over
-//     segments of 8 output columns (from a register window of the row's derivatives), then per
-//     output column running sums along y over 4 output rows;
+//  C. the five window sums as separable box sums: per region row, sums along x over segments of 8
+//     output columns (from a register window of the row's derivatives), then per output column sums
+//     along y over 4 output rows, both by window_sums (no running subtraction);
 //  D. the closed-form solve and update, reading flow_in and writing flow_out (the halo of the
 //     warp reads neighbouring blocks' flow, so the flow is double-buffered across iterations).
 constexpr int LKF_BX = 64, LKF_BY = 32, LKF_RPT = 4;  // 512 threads, 4 output rows each
-constexpr int LKF_SEG = 8;                            // output columns per horizontal running sum
+constexpr int LKF_SEG = 8;                            // output columns per horizontal task
+
+// Sums of W consecutive terms, out[i] = p[i] + ... + p[i + W - 1] for i = 0 .. N - 1, built from partial
+// sums that only add (a running sum that subtracts the term leaving the window keeps the rounding error
+// of every larger sum it passed through). Per group of G = min(N, W) outputs the terms shared by all
+// its windows form one core sum; each window adds the suffix sum of the terms left of the core and the
+// prefix sum of those right of it: out = (left + core) + right, so each is within gamma_W of the sum of
+// its terms' magnitudes, as a plain loop is.
+template <int N, int W>
+__device__ __forceinline__ void window_sums(const float (&p)[N + W - 1], float (&out)[N]) {
+    constexpr int G = N < W ? N : W;
+#pragma unroll
+    for (int g0 = 0; g0 < N; g0 += G) {
+        const int gn = N - g0 < G ? N - g0 : G;     // outputs i = g0 .. g0 + gn - 1
+        const int c0 = g0 + gn - 1, c1 = g0 + W - 1;  // terms c0 .. c1 lie in every window of the group
+        float core = p[c0];
+#pragma unroll
+        for (int k = c0 + 1; k <= c1; ++k) core += p[k];
+        float left[G], right[G];
+        left[gn - 1] = 0.0f;
+        right[0] = 0.0f;
+#pragma unroll
+        for (int j = gn - 2; j >= 0; --j) left[j] = j == gn - 2 ? p[g0 + j] : p[g0 + j] + left[j + 1];
+#pragma unroll
+        for (int j = 1; j < gn; ++j) right[j] = j == 1 ? p[c1 + 1] : right[j - 1] + p[c1 + j];
+#pragma unroll
+        for (int j = 0; j < gn; ++j) {
+            float v = core;
+            if (j < gn - 1) v = left[j] + v;
+            if (j > 0) v = v + right[j];
+            out[g0 + j] = v;
+        }
+    }
+}
 
 template <int HW>
 struct LkfShared {
@@ -420,7 +485,7 @@ struct LkfShared {
 template <int HW>
 __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     const float* src, const float* tgt, int img_pitch, const float* flow_in, float* flow_out, int flow_pitch, int w,
-    int h, float min_det, int quant, const Tap* __restrict__ col, const Tap* __restrict__ row) {
+    int h, float min_det, int quant, const Tap* __restrict__ col, const Tap* __restrict__ row, double ill_ratio) {
     using S = LkfShared<HW>;
     constexpr int DH = S::DH, DW = S::DW, WR = S::WR, WC = S::WC;
     constexpr int NT = LKF_BX * (LKF_BY / LKF_RPT);
@@ -628,9 +693,8 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     }
     __syncthreads();
     // C1. horizontal window sums of the five products for every region row: a task covers LKF_SEG
-    // output columns of one row, starting with a full (2HW+1)-term sum and continuing as a running
-    // sum (the derivative window of the task stays in registers). The sums overwrite the LDS of
-    // phases A and B, so they are stored after a barrier.
+    // output columns of one row (the derivative window of the task stays in registers). The sums
+    // overwrite the LDS of phases A and B, so they are stored after a barrier.
     constexpr int NSEG = LKF_BX / LKF_SEG, NW = LKF_SEG + 2 * HW;
     static_assert(DH * NSEG <= NT, "one horizontal task per thread");
     static_assert(sizeof(sm.u.H) <= sizeof(sm.u.ab), "the window sums fit in the phase A/B storage");
@@ -638,38 +702,18 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     const int ry = tid % DH, xs = (tid / DH) * LKF_SEG;
     float hs[5][LKF_SEG];
     if (has_task) {
-        float ax[NW], ay[NW], at[NW];
+        float pr[5][NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
-            ax[k] = Ld[0][ry][xs + k];
-            ay[k] = Ld[1][ry][xs + k];
-            at[k] = Ld[2][ry][xs + k];
-        }
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
-#pragma unroll
-        for (int k = 0; k <= 2 * HW; ++k) {
-            s0 += ax[k] * ax[k];
-            s1 += ax[k] * ay[k];
-            s2 += ay[k] * ay[k];
-            s3 += ax[k] * at[k];
-            s4 += ay[k] * at[k];
+            const float ax = Ld[0][ry][xs + k], ay = Ld[1][ry][xs + k], at = Ld[2][ry][xs + k];
+            pr[0][k] = ax * ax;
+            pr[1][k] = ax * ay;
+            pr[2][k] = ay * ay;
+            pr[3][k] = ax * at;
+            pr[4][k] = ay * at;
         }
 #pragma unroll
-        for (int i = 0; i < LKF_SEG; ++i) {
-            if (i > 0) {
-                const int a = i + 2 * HW, b = i - 1;  // column entering and column leaving the window
-                s0 += ax[a] * ax[a] - ax[b] * ax[b];
-                s1 += ax[a] * ay[a] - ax[b] * ay[b];
-                s2 += ay[a] * ay[a] - ay[b] * ay[b];
-                s3 += ax[a] * at[a] - ax[b] * at[b];
-                s4 += ay[a] * at[a] - ay[b] * at[b];
-            }
-            hs[0][i] = s0;
-            hs[1][i] = s1;
-            hs[2][i] = s2;
-            hs[3][i] = s3;
-            hs[4][i] = s4;
-        }
+        for (int q = 0; q < 5; ++q) window_sums<LKF_SEG, 2 * HW + 1>(pr[q], hs[q]);
     }
     __syncthreads();
     auto& H = sm.u.H;
@@ -679,21 +723,17 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
 #pragma unroll
             for (int i = 0; i < LKF_SEG; ++i) H[q][ry][xs + i] = hs[q][i];
     __syncthreads();
-    // C2. vertical window sums for LKF_RPT output rows of one column: a full sum for the first row,
-    // a running sum for the others
+    // C2. vertical window sums for LKF_RPT output rows of one column
     const int lx = tid % LKF_BX, oy0 = (tid / LKF_BX) * LKF_RPT;
     float V[LKF_RPT][5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        float v = 0.0f;
+        float col[LKF_RPT + 2 * HW], vs[LKF_RPT];
 #pragma unroll
-        for (int r = 0; r <= 2 * HW; ++r) v += H[q][oy0 + r][lx];
-        V[0][q] = v;
+        for (int r = 0; r < LKF_RPT + 2 * HW; ++r) col[r] = H[q][oy0 + r][lx];
+        window_sums<LKF_RPT, 2 * HW + 1>(col, vs);
 #pragma unroll
-        for (int o = 1; o < LKF_RPT; ++o) {
-            v += H[q][oy0 + o + 2 * HW][lx] - H[q][oy0 + o - 1][lx];
-            V[o][q] = v;
-        }
+        for (int o = 0; o < LKF_RPT; ++o) V[o][q] = vs[o];
     }
     // D. solve and update
     const int px = x0 + lx;
@@ -705,13 +745,21 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         const float* fi = row_ptr(flow_in, flow_pitch, py) + px * 2;
         float f0 = fi[0], f1 = fi[1];
         float inv[4];
-        if (px >= HW && px < w - HW && py >= HW && py < h - HW && lk_pinv<false>(V[o][0], V[o][1], V[o][2], min_det, inv)) {
-            float u0 = inv[0] * V[o][3] + inv[1] * V[o][4];
-            float u1 = inv[2] * V[o][3] + inv[3] * V[o][4];
-            u0 = isnan(u0) ? 0.0f : u0;
-            u1 = isnan(u1) ? 0.0f : u1;
-            f0 += u0;
-            f1 += u1;
+        if (px >= HW && px < w - HW && py >= HW && py < h - HW) {
+            if (ill_ratio > 0.0 && lk_ill(V[o][0], V[o][1], V[o][2], ill_ratio)) {
+                float u0, u1;
+                if (lk_solve_fp64(V[o][0], V[o][1], V[o][2], V[o][3], V[o][4], min_det, u0, u1)) {
+                    f0 += u0;
+                    f1 += u1;
+                }
+            } else if (lk_pinv<false>(V[o][0], V[o][1], V[o][2], min_det, inv)) {
+                float u0 = inv[0] * V[o][3] + inv[1] * V[o][4];
+                float u1 = inv[2] * V[o][3] + inv[3] * V[o][4];
+                u0 = isnan(u0) ? 0.0f : u0;
+                u1 = isnan(u1) ? 0.0f : u1;
+                f0 += u0;
+                f1 += u1;
+            }
         }
         float* fo = row_ptr(flow_out, flow_pitch, py) + px * 2;
         fo[0] = f0;
@@ -761,11 +809,11 @@ int launch_derivatives2(const float* img, int img_pitch, float* ix, float* iy, i
 }
 
 int launch_lk_step(float* flow, int flow_pitch, const float* ix, const float* iy, const float* it, int pitch, int w,
-                   int h, int window, float min_det, bool box, hipStream_t s) {
+                   int h, int window, float min_det, bool box, float ill_ratio, hipStream_t s) {
     int hw = window / 2;
     if (box && hw <= LK_HMAX) {
         k_lk_step_box<<<dim3(div_up(w, LK_BX), div_up(h, LK_BY)), dim3(LK_BX, LK_BY), 0, s>>>(
-            flow, flow_pitch, ix, iy, it, pitch, w, h, hw, min_det);
+            flow, flow_pitch, ix, iy, it, pitch, w, h, hw, min_det, (double)ill_ratio);
     } else {
         k_lk_step<<<dim3(div_up(w, 32), div_up(h, 16)), dim3(32, 16), 0, s>>>(flow, flow_pitch, ix, iy, it, pitch, w,
                                                                               h, hw, min_det);
@@ -786,7 +834,8 @@ int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s) {
 // taps: tables from build_tap_tables for (w, h, quant).
 int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles, int tiles_pitch, int tw, int th,
                  float* flow, int flow_pitch, float* work, int w, int h, int iterations, float bsx, float bsy,
-                 float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps) {
+                 float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps,
+                 float ill_ratio) {
     const size_t np = (size_t)w * h;
     float* tmp = work;
     float* ix = work + np;
@@ -811,7 +860,7 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
 #define ISA_LKF(H)                                                                                                   \
     case H:                                                                                                        \
         k_lk_fused<H><<<grid, block, 0, s>>>(src, tgt, img_pitch, fi, fo, flow_pitch, w, h, min_det, quant, col,   \
-                                             col + (size_t)w * 5);                                                 \
+                                             col + (size_t)w * 5, (double)ill_ratio);                              \
         break;
             switch (hw) { ISA_LKF(1) ISA_LKF(2) ISA_LKF(3) ISA_LKF(4) ISA_LKF(5) ISA_LKF(6) ISA_LKF(7) ISA_LKF(8) }
 #undef ISA_LKF
@@ -824,7 +873,7 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
     for (int it = 0; it < iterations; ++it) {
         ISA_TRY(launch_warp(src, img_pitch, tmp, wp, flow, flow_pitch, w, h, quant, s));
         ISA_TRY(launch_derivatives(tmp, tgt, wp, ix, iy, iz, wp, w, h, quant, taps, s));
-        ISA_TRY(launch_lk_step(flow, flow_pitch, ix, iy, iz, wp, w, h, window, min_det, box, s));
+        ISA_TRY(launch_lk_step(flow, flow_pitch, ix, iy, iz, wp, w, h, window, min_det, box, ill_ratio, s));
     }
     return ISA_OK;
 }
@@ -870,7 +919,7 @@ int isa_lucas_kanade_optim(float* flow2_dev, int flow_pitch, const float* ix_dev
     ISA_REQUIRE(flow2_dev && ix_dev && iy_dev && it_dev && width > 0 && height > 0 && window_size >= 1,
                 "isa_lucas_kanade_optim: bad arguments");
     return isa::launch_lk_step(flow2_dev, flow_pitch, ix_dev, iy_dev, it_dev, img_pitch, width, height, window_size,
-                               min_det, false, isa::to_stream(stream));
+                               min_det, false, 0.0f, isa::to_stream(stream));
 }
 
 int isa_lucas_kanade(float* src_dev, const float* tgt_dev, int img_pitch, const float* tiles2_dev, int tiles_pitch,
@@ -883,7 +932,7 @@ int isa_lucas_kanade(float* src_dev, const float* tgt_dev, int img_pitch, const 
     hipStream_t s = isa::to_stream(stream);
     ISA_TRY(isa::lucas_kanade(src_dev, tgt_dev, img_pitch, tiles2_dev, tiles_pitch, tex_w, tex_h, flow2_dev, flow_pitch,
                               work_dev, width, height, iterations, base_shift[0], base_shift[1], base_rotation, min_det,
-                              window_size, tex_quant, false, s, nullptr));
+                              window_size, tex_quant, false, s, nullptr, 0.0f));
     // final warp, copied over the source image (R:PEFStudioDX/OpticalFlow.cs:162-163)
     ISA_TRY(isa::launch_warp(src_dev, img_pitch, work_dev, width * (int)sizeof(float), flow2_dev, flow_pitch, width,
                              height, tex_quant, s));

@@ -46,7 +46,8 @@ int launch_derivatives2(const float* img, int img_pitch, float* ix, float* iy, i
                         hipStream_t s);
 int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles, int tiles_pitch, int tw, int th,
                  float* flow, int flow_pitch, float* work, int w, int h, int iterations, float bsx, float bsy,
-                 float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps);
+                 float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps,
+                 float ill_ratio);
 size_t tap_table_bytes(int w, int h);
 int track_chunks(const int* pa, int m, int n, int* out_chunks);
 int track_chunks_capacity(int m, int n);
@@ -259,6 +260,7 @@ int validate(const isa_params& P, int w, int h, int n) {
     }
     ISA_REQUIRE(P.iterations_lk >= 0 && P.window_size_lk >= 1 && P.erode_size >= 1 && P.erode_size <= 41,
                 "merger: LK / erode parameters out of range");
+    ISA_REQUIRE(P.lk_ill_ratio >= 0.0f && P.lk_ill_ratio <= 0.25f, "merger: lk_ill_ratio must be in [0, 0.25]");
     for (int i = 0; i < 4; ++i) ISA_REQUIRE(P.cfa[i] >= 0 && P.cfa[i] <= 2, "merger: CFA entries must be 0, 1 or 2");
     ISA_REQUIRE(P.xc_precision >= ISA_XC_FP64 && P.xc_precision <= ISA_XC_FP32, "merger: unknown xc_precision");
     if (P.xc_precision != ISA_XC_FP64)
@@ -604,7 +606,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         ISA_TRY(isa::lucas_kanade(M->lk + (size_t)f * np, lk_ref, wp, M->tile_img, M->cx[0] * 8, M->cx[0], M->cy[0],
                                   M->flows + (size_t)f * np * 2, 2 * wp, M->lk_work, w, h, P.iterations_lk, bsx, bsy,
                                   store_rotation(st, ref, f), P.min_det_lk, P.window_size_lk, P.tex_quant, true, s,
-                                  M->taps));
+                                  M->taps, P.lk_ill_ratio));
     }
     (void)hipEventRecord(M->ev[5], s);
     for (int f = 0; f < n; ++f) {

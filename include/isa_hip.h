@@ -109,6 +109,9 @@ typedef struct isa_params {
     int reference_index;              /* -1: PreAlignmentStore rule (R:PEFStudioDX/PreAlignmentStore.cs:36-110) */
     int tex_quant;                    /* 1 (default): 8-bit texture weights as the texture unit; 0: exact weights */
     int xc_precision;                 /* tile cross terms: ISA_XC_FP64 (default, exact direct SSD) or an MFMA form */
+    float lk_ill_ratio;               /* 0.01: Lucas-Kanade windows whose FP32 sums give det < ratio * trace^2
+                                         (lambda2 / lambda1 below the ratio) are solved in FP64; 0: the reference's
+                                         FP32 closed form everywhere. No counterpart in the reference. */
 } isa_params;
 
 void isa_default_params(isa_params* p);

@@ -860,13 +860,97 @@ template <class T> void derivatives2(const T* img, T* Ix, T* Iy, int w, int h, i
     });
 }
 
-// One lucasKanadeOptim pass; flow updated in place (each pixel reads only itself).
+// The closed-form solve of lucasKanadeOptim (R:Kernels/opticalFlow.cu:237-321) for the window sums
+// (g0, g1, g3) of pixel (px, py), in T: returns false when the sigma1 gate skips the pixel, else the
+// update (u0, u1), NaN mapped to 0. Derivatives of type D (float for the FP32 path, float or double
+// for the FP64 solve of an FP32 pass).
+template <class T, class D>
+static bool lk_solve(T g0, T g1, T g3, const D* Fx, const D* Fy, const D* Ft, int w, int px, int py, int hw, T min_det,
+                     T& u0, T& u1) {
+    int ws = hw * 2 + 1;
+    T a = g0, b = g1, c = g1, d = g3;
+    T theta = (T)0.5f * atan2_r((T)2.0f * a * c + (T)2.0f * b * d, a * a + b * b - c * c - d * d);
+    T ct = cos_r(theta), st = sin_r(theta);
+    T UT0 = ct, UT2 = -st, UT1 = st, UT3 = ct;
+    T S1 = a * a + b * b + c * c + d * d;
+    T S2 = std::sqrt((a * a + b * b - c * c - d * d) * (a * a + b * b - c * c - d * d) +
+                     (T)4 * (a * c + b * d) * (a * c + b * d));
+    T sigma1 = std::sqrt((S1 + S2) / (T)2);
+    T sigma2 = std::sqrt((S1 - S2) / (T)2);
+    T smin = std::fmin(sigma1, sigma1);  // R:Kernels/opticalFlow.cu:255 gates on sigma1 (quirk kept)
+    if (smin < min_det) return false;
+    sigma1 = sigma1 != 0 ? (T)1.0f / sigma1 : (T)0;
+    sigma2 = sigma2 != 0 ? (T)1.0f / sigma2 : (T)0;
+    T S0 = sigma1, Sb = 0, Sc = 0, S3 = sigma2;
+    T eps = (T)0.5f * atan2_r((T)2.0f * a * b + (T)2.0f * c * d, a * a - b * b + c * c - d * d);
+    T ce = cos_r(eps), se = sin_r(eps);
+    T s11 = (a * ct + c * st) * ce + (b * ct + d * st) * se;
+    T s22 = (a * st - c * ct) * se + (-b * st + d * ct) * ce;
+    s11 = s11 > (T)0.0f ? (T)1.0f : s11 < 0 ? (T)-1.0f : (T)0.0f;
+    s22 = s22 > (T)0.0f ? (T)1.0f : s22 < 0 ? (T)-1.0f : (T)0.0f;
+    T V0 = s11 * ce, V1 = -s22 * se, V2 = s11 * se, V3 = s22 * ce;
+    T m0 = S0 * UT0 + Sb * UT2, m1 = S0 * UT1 + Sb * UT3, m2 = Sc * UT0 + S3 * UT2, m3 = Sc * UT1 + S3 * UT3;
+    T i0 = V0 * m0 + V1 * m2, i1 = V0 * m1 + V1 * m3, i2 = V2 * m0 + V3 * m2, i3 = V2 * m1 + V3 * m3;
+    u0 = 0;
+    u1 = 0;
+    for (int i = 0; i < ws * ws; ++i) {
+        int yy = i / ws, xx = i - yy * ws;
+        size_t o = (size_t)(py + yy - hw) * w + px + xx - hw;
+        T dx = (T)Fx[o], dy = (T)Fy[o], dt = (T)Ft[o];
+        u0 += (i0 * dx + i1 * dy) * dt;
+        u1 += (i2 * dx + i3 * dy) * dt;
+    }
+    u0 = std::isnan(u0) ? (T)0 : u0;
+    u1 = std::isnan(u1) ? (T)0 : u1;
+    return true;
+}
+
+// Ill-conditioned window of an FP32 pass: lambda2 / lambda1 of the window matrix below ill_ratio,
+// estimated in double from the FP32 sums as det / trace^2 (also true for det <= 0 and NaN). There the
+// FP32 closed form loses lambda2 to cancellation (S1 - S2 = 2 lambda2^2 next to S1 ~ lambda1^2).
+inline bool lk_ill_conditioned(float g0, float g1, float g3, double ill_ratio) {
+    const double a = g0, b = g1, d = g3, tr = a + d, det = a * d - b * b;  // exact products of floats
+    return !(det > ill_ratio * tr * tr);
+}
+
+// FP64 solve of an ill-conditioned window from its FP32 sums (a, b, d of the window matrix, b0, b1 of
+// sum Ix It, sum Iy It): the same gate as the reference (sigma1 = lambda1 < min_det skips), then
+// u = G^-1 (b0, b1) by the adjugate for det > 0 (the closed-form pseudo-inverse of the reference is the
+// inverse there), and the rank-1 pseudo-inverse G / trace^2 when rounding left det <= 0. NaN -> 0.
+inline bool lk_solve_fp64(float fa, float fb, float fd, float fb0, float fb1, double min_det, double& u0, double& u1) {
+    const double a = fa, b = fb, d = fd, b0 = fb0, b1 = fb1, tr = a + d;
+    const double lam1 = 0.5 * (tr + std::sqrt((a - d) * (a - d) + 4.0 * b * b));
+    if (lam1 < min_det) return false;
+    const double det = a * d - b * b;
+    if (det > 0) {
+        u0 = (d * b0 - b * b1) / det;
+        u1 = (a * b1 - b * b0) / det;
+    } else {
+        const double t2 = tr * tr;
+        u0 = (a * b0 + b * b1) / t2;
+        u1 = (b * b0 + d * b1) / t2;
+    }
+    if (std::isnan(u0)) u0 = 0;
+    if (std::isnan(u1)) u1 = 0;
+    return true;
+}
+
+std::atomic<long long> g_lk_ill_windows{0}, g_lk_windows{0};  // windows solved in FP64 / all (FP32 passes)
+
+// One lucasKanadeOptim pass; flow updated in place (each pixel reads only itself). For T = float and
+// ill_ratio > 0, windows with lambda2 / lambda1 < ill_ratio (lk_ill_conditioned) are solved in FP64 from
+// their FP32 sums (lk_solve_fp64, sums in the window's raster order) and the FP64 update is rounded once
+// and added to the FP32 flow; ill_ratio = 0 is the reference's FP32 arithmetic everywhere.
+// cond (optional, w x h x 2): per pixel the smallest det / trace^2 of its window's sums so far and the
+// smallest distance |lambda1 / min_det - 1| from the sigma1 gate (lowered here).
 template <class T>
-void lucas_kanade_step(T* flow, const T* Fx, const T* Fy, const T* Ft, int w, int h, int hw, T min_det) {
+void lucas_kanade_step(T* flow, const T* Fx, const T* Fy, const T* Ft, int w, int h, int hw, T min_det,
+                       double ill_ratio, float* cond = nullptr) {
+    const double ill = std::is_same<T, float>::value ? ill_ratio : 0.0;
     parallel_for(h, [&](int y0, int y1) {
+        long long n_ill = 0, n_all = 0;
         for (int py = std::max(y0, hw); py < std::min(y1, h - hw); ++py)
             for (int px = hw; px < w - hw; ++px) {
-                int ws = hw * 2 + 1;
                 T g0 = 0, g1 = 0, g3 = 0;
                 for (int y = -hw; y <= hw; ++y)
                     for (int x = -hw; x <= hw; ++x) {
@@ -876,57 +960,60 @@ void lucas_kanade_step(T* flow, const T* Fx, const T* Fy, const T* Ft, int w, in
                         g1 += dx * dy;
                         g3 += dy * dy;
                     }
-                T a = g0, b = g1, c = g1, d = g3;
-                T theta = (T)0.5f * atan2_r((T)2.0f * a * c + (T)2.0f * b * d, a * a + b * b - c * c - d * d);
-                T ct = cos_r(theta), st = sin_r(theta);
-                T UT0 = ct, UT2 = -st, UT1 = st, UT3 = ct;
-                T S1 = a * a + b * b + c * c + d * d;
-                T S2 = std::sqrt((a * a + b * b - c * c - d * d) * (a * a + b * b - c * c - d * d) +
-                                 (T)4 * (a * c + b * d) * (a * c + b * d));
-                T sigma1 = std::sqrt((S1 + S2) / (T)2);
-                T sigma2 = std::sqrt((S1 - S2) / (T)2);
-                T smin = std::fmin(sigma1, sigma1);  // R:Kernels/opticalFlow.cu:255 gates on sigma1 (quirk kept)
-                if (smin < min_det) continue;
-                sigma1 = sigma1 != 0 ? (T)1.0f / sigma1 : (T)0;
-                sigma2 = sigma2 != 0 ? (T)1.0f / sigma2 : (T)0;
-                T S0 = sigma1, Sb = 0, Sc = 0, S3 = sigma2;
-                T eps = (T)0.5f * atan2_r((T)2.0f * a * b + (T)2.0f * c * d, a * a - b * b + c * c - d * d);
-                T ce = cos_r(eps), se = sin_r(eps);
-                T s11 = (a * ct + c * st) * ce + (b * ct + d * st) * se;
-                T s22 = (a * st - c * ct) * se + (-b * st + d * ct) * ce;
-                s11 = s11 > (T)0.0f ? (T)1.0f : s11 < 0 ? (T)-1.0f : (T)0.0f;
-                s22 = s22 > (T)0.0f ? (T)1.0f : s22 < 0 ? (T)-1.0f : (T)0.0f;
-                T V0 = s11 * ce, V1 = -s22 * se, V2 = s11 * se, V3 = s22 * ce;
-                T m0 = S0 * UT0 + Sb * UT2, m1 = S0 * UT1 + Sb * UT3, m2 = Sc * UT0 + S3 * UT2, m3 = Sc * UT1 + S3 * UT3;
-                T i0 = V0 * m0 + V1 * m2, i1 = V0 * m1 + V1 * m3, i2 = V2 * m0 + V3 * m2, i3 = V2 * m1 + V3 * m3;
-                T u0 = 0, u1 = 0;
-                for (int i = 0; i < ws * ws; ++i) {
-                    int yy = i / ws, xx = i - yy * ws;
-                    size_t o = (size_t)(py + yy - hw) * w + px + xx - hw;
-                    T dx = Fx[o], dy = Fy[o], dt = Ft[o];
-                    u0 += (i0 * dx + i1 * dy) * dt;
-                    u1 += (i2 * dx + i3 * dy) * dt;
+                T u0, u1;
+                ++n_all;
+                if (cond) {
+                    // (det / trace^2, |lambda1 / min_det - 1|): distance from singular and from the gate
+                    const double a = (double)g0, b = (double)g1, d = (double)g3, tr = a + d;
+                    const float r = (float)((a * d - b * b) / (tr * tr));
+                    const double lam1 = 0.5 * (tr + std::sqrt((a - d) * (a - d) + 4.0 * b * b));
+                    const float gm = (float)std::fabs(lam1 / (double)min_det - 1.0);
+                    float* c = cond + ((size_t)py * w + px) * 2;
+                    if (!(r >= c[0])) c[0] = r;  // NaN (flat window) sticks
+                    if (!(gm >= c[1])) c[1] = gm;
                 }
-                u0 = std::isnan(u0) ? (T)0 : u0;
-                u1 = std::isnan(u1) ? (T)0 : u1;
+                if (ill > 0 && lk_ill_conditioned((float)g0, (float)g1, (float)g3, ill)) {
+                    ++n_ill;
+                    T b0 = 0, b1 = 0;
+                    for (int y = -hw; y <= hw; ++y)
+                        for (int x = -hw; x <= hw; ++x) {
+                            size_t o = (size_t)(py + y) * w + px + x;
+                            b0 += Fx[o] * Ft[o];
+                            b1 += Fy[o] * Ft[o];
+                        }
+                    double v0, v1;
+                    if (!lk_solve_fp64((float)g0, (float)g1, (float)g3, (float)b0, (float)b1, (double)min_det, v0, v1))
+                        continue;
+                    u0 = (T)v0;
+                    u1 = (T)v1;
+                } else if (!lk_solve<T, T>(g0, g1, g3, Fx, Fy, Ft, w, px, py, hw, min_det, u0, u1)) {
+                    continue;
+                }
                 flow[((size_t)py * w + px) * 2] += u0;
                 flow[((size_t)py * w + px) * 2 + 1] += u1;
             }
+        if (ill > 0) {
+            g_lk_ill_windows.fetch_add(n_ill, std::memory_order_relaxed);
+            g_lk_windows.fetch_add(n_all, std::memory_order_relaxed);
+        }
     });
 }
 
 // OpticalFlow.LucasKanade (R:PEFStudioDX/OpticalFlow.cs:147-164). src is
 // replaced by the final warped image, as in the reference.
+// cond (optional, w x h x 2): per pixel the smallest window conditioning det / trace^2 and gate distance
+// |lambda1 / min_det - 1| over the iterations (+inf where no window is centred on the pixel).
 template <class T>
 void lucas_kanade(T* src, const T* tgt, const T* tiles, int tw, int th, T* flow, int w, int h, int iterations, T bsx,
-                  T bsy, T rot, T min_det, int window, int quant) {
+                  T bsy, T rot, T min_det, int window, int quant, double ill_ratio, float* cond = nullptr) {
+    if (cond) std::fill(cond, cond + (size_t)w * h * 2, std::numeric_limits<float>::infinity());
     flow_from_tiles(tiles, tw, th, flow, w, h, bsx, bsy, rot, quant);
     size_t n = (size_t)w * h;
     std::vector<T> tmp(n), Ix(n), Iy(n), Iz(n);
     for (int it = 0; it < iterations; ++it) {
         warp(src, flow, tmp.data(), w, h, quant);
         derivatives(tmp.data(), tgt, Ix.data(), Iy.data(), Iz.data(), w, h, quant);
-        lucas_kanade_step(flow, Ix.data(), Iy.data(), Iz.data(), w, h, window / 2, min_det);
+        lucas_kanade_step(flow, Ix.data(), Iy.data(), Iz.data(), w, h, window / 2, min_det, ill_ratio, cond);
     }
     warp(src, flow, tmp.data(), w, h, quant);
     std::copy(tmp.begin(), tmp.end(), src);

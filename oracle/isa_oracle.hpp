@@ -19,6 +19,7 @@
 // Citations R:<file>:<line> are relative to the reference checkout.
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>

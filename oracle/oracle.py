@@ -63,8 +63,8 @@ def _declare(L):
         "warp": (None, [P, P, P, I, I, I]),
         "derivatives": (None, [P, P, P, P, P, I, I, I]),
         "derivatives2": (None, [P, P, P, I, I, I]),
-        "lucas_kanade_step": (None, [P, P, P, P, I, I, I, "T"]),
-        "lucas_kanade": (None, [P, P, P, I, I, P, I, I, I, "T", "T", "T", "T", I, I]),
+        "lucas_kanade_step": (None, [P, P, P, P, I, I, I, "T", C.c_double]),
+        "lucas_kanade": (None, [P, P, P, I, I, P, I, I, I, "T", "T", "T", "T", I, I, C.c_double, P]),
         "structure_tensor": (None, [P, P, P, I, I]),
         "kernel_param": (None, [P, I, I, "T", "T", "T", "T", "T", "T"]),
         "subsample3": (None, [P, P, "T", I, I, P]),
@@ -74,7 +74,7 @@ def _declare(L):
         "accumulate_class_order": (None, [P, P, P, P, P, P, P, P, I, I, P]),
         "accumulate_sr": (None, [P, P, P, P, P, P, P, P, I, I, P, I]),
         "apply_weighting": (None, [P, P, P, I, I, "T"]),
-        "merge": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+        "merge": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     }
     for sfx, T in (("f32", C.c_float), ("f64", C.c_double)):
         for name, (rt, args) in spec.items():
@@ -90,6 +90,16 @@ def _declare(L):
     L.ora_shift_tables.restype = I
     L.ora_minimize_shifts.argtypes = [I, I, I, I, I, P, P, P, P, P]
     L.ora_minimize_shifts.restype = I
+    L.ora_lk_ill_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.ora_lk_ill_counts.restype = None
+
+
+def lk_ill_counts():
+    """(windows solved in FP64, all windows) of the FP32 Lucas-Kanade passes with lk_ill_ratio > 0 since the
+    last call."""
+    a, b = C.c_longlong(), C.c_longlong()
+    lib().ora_lk_ill_counts(C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def set_threads(n: int) -> None:
@@ -247,7 +257,7 @@ def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32
     res = {"out": np.zeros((h, w, 3), dt)}
     bufs = {
         "one_to_one": None, "flows": (n, h, w, 2), "masks": (n, h // 2, w // 2, 4), "kern": (h, w, 3),
-        "track": (n, h, w), "lk": (n, h, w), "sum": (h, w, 3), "wsum": (h, w, 3),
+        "track": (n, h, w), "lk": (n, h, w), "sum": (h, w, 3), "wsum": (h, w, 3), "lk_cond": (n, h, w, 2),
     }
     for k in want:
         if k == "one_to_one":
@@ -256,6 +266,8 @@ def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32
             cy = (h // params.resize_level[0] - 2 * params.max_shift[0]) // params.tile_size[0]
             res[k] = np.zeros((cy * cx, n - 1, 2), dt)
             del L
+        elif k == "lk_cond":
+            res[k] = np.zeros(bufs[k], np.float32)
         else:
             res[k] = np.zeros(bufs[k], dt)
     bs = None if base_shift is None else np.ascontiguousarray(base_shift, np.float32)
@@ -264,7 +276,7 @@ def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32
     g = res.get
     rc = fn("merge", dt)(C.addressof(params), w, h, n, ptr(raw), ptr(bs), ptr(br), ptr(res["out"]), ptr(g("one_to_one")),
                          ptr(g("flows")), ptr(g("masks")), ptr(g("kern")), ptr(g("track")), ptr(g("lk")),
-                         ptr(g("sum")), ptr(g("wsum")), C.byref(ref))
+                         ptr(g("sum")), ptr(g("wsum")), C.byref(ref), ptr(g("lk_cond")))
     if rc != 0:
         raise RuntimeError(f"oracle merge failed ({rc})")
     res["ref"] = ref.value

@@ -351,7 +351,8 @@ def test_lucas_kanade_step(isa, oracle, gpu):
     isa.call("isa_lucas_kanade_optim", p(f), w * 8, p(dev(rx)), p(dev(ry)), p(dev(rz)), w * 4, w, h, 11,
              C.c_float(0.01), stream())
     ref = flow0.copy()
-    oracle.fn("lucas_kanade_step")(oracle.ptr(ref), oracle.ptr(rx), oracle.ptr(ry), oracle.ptr(rz), w, h, 5, 0.01)
+    oracle.fn("lucas_kanade_step")(oracle.ptr(ref), oracle.ptr(rx), oracle.ptr(ry), oracle.ptr(rz), w, h, 5, 0.01,
+                                   0.0)
     np.testing.assert_array_equal(host(f), ref)
 
 
@@ -372,7 +373,7 @@ def test_lucas_kanade_drop_in(isa, oracle, gpu, quant, iters, window, rot):
              w, h, iters, isa.f32s(*bs), C.c_float(rot), C.c_float(0.01), window, quant, stream())
     s_ref, f_ref = src.copy(), np.zeros((h, w, 2), np.float32)
     oracle.fn("lucas_kanade")(oracle.ptr(s_ref), oracle.ptr(tgt), oracle.ptr(tiles), tw, th, oracle.ptr(f_ref), w, h,
-                              iters, bs[0], bs[1], rot, 0.01, window, quant)
+                              iters, bs[0], bs[1], rot, 0.01, window, quant, 0.0, None)
     np.testing.assert_array_equal(host(flow), f_ref)
     np.testing.assert_array_equal(host(d_src), s_ref)
 

@@ -1,12 +1,14 @@
 """GPU end-to-end: the device-resident merge (isa_merger_run) against the CPU
-oracle's pipeline (reference semantics, FP32) on seeded synthetic bursts.
+oracle's pipeline (FP32 with the same FP64 solve of ill-conditioned Lucas-Kanade
+windows, and FP64) on seeded synthetic bursts.
 
 The tile search is exact (FP64, canonical order), so the per-tile shifts agree
-to float rounding. The LK step in the merge uses separable box sums for its
-window terms (a different summation order than the reference's per-pixel
-loops), so flows agree to ~1e-4 px and a few pixels may round to a different
-integer sample in robustness/accumulation; the merged image is therefore
-compared by PSNR and by the fraction of pixels within 1e-4."""
+to float rounding. The LK step in the merge uses separable window sums (a
+different summation order than the reference's per-pixel loops) and, with the
+default 8-bit texture weights, a warp that is discontinuous in the flow, so a
+few pixels may round to a different integer sample in robustness/accumulation;
+the merged image is therefore compared by PSNR and by the fraction of pixels
+within 1e-4, and the flows by their distance from the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -59,30 +61,105 @@ def test_merge_matches_oracle(isa, oracle, gpu, case):
     # kernel parameters of the reference frame: identical operations
     kern = host(m.buffer_tensor(5, 0, 4 if c["sr"] else 3))[..., :3]
     np.testing.assert_array_equal(kern, ref["kern"])
-    # flows: Lucas-Kanade windows are box sums on the GPU (a different summation order than the
-    # reference's per-pixel loops); well-conditioned pixels agree to ~1e-5 px, ill-conditioned
-    # ones (flat or edge-only windows, where 1/sigma2 amplifies rounding) may not.
-    for f in range(1, c["n"]):
-        fl = host(m.buffer_tensor(3, f, 2))
-        d = np.abs(fl - ref["flows"][f]).max(axis=2)
-        frac = np.mean(d < 1e-3)
-        print(f"{case} frame {f}: flow |d|<1e-3 px for {frac:.4f} of pixels, median {np.median(d):.2e}")
-        assert frac > 0.93 and np.median(d) < 1e-4
+    # flows against the FP64 oracle, the accuracy target: the merger solves Lucas-Kanade windows with
+    # lambda2 / lambda1 < lk_ill_ratio (0.01) in FP64 and the rest in FP32. The reference's FP32 closed
+    # form everywhere (the FP32 oracle with lk_ill_ratio = 0) is 0.018 / 0.48 / 0.30 px RMS from FP64 on
+    # these cases with 0.6-2.6 % of flows off by more than 0.01 px; the merger is at 1e-4 / 2e-3 px and
+    # <= 0.06 %. The bars sit between the two. Agreement with the FP32 oracle pixel by pixel is pinned
+    # separately (test_lucas_kanade_matches_oracle_on_well_conditioned_windows): with the default 8-bit
+    # texture weights the warp is discontinuous in the flow, so float-level flow differences flip taps.
+    flows64 = oracle.merge(params, raw, dt=np.float64, want=("flows",))["flows"]
+    b = 8
+    d64, d32 = [], []
+    for f in range(c["n"]):
+        if f == ref["ref"]:
+            continue
+        d64.append((host(m.buffer_tensor(3, f, 2)) - flows64[f])[b:-b, b:-b])
+        d32.append(np.abs(host(m.buffer_tensor(3, f, 2)) - ref["flows"][f]).max(axis=2)[b:-b, b:-b])
+    d64, d32 = np.stack(d64), np.stack(d32)
+    rms = float(np.sqrt(np.mean(np.where(np.isfinite(d64), d64, 0.0) ** 2)))
+    far = float(np.mean(np.abs(d64).max(axis=-1) > 0.01))
+    print(f"{case}: flow vs FP64 oracle RMS {rms:.2e} px, {far * 100:.3f} % > 0.01 px; vs FP32 oracle "
+          f"{np.mean(d32 <= 1e-5) * 100:.1f} % within 1e-5 px, {np.mean(d32 <= 1e-3) * 100:.2f} % within 1e-3 px")
+    assert rms < 0.05 and far < 0.003, (rms, far)
+    assert np.mean(d32 <= 1e-3) > 0.98
     # merged image
     fin = np.isfinite(ref["out"]) & np.isfinite(out)
     assert fin.mean() > 0.999
     p = _psnr(out[fin], ref["out"][fin])
     close = np.mean(np.abs(out[fin] - ref["out"][fin]) < 1e-4)
     print(f"{case}: merged PSNR vs FP32 oracle {p:.1f} dB, {close:.4f} of values within 1e-4")
-    assert p > 45.0 and close > 0.97, (p, close)
+    assert p > 75.0 and close > 0.999, (p, close)  # measured 83-90 dB, >= 0.9998
     if case == "c0_small":
-        # error budget against the FP64 oracle: the GPU path is as close as the FP32 restatement
+        # error budget against the FP64 oracle (the reference's FP32 arithmetic everywhere, lk_ill_ratio = 0,
+        # is 41.5 dB from it on this case); the GPU's extra distance is the fp16-split high-pass moving
+        # tile shifts by ~1e-5 px, which now and then flips a rounded sample shift
         ref64 = oracle.merge(params, raw, dt=np.float64)["out"]
         f2 = fin & np.isfinite(ref64)
         p_gpu = _psnr(out[f2], ref64[f2])
         p_f32 = _psnr(ref["out"][f2], ref64[f2])
         print(f"{case}: PSNR vs FP64 oracle: GPU {p_gpu:.1f} dB, FP32 oracle {p_f32:.1f} dB")
-        assert p_gpu > p_f32 - 3.0
+        assert p_gpu > 75.0 and p_f32 > 75.0  # measured 81.1 and 83.7 dB
+
+
+def _min_filter(a, radius):
+    """Minimum over the (2 radius + 1)^2 neighbourhood, edges replicated."""
+    p = np.pad(a, radius, mode="edge")
+    out = np.full_like(a, np.inf)
+    h, w = a.shape
+    for dy in range(2 * radius + 1):
+        for dx in range(2 * radius + 1):
+            out = np.minimum(out, p[dy:dy + h, dx:dx + w])
+    return out
+
+
+def test_lucas_kanade_matches_oracle_on_well_conditioned_windows(isa, oracle, gpu):
+    """The merger's fused Lucas-Kanade (separable window sums, fast closed form, FP64 solve of
+    ill-conditioned windows) against the oracle's OpticalFlow.LucasKanade on the same inputs (the
+    merger's own finest-level tile shifts and LK images), with exact texture weights (tex_quant = 0:
+    the 8-bit weights make the warp discontinuous in the flow). Pixels are classed by the oracle's
+    window conditioning over the three iterations: a pixel is well conditioned when every window within
+    14 px (the reach of three iterations of 11 x 11 windows and 5-point stencils) has det / trace^2 >= t
+    and lies more than 0.1 % from the sigma1 gate. Well-conditioned pixels agree to 1e-5 px."""
+    w, h, n = 1024, 1024, 4
+    cfg = isa.default_synth(seed=20240602, width=w, height=h, frames=n, global_shift_max=16.0, flow_max=1.5,
+                            occluded_fraction=0.02)
+    raw_t, _ = isa.synth_burst(cfg)
+    levels = [(1, 32, 2), (2, 32, 4), (4, 32, 4)]
+    params = isa.default_params(levels=levels, tex_quant=0)
+    m = isa.BurstMerger(w, h, n, params)
+    m.run(raw_t)
+    ref = m.reference_index
+    l, T, M = levels[0]
+    cx, cy = (w - 2 * M) // T, (h - 2 * M) // T
+    r = host(m.buffer_tensor(2, 0, 2)).reshape(cx * cy, n - 1, 2).astype(np.float32)
+    lk = [np.ascontiguousarray(host(m.buffer_tensor(1, f, 1))) for f in range(n)]
+    n_ill = n_all = 0
+    for f in range(n):
+        if f == ref:
+            continue
+        print(f"lucas-kanade frame {f}", flush=True)
+        got = host(m.buffer_tensor(3, f, 2))
+        tiles = np.ascontiguousarray(oracle.get_optimal_shifts(r, n, ref, f))
+        src, flow = lk[f].copy(), np.zeros((h, w, 2), np.float32)
+        cond = np.zeros((h, w, 2), np.float32)
+        oracle.fn("lucas_kanade")(oracle.ptr(src), oracle.ptr(lk[ref]), oracle.ptr(tiles), cx, cy, oracle.ptr(flow), w,
+                                  h, params.iterations_lk, 0.0, 0.0, 0.0, params.min_det_lk, params.window_size_lk, 0,
+                                  float(params.lk_ill_ratio), oracle.ptr(cond))
+        a, b = oracle.lk_ill_counts()
+        n_ill, n_all = n_ill + a, n_all + b
+        d = np.abs(got - flow).max(axis=-1)
+        r_nb = _min_filter(np.where(np.isnan(cond[..., 0]), -1.0, cond[..., 0]), 14)
+        gate = _min_filter(cond[..., 1], 14) >= 1e-3
+        for t, frac_bar, max_bar in ((0.05, 0.999, 2e-4), (0.1, 1.0, 1e-5)):
+            sel = (r_nb >= t) & gate
+            within = float(np.mean(d[sel] <= 1e-5))
+            print(f"  t={t}: {sel.mean() * 100:.1f} % of pixels, {within * 100:.3f} % within 1e-5 px, "
+                  f"max {d[sel].max():.2e} px")
+            assert sel.mean() > 0.02 and within >= frac_bar and d[sel].max() <= max_bar
+        assert np.mean(d <= 1e-3) > 0.99
+    print(f"windows solved in FP64: {n_ill / max(n_all, 1) * 100:.2f} %")
+    assert 0.001 < n_ill / n_all < 0.1
 
 
 def test_merge_is_deterministic_and_reusable(isa, gpu):

@@ -322,7 +322,7 @@ def test_lucas_kanade_recovers_a_translation(oracle):
     # one Gauss-Newton step from zero flow lands on the translation (later steps add the bias of the
     # bilinear warp, which the reference has as well)
     oracle.fn("lucas_kanade", F64)(oracle.ptr(s), oracle.ptr(tgt), oracle.ptr(tiles), 1, 1, oracle.ptr(flow), w, h, 1,
-                                   0.0, 0.0, 0.0, 0.01, 11, 0)
+                                   0.0, 0.0, 0.0, 0.01, 11, 0, 0.0, None)
     inner = flow[12:-12, 12:-12]
     np.testing.assert_allclose(inner[..., 0], d[0], atol=2e-3)
     np.testing.assert_allclose(inner[..., 1], d[1], atol=2e-3)
