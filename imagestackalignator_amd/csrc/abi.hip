@@ -105,6 +105,7 @@ void isa_default_params(isa_params* p) {
     p->tex_quant = 1;
     p->xc_precision = ISA_XC_FP64;
     p->lk_ill_ratio = 0.01f;
+    p->prealign_max_shift = 0;
 }
 
 }  // extern "C"

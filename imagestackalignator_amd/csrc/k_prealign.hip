@@ -1,0 +1,286 @@
+// Global pre-alignment: the translation part of PreAlignment.ScanAngles (R:PEFStudioDX/PreAlignment.cs:177-261)
+// as driven by ComputePreAlignment (R:PEFStudioDX/ImageStackAlignatorController.cs:1451-1528).
+//
+// The reference correlates each tracking image with frame 0 by a full-frame cuFFT: the circular
+// cross-correlation c(s) = sum_x r(x) t((x + s) mod (W, H)) (conjugateComplexMulKernel on R2C spectra,
+// C2R, divided by W H), takes the first maximum, wraps it into (-W/2, W/2] x (-H/2, H/2] and returns
+// minus it. Here the same c(s) is evaluated directly at the lags that can hold the maximum: every lag
+// of the search radius on a box-downsampled image (the coarsest level whose radius is at most 32 lags),
+// then a window around twice the previous maximum at each finer level (15 x 15 lags, 5 x 5 at full
+// resolution). Sums are FP64 per thread and
+// reduced across blocks in a fixed order, so the result is deterministic; nothing is allocated and the
+// host never waits (the window centre of each level is read on the device).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "isa_device.h"
+#include "isa_host.h"
+
+namespace isa {
+int launch_resize_box(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int l, hipStream_t s);
+}
+
+namespace {
+
+constexpr int PA_K = 5, PA_LAGS = PA_K * PA_K;  // lags per tile: 5 x 5
+constexpr int PA_THREADS = 256;
+constexpr int PA_MAX_LEVELS = 6;               // levels 1, 2, 4, 8, 16, 32
+
+struct LagWindow {
+    int tiles;       // lag tiles per row and column (odd); lag tile (tx, ty) covers 5 x 5 lags
+    int radius;      // the window is centre + [-radius, radius]^2, radius = (5 tiles - 1) / 2
+};
+
+// Partial c(s) over pixel chunk blockIdx.x for the 25 lags of lag tile blockIdx.y: s = centre + o with
+// o in the tile. centre = scale * (*centre_in) (the previous level's maximum), or 0 without one.
+// Lags are reduced modulo the image size first, so one wrap per coordinate suffices.
+__global__ __launch_bounds__(PA_THREADS) void k_cc_partial(const float* __restrict__ a, const float* __restrict__ b,
+                                                           int pitch, int w, int h, const int* centre_in, int scale,
+                                                           LagWindow win, int chunk, double* partial) {
+    const int tile = blockIdx.y, ty = tile / win.tiles, tx = tile - ty * win.tiles;
+    int cx = 0, cy = 0;
+    if (centre_in) {
+        cx = centre_in[0] * scale;
+        cy = centre_in[1] * scale;
+    }
+    int sx0 = cx - win.radius + PA_K * tx, sy0 = cy - win.radius + PA_K * ty;
+    sx0 %= w;
+    sy0 %= h;
+    if (sx0 < 0) sx0 += w;
+    if (sy0 < 0) sy0 += h;
+    double acc[PA_LAGS];
+#pragma unroll
+    for (int l = 0; l < PA_LAGS; ++l) acc[l] = 0.0;
+    const long long npx = (long long)w * h;
+    const long long p0 = (long long)blockIdx.x * chunk, p1 = p0 + chunk < npx ? p0 + chunk : npx;
+    for (long long p = p0 + threadIdx.x; p < p1; p += PA_THREADS) {
+        const int y = (int)(p / w), x = (int)(p - (long long)y * w);
+        const double av = row_ptr(a, pitch, y)[x];
+        int xs[PA_K];
+#pragma unroll
+        for (int i = 0; i < PA_K; ++i) {
+            int xx = x + sx0 + i;
+            while (xx >= w) xx -= w;
+            xs[i] = xx;
+        }
+#pragma unroll
+        for (int j = 0; j < PA_K; ++j) {
+            int yy = y + sy0 + j;
+            while (yy >= h) yy -= h;
+            const float* br = row_ptr(b, pitch, yy);
+#pragma unroll
+            for (int i = 0; i < PA_K; ++i) acc[j * PA_K + i] += av * (double)br[xs[i]];
+        }
+    }
+    // block sum: lanes of a wave, then the waves through LDS
+    __shared__ double s_part[PA_THREADS / 64][PA_LAGS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int l = 0; l < PA_LAGS; ++l) {
+        double v = acc[l];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) s_part[wave][l] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < PA_LAGS) {
+        double v = s_part[0][threadIdx.x];
+        for (int k = 1; k < PA_THREADS / 64; ++k) v += s_part[k][threadIdx.x];
+        partial[((size_t)tile * gridDim.x + blockIdx.x) * PA_LAGS + threadIdx.x] = v;
+    }
+}
+
+// c(s) of every lag of the window (chunks summed in order), then the first maximum in raster order of
+// the window (lag y, then lag x). Writes the maximum's lag to best (2 ints, absolute lag) and its value.
+__global__ __launch_bounds__(256) void k_cc_argmax(const double* partial, int chunks, const int* centre_in, int scale,
+                                                   LagWindow win, int* best, double* best_value) {
+    const int n = win.tiles * win.tiles * PA_LAGS;
+    const int side = win.tiles * PA_K;
+    double bv = -INFINITY;
+    int bi = 0x7fffffff;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int tile = i / PA_LAGS, l = i - tile * PA_LAGS;
+        double v = 0.0;
+        for (int c = 0; c < chunks; ++c) v += partial[((size_t)tile * chunks + c) * PA_LAGS + l];
+        const int ty = tile / win.tiles, tx = tile - ty * win.tiles;
+        const int oy = ty * PA_K + l / PA_K, ox = tx * PA_K + l % PA_K;
+        const int ri = oy * side + ox;  // raster index within the window
+        if (v > bv || (v == bv && ri < bi)) {
+            bv = v;
+            bi = ri;
+        }
+    }
+    __shared__ double s_v[256];
+    __shared__ int s_i[256];
+    s_v[threadIdx.x] = bv;
+    s_i[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const double v2 = s_v[threadIdx.x + o];
+            const int i2 = s_i[threadIdx.x + o];
+            if (v2 > s_v[threadIdx.x] || (v2 == s_v[threadIdx.x] && i2 < s_i[threadIdx.x])) {
+                s_v[threadIdx.x] = v2;
+                s_i[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int cx = 0, cy = 0;
+        if (centre_in) {
+            cx = centre_in[0] * scale;
+            cy = centre_in[1] * scale;
+        }
+        const int ri = s_i[0];
+        best[0] = cx - win.radius + ri % side;
+        best[1] = cy - win.radius + ri / side;
+        *best_value = s_v[0];
+    }
+}
+
+// Final result of one frame: the full-resolution maximum s wrapped into (-W/2, W/2] x (-H/2, H/2] as
+// the reference does, returned negated (R:PEFStudioDX/PreAlignment.cs:248-258), with angle 0 and the
+// correlation value.
+__global__ void k_cc_result(const int* best, const double* value, int w, int h, float* out4) {
+    int x = best[0] % w, y = best[1] % h;
+    if (x < 0) x += w;
+    if (y < 0) y += h;
+    if (x > w / 2) x -= w;
+    if (y > h / 2) y -= h;
+    out4[0] = (float)(-x);
+    out4[1] = (float)(-y);
+    out4[2] = 0.0f;
+    out4[3] = (float)*value;
+}
+
+__global__ void k_zero4(float* out4) {
+    if (threadIdx.x < 4) out4[threadIdx.x] = 0.0f;
+}
+
+struct Plan {
+    int levels;                       // pyramid levels used: 1, 2, ..., 2^(levels-1)
+    int lw[PA_MAX_LEVELS], lh[PA_MAX_LEVELS], lpitch[PA_MAX_LEVELS];
+    size_t pyr_off[PA_MAX_LEVELS];    // offsets of the downsampled copies (level >= 1) in one pyramid
+    size_t pyr_bytes;                 // one pyramid (levels 2, 4, ...)
+    LagWindow coarse;
+    int chunk[PA_MAX_LEVELS], chunks[PA_MAX_LEVELS];
+    size_t partial_bytes, bytes;      // workspace: two pyramids, partial sums, maxima
+};
+
+size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+bool make_plan(int w, int h, int max_shift, Plan& P) {
+    if (w < 8 || h < 8 || max_shift < 1) return false;
+    // coarsest level: at most 32 lags each way for the search radius, at least 32 x 32 pixels
+    int L = 1;
+    while (L < PA_MAX_LEVELS && ((max_shift + (1 << (L - 1)) - 1) >> (L - 1)) > 32 && (w >> L) >= 32 && (h >> L) >= 32)
+        ++L;
+    P.levels = L;
+    size_t off = 0;
+    for (int i = 0; i < L; ++i) {
+        P.lw[i] = w >> i;
+        P.lh[i] = h >> i;
+        P.lpitch[i] = P.lw[i] * 4;
+        P.pyr_off[i] = off;
+        if (i > 0) off += al256((size_t)P.lw[i] * P.lh[i] * 4);
+    }
+    P.pyr_bytes = off;
+    const int lc = L - 1;
+    const int rc = ((max_shift + (1 << lc) - 1) >> lc) + 1;  // coarse radius, one lag of slack
+    const int k = std::max(0, (rc - 2 + PA_K - 1) / PA_K);
+    P.coarse.tiles = 2 * k + 1;
+    P.coarse.radius = (PA_K * P.coarse.tiles - 1) / 2;
+    size_t part = 0;
+    for (int i = 0; i < L; ++i) {
+        const long long npx = (long long)P.lw[i] * P.lh[i];
+        const int tiles = i == lc ? P.coarse.tiles * P.coarse.tiles : (i > 0 ? 9 : 1);
+        // enough blocks to fill the chip: about 2048 (tile, chunk) blocks, chunks of >= 1024 pixels
+        int chunks = (int)std::max<long long>(1, std::min<long long>((npx + 1023) / 1024, std::max(1, 2048 / tiles)));
+        P.chunk[i] = (int)((npx + chunks - 1) / chunks);
+        P.chunks[i] = (int)((npx + P.chunk[i] - 1) / P.chunk[i]);
+        part = std::max(part, (size_t)tiles * P.chunks[i] * PA_LAGS * sizeof(double));
+    }
+    P.partial_bytes = al256(part);
+    P.bytes = 2 * P.pyr_bytes + P.partial_bytes + al256((size_t)PA_MAX_LEVELS * 2 * sizeof(int)) +
+              al256(PA_MAX_LEVELS * sizeof(double));
+    return true;
+}
+
+}  // namespace
+
+namespace isa {
+
+size_t prealign_workspace(int width, int height, int max_shift) {
+    Plan P;
+    return make_plan(width, height, max_shift, P) ? P.bytes : 0;
+}
+
+// frames: host array of `n` device pointers (tracking images, pitch img_pitch bytes, 4-byte aligned).
+int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int max_shift, float* shifts4,
+             void* workspace, hipStream_t s) {
+    Plan P;
+    ISA_REQUIRE(make_plan(w, h, max_shift, P), "prealign: bad size or search radius");
+    ISA_REQUIRE(img_pitch % 4 == 0 && img_pitch >= w * 4, "prealign: bad image pitch");
+    char* ws = static_cast<char*>(workspace);
+    char* pyr_ref = ws;
+    char* pyr_trk = ws + P.pyr_bytes;
+    double* partial = (double*)(ws + 2 * P.pyr_bytes);
+    int* best = (int*)(ws + 2 * P.pyr_bytes + P.partial_bytes);
+    double* best_value = (double*)(ws + 2 * P.pyr_bytes + P.partial_bytes + al256((size_t)PA_MAX_LEVELS * 2 * sizeof(int)));
+    auto level_ptr = [&](char* pyr, const float* full, int i) -> const float* {
+        return i == 0 ? full : (const float*)(pyr + P.pyr_off[i]);
+    };
+    auto build = [&](char* pyr, const float* full) -> int {
+        for (int i = 1; i < P.levels; ++i)
+            ISA_TRY(launch_resize_box(full, img_pitch, (float*)(pyr + P.pyr_off[i]), P.lpitch[i], w, h, 1 << i, s));
+        return ISA_OK;
+    };
+    k_zero4<<<1, 64, 0, s>>>(shifts4);
+    ISA_TRY(check_launch("prealign: frame 0"));
+    if (n < 2) return ISA_OK;
+    ISA_TRY(build(pyr_ref, frames[0]));
+    const int lc = P.levels - 1;
+    for (int f = 1; f < n; ++f) {
+        ISA_TRY(build(pyr_trk, frames[f]));
+        for (int i = lc; i >= 0; --i) {
+            const bool coarse = i == lc;
+            // intermediate levels search 15 x 15 lags around twice the coarser maximum, full resolution 5 x 5
+            const LagWindow win = coarse ? P.coarse : (i > 0 ? LagWindow{3, 7} : LagWindow{1, 2});
+            const int* centre = coarse ? nullptr : best + 2 * (i + 1);
+            const float* a = level_ptr(pyr_ref, frames[0], i);
+            const float* b = level_ptr(pyr_trk, frames[f], i);
+            const int pitch = i == 0 ? img_pitch : P.lpitch[i];
+            k_cc_partial<<<dim3(P.chunks[i], win.tiles * win.tiles), PA_THREADS, 0, s>>>(
+                a, b, pitch, P.lw[i], P.lh[i], centre, 2, win, P.chunk[i], partial);
+            ISA_TRY(check_launch("prealign: correlation"));
+            k_cc_argmax<<<1, 256, 0, s>>>(partial, P.chunks[i], centre, 2, win, best + 2 * i, best_value + i);
+            ISA_TRY(check_launch("prealign: maximum"));
+        }
+        k_cc_result<<<1, 1, 0, s>>>(best, best_value, w, h, shifts4 + 4 * f);
+        ISA_TRY(check_launch("prealign: result"));
+    }
+    return ISA_OK;
+}
+
+}  // namespace isa
+
+extern "C" {
+
+size_t isa_prealign_workspace(int width, int height, int max_shift) {
+    return isa::prealign_workspace(width, height, max_shift);
+}
+
+int isa_prealign(const float* const* frames_dev, int img_pitch, int width, int height, int frames, int max_shift,
+                 float* shifts4_dev, void* workspace_dev, isa_stream_t stream) {
+    ISA_REQUIRE(frames_dev && shifts4_dev && workspace_dev && frames >= 1 && width > 0 && height > 0,
+                "isa_prealign: bad arguments");
+    ISA_REQUIRE(reinterpret_cast<uintptr_t>(workspace_dev) % 256 == 0, "isa_prealign: workspace must be 256-byte aligned");
+    for (int f = 0; f < frames; ++f) ISA_REQUIRE(frames_dev[f], "isa_prealign: null frame pointer");
+    return isa::prealign(frames_dev, img_pitch, width, height, frames, max_shift, shifts4_dev, workspace_dev,
+                         isa::to_stream(stream));
+}
+
+}  // extern "C"

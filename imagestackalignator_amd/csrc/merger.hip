@@ -63,6 +63,9 @@ bool track_mfma_supported(int tile, int M);
 int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s);
 int launch_subsample3(const uint16_t* raw, float* out, float max_val, int hw, int hh, int out_pitch, const int cfa[4],
                       hipStream_t s);
+size_t prealign_workspace(int width, int height, int max_shift);
+int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int max_shift, float* shifts4,
+             void* workspace, hipStream_t s);
 int launch_robustness(const float* ref, const float* mov, int img_pitch, float* mask, int mask_pitch, const float* flow,
                       int flow_pitch, int fw, int fh, int w, int h, float alpha, float beta, float thr, int quant,
                       hipStream_t s);
@@ -147,6 +150,10 @@ struct isa_merger {
     char* tab = nullptr;
     char* tab_host = nullptr;  // pinned staging copy
     size_t tab_bytes = 0;
+    void* pre_ws = nullptr;      // global pre-alignment (prealign_max_shift > 0): workspace,
+    float* pre_dev = nullptr;    // per-frame (x, y, angle, value) on the device
+    float* pre_host = nullptr;   // and their pinned host copy
+    hipEvent_t pre_ev[2]{};      // start and end of the pre-alignment
     hipEvent_t upload_done = nullptr;
     hipEvent_t ev[ST_COUNT + 1]{};
     hipEvent_t lev_ev[ISA_MAX_LEVELS][4]{};  // per level: start, track launch, minimizer launch, end
@@ -154,10 +161,14 @@ struct isa_merger {
     bool have_times = false;
     int ref = 0;
     int m_last = 0;  // measurements (pairs) of the last run
+    bool pre_last = false;  // the last run computed its base shifts by the pre-alignment
 
     ~isa_merger() {
         for (void* p : allocs) (void)hipFree(p);
         if (tab_host) (void)hipHostFree(tab_host);
+        if (pre_host) (void)hipHostFree(pre_host);
+        for (auto& e : pre_ev)
+            if (e) (void)hipEventDestroy(e);
         if (upload_done) (void)hipEventDestroy(upload_done);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -261,6 +272,8 @@ int validate(const isa_params& P, int w, int h, int n) {
     ISA_REQUIRE(P.iterations_lk >= 0 && P.window_size_lk >= 1 && P.erode_size >= 1 && P.erode_size <= 41,
                 "merger: LK / erode parameters out of range");
     ISA_REQUIRE(P.lk_ill_ratio >= 0.0f && P.lk_ill_ratio <= 0.25f, "merger: lk_ill_ratio must be in [0, 0.25]");
+    ISA_REQUIRE(P.prealign_max_shift >= 0 && P.prealign_max_shift <= std::min(w, h) / 2,
+                "merger: prealign_max_shift must be in [0, min(width, height) / 2]");
     for (int i = 0; i < 4; ++i) ISA_REQUIRE(P.cfa[i] >= 0 && P.cfa[i] <= 2, "merger: CFA entries must be 0, 1 or 2");
     ISA_REQUIRE(P.xc_precision >= ISA_XC_FP64 && P.xc_precision <= ISA_XC_FP32, "merger: unknown xc_precision");
     if (P.xc_precision != ISA_XC_FP64)
@@ -360,6 +373,18 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
         A(ws, isa::highpass_workspace(width, height, P.high_pass, P.high_pass_sigma));
         M->hp_ws = ws;
     }
+    if (rc == ISA_OK && P.prealign_max_shift > 0) {
+        char* pw = nullptr;
+        A(pw, isa::prealign_workspace(width, height, P.prealign_max_shift));
+        M->pre_ws = pw;
+        A(M->pre_dev, (size_t)frames * 4);
+        if (rc == ISA_OK && hipHostMalloc((void**)&M->pre_host, (size_t)frames * 4 * sizeof(float),
+                                          hipHostMallocDefault) != hipSuccess) {
+            isa::set_error("isa_merger_create: pinned host allocation failed");
+            rc = ISA_ERR_ALLOC;
+        }
+        for (int j = 0; j < 2 && rc == ISA_OK; ++j) rc = isa::check_hip(hipEventCreate(&M->pre_ev[j]), "event");
+    }
     TabLayout tl = tab_layout(M->m_alloc, frames, M->L);
     M->tab_bytes = tl.bytes;
     A(M->tab, tl.bytes);
@@ -399,6 +424,56 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const int w = M->w, h = M->h, n = M->n, n1 = n - 1, hw = w / 2, hh = h / 2;
     const size_t np = (size_t)w * h, nh = (size_t)hw * hh;
     const int wp = w * (int)sizeof(float);
+    float taps_track[99], taps_acc[99], taps_st[99];
+    int nt_track = isa_gaussian_taps(P.sigma_debayer_tracking, taps_track, 99);
+    int nt_acc = isa_gaussian_taps(P.sigma_debayer_accumulation, taps_acc, 99);
+    int nt_st = isa_gaussian_taps(P.sigma_structure_tensor, taps_st, 99);
+    ISA_REQUIRE(nt_track > 0 && nt_acc > 0 && nt_st > 0, "isa_merger_run: invalid Gaussian sigma");
+    // ---- 1. tracking images (DeBayerBWGaussWB twice per frame, sharing the debayer)
+    auto tracking_images = [&]() -> int {
+        (void)hipEventRecord(M->ev[0], s);
+        for (int f = 0; f < n; ++f) {
+            ISA_TRY(isa::launch_raw_to_gray(raw_dev + (size_t)f * np, wp / 2, M->gray, wp, w, h, P.black, P.white,
+                                            P.camera_white, P.cfa, P.green_channel_only, s));
+            ISA_TRY(isa::launch_gauss1(M->gray, wp, M->lk + (size_t)f * np, wp, M->gtmp, w, h, taps_acc, nt_acc, s));
+            if (P.use_high_pass) {
+                // the grey image is a convex combination of clamped sRGB values: within [0, 1] up to
+                // rounding, so 2 bounds it
+                ISA_TRY(isa::highpass(M->gray, wp, w, h, P.high_pass, P.high_pass_sigma, M->hp_ws, M->hp_tables, 2.0f,
+                                      s));
+                M->hp_tables = true;
+            }
+            ISA_TRY(isa::launch_gauss1(M->gray, wp, M->trk + (size_t)f * np, wp, M->gtmp, w, h, taps_track, nt_track,
+                                       s));
+        }
+        (void)hipEventRecord(M->ev[1], s);
+        return ISA_OK;
+    };
+    // ---- 1b. ComputePreAlignment (R:...Controller.cs:1451-1528) when asked and no base shifts are given: the
+    // initial shifts come from the tracking images, so they are computed first and read back once
+    const bool pre = P.prealign_max_shift > 0 && base_shift == nullptr;
+    std::vector<float> pre_shift, pre_rot;
+    if (pre) {
+        ISA_TRY(tracking_images());
+        std::vector<const float*> fptr(n);
+        for (int f = 0; f < n; ++f) fptr[f] = M->trk + (size_t)f * np;
+        (void)hipEventRecord(M->pre_ev[0], s);
+        ISA_TRY(isa::prealign(fptr.data(), wp, w, h, n, P.prealign_max_shift, M->pre_dev, M->pre_ws, s));
+        (void)hipEventRecord(M->pre_ev[1], s);
+        ISA_TRY(isa::check_hip(hipMemcpyAsync(M->pre_host, M->pre_dev, (size_t)n * 4 * sizeof(float),
+                                              hipMemcpyDeviceToHost, s),
+                               "isa_merger_run: pre-alignment read-back"));
+        ISA_TRY(isa::check_hip(hipStreamSynchronize(s), "isa_merger_run: pre-alignment read-back"));
+        pre_shift.resize(2 * n);
+        pre_rot.resize(n);
+        for (int f = 0; f < n; ++f) {
+            pre_shift[2 * f] = M->pre_host[4 * f];
+            pre_shift[2 * f + 1] = M->pre_host[4 * f + 1];
+            pre_rot[f] = M->pre_host[4 * f + 2];
+        }
+        base_shift = pre_shift.data();
+        base_rotation = pre_rot.data();
+    }
     // ---- host tables (pre-alignment store, shift collection), uploaded through pinned memory
     Store st = make_store(n, base_shift, base_rotation, P.reference_index);
     ISA_REQUIRE(st.ref >= 0 && st.ref < n, "isa_merger_run: reference index out of range");
@@ -477,28 +552,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     const float* const* d_rframes = (const float* const*)(M->tab + tl.rframes);
     const float* const* d_frames = (const float* const*)(M->tab + tl.frames);
 
-    float taps_track[99], taps_acc[99], taps_st[99];
-    int nt_track = isa_gaussian_taps(P.sigma_debayer_tracking, taps_track, 99);
-    int nt_acc = isa_gaussian_taps(P.sigma_debayer_accumulation, taps_acc, 99);
-    int nt_st = isa_gaussian_taps(P.sigma_structure_tensor, taps_st, 99);
-    ISA_REQUIRE(nt_track > 0 && nt_acc > 0 && nt_st > 0, "isa_merger_run: invalid Gaussian sigma");
-
-    (void)hipEventRecord(M->ev[0], s);
-    // ---- 1. tracking images (DeBayerBWGaussWB twice per frame, sharing the debayer)
-    for (int f = 0; f < n; ++f) {
-        ISA_TRY(isa::launch_raw_to_gray(raw_dev + (size_t)f * np, wp / 2, M->gray, wp, w, h, P.black, P.white,
-                                        P.camera_white, P.cfa, P.green_channel_only, s));
-        ISA_TRY(isa::launch_gauss1(M->gray, wp, M->lk + (size_t)f * np, wp, M->gtmp, w, h, taps_acc, nt_acc, s));
-        if (P.use_high_pass) {
-            // the grey image is a convex combination of clamped sRGB values: within [0, 1] up to
-            // rounding, so 2 bounds it
-            ISA_TRY(isa::highpass(M->gray, wp, w, h, P.high_pass, P.high_pass_sigma, M->hp_ws, M->hp_tables, 2.0f,
-                                  s));
-            M->hp_tables = true;
-        }
-        ISA_TRY(isa::launch_gauss1(M->gray, wp, M->trk + (size_t)f * np, wp, M->gtmp, w, h, taps_track, nt_track, s));
-    }
-    (void)hipEventRecord(M->ev[1], s);
+    if (!pre) ISA_TRY(tracking_images());
     // ---- 2. TrackPatches: levels coarse -> fine
     const size_t mtiles = (size_t)M->mcx * M->mcy, mstride = mtiles * 2;
     ISA_TRY(isa::check_hip(hipMemsetAsync(M->shifts, 0, (size_t)m * mstride * sizeof(float), s), "memset"));
@@ -647,6 +701,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     }
     (void)hipEventRecord(M->ev[8], s);
     M->have_times = true;
+    M->pre_last = pre;
     (void)n1;
     (void)kch;
     return isa::check_launch("isa_merger_run");
@@ -675,6 +730,11 @@ int isa_merger_stage_ms(isa_merger* M, float* ms, int capacity) {
     t[ST_MINIMIZE] = mn;
     int k = 0;
     for (; k < ST_COUNT && k < capacity; ++k) ms[k] = t[k];
+    if (k < capacity) {  // stage 8: global pre-alignment (0 when the base shifts came from the caller)
+        float pa = 0.0f;
+        if (M->pre_last) (void)hipEventElapsedTime(&pa, M->pre_ev[0], M->pre_ev[1]);
+        ms[k++] = pa;
+    }
     return k;
 }
 
@@ -730,6 +790,9 @@ int isa_merger_buffer(isa_merger* M, int which, int index, void** ptr, int* pitc
             return set(M->shifts, (int)((size_t)M->mcx * M->mcy * 8), M->cx[0] * M->cy[0], M->m_alloc);
         }
         case 9: return set(M->trk_half, (w / 2) * 12, w / 2, h / 2);
+        case 10:
+            ISA_REQUIRE(M->pre_dev, "isa_merger_buffer: no pre-alignment (prealign_max_shift is 0)");
+            return set(M->pre_dev, 16, 1, n);
     }
     isa::set_error("isa_merger_buffer: unknown buffer %d", which);
     return ISA_ERR_ARG;

@@ -7,7 +7,7 @@ import ctypes as C
 from . import IsaParams, IsaSynthConfig, call, lib
 
 STAGES = ["tracking_images", "track_patches", "minimizer", "prepare_accumulation", "lucas_kanade", "robustness",
-          "accumulate", "final_weighting"]
+          "accumulate", "final_weighting", "prealign"]
 
 
 def _torch():

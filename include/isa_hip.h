@@ -16,6 +16,7 @@
  *    are the calls that return values in host memory, which synchronise the
  *    stream: isa_merge_forward (loss), isa_merge_backward (dtheta),
  *    isa_merge_refine_step, isa_merger_stage_ms / _level_ms / _exact_items,
+ *    isa_merger_run with the pre-alignment (it reads the initial shifts back),
  *    isa_synth_burst and isa_synth_ground_truth (which also stage their scene
  *    tables in memory of their own). isa_merger_create allocates every buffer
  *    of the whole-path merger once, as the reference's controller did.
@@ -112,6 +113,9 @@ typedef struct isa_params {
     float lk_ill_ratio;               /* 0.01: Lucas-Kanade windows whose FP32 sums give det < ratio * trace^2
                                          (lambda2 / lambda1 below the ratio) are solved in FP64; 0: the reference's
                                          FP32 closed form everywhere. No counterpart in the reference. */
+    int prealign_max_shift;           /* 0 (SkipPreAlignment): base shifts from the caller; > 0: when the caller
+                                         passes none, ComputePreAlignment's translation search (isa_prealign) within
+                                         this radius in pixels sets them */
 } isa_params;
 
 void isa_default_params(isa_params* p);
@@ -330,6 +334,18 @@ int isa_compute_robustness_mask(const float* ref3_dev, const float* moved3_dev, 
 int isa_erode_border4(const float* in4_dev, int in_pitch, float* out4_dev, int out_pitch, int width, int height,
                       int size, isa_stream_t stream);
 
+/* Global pre-alignment, the translation part of PreAlignment.ScanAngles (R:PEFStudioDX/PreAlignment.cs:177-261) as
+ * ComputePreAlignment runs it (R:PEFStudioDX/ImageStackAlignatorController.cs:1451-1528): for every frame f >= 1 the
+ * lag s maximising the circular cross-correlation sum_x img_0(x) img_f((x + s) mod (W, H)) of the tracking images
+ * (the reference's cuFFT R2C, conjugateComplexMulKernel, C2R, / (W H), MaxIndex), searched within |s| <= max_shift
+ * per axis, wrapped into (-W/2, W/2] x (-H/2, H/2]. shifts4_dev (device, frames x float4): (-s.x, -s.y, angle 0,
+ * correlation value) per frame, frame 0 all zero: the initialShifts the PreAlignmentStore is built from
+ * (isa_merger_run's base_shift). The rotation scan is not implemented (angle 0). frames_dev: host array of device
+ * pointers; workspace_dev: isa_prealign_workspace() bytes, 256-byte aligned. No allocation, no host synchronisation. */
+size_t isa_prealign_workspace(int width, int height, int max_shift);
+int isa_prealign(const float* const* frames_dev, int img_pitch, int width, int height, int frames, int max_shift,
+                 float* shifts4_dev, void* workspace_dev, isa_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Whole-path entry points (device-resident; the fast path).
  * --------------------------------------------------------------------- */
@@ -341,7 +357,10 @@ int isa_merger_destroy(isa_merger* m);
 /* Runs SkipPreAlignment-style tracking-image preparation, TrackPatches, PrepareAccumulation, Accumulate and the
  * final ApplyWeighting (R:...Controller.cs:1530-1569, 1601-1746, 1795-2002). raw_dev: frames x height x width u16,
  * contiguous. base_shift (2*frames floats, host, may be NULL = 0) and base_rotation (frames floats, host, degrees,
- * may be NULL) play the role of the PreAlignmentStore. out3_dev: width x height float3 (pitch out_pitch). */
+ * may be NULL) are the initial shifts the PreAlignmentStore is built from. With base_shift NULL and
+ * params.prealign_max_shift > 0 they are computed as ComputePreAlignment does (isa_prealign on the tracking images,
+ * R:...Controller.cs:1451-1528); the run then reads them back once (one stream synchronisation).
+ * out3_dev: width x height float3 (pitch out_pitch). */
 int isa_merger_run(isa_merger* m, const uint16_t* raw_dev, const float* base_shift, const float* base_rotation,
                    float* out3_dev, int out_pitch, isa_stream_t stream);
 /* Device pointer and geometry of an internal buffer, for inspection:
@@ -351,11 +370,12 @@ int isa_merger_run(isa_merger* m, const uint16_t* raw_dev, const float* base_shi
  *   6 accumulated sum (float3)                        7 accumulated weight (float3)
  *   8 per-pair shifts of level `index` after minimisation, [m][tiles] float2
  *   9 half-resolution debayer of frame `index` (float3)
+ *  10 pre-alignment result of the last run, frames x float4 (x, y, angle, value); needs prealign_max_shift > 0
  * Returns ISA_ERR_ARG for an unknown buffer. */
 int isa_merger_buffer(isa_merger* m, int which, int index, void** ptr, int* pitch, int* width, int* height);
 /* Reference index chosen by the last run, and per-stage milliseconds of the last run (HIP events): stages are
  * 0 tracking images, 1 track patches, 2 minimizer, 3 prepare accumulation, 4 LK, 5 robustness, 6 accumulate,
- * 7 final weighting. Returns the number of stages written. */
+ * 7 final weighting, 8 global pre-alignment. Returns the number of stages written. */
 int isa_merger_reference_index(isa_merger* m);
 int isa_merger_stage_ms(isa_merger* m, float* ms, int capacity);
 /* Milliseconds of the last run at pyramid level `level` (0 = finest): ms[0] shift up-sampling and level

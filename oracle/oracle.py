@@ -270,6 +270,14 @@ def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32
             res[k] = np.zeros(bufs[k], np.float32)
         else:
             res[k] = np.zeros(bufs[k], dt)
+    if base_shift is None and getattr(params, "prealign_max_shift", 0) > 0:
+        # ComputePreAlignment (R:PEFStudioDX/ImageStackAlignatorController.cs:1451-1528): initial shifts from the
+        # tracking images (DeBayerBWGaussWB with the high-pass), global maximum of the FFT cross-correlation
+        trk = [tracking_image(raw[f], params, params.use_high_pass, params.sigma_debayer_tracking, dt) for f in range(n)]
+        init = prealign(trk)
+        res["prealign"] = init
+        base_shift = init[:, :2].ravel()
+        base_rot = init[:, 2]
     bs = None if base_shift is None else np.ascontiguousarray(base_shift, np.float32)
     br = None if base_rot is None else np.ascontiguousarray(base_rot, np.float32)
     ref = C.c_int(-1)
@@ -281,6 +289,58 @@ def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32
         raise RuntimeError(f"oracle merge failed ({rc})")
     res["ref"] = ref.value
     return res
+
+
+# ---------------------------------------------------------------------------
+# Global pre-alignment (translation): PreAlignment.ScanAngles at a single angle
+# (R:PEFStudioDX/PreAlignment.cs:177-261) as ComputePreAlignment drives it
+# (R:PEFStudioDX/ImageStackAlignatorController.cs:1451-1528), with numpy's FFT in double.
+def cross_correlation(ref: np.ndarray, trk: np.ndarray) -> np.ndarray:
+    """C2R(conj(R2C(ref)) * R2C(trk)) / (W H): c[s] = sum_x ref(x) trk((x + s) mod (W, H)), indexed [sy, sx]
+    (conjugateComplexMulKernel conjugates its first operand, R:Kernels/kernel.cu:394-411)."""
+    h, w = ref.shape
+    a = np.fft.rfft2(np.asarray(ref, np.float64))
+    b = np.fft.rfft2(np.asarray(trk, np.float64))
+    return np.fft.irfft2(np.conj(a) * b, s=(h, w))
+
+
+def prealign(frames) -> np.ndarray:
+    """initialShifts of every frame against frame 0: (n, 4) = (-sx, -sy, angle 0, c[s]) for the first maximum s
+    of the circular cross-correlation in memory order, wrapped into (-W/2, W/2] x (-H/2, H/2]
+    (R:PEFStudioDX/PreAlignment.cs:248-258); frame 0 gets zeros."""
+    frames = [np.asarray(f, np.float64) for f in frames]
+    h, w = frames[0].shape
+    out = np.zeros((len(frames), 4))
+    for i in range(1, len(frames)):
+        cc = cross_correlation(frames[0], frames[i])
+        idx = int(np.argmax(cc))
+        y, x = divmod(idx, w)
+        v = cc[y, x]
+        if x > w // 2:
+            x -= w
+        if y > h // 2:
+            y -= h
+        out[i] = (-x, -y, 0.0, v)
+    return out
+
+
+def prealignment_store(initial_shifts) -> tuple:
+    """PreAlignmentStore (R:PEFStudioDX/PreAlignmentStore.cs:36-110): the reference index (the frame minimising the
+    length of the summed shifts seen from it) and each frame's shift relative to it, GetShift(ref, f)."""
+    s = np.asarray(initial_shifts, np.float64)[:, :2]
+    n = len(s)
+    basic = s[1:] - s[:-1]
+    best, best_d = -1, np.inf
+    for i in range(n - 1):
+        tot = -basic[:i].sum(axis=0) + basic[i:].sum(axis=0)
+        d = float(np.hypot(*tot))
+        if d < best_d:
+            best, best_d = i, d
+    best = max(best, 0)
+    per = np.zeros((n, 2))
+    for i in range(n):
+        per[i] = -basic[i:best].sum(axis=0) + basic[best:i].sum(axis=0)
+    return best, per.astype(np.float32)
 
 
 # ---------------------------------------------------------------------------

@@ -208,3 +208,50 @@ def test_merger_accumulation_is_bit_exact_on_its_own_inputs(isa, oracle, gpu):
     np.testing.assert_array_equal(got_s, res["accumulate_class_order"][0])
     np.testing.assert_array_equal(got_w, res["accumulate_class_order"][1])
     assert within_reorder_bound(got_s, got_w, *res["accumulate"], 0.0, 0.0, 25 * n)
+
+
+def test_prealigned_merge_recovers_large_shifts(isa, oracle, gpu):
+    """Global shifts of up to 48 px on a pyramid that reaches about 26 px (levels 1/2/4, max shift 2/4/4):
+    with prealign_max_shift the merger first runs ComputePreAlignment's translation search. Its initial
+    shifts equal the oracle's FFT maxima (g_f - g_0 to the nearest pixel, g the synthesiser's truth), the
+    merged flows recover g_ref - g_f to a few hundredths of a pixel, and the merge matches the oracle's."""
+    w, h, n = 512, 384, 5
+    cfg = isa.default_synth(seed=4242, width=w, height=h, frames=n, global_shift_max=48.0, flow_max=0.0,
+                            occluded_fraction=0.0)
+    raw_t, truth = isa.synth_burst(cfg)
+    truth = np.asarray(truth, np.float64)
+    raw = host(raw_t).view(np.uint16)
+    levels = [(1, 32, 2), (2, 32, 4), (4, 16, 4)]
+    assert np.abs(truth).max() > 30  # beyond the pyramid's reach
+    params = isa.default_params(levels=levels, prealign_max_shift=64)
+    m = isa.BurstMerger(w, h, n, params)
+    out = host(m.run(raw_t))
+    pre = host(m.buffer_tensor(10, 0, 4)).reshape(n, 4)
+    ref = oracle.merge(params, raw, want=("flows",))
+    print("prealign (GPU):", pre[:, :2].tolist(), "truth g_f - g_0:", (truth - truth[0]).round(2).tolist())
+    np.testing.assert_array_equal(pre[:, :3], ref["prealign"][:, :3])
+    assert np.all(np.abs(pre[:, :2] - (truth - truth[0])) <= 1.0)
+    r = m.reference_index
+    assert r == ref["ref"]
+    b = 24
+    for f in range(n):
+        if f == r:
+            continue
+        fl = host(m.buffer_tensor(3, f, 2))[b:-b, b:-b].reshape(-1, 2)
+        med = np.median(fl, axis=0)
+        want = truth[r] - truth[f]
+        print(f"frame {f}: median flow {med.round(3).tolist()} truth {want.round(3).tolist()}")
+        assert np.all(np.abs(med - want) < 0.05)
+    fin = np.isfinite(ref["out"]) & np.isfinite(out)
+    p = _psnr(out[fin], ref["out"][fin])
+    # the same burst without the pre-alignment (reference frame 0): the tile search cannot reach the
+    # largest shifts, so those frames' flows miss the truth by pixels
+    m0 = isa.BurstMerger(w, h, n, isa.default_params(levels=levels))
+    m0.run(raw_t)
+    r0 = m0.reference_index
+    miss = max(float(np.abs(np.median(host(m0.buffer_tensor(3, f, 2))[b:-b, b:-b].reshape(-1, 2), axis=0)
+                            - (truth[r0] - truth[f])).max()) for f in range(n) if f != r0)
+    st = m.stage_ms()
+    print(f"merged PSNR vs FP32 oracle {p:.1f} dB; largest median flow error without pre-alignment {miss:.2f} px; "
+          f"pre-alignment {st['prealign']:.3f} ms")
+    assert p > 75.0 and miss > 5.0 and st["prealign"] > 0
