@@ -432,40 +432,68 @@ __global__ __launch_bounds__(256) void k_accumulate_frames_wide(float* sum, floa
     }
 }
 
-// accumulateImagesSuperRes (R:Kernels/DeBayerKernels.cu:378-468): w x h output of the central half crop.
+// accumulateImagesSuperRes (R:Kernels/DeBayerKernels.cu:378-468) over several frames in one pass, for an output grid
+// of g.ow x g.oh pixels at half the raw pixel spacing whose pixel 0 lies g.ox, g.oy output pixels from the raw
+// frame's origin. The reference's form is ow x oh = w x h with offset (w/2, h/2), the central half of the frame;
+// the full-frame 2x merge is 2w x 2h with offset 0. Per output pixel the kernel parameters (texture at posX,
+// clamp) and the 25 weights are frame independent and computed once; per frame the flow (texture, mirror) gives
+// the sample shift round(2 flow), and the 25 taps are visited in the reference's raster order. Raw and
+// certainty coordinates are clamped to the rows and columns the output grid covers
+// ([ox/2, ox/2 + ow/2 - 1] x [oy/2, oy/2 + oh/2 - 1]). A null flow is the zero flow.
+struct SrGeom {
+    int ow, oh, ox, oy;
+};
+
 template <bool DIVN>
-__global__ void k_accumulate_sr(const uint16_t* __restrict__ raw, float* sum, float* wsum, const float* mask,
-                                const float* kern4, const float* flow, int w, int h, int out_pitch, int mask_pitch,
-                                int kern_pitch, int flow_pitch, AccArgs a, int quant) {
+__global__ void k_accumulate_sr_frames(float* sum, float* wsum, const float* kern4, int w, int h, int out_pitch,
+                                       int mask_pitch, int kern_pitch, int flow_pitch, FramesArgs fa, SrGeom g,
+                                       AccArgs a, int quant) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x < 1 || y < 1 || x >= w - 1 || y >= h - 1) return;
+    if (x < 1 || y < 1 || x >= g.ow - 1 || y >= g.oh - 1) return;
     float* ps = row_ptr(sum, out_pitch, y) + x * 3;
     float* pw = row_ptr(wsum, out_pitch, y) + x * 3;
     float acc[3] = {ps[0], ps[1], ps[2]}, ws[3] = {pw[0], pw[1], pw[2]};
-    float posx = div_n(((float)x + 0.5f + (float)(w / 2)) / 2.0f, 1.0 / (double)w);
-    float posy = div_n(((float)y + 0.5f + (float)(h / 2)) / 2.0f, 1.0 / (double)h);
-    float k[4], f[2];
+    const float posx = div_n(((float)x + 0.5f + (float)g.ox) / 2.0f, 1.0 / (double)w);
+    const float posy = div_n(((float)y + 0.5f + (float)g.oy) / 2.0f, 1.0 / (double)h);
+    float k[4];
     tex_linear_all<4>(kern4, kern_pitch, w, h, posx, posy, ISA_ADDR_CLAMP, quant, k);
-    tex_linear_all<2>(flow, flow_pitch, w, h, posx, posy, ISA_ADDR_MIRROR, quant, f);
-    int sx = (int)roundf(f[0] * 2.0f), sy = (int)roundf(f[1] * 2.0f);
+    float wt[25];
+#pragma unroll
     for (int py = -2; py <= 2; ++py)
-        for (int px = -2; px <= 2; ++px) {
-            int ppsx = x + px + sx + w / 2, ppsy = y + py + sy + h / 2;
-            int ppx = x + px + w / 2, ppy = y + py + h / 2;
-            ppsx = clampi(ppsx / 2, 0 + w / 4, w / 2 - 1 + w / 4);
-            ppsy = clampi(ppsy / 2, 0 + h / 4, h / 2 - 1 + h / 4);
-            ppx = clampi(ppx / 2, 0 + w / 4, w / 2 - 1 + w / 4);
-            ppy = clampi(ppy / 2, 0 + h / 4, h / 2 - 1 + h / 4);
-            int site = a.cfa.at(ppsy, ppsx);
-            int c = site == ISA_CFA_GREEN ? 1 : site == ISA_CFA_RED ? 0 : site == ISA_CFA_BLUE ? 2 : -1;
-            if (c < 0) continue;
-            float wgt = kernel_weight(px, py, k[0], k[1], k[2]);
-            float v = norm_raw<DIVN>((float)raw[(size_t)ppsy * w + ppsx], c, a);
-            float cert = row_ptr(mask, mask_pitch, ppy / 2)[(ppx / 2) * 4 + c];
-            if (!isfinite(cert)) cert = 0.0f;
-            acc[c] += v * wgt * cert;
-            ws[c] += wgt * cert;
+#pragma unroll
+        for (int px = -2; px <= 2; ++px) wt[(py + 2) * 5 + px + 2] = kernel_weight(px, py, k[0], k[1], k[2]);
+    const int cx0 = g.ox / 2, cx1 = g.ox / 2 + g.ow / 2 - 1, cy0 = g.oy / 2, cy1 = g.oy / 2 + g.oh / 2 - 1;
+    for (int f = 0; f < fa.nframes; ++f) {
+        int sx = 0, sy = 0;
+        if (fa.flow[f]) {
+            float fl[2];
+            tex_linear_all<2>(fa.flow[f], flow_pitch, w, h, posx, posy, ISA_ADDR_MIRROR, quant, fl);
+            sx = (int)roundf(fl[0] * 2.0f);
+            sy = (int)roundf(fl[1] * 2.0f);
         }
+        const uint16_t* __restrict__ raw = fa.raw[f];
+        const float* mask = fa.mask[f];
+#pragma unroll
+        for (int py = -2; py <= 2; ++py) {
+            const int ppsy = clampi((y + py + sy + g.oy) / 2, cy0, cy1), ppy = clampi((y + py + g.oy) / 2, cy0, cy1);
+            const uint16_t* rrow = raw + (size_t)ppsy * w;
+            const float* mrow = row_ptr(mask, mask_pitch, ppy / 2);
+#pragma unroll
+            for (int px = -2; px <= 2; ++px) {
+                const int ppsx = clampi((x + px + sx + g.ox) / 2, cx0, cx1), ppx = clampi((x + px + g.ox) / 2, cx0, cx1);
+                const int c = site_channel(a.cfa.at(ppsy, ppsx));
+                if (c < 0) continue;
+                const float wgt = wt[(py + 2) * 5 + px + 2];
+                const float v = norm_raw<DIVN>((float)rrow[ppsx], c, a);
+                float cert = mrow[(ppx / 2) * 4 + c];
+                if (!isfinite(cert)) cert = 0.0f;
+                const float t = v * wgt * cert, u = wgt * cert;
+                if (c == 0) { acc[0] += t; ws[0] += u; }
+                else if (c == 1) { acc[1] += t; ws[1] += u; }
+                else { acc[2] += t; ws[2] += u; }
+            }
+        }
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         ps[c] = acc[c];
@@ -555,15 +583,36 @@ int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, 
     return ISA_OK;
 }
 
+// Super-resolution accumulation of `nframes` frames (host arrays of device pointers, in accumulation order) onto
+// an out_w x out_h grid with offset (ox, oy); launches of up to ACC_MAXF frames run in order on the stream.
+int launch_accumulate_sr_frames(float* sum, float* wsum, const float* kern4, int w, int h, int out_pitch,
+                                int mask_pitch, int kern_pitch, int flow_pitch, const uint16_t* const* raw,
+                                const float* const* mask, const float* const* flow, int nframes, int out_w, int out_h,
+                                int ox, int oy, const float white[3], const float black[3], const int cfa[4], int quant,
+                                hipStream_t s) {
+    const AccArgs a = make_acc(white, black, cfa);
+    auto kern_fn = a.white_int ? k_accumulate_sr_frames<true> : k_accumulate_sr_frames<false>;
+    const SrGeom g{out_w, out_h, ox, oy};
+    for (int f0 = 0; f0 < nframes; f0 += ACC_MAXF) {
+        FramesArgs fa{};
+        fa.nframes = std::min(ACC_MAXF, nframes - f0);
+        for (int i = 0; i < fa.nframes; ++i) {
+            fa.raw[i] = raw[f0 + i];
+            fa.mask[i] = mask[f0 + i];
+            fa.flow[i] = flow[f0 + i];
+        }
+        kern_fn<<<dim3(div_up(out_w, 32), div_up(out_h, 8)), dim3(32, 8), 0, s>>>(
+            sum, wsum, kern4, w, h, out_pitch, mask_pitch, kern_pitch, flow_pitch, fa, g, a, quant);
+        ISA_TRY(check_launch("accumulate_superres"));
+    }
+    return ISA_OK;
+}
+
 int launch_accumulate_sr(const uint16_t* raw, float* sum, float* wsum, const float* mask, const float* kern4,
                          const float* flow, const float white[3], const float black[3], int w, int h, int out_pitch,
                          int mask_pitch, int kern_pitch, int flow_pitch, const int cfa[4], int quant, hipStream_t s) {
-    const AccArgs a = make_acc(white, black, cfa);
-    auto kern_fn = a.white_int ? k_accumulate_sr<true> : k_accumulate_sr<false>;
-    kern_fn<<<dim3(div_up(w, 16), div_up(h, 16)), dim3(16, 16), 0, s>>>(raw, sum, wsum, mask, kern4, flow, w, h,
-                                                                        out_pitch, mask_pitch, kern_pitch, flow_pitch, a,
-                                                                        quant);
-    return check_launch("accumulate_superres");
+    return launch_accumulate_sr_frames(sum, wsum, kern4, w, h, out_pitch, mask_pitch, kern_pitch, flow_pitch, &raw,
+                                       &mask, &flow, 1, w, h, w / 2, h / 2, white, black, cfa, quant, s);
 }
 
 }  // namespace isa
@@ -616,6 +665,28 @@ int isa_accumulate_frames(const uint16_t* const* raw_devs, const float* const* m
     return isa::launch_accumulate_frames(sum3_dev, weight3_dev, kernel3_dev, dim_x, dim_y, out_pitch, mask_pitch,
                                          flow_pitch, raw_devs, mask4_devs, flow2_devs, frames, white, black, cfa,
                                          isa::to_stream(stream));
+}
+
+int isa_accumulate_frames_superres(const uint16_t* const* raw_devs, const float* const* mask4_devs,
+                                   const float* const* flow2_devs, int frames, float* sum3_dev, float* weight3_dev,
+                                   const float* kernel4_dev, const float white[3], const float black[3], int dim_x,
+                                   int dim_y, int out_w, int out_h, int offset_x, int offset_y, int out_pitch,
+                                   int mask_pitch, int kernel_pitch, int flow_pitch, const int cfa[4], int tex_quant,
+                                   isa_stream_t stream) {
+    ISA_REQUIRE(raw_devs && mask4_devs && flow2_devs && frames >= 0 && sum3_dev && weight3_dev && kernel4_dev &&
+                    dim_x > 0 && dim_y > 0 && out_w > 2 && out_h > 2 && offset_x >= 0 && offset_y >= 0 &&
+                    cfa_valid(cfa),
+                "isa_accumulate_frames_superres: bad arguments");
+    // every raw and certainty coordinate stays inside the frame
+    ISA_REQUIRE(offset_x / 2 + out_w / 2 <= dim_x && offset_y / 2 + out_h / 2 <= dim_y,
+                "isa_accumulate_frames_superres: the output grid must lie inside the frame");
+    ISA_REQUIRE(out_pitch >= out_w * 12, "isa_accumulate_frames_superres: bad output pitch");
+    for (int f = 0; f < frames; ++f)
+        ISA_REQUIRE(raw_devs[f] && mask4_devs[f], "isa_accumulate_frames_superres: null frame pointer");
+    return isa::launch_accumulate_sr_frames(sum3_dev, weight3_dev, kernel4_dev, dim_x, dim_y, out_pitch, mask_pitch,
+                                            kernel_pitch, flow_pitch, raw_devs, mask4_devs, flow2_devs, frames, out_w,
+                                            out_h, offset_x, offset_y, white, black, cfa, tex_quant,
+                                            isa::to_stream(stream));
 }
 
 int isa_accumulate_images_superres(const uint16_t* raw_dev, float* sum3_dev, float* weight3_dev, const float* mask4_dev,

@@ -73,9 +73,11 @@ int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, 
                              int flow_pitch, const uint16_t* const* raw, const float* const* mask,
                              const float* const* flow, int nframes, const float white[3], const float black[3],
                              const int cfa[4], hipStream_t s);
-int launch_accumulate_sr(const uint16_t* raw, float* sum, float* wsum, const float* mask, const float* kern4,
-                         const float* flow, const float white[3], const float black[3], int w, int h, int out_pitch,
-                         int mask_pitch, int kern_pitch, int flow_pitch, const int cfa[4], int quant, hipStream_t s);
+int launch_accumulate_sr_frames(float* sum, float* wsum, const float* kern4, int w, int h, int out_pitch,
+                                int mask_pitch, int kern_pitch, int flow_pitch, const uint16_t* const* raw,
+                                const float* const* mask, const float* const* flow, int nframes, int out_w, int out_h,
+                                int ox, int oy, const float white[3], const float black[3], const int cfa[4], int quant,
+                                hipStream_t s);
 }  // namespace isa
 
 extern "C" int isa_compute_structure_tensor(const float*, const float*, int, float*, int, int, int, isa_stream_t);
@@ -106,6 +108,7 @@ enum Stage { ST_TRACKIMG = 0, ST_TRACK, ST_MINIMIZE, ST_PREPARE, ST_LK, ST_ROBUS
 
 struct isa_merger {
     int w = 0, h = 0, n = 0;
+    int ow = 0, oh = 0;  // output size: w x h, or 2w x 2h for the full-frame 2x merge
     isa_params P{};
     int L = 0, mcx = 0, mcy = 0;
     int cx[ISA_MAX_LEVELS]{}, cy[ISA_MAX_LEVELS]{};
@@ -271,6 +274,7 @@ int validate(const isa_params& P, int w, int h, int n) {
     }
     ISA_REQUIRE(P.iterations_lk >= 0 && P.window_size_lk >= 1 && P.erode_size >= 1 && P.erode_size <= 41,
                 "merger: LK / erode parameters out of range");
+    ISA_REQUIRE(P.super_resolution >= 0 && P.super_resolution <= 2, "merger: super_resolution must be 0, 1 or 2");
     ISA_REQUIRE(P.lk_ill_ratio >= 0.0f && P.lk_ill_ratio <= 0.25f, "merger: lk_ill_ratio must be in [0, 0.25]");
     ISA_REQUIRE(P.prealign_max_shift >= 0 && P.prealign_max_shift <= std::min(w, h) / 2,
                 "merger: prealign_max_shift must be in [0, min(width, height) / 2]");
@@ -337,8 +341,11 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
     A(M->flows, (size_t)frames * np * 2);
     A(M->lk_src, np);
     A(M->lk_work, 4 * np);
-    A(M->sum, np * 3);
-    A(M->wsum, np * 3);
+    M->ow = P.super_resolution == 2 ? 2 * width : width;
+    M->oh = P.super_resolution == 2 ? 2 * height : height;
+    const size_t nout = (size_t)M->ow * M->oh;
+    A(M->sum, nout * 3);
+    A(M->wsum, nout * 3);
     A(M->taps, isa::tap_table_bytes(width, height));
     if (P.xc_precision == ISA_XC_BF16X3 || P.xc_precision == ISA_XC_FP16X3) {
         size_t tot = 0;
@@ -418,7 +425,7 @@ int isa_merger_destroy(isa_merger* m) {
 
 int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shift, const float* base_rotation,
                    float* out3_dev, int out_pitch, isa_stream_t stream) {
-    ISA_REQUIRE(M && raw_dev && out3_dev && out_pitch >= M->w * 12, "isa_merger_run: bad arguments");
+    ISA_REQUIRE(M && raw_dev && out3_dev && out_pitch >= M->ow * 12, "isa_merger_run: bad arguments");
     hipStream_t s = isa::to_stream(stream);
     const isa_params& P = M->P;
     const int w = M->w, h = M->h, n = M->n, n1 = n - 1, hw = w / 2, hh = h / 2;
@@ -647,8 +654,9 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     ISA_TRY(isa::launch_subsample3(raw_dev + (size_t)ref * np, M->ref_half, maxv, hw, hh, hw * 12, P.cfa, s));
     ISA_TRY(fill(M->mask, nh * 4, 1.0f, s));
     ISA_TRY(fill(M->masks + (size_t)ref * nh * 4, nh * 4, 1.0f, s));
-    ISA_TRY(isa::check_hip(hipMemsetAsync(M->sum, 0, np * 3 * sizeof(float), s), "memset"));
-    ISA_TRY(isa::check_hip(hipMemsetAsync(M->wsum, 0, np * 3 * sizeof(float), s), "memset"));
+    const size_t nout = (size_t)M->ow * M->oh;
+    ISA_TRY(isa::check_hip(hipMemsetAsync(M->sum, 0, nout * 3 * sizeof(float), s), "memset"));
+    ISA_TRY(isa::check_hip(hipMemsetAsync(M->wsum, 0, nout * 3 * sizeof(float), s), "memset"));
     (void)hipEventRecord(M->ev[4], s);
     // ---- 4. Accumulate: flow per frame (LK), robustness, erosion
     for (int f = 0; f < n; ++f) {
@@ -680,14 +688,15 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                               (const float* const*)(M->tab_host + tl.acc_flow), n, P.white, P.black,
                                               P.cfa, s));
     } else {
-        float* zero_flow = M->flows + (size_t)ref * np * 2;
-        ISA_TRY(isa::check_hip(hipMemsetAsync(zero_flow, 0, np * 2 * sizeof(float), s), "memset"));
-        for (int k = 0; k < n; ++k) {
-            int f = k == 0 ? ref : (k - 1 < ref ? k - 1 : k);
-            ISA_TRY(isa::launch_accumulate_sr(raw_dev + (size_t)f * np, M->sum, M->wsum, M->masks + (size_t)f * nh * 4,
-                                              M->kern, M->flows + (size_t)f * np * 2, P.white, P.black, w, h, 3 * wp,
-                                              hw * 16, kch * wp, 2 * wp, P.cfa, P.tex_quant, s));
-        }
+        // accumulateImagesSuperRes: the reference's central-half form (w x h grid, offset (w/2, h/2)) or the
+        // full-frame 2x grid (offset 0); the reference frame's flow is the zero flow
+        const bool full = P.super_resolution == 2;
+        ISA_TRY(isa::launch_accumulate_sr_frames(M->sum, M->wsum, M->kern, w, h, M->ow * 12, hw * 16, kch * wp, 2 * wp,
+                                                 (const uint16_t* const*)(M->tab_host + tl.acc_raw),
+                                                 (const float* const*)(M->tab_host + tl.acc_mask),
+                                                 (const float* const*)(M->tab_host + tl.acc_flow), n, M->ow, M->oh,
+                                                 full ? 0 : w / 2, full ? 0 : h / 2, P.white, P.black, P.cfa,
+                                                 P.tex_quant, s));
     }
     (void)hipEventRecord(M->ev[7], s);
     // ---- 6. final weighting (1x: fill with the reference debayer, threshold 1; SR: threshold 0)
@@ -696,8 +705,8 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                  P.camera_white, P.cfa, s));
         ISA_TRY(isa_apply_weighting(out3_dev, M->sum, M->wsum, w, h, out_pitch, 1.0f, s));
     } else {
-        ISA_TRY(isa::check_hip(hipMemset2DAsync(out3_dev, out_pitch, 0, (size_t)w * 12, h, s), "memset"));
-        ISA_TRY(isa_apply_weighting(out3_dev, M->sum, M->wsum, w, h, out_pitch, 0.0f, s));
+        ISA_TRY(isa::check_hip(hipMemset2DAsync(out3_dev, out_pitch, 0, (size_t)M->ow * 12, M->oh, s), "memset"));
+        ISA_TRY(isa_apply_weighting(out3_dev, M->sum, M->wsum, M->ow, M->oh, out_pitch, 0.0f, s));
     }
     (void)hipEventRecord(M->ev[8], s);
     M->have_times = true;
@@ -781,8 +790,8 @@ int isa_merger_buffer(isa_merger* M, int which, int index, void** ptr, int* pitc
         case 3: ISA_REQUIRE(fi, "frame index"); return set(M->flows + (size_t)index * np * 2, w * 8, w, h);
         case 4: ISA_REQUIRE(fi, "frame index"); return set(M->masks + (size_t)index * nh * 4, (w / 2) * 16, w / 2, h / 2);
         case 5: return set(M->kern, w * 4 * (M->P.super_resolution ? 4 : 3), w, h);
-        case 6: return set(M->sum, w * 12, w, h);
-        case 7: return set(M->wsum, w * 12, w, h);
+        case 6: return set(M->sum, M->ow * 12, M->ow, M->oh);
+        case 7: return set(M->wsum, M->ow * 12, M->ow, M->oh);
         case 8: {
             ISA_REQUIRE(index >= 0 && index < M->L, "level index");
             // shifts of the last run are those of the finest level; only level 0 is retained

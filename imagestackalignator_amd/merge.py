@@ -68,21 +68,28 @@ class BurstMerger:
 
     def run(self, raw, out=None, base_shift=None, base_rotation=None):
         """raw: uint16-valued torch tensor (frames, H, W) on the GPU (torch.int16/uint16 storage).
-        Returns the merged (H, W, 3) float32 image."""
+        Returns the merged (H, W, 3) float32 image ((2H, 2W, 3) with super_resolution = 2)."""
         torch = _torch()
         check_tensor("raw", raw, (self.frames, self.height, self.width), raw_dtypes())
+        oh, ow = self.out_shape
         if out is None:
-            out = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=raw.device)
-        check_tensor("out", out, (self.height, self.width, 3), (torch.float32,))
+            out = torch.empty((oh, ow, 3), dtype=torch.float32, device=raw.device)
+        check_tensor("out", out, (oh, ow, 3), (torch.float32,))
         if base_shift is not None and len(base_shift) != 2 * self.frames:
             raise ValueError(f"base_shift: expected {2 * self.frames} values (x, y per frame)")
         if base_rotation is not None and len(base_rotation) != self.frames:
             raise ValueError(f"base_rotation: expected {self.frames} values")
         bs = None if base_shift is None else (C.c_float * (2 * self.frames))(*[float(v) for v in base_shift])
         br = None if base_rotation is None else (C.c_float * self.frames)(*[float(v) for v in base_rotation])
-        call("isa_merger_run", self._h, C.c_void_p(raw.data_ptr()), bs, br, C.c_void_p(out.data_ptr()),
-             self.width * 12, _stream_ptr())
+        call("isa_merger_run", self._h, C.c_void_p(raw.data_ptr()), bs, br, C.c_void_p(out.data_ptr()), ow * 12,
+             _stream_ptr())
         return out
+
+    @property
+    def out_shape(self):
+        """(rows, columns) of the merged image: the frame size, doubled for the full-frame 2x merge."""
+        s = 2 if self.params.super_resolution == 2 else 1
+        return s * self.height, s * self.width
 
     @property
     def reference_index(self) -> int:

@@ -100,7 +100,8 @@ typedef struct isa_params {
     float min_det_lk;                 /* 0.01 */
     int erode_size;                   /* 5 */
     float threshold_m;                /* 1e5 */
-    int super_resolution;             /* 0: 1x merge; 1: reference 2x merge of the central half */
+    int super_resolution;             /* 0: 1x merge; 1: reference 2x merge of the central half (W x H output);
+                                         2: full-frame 2x merge (2W x 2H output, same per-pixel rule, offset 0) */
     int green_channel_only;           /* 0 */
     float noise_alpha, noise_beta;    /* noise model sigma^2 = alpha x + beta on the [0,1] scale */
     float black[3];                   /* black level per colour */
@@ -163,6 +164,20 @@ int isa_accumulate_images_superres(const uint16_t* raw_dev, float* sum3_dev, flo
                                    const float* kernel4_dev, const float* flow2_dev, const float white[3],
                                    const float black[3], int dim_x, int dim_y, int out_pitch, int mask_pitch,
                                    int kernel_pitch, int flow_pitch, const int cfa[4], int tex_quant,
+                                   isa_stream_t stream);
+/* accumulateImagesSuperRes (R:Kernels/DeBayerKernels.cu:378-468) over several frames in one pass and for any output
+ * grid at half the raw pixel spacing: out_w x out_h pixels whose pixel 0 lies (offset_x, offset_y) output pixels from
+ * the frame's origin (output pixel x sits at raw position (x + 0.5 + offset_x) / 2). The reference's form is
+ * out = dim (dim_x x dim_y) with offset (dim_x/2, dim_y/2), the central half of the frame
+ * (isa_accumulate_images_superres); the full-frame 2x merge is 2 dim_x x 2 dim_y with offset 0. Raw and certainty
+ * coordinates are clamped to the frame rows and columns the grid covers. raw_devs, mask4_devs, flow2_devs: host
+ * arrays of `frames` device pointers in accumulation order (a NULL flow is the zero flow); the sums equal one
+ * single-frame accumulation per frame in that order, taps in the reference's raster order. */
+int isa_accumulate_frames_superres(const uint16_t* const* raw_devs, const float* const* mask4_devs,
+                                   const float* const* flow2_devs, int frames, float* sum3_dev, float* weight3_dev,
+                                   const float* kernel4_dev, const float white[3], const float black[3], int dim_x,
+                                   int dim_y, int out_w, int out_h, int offset_x, int offset_y, int out_pitch,
+                                   int mask_pitch, int kernel_pitch, int flow_pitch, const int cfa[4], int tex_quant,
                                    isa_stream_t stream);
 /* ApplyWeighting, R:Kernels/kernel.cu:316-372; wrapper R:PEFStudioDX/BasicKernels.cs:259-285. */
 int isa_apply_weighting(float* inout3_dev, const float* sum3_dev, const float* weight3_dev, int width, int height,
@@ -360,7 +375,7 @@ int isa_merger_destroy(isa_merger* m);
  * may be NULL) are the initial shifts the PreAlignmentStore is built from. With base_shift NULL and
  * params.prealign_max_shift > 0 they are computed as ComputePreAlignment does (isa_prealign on the tracking images,
  * R:...Controller.cs:1451-1528); the run then reads them back once (one stream synchronisation).
- * out3_dev: width x height float3 (pitch out_pitch). */
+ * out3_dev: width x height float3 (pitch out_pitch); 2 width x 2 height with super_resolution = 2. */
 int isa_merger_run(isa_merger* m, const uint16_t* raw_dev, const float* base_shift, const float* base_rotation,
                    float* out3_dev, int out_pitch, isa_stream_t stream);
 /* Device pointer and geometry of an internal buffer, for inspection:

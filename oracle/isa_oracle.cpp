@@ -1240,19 +1240,23 @@ void accumulate_class_order(const uint16_t* raw, T* sum3, T* wsum3, const T* mas
     accumulate_ordered(raw, sum3, wsum3, mask4, kern3, flow, white, black, w, h, cfa, true);
 }
 
-// kern4, flow: full-resolution W x H textures (clamp / mirror, linear).
+// kern4, flow: full-resolution W x H textures (clamp / mirror, linear). The output grid has ow x oh pixels at
+// half the raw spacing whose pixel 0 lies (ox, oy) output pixels from the frame origin: the reference's
+// central-half form is (w, h, w/2, h/2) (R:Kernels/DeBayerKernels.cu:395-423, which writes w / 2 and w / 4 for
+// ox and ox / 2), the full-frame 2x merge (2w, 2h, 0, 0) -- the same per-pixel rule with zero offset.
 template <class T>
-void accumulate_sr(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern4, const T* flow, const T white[3],
-                   const T black[3], int w, int h, const int cfa[4], int quant) {
+void accumulate_sr_grid(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern4, const T* flow,
+                        const T white[3], const T black[3], int w, int h, const int cfa[4], int quant, int ow, int oh,
+                        int ox, int oy) {
     int mw = w / 2;
-    parallel_for(h, [&](int y0, int y1) {
-        for (int y = std::max(y0, 1); y < std::min(y1, h - 1); ++y)
-            for (int x = 1; x < w - 1; ++x) {
-                size_t o = (size_t)y * w + x;
+    parallel_for(oh, [&](int y0, int y1) {
+        for (int y = std::max(y0, 1); y < std::min(y1, oh - 1); ++y)
+            for (int x = 1; x < ow - 1; ++x) {
+                size_t o = (size_t)y * ow + x;
                 T acc[3] = {sum3[o * 3], sum3[o * 3 + 1], sum3[o * 3 + 2]};
                 T wt[3] = {wsum3[o * 3], wsum3[o * 3 + 1], wsum3[o * 3 + 2]};
-                T posx = ((T)x + (T)0.5f + (T)(w / 2)) / (T)2.0f / (T)w;
-                T posy = ((T)y + (T)0.5f + (T)(h / 2)) / (T)2.0f / (T)h;
+                T posx = ((T)x + (T)0.5f + (T)ox) / (T)2.0f / (T)w;
+                T posy = ((T)y + (T)0.5f + (T)oy) / (T)2.0f / (T)h;
                 T k0 = tex_linear(kern4, w, h, 4, 0, posx, posy, ADDR_CLAMP, quant);
                 T k1 = tex_linear(kern4, w, h, 4, 1, posx, posy, ADDR_CLAMP, quant);
                 T k2 = tex_linear(kern4, w, h, 4, 2, posx, posy, ADDR_CLAMP, quant);
@@ -1261,12 +1265,12 @@ void accumulate_sr(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const
                 int sx = (int)std::round(fx * (T)2), sy = (int)std::round(fy * (T)2);
                 for (int py = -2; py <= 2; ++py)
                     for (int px = -2; px <= 2; ++px) {
-                        int ppsx = x + px + sx + w / 2, ppsy = y + py + sy + h / 2;
-                        int ppx = x + px + w / 2, ppy = y + py + h / 2;
-                        ppsx = std::min(std::max(ppsx / 2, 0 + w / 4), w / 2 - 1 + w / 4);
-                        ppsy = std::min(std::max(ppsy / 2, 0 + h / 4), h / 2 - 1 + h / 4);
-                        ppx = std::min(std::max(ppx / 2, 0 + w / 4), w / 2 - 1 + w / 4);
-                        ppy = std::min(std::max(ppy / 2, 0 + h / 4), h / 2 - 1 + h / 4);
+                        int ppsx = x + px + sx + ox, ppsy = y + py + sy + oy;
+                        int ppx = x + px + ox, ppy = y + py + oy;
+                        ppsx = std::min(std::max(ppsx / 2, 0 + ox / 2), ow / 2 - 1 + ox / 2);
+                        ppsy = std::min(std::max(ppsy / 2, 0 + oy / 2), oh / 2 - 1 + oy / 2);
+                        ppx = std::min(std::max(ppx / 2, 0 + ox / 2), ow / 2 - 1 + ox / 2);
+                        ppy = std::min(std::max(ppy / 2, 0 + oy / 2), oh / 2 - 1 + oy / 2);
                         int site = cfa_at(cfa, ppsy, ppsx);
                         T wgt = kernel_weight(px, py, k0, k1, k2);
                         T v = (T)raw[(size_t)ppsy * w + ppsx];
@@ -1281,6 +1285,13 @@ void accumulate_sr(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const
                 for (int c = 0; c < 3; ++c) { sum3[o * 3 + c] = acc[c]; wsum3[o * 3 + c] = wt[c]; }
             }
     });
+}
+
+// accumulateImagesSuperRes as the reference launches it: the central half at w x h.
+template <class T>
+void accumulate_sr(const uint16_t* raw, T* sum3, T* wsum3, const T* mask4, const T* kern4, const T* flow, const T white[3],
+                   const T black[3], int w, int h, const int cfa[4], int quant) {
+    accumulate_sr_grid(raw, sum3, wsum3, mask4, kern4, flow, white, black, w, h, cfa, quant, w, h, w / 2, h / 2);
 }
 
 // a21  ApplyWeighting (R:Kernels/kernel.cu:316-372).

@@ -73,6 +73,7 @@ def _declare(L):
         "accumulate": (None, [P, P, P, P, P, P, P, P, I, I, P]),
         "accumulate_class_order": (None, [P, P, P, P, P, P, P, P, I, I, P]),
         "accumulate_sr": (None, [P, P, P, P, P, P, P, P, I, I, P, I]),
+        "accumulate_sr_grid": (None, [P, P, P, P, P, P, P, P, I, I, P, I, I, I, I, I]),
         "apply_weighting": (None, [P, P, P, I, I, "T"]),
         "merge": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     }
@@ -254,10 +255,12 @@ def merge(params, raw: np.ndarray, base_shift=None, base_rot=None, dt=np.float32
     """Full oracle pipeline. raw: (n, h, w) uint16. Returns dict with 'out' and the requested intermediates."""
     raw = np.ascontiguousarray(raw, np.uint16)
     n, h, w = raw.shape
-    res = {"out": np.zeros((h, w, 3), dt)}
+    s2 = 2 if params.super_resolution == 2 else 1  # full-frame 2x merge: 2w x 2h output
+    res = {"out": np.zeros((s2 * h, s2 * w, 3), dt)}
     bufs = {
         "one_to_one": None, "flows": (n, h, w, 2), "masks": (n, h // 2, w // 2, 4), "kern": (h, w, 3),
-        "track": (n, h, w), "lk": (n, h, w), "sum": (h, w, 3), "wsum": (h, w, 3), "lk_cond": (n, h, w, 2),
+        "track": (n, h, w), "lk": (n, h, w), "sum": (s2 * h, s2 * w, 3), "wsum": (s2 * h, s2 * w, 3),
+        "lk_cond": (n, h, w, 2),
     }
     for k in want:
         if k == "one_to_one":

@@ -598,3 +598,50 @@ def test_srgb_fast_power_is_exact(isa, oracle, gpu):
     img = np.ascontiguousarray(np.repeat(xs[:, None], 3, axis=1))
     oracle.fn("gamma_srgb")(xs.size, 1, oracle.ptr(img))
     np.testing.assert_array_equal(host(d_out), img[:, 0])
+
+
+@pytest.mark.parametrize("grid", ["reference", "full"])
+@pytest.mark.parametrize("quant", [0, 1])
+def test_accumulate_frames_superres(isa, oracle, gpu, grid, quant):
+    """accumulateImagesSuperRes over four frames in one pass, on the reference's central-half grid (w x h,
+    offset (w/2, h/2)) and on the full-frame 2x grid (2w x 2h, offset 0), bit for bit against the oracle's
+    per-frame accumulation in the same order (raster taps); frame 0 has the zero (null) flow, with NaN kernel
+    and certainty values."""
+    rng = np.random.default_rng(81 + quant)
+    h, w, n = 40, 56, 4
+    hw, hh = w // 2, h // 2
+    ow, oh, ox, oy = (w, h, w // 2, h // 2) if grid == "reference" else (2 * w, 2 * h, 0, 0)
+    raws = [_raw(rng, h, w) for _ in range(n)]
+    kern = np.zeros((h, w, 4), np.float32)
+    kern[..., 0] = rng.uniform(0.2, 2.0, (h, w))
+    kern[..., 1] = rng.uniform(0.2, 2.0, (h, w))
+    kern[..., 2] = rng.uniform(-0.1, 0.1, (h, w))
+    kern[7, 11, :3] = np.nan
+    masks = [rng.uniform(0, 1, (hh, hw, 4)).astype(np.float32) for _ in range(n)]
+    masks[1][5, 6, 0] = np.nan
+    flows = [None] + [rng.uniform(-2.6, 2.6, (h, w, 2)).astype(np.float32) for _ in range(n - 1)]
+    s0 = rng.uniform(0, 1, (oh, ow, 3)).astype(np.float32)
+    w0 = rng.uniform(0, 1, (oh, ow, 3)).astype(np.float32)
+    white, black = [16383.0, 16000.0, 15000.0], [10.0, 20.0, 30.0]
+    print(f"superres accumulation {grid} {ow}x{oh} quant={quant}", flush=True)
+    d_raw, d_mask = [dev(r) for r in raws], [dev(m) for m in masks]
+    d_flow = [None if f is None else dev(f) for f in flows]
+    ptrs = lambda ts: (C.c_void_p * n)(*[None if t is None else t.data_ptr() for t in ts])  # noqa: E731
+    ds, dw = dev(s0), dev(w0)
+    isa.call("isa_accumulate_frames_superres", ptrs(d_raw), ptrs(d_mask), ptrs(d_flow), n, p(ds), p(dw), p(dev(kern)),
+             isa.f32s(*white), isa.f32s(*black), w, h, ow, oh, ox, oy, ow * 12, hw * 16, w * 16, w * 8,
+             isa.i32s(*RGGB), quant, stream())
+    sr_, wr_ = s0.copy(), w0.copy()
+    zero = np.zeros((h, w, 2), np.float32)
+    for f in range(n):
+        oracle.fn("accumulate_sr_grid")(oracle.ptr(raws[f]), oracle.ptr(sr_), oracle.ptr(wr_), oracle.ptr(masks[f]),
+                                        oracle.ptr(kern), oracle.ptr(zero if flows[f] is None else flows[f]),
+                                        oracle.ptr(oracle.arr(white, np.float32)),
+                                        oracle.ptr(oracle.arr(black, np.float32)), w, h,
+                                        oracle.ptr(oracle.arr(RGGB, np.int32)), quant, ow, oh, ox, oy)
+    np.testing.assert_array_equal(host(ds), sr_)
+    np.testing.assert_array_equal(host(dw), wr_)
+    with pytest.raises(isa.IsaError):  # a grid reaching past the frame is rejected before any launch
+        isa.call("isa_accumulate_frames_superres", ptrs(d_raw), ptrs(d_mask), ptrs(d_flow), n, p(ds), p(dw),
+                 p(dev(kern)), isa.f32s(*white), isa.f32s(*black), w, h, 2 * w, 2 * h, 2, 0, 2 * w * 12, hw * 16,
+                 w * 16, w * 8, isa.i32s(*RGGB), quant, stream())

@@ -26,6 +26,8 @@ CASES = {
     "c0_small": dict(w=256, h=192, n=5, gmax=4.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0),
     "three_levels": dict(w=320, h=256, n=4, gmax=10.0, levels=[(1, 32, 2), (2, 32, 4), (4, 16, 4)], sr=0, occ=0.05),
     "superres": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=1, occ=0.0),
+    # the full-frame 2x merge: 2w x 2h output, the reference's super-resolution rule with zero offset
+    "superres_full": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=2, occ=0.02),
     # non-zero, unequal black and white levels: the merge normalises with the levels as given (the
     # reference's in-place WhiteLevel += black of each PrepareAccumulation / Accumulate call is not
     # reproduced, on either side)

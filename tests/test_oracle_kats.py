@@ -399,7 +399,7 @@ def test_erode_is_a_replicated_min_filter(oracle):
     np.testing.assert_array_equal(out, want)
 
 
-@pytest.mark.parametrize("sr", [0, 1])
+@pytest.mark.parametrize("sr", [0, 1, 2])
 def test_identical_frames_merge_to_the_constant_colour(oracle, isa, sr):
     h, w, n = 32, 40, 3
     rgb = {0: 5000, 1: 9000, 2: 3000}

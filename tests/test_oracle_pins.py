@@ -253,3 +253,37 @@ def test_accumulate_tap_orders_agree_within_the_summation_bound(oracle):
     inner = np.s_[6:-6, 8:-6]  # no tap clamped: |flow| <= 4 there
     np.testing.assert_array_equal(sr[inner][..., 0], sc[inner][..., 0])
     np.testing.assert_array_equal(sr[inner][..., 2], sc[inner][..., 2])
+
+
+# ---------------------------------------------------------------- a20 full-frame 2x grid
+def test_full_frame_superres_contains_the_reference_crop(oracle):
+    """The full-frame 2x grid (2w x 2h, offset 0) is the reference's accumulateImagesSuperRes rule with a zero
+    offset: inside the central half, away from where the reference's clamp to that half bites, its output
+    pixel (x + w/2, y + h/2) equals the reference's pixel (x, y) bit for bit; the two differ near the crop edge,
+    where only the reference clamps."""
+    rng = np.random.default_rng(71)
+    h, w = 40, 48
+    raw = rng.integers(0, 16384, (h, w)).astype(np.uint16)
+    kern4 = np.zeros((h, w, 4), F32)
+    kern4[..., 0] = rng.uniform(0.2, 2.0, (h, w))
+    kern4[..., 1] = rng.uniform(0.2, 2.0, (h, w))
+    kern4[..., 2] = rng.uniform(-0.1, 0.1, (h, w))
+    mask = rng.uniform(0, 1, (h // 2, w // 2, 4)).astype(F32)
+    flow = rng.uniform(-1.5, 1.5, (h, w, 2)).astype(F32)
+    white, black = oracle.arr([16383.0] * 3, F32), oracle.arr([0.0] * 3, F32)
+    cfa = oracle.arr([0, 1, 1, 2], np.int32)
+    s_ref, w_ref = np.zeros((h, w, 3), F32), np.zeros((h, w, 3), F32)
+    oracle.fn("accumulate_sr")(oracle.ptr(raw), oracle.ptr(s_ref), oracle.ptr(w_ref), oracle.ptr(mask),
+                               oracle.ptr(kern4), oracle.ptr(flow), oracle.ptr(white), oracle.ptr(black), w, h,
+                               oracle.ptr(cfa), 1)
+    s_full, w_full = np.zeros((2 * h, 2 * w, 3), F32), np.zeros((2 * h, 2 * w, 3), F32)
+    oracle.fn("accumulate_sr_grid")(oracle.ptr(raw), oracle.ptr(s_full), oracle.ptr(w_full), oracle.ptr(mask),
+                                    oracle.ptr(kern4), oracle.ptr(flow), oracle.ptr(white), oracle.ptr(black), w, h,
+                                    oracle.ptr(cfa), 1, 2 * w, 2 * h, 0, 0)
+    m = 8  # output pixels: sample shifts up to round(2 * 1.5) = 3 plus the 2-tap window, halved to raw pixels
+    crop = np.s_[h // 2 + m:h // 2 + h - m, w // 2 + m:w // 2 + w - m]
+    np.testing.assert_array_equal(s_full[crop], s_ref[m:h - m, m:w - m])
+    np.testing.assert_array_equal(w_full[crop], w_ref[m:h - m, m:w - m])
+    assert np.any(s_full[h // 2 + 1:h // 2 + h - 1, w // 2 + 1] != s_ref[1:h - 1, 1])  # the reference's clamp
+    # outside the crop the full grid is written too (the reference's grid has no such pixels)
+    assert np.all(w_full[2:6, 2:-2].sum(axis=-1) > 0)
