@@ -1,12 +1,11 @@
 #!/usr/bin/env python3
 """Burst-merge benchmark on one MI355X.
 
-Metric: burst frames*MPix/s merged = N * (W*H / 1e6) / t, where t is the time of
-one full merge (tracking images, tile alignment + ShiftMinimizer, kernel
-parameters, Lucas-Kanade flow, robustness, accumulation, final weighting) with
-the raw u16 frames already resident in HBM. The default workload is the
-full-size FP32 configuration: 20-frame 6000x4000 RGGB 14-bit synthetic burst,
-4-level pyramid (levels 1/2/4/8, tiles 64/32/32/16, max shift 2/4/4/4), merge at 1x.
+Metric: burst frames*MPix/s merged = N * (W*H / 1e6) / t, where t is the time of one full merge (global
+pre-alignment, tracking images, tile alignment + ShiftMinimizer, kernel parameters, Lucas-Kanade flow,
+robustness, accumulation, final weighting) with the raw u16 frames already resident in HBM. The headline
+workload is the full-size FP32 plain configuration: 20-frame 6000x4000 RGGB 14-bit synthetic burst with global
+shifts U(-64, 64) px, 4-level pyramid (levels 1/2/4/8, tiles 64/32/32/16, max shift 2/4/4/4), merge at 1x.
 
 Prints ONE JSON line. Extra diagnostics go to stderr.
 """
@@ -15,6 +14,8 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -22,39 +23,64 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+C1_LEVELS = [(1, 64, 2), (2, 32, 4), (4, 32, 4), (8, 16, 4)]
 CONFIGS = {
-    # full-size FP32 plain configuration (the metric's workload)
-    "full_fp32": dict(w=6000, h=4000, n=20, levels=[(1, 64, 2), (2, 32, 4), (4, 32, 4), (8, 16, 4)],
-                      gmax=64.0, flow=1.5, occ=0.02),
+    # full-size FP32 plain configuration (the metric's workload); global shifts up to +-64 px per frame, so up to
+    # 128 px between two frames: the pre-alignment searches +-128 px
+    "full_fp32": dict(w=6000, h=4000, n=20, levels=C1_LEVELS, gmax=64.0, flow=1.5, occ=0.02, prealign=128),
     # small CPU-runnable configuration
-    "small": dict(w=1024, h=1024, n=8, levels=[(1, 32, 2), (2, 32, 4), (4, 32, 4)], gmax=16.0, flow=1.5, occ=0.0),
+    "small": dict(w=1024, h=1024, n=8, levels=[(1, 32, 2), (2, 32, 4), (4, 32, 4)], gmax=16.0, flow=1.5, occ=0.0,
+                  prealign=0),
+    # precision variants of the tile search at their own size
+    "c2": dict(w=4032, h=3024, n=30, levels=[(1, 64, 2), (2, 32, 4), (4, 32, 4), (8, 16, 4)], gmax=16.0, flow=1.5,
+               occ=0.02, prealign=0),
+    # combined largest: 50 frames, global shifts up to +-128 px, local flow up to 3 px, 5 % occluders
+    "c4": dict(w=6000, h=4000, n=50, levels=C1_LEVELS, gmax=128.0, flow=3.0, occ=0.05, prealign=256),
 }
 
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_MFMA_TFLOPS = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp32": 157.3}
+XC_CODES = {"bf16x3": 1, "fp16x3": 2, "fp32": 3}
+KERNEL_TRAFFIC = ROOT / "profiles" / "kernel_traffic_full_fp32.json"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def track_flops(cfg, level_idx, pairs):
-    """Algorithmic FP64 flops of one tile-search launch: the cross terms sum_x r(x) t(x+s)
-    (one FMA = 2 flops per interior pixel and lag), per tile and pair."""
+def track_flops(cfg, level_idx, pairs, products=1):
+    """Algorithmic flops of one tile-search launch: the cross terms sum_x r(x) t(x+s) (one multiply-add = 2 flops
+    per interior pixel and lag), per tile and pair, times the products per term (3 for the split forms)."""
     l, t, m = cfg["levels"][level_idx]
     lw, lh = cfg["w"] // l, cfg["h"] // l
     tiles = ((lw - 2 * m) // t) * ((lh - 2 * m) // t)
-    return 2.0 * t * t * (2 * m + 1) ** 2 * tiles * pairs
+    return 2.0 * t * t * (2 * m + 1) ** 2 * tiles * pairs * products
 
 
-def cpu_baseline(cfg, seconds_hint: float):
-    """The oracle (reference semantics, FP32) on a crop of the same burst, timed on this host."""
+def host_info():
+    model = platform.processor() or "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def cpu_baseline(cfg):
+    """The oracle (reference semantics, FP32) on a crop of the same burst, timed on this host with every usable
+    core."""
     import numpy as np
 
     import imagestackalignator_amd as isa
     from oracle import oracle as O
 
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    info = host_info()
+    # every core this process may use, within the host's thread budget for one GPU (OMP_NUM_THREADS when set)
+    threads = min(info["usable_cores"], int(os.environ.get("OMP_NUM_THREADS", info["usable_cores"])))
     O.set_threads(threads)
     cw, ch = 2048, 1536
     scfg = isa.default_synth(seed=7, width=cw, height=ch, frames=cfg["n"], global_shift_max=cfg["gmax"] / 8,
@@ -73,8 +99,11 @@ def cpu_baseline(cfg, seconds_hint: float):
         "unit": "burst frames*MPix/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{cfg['n']}-frame {cw}x{ch} crop of the same workload (same levels/tiles), oracle FP32 "
-                  f"pipeline, {dt:.1f} s",
+        "nproc": info["nproc"],
+        "cpu_model": info["cpu_model"],
+        "sample": f"{cfg['n']}-frame {cw}x{ch} crop of the same workload (same levels/tiles, shifts /8 so no "
+                  f"pre-alignment is needed), oracle FP32 pipeline with std::thread over {threads} threads "
+                  f"(usable cores {info['usable_cores']}, thread budget OMP_NUM_THREADS), {dt:.1f} s",
     }
 
 
@@ -92,7 +121,7 @@ STAGE_BYTES = {
 
 def stage_rooflines(stages, n, w, h, levels):
     """Achieved rate of every stage against the MI355X peak that bounds it: HBM bytes for the image stages,
-    FP64 flops of the tile cross terms for track_patches (the minimizer is latency-bound: time only)."""
+    FP64 flops of the tile cross terms for track_patches (FP64 VALU), time only for the latency-bound rest."""
     P = w * h
     out = {}
     for k, ms in stages.items():
@@ -105,17 +134,80 @@ def stage_rooflines(stages, n, w, h, levels):
         elif k == "track_patches":
             gf = sum(l["track_gflop"] for l in levels)
             tf = gf / ms if ms > 0 else 0.0
-            out[k] = {"ms": round(ms, 3), "bound": "fp64", "algorithmic_gflop": round(gf, 1),
+            out[k] = {"ms": round(ms, 3), "bound": "fp64-valu", "algorithmic_gflop": round(gf, 1),
                       "achieved_tflops": round(tf, 2), "frac": round(tf / PEAK_FP64_TFLOPS, 4)}
         else:
             out[k] = {"ms": round(ms, 3), "bound": "latency"}
     return out
 
 
+def timed_runs(torch, fn, steps, warmup):
+    """Warm-up, then `steps` runs each bracketed by device synchronisation: (mean ms, median ms, all)."""
+    for i in range(warmup):
+        fn()
+        torch.cuda.synchronize()
+    times = []
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1000.0)
+    total = (time.perf_counter() - t_all) * 1000.0
+    return total / max(1, steps), statistics.median(times), times
+
+
+def psnr_t(torch, x, y, b=4):
+    import numpy as np
+
+    d = (x - y)[b:-b, b:-b].double()
+    fin = torch.isfinite(d)
+    mse = float(torch.mean(d[fin] ** 2))
+    return 99.0 if mse == 0 else round(10 * np.log10(1.0 / mse), 3)
+
+
+def shifts_vs_truth(torch, merger, truth, b=64):
+    """Recovered per-frame global motion against the synthesiser's truth: the pre-alignment's integer initial
+    shifts (g_f - g_0) and the median over the interior of each frame's final flow (g_ref - g_f)."""
+    import numpy as np
+
+    truth = np.asarray(truth, np.float64)
+    n = len(truth)
+    out = {}
+    if merger.params.prealign_max_shift > 0:
+        pre = merger.buffer_tensor(10, 0, 4).cpu().numpy().reshape(n, 4)[:, :2].astype(np.float64)
+        err = pre - (truth - truth[0])
+        out["prealign_max_err_px"] = round(float(np.abs(err).max()), 3)
+    r = merger.reference_index
+    errs = []
+    for f in range(n):
+        if f == r:
+            continue
+        fl = merger.buffer_tensor(3, f, 2)[b:-b, b:-b].reshape(-1, 2)
+        med = fl.median(dim=0).values.cpu().numpy().astype(np.float64)
+        errs.append(np.abs(med - (truth[r] - truth[f])))
+    errs = np.array(errs)
+    out["median_flow_max_err_px"] = round(float(errs.max()), 4)
+    out["median_flow_rms_err_px"] = round(float(np.sqrt(np.mean(errs ** 2))), 4)
+    out["truth_max_abs_px"] = round(float(np.abs(truth).max()), 2)
+    return out
+
+
+def kernel_times(isa, merger, n_levels):
+    res = {}
+    for name in merger.KERNELS:
+        levels = range(n_levels) if name in ("track", "exact_queue") else [-1]
+        res[name] = [merger.kernel_ms(name, lv) for lv in levels]
+    return res
+
+
 def precision_vs_fp64(isa, torch):
     """Errors of the GPU paths against the FP64 oracle on the 8-frame 1024x1024 configuration: finest-level
     one-to-one shifts (max, RMS px), flow (RMS px over the interior), eroded mask (max abs), merged image (PSNR
-    and max relative error over the interior) and the PSNR loss against the noiseless ground truth."""
+    and max relative error over the interior) and the PSNR loss against the noiseless ground truth. The FP32
+    oracle rows are the reference's arithmetic (lk_ill_ratio 0) and the same with the FP64 solve of
+    ill-conditioned Lucas-Kanade windows the GPU paths use (lk_ill_ratio 0.01)."""
     import numpy as np
 
     from oracle import oracle as O
@@ -127,7 +219,7 @@ def precision_vs_fp64(isa, torch):
     raw_t, _ = isa.synth_burst(scfg)
     torch.cuda.synchronize()
     raw = raw_t.cpu().numpy().view(np.uint16)
-    O.set_threads(min(len(os.sched_getaffinity(0)), 16))
+    O.set_threads(len(os.sched_getaffinity(0)))
     t0 = time.perf_counter()
     params = isa.default_params(levels=cfg["levels"])
     ref = O.merge(params, raw, dt=np.float64, want=("one_to_one", "flows", "masks"))
@@ -146,21 +238,23 @@ def precision_vs_fp64(isa, torch):
     res = {"workload": f"{n}-frame {w}x{h} RGGB 14-bit burst, levels "
                        + "/".join(str(l[0]) for l in cfg["levels"]) + ", 1x merge",
            "oracle_s": round(t_oracle, 2), "psnr_vs_gt_fp64_db": round(p_gt_ref, 3)}
-    # the FP32 oracle (the reference's arithmetic) against the FP64 one: the errors the reference itself has
-    r32 = O.merge(params, raw, dt=np.float32, want=("one_to_one", "flows", "masks"))
-    paths = [("oracle_fp32", None), ("plain", 0), ("bf16x3", 1), ("fp16x3", 2), ("fp32", 3)]
-    for name, code in paths:
+    paths = [("oracle_fp32_reference", None, 0.0), ("oracle_fp32", None, 0.01), ("plain", 0, 0.01),
+             ("bf16x3", 1, 0.01), ("fp16x3", 2, 0.01), ("fp32", 3, 0.01)]
+    for name, code, ratio in paths:
         if code is None:
+            r32 = O.merge(isa.default_params(levels=cfg["levels"], lk_ill_ratio=ratio), raw, dt=np.float32,
+                          want=("one_to_one", "flows", "masks"))
             out, o2o, ref_index = r32["out"].astype(np.float64), r32["one_to_one"], r32["ref"]
-            flow_of = lambda f: r32["flows"][f].astype(np.float64)  # noqa: E731
-            mask_of = lambda f: r32["masks"][f].astype(np.float64)[..., :3]  # noqa: E731
+            flow_of = lambda f, r=r32: r["flows"][f].astype(np.float64)  # noqa: E731
+            mask_of = lambda f, r=r32: r["masks"][f].astype(np.float64)[..., :3]  # noqa: E731
         else:
-            m = isa.BurstMerger(w, h, n, isa.default_params(levels=cfg["levels"], xc_precision=code))
+            m = isa.BurstMerger(w, h, n, isa.default_params(levels=cfg["levels"], xc_precision=code,
+                                                            lk_ill_ratio=ratio))
             out = m.run(raw_t).cpu().numpy().astype(np.float64)
             ref_index = m.reference_index
             o2o = m.buffer_tensor(2, 0, 2).cpu().numpy().reshape(ref["one_to_one"].shape)
-            flow_of = lambda f: m.buffer_tensor(3, f, 2).cpu().numpy().astype(np.float64)  # noqa: E731
-            mask_of = lambda f: m.buffer_tensor(4, f, 4).cpu().numpy().astype(np.float64)[..., :3]  # noqa: E731
+            flow_of = lambda f, m=m: m.buffer_tensor(3, f, 2).cpu().numpy().astype(np.float64)  # noqa: E731
+            mask_of = lambda f, m=m: m.buffer_tensor(4, f, 4).cpu().numpy().astype(np.float64)[..., :3]  # noqa: E731
         if ref_index != ref["ref"]:
             res[name] = {"error": "reference frame differs from the oracle's"}
             continue
@@ -175,7 +269,8 @@ def precision_vs_fp64(isa, torch):
             mk_err = max(mk_err, float(np.nanmax(dm)))
         fl_err = np.concatenate(fl_err)
         rel = np.abs(out - ref["out"])[sl] / np.maximum(np.abs(ref["out"][sl]), 1e-3)
-        res[name] = {"shift_max_px": float(f"{ds.max():.3g}"), "shift_rms_px": float(f"{np.sqrt(np.mean(ds ** 2)):.3g}"),
+        res[name] = {"lk_ill_ratio": ratio,
+                     "shift_max_px": float(f"{ds.max():.3g}"), "shift_rms_px": float(f"{np.sqrt(np.mean(ds ** 2)):.3g}"),
                      "flow_rms_px": float(f"{np.sqrt(np.mean(fl_err)):.3g}"), "mask_max_abs": float(f"{mk_err:.3g}"),
                      "psnr_vs_fp64_db": round(psnr(out, ref["out"]), 2),
                      "max_rel_vs_fp64": float(f"{np.nanmax(rel):.3g}"),
@@ -188,7 +283,7 @@ def adjoint_bench(isa, torch, steps: int, warmup: int):
     """Differentiable merge on a 16-frame 2048x2048 burst at 2x output: forward (relaxed robustness +
     accumulation, loss against the noiseless 2x rendering) and backward (flow and kernel parameters)
     timed with HIP events on the launch stream, a central-difference gradcheck (eps 1e-3 px) at 1,000
-    random pixels, and 5 refinement steps."""
+    random pixels, and 5 refinement steps through isa_merge_refine_step."""
     import numpy as np
 
     from imagestackalignator_amd.merge_diff import MergeDiff
@@ -251,7 +346,6 @@ def adjoint_bench(isa, torch, steps: int, warmup: int):
         an = g[f, ys_t, xs_t, comp].double()
         rel.append((fd - an).abs() / torch.clamp(an.abs(), min=1e-3 * float(g.abs().max())))
     rel = torch.cat(rel).cpu().numpy()
-    # refinement
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     _, theta_r, losses = md.refine(flow, theta, gt, steps=5, lr_flow=2.0)
@@ -273,6 +367,117 @@ def adjoint_bench(isa, torch, steps: int, warmup: int):
     }
 
 
+def mixed_section(isa, torch, cfg, raw, out_exact, gt, psnr_gt_exact, steps, warmup, names, m_pairs):
+    """The tile search's cross terms on the matrix cores, same workload: merge rate, MFMA launch rate against the
+    dense matrix-core peak (3 products per term for the split forms), exact-search fractions, PSNR."""
+    w, h, n = cfg["w"], cfg["h"], cfg["n"]
+    res = {}
+    for name in names:
+        pm = isa.default_params(levels=cfg["levels"], xc_precision=XC_CODES[name],
+                                prealign_max_shift=cfg["prealign"])
+        mm = isa.BurstMerger(w, h, n, pm)
+        o2 = torch.empty_like(out_exact)
+        mean_ms, med_ms, _ = timed_runs(torch, lambda: mm.run(raw, o2), steps, warmup)
+        st2 = mm.stage_ms()
+        kt = kernel_times(isa, mm, len(cfg["levels"]))
+        prods = 1 if name == "fp32" else 3
+        levels = []
+        for i, (ms_launch, _) in enumerate(kt["track"]):
+            fl = track_flops(cfg, i, m_pairs, prods)
+            tf = fl / 1e9 / ms_launch if ms_launch > 0 else 0.0
+            levels.append({"level": cfg["levels"][i][0], "mfma_ms": round(ms_launch, 3),
+                           "exact_queue_ms": round(kt["exact_queue"][i][0], 3),
+                           "mfma_tflops": round(tf, 1), "mfma_util": round(tf / PEAK_MFMA_TFLOPS[name], 4)})
+        ex_items = mm.exact_items()
+        p_gt = psnr_t(torch, o2, gt)
+        res[name] = {"value": round(n * (w * h / 1e6) / (med_ms / 1000.0), 3), "ms_per_step": round(mean_ms, 3),
+                     "median_ms": round(med_ms, 3), "track_patches_ms": round(st2["track_patches"], 3),
+                     "psnr_vs_exact_db": psnr_t(torch, o2, out_exact), "psnr_vs_gt_db": p_gt,
+                     "psnr_loss_vs_exact_db": round(psnr_gt_exact - p_gt, 4),
+                     "exact_search_fraction": [round(a / max(1, b), 5) for a, b in ex_items],
+                     "levels": levels,
+                     "mfma_peak_tflops": PEAK_MFMA_TFLOPS[name]}
+        log(f"[bench] xc {name}: {med_ms:.2f} ms/merge (median), track {st2['track_patches']:.2f} ms, "
+            f"levels {levels}")
+        del mm
+    return res
+
+
+def c2_section(isa, torch, steps, warmup):
+    """The precision workload at its size (30 x 4032x3024): the exact FP64 search and the three matrix-core forms,
+    timed; their merged images against the exact path's and the ground truth. (Errors against the FP64 oracle are
+    measured on the small configuration, where the oracle runs in seconds.)"""
+    cfg = CONFIGS["c2"]
+    w, h, n = cfg["w"], cfg["h"], cfg["n"]
+    scfg = isa.default_synth(seed=20240604, width=w, height=h, frames=n, global_shift_max=cfg["gmax"],
+                             flow_max=cfg["flow"], occluded_fraction=cfg["occ"])
+    raw, _ = isa.synth_burst(scfg)
+    m = isa.BurstMerger(w, h, n, isa.default_params(levels=cfg["levels"]))
+    out = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
+    mean_ms, med_ms, _ = timed_runs(torch, lambda: m.run(raw, out), steps, warmup)
+    gt = isa.ground_truth(scfg, m.reference_index, w, h)
+    p_gt = psnr_t(torch, out, gt)
+    m_pairs = isa.lib().isa_shift_tables(n, m.reference_index, 0, 5, None, None, None)
+    res = {"workload": f"c2: {n}-frame {w}x{h} RGGB 14-bit burst, levels 1/2/4/8, tiles 64/32/32/16, 1x merge",
+           "exact": {"value": round(n * (w * h / 1e6) / (med_ms / 1000.0), 3), "median_ms": round(med_ms, 3),
+                     "track_patches_ms": round(m.stage_ms()["track_patches"], 3), "psnr_vs_gt_db": p_gt}}
+    del m
+    res.update(mixed_section(isa, torch, cfg, raw, out, gt, p_gt, steps, warmup, list(XC_CODES), m_pairs))
+    return res
+
+
+def c4_section(isa, torch, refine_steps=3):
+    """Combined largest: 50 x 6000x4000 burst (global shifts U(-128, 128), local flow up to 3 px, 5 % occluders).
+    Pre-alignment, bf16x3 matrix-core tile search, full-frame 2x merge; then the differentiable merge on the
+    merger's flows with `refine_steps` refinement steps against the noiseless 2x rendering (isa_merge_refine_step).
+    The same merge with the exact FP64 tile search is the plain reference for PSNR."""
+    from imagestackalignator_amd.merge_diff import MergeDiff
+
+    cfg = CONFIGS["c4"]
+    w, h, n = cfg["w"], cfg["h"], cfg["n"]
+    scfg = isa.default_synth(seed=20240605, width=w, height=h, frames=n, global_shift_max=cfg["gmax"],
+                             flow_max=cfg["flow"], occluded_fraction=cfg["occ"])
+    raw, truth = isa.synth_burst(scfg)
+    res = {"workload": f"c4: {n}-frame {w}x{h} RGGB 14-bit burst, shifts U(-128,128), flow <= 3 px, 5 % occluders; "
+                       f"pre-alignment, bf16x3 tile search, 2x merge ({2 * w}x{2 * h}), {refine_steps} refinement steps"}
+    outs = {}
+    for name, code in (("plain", 0), ("bf16x3", 1)):
+        p = isa.default_params(levels=cfg["levels"], xc_precision=code, super_resolution=2,
+                               prealign_max_shift=cfg["prealign"])
+        m = isa.BurstMerger(w, h, n, p)
+        out = torch.empty((2 * h, 2 * w, 3), dtype=torch.float32, device="cuda")
+        mean_ms, med_ms, _ = timed_runs(torch, lambda: m.run(raw, out), 2, 1)
+        st = m.stage_ms()
+        outs[name] = out
+        gt = isa.ground_truth(scfg, m.reference_index, 2 * w, 2 * h, 0.5, -0.25, -0.25)
+        res[name] = {"merge_ms": round(med_ms, 2), "value": round(n * (w * h / 1e6) / (med_ms / 1000.0), 3),
+                     "stages_ms": {k: round(v, 3) for k, v in st.items()},
+                     "psnr_vs_gt_db": psnr_t(torch, out, gt), "shifts_vs_truth": shifts_vs_truth(torch, m, truth)}
+        log(f"[bench] c4 {name}: merge {med_ms:.1f} ms, stages {res[name]['stages_ms']}")
+        if name == "bf16x3":
+            res[name]["psnr_vs_plain_db"] = psnr_t(torch, out, outs["plain"])
+            res[name]["exact_search_fraction"] = [round(a / max(1, b), 5) for a, b in m.exact_items()]
+            # refinement on the merger's flows
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            md, flow = MergeDiff.from_merger(m, raw)
+            theta = md.theta_of(p)
+            flow_r, theta_r, losses = md.refine(flow, theta, gt, steps=refine_steps, lr_flow=2.0)
+            rel_out, _ = md.forward(flow_r, theta_r, None)
+            torch.cuda.synchronize()
+            ref_ms = (time.perf_counter() - t0) * 1000.0
+            res["refine"] = {"steps": refine_steps, "ms": round(ref_ms, 1), "losses": [round(v, 3) for v in losses],
+                             "psnr_vs_gt_db": psnr_t(torch, rel_out, gt)}
+            total = med_ms + ref_ms
+            res["value"] = round(n * (w * h / 1e6) / (total / 1000.0), 3)
+            res["total_ms"] = round(total, 1)
+            res["unit"] = "frames*MPix/s (bf16x3 merge at 2x + refinement)"
+            log(f"[bench] c4 refinement: {ref_ms:.1f} ms, losses {res['refine']['losses']}")
+            del md, flow, flow_r, rel_out
+        del m
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,8 +487,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--xc", default="bf16x3,fp16x3,fp32",
                     help="mixed-precision (MFMA) tile-search variants timed beside the exact path; '' for none")
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-merge section")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-merge section (c3)")
     ap.add_argument("--no-precision", action="store_true", help="skip the errors against the FP64 oracle")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 2x, c2 and c4 sections")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the differentiable-merge section (prints its JSON object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -316,23 +522,14 @@ def main():
     scfg = isa.default_synth(seed=20240601, width=w, height=h, frames=n, global_shift_max=cfg["gmax"],
                              flow_max=cfg["flow"], occluded_fraction=cfg["occ"])
     log(f"[bench] synthesising {n} x {w}x{h} burst")
-    raw, _ = isa.synth_burst(scfg)
-    params = isa.default_params(levels=cfg["levels"])
+    raw, truth = isa.synth_burst(scfg)
+    params = isa.default_params(levels=cfg["levels"], prealign_max_shift=cfg["prealign"])
     merger = isa.BurstMerger(w, h, n, params)
     out = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
-    for i in range(args.warmup):
-        merger.run(raw, out)
-        torch.cuda.synchronize()
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        merger.run(raw, out)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1000.0 / max(1, args.steps)
-    value = n * (w * h / 1e6) / (ms / 1000.0)
+    mean_ms, med_ms, all_ms = timed_runs(torch, lambda: merger.run(raw, out), args.steps, args.warmup)
+    value = n * (w * h / 1e6) / (mean_ms / 1000.0)
     stages = merger.stage_ms()
+    kt = kernel_times(isa, merger, len(cfg["levels"]))
     import ctypes as C
 
     m_pairs = isa.lib().isa_shift_tables(n, merger.reference_index, params.tracking_strategy,
@@ -345,34 +542,62 @@ def main():
         tiles = ((w // l - 2 * m) // t) * ((h // l - 2 * m) // t)
         levels.append({"level": l, "tiles": tiles, "prep_ms": round(buf[0], 4), "track_ms": round(buf[1], 4),
                        "minimize_ms": round(buf[2], 4), "track_gflop": round(track_flops(cfg, i, m_pairs) / 1e9, 2)})
-    log(f"[bench] {ms:.2f} ms/merge, stages: " + ", ".join(f"{k}={v:.2f}" for k, v in stages.items()))
-    # dominant kernel: the tile search (k_track_cols, one launch per level covering every pair and
-    # tile); its longest launch is timed with HIP events recorded on the merger's stream
-    dom = max(levels, key=lambda d: d["track_ms"])
-    dl, dt, dm = cfg["levels"][levels.index(dom)]
-    achieved = dom["track_gflop"] / dom["track_ms"]  # GFLOP/ms = TFLOP/s
-    roofline = {
-        "kernel": f"k_track_cols<{dm}> at level {dom['level']} (tile {dt}, max shift {dm}): all {m_pairs} pairs x "
-                  f"{dom['tiles']} tiles in one launch",
-        "bound": "mfma",
-        "achieved": round(achieved, 3),
-        "peak": PEAK_FP64_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-        "traffic": None,
-        "note": "compute-bound FP64 FMA work of the exact SSD search (2 T^2 (2M+1)^2 flops per pair and tile); "
-                "peak = MI355X FP64 78.6 TFLOP/s; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / "
-                "WRITE_SIZE, scaled by a plain copy of known size",
-    }
-    traffic_file = ROOT / "profiles" / "track_level_traffic.json"
-    if traffic_file.exists():
+    log(f"[bench] {mean_ms:.2f} ms/merge (median {med_ms:.2f}), stages: "
+        + ", ".join(f"{k}={v:.2f}" for k, v in stages.items()))
+    # H2D-inclusive: the raw frames start in pinned host memory and are uploaded on the merger's stream
+    raw_host = raw.cpu().pin_memory()
+    raw_dev2 = torch.empty_like(raw)
+
+    def with_upload():
+        raw_dev2.copy_(raw_host, non_blocking=True)
+        merger.run(raw_dev2, out)
+
+    h2d_mean, h2d_med, _ = timed_runs(torch, with_upload, args.steps, 1)
+    del raw_dev2, raw_host
+    # per-kernel rooflines from the live HIP-event timings of the last run
+    traffic = {}
+    if KERNEL_TRAFFIC.exists():
         try:
-            t = json.loads(traffic_file.read_text())
-            key = f"{args.config}:level{dom['level']}"
-            if key in t.get("bytes_per_launch", {}):
-                roofline["traffic"] = t["bytes_per_launch"][key]
-        except Exception:
-            pass
+            traffic = json.loads(KERNEL_TRAFFIC.read_text()).get("bytes_per_launch", {})
+        except (OSError, ValueError):
+            traffic = {}
+    P = w * h
+    lk_ms, lk_n = kt["lucas_kanade"][0]
+    lk_avg = lk_ms / max(1, lk_n)
+    lk_bytes = 24.0 * P  # source, target, flow in (8 B), flow out (8 B) per pixel and iteration
+    lk_gbs = lk_bytes / 1e6 / lk_avg if lk_avg > 0 else 0.0
+    roofline = {
+        "kernel": f"k_lk_fused<5>: one Lucas-Kanade iteration of one frame ({w}x{h}), warp + derivatives + 11x11 "
+                  f"window sums + solve; {lk_n} launches per merge",
+        "bound": "hbm",
+        "achieved": round(lk_gbs, 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(lk_gbs / PEAK_HBM_GBS, 4),
+        "traffic": traffic.get("k_lk_fused<5>"),
+        "traffic_source": str(KERNEL_TRAFFIC.relative_to(ROOT)) if "k_lk_fused<5>" in traffic else None,
+        "avg_launch_ms": round(lk_avg, 4),
+        "algorithmic_bytes": lk_bytes,
+        "note": "the dominant kernel by time; algorithmic bytes 24 per pixel and iteration, timed live with HIP "
+                "events on the merger's stream around every frame's iterations; in practice bound by VALU issue",
+    }
+    kernels = [roofline]
+    for i, (ms_l, cnt) in enumerate(kt["track"]):
+        fl = track_flops(cfg, i, m_pairs)
+        tf = fl / 1e9 / ms_l if ms_l > 0 else 0.0
+        l, t, mm = cfg["levels"][i]
+        key = f"k_track_cols<{mm}>:level{l}"
+        kernels.append({"kernel": f"k_track_cols<{mm}> level {l} (tile {t}, max shift {mm}): {m_pairs} pairs x "
+                                  f"{levels[i]['tiles']} tiles in one launch",
+                        "bound": "fp64-valu", "achieved": round(tf, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tf / PEAK_FP64_TFLOPS, 4), "avg_launch_ms": round(ms_l, 4),
+                        "traffic": traffic.get(key)})
+    acc_ms = kt["accumulate"][0][0]
+    acc_bytes = P * (14.0 * n + 36.0)
+    kernels.append({"kernel": f"k_accumulate_frames_wide: all {n} frames in one pass", "bound": "hbm",
+                    "achieved": round(acc_bytes / 1e6 / acc_ms, 1) if acc_ms > 0 else 0.0, "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": round(acc_bytes / 1e6 / acc_ms / PEAK_HBM_GBS, 4) if acc_ms > 0 else 0.0,
+                    "avg_launch_ms": round(acc_ms, 4), "traffic": traffic.get("k_accumulate_frames_wide")})
     result = {
         "metric": "burst frames·MPix/s merged (FP32 plain vs MFMA mixed) at stated PSNR vs FP64",
         "value": round(value, 3),
@@ -380,63 +605,62 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(mean_ms, 3),
+        "median_ms": round(med_ms, 3),
+        "value_median": round(n * (w * h / 1e6) / (med_ms / 1000.0), 3),
+        "step_ms": [round(v, 3) for v in all_ms],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: {n}-frame {w}x{h} RGGB 14-bit burst, levels "
+        "config": {"workload": f"{args.config}: {n}-frame {w}x{h} RGGB 14-bit burst, global shifts U(-{cfg['gmax']:g}, "
+                               f"{cfg['gmax']:g}) px with pre-alignment (+-{cfg['prealign']} px), levels "
                                + "/".join(str(l[0]) for l in cfg["levels"]) + ", tiles "
                                + "/".join(str(l[1]) for l in cfg["levels"]) + ", 1x merge",
                    "frames": n, "width": w, "height": h, "pairs": m_pairs},
+        "h2d_inclusive": {"value": round(n * (w * h / 1e6) / (h2d_med / 1000.0), 3), "median_ms": round(h2d_med, 3),
+                          "upload_bytes": int(raw.numel() * 2),
+                          "note": "raw frames copied from pinned host memory on the merge stream, then merged"},
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
         "stage_roofline": stage_rooflines(stages, n, w, h, levels),
         "levels": levels,
         "roofline": roofline,
+        "roofline_kernels": kernels,
     }
-    import numpy as np
-
-    def psnr(x, y):
-        d = (x - y)[4:-4, 4:-4].double()
-        fin = torch.isfinite(d)
-        mse = float(torch.mean(d[fin] ** 2))
-        return 99.0 if mse == 0 else round(10 * np.log10(1.0 / mse), 3)
-
     gt = isa.ground_truth(scfg, merger.reference_index, w, h)
     torch.cuda.synchronize()
-    result["psnr_vs_gt_db"] = psnr(out, gt)
-    # mixed-precision tile search: same workload, cross terms on MFMA (the exact FP64 path above is
-    # the reference for its error)
-    codes = {"bf16x3": 1, "fp16x3": 2, "fp32": 3}
-    mixed = {}
-    for name in [v for v in args.xc.split(",") if v]:
-        pm = isa.default_params(levels=cfg["levels"], xc_precision=codes[name])
-        mm = isa.BurstMerger(w, h, n, pm)
-        o2 = torch.empty_like(out)
-        for _ in range(args.warmup):
-            mm.run(raw, o2)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            mm.run(raw, o2)
-        torch.cuda.synchronize()
-        ms2 = (time.perf_counter() - t0) * 1000.0 / max(1, args.steps)
-        st2 = mm.stage_ms()
-        ex_items = mm.exact_items()
-        p_gt = psnr(o2, gt)
-        mixed[name] = {"value": round(n * (w * h / 1e6) / (ms2 / 1000.0), 3), "ms_per_step": round(ms2, 3),
-                       "track_patches_ms": round(st2["track_patches"], 3),
-                       "psnr_vs_exact_db": psnr(o2, out), "psnr_vs_gt_db": p_gt,
-                       "psnr_loss_vs_exact_db": round(result["psnr_vs_gt_db"] - p_gt, 4),
-                       "exact_search_fraction": [round(a / max(1, b), 5) for a, b in ex_items]}
-        log(f"[bench] xc {name}: {ms2:.2f} ms/merge, track {st2['track_patches']:.2f} ms, items searched exactly "
-            f"per level {ex_items}")
-        del mm
-    if mixed:
-        result["mixed"] = mixed
+    result["psnr_vs_gt_db"] = psnr_t(torch, out, gt)
+    result["shifts_vs_truth"] = shifts_vs_truth(torch, merger, truth)
+    log(f"[bench] shifts vs truth: {result['shifts_vs_truth']}")
+    names = [v for v in args.xc.split(",") if v]
+    if names:
+        result["mixed"] = mixed_section(isa, torch, cfg, raw, out, gt, result["psnr_vs_gt_db"], args.steps,
+                                        args.warmup, names, m_pairs)
+    del merger
+    if not args.no_extra and args.config == "full_fp32":
+        # the same burst merged at 2x over the whole frame (12000 x 8000 output)
+        p2 = isa.default_params(levels=cfg["levels"], super_resolution=2, prealign_max_shift=cfg["prealign"])
+        m2 = isa.BurstMerger(w, h, n, p2)
+        out2 = torch.empty((2 * h, 2 * w, 3), dtype=torch.float32, device="cuda")
+        mean2, med2, _ = timed_runs(torch, lambda: m2.run(raw, out2), args.steps, args.warmup)
+        gt2 = isa.ground_truth(scfg, m2.reference_index, 2 * w, 2 * h, 0.5, -0.25, -0.25)
+        result["full_2x"] = {"workload": f"the same burst merged at 2x over the whole frame ({2 * w}x{2 * h})",
+                             "value": round(n * (w * h / 1e6) / (med2 / 1000.0), 3), "median_ms": round(med2, 3),
+                             "stages_ms": {k: round(v, 3) for k, v in m2.stage_ms().items()},
+                             "psnr_vs_gt_db": psnr_t(torch, out2, gt2)}
+        log(f"[bench] 2x: {med2:.2f} ms/merge, stages {result['full_2x']['stages_ms']}")
+        del m2, out2, gt2
+    del raw
+    torch.cuda.empty_cache()
+    if not args.no_extra and args.config == "full_fp32":
+        log("[bench] c2 precision workload")
+        result["c2"] = c2_section(isa, torch, args.steps, args.warmup)
+        torch.cuda.empty_cache()
+        log("[bench] c4 combined workload")
+        result["c4"] = c4_section(isa, torch)
+        torch.cuda.empty_cache()
     if not args.no_adjoint:
-        del merger
         log("[bench] differentiable merge")
         result["adjoint"] = adjoint_bench(isa, torch, args.steps, args.warmup)
         log(f"[bench] adjoint: {json.dumps(result['adjoint'])}")
@@ -446,7 +670,7 @@ def main():
         log(f"[bench] precision: {json.dumps(result['precision_vs_fp64'])}")
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
-        result["cpu_baseline"] = cpu_baseline(cfg, 20.0)
+        result["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(result), flush=True)
 
 

@@ -514,9 +514,63 @@ int masks(const DiffArgs& a, const Ws& w, hipStream_t s) {
     return isa::check_launch("merge_diff: mask");
 }
 
+// Candidate of one refinement step: flow - clip(step g, +-max_step), kept within 0.5 px of the anchor (the
+// fixed tap centre) so the 5 x 5 tap window stays centred on the sample position.
+__global__ void k_refine_candidate(const float* flow, const float* g, const float* anchor, float* cand, size_t n,
+                                   float step, float max_step) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float d = fminf(fmaxf(step * g[i], -max_step), max_step);
+    const float a = anchor[i];
+    cand[i] = fminf(fmaxf(flow[i] - d, a - 0.5f), a + 0.5f);
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t isa_merge_refine_workspace(const isa_merge_diff* d) {
+    if (check_problem(d) != ISA_OK) return 0;
+    const size_t nf = (size_t)d->frames * d->width * d->height * 2 * sizeof(float);
+    return ((ws_layout(d, nullptr).bytes + 255) & ~(size_t)255) + 2 * ((nf + 255) & ~(size_t)255);
+}
+
+int isa_merge_refine_step(const isa_merge_diff* d, float* flow_dev, float theta[4], const float* gt3_dev,
+                          float lr_flow, const float lr_theta[4], float max_step, float* scale, double* loss,
+                          int* accepted, void* workspace, isa_stream_t stream) {
+    ISA_TRY(check_problem(d));
+    ISA_REQUIRE(flow_dev && theta && gt3_dev && scale && loss && workspace && lr_flow > 0.0f && max_step > 0.0f &&
+                    *scale > 0.0f,
+                "isa_merge_refine_step: bad arguments");
+    hipStream_t s = isa::to_stream(stream);
+    const size_t nf = (size_t)d->frames * d->width * d->height * 2;
+    char* base = static_cast<char*>(workspace);
+    const size_t wsb = (ws_layout(d, nullptr).bytes + 255) & ~(size_t)255;
+    float* g = (float*)(base + wsb);
+    float* cand = (float*)(base + wsb + ((nf * sizeof(float) + 255) & ~(size_t)255));
+    float gth[4];
+    ISA_TRY(isa_merge_backward(d, flow_dev, theta, gt3_dev, g, gth, workspace, stream));
+    if (accepted) *accepted = 0;
+    for (int attempt = 0; attempt < 9; ++attempt) {
+        k_refine_candidate<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(flow_dev, g, d->anchor_dev, cand, nf,
+                                                                      *scale * lr_flow, max_step);
+        ISA_TRY(isa::check_launch("isa_merge_refine_step: candidate"));
+        float cth[4];
+        for (int k = 0; k < 4; ++k) cth[k] = lr_theta ? theta[k] - *scale * lr_theta[k] * gth[k] : theta[k];
+        double l = 0.0;
+        ISA_TRY(isa_merge_forward(d, cand, cth, nullptr, gt3_dev, &l, workspace, stream));
+        if (l <= *loss) {
+            ISA_TRY(isa::check_hip(hipMemcpyAsync(flow_dev, cand, nf * sizeof(float), hipMemcpyDeviceToDevice, s),
+                                   "isa_merge_refine_step: accept"));
+            for (int k = 0; k < 4; ++k) theta[k] = cth[k];
+            *loss = l;
+            if (accepted) *accepted = 1;
+            return ISA_OK;
+        }
+        *scale *= 0.5f;
+    }
+    return ISA_OK;
+}
 
 size_t isa_merge_diff_workspace(const isa_merge_diff* d) {
     if (check_problem(d) != ISA_OK) return 0;

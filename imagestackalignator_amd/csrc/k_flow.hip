@@ -835,7 +835,7 @@ int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s) {
 int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles, int tiles_pitch, int tw, int th,
                  float* flow, int flow_pitch, float* work, int w, int h, int iterations, float bsx, float bsy,
                  float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps,
-                 float ill_ratio) {
+                 float ill_ratio, hipEvent_t* iter_ev) {
     const size_t np = (size_t)w * h;
     float* tmp = work;
     float* ix = work + np;
@@ -852,6 +852,7 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
         int cur = iterations % 2;  // buffer holding the initial field
         ISA_TRY(launch_flow_from_tiles(bufs[cur], flow_pitch, tiles, tiles_pitch, tw, th, w, h, bsx, bsy, rot, quant,
                                        s));
+        if (iter_ev) (void)hipEventRecord(iter_ev[0], s);
         const Tap* col = (const Tap*)taps;
         const dim3 grid(div_up(w, LKF_BX), div_up(h, LKF_BY)), block(LKF_BX * (LKF_BY / LKF_RPT));
         for (int it = 0; it < iterations; ++it) {
@@ -867,14 +868,17 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
             ISA_TRY(check_launch("lucas_kanade_fused"));
             cur ^= 1;
         }
+        if (iter_ev) (void)hipEventRecord(iter_ev[1], s);
         return ISA_OK;
     }
     ISA_TRY(launch_flow_from_tiles(flow, flow_pitch, tiles, tiles_pitch, tw, th, w, h, bsx, bsy, rot, quant, s));
+    if (iter_ev) (void)hipEventRecord(iter_ev[0], s);
     for (int it = 0; it < iterations; ++it) {
         ISA_TRY(launch_warp(src, img_pitch, tmp, wp, flow, flow_pitch, w, h, quant, s));
         ISA_TRY(launch_derivatives(tmp, tgt, wp, ix, iy, iz, wp, w, h, quant, taps, s));
         ISA_TRY(launch_lk_step(flow, flow_pitch, ix, iy, iz, wp, w, h, window, min_det, box, ill_ratio, s));
     }
+    if (iter_ev) (void)hipEventRecord(iter_ev[1], s);
     return ISA_OK;
 }
 
@@ -932,7 +936,7 @@ int isa_lucas_kanade(float* src_dev, const float* tgt_dev, int img_pitch, const 
     hipStream_t s = isa::to_stream(stream);
     ISA_TRY(isa::lucas_kanade(src_dev, tgt_dev, img_pitch, tiles2_dev, tiles_pitch, tex_w, tex_h, flow2_dev, flow_pitch,
                               work_dev, width, height, iterations, base_shift[0], base_shift[1], base_rotation, min_det,
-                              window_size, tex_quant, false, s, nullptr, 0.0f));
+                              window_size, tex_quant, false, s, nullptr, 0.0f, nullptr));
     // final warp, copied over the source image (R:PEFStudioDX/OpticalFlow.cs:162-163)
     ISA_TRY(isa::launch_warp(src_dev, img_pitch, work_dev, width * (int)sizeof(float), flow2_dev, flow_pitch, width,
                              height, tex_quant, s));

@@ -47,7 +47,7 @@ int launch_derivatives2(const float* img, int img_pitch, float* ix, float* iy, i
 int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles, int tiles_pitch, int tw, int th,
                  float* flow, int flow_pitch, float* work, int w, int h, int iterations, float bsx, float bsy,
                  float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps,
-                 float ill_ratio);
+                 float ill_ratio, hipEvent_t* iter_ev);
 size_t tap_table_bytes(int w, int h);
 int track_chunks(const int* pa, int m, int n, int* out_chunks);
 int track_chunks_capacity(int m, int n);
@@ -159,7 +159,10 @@ struct isa_merger {
     hipEvent_t pre_ev[2]{};      // start and end of the pre-alignment
     hipEvent_t upload_done = nullptr;
     hipEvent_t ev[ST_COUNT + 1]{};
-    hipEvent_t lev_ev[ISA_MAX_LEVELS][4]{};  // per level: start, track launch, minimizer launch, end
+    // per level: start, track launch, minimizer launch, end, end of the main track launch (the matrix-core
+    // search before its exact-search queue)
+    hipEvent_t lev_ev[ISA_MAX_LEVELS][5]{};
+    hipEvent_t lk_ev[64][2]{};  // per frame: start and end of the Lucas-Kanade iterations
     float stage_ms[ST_COUNT]{};
     bool have_times = false;
     int ref = 0;
@@ -176,6 +179,9 @@ struct isa_merger {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& l : lev_ev)
+            for (auto& e : l)
+                if (e) (void)hipEventDestroy(e);
+        for (auto& l : lk_ev)
             for (auto& e : l)
                 if (e) (void)hipEventDestroy(e);
     }
@@ -402,7 +408,9 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
     if (rc == ISA_OK) rc = isa::check_hip(hipEventCreateWithFlags(&M->upload_done, hipEventDisableTiming), "event");
     for (int i = 0; i <= ST_COUNT && rc == ISA_OK; ++i) rc = isa::check_hip(hipEventCreate(&M->ev[i]), "event");
     for (int i = 0; i < M->L && rc == ISA_OK; ++i)
-        for (int j = 0; j < 4 && rc == ISA_OK; ++j) rc = isa::check_hip(hipEventCreate(&M->lev_ev[i][j]), "event");
+        for (int j = 0; j < 5 && rc == ISA_OK; ++j) rc = isa::check_hip(hipEventCreate(&M->lev_ev[i][j]), "event");
+    for (int f = 0; f < frames && rc == ISA_OK; ++f)
+        for (int j = 0; j < 2 && rc == ISA_OK; ++j) rc = isa::check_hip(hipEventCreate(&M->lk_ev[f][j]), "event");
     // derivative tap tables depend only on (width, height, tex_quant): built once, on the null stream,
     // and complete before create returns
     if (rc == ISA_OK) rc = isa::build_tap_tables(M->taps, width, height, P.tex_quant, 0);
@@ -612,6 +620,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                             P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk, m,
                                             d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold,
                                             nullptr, nullptr, s));
+            (void)hipEventRecord(M->lev_ev[i][4], s);
         } else {
             ISA_TRY(isa::launch_track_mfma(P.xc_precision, d_frames + (size_t)i * n, d_xframes + (size_t)i * n,
                                            d_vframes + (size_t)i * n, d_rframes + (size_t)i * n,
@@ -619,6 +628,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                            M->cy[i], d_pa, d_pb, d_pk, d_chunks, n_chunks, d_base + (size_t)i * n * 4,
                                            M->shifts, mstride, P.matching_threshold, nullptr, nullptr, M->xc_queue,
                                            M->xc_count + i, s));
+            (void)hipEventRecord(M->lev_ev[i][4], s);
             ISA_TRY(isa::launch_track_queue(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
                                             P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk,
                                             d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold,
@@ -668,7 +678,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         ISA_TRY(isa::lucas_kanade(M->lk + (size_t)f * np, lk_ref, wp, M->tile_img, M->cx[0] * 8, M->cx[0], M->cy[0],
                                   M->flows + (size_t)f * np * 2, 2 * wp, M->lk_work, w, h, P.iterations_lk, bsx, bsy,
                                   store_rotation(st, ref, f), P.min_det_lk, P.window_size_lk, P.tex_quant, true, s,
-                                  M->taps, P.lk_ill_ratio));
+                                  M->taps, P.lk_ill_ratio, M->lk_ev[f]));
     }
     (void)hipEventRecord(M->ev[5], s);
     for (int f = 0; f < n; ++f) {
@@ -755,6 +765,39 @@ int isa_merger_level_ms(isa_merger* M, int level, float ms[3]) {
         ms[j] = 0;
         (void)hipEventElapsedTime(&ms[j], M->lev_ev[level][j], M->lev_ev[level][j + 1]);
     }
+    return ISA_OK;
+}
+
+int isa_merger_kernel_ms(isa_merger* M, int kernel, int level, float* ms, int* launches) {
+    ISA_REQUIRE(M && ms && launches && kernel >= 0 && kernel <= 3 && level >= -1 && level < M->L,
+                "isa_merger_kernel_ms: bad arguments");
+    if (!M->have_times) return ISA_ERR_ARG;
+    ISA_TRY(isa::check_hip(hipEventSynchronize(M->ev[ST_COUNT]), "isa_merger_kernel_ms: sync"));
+    float tot = 0.0f;
+    int cnt = 0;
+    auto el = [](hipEvent_t a, hipEvent_t b) {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, a, b);
+        return t;
+    };
+    if (kernel == 0) {  // Lucas-Kanade iterations of every non-reference frame
+        for (int f = 0; f < M->n; ++f) {
+            if (f == M->ref) continue;
+            tot += el(M->lk_ev[f][0], M->lk_ev[f][1]);
+            cnt += M->P.iterations_lk;
+        }
+    } else if (kernel == 1 || kernel == 2) {  // tile-search launch / exact-search queue per level
+        for (int i = 0; i < M->L; ++i) {
+            if (level >= 0 && i != level) continue;
+            tot += kernel == 1 ? el(M->lev_ev[i][1], M->lev_ev[i][4]) : el(M->lev_ev[i][4], M->lev_ev[i][2]);
+            cnt += kernel == 1 || M->P.xc_precision != ISA_XC_FP64 ? 1 : 0;
+        }
+    } else {  // accumulation of every frame
+        tot = el(M->ev[ST_ACCUM], M->ev[ST_ACCUM + 1]);
+        cnt = 1;
+    }
+    *ms = tot;
+    *launches = cnt;
     return ISA_OK;
 }
 

@@ -95,6 +95,15 @@ class BurstMerger:
     def reference_index(self) -> int:
         return lib().isa_merger_reference_index(self._h)
 
+    KERNELS = {"lucas_kanade": 0, "track": 1, "exact_queue": 2, "accumulate": 3}
+
+    def kernel_ms(self, kernel: str, level: int = -1):
+        """(device ms, launches) of the last run spent in one kernel family (KERNELS); `level` selects one pyramid
+        level of the tile search (-1: all)."""
+        ms, cnt = C.c_float(), C.c_int()
+        call("isa_merger_kernel_ms", self._h, self.KERNELS[kernel], level, C.byref(ms), C.byref(cnt))
+        return ms.value, cnt.value
+
     def stage_ms(self):
         buf = (C.c_float * 16)()
         n = lib().isa_merger_stage_ms(self._h, buf, 16)

@@ -110,6 +110,7 @@ class MergeDiff:
         if nbytes == 0:
             call("isa_merge_forward", C.byref(d), None, None, None, None, None, None, None)  # raises with the reason
         self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=raw.device)
+        self.refine_workspace = None  # allocated by the first refine()
 
     @classmethod
     def from_merger(cls, merger, raw):
@@ -170,32 +171,26 @@ class MergeDiff:
         return dflow, list(dth)
 
     def refine(self, flow, theta, gt, steps: int, lr_flow: float, lr_theta=None, max_step: float = 0.25):
-        """Gradient descent with backtracking on the flow (each step clipped to max_step px, the flow kept
-        within 0.5 px of its anchor so the tap window stays centred) and, with lr_theta (4 step sizes), on
-        theta. A step that raises the loss is retried at half the step size (at most 8 times). Returns
-        (flow, theta, losses: the initial loss and the loss after each step)."""
+        """`steps` calls of isa_merge_refine_step: gradient descent with backtracking on the flow (each step clipped
+        to max_step px, the flow kept within 0.5 px of its anchor so the tap window stays centred) and, with
+        lr_theta (4 step sizes), on theta. A step that raises the loss is retried at half the step size (at most 9
+        tries). Returns (flow, theta, losses: the initial loss and the loss after each step)."""
         torch = _torch()
+        self._flow(flow)
+        self._gt(gt)
         flow = flow.clone()
-        theta = [float(v) for v in theta]
-        loss = self.forward(flow, theta, gt)[1]
-        losses = [loss]
-        scale = 1.0
+        if self.refine_workspace is None:
+            nbytes = lib().isa_merge_refine_workspace(C.byref(self.problem))
+            self.refine_workspace = torch.empty(nbytes, dtype=torch.uint8, device=flow.device)
+        th = (C.c_float * 4)(*[float(v) for v in theta])
+        lr_th = None if lr_theta is None else (C.c_float * 4)(*[float(v) for v in lr_theta])
+        loss = C.c_double(self.forward(flow, list(th), gt)[1])
+        losses = [loss.value]
+        scale = C.c_float(1.0)
+        acc = C.c_int(0)
         for _ in range(steps):
-            g, gth = self.backward(flow, theta, gt)
-            g = g.clone()
-            for _try in range(9):
-                cand = flow - torch.clamp(scale * lr_flow * g, -max_step, max_step)
-                cand = torch.minimum(torch.maximum(cand, self.anchor - 0.5), self.anchor + 0.5).contiguous()
-                cth = list(theta)
-                if lr_theta is not None:
-                    cth = [t - scale * l * d for t, l, d in zip(theta, lr_theta, gth)]
-                new = self.forward(cand, cth, gt)[1]
-                if new <= loss:
-                    break
-                scale *= 0.5
-            else:
-                losses.append(loss)
-                continue
-            flow, theta, loss = cand, cth, new
-            losses.append(loss)
-        return flow, theta, losses
+            call("isa_merge_refine_step", C.byref(self.problem), C.c_void_p(flow.data_ptr()), th,
+                 C.c_void_p(gt.data_ptr()), C.c_float(lr_flow), lr_th, C.c_float(max_step), C.byref(scale),
+                 C.byref(loss), C.byref(acc), C.c_void_p(self.refine_workspace.data_ptr()), _stream_ptr())
+            losses.append(loss.value)
+        return flow, list(th), losses

@@ -396,6 +396,12 @@ int isa_merger_stage_ms(isa_merger* m, float* ms, int capacity);
 /* Milliseconds of the last run at pyramid level `level` (0 = finest): ms[0] shift up-sampling and level
  * images, ms[1] the fused tile-search launch (all pairs), ms[2] the ShiftMinimizer launch. */
 int isa_merger_level_ms(isa_merger* m, int level, float ms[3]);
+/* Device time of the last run spent in one kernel family, from HIP events on the merger's stream, and its number of
+ * launches: kernel 0 the Lucas-Kanade iterations (fused warp, derivatives, window sums, solve; every non-reference
+ * frame), 1 the tile-search launch at pyramid level `level` (-1: all levels; the matrix-core search for a mixed
+ * precision), 2 the exact-search queue after a matrix-core search, 3 the accumulation of every frame. No counterpart
+ * in the reference. */
+int isa_merger_kernel_ms(isa_merger* m, int kernel, int level, float* ms, int* launches);
 /* (pair, tile) items of the last run at pyramid level `level`, and how many of them the exact FP64 search decided
  * (all of them for ISA_XC_FP64; with a matrix-core precision, those whose error bounds left the integer minimum or
  * the threshold test open). No counterpart in the reference. */
@@ -433,6 +439,19 @@ int isa_merge_forward(const isa_merge_diff* problem, const float* flow_dev, cons
 /* Backward of the loss: dflow (frames x H x W x 2, device) and dtheta (host, 4 floats). */
 int isa_merge_backward(const isa_merge_diff* problem, const float* flow_dev, const float theta[4], const float* gt3_dev,
                        float* dflow_dev, float dtheta[4], void* workspace, isa_stream_t stream);
+
+/* One step of joint refinement of the flow (and optionally theta) by gradient descent with backtracking, the
+ * refinement the differentiable merge exists for: the gradient at (flow, theta), then candidates
+ * flow - clip(scale lr_flow dflow, +-max_step), each kept within 0.5 px of its anchor, and theta - scale lr_theta
+ * dtheta (lr_theta may be NULL: theta fixed), tried at most 9 times with scale halved after each one that raises
+ * the loss. The first that does not replaces flow_dev / theta / loss (*accepted = 1); *scale keeps its halvings
+ * for the next step. loss (in/out, host): the loss at (flow, theta) on entry. workspace:
+ * isa_merge_refine_workspace() bytes (the forward/backward workspace plus the gradient and a candidate flow).
+ * Synchronises the stream (losses are read on the host). No counterpart in the reference. */
+size_t isa_merge_refine_workspace(const isa_merge_diff* problem);
+int isa_merge_refine_step(const isa_merge_diff* problem, float* flow_dev, float theta[4], const float* gt3_dev,
+                          float lr_flow, const float lr_theta[4], float max_step, float* scale, double* loss,
+                          int* accepted, void* workspace, isa_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Synthetic bursts (benchmark / test input; not part of the reference).

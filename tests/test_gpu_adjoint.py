@@ -114,7 +114,12 @@ def test_refinement_lowers_loss_on_a_synthetic_burst(isa, gpu):
     md, flow = MergeDiff.from_merger(m, raw)
     gt = isa.ground_truth(cfg, m.reference_index, 2 * w, 2 * h, 0.5, -0.25, -0.25)
     theta = md.theta_of(m.params)
-    _, _, losses = md.refine(flow, theta, gt, steps=5, lr_flow=2.0)
+    new_flow, new_theta, losses = md.refine(flow, theta, gt, steps=5, lr_flow=2.0)
     print("  losses", ", ".join(f"{v:.4f}" for v in losses))
     assert all(b <= a for a, b in zip(losses, losses[1:]))
     assert losses[-1] < 0.995 * losses[0]
+    # the step's reported loss is the forward loss at the flow it returned (the forward is deterministic), the
+    # flow stays within 0.5 px of its anchor and theta is unchanged without lr_theta
+    assert md.forward(new_flow, new_theta, gt)[1] == losses[-1]
+    assert float((new_flow - md.anchor).abs().max()) <= 0.5 + 1e-6
+    assert new_theta == [float(np.float32(v)) for v in theta]

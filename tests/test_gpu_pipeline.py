@@ -174,6 +174,13 @@ def test_merge_is_deterministic_and_reusable(isa, gpu):
     np.testing.assert_array_equal(a, b)
     st = m.stage_ms()
     assert set(st) >= {"tracking_images", "track_patches", "accumulate"}
+    # kernel timings of the last run: LK iterations of the 3 non-reference frames, one tile-search launch per
+    # level, the accumulation within its stage
+    lk_ms, lk_n = m.kernel_ms("lucas_kanade")
+    assert lk_n == 3 * params.iterations_lk and 0.0 < lk_ms <= st["lucas_kanade"] + 1e-3
+    assert [m.kernel_ms("track", lv)[1] for lv in range(2)] == [1, 1]
+    acc_ms, acc_n = m.kernel_ms("accumulate")
+    assert acc_n == 1 and 0.0 < acc_ms <= st["accumulate"] + 1e-3
 
 
 def test_merger_accumulation_is_bit_exact_on_its_own_inputs(isa, oracle, gpu):
