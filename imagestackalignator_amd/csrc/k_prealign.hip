@@ -5,11 +5,11 @@
 // cross-correlation c(s) = sum_x r(x) t((x + s) mod (W, H)) (conjugateComplexMulKernel on R2C spectra,
 // C2R, divided by W H), takes the first maximum, wraps it into (-W/2, W/2] x (-H/2, H/2] and returns
 // minus it. Here the same c(s) is evaluated directly at the lags that can hold the maximum: every lag
-// of the search radius on a box-downsampled image (the coarsest level whose radius is at most 32 lags),
-// then a window around twice the previous maximum at each finer level (15 x 15 lags, 5 x 5 at full
-// resolution). Sums are FP64 per thread and
-// reduced across blocks in a fixed order, so the result is deterministic; nothing is allocated and the
-// host never waits (the window centre of each level is read on the device).
+// of the search radius on a box-downsampled image (the first level whose radius is at most 8 lags, or
+// the last whose sides stay at least 64 px), then a window around twice the previous maximum at each
+// finer level (15 x 15 lags, 5 x 5 at full resolution). Sums are FP64 per thread and reduced across
+// blocks in a fixed order, so the result is deterministic; nothing is allocated and the host never
+// waits (the window centre of each level is read on the device).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -25,53 +25,74 @@ int launch_resize_box(const float* in, int in_pitch, float* out, int out_pitch, 
 namespace {
 
 constexpr int PA_K = 5, PA_LAGS = PA_K * PA_K;  // lags per tile: 5 x 5
-constexpr int PA_THREADS = 256;
+constexpr int PA_THREADS = 256;                 // columns per block, one per thread
 constexpr int PA_MAX_LEVELS = 6;               // levels 1, 2, 4, 8, 16, 32
+constexpr int PA_COARSE_RADIUS = 8;            // lags each way at the coarsest level (before the slack)
+constexpr int PA_MIN_SIDE = 64;                // smallest side of the coarsest level
+constexpr int PA_MIN_ROWS = 16;                // smallest row chunk
 
 struct LagWindow {
     int tiles;       // lag tiles per row and column (odd); lag tile (tx, ty) covers 5 x 5 lags
     int radius;      // the window is centre + [-radius, radius]^2, radius = (5 tiles - 1) / 2
 };
 
-// Partial c(s) over pixel chunk blockIdx.x for the 25 lags of lag tile blockIdx.y: s = centre + o with
-// o in the tile. centre = scale * (*centre_in) (the previous level's maximum), or 0 without one.
-// Lags are reduced modulo the image size first, so one wrap per coordinate suffices.
-__global__ __launch_bounds__(PA_THREADS) void k_cc_partial(const float* __restrict__ a, const float* __restrict__ b,
-                                                           int pitch, int w, int h, const int* centre_in, int scale,
-                                                           LagWindow win, int chunk, double* partial) {
-    const int tile = blockIdx.y, ty = tile / win.tiles, tx = tile - ty * win.tiles;
+// Partial c(s) over one (column block, row chunk) of the image for the 25 lags of lag tile blockIdx.z:
+// s = centre + o with o in the tile, centre = scale * (*centre_in) (the previous level's maximum), or 0.
+// Thread x walks the rows of its column: each b row is loaded once (5 neighbouring columns) and meets the
+// 5 a rows of the tile's row lags, held in a ring of registers (zero outside the chunk), so a pixel costs
+// 6 loads for 25 FP64 products (float products are exact in FP64). Lags are reduced modulo the image
+// size first, so one wrap per coordinate suffices.
+__global__ __launch_bounds__(PA_THREADS) void k_cc_rows(const float* __restrict__ a, const float* __restrict__ b,
+                                                        int pitch, int w, int h, const int* centre_in, int scale,
+                                                        LagWindow win, int rows, double* partial) {
+    const int tile = blockIdx.z, ty = tile / win.tiles, tx = tile - ty * win.tiles;
     int cx = 0, cy = 0;
     if (centre_in) {
         cx = centre_in[0] * scale;
         cy = centre_in[1] * scale;
     }
-    int sx0 = cx - win.radius + PA_K * tx, sy0 = cy - win.radius + PA_K * ty;
-    sx0 %= w;
-    sy0 %= h;
+    int sx0 = (cx - win.radius + PA_K * tx) % w, sy0 = (cy - win.radius + PA_K * ty) % h;
     if (sx0 < 0) sx0 += w;
     if (sy0 < 0) sy0 += h;
-    double acc[PA_LAGS];
+    const int x = blockIdx.x * PA_THREADS + threadIdx.x;
+    const bool active = x < w;
+    int xs[PA_K];
+#pragma unroll
+    for (int i = 0; i < PA_K; ++i) {
+        int xx = (active ? x : 0) + sx0 + i;  // < 2 w + 4 <= 3 w (w >= 8)
+        if (xx >= w) xx -= w;
+        if (xx >= w) xx -= w;
+        xs[i] = xx;
+    }
+    const int y0 = blockIdx.y * rows;
+    const int nrows = min(rows, h - y0), tend = nrows + PA_K - 1;
+    int rb = y0 + sy0;  // b row of step t = 0 (y0 < h, sy0 < h)
+    if (rb >= h) rb -= h;
+    double acc[PA_LAGS], ar[PA_K];
 #pragma unroll
     for (int l = 0; l < PA_LAGS; ++l) acc[l] = 0.0;
-    const long long npx = (long long)w * h;
-    const long long p0 = (long long)blockIdx.x * chunk, p1 = p0 + chunk < npx ? p0 + chunk : npx;
-    for (long long p = p0 + threadIdx.x; p < p1; p += PA_THREADS) {
-        const int y = (int)(p / w), x = (int)(p - (long long)y * w);
-        const double av = row_ptr(a, pitch, y)[x];
-        int xs[PA_K];
 #pragma unroll
-        for (int i = 0; i < PA_K; ++i) {
-            int xx = x + sx0 + i;
-            while (xx >= w) xx -= w;
-            xs[i] = xx;
-        }
+    for (int j = 0; j < PA_K; ++j) ar[j] = 0.0;
+    // step t pairs b row y0 + sy0 + t with a rows y0 + t - j (lag row j); a row y0 + t sits in ring slot
+    // t mod 5, so with t0 a multiple of 5 every slot index below is a constant
+    for (int t0 = 0; t0 < tend; t0 += PA_K) {
 #pragma unroll
-        for (int j = 0; j < PA_K; ++j) {
-            int yy = y + sy0 + j;
-            while (yy >= h) yy -= h;
-            const float* br = row_ptr(b, pitch, yy);
+        for (int u = 0; u < PA_K; ++u) {
+            const int t = t0 + u;
+            if (t < tend) {
+                ar[u] = (active && t < nrows) ? (double)row_ptr(a, pitch, y0 + t)[x] : 0.0;
+                const float* br = row_ptr(b, pitch, rb);
+                double bv[PA_K];
 #pragma unroll
-            for (int i = 0; i < PA_K; ++i) acc[j * PA_K + i] += av * (double)br[xs[i]];
+                for (int i = 0; i < PA_K; ++i) bv[i] = active ? (double)br[xs[i]] : 0.0;
+#pragma unroll
+                for (int j = 0; j < PA_K; ++j) {
+                    const double av = ar[(u - j + PA_K) % PA_K];
+#pragma unroll
+                    for (int i = 0; i < PA_K; ++i) acc[j * PA_K + i] = fma(av, bv[i], acc[j * PA_K + i]);
+                }
+                if (++rb == h) rb = 0;
+            }
         }
     }
     // block sum: lanes of a wave, then the waves through LDS
@@ -88,7 +109,8 @@ __global__ __launch_bounds__(PA_THREADS) void k_cc_partial(const float* __restri
     if (threadIdx.x < PA_LAGS) {
         double v = s_part[0][threadIdx.x];
         for (int k = 1; k < PA_THREADS / 64; ++k) v += s_part[k][threadIdx.x];
-        partial[((size_t)tile * gridDim.x + blockIdx.x) * PA_LAGS + threadIdx.x] = v;
+        const int chunks = gridDim.x * gridDim.y, chunk = blockIdx.y * gridDim.x + blockIdx.x;
+        partial[((size_t)tile * chunks + chunk) * PA_LAGS + threadIdx.x] = v;
     }
 }
 
@@ -166,17 +188,25 @@ struct Plan {
     size_t pyr_off[PA_MAX_LEVELS];    // offsets of the downsampled copies (level >= 1) in one pyramid
     size_t pyr_bytes;                 // one pyramid (levels 2, 4, ...)
     LagWindow coarse;
-    int chunk[PA_MAX_LEVELS], chunks[PA_MAX_LEVELS];
+    int rows[PA_MAX_LEVELS], row_chunks[PA_MAX_LEVELS];  // row chunk of the correlation blocks, and their count
     size_t partial_bytes, bytes;      // workspace: two pyramids, partial sums, maxima
 };
 
 size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+LagWindow window_of(const Plan& P, int i) {
+    // the coarsest level searches the whole radius; finer levels 15 x 15 lags around twice the coarser
+    // maximum, full resolution 5 x 5
+    return i == P.levels - 1 ? P.coarse : (i > 0 ? LagWindow{3, 7} : LagWindow{1, 2});
+}
+
 bool make_plan(int w, int h, int max_shift, Plan& P) {
     if (w < 8 || h < 8 || max_shift < 1) return false;
-    // coarsest level: at most 32 lags each way for the search radius, at least 32 x 32 pixels
+    // coarsest level: the first whose search radius is at most PA_COARSE_RADIUS lags, or the last whose
+    // sides stay >= PA_MIN_SIDE
     int L = 1;
-    while (L < PA_MAX_LEVELS && ((max_shift + (1 << (L - 1)) - 1) >> (L - 1)) > 32 && (w >> L) >= 32 && (h >> L) >= 32)
+    while (L < PA_MAX_LEVELS && ((max_shift + (1 << (L - 1)) - 1) >> (L - 1)) > PA_COARSE_RADIUS &&
+           (w >> L) >= PA_MIN_SIDE && (h >> L) >= PA_MIN_SIDE)
         ++L;
     P.levels = L;
     size_t off = 0;
@@ -195,13 +225,14 @@ bool make_plan(int w, int h, int max_shift, Plan& P) {
     P.coarse.radius = (PA_K * P.coarse.tiles - 1) / 2;
     size_t part = 0;
     for (int i = 0; i < L; ++i) {
-        const long long npx = (long long)P.lw[i] * P.lh[i];
-        const int tiles = i == lc ? P.coarse.tiles * P.coarse.tiles : (i > 0 ? 9 : 1);
-        // enough blocks to fill the chip: about 2048 (tile, chunk) blocks, chunks of >= 1024 pixels
-        int chunks = (int)std::max<long long>(1, std::min<long long>((npx + 1023) / 1024, std::max(1, 2048 / tiles)));
-        P.chunk[i] = (int)((npx + chunks - 1) / chunks);
-        P.chunks[i] = (int)((npx + P.chunk[i] - 1) / P.chunk[i]);
-        part = std::max(part, (size_t)tiles * P.chunks[i] * PA_LAGS * sizeof(double));
+        const LagWindow win = window_of(P, i);
+        const int tiles = win.tiles * win.tiles, cols = isa::div_up(P.lw[i], PA_THREADS);
+        // about 2048 (tile, column block, row chunk) blocks, row chunks of at least PA_MIN_ROWS rows
+        const int want = std::max(1, 2048 / (tiles * cols));
+        const int rc_max = isa::div_up(P.lh[i], PA_MIN_ROWS);
+        P.rows[i] = isa::div_up(P.lh[i], std::min(want, rc_max));
+        P.row_chunks[i] = isa::div_up(P.lh[i], P.rows[i]);
+        part = std::max(part, (size_t)tiles * cols * P.row_chunks[i] * PA_LAGS * sizeof(double));
     }
     P.partial_bytes = al256(part);
     P.bytes = 2 * P.pyr_bytes + P.partial_bytes + al256((size_t)PA_MAX_LEVELS * 2 * sizeof(int)) +
@@ -234,8 +265,10 @@ int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int
         return i == 0 ? full : (const float*)(pyr + P.pyr_off[i]);
     };
     auto build = [&](char* pyr, const float* full) -> int {
+        // each level from the one above: a 2 x 2 box of 2^(i-1) boxes is the 2^i box of the same pixels
         for (int i = 1; i < P.levels; ++i)
-            ISA_TRY(launch_resize_box(full, img_pitch, (float*)(pyr + P.pyr_off[i]), P.lpitch[i], w, h, 1 << i, s));
+            ISA_TRY(launch_resize_box(level_ptr(pyr, full, i - 1), i == 1 ? img_pitch : P.lpitch[i - 1],
+                                      (float*)(pyr + P.pyr_off[i]), P.lpitch[i], P.lw[i - 1], P.lh[i - 1], 2, s));
         return ISA_OK;
     };
     k_zero4<<<1, 64, 0, s>>>(shifts4);
@@ -246,17 +279,17 @@ int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int
     for (int f = 1; f < n; ++f) {
         ISA_TRY(build(pyr_trk, frames[f]));
         for (int i = lc; i >= 0; --i) {
-            const bool coarse = i == lc;
-            // intermediate levels search 15 x 15 lags around twice the coarser maximum, full resolution 5 x 5
-            const LagWindow win = coarse ? P.coarse : (i > 0 ? LagWindow{3, 7} : LagWindow{1, 2});
-            const int* centre = coarse ? nullptr : best + 2 * (i + 1);
+            const LagWindow win = window_of(P, i);
+            const int* centre = i == lc ? nullptr : best + 2 * (i + 1);
             const float* a = level_ptr(pyr_ref, frames[0], i);
             const float* b = level_ptr(pyr_trk, frames[f], i);
             const int pitch = i == 0 ? img_pitch : P.lpitch[i];
-            k_cc_partial<<<dim3(P.chunks[i], win.tiles * win.tiles), PA_THREADS, 0, s>>>(
-                a, b, pitch, P.lw[i], P.lh[i], centre, 2, win, P.chunk[i], partial);
+            const int cols = div_up(P.lw[i], PA_THREADS);
+            k_cc_rows<<<dim3(cols, P.row_chunks[i], win.tiles * win.tiles), PA_THREADS, 0, s>>>(
+                a, b, pitch, P.lw[i], P.lh[i], centre, 2, win, P.rows[i], partial);
             ISA_TRY(check_launch("prealign: correlation"));
-            k_cc_argmax<<<1, 256, 0, s>>>(partial, P.chunks[i], centre, 2, win, best + 2 * i, best_value + i);
+            k_cc_argmax<<<1, 256, 0, s>>>(partial, cols * P.row_chunks[i], centre, 2, win, best + 2 * i,
+                                          best_value + i);
             ISA_TRY(check_launch("prealign: maximum"));
         }
         k_cc_result<<<1, 1, 0, s>>>(best, best_value, w, h, shifts4 + 4 * f);

@@ -36,9 +36,10 @@ def _run(isa, frames, max_shift):
 
 
 @pytest.mark.parametrize("h,w,max_shift,shifts", [
-    (256, 320, 64, [(0, 0), (37, -21), (-60, 44), (5, 63)]),                   # coarse level 2
-    (768, 1024, 256, [(0, 0), (-130, 97), (201, -188), (3, -2), (-255, 0)]),   # coarse level 8
-    (250, 333, 40, [(0, 0), (17, -39), (-40, 12)]),                           # odd width, sizes not multiples of 8
+    (256, 320, 64, [(0, 0), (37, -21), (-60, 44), (5, 63)]),                   # coarsest level 4
+    (768, 1024, 256, [(0, 0), (-130, 97), (201, -188), (3, -2), (-255, 0)]),   # coarsest level 8
+    (250, 333, 40, [(0, 0), (17, -39), (-40, 12)]),                           # coarsest level 2, odd width
+    (1024, 1536, 128, [(0, 0), (-101, 77), (120, -127), (64, 9)]),            # coarsest level 16
 ])
 def test_prealign_circular_shifts(isa, oracle, gpu, h, w, max_shift, shifts):
     """Frames circularly shifted by known integer offsets (np.roll by d: the content moves by d): every frame's
