@@ -9,6 +9,12 @@
 //   2. column DFT at the band ky  Y = (1-H) c_kx/(WH) . sum_y U e^{-i..}
 //   3. inverse column DFT         V = sum_j Y e^{+i..}
 //   4. inverse row DFT, epilogue  img = clamp(img - V . Tw^T, 0, 1)   (GEMM, K = 2 kxn)
+// clearAxis c > 0 also zeroes every bin with kx < c or |ky| < c (the reference's float test on |ky|). Those
+// bins take weight 0 in the band above and are removed by three FP64 axis terms computed from the image:
+//   L_A  = inverse of the row DFT at kx < c (all ky)                     per row:    U_A(y, kx)
+//   L_B  = inverse of the column DFT at the cleared ky (all kx)          per column: Z(x, ky)
+//   L_AB = their common bins, subtracted from U_A before its inverse
+// img -= L_A + L_B - L_AB runs after the band's row DFT has read the image and before its clamped epilogue.
 // The two GEMMs run on the matrix cores when the image rows are 16-byte aligned, else as a tiled
 // FP32 SIMT GEMM. On the matrix cores they use FP32 products (v_mfma_f32_16x16x4_f32), or, when the
 // caller bounds the image values, three fp16 products of two-term fp16 splits of both operands
@@ -114,6 +120,14 @@ __device__ float filter_value(int kx, int ky, int w, int h, float lp, float hp, 
         }
     }
     return fil;
+}
+
+// The reference's test on the row index (R:Kernels/kernel.cu:773): |ky / h| * h < c in float.
+__device__ __forceinline__ bool axis_cleared_ky(int ky, int h, int c) {
+    float my = (float)ky;
+    if (my > (float)h * 0.5f) my = ((float)h - my) * -1.0f;
+    my /= (float)h;
+    return fabsf(my) * (float)h < (float)c;
 }
 
 // Tiled FP32 GEMM, 64 x 64 output tile per 256 threads (4 x 4 per thread), K step 16.
@@ -501,7 +515,7 @@ __global__ void k_col_dft_partial(const float* __restrict__ U, const double* __r
 
 // Sum of the chunks (fixed order), times (1 - H) c_kx / (W H). Y: [ny][kxn] complex.
 __global__ void k_col_dft_finish(const double* __restrict__ P, double* __restrict__ Y, int w, int h, int kxn, int kyn,
-                                 int ny, float hp, float hps) {
+                                 int ny, int clear_axis, float hp, float hps) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
     if (k >= kxn) return;
@@ -514,6 +528,7 @@ __global__ void k_col_dft_finish(const double* __restrict__ P, double* __restric
     double fil = (double)filter_value(k, ky, w, h, 1.0f, hp, 1.0f, hps);
     double ck = (k == 0 || (w % 2 == 0 && k == w / 2)) ? 1.0 : 2.0;
     double g = (1.0 - fil) * ck / ((double)w * (double)h);
+    if (clear_axis > 0 && (k < clear_axis || axis_cleared_ky(ky, h, clear_axis))) g = 0.0;  // in the axis terms
     Y[((size_t)j * kxn + k) * 2] = sr * g;
     Y[((size_t)j * kxn + k) * 2 + 1] = si * g;
 }
@@ -547,14 +562,196 @@ __global__ void k_col_idft(const double* __restrict__ Y, const double* __restric
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// clearAxis terms (FP64). Axis rows: ky_j = band_ky(j, cyn, h) for j < nyc covers |ky| <= c; which of them
+// the reference clears is decided per ky by its float test.
+
+// atw[m] = (cos, sin)(2 pi m / w), m < w.
+__global__ void k_axis_twiddles(double* atw, int w) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= w) return;
+    double s, c;
+    sincos(kTwoPi * (double)m / (double)w, &s, &c);
+    atw[2 * m] = c;
+    atw[2 * m + 1] = s;
+}
+
+constexpr int AX_Q = 8;  // frequencies per thread in the axis DFTs
+
+// Sum of a double over a block of 256 threads in a fixed order: the lanes of each wave, then the 4 waves.
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    v = __shfl(v, 0);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+// U_A(y, kx) = sum_x img(x, y) e^{-2 pi i kx x / w} for kx < cx: block (row y, kx group of AX_Q).
+__global__ __launch_bounds__(256) void k_axis_row_dft(const float* __restrict__ img, int pitch, int w, int cx,
+                                                      const double* __restrict__ atw, double* __restrict__ UA) {
+    __shared__ double red[4];
+    const int y = blockIdx.x, kx0 = blockIdx.y * AX_Q;
+    double re[AX_Q], im[AX_Q];
+#pragma unroll
+    for (int q = 0; q < AX_Q; ++q) re[q] = im[q] = 0.0;
+    const float* row = row_ptr(img, pitch, y);
+    for (int x = threadIdx.x; x < w; x += 256) {
+        const double v = (double)row[x];
+        int idx = (int)(((long long)kx0 * x) % w);  // (kx x) mod w, then + x per kx
+#pragma unroll
+        for (int q = 0; q < AX_Q; ++q) {
+            re[q] = fma(v, atw[2 * idx], re[q]);
+            im[q] = fma(-v, atw[2 * idx + 1], im[q]);
+            idx += x;
+            if (idx >= w) idx -= w;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < AX_Q; ++q) {
+        const double r = block_sum256(re[q], red), i = block_sum256(im[q], red);
+        if (threadIdx.x == 0 && kx0 + q < cx) {
+            UA[((size_t)y * cx + kx0 + q) * 2] = r;
+            UA[((size_t)y * cx + kx0 + q) * 2 + 1] = i;
+        }
+    }
+}
+
+// Partial Z(x, ky_j) = sum_y img(x, y) e^{-2 pi i ky_j y / h} over the rows of chunk blockIdx.z; actab holds
+// (cos, sin)(2 pi ky_j y / h) at [y][j]. Zp: [chunk][j][x] complex.
+__global__ __launch_bounds__(256) void k_axis_col_dft(const float* __restrict__ img, int pitch, int w, int h, int nyc,
+                                                      const double* __restrict__ actab, int rows,
+                                                      double* __restrict__ Zp) {
+    const int x = blockIdx.x * 256 + threadIdx.x, j0 = blockIdx.y * AX_Q, ch = blockIdx.z;
+    if (x >= w) return;
+    const int nq = min(AX_Q, nyc - j0);
+    const int y0 = ch * rows, y1 = min(h, y0 + rows);
+    double re[AX_Q], im[AX_Q];
+#pragma unroll
+    for (int q = 0; q < AX_Q; ++q) re[q] = im[q] = 0.0;
+    for (int y = y0; y < y1; ++y) {
+        const double v = (double)row_ptr(img, pitch, y)[x];
+        const double* ct = actab + ((size_t)y * nyc + j0) * 2;
+#pragma unroll
+        for (int q = 0; q < AX_Q; ++q)
+            if (q < nq) {
+                re[q] = fma(v, ct[2 * q], re[q]);
+                im[q] = fma(-v, ct[2 * q + 1], im[q]);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < AX_Q; ++q)
+        if (q < nq) {
+            const size_t o = (((size_t)ch * nyc + j0 + q) * w + x) * 2;
+            Zp[o] = re[q];
+            Zp[o + 1] = im[q];
+        }
+}
+
+// Z(x, ky_j) / h summed over the chunks in order, zero where the reference keeps ky_j. Z: [j][x] complex.
+__global__ void k_axis_col_finish(const double* __restrict__ Zp, double* __restrict__ Z, int w, int h, int c, int cyn,
+                                  int nyc, int chunks) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (x >= w) return;
+    double re = 0.0, im = 0.0;
+    if (axis_cleared_ky(band_ky(j, cyn, h), h, c))
+        for (int ch = 0; ch < chunks; ++ch) {
+            const size_t o = (((size_t)ch * nyc + j) * w + x) * 2;
+            re += Zp[o];
+            im += Zp[o + 1];
+        }
+    Z[((size_t)j * w + x) * 2] = re / (double)h;
+    Z[((size_t)j * w + x) * 2 + 1] = im / (double)h;
+}
+
+// X(kx, ky_j) = sum_y U_A(y, kx) e^{-2 pi i ky_j y / h} for the cleared ky_j (else 0): block (kx, j).
+__global__ __launch_bounds__(256) void k_axis_xab(const double* __restrict__ UA, const double* __restrict__ actab, int h,
+                                                  int cx, int c, int cyn, int nyc, double* __restrict__ XAB) {
+    __shared__ double red[4];
+    const int kx = blockIdx.x, j = blockIdx.y;
+    double re = 0.0, im = 0.0;
+    const bool cleared = axis_cleared_ky(band_ky(j, cyn, h), h, c);
+    if (cleared)
+        for (int y = threadIdx.x; y < h; y += 256) {
+            const double ur = UA[((size_t)y * cx + kx) * 2], ui = UA[((size_t)y * cx + kx) * 2 + 1];
+            const double co = actab[((size_t)y * nyc + j) * 2], si = actab[((size_t)y * nyc + j) * 2 + 1];
+            re += ur * co + ui * si;
+            im += ui * co - ur * si;
+        }
+    re = block_sum256(re, red);
+    im = block_sum256(im, red);
+    if (threadIdx.x == 0) {
+        XAB[((size_t)kx * nyc + j) * 2] = re;
+        XAB[((size_t)kx * nyc + j) * 2 + 1] = im;
+    }
+}
+
+// U'(y, kx) = U_A(y, kx) - (1/h) sum_j X(kx, ky_j) e^{+2 pi i ky_j y / h}: the row spectrum without the bins
+// L_B already removes.
+__global__ void k_axis_uprime(double* __restrict__ UA, const double* __restrict__ XAB, const double* __restrict__ actab,
+                              int h, int cx, int nyc) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x, kx = blockIdx.y;
+    if (y >= h) return;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < nyc; ++j) {
+        const double xr = XAB[((size_t)kx * nyc + j) * 2], xi = XAB[((size_t)kx * nyc + j) * 2 + 1];
+        const double co = actab[((size_t)y * nyc + j) * 2], si = actab[((size_t)y * nyc + j) * 2 + 1];
+        re += xr * co - xi * si;
+        im += xr * si + xi * co;
+    }
+    UA[((size_t)y * cx + kx) * 2] -= re / (double)h;
+    UA[((size_t)y * cx + kx) * 2 + 1] -= im / (double)h;
+}
+
+// img(x, y) -= (1/w) sum_{kx < cx} c_kx Re(U'(y, kx) e^{+2 pi i kx x / w}) + sum_j Re(Z(x, ky_j) e^{+2 pi i ky_j y / h}),
+// in FP64, rounded once to float; c_kx = 1 at kx = 0 and kx = w/2 (w even), else 2.
+__global__ void k_axis_apply(float* img, int pitch, int w, int h, int cx, int nyc, const double* __restrict__ UA,
+                             const double* __restrict__ Z, const double* __restrict__ atw,
+                             const double* __restrict__ actab) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    double la = 0.0;
+    int idx = 0;
+    for (int kx = 0; kx < cx; ++kx) {
+        const double ck = (kx == 0 || (w % 2 == 0 && kx == w / 2)) ? 1.0 : 2.0;
+        const double ur = UA[((size_t)y * cx + kx) * 2], ui = UA[((size_t)y * cx + kx) * 2 + 1];
+        la += ck * (ur * atw[2 * idx] - ui * atw[2 * idx + 1]);
+        idx += x;
+        if (idx >= w) idx -= w;
+    }
+    double lb = 0.0;
+    for (int j = 0; j < nyc; ++j) {
+        const double zr = Z[((size_t)j * w + x) * 2], zi = Z[((size_t)j * w + x) * 2 + 1];
+        const double co = actab[((size_t)y * nyc + j) * 2], si = actab[((size_t)y * nyc + j) * 2 + 1];
+        lb += zr * co - zi * si;
+    }
+    float* p = row_ptr(img, pitch, y) + x;
+    *p = (float)((double)*p - (la / (double)w + lb));
+}
+
 struct HpPlan {
     Band b;
     int n2, n2p, twp;  // band columns 2 kxn, padded to 32; row pitch of twT (w padded to 32)
     int kc, nz;        // K chunk and chunk count of the MFMA row DFT
+    // clearAxis c (0: none): kx < cx (cx = min(c, w/2 + 1)); axis rows |ky| <= c: nyc of them (band_ky with
+    // cyn = min(c + 1, h/2 + 1)); the column DFT in ych chunks of ych_rows rows
+    int c, cx, cyn, nyc, ych, ych_rows;
 };
 
-HpPlan plan_of(int w, int h, float hp, float hps) {
+HpPlan plan_of(int w, int h, int c, float hp, float hps) {
     HpPlan P;
+    P.c = c > 0 ? c : 0;
+    P.cx = P.c > 0 ? std::min(P.c, w / 2 + 1) : 0;
+    P.cyn = P.c > 0 ? std::min(P.c + 1, h / 2 + 1) : 0;
+    P.nyc = 0;
+    if (P.c > 0) {
+        const int lo = std::min(P.cyn, h), hi_start = std::max(P.cyn, h - P.cyn + 1);
+        P.nyc = lo + std::max(0, h - hi_start);
+    }
+    P.ych_rows = isa::div_up(h, std::min(8, h));
+    P.ych = isa::div_up(h, P.ych_rows);
     P.b = band_of(w, h, hp, hps);
     P.n2 = 2 * P.b.kxn;
     P.n2p = isa::div_up(P.n2, 32) * 32;
@@ -577,6 +774,9 @@ struct HpLayout {
     double* P;
     double* Y;
     _Float16 *twh, *twl, *twTh, *twTl, *Vh, *Vl;  // fp16 splits of tw, twT and V (same pitches)
+    // clearAxis terms: atw [w] and actab [h][nyc] twiddles, U_A [h][cx], X_AB [cx][nyc], Z partials
+    // [ych][nyc][w] and Z [nyc][w], all complex double
+    double *atw, *actab, *UA, *XAB, *Zp, *Z;
     size_t bytes;
 };
 
@@ -609,6 +809,13 @@ HpLayout hp_layout(void* ws, int w, int h, const HpPlan& pl) {
         *halves[i] = (_Float16*)(base + off);
         off += al(counts[i] * sizeof(_Float16));
     }
+    double** axis[6] = {&L.atw, &L.actab, &L.UA, &L.XAB, &L.Zp, &L.Z};
+    const size_t acounts[6] = {pl.c ? (size_t)w : 0, (size_t)h * pl.nyc, (size_t)h * pl.cx, (size_t)pl.cx * pl.nyc,
+                               (size_t)pl.ych * pl.nyc * w, (size_t)pl.nyc * w};
+    for (int i = 0; i < 6; ++i) {
+        *axis[i] = (double*)(base + off);
+        off += al(acounts[i] * 2 * sizeof(double));
+    }
     L.bytes = off;
     return L;
 }
@@ -622,17 +829,17 @@ __global__ void k_hp_zero(float* p, size_t n) {
 
 namespace isa {
 
-size_t highpass_workspace(int w, int h, float hp, float hps) {
-    return hp_layout(nullptr, w, h, plan_of(w, h, hp, hps)).bytes;
+size_t highpass_workspace(int w, int h, int clear_axis, float hp, float hps) {
+    return hp_layout(nullptr, w, h, plan_of(w, h, clear_axis, hp, hps)).bytes;
 }
 
 // If tables_ready, the twiddle tables (and the zero padding of V) in the workspace are reused.
 // value_bound > 0 promises |img| <= value_bound: the two GEMMs then run on fp16 splits of their
 // operands (three fp16 products per product) when every operand stays well inside the fp16 range:
 // |img| <= B, |twiddles| <= 1 and |V| <= 4 ny B (|U| <= sqrt(2) w B, |Y| <= 2 sqrt(2) B).
-int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, float value_bound,
-             hipStream_t s) {
-    const HpPlan pl = plan_of(w, h, hp, hps);
+int highpass(float* img, int pitch, int w, int h, int clear_axis, float hp, float hps, void* ws, bool tables_ready,
+             float value_bound, hipStream_t s) {
+    const HpPlan pl = plan_of(w, h, clear_axis, hp, hps);
     const Band& b = pl.b;
     HpLayout L = hp_layout(ws, w, h, pl);
     const int n2 = pl.n2, n2p = pl.n2p;
@@ -647,6 +854,10 @@ int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws,
         k_row_twiddles<<<dim3(div_up(w, 256), b.kxn), 256, 0, s>>>(L.tw, n2p, L.twT, pl.twp, L.twh, L.twl, L.twTh,
                                                                     L.twTl, w, b.kxn);
         k_col_twiddles<<<dim3(div_up(h, 256), b.ny), 256, 0, s>>>(L.ctab, h, b.kyn, b.ny);
+        if (pl.c > 0) {
+            k_axis_twiddles<<<div_up(w, 256), 256, 0, s>>>(L.atw, w);
+            k_col_twiddles<<<dim3(div_up(h, 256), pl.nyc), 256, 0, s>>>(L.actab, h, pl.cyn, pl.nyc);
+        }
     }
     const bool mfma = pitch % 16 == 0 && reinterpret_cast<uintptr_t>(img) % 16 == 0;
     const bool split = mfma && value_bound > 0.0f && 4.0 * b.ny * (double)value_bound <= 16384.0;
@@ -663,9 +874,21 @@ int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws,
         const size_t nu = (size_t)h * n2;
         k_sum_partials<<<(unsigned)((nu + 255) / 256), 256, 0, s>>>(L.part, L.U, nu, div_up(w, kc));
     }
-    // 2. band column DFT with the transfer function
+    // clearAxis: the axis terms from the image, subtracted from it (the row DFT above has read it)
+    if (pl.c > 0) {
+        k_axis_row_dft<<<dim3(h, div_up(pl.cx, AX_Q)), 256, 0, s>>>(img, pitch, w, pl.cx, L.atw, L.UA);
+        k_axis_col_dft<<<dim3(div_up(w, 256), div_up(pl.nyc, AX_Q), pl.ych), 256, 0, s>>>(img, pitch, w, h, pl.nyc,
+                                                                                          L.actab, pl.ych_rows, L.Zp);
+        k_axis_col_finish<<<dim3(div_up(w, 256), pl.nyc), 256, 0, s>>>(L.Zp, L.Z, w, h, pl.c, pl.cyn, pl.nyc, pl.ych);
+        k_axis_xab<<<dim3(pl.cx, pl.nyc), 256, 0, s>>>(L.UA, L.actab, h, pl.cx, pl.c, pl.cyn, pl.nyc, L.XAB);
+        k_axis_uprime<<<dim3(div_up(h, 256), pl.cx), 256, 0, s>>>(L.UA, L.XAB, L.actab, h, pl.cx, pl.nyc);
+        k_axis_apply<<<dim3(div_up(w, 64), div_up(h, 4)), dim3(64, 4), 0, s>>>(img, pitch, w, h, pl.cx, pl.nyc, L.UA,
+                                                                              L.Z, L.atw, L.actab);
+        ISA_TRY(check_launch("fourier_highpass: clear axis"));
+    }
+    // 2. band column DFT with the transfer function (weight 0 on the cleared bins)
     k_col_dft_partial<<<dim3(div_up(b.kxn, 64), b.ny, kColChunks), 64, 0, s>>>(L.U, L.ctab, L.P, h, b.kxn, b.ny);
-    k_col_dft_finish<<<dim3(div_up(b.kxn, 64), b.ny), 64, 0, s>>>(L.P, L.Y, w, h, b.kxn, b.kyn, b.ny, hp, hps);
+    k_col_dft_finish<<<dim3(div_up(b.kxn, 64), b.ny), 64, 0, s>>>(L.P, L.Y, w, h, b.kxn, b.kyn, b.ny, pl.c, hp, hps);
     // 3. inverse column DFT
     k_col_idft<<<dim3(div_up(b.kxn, 64), h), 64, 0, s>>>(L.Y, L.ctab, L.V, split ? L.Vh : nullptr,
                                                          split ? L.Vl : nullptr, h, b.kxn, b.ny, n2p);
@@ -684,9 +907,9 @@ int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws,
 
 extern "C" {
 
-size_t isa_fourier_highpass_workspace(int width, int height, float high_pass, float high_pass_sigma) {
-    if (width <= 0 || height <= 0) return 0;
-    return isa::highpass_workspace(width, height, high_pass, high_pass_sigma);
+size_t isa_fourier_highpass_workspace(int width, int height, int clear_axis, float high_pass, float high_pass_sigma) {
+    if (width <= 0 || height <= 0 || clear_axis < 0) return 0;
+    return isa::highpass_workspace(width, height, clear_axis, high_pass, high_pass_sigma);
 }
 
 int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int clear_axis, float high_pass,
@@ -694,12 +917,9 @@ int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int c
     ISA_REQUIRE(width > 0 && height > 0 && img_dev && workspace_dev, "isa_fourier_highpass: bad arguments");
     ISA_REQUIRE(high_pass >= 0 && high_pass_sigma >= 0 && (double)high_pass + high_pass_sigma < 0.25,
                 "isa_fourier_highpass: high_pass + high_pass_sigma must lie in [0, 0.25)");
-    if (clear_axis != 0) {
-        isa::set_error("isa_fourier_highpass: clear_axis != 0 is not supported");
-        return ISA_ERR_UNSUPPORTED;
-    }
-    return isa::highpass(img_dev, pitch, width, height, high_pass, high_pass_sigma, workspace_dev, false, 0.0f,
-                         isa::to_stream(stream));
+    ISA_REQUIRE(clear_axis >= 0, "isa_fourier_highpass: clear_axis must be >= 0");
+    return isa::highpass(img_dev, pitch, width, height, clear_axis, high_pass, high_pass_sigma, workspace_dev, false,
+                         0.0f, isa::to_stream(stream));
 }
 
 int isa_fourier_highpass_bounded(float* img_dev, int pitch, int width, int height, int clear_axis, float high_pass,
@@ -708,12 +928,9 @@ int isa_fourier_highpass_bounded(float* img_dev, int pitch, int width, int heigh
     ISA_REQUIRE(high_pass >= 0 && high_pass_sigma >= 0 && (double)high_pass + high_pass_sigma < 0.25,
                 "isa_fourier_highpass_bounded: high_pass + high_pass_sigma must lie in [0, 0.25)");
     ISA_REQUIRE(!(value_bound != value_bound), "isa_fourier_highpass_bounded: value_bound is NaN");
-    if (clear_axis != 0) {
-        isa::set_error("isa_fourier_highpass_bounded: clear_axis != 0 is not supported");
-        return ISA_ERR_UNSUPPORTED;
-    }
-    return isa::highpass(img_dev, pitch, width, height, high_pass, high_pass_sigma, workspace_dev, false, value_bound,
-                         isa::to_stream(stream));
+    ISA_REQUIRE(clear_axis >= 0, "isa_fourier_highpass_bounded: clear_axis must be >= 0");
+    return isa::highpass(img_dev, pitch, width, height, clear_axis, high_pass, high_pass_sigma, workspace_dev, false,
+                         value_bound, isa::to_stream(stream));
 }
 
 }  // extern "C"

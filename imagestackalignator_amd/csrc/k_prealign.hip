@@ -7,9 +7,11 @@
 // minus it. Here the same c(s) is evaluated directly at the lags that can hold the maximum: every lag
 // of the search radius on a box-downsampled image (the first level whose radius is at most 8 lags, or
 // the last whose sides stay at least 64 px), then a window around twice the previous maximum at each
-// finer level (15 x 15 lags, 5 x 5 at full resolution). Sums are FP64 per thread and reduced across
-// blocks in a fixed order, so the result is deterministic; nothing is allocated and the host never
-// waits (the window centre of each level is read on the device).
+// finer level: 15 x 15 lags at the level below the coarsest, then 5 x 5, searched twice above full
+// resolution (the second window centred on the first's maximum: a maximum that stays put is the largest
+// of its own 5 x 5 neighbourhood). Sums are FP64 per thread and reduced across blocks in a fixed order,
+// so the result is deterministic; nothing is allocated and the host never waits (the window centre of
+// each level is read on the device).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -29,7 +31,7 @@ constexpr int PA_THREADS = 256;                 // columns per block, one per th
 constexpr int PA_MAX_LEVELS = 6;               // levels 1, 2, 4, 8, 16, 32
 constexpr int PA_COARSE_RADIUS = 8;            // lags each way at the coarsest level (before the slack)
 constexpr int PA_MIN_SIDE = 64;                // smallest side of the coarsest level
-constexpr int PA_MIN_ROWS = 16;                // smallest row chunk
+constexpr int PA_MIN_ROWS = 32;                // smallest row chunk
 
 struct LagWindow {
     int tiles;       // lag tiles per row and column (odd); lag tile (tx, ty) covers 5 x 5 lags
@@ -56,10 +58,11 @@ __global__ __launch_bounds__(PA_THREADS) void k_cc_rows(const float* __restrict_
     if (sy0 < 0) sy0 += h;
     const int x = blockIdx.x * PA_THREADS + threadIdx.x;
     const bool active = x < w;
+    const int xa = active ? x : w - 1;  // lanes past the last column read valid pixels and add zero
     int xs[PA_K];
 #pragma unroll
     for (int i = 0; i < PA_K; ++i) {
-        int xx = (active ? x : 0) + sx0 + i;  // < 2 w + 4 <= 3 w (w >= 8)
+        int xx = xa + sx0 + i;  // < 2 w + 4 <= 3 w (w >= 8)
         if (xx >= w) xx -= w;
         if (xx >= w) xx -= w;
         xs[i] = xx;
@@ -73,26 +76,46 @@ __global__ __launch_bounds__(PA_THREADS) void k_cc_rows(const float* __restrict_
     for (int l = 0; l < PA_LAGS; ++l) acc[l] = 0.0;
 #pragma unroll
     for (int j = 0; j < PA_K; ++j) ar[j] = 0.0;
-    // step t pairs b row y0 + sy0 + t with a rows y0 + t - j (lag row j); a row y0 + t sits in ring slot
-    // t mod 5, so with t0 a multiple of 5 every slot index below is a constant
-    for (int t0 = 0; t0 < tend; t0 += PA_K) {
+    // Step t pairs b row y0 + sy0 + t with a rows y0 + t - j (lag row j); a row y0 + t sits in ring slot
+    // t mod 5, so with t0 a multiple of 5 every slot index below is a constant. Whole groups of 5 steps
+    // issue their 30 loads before the products; the last partial group checks each step.
+    int t0 = 0;
+    for (; t0 + PA_K <= tend; t0 += PA_K) {
+        float av[PA_K], bv[PA_K][PA_K];
 #pragma unroll
         for (int u = 0; u < PA_K; ++u) {
             const int t = t0 + u;
-            if (t < tend) {
-                ar[u] = (active && t < nrows) ? (double)row_ptr(a, pitch, y0 + t)[x] : 0.0;
-                const float* br = row_ptr(b, pitch, rb);
-                double bv[PA_K];
+            av[u] = row_ptr(a, pitch, y0 + min(t, nrows - 1))[xa];
+            if (!active || t >= nrows) av[u] = 0.0f;
+            const float* br = row_ptr(b, pitch, rb);
 #pragma unroll
-                for (int i = 0; i < PA_K; ++i) bv[i] = active ? (double)br[xs[i]] : 0.0;
+            for (int i = 0; i < PA_K; ++i) bv[u][i] = br[xs[i]];
+            if (++rb == h) rb = 0;
+        }
 #pragma unroll
-                for (int j = 0; j < PA_K; ++j) {
-                    const double av = ar[(u - j + PA_K) % PA_K];
+        for (int u = 0; u < PA_K; ++u) {
+            ar[u] = (double)av[u];
 #pragma unroll
-                    for (int i = 0; i < PA_K; ++i) acc[j * PA_K + i] = fma(av, bv[i], acc[j * PA_K + i]);
-                }
-                if (++rb == h) rb = 0;
+            for (int i = 0; i < PA_K; ++i) {
+                const double bd = (double)bv[u][i];
+#pragma unroll
+                for (int j = 0; j < PA_K; ++j) acc[j * PA_K + i] = fma(ar[(u - j + PA_K) % PA_K], bd, acc[j * PA_K + i]);
             }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PA_K; ++u) {
+        const int t = t0 + u;
+        if (t < tend) {
+            ar[u] = (active && t < nrows) ? (double)row_ptr(a, pitch, y0 + t)[xa] : 0.0;
+            const float* br = row_ptr(b, pitch, rb);
+#pragma unroll
+            for (int i = 0; i < PA_K; ++i) {
+                const double bd = (double)br[xs[i]];
+#pragma unroll
+                for (int j = 0; j < PA_K; ++j) acc[j * PA_K + i] = fma(ar[(u - j + PA_K) % PA_K], bd, acc[j * PA_K + i]);
+            }
+            if (++rb == h) rb = 0;
         }
     }
     // block sum: lanes of a wave, then the waves through LDS
@@ -110,36 +133,56 @@ __global__ __launch_bounds__(PA_THREADS) void k_cc_rows(const float* __restrict_
         double v = s_part[0][threadIdx.x];
         for (int k = 1; k < PA_THREADS / 64; ++k) v += s_part[k][threadIdx.x];
         const int chunks = gridDim.x * gridDim.y, chunk = blockIdx.y * gridDim.x + blockIdx.x;
-        partial[((size_t)tile * chunks + chunk) * PA_LAGS + threadIdx.x] = v;
+        partial[((size_t)tile * PA_LAGS + threadIdx.x) * chunks + chunk] = v;
     }
 }
 
-// c(s) of every lag of the window (chunks summed in order), then the first maximum in raster order of
-// the window (lag y, then lag x). Writes the maximum's lag to best (2 ints, absolute lag) and its value.
-__global__ __launch_bounds__(256) void k_cc_argmax(const double* partial, int chunks, const int* centre_in, int scale,
-                                                   LagWindow win, int* best, double* best_value) {
+// c(s) of every lag of the window, then the first maximum in raster order of the window (lag y, then
+// lag x). Each lag is summed by a group of G lanes (a power of two up to 64, so that a lane sums a few
+// chunks): lane j of the group sums chunks j, j + G, ... in order, the group is combined by a fixed
+// butterfly and its first lane's sum is the value. Writes the maximum's lag to best (2 ints, absolute lag)
+// and its value. partial is lag-major: [tile][lag][chunk].
+constexpr int PA_ARGMAX_THREADS = 1024;
+__global__ __launch_bounds__(PA_ARGMAX_THREADS) void k_cc_argmax(const double* partial, int chunks, int group,
+                                                                 const int* centre_in, int scale, LagWindow win,
+                                                                 int* best, double* best_value) {
+    constexpr int waves = PA_ARGMAX_THREADS / 64;
     const int n = win.tiles * win.tiles * PA_LAGS;
     const int side = win.tiles * PA_K;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane & (group - 1), first = lane - g;
+    const int per_wave = 64 / group;
     double bv = -INFINITY;
     int bi = 0x7fffffff;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int tile = i / PA_LAGS, l = i - tile * PA_LAGS;
+    // every lane of a wave runs the same number of rounds (the shuffles need all of them)
+    const int rounds = (n + waves * per_wave - 1) / (waves * per_wave);
+    for (int r = 0; r < rounds; ++r) {
+        const int i = (r * waves + wave) * per_wave + first / group;
         double v = 0.0;
-        for (int c = 0; c < chunks; ++c) v += partial[((size_t)tile * chunks + c) * PA_LAGS + l];
-        const int ty = tile / win.tiles, tx = tile - ty * win.tiles;
-        const int oy = ty * PA_K + l / PA_K, ox = tx * PA_K + l % PA_K;
-        const int ri = oy * side + ox;  // raster index within the window
-        if (v > bv || (v == bv && ri < bi)) {
-            bv = v;
-            bi = ri;
+        if (i < n) {
+            const double* pl = partial + (size_t)i * chunks;
+#pragma unroll 8
+            for (int c = g; c < chunks; c += group) v += pl[c];
+        }
+        for (int o = group >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        v = __shfl(v, first);
+        if (i < n && g == 0) {
+            const int tile = i / PA_LAGS, l = i - tile * PA_LAGS;
+            const int ty = tile / win.tiles, tx = tile - ty * win.tiles;
+            const int oy = ty * PA_K + l / PA_K, ox = tx * PA_K + l % PA_K;
+            const int ri = oy * side + ox;  // raster index within the window
+            if (v > bv || (v == bv && ri < bi)) {
+                bv = v;
+                bi = ri;
+            }
         }
     }
-    __shared__ double s_v[256];
-    __shared__ int s_i[256];
+    __shared__ double s_v[PA_ARGMAX_THREADS];
+    __shared__ int s_i[PA_ARGMAX_THREADS];
     s_v[threadIdx.x] = bv;
     s_i[threadIdx.x] = bi;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = PA_ARGMAX_THREADS / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
             const double v2 = s_v[threadIdx.x + o];
             const int i2 = s_i[threadIdx.x + o];
@@ -161,6 +204,13 @@ __global__ __launch_bounds__(256) void k_cc_argmax(const double* partial, int ch
         best[1] = cy - win.radius + ri / side;
         *best_value = s_v[0];
     }
+}
+
+// Lanes per lag in k_cc_argmax: the smallest power of two that leaves each lane at most 8 chunks.
+int argmax_group(int chunks) {
+    int g = 1;
+    while (g < 64 && g * 8 < chunks) g <<= 1;
+    return g;
 }
 
 // Final result of one frame: the full-resolution maximum s wrapped into (-W/2, W/2] x (-H/2, H/2] as
@@ -195,10 +245,14 @@ struct Plan {
 size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 LagWindow window_of(const Plan& P, int i) {
-    // the coarsest level searches the whole radius; finer levels 15 x 15 lags around twice the coarser
-    // maximum, full resolution 5 x 5
-    return i == P.levels - 1 ? P.coarse : (i > 0 ? LagWindow{3, 7} : LagWindow{1, 2});
+    // the coarsest level searches the whole radius, the next finer one 15 x 15 lags around twice its
+    // maximum, the others 5 x 5
+    return i == P.levels - 1 ? P.coarse : (i == P.levels - 2 ? LagWindow{3, 7} : LagWindow{1, 2});
 }
+
+// Windows searched at level i: the 5 x 5 windows above full resolution twice (the second centred on the
+// first's maximum), every other window once.
+int passes_of(const Plan& P, int i) { return i < P.levels - 2 && i >= 1 ? 2 : 1; }
 
 bool make_plan(int w, int h, int max_shift, Plan& P) {
     if (w < 8 || h < 8 || max_shift < 1) return false;
@@ -227,8 +281,9 @@ bool make_plan(int w, int h, int max_shift, Plan& P) {
     for (int i = 0; i < L; ++i) {
         const LagWindow win = window_of(P, i);
         const int tiles = win.tiles * win.tiles, cols = isa::div_up(P.lw[i], PA_THREADS);
-        // about 2048 (tile, column block, row chunk) blocks, row chunks of at least PA_MIN_ROWS rows
-        const int want = std::max(1, 2048 / (tiles * cols));
+        // about 1024 (tile, column block, row chunk) blocks (4 waves of 126 VGPRs per SIMD), row chunks of at
+        // least PA_MIN_ROWS rows
+        const int want = std::max(1, 1024 / (tiles * cols));
         const int rc_max = isa::div_up(P.lh[i], PA_MIN_ROWS);
         P.rows[i] = isa::div_up(P.lh[i], std::min(want, rc_max));
         P.row_chunks[i] = isa::div_up(P.lh[i], P.rows[i]);
@@ -280,17 +335,21 @@ int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int
         ISA_TRY(build(pyr_trk, frames[f]));
         for (int i = lc; i >= 0; --i) {
             const LagWindow win = window_of(P, i);
-            const int* centre = i == lc ? nullptr : best + 2 * (i + 1);
             const float* a = level_ptr(pyr_ref, frames[0], i);
             const float* b = level_ptr(pyr_trk, frames[f], i);
             const int pitch = i == 0 ? img_pitch : P.lpitch[i];
-            const int cols = div_up(P.lw[i], PA_THREADS);
-            k_cc_rows<<<dim3(cols, P.row_chunks[i], win.tiles * win.tiles), PA_THREADS, 0, s>>>(
-                a, b, pitch, P.lw[i], P.lh[i], centre, 2, win, P.rows[i], partial);
-            ISA_TRY(check_launch("prealign: correlation"));
-            k_cc_argmax<<<1, 256, 0, s>>>(partial, cols * P.row_chunks[i], centre, 2, win, best + 2 * i,
-                                          best_value + i);
-            ISA_TRY(check_launch("prealign: maximum"));
+            const int cols = div_up(P.lw[i], PA_THREADS), chunks = cols * P.row_chunks[i];
+            for (int pass = 0; pass < passes_of(P, i); ++pass) {
+                // centre: none at the coarsest level, twice the coarser maximum, then this level's own
+                const int* centre = i == lc ? nullptr : (pass == 0 ? best + 2 * (i + 1) : best + 2 * i);
+                const int scale = pass == 0 ? 2 : 1;
+                k_cc_rows<<<dim3(cols, P.row_chunks[i], win.tiles * win.tiles), PA_THREADS, 0, s>>>(
+                    a, b, pitch, P.lw[i], P.lh[i], centre, scale, win, P.rows[i], partial);
+                ISA_TRY(check_launch("prealign: correlation"));
+                k_cc_argmax<<<1, PA_ARGMAX_THREADS, 0, s>>>(partial, chunks, argmax_group(chunks), centre, scale, win,
+                                                            best + 2 * i, best_value + i);
+                ISA_TRY(check_launch("prealign: maximum"));
+            }
         }
         k_cc_result<<<1, 1, 0, s>>>(best, best_value, w, h, shifts4 + 4 * f);
         ISA_TRY(check_launch("prealign: result"));

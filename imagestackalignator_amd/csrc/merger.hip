@@ -25,9 +25,9 @@ int launch_resize_box248(const float* in, int in_pitch, float* o2, int p2, float
                          int h, hipStream_t s);
 int launch_kernel_param4(float* img, int w, int h, int pitch, float dth, float dtr, float kd, float kn, float ks,
                          float kh, hipStream_t s);
-size_t highpass_workspace(int w, int h, float hp, float hps);
-int highpass(float* img, int pitch, int w, int h, float hp, float hps, void* ws, bool tables_ready, float value_bound,
-             hipStream_t s);
+size_t highpass_workspace(int w, int h, int clear_axis, float hp, float hps);
+int highpass(float* img, int pitch, int w, int h, int clear_axis, float hp, float hps, void* ws, bool tables_ready,
+             float value_bound, hipStream_t s);
 int launch_track_level(const float* const* frames, int img_pitch, int w, int h, int tile, int M, int cx, int cy,
                        const int* pa, const int* pb, const int* pk, int pairs, const float* base, float* shifts,
                        size_t meas_stride, float threshold, float* shift_images, int* decisions, hipStream_t s);
@@ -293,10 +293,7 @@ int validate(const isa_params& P, int w, int h, int n) {
                                P.tile_size[i], P.max_shift[i]);
                 return ISA_ERR_UNSUPPORTED;
             }
-    if (P.use_high_pass && P.clear_axis != 0) {
-        isa::set_error("merger: clear_axis != 0 is not supported");
-        return ISA_ERR_UNSUPPORTED;
-    }
+    ISA_REQUIRE(P.clear_axis >= 0, "merger: clear_axis must be >= 0");
     return ISA_OK;
 }
 
@@ -383,7 +380,7 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
     }
     if (rc == ISA_OK && P.use_high_pass) {
         char* ws = nullptr;
-        A(ws, isa::highpass_workspace(width, height, P.high_pass, P.high_pass_sigma));
+        A(ws, isa::highpass_workspace(width, height, P.clear_axis, P.high_pass, P.high_pass_sigma));
         M->hp_ws = ws;
     }
     if (rc == ISA_OK && P.prealign_max_shift > 0) {
@@ -454,8 +451,8 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             if (P.use_high_pass) {
                 // the grey image is a convex combination of clamped sRGB values: within [0, 1] up to
                 // rounding, so 2 bounds it
-                ISA_TRY(isa::highpass(M->gray, wp, w, h, P.high_pass, P.high_pass_sigma, M->hp_ws, M->hp_tables, 2.0f,
-                                      s));
+                ISA_TRY(isa::highpass(M->gray, wp, w, h, P.clear_axis, P.high_pass, P.high_pass_sigma, M->hp_ws,
+                                      M->hp_tables, 2.0f, s));
                 M->hp_tables = true;
             }
             ISA_TRY(isa::launch_gauss1(M->gray, wp, M->trk + (size_t)f * np, wp, M->gtmp, w, h, taps_track, nt_track,

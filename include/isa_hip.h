@@ -84,7 +84,7 @@ typedef struct isa_params {
     float sigma_debayer_accumulation; /* 0.5 */
     float high_pass;                  /* 0.01 */
     float high_pass_sigma;            /* 0.0025 */
-    int clear_axis;                   /* 0 (only 0 is supported by the high-pass) */
+    int clear_axis;                   /* 0; > 0 zeroes the high-pass spectrum at kx < clear_axis and |ky| < clear_axis */
     int use_high_pass;                /* 1: tracking images are high-passed (SkipPreAlignment/ComputePreAlignment path) */
     float matching_threshold;         /* 1.0 */
     int n_levels;                     /* 1 */
@@ -201,8 +201,11 @@ int isa_gauss_filter_border(const float* in_dev, int in_pitch, float* out_dev, i
 /* PreAlignment.FourierFilter (R:PEFStudioDX/PreAlignment.cs:268-281 with fourierFilter R:Kernels/kernel.cu:702-779,
  * lp = lps = 1): high-pass in place, then clamp to [0,1]. The transfer function equals 1 outside a low band, so the
  * filter subtracts the low band's part, computed as a partial DFT (row DFT at the band's columns, column DFT at its
- * rows, and their inverses) in FP32 products. workspace_dev: isa_fourier_highpass_workspace() bytes. */
-size_t isa_fourier_highpass_workspace(int width, int height, float high_pass, float high_pass_sigma);
+ * rows, and their inverses) in FP32 products. clear_axis > 0 also zeroes the bins with kx < clear_axis or
+ * |ky / height| * height < clear_axis (float, as the reference tests it, R:Kernels/kernel.cu:773); their part is
+ * removed in FP64 from the row DFT at those kx and the column DFT at those ky (cost grows with clear_axis).
+ * workspace_dev: isa_fourier_highpass_workspace() bytes (0 for bad sizes or clear_axis < 0). */
+size_t isa_fourier_highpass_workspace(int width, int height, int clear_axis, float high_pass, float high_pass_sigma);
 int isa_fourier_highpass(float* img_dev, int pitch, int width, int height, int clear_axis, float high_pass,
                          float high_pass_sigma, void* workspace_dev, isa_stream_t stream);
 /* The same filter (R:PEFStudioDX/PreAlignment.cs:268-281) for an image the caller bounds: value_bound > 0 promises

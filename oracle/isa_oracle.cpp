@@ -150,7 +150,15 @@ template <class T> T fourier_filter_value(int kx, int ky, int w, int h, T lp, T 
             if (fil3 > (T)0.001f) fil = fil3;
         }
     }
-    if (kx < clear_axis || std::fabs(my) * (T)h < (T)clear_axis) fil = 0;
+    if (clear_axis > 0) {
+        // the axis test of the reference runs in float whatever T is: (|ky| / h) * h rounds below ky for
+        // many (h, ky), so bin |ky| = clear_axis is cleared as well there
+        float myf = (float)ky;
+        if (myf > (float)h * 0.5f) myf = ((float)h - myf) * -1.0f;
+        myf /= (float)h;
+        const float ay = std::fabs(myf) * (float)h;
+        if (kx < clear_axis || ay < (float)clear_axis) fil = 0;
+    }
     return fil;
 }
 

@@ -121,7 +121,7 @@ def test_fourier_highpass(isa, oracle, gpu, w, h, hp, hps):
     rng = np.random.default_rng(5)
     yy, xx = np.mgrid[0:h, 0:w]
     img = (0.5 + 0.3 * np.sin(xx / 17.0) * np.cos(yy / 23.0) + 0.05 * rng.standard_normal((h, w))).astype(np.float32)
-    ws_bytes = isa.lib().isa_fourier_highpass_workspace(w, h, hp, hps)
+    ws_bytes = isa.lib().isa_fourier_highpass_workspace(w, h, 0, C.c_float(hp), C.c_float(hps))
     ws = empty((ws_bytes // 4 + 64,))
     d = dev(img)
     isa.call("isa_fourier_highpass", p(d), w * 4, w, h, 0, C.c_float(hp), C.c_float(hps), p(ws), stream())
@@ -140,7 +140,7 @@ def test_fourier_highpass_bounded(isa, oracle, gpu, w, h, hp, hps):
     yy, xx = np.mgrid[0:h, 0:w]
     img = 0.5 + 0.3 * np.sin(xx / 17.0) * np.cos(yy / 23.0) + 0.05 * rng.standard_normal((h, w))
     img = np.clip(img, 0.0, 1.0).astype(np.float32)
-    ws = empty((isa.lib().isa_fourier_highpass_workspace(w, h, hp, hps) // 4 + 64,))
+    ws = empty((isa.lib().isa_fourier_highpass_workspace(w, h, 0, C.c_float(hp), C.c_float(hps)) // 4 + 64,))
     d32, d16, dbig = dev(img), dev(img), dev(img)
     args = (w * 4, w, h, 0, C.c_float(hp), C.c_float(hps))
     isa.call("isa_fourier_highpass", p(d32), *args, p(ws), stream())
@@ -154,6 +154,37 @@ def test_fourier_highpass_bounded(isa, oracle, gpu, w, h, hp, hps):
     np.testing.assert_array_equal(host(dbig), host(d32))
     with pytest.raises(isa.IsaError):
         isa.call("isa_fourier_highpass_bounded", p(d16), *args, C.c_float(float("nan")), p(ws), stream())
+
+
+# clearAxis: h = 41 with c = 1 clears |ky| = 1 by the reference's float test; c = 40 on 96 x 64 clears every bin
+# (output 0); 130 x 90 runs the SIMT GEMMs, 700 x 300 the matrix cores with a ragged band.
+@pytest.mark.parametrize("w,h,c,hp,hps", [(64, 41, 1, 0.01, 0.0025), (96, 64, 3, 0.01, 0.0025), (130, 90, 2, 0.01, 0.0025),
+                                          (700, 300, 5, 0.1, 0.02), (96, 64, 40, 0.01, 0.0025)])
+def test_fourier_highpass_clear_axis(isa, oracle, gpu, w, h, c, hp, hps):
+    """isa_fourier_highpass and its bounded form with clear_axis > 0 against the FP64 oracle (a full DFT with the
+    reference's cleared bins, itself pinned against numpy's FFT), at the bar of the plain filter."""
+    rng = np.random.default_rng(50 + c)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = 0.5 + 0.2 * np.sin(xx / 7.0) + 0.1 * np.cos(yy / 5.0) + 0.1 * np.sin(xx / 11.0 + yy / 9.0)
+    img = np.clip(img + 0.05 * rng.standard_normal((h, w)), 0.0, 1.0).astype(np.float32)
+    nbytes = isa.lib().isa_fourier_highpass_workspace(w, h, c, C.c_float(hp), C.c_float(hps))
+    assert nbytes > isa.lib().isa_fourier_highpass_workspace(w, h, 0, C.c_float(hp), C.c_float(hps))
+    ws = empty((nbytes // 4 + 64,))
+    d, db = dev(img), dev(img)
+    args = (w * 4, w, h, c, C.c_float(hp), C.c_float(hps))
+    print(f"highpass clear_axis {c} {w}x{h}", flush=True)
+    isa.call("isa_fourier_highpass", p(d), *args, p(ws), stream())
+    isa.call("isa_fourier_highpass_bounded", p(db), *args, C.c_float(1.0), p(ws), stream())
+    ref = oracle.fourier_highpass(img, hp, hps, clear_axis=c, dt=np.float64)
+    e = np.abs(host(d).astype(np.float64) - ref).max()
+    eb = np.abs(host(db).astype(np.float64) - ref).max()
+    print(f"  max |GPU - FP64| {e:.3g}, bounded {eb:.3g}")
+    assert e < 2e-5 and eb < 2e-5
+    # the cleared axes matter: the result differs from the plain filter's
+    plain = oracle.fourier_highpass(img, hp, hps, dt=np.float64)
+    assert np.abs(plain - ref).max() > 1e-3
+    with pytest.raises(isa.IsaError):
+        isa.call("isa_fourier_highpass", p(d), w * 4, w, h, -1, C.c_float(hp), C.c_float(hps), p(ws), stream())
 
 
 @pytest.mark.parametrize("h,w,which", [(64, 96, (2, 4, 8)), (40, 520, (2, 4, 8)), (16, 8, (2, 8)), (48, 1032, (4,))])

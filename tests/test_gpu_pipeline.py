@@ -33,6 +33,9 @@ CASES = {
     # reproduced, on either side)
     "black_level": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0,
                         extra=dict(black=(64.0, 60.0, 72.0), white=(16000.0, 16200.0, 15800.0))),
+    # the high-pass with clearAxis = 3: the tracking images lose their spectrum along both axes
+    "clear_axis": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0,
+                       extra=dict(clear_axis=3)),
 }
 
 

@@ -99,6 +99,51 @@ def test_highpass_removes_dc_and_keeps_high_frequencies(oracle):
     np.testing.assert_allclose(got, np.clip(0.2 * np.cos(2 * np.pi * 16 * xx / w), 0, 1), atol=1e-12)
 
 
+def _cleared_numpy(h, w, c):
+    """Bins the reference's clearAxis zeroes: kx < c, or |ky / h| * h < c evaluated in float32."""
+    ky = np.arange(h, dtype=np.float32)
+    hf = np.float32(h)
+    my = np.where(ky > hf * np.float32(0.5), (hf - ky) * np.float32(-1.0), ky).astype(np.float32)
+    ay = (np.abs(my / hf) * hf).astype(np.float32)
+    return (np.arange(w // 2 + 1)[None, :] < c) | (ay < np.float32(c))[:, None]
+
+
+def test_clear_axis_float_test_known_cases():
+    """In float, (1 / 41) * 41 rounds below 1 while (1 / 40) * 40 does not: with clearAxis = 1 the row
+    ky = 1 is cleared at h = 41 only; |ky| = 2 stays at both."""
+    assert _cleared_numpy(41, 8, 1)[1].all() and _cleared_numpy(41, 8, 1)[40].all()
+    assert not _cleared_numpy(40, 8, 1)[1, 1:].any() and not _cleared_numpy(41, 8, 1)[2, 1:].any()
+
+
+@pytest.mark.parametrize("h,w,c", [(41, 64, 1), (64, 96, 3), (77, 50, 2), (47, 60, 4)])
+def test_highpass_clear_axis_matches_fft_definition(oracle, h, w, c):
+    """clearAxis: the bins with kx < c or |ky| < c (the reference's float test) are zeroed after the transfer
+    function; the oracle in FP64 and in FP32 against numpy's FFT with that mask."""
+    rng = np.random.default_rng(h * w + c)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = 0.5 + 0.3 * np.sin(xx / 9.0) * np.cos(yy / 13.0) + 0.05 * rng.standard_normal((h, w))
+    H = np.where(_cleared_numpy(h, w, c), 0.0, _filter_numpy(h, w))
+    want = np.clip(np.fft.irfft2(H * np.fft.rfft2(img), s=(h, w)), 0, 1)
+    np.testing.assert_allclose(oracle.fourier_highpass(img, clear_axis=c, dt=F64), want, atol=1e-10)
+    np.testing.assert_allclose(oracle.fourier_highpass(img.astype(np.float32), clear_axis=c), want, atol=2e-6)
+    for ky in range(h):
+        for kx in range(0, w // 2 + 1, 2):
+            assert abs(oracle.fn("fourier_filter_value", F64)(kx, ky, w, h, 0.01, 0.0025, c) - H[ky, kx]) < 1e-12
+
+
+def test_highpass_clear_axis_removes_axis_frequencies(oracle):
+    """Pure cosines: the mean and the components on the axes (ky = 0; kx = 1 < c; ky = 1 at h = 41, cleared by
+    the float test) are removed, an off-axis component well above the band passes unchanged, clamped at 0."""
+    h, w, c = 41, 64, 1
+    yy, xx = np.mgrid[0:h, 0:w]
+    keep = 0.2 * np.cos(2 * np.pi * (12 * xx / w + 9 * yy / h))
+    img = (0.5 + 0.1 * np.cos(2 * np.pi * 10 * xx / w) + 0.1 * np.cos(2 * np.pi * 7 * yy / h + 0.3)
+           + 0.1 * np.cos(2 * np.pi * (10 * xx / w + yy / h)) + keep)
+    got = oracle.fourier_highpass(img, clear_axis=c, dt=F64)
+    # the (kx = 0, ky = 7) cosine has kx < c; the (10, 0) one |ky| < c; (10, 1) the float test at h = 41
+    np.testing.assert_allclose(got, np.clip(keep, 0, 1), atol=1e-12)
+
+
 # ---------------------------------------------------------------- a4 / a5
 def test_gaussian_filter_invariants(oracle):
     taps = oracle.gaussian_taps(1.0, F64)
