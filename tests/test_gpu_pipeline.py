@@ -33,6 +33,10 @@ CASES = {
     # reproduced, on either side)
     "black_level": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0,
                         extra=dict(black=(64.0, 60.0, 72.0), white=(16000.0, 16200.0, 15800.0))),
+    # the full-size configuration's pyramid (levels 1/2/4/8, tiles 64/32/32/16, max shifts 2/4/4/4) on 20 frames:
+    # 190 pairs per level, the minimizer over 20 frames
+    "four_levels_20": dict(w=512, h=384, n=20, gmax=12.0, levels=[(1, 64, 2), (2, 32, 4), (4, 32, 4), (8, 16, 4)],
+                           sr=0, occ=0.02),
     # the high-pass with clearAxis = 3: the tracking images lose their spectrum along both axes
     "clear_axis": dict(w=192, h=160, n=4, gmax=3.0, levels=[(1, 32, 2), (2, 32, 4)], sr=0, occ=0.0,
                        extra=dict(clear_axis=3)),
