@@ -93,6 +93,22 @@ def _declare(L):
     L.ora_minimize_shifts.restype = I
     L.ora_lk_ill_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.ora_lk_ill_counts.restype = None
+    L.ora_lk_solve_fp64.argtypes = [C.c_float] * 5 + [C.c_double, C.POINTER(C.c_double)]
+    L.ora_lk_solve_fp64.restype = C.c_int
+    L.ora_lk_ill_conditioned.argtypes = [C.c_float] * 3 + [C.c_double]
+    L.ora_lk_ill_conditioned.restype = C.c_int
+
+
+def lk_solve_fp64(a, b, d, b0, b1, min_det):
+    """The FP64 solve of one Lucas-Kanade window (G = [[a, b], [b, d]], rhs (b0, b1)): (u0, u1), or None when the
+    sigma1 gate skips the window."""
+    u = (C.c_double * 2)()
+    ok = lib().ora_lk_solve_fp64(a, b, d, b0, b1, min_det, u)
+    return (u[0], u[1]) if ok else None
+
+
+def lk_ill_conditioned(a, b, d, ratio):
+    return bool(lib().ora_lk_ill_conditioned(a, b, d, ratio))
 
 
 def lk_ill_counts():

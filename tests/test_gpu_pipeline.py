@@ -74,7 +74,9 @@ def test_merge_matches_oracle(isa, oracle, gpu, case):
     # lambda2 / lambda1 < lk_ill_ratio (0.01) in FP64 and the rest in FP32. The reference's FP32 closed
     # form everywhere (the FP32 oracle with lk_ill_ratio = 0) is 0.018 / 0.48 / 0.30 px RMS from FP64 on
     # these cases with 0.6-2.6 % of flows off by more than 0.01 px; the merger is at 1e-4 / 2e-3 px and
-    # <= 0.06 %. The bars sit between the two. Agreement with the FP32 oracle pixel by pixel is pinned
+    # <= 0.06 %. The bars sit between the two; the fraction bar also allows 1.5 times the FP32 oracle's own
+    # fraction (the same algorithm, FP32 arithmetic, FP64 solves of the same windows), for bursts whose
+    # FP32 rounding alone moves more flows. Agreement with the FP32 oracle pixel by pixel is pinned
     # separately (test_lucas_kanade_matches_oracle_on_well_conditioned_windows): with the default 8-bit
     # texture weights the warp is discontinuous in the flow, so float-level flow differences flip taps.
     flows64 = oracle.merge(params, raw, dt=np.float64, want=("flows",))["flows"]
@@ -86,11 +88,15 @@ def test_merge_matches_oracle(isa, oracle, gpu, case):
         d64.append((host(m.buffer_tensor(3, f, 2)) - flows64[f])[b:-b, b:-b])
         d32.append(np.abs(host(m.buffer_tensor(3, f, 2)) - ref["flows"][f]).max(axis=2)[b:-b, b:-b])
     d64, d32 = np.stack(d64), np.stack(d32)
+    o64 = np.stack([(ref["flows"][f] - flows64[f])[b:-b, b:-b] for f in range(c["n"]) if f != ref["ref"]])
     rms = float(np.sqrt(np.mean(np.where(np.isfinite(d64), d64, 0.0) ** 2)))
     far = float(np.mean(np.abs(d64).max(axis=-1) > 0.01))
-    print(f"{case}: flow vs FP64 oracle RMS {rms:.2e} px, {far * 100:.3f} % > 0.01 px; vs FP32 oracle "
-          f"{np.mean(d32 <= 1e-5) * 100:.1f} % within 1e-5 px, {np.mean(d32 <= 1e-3) * 100:.2f} % within 1e-3 px")
-    assert rms < 0.05 and far < 0.003, (rms, far)
+    far_o32 = float(np.mean(np.abs(o64).max(axis=-1) > 0.01))
+    rms_o32 = float(np.sqrt(np.mean(np.where(np.isfinite(o64), o64, 0.0) ** 2)))
+    print(f"{case}: flow vs FP64 oracle RMS {rms:.2e} px, {far * 100:.3f} % > 0.01 px (FP32 oracle: {rms_o32:.2e} px, "
+          f"{far_o32 * 100:.3f} %); vs FP32 oracle {np.mean(d32 <= 1e-5) * 100:.1f} % within 1e-5 px, "
+          f"{np.mean(d32 <= 1e-3) * 100:.2f} % within 1e-3 px")
+    assert rms < 0.05 and far < max(0.003, 1.5 * far_o32), (rms, far, far_o32)
     assert np.mean(d32 <= 1e-3) > 0.98
     # merged image
     fin = np.isfinite(ref["out"]) & np.isfinite(out)

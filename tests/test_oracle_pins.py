@@ -287,3 +287,24 @@ def test_full_frame_superres_contains_the_reference_crop(oracle):
     assert np.any(s_full[h // 2 + 1:h // 2 + h - 1, w // 2 + 1] != s_ref[1:h - 1, 1])  # the reference's clamp
     # outside the crop the full grid is written too (the reference's grid has no such pixels)
     assert np.all(w_full[2:6, 2:-2].sum(axis=-1) > 0)
+
+
+def test_lk_fp64_solve_of_ill_conditioned_windows(oracle):
+    """The FP64 Lucas-Kanade solve used for ill-conditioned windows: G^-1 b for det > 0 (numpy's solve), the
+    Moore-Penrose pseudo-inverse for a singular G (numpy's pinv: G / trace^2 for rank one), the reference's
+    sigma1 gate (lambda1 < min_det skips), NaN -> 0; the selection test is det <= ratio * trace^2."""
+    from oracle import oracle as O
+
+    G, rhs = np.array([[4.0, 1.0], [1.0, 2.0]]), np.array([1.0, 2.0])
+    np.testing.assert_allclose(O.lk_solve_fp64(4.0, 1.0, 2.0, 1.0, 2.0, 0.1), np.linalg.solve(G, rhs), rtol=1e-15)
+    G1, rhs1 = np.array([[1.0, 2.0], [2.0, 4.0]]), np.array([0.5, -3.0])  # det 0
+    np.testing.assert_allclose(O.lk_solve_fp64(1.0, 2.0, 4.0, 0.5, -3.0, 0.1), np.linalg.pinv(G1) @ rhs1, rtol=1e-14)
+    # lambda1 of [[0.3, 0], [0, 0.2]] is 0.3: below min_det 0.5 the window is skipped, above it solved
+    assert O.lk_solve_fp64(0.3, 0.0, 0.2, 1.0, 1.0, 0.5) is None
+    a32, d32 = float(np.float32(0.3)), float(np.float32(0.2))  # the sums are floats
+    np.testing.assert_allclose(O.lk_solve_fp64(0.3, 0.0, 0.2, 1.0, 1.0, 0.25), (1 / a32, 1 / d32), rtol=1e-15)
+    assert O.lk_solve_fp64(np.inf, 0.0, 1.0, 1.0, 1.0, 0.1) == (0.0, 0.0)  # u0 = 1 / inf, u1 = inf / inf -> NaN -> 0
+    # det / trace^2 against the ratio: diag(1, 0.01) has det / tr^2 = 0.0098 (ill at 0.01), diag(1, 0.02) 0.0192
+    assert O.lk_ill_conditioned(1.0, 0.0, 0.01, 0.01) and not O.lk_ill_conditioned(1.0, 0.0, 0.02, 0.01)
+    assert O.lk_ill_conditioned(1.0, 2.0, 4.0, 1e-9)  # singular
+    assert not O.lk_ill_conditioned(1.0, 0.0, 1.0, 0.0) and O.lk_ill_conditioned(0.0, 0.0, 0.0, 0.0)
