@@ -7,10 +7,17 @@
 #define ISA_CFA_GREEN 1
 #define ISA_CFA_BLUE 2
 
-// Bayer pattern passed by value (replaces the reference's __constant__ c_cfaPattern).
+// One of three values by a run-time index, from registers: an array in the kernel arguments indexed by a
+// per-lane value would be read from memory on every use.
+template <class T> __device__ __forceinline__ T pick3(int i, T a0, T a1, T a2) { return i == 0 ? a0 : (i == 1 ? a1 : a2); }
+
+// Bayer pattern passed by value (replaces the reference's __constant__ c_cfaPattern); coordinates >= 0.
 struct Cfa {
     int c[4];
-    __device__ __forceinline__ int at(int y, int x) const { return c[(y % 2) * 2 + (x % 2)]; }
+    __device__ __forceinline__ int at(int y, int x) const {
+        const int r0 = (x & 1) ? c[1] : c[0], r1 = (x & 1) ? c[3] : c[2];
+        return (y & 1) ? r1 : r0;
+    }
 };
 
 template <class T> __device__ __forceinline__ T* row_ptr(T* base, int pitch, int y) {

@@ -215,7 +215,9 @@ __global__ void k_relaxed_mask(DiffArgs a, const float* mean, const float* var, 
     for (int c = 0; c < 3; ++c) o[c] = mask_of(mr[c], vr[c], mm[c], a.alpha, a.beta, c, nullptr);
 }
 
-__device__ __forceinline__ float norm_raw(const DiffArgs& a, float v, int c) { return (v - a.black[c]) / a.white[c]; }
+__device__ __forceinline__ float norm_raw(const DiffArgs& a, float v, int c) {
+    return (v - pick3(c, a.black[0], a.black[1], a.black[2])) / pick3(c, a.white[0], a.white[1], a.white[2]);
+}
 
 // ---- forward: out (2W x 2H x 3), W (same), loss partials per block (double). One thread per raw
 // pixel b computes its four output pixels 2b + {0,1}^2, which share b's taps, flow and Omega.

@@ -22,8 +22,8 @@ struct DebayerArgs {
 // (or v = 0) with d in [2^-26, 2^26] (fast_div); other v take the division.
 __device__ __forceinline__ float div_c(float v, const DebayerArgs& a, int c) {
     const float av = fabsf(v);
-    if (!a.fast_div || (av != 0.0f && av < 0x1p-100f)) return v / a.div[c];
-    return (float)((double)v * a.inv_div[c]);
+    if (!a.fast_div || (av != 0.0f && av < 0x1p-100f)) return v / pick3(c, a.div[0], a.div[1], a.div[2]);
+    return (float)((double)v * pick3(c, a.inv_div[0], a.inv_div[1], a.inv_div[2]));
 }
 
 // Green estimate at a pixel of CFA colour `site` (R:Kernels/DeBayerKernels.cu:53-149).
@@ -33,7 +33,8 @@ __device__ __forceinline__ float green_at_site(int site, int x, int y, const Deb
     if (site == ISA_CFA_GREEN) return (raw(x, y) - a.black[1]) * a.scale[1];
     if (site != ISA_CFA_RED && site != ISA_CFA_BLUE) return 0.0f;
     int c = site == ISA_CFA_RED ? 0 : 2;
-    float bc = a.black[c], sc = a.scale[c], bg = a.black[1], sg = a.scale[1];
+    float bc = pick3(c, a.black[0], a.black[1], a.black[2]), sc = pick3(c, a.scale[0], a.scale[1], a.scale[2]);
+    float bg = a.black[1], sg = a.scale[1];
     float p = (raw(x, y) - bc) * sc;
     float xm2 = (raw(x - 2, y) - bc) * sc, xm1 = (raw(x - 1, y) - bg) * sg;
     float xp1 = (raw(x + 1, y) - bg) * sg, xp2 = (raw(x + 2, y) - bc) * sc;
@@ -61,7 +62,9 @@ template <class RawF, class GreenF>
 __device__ __forceinline__ void redblue_at_site(int site, int row, int x, int y, const DebayerArgs& a, RawF raw,
                                                 GreenF G, float& r, float& b) {
     float g = G(x, y);
-    auto RC = [&](int c, int xx, int yy) { return (raw(xx, yy) - a.black[c]) * a.scale[c]; };
+    auto RC = [&](int c, int xx, int yy) {
+        return (raw(xx, yy) - pick3(c, a.black[0], a.black[1], a.black[2])) * pick3(c, a.scale[0], a.scale[1], a.scale[2]);
+    };
     r = 0.0f;
     b = 0.0f;
     if (site == ISA_CFA_GREEN) {

@@ -154,8 +154,9 @@ __device__ __forceinline__ int site_channel(int site) { return (site >= 0 && sit
 
 // (v - black) / white; DIVN when every white level is an integer (div_n is then exact)
 template <bool DIVN> __device__ __forceinline__ float norm_raw(float v, int c, const AccArgs& a) {
-    float d = v - a.black[c];
-    return DIVN ? div_n(d, a.inv_white[c]) : d / a.white[c];
+    float d = v - pick3(c, a.black[0], a.black[1], a.black[2]);
+    return DIVN ? div_n(d, pick3(c, a.inv_white[0], a.inv_white[1], a.inv_white[2]))
+                : d / pick3(c, a.white[0], a.white[1], a.white[2]);
 }
 
 // One 5x5 gather of one frame at output pixel (x, y): integer sample shift (sx, sy), weights w[25].
@@ -445,9 +446,10 @@ struct SrGeom {
 };
 
 template <bool DIVN>
-__global__ void k_accumulate_sr_frames(float* sum, float* wsum, const float* kern4, int w, int h, int out_pitch,
-                                       int mask_pitch, int kern_pitch, int flow_pitch, FramesArgs fa, SrGeom g,
-                                       AccArgs a, int quant) {
+__global__ __launch_bounds__(256) void k_accumulate_sr_frames(float* sum, float* wsum, const float* kern4, int w,
+                                                              int h, int out_pitch, int mask_pitch, int kern_pitch,
+                                                              int flow_pitch, FramesArgs fa, SrGeom g, AccArgs a,
+                                                              int quant) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x < 1 || y < 1 || x >= g.ow - 1 || y >= g.oh - 1) return;
     float* ps = row_ptr(sum, out_pitch, y) + x * 3;
@@ -463,6 +465,14 @@ __global__ void k_accumulate_sr_frames(float* sum, float* wsum, const float* ker
 #pragma unroll
         for (int px = -2; px <= 2; ++px) wt[(py + 2) * 5 + px + 2] = kernel_weight(px, py, k[0], k[1], k[2]);
     const int cx0 = g.ox / 2, cx1 = g.ox / 2 + g.ow / 2 - 1, cy0 = g.oy / 2, cy1 = g.oy / 2 + g.oh / 2 - 1;
+    // certainty rows and columns do not move with the flow: byte offsets of the 5 x 5 window, once per pixel
+    size_t mrow[5];
+    int mcol[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        mrow[i] = (size_t)(clampi((y + i - 2 + g.oy) / 2, cy0, cy1) / 2) * (size_t)mask_pitch;
+        mcol[i] = (clampi((x + i - 2 + g.ox) / 2, cx0, cx1) / 2) * 16;
+    }
     for (int f = 0; f < fa.nframes; ++f) {
         int sx = 0, sy = 0;
         if (fa.flow[f]) {
@@ -471,26 +481,51 @@ __global__ void k_accumulate_sr_frames(float* sum, float* wsum, const float* ker
             sx = (int)roundf(fl[0] * 2.0f);
             sy = (int)roundf(fl[1] * 2.0f);
         }
-        const uint16_t* __restrict__ raw = fa.raw[f];
-        const float* mask = fa.mask[f];
+        typedef const __attribute__((address_space(1))) uint16_t* graw;
+        typedef const __attribute__((address_space(1))) char* gbytes;
+        const graw raw = (graw)fa.raw[f];
+        const gbytes mask = (gbytes)fa.mask[f];
+        int rx[5], rpar[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            rx[i] = clampi((x + i - 2 + sx + g.ox) / 2, cx0, cx1);
+            rpar[i] = rx[i] & 1;
+        }
+        // all 50 loads of the frame first (a site outside the three channels reads channel 0 and adds nothing)
+        int ch[25];
+        float rv[25], cv[25];
 #pragma unroll
         for (int py = -2; py <= 2; ++py) {
-            const int ppsy = clampi((y + py + sy + g.oy) / 2, cy0, cy1), ppy = clampi((y + py + g.oy) / 2, cy0, cy1);
-            const uint16_t* rrow = raw + (size_t)ppsy * w;
-            const float* mrow = row_ptr(mask, mask_pitch, ppy / 2);
+            const int ppsy = clampi((y + py + sy + g.oy) / 2, cy0, cy1);
+            const graw rrow = raw + (size_t)ppsy * w;
+            const gbytes mr = mask + mrow[py + 2];
+            const int ce = site_channel((ppsy & 1) ? a.cfa.c[2] : a.cfa.c[0]);  // channel at an even column
+            const int co = site_channel((ppsy & 1) ? a.cfa.c[3] : a.cfa.c[1]);  // and at an odd one
 #pragma unroll
             for (int px = -2; px <= 2; ++px) {
-                const int ppsx = clampi((x + px + sx + g.ox) / 2, cx0, cx1), ppx = clampi((x + px + g.ox) / 2, cx0, cx1);
-                const int c = site_channel(a.cfa.at(ppsy, ppsx));
-                if (c < 0) continue;
-                const float wgt = wt[(py + 2) * 5 + px + 2];
-                const float v = norm_raw<DIVN>((float)rrow[ppsx], c, a);
-                float cert = mrow[(ppx / 2) * 4 + c];
+                const int i = (py + 2) * 5 + px + 2, c = rpar[px + 2] ? co : ce;
+                ch[i] = c;
+                rv[i] = (float)rrow[rx[px + 2]];
+                cv[i] = *(const __attribute__((address_space(1))) float*)(mr + mcol[px + 2] + 4 * (c < 0 ? 0 : c));
+            }
+        }
+        // taps in the reference's raster order; each adds to the sums of its site's channel only
+#pragma unroll
+        for (int py = -2; py <= 2; ++py) {
+#pragma unroll
+            for (int px = -2; px <= 2; ++px) {
+                const int i = (py + 2) * 5 + px + 2, c = ch[i];
+                const float wgt = wt[i];
+                const float v = norm_raw<DIVN>(rv[i], c, a);
+                float cert = cv[i];
                 if (!isfinite(cert)) cert = 0.0f;
                 const float t = v * wgt * cert, u = wgt * cert;
-                if (c == 0) { acc[0] += t; ws[0] += u; }
-                else if (c == 1) { acc[1] += t; ws[1] += u; }
-                else { acc[2] += t; ws[2] += u; }
+                acc[0] = c == 0 ? acc[0] + t : acc[0];
+                ws[0] = c == 0 ? ws[0] + u : ws[0];
+                acc[1] = c == 1 ? acc[1] + t : acc[1];
+                ws[1] = c == 1 ? ws[1] + u : ws[1];
+                acc[2] = c == 2 ? acc[2] + t : acc[2];
+                ws[2] = c == 2 ? ws[2] + u : ws[2];
             }
         }
     }
@@ -601,7 +636,7 @@ int launch_accumulate_sr_frames(float* sum, float* wsum, const float* kern4, int
             fa.mask[i] = mask[f0 + i];
             fa.flow[i] = flow[f0 + i];
         }
-        kern_fn<<<dim3(div_up(out_w, 32), div_up(out_h, 8)), dim3(32, 8), 0, s>>>(
+        kern_fn<<<dim3(div_up(out_w, 64), div_up(out_h, 4)), dim3(64, 4), 0, s>>>(
             sum, wsum, kern4, w, h, out_pitch, mask_pitch, kern_pitch, flow_pitch, fa, g, a, quant);
         ISA_TRY(check_launch("accumulate_superres"));
     }
