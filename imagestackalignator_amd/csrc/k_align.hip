@@ -675,7 +675,8 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
     double* x = rhs + 2 * n1;           // n x 2
     double* red_v = x + 2 * n;          // kMinThreads
     int* red_i = (int*)(red_v + kMinThreads);  // kMinThreads
-    char* active = (char*)(red_i + kMinThreads);
+    int* list = red_i + kMinThreads;    // n1 x (n - 1): rows of each reduced node
+    char* active = (char*)(list + n1 * (n - 1));
     __shared__ unsigned long long s_nb[kMinFrames];
     __shared__ int s_flag;
     static_assert(kMinThreads == 64 && kMinFrames <= 64, "one wave, one lane per frame");
@@ -686,6 +687,15 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
         d[2 * k + 1] = (double)v[1];
         active[k] = 1;
     }
+    // rows of node tid + 1 in increasing k: 2 k when it is the row's second frame, 2 k + 1 when the first
+    // (the rows are distinct frame pairs: at most n - 1 per node)
+    int cnt = 0;
+    if (tid < n1)
+        for (int k = 0; k < m; ++k) {
+            const int ra = a.rows[2 * k], rb = a.rows[2 * k + 1];
+            const int side = rb == tid + 1 ? 0 : (ra == tid + 1 ? 1 : -1);
+            if (side >= 0 && cnt < n - 1) list[tid * (n - 1) + cnt++] = 2 * k + side;
+        }
     __syncthreads();
     int status = 0, it;
     for (it = 0; it < 10; ++it) {
@@ -727,35 +737,31 @@ __global__ __launch_bounds__(kMinThreads) void k_minimize(MinArgs a) {
             __syncthreads();
             break;
         }
-        // reduced Laplacian (nodes 1..n-1) and right-hand side; lane i owns row i
+        // reduced Laplacian (nodes 1..n-1) and right-hand side; lane i owns row i. The Laplacian comes from
+        // the neighbour masks of the connectivity test (an off-diagonal entry is -1 or 0, a diagonal one the
+        // node's degree), the right-hand side from the node's own active rows in increasing order.
         if (pre)
             for (int i = tid; i < n1 * n1; i += kMinThreads) L[i] = a.lfull[i];
-        for (int i = tid; i < n1; i += kMinThreads) {
-            if (!pre)
-                for (int j = 0; j < n1; ++j) L[i * n1 + j] = 0.0;
+        if (tid < n1) {
+            if (!pre) {
+                const unsigned long long nbi = s_nb[tid + 1];
+                for (int j = 0; j < n1; ++j)
+                    L[tid * n1 + j] = j == tid ? (double)__popcll(nbi) : -(double)((nbi >> (j + 1)) & 1ull);
+            }
             double bx = 0.0, by = 0.0;
-            const int node = i + 1;
-            for (int k = 0; k < m; ++k) {
+            for (int e = 0; e < cnt; ++e) {
+                const int v = list[tid * (n - 1) + e], k = v >> 1;
                 if (!active[k]) continue;
-                int ra = a.rows[2 * k], rb = a.rows[2 * k + 1];
-                if (rb == node) {
-                    if (!pre) {
-                        L[i * n1 + i] += 1.0;
-                        if (ra != 0) L[i * n1 + ra - 1] -= 1.0;
-                    }
-                    bx += d[2 * k];
-                    by += d[2 * k + 1];
-                } else if (ra == node) {
-                    if (!pre) {
-                        L[i * n1 + i] += 1.0;
-                        if (rb != 0) L[i * n1 + rb - 1] -= 1.0;
-                    }
+                if (v & 1) {
                     bx -= d[2 * k];
                     by -= d[2 * k + 1];
+                } else {
+                    bx += d[2 * k];
+                    by += d[2 * k + 1];
                 }
             }
-            rhs[2 * i] = bx;
-            rhs[2 * i + 1] = by;
+            rhs[2 * tid] = bx;
+            rhs[2 * tid + 1] = by;
         }
         __syncthreads();
         // Cholesky, lower triangle in place (the same steps as minimize_full_factor on the host)
@@ -1058,7 +1064,8 @@ size_t minimize_reg_smem(int n, int m) {
 
 size_t minimize_smem(int n, int m) {
     int n1 = n - 1;
-    return (size_t)(2 * m + n1 * n1 + 2 * n1 + 2 * n + kMinThreads) * 8 + kMinThreads * 4 + m + 16;
+    return (size_t)(2 * m + n1 * n1 + 2 * n1 + 2 * n + kMinThreads) * 8 + (size_t)(kMinThreads + n1 * (n - 1)) * 4 +
+           m + 16;
 }
 
 // getOptimalShifts (R:Kernels/ShiftMinimizerKernels.cu:178-218).

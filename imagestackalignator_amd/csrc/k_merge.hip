@@ -445,6 +445,12 @@ struct SrGeom {
     int ow, oh, ox, oy;
 };
 
+// v0, v1 or v2 for i = 0, 1, 2 as two unconditional selects (nested conditionals may become branches)
+template <class T> __device__ __forceinline__ T sel012(int i, T v0, T v1, T v2) {
+    T r = i >= 1 ? v1 : v0;
+    return i >= 2 ? v2 : r;
+}
+
 template <bool DIVN>
 __global__ __launch_bounds__(256) void k_accumulate_sr_frames(float* sum, float* wsum, const float* kern4, int w,
                                                               int h, int out_pitch, int mask_pitch, int kern_pitch,
@@ -485,27 +491,53 @@ __global__ __launch_bounds__(256) void k_accumulate_sr_frames(float* sum, float*
         typedef const __attribute__((address_space(1))) char* gbytes;
         const graw raw = (graw)fa.raw[f];
         const gbytes mask = (gbytes)fa.mask[f];
-        int rx[5], rpar[5];
+        // The 5 x 5 taps read at most 3 x 3 distinct raw samples: tap column px reads column rx[px], one of
+        // bx[0..2] (rx is non-decreasing in px and grows by at most one per step), and likewise for rows.
+        // The 9 samples are loaded and normalised once; a tap selects its own.
+        int rx[5], ry[5], kx[5], ky[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             rx[i] = clampi((x + i - 2 + sx + g.ox) / 2, cx0, cx1);
-            rpar[i] = rx[i] & 1;
+            ry[i] = clampi((y + i - 2 + sy + g.oy) / 2, cy0, cy1);
         }
-        // all 50 loads of the frame first (a site outside the three channels reads channel 0 and adds nothing)
+        const int bx[3] = {rx[0], min(rx[0] + 1, cx1), min(rx[0] + 2, cx1)};
+        const int by[3] = {ry[0], min(ry[0] + 1, cy1), min(ry[0] + 2, cy1)};
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            kx[i] = rx[i] - rx[0];
+            ky[i] = ry[i] - ry[0];
+        }
+        float sv[3][3];
+        int sc[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const graw rrow = raw + (size_t)by[r] * w;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int c = site_channel(a.cfa.at(by[r], bx[q]));
+                sc[r][q] = c;
+                sv[r][q] = norm_raw<DIVN>((float)rrow[bx[q]], c, a);
+            }
+        }
+        // certainties of the 25 taps (a site outside the three channels reads channel 0 and adds nothing)
+        float cv[25];
         int ch[25];
-        float rv[25], cv[25];
+        float rowv[5][3];
 #pragma unroll
         for (int py = -2; py <= 2; ++py) {
-            const int ppsy = clampi((y + py + sy + g.oy) / 2, cy0, cy1);
-            const graw rrow = raw + (size_t)ppsy * w;
+            const int k = ky[py + 2];
+            int rowc[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                rowv[py + 2][q] = sel012(k, sv[0][q], sv[1][q], sv[2][q]);
+                rowc[q] = sel012(k, sc[0][q], sc[1][q], sc[2][q]);
+            }
             const gbytes mr = mask + mrow[py + 2];
-            const int ce = site_channel((ppsy & 1) ? a.cfa.c[2] : a.cfa.c[0]);  // channel at an even column
-            const int co = site_channel((ppsy & 1) ? a.cfa.c[3] : a.cfa.c[1]);  // and at an odd one
 #pragma unroll
             for (int px = -2; px <= 2; ++px) {
-                const int i = (py + 2) * 5 + px + 2, c = rpar[px + 2] ? co : ce;
+                const int i = (py + 2) * 5 + px + 2, j = kx[px + 2];
+                const int c = sel012(j, rowc[0], rowc[1], rowc[2]);
                 ch[i] = c;
-                rv[i] = (float)rrow[rx[px + 2]];
                 cv[i] = *(const __attribute__((address_space(1))) float*)(mr + mcol[px + 2] + 4 * (c < 0 ? 0 : c));
             }
         }
@@ -514,9 +546,9 @@ __global__ __launch_bounds__(256) void k_accumulate_sr_frames(float* sum, float*
         for (int py = -2; py <= 2; ++py) {
 #pragma unroll
             for (int px = -2; px <= 2; ++px) {
-                const int i = (py + 2) * 5 + px + 2, c = ch[i];
+                const int i = (py + 2) * 5 + px + 2, c = ch[i], j = kx[px + 2];
                 const float wgt = wt[i];
-                const float v = norm_raw<DIVN>(rv[i], c, a);
+                const float v = sel012(j, rowv[py + 2][0], rowv[py + 2][1], rowv[py + 2][2]);
                 float cert = cv[i];
                 if (!isfinite(cert)) cert = 0.0f;
                 const float t = v * wgt * cert, u = wgt * cert;
