@@ -281,6 +281,95 @@ __global__ __launch_bounds__(DQ_X* DQ_Y) void k_debayer_quad(int w, int h, const
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Colour pipeline of the merged image (R:PEFStudioDX/ImageStackAlignatorController.cs:2198-2262,
+// SaveAs16BitTiff; 2264-2322, ProcessColorsAndShow), one pass per output pixel instead of nine NPP calls:
+// ThresholdLTGT to [0, CameraWhite], ColorTwist camera -> ProPhoto, ThresholdLTGT to [0, 1], LUTCubic (the
+// tone curve), ColorTwist ProPhoto -> sRGB, Mul 2^Exposure, GammasRGB, Mul scale, the crop ROI and Convert
+// with round-to-nearest-even and saturation, to RGB u16 (the TIFF) or BGRA u8 with alpha 255 (the display
+// image). Every step is float arithmetic in the written order.
+struct ColorArgs {
+    float wb[3];     // upper bounds of the first threshold (CameraWhite)
+    float m1[12];    // camera -> ProPhoto, 3 x 4 row-major: out_r = ((m[0] v0 + m[1] v1) + m[2] v2) + m[3]
+    float m2[12];    // ProPhoto -> sRGB
+    float amp;       // 2^Exposure
+    float scale;     // 65535 (u16) or 255 (u8)
+    int lut_n;       // tone-curve levels (4 .. kLutMax), x strictly increasing
+    int crop_x, crop_y, w, h;  // output = the crop w x h at (crop_x, crop_y) of the input
+    int bgra8;       // 0: RGB u16, 1: BGRA u8
+};
+constexpr int kLutMax = 1024;
+
+__device__ __forceinline__ float twist_row(const float* m, float a, float b, float c) {
+    return ((m[0] * a + m[1] * b) + m[2] * c) + m[3];
+}
+
+// Cubic through the four levels around v (Lagrange form, weights in float in a fixed order): the window
+// starts one level left of v's interval and is clamped into [0, n - 4]; v is clamped to [x0, x(n-1)].
+__device__ __forceinline__ float lut_cubic(float v, const float* lx, const float* ly, int n) {
+    v = fminf(fmaxf(v, lx[0]), lx[n - 1]);
+    int lo = 0, hi = n - 1;  // largest i with lx[i] <= v, i <= n - 2
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lx[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    const int j0 = min(max(lo - 1, 0), n - 4);
+    float r = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float wk = 1.0f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            if (m != k) wk = wk * ((v - lx[j0 + m]) / (lx[j0 + k] - lx[j0 + m]));
+        r = k == 0 ? ly[j0] * wk : r + ly[j0 + k] * wk;
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_color_pipeline(const float* __restrict__ img, int pitch, const float* lut_x,
+                                                        const float* lut_y, void* out, int out_pitch, ColorArgs a) {
+    __shared__ float sx[kLutMax], sy[kLutMax];
+    for (int i = threadIdx.y * blockDim.x + threadIdx.x; i < a.lut_n; i += blockDim.x * blockDim.y) {
+        sx[i] = lut_x[i];
+        sy[i] = lut_y[i];
+    }
+    __syncthreads();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= a.w || y >= a.h) return;
+    const float* p = row_ptr(img, pitch, y + a.crop_y) + (size_t)(x + a.crop_x) * 3;
+    float v[3] = {p[0], p[1], p[2]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (v[c] < 0.0f) v[c] = 0.0f;
+        else if (v[c] > a.wb[c]) v[c] = a.wb[c];
+    }
+    float t[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) t[r] = twist_row(a.m1 + 4 * r, v[0], v[1], v[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (t[c] < 0.0f) t[c] = 0.0f;
+        else if (t[c] > 1.0f) t[c] = 1.0f;
+        t[c] = lut_cubic(t[c], sx, sy, a.lut_n);
+    }
+    float o[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) o[r] = twist_row(a.m2 + 4 * r, t[0], t[1], t[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = srgb_prepared(o[c] * a.amp) * a.scale;
+    if (a.bgra8) {
+        unsigned char* q = (unsigned char*)out + (size_t)y * out_pitch + (size_t)x * 4;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[2 - c] = (unsigned char)fminf(fmaxf(rintf(o[c]), 0.0f), 255.0f);
+        q[3] = 255;
+    } else {
+        unsigned short* q = (unsigned short*)((char*)out + (size_t)y * out_pitch) + (size_t)x * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[c] = (unsigned short)fminf(fmaxf(rintf(o[c]), 0.0f), 65535.0f);
+    }
+}
+
 __global__ void k_gamma(float* img, int w, int h, int pitch) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
@@ -862,6 +951,38 @@ int isa_srgb_oetf(const float* in_dev, float* out_dev, size_t count, int exact, 
     else
         k_srgb_1d<true><<<blocks, 256, 0, isa::to_stream(stream)>>>(in_dev, out_dev, count);
     return isa::check_launch("isa_srgb_oetf");
+}
+
+int isa_color_pipeline(const float* img3_dev, int img_pitch, int width, int height, const float camera_white[3],
+                       const float camera_to_prophoto[12], const float* lut_x_dev, const float* lut_y_dev, int lut_n,
+                       const float prophoto_to_srgb[12], float exposure, int crop_left, int crop_top, int crop_width,
+                       int crop_height, int format, void* out_dev, int out_pitch, isa_stream_t stream) {
+    ISA_REQUIRE(img3_dev && out_dev && lut_x_dev && lut_y_dev && camera_white && camera_to_prophoto && prophoto_to_srgb,
+                "isa_color_pipeline: null argument");
+    ISA_REQUIRE(width > 0 && height > 0 && img_pitch >= width * 12, "isa_color_pipeline: bad image size or pitch");
+    ISA_REQUIRE(lut_n >= 4 && lut_n <= kLutMax, "isa_color_pipeline: the tone curve needs 4 .. %d levels", kLutMax);
+    ISA_REQUIRE(format == ISA_COLOR_RGB16 || format == ISA_COLOR_BGRA8, "isa_color_pipeline: unknown format");
+    ISA_REQUIRE(crop_left >= 0 && crop_top >= 0 && crop_width > 0 && crop_height > 0 &&
+                    crop_left + crop_width <= width && crop_top + crop_height <= height,
+                "isa_color_pipeline: crop outside the image");
+    ISA_REQUIRE(out_pitch >= crop_width * (format == ISA_COLOR_RGB16 ? 6 : 4), "isa_color_pipeline: bad output pitch");
+    ColorArgs a;
+    for (int c = 0; c < 3; ++c) a.wb[c] = camera_white[c];
+    for (int i = 0; i < 12; ++i) {
+        a.m1[i] = camera_to_prophoto[i];
+        a.m2[i] = prophoto_to_srgb[i];
+    }
+    a.amp = (float)std::pow(2.0, (double)exposure);
+    a.scale = format == ISA_COLOR_RGB16 ? 65535.0f : 255.0f;
+    a.lut_n = lut_n;
+    a.crop_x = crop_left;
+    a.crop_y = crop_top;
+    a.w = crop_width;
+    a.h = crop_height;
+    a.bgra8 = format == ISA_COLOR_BGRA8;
+    k_color_pipeline<<<dim3(isa::div_up(crop_width, 32), isa::div_up(crop_height, 8)), dim3(32, 8), 0,
+                       isa::to_stream(stream)>>>(img3_dev, img_pitch, lut_x_dev, lut_y_dev, out_dev, out_pitch, a);
+    return isa::check_launch("isa_color_pipeline");
 }
 
 int isa_color_to_gray(const float* in3_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height,

@@ -73,6 +73,12 @@ enum isa_xc_precision {
     ISA_XC_FP32 = 3    /* fp32 operands */
 };
 
+/* Output of isa_color_pipeline. */
+enum isa_color_format {
+    ISA_COLOR_RGB16 = 0, /* 3 x u16 per pixel, scale 65535: SaveAs16BitTiff */
+    ISA_COLOR_BGRA8 = 1  /* 4 x u8 per pixel (B, G, R, 255), scale 255: ProcessColorsAndShow's display image */
+};
+
 #define ISA_MAX_LEVELS 8
 
 /* Parameters of the merge path: the controller's DependencyProperties with
@@ -191,6 +197,25 @@ int isa_srgb_oetf(const float* in_dev, float* out_dev, size_t count, int exact, 
 /* NPP ColorToGray (R:...Controller.cs:2404-2408): gray = c0 R + c1 G + c2 B. */
 int isa_color_to_gray(const float* in3_dev, int in_pitch, float* out_dev, int out_pitch, int width, int height,
                       const float coef[3], isa_stream_t stream);
+/* The colour pipeline after the merge (R:...Controller.cs:2198-2262 SaveAs16BitTiff, 2264-2322
+ * ProcessColorsAndShow) in one pass: NPP ThresholdLTGT to [0, camera_white], ColorTwist camera_to_prophoto
+ * (3 x 4 row-major, the host's DNGColorSpace.ProPhoto.FromPCS * CameraToPCS), ThresholdLTGT to [0, 1],
+ * LUTCubic through the tone curve (lut_n levels lut_x_dev -> lut_y_dev, x increasing, 4..1024; cubic through
+ * the four levels around the value), ColorTwist prophoto_to_srgb, Mul 2^exposure, GammasRGB, Mul 65535 or
+ * 255, then the crop ROI converted with round-to-nearest-even and saturation into out_dev (crop_width x
+ * crop_height, pitch out_pitch bytes) as isa_color_format. The NPP steps' own rounding is not documented: the
+ * float operations here run in the written order. */
+int isa_color_pipeline(const float* img3_dev, int img_pitch, int width, int height, const float camera_white[3],
+                       const float camera_to_prophoto[12], const float* lut_x_dev, const float* lut_y_dev, int lut_n,
+                       const float prophoto_to_srgb[12], float exposure, int crop_left, int crop_top, int crop_width,
+                       int crop_height, int format, void* out_dev, int out_pitch, isa_stream_t stream);
+/* ImageFileDirectory(width, height).SaveAsTiff (R:PefFile/ImageFileDirectory.cs:76-123): the same little-endian
+ * baseline TIFF (13 entries in the reference's order, uncompressed 16-bit RGB in one strip, 72 dpi, "MK TIFF"),
+ * byte for byte. isa_tiff16_encode writes it to dst (host memory, capacity bytes) and returns its size, or 0
+ * when capacity is too small (dst may be NULL to query the size); isa_write_tiff16 writes it to a file.
+ * rgb_host: width x height x 3 u16, row-major. */
+size_t isa_tiff16_encode(void* dst, size_t capacity, const uint16_t* rgb_host, int width, int height);
+int isa_write_tiff16(const char* path, const uint16_t* rgb_host, int width, int height);
 /* gaussian_filter_1D (R:...Controller.cs:2351-2382): writes the taps to taps_out (host), returns their count
  * (<= 99) or a negative value when capacity is too small. */
 int isa_gaussian_taps(float sigma, float* taps_out, int capacity);

@@ -119,6 +119,74 @@ template <class T> void color_to_gray(int w, int h, const T* img3, T* gray, cons
 }
 
 // ---------------------------------------------------------------------------
+// f3  Colour pipeline of the merged image (R:PEFStudioDX/ImageStackAlignatorController.cs:2198-2262,
+//     SaveAs16BitTiff; 2264-2322, ProcessColorsAndShow): NPP ThresholdLTGT(0, 0, CameraWhite, CameraWhite),
+//     ColorTwist camera -> ProPhoto, ThresholdLTGT(0, 0, 1, 1), LUTCubic through the tone curve, ColorTwist
+//     ProPhoto -> sRGB, Mul 2^Exposure, GammasRGB, Mul 65535 (255), ROI, Convert (round to nearest even,
+//     saturate). NPP's own rounding order is undocumented; the float operations run in the written order,
+//     and LUTCubic is the cubic through the four levels around the value (Lagrange form).
+
+// Tone curve: v clamped to [x0, x(n-1)], interval i = the largest with x[i] <= v (i <= n - 2), window of four
+// levels from i - 1 clamped into [0, n - 4], weights prod_(m != k) (v - x_m) / (x_k - x_m) in float.
+inline float lut_cubic(float v, const float* lx, const float* ly, int n) {
+    v = std::fmin(std::fmax(v, lx[0]), lx[n - 1]);
+    int i = 0;
+    while (i + 1 < n - 1 && lx[i + 1] <= v) ++i;
+    const int j0 = std::min(std::max(i - 1, 0), n - 4);
+    float r = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float wk = 1.0f;
+        for (int m = 0; m < 4; ++m)
+            if (m != k) wk = wk * ((v - lx[j0 + m]) / (lx[j0 + k] - lx[j0 + m]));
+        r = k == 0 ? ly[j0] * wk : r + ly[j0 + k] * wk;
+    }
+    return r;
+}
+
+// Convert 32f -> integer with NPP_RND_NEAR (ties to even) and saturation to [0, maxv].
+inline float convert_near(float v, float maxv) { return std::fmin(std::fmax(std::nearbyint(v), 0.0f), maxv); }
+
+inline float twist_row(const float* m, float a, float b, float c) { return ((m[0] * a + m[1] * b) + m[2] * c) + m[3]; }
+
+// img3: w x h float3; out: crop w x crop h, 3 x u16 (bgra8 = 0) or 4 x u8 (B, G, R, 255).
+void color_pipeline(int w, int h, const float* img3, const float wb[3], const float m1[12], const float* lx,
+                    const float* ly, int n, const float m2[12], float amp, int cx, int cy, int cw, int ch, int bgra8,
+                    void* out) {
+    const float scale = bgra8 ? 255.0f : 65535.0f;
+    (void)h;
+    for (int y = 0; y < ch; ++y)
+        for (int x = 0; x < cw; ++x) {
+            const float* p = img3 + ((size_t)(y + cy) * w + (x + cx)) * 3;
+            float v[3] = {p[0], p[1], p[2]}, t[3], o[3];
+            for (int c = 0; c < 3; ++c) {
+                if (v[c] < 0.0f) v[c] = 0.0f;
+                else if (v[c] > wb[c]) v[c] = wb[c];
+            }
+            for (int r = 0; r < 3; ++r) t[r] = twist_row(m1 + 4 * r, v[0], v[1], v[2]);
+            for (int c = 0; c < 3; ++c) {
+                if (t[c] < 0.0f) t[c] = 0.0f;
+                else if (t[c] > 1.0f) t[c] = 1.0f;
+                t[c] = lut_cubic(t[c], lx, ly, n);
+            }
+            for (int r = 0; r < 3; ++r) o[r] = twist_row(m2 + 4 * r, t[0], t[1], t[2]);
+            for (int c = 0; c < 3; ++c) {
+                float g = o[c] * amp;  // GammasRGB: NaN -> 0, clamp to [0, 1], OETF
+                if (std::isnan(g)) g = 0.0f;
+                g = std::fmax(std::fmin(g, 1.0f), 0.0f);
+                o[c] = srgb_oetf(g) * scale;
+            }
+            if (bgra8) {
+                unsigned char* q = (unsigned char*)out + ((size_t)y * cw + x) * 4;
+                for (int c = 0; c < 3; ++c) q[2 - c] = (unsigned char)convert_near(o[c], 255.0f);
+                q[3] = 255;
+            } else {
+                uint16_t* q = (uint16_t*)out + ((size_t)y * cw + x) * 3;
+                for (int c = 0; c < 3; ++c) q[c] = (uint16_t)convert_near(o[c], 65535.0f);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // a3  Fourier high-pass (R:PEFStudioDX/PreAlignment.cs:268-281 with the
 //     transfer function of R:Kernels/kernel.cu:702-779, lp = lps = 1).
 // The transfer H(kx,ky) equals 1 exactly for every bin with dist >= hp+hps,

@@ -97,6 +97,12 @@ def _declare(L):
     L.ora_lk_solve_fp64.restype = C.c_int
     L.ora_lk_ill_conditioned.argtypes = [C.c_float] * 3 + [C.c_double]
     L.ora_lk_ill_conditioned.restype = C.c_int
+    L.ora_color_pipeline.argtypes = [I, I, P, P, P, P, P, I, P, C.c_float, I, I, I, I, I, P]
+    L.ora_color_pipeline.restype = None
+    L.ora_lut_cubic.argtypes = [C.c_float, P, P, I]
+    L.ora_lut_cubic.restype = C.c_float
+    L.ora_convert_near.argtypes = [C.c_float, C.c_float]
+    L.ora_convert_near.restype = C.c_float
 
 
 def lk_solve_fp64(a, b, d, b0, b1, min_det):
@@ -109,6 +115,71 @@ def lk_solve_fp64(a, b, d, b0, b1, min_det):
 
 def lk_ill_conditioned(a, b, d, ratio):
     return bool(lib().ora_lk_ill_conditioned(a, b, d, ratio))
+
+
+def color_pipeline(img3, camera_white, cam_to_prophoto, lut_x, lut_y, prophoto_to_srgb, exposure, crop=None,
+                   bgra8=False):
+    """The colour pipeline after the merge (float32): img3 (h, w, 3) -> (ch, cw, 3) u16, or (ch, cw, 4) u8 BGRA.
+    crop = (left, top, width, height); the amplification 2^exposure is formed in double and rounded to float."""
+    img = np.ascontiguousarray(img3, np.float32)
+    h, w = img.shape[:2]
+    cx, cy, cw, ch = crop if crop is not None else (0, 0, w, h)
+    f = lambda a, n: np.ascontiguousarray(np.asarray(a, np.float32).reshape(n))
+    wb, m1, m2 = f(camera_white, 3), f(cam_to_prophoto, 12), f(prophoto_to_srgb, 12)
+    lx, ly = f(lut_x, -1), f(lut_y, -1)
+    out = np.zeros((ch, cw, 4), np.uint8) if bgra8 else np.zeros((ch, cw, 3), np.uint16)
+    lib().ora_color_pipeline(w, h, ptr(img), ptr(wb), ptr(m1), ptr(lx), ptr(ly), len(lx), ptr(m2),
+                             float(np.float32(2.0 ** exposure)), cx, cy, cw, ch, int(bgra8), ptr(out))
+    return out
+
+
+def lut_cubic(v, lut_x, lut_y):
+    lx, ly = np.ascontiguousarray(lut_x, np.float32), np.ascontiguousarray(lut_y, np.float32)
+    return lib().ora_lut_cubic(float(v), ptr(lx), ptr(ly), len(lx))
+
+
+def convert_near(v, maxv):
+    return lib().ora_convert_near(float(v), float(maxv))
+
+
+def tiff16_bytes(rgb):
+    """ImageFileDirectory(width, height).SaveAsTiff (R:PefFile/ImageFileDirectory.cs:76-123) restated: header,
+    13 entries in the reference's construction order, the values that do not fit an entry appended in that
+    order (each entry's offset field pointing at them), then the pixels, whose position is StripOffsets."""
+    import struct
+
+    rgb = np.ascontiguousarray(rgb, np.uint16)
+    h, w = rgb.shape[:2]
+    SHORT, LONG, RATIONAL, ASCII = 3, 4, 5, 2
+    entries = [(256, LONG, 1, w), (257, LONG, 1, h), (258, SHORT, 3, None), (259, SHORT, 1, 1), (262, SHORT, 1, 2),
+               (273, LONG, 1, "pixels"), (277, SHORT, 1, 3), (278, LONG, 1, h), (279, LONG, 1, w * h * 6),
+               (282, RATIONAL, 1, None), (283, RATIONAL, 1, None), (296, SHORT, 1, 1), (271, ASCII, 8, None)]
+    extra = {258: struct.pack("<3H", 16, 16, 16), 282: struct.pack("<2I", 720, 10), 283: struct.pack("<2I", 720, 10),
+             271: b"MK TIFF\0"}
+    pos = 8 + 2 + 12 * len(entries) + 4
+    offsets = {}
+    for tag, _, _, v in entries:
+        if v is None:
+            offsets[tag] = pos
+            pos += len(extra[tag])
+    pix_at = pos
+    out = bytearray(b"II*\0" + struct.pack("<I", 8) + struct.pack("<H", len(entries)))
+    for tag, typ, cnt, v in entries:
+        if v is None:
+            val = offsets[tag]
+        elif v == "pixels":
+            val = pix_at
+        else:
+            val = v
+        out += struct.pack("<HHI", tag, typ, cnt)
+        out += struct.pack("<HH", val, 0) if (typ == SHORT and cnt == 1) else struct.pack("<I", val)
+    out += struct.pack("<I", 0)
+    for tag, _, _, v in entries:
+        if v is None:
+            out += extra[tag]
+    assert len(out) == pix_at
+    out += rgb.astype("<u2").tobytes()
+    return bytes(out)
 
 
 def lk_ill_counts():

@@ -96,3 +96,24 @@ def test_errors_are_reported_without_a_gpu(isa):
     assert e.value.code == 1 and "isa_resize_box" in str(e.value)
     assert isa.lib().isa_shift_tables(1, 0, 0, 5, None, None, None) < 0
     assert isa.lib().isa_shift_tables(5, 0, 7, 5, None, None, None) < 0
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (5, 7), (64, 33)])
+def test_tiff16_encode_matches_oracle(isa, oracle, tmp_path, h, w):
+    """isa_tiff16_encode / isa_write_tiff16 (host code) write ImageFileDirectory.SaveAsTiff's bytes: the
+    oracle's restatement, byte for byte, in memory and in a file."""
+    from imagestackalignator_amd import export
+
+    rng = np.random.default_rng(h * w)
+    rgb = rng.integers(0, 65536, (h, w, 3)).astype(np.uint16)
+    want = oracle.tiff16_bytes(rgb)
+    assert export.tiff16_bytes(rgb) == want
+    path = tmp_path / "out.tif"
+    export.write_tiff16(str(path), rgb)
+    assert path.read_bytes() == want
+    assert isa.lib().isa_tiff16_encode(None, 0, rgb.ctypes.data, w, h) == len(want)
+    small = (C.c_ubyte * 16)()
+    assert isa.lib().isa_tiff16_encode(small, 16, rgb.ctypes.data, w, h) == 0  # capacity too small
+    assert isa.lib().isa_tiff16_encode(None, 0, rgb.ctypes.data, 0, h) == 0
+    with pytest.raises(isa.IsaError):
+        isa.call("isa_write_tiff16", str(tmp_path / "no_such_dir" / "x.tif").encode(), rgb.ctypes.data, w, h)

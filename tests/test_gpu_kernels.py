@@ -676,3 +676,31 @@ def test_accumulate_frames_superres(isa, oracle, gpu, grid, quant):
         isa.call("isa_accumulate_frames_superres", ptrs(d_raw), ptrs(d_mask), ptrs(d_flow), n, p(ds), p(dw),
                  p(dev(kern)), isa.f32s(*white), isa.f32s(*black), w, h, 2 * w, 2 * h, 2, 0, 2 * w * 12, hw * 16,
                  w * 16, w * 8, isa.i32s(*RGGB), quant, stream())
+
+
+@pytest.mark.parametrize("fmt,crop", [("rgb16", None), ("rgb16", (3, 2, 41, 27)), ("bgra8", (1, 3, 50, 30))])
+def test_color_pipeline(isa, oracle, gpu, fmt, crop):
+    """isa_color_pipeline (ThresholdLTGT, ColorTwist, ThresholdLTGT, LUTCubic, ColorTwist, Mul, GammasRGB, Mul,
+    ROI, Convert) equals the oracle bit for bit: values outside [0, white] and NaN, a tone curve of 256 levels,
+    realistic twists with offsets, a non-zero exposure, both output formats."""
+    from imagestackalignator_amd import export
+
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(91 + len(fmt))
+    h, w = 33, 57
+    img = rng.uniform(-0.1, 1.3, (h, w, 3)).astype(np.float32)
+    img[5, 7, 1] = np.nan
+    wb = [0.92, 1.0, 0.81]
+    m1 = [[0.71, 0.21, 0.05, 0.002], [0.12, 0.88, -0.03, 0.0], [0.01, -0.09, 1.07, -0.001]]
+    m2 = [[1.82, -0.51, -0.21, 0.0], [-0.19, 1.22, -0.03, 0.0], [0.01, -0.13, 1.12, 0.0]]
+    lx = (np.arange(256) / 255.0).astype(np.float32)
+    ly = (lx + 0.15 * np.sin(2 * np.pi * lx) * lx * (1 - lx)).astype(np.float32)  # a mild S-curve
+    out = export.color_pipeline(dev(img), wb, m1, dev(lx), dev(ly), m2, exposure=0.35, crop=crop, fmt=fmt)
+    want = oracle.color_pipeline(img, wb, m1, lx, ly, m2, 0.35, crop=crop, bgra8=fmt == "bgra8")
+    got = host(out)
+    if fmt == "rgb16":
+        got = got.view(np.uint16)
+    np.testing.assert_array_equal(got, want)
+    assert want.max() > (200 if fmt == "bgra8" else 50000)
+    with pytest.raises(isa.IsaError):  # a crop outside the image
+        export.color_pipeline(dev(img), wb, m1, dev(lx), dev(ly), m2, crop=(0, 0, w + 1, h), fmt=fmt)
