@@ -53,7 +53,9 @@ enum isa_status {
     ISA_ERR_ARG = 1,         /* invalid size / argument (reference: ArgumentOutOfRangeException) */
     ISA_ERR_HIP = 2,         /* HIP runtime error (reference: CudaException) */
     ISA_ERR_UNSUPPORTED = 3, /* combination not implemented by this library */
-    ISA_ERR_ALLOC = 4        /* device allocation failed */
+    ISA_ERR_ALLOC = 4,       /* device allocation failed */
+    ISA_ERR_DATA = 5         /* malformed file or pixel stream (reference: FileLoadException, ArgumentException,
+                                IndexOutOfRangeException or EndOfStreamException while reading a raw file) */
 };
 
 enum isa_tracking_strategy { /* R:PEFStudioDX/ShiftCollection.cs:37-47 */
@@ -480,6 +482,63 @@ size_t isa_merge_refine_workspace(const isa_merge_diff* problem);
 int isa_merge_refine_step(const isa_merge_diff* problem, float* flow_dev, float theta[4], const float* gt3_dev,
                           float lr_flow, const float lr_theta[4], float max_step, float* scale, double* loss,
                           int* accepted, void* workspace, isa_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Raw ingest: PEF and DNG files to the u16 Bayer image and its levels.
+ * Replaces RawFile / PEFFile / DNGFile (R:PefFile/RawFile.cs:28-279, R:PefFile/PEFFile.cs:28-316,
+ * R:PefFile/DNGFile.cs:36-950), which the controller constructs one task per file (".pef" -> PEFFile, else
+ * DNGFile; R:PEFStudioDX/ImageStackAlignatorController.cs:1066-1119). The container (TIFF IFDs,
+ * SubIFDs, the Exif IFD, the Pentax maker note, DNGPrivateData) is parsed on the host by isa_raw_open; the
+ * pixel data (Pentax Huffman stream, lossless-JPEG tiles or strips, uncompressed strips) is decoded on the
+ * device by isa_raw_decode, many files per call.
+ * --------------------------------------------------------------------- */
+enum isa_raw_format { ISA_RAW_AUTO = 0, ISA_RAW_PEF = 1, ISA_RAW_DNG = 2 };
+enum isa_raw_compression {
+    ISA_RAW_UNCOMPRESSED = 1,      /* TIFF Compression 1 */
+    ISA_RAW_LOSSLESS_JPEG = 7,     /* DNG lossless JPEG (SOF3) tiles or strips */
+    ISA_RAW_PENTAX_HUFFMAN = 65535 /* PEF: maker-note tag 0x0220 present */
+};
+/* The RawFile properties (R:PefFile/RawFile.cs:121-229), in R, G, B order where the reference says so. */
+typedef struct isa_raw_info {
+    int format;                        /* ISA_RAW_PEF or ISA_RAW_DNG */
+    int compression;                   /* isa_raw_compression */
+    int width, height;                 /* RawWidth, RawHeight */
+    int bit_depth;                     /* BitsPerSample */
+    int crop_left, crop_top;           /* CropLeft, CropTop */
+    int cropped_width, cropped_height; /* CroppedWidth, CroppedHeight */
+    int bayer_pattern[4];              /* BayerPattern: 0 red, 1 green, 2 blue, 3 cyan, 4 magenta, 5 yellow, 6 white */
+    float white_level[3];              /* WhiteLevel (DNG: after the decode, less the largest black delta) */
+    float black_level[3];              /* BlackLevel */
+    float white_balance[3];            /* WhiteBalance */
+    float color_twist[12];             /* ColorTwist, 3 x 4 row-major */
+    int color_twist_is_identity;       /* ColorTwistIsIdentity */
+    float roll_angle;                  /* RollAngle (degrees) */
+    int roll_angle_present;            /* RollAnglePresent */
+    float noise_alpha, noise_beta;     /* NoiseModelAlpha, NoiseModelBeta (0 when the file holds none) */
+    int iso;                           /* ISO */
+    unsigned exposure_num, exposure_den; /* ExposureTime */
+    int orientation;                   /* Orientation (TIFF value, 1 = normal) */
+    int max_black_delta;               /* DNG: largest BlackLevelDeltaH + BlackLevelDeltaV over the decoded pixels */
+    char make[64];                     /* Make */
+    char unique_model[64];             /* UniqueModelName */
+    char recording_date[32];           /* RecordingDate as written in the file ("YYYY:MM:DD HH:MM:SS") */
+} isa_raw_info;
+typedef struct isa_raw_file isa_raw_file;
+/* Parses the file held in host memory (file, size bytes) as the PEFFile / DNGFile constructors do (format
+ * ISA_RAW_AUTO: DNG when IFD0 holds DNGVersion or UniqueCameraModel, else PEF). Keeps a copy of what the
+ * decode needs; the caller's buffer may be freed on return. *out receives a handle for isa_raw_close. */
+int isa_raw_open(const void* file, size_t size, int format, isa_raw_file** out);
+int isa_raw_get_info(const isa_raw_file* f, isa_raw_info* info);
+void isa_raw_close(isa_raw_file* f);
+/* Workspace bytes isa_raw_decode needs for these n files. */
+int isa_raw_decode_workspace(const isa_raw_file* const* files, int n, size_t* bytes);
+/* Decodes n files at once: file_dev[i] is the device copy of file i's bytes (its size as given to isa_raw_open),
+ * raw_dev[i] its width x height u16 RawImage (raw_pitch[i] bytes per row; pixels no tile or strip covers are
+ * 0). Synchronises the stream and returns ISA_ERR_DATA, naming the file, where the reference's decode throws.
+ * PEF Huffman and lossless-JPEG streams are decoded in parallel by self-synchronising chunks (one work-group
+ * per stream, or per restart interval), then the predictors are undone by scans. */
+int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* file_dev, void* const* raw_dev,
+                   const int* raw_pitch, void* workspace_dev, size_t workspace_bytes, isa_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Synthetic bursts (benchmark / test input; not part of the reference).
