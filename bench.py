@@ -508,6 +508,78 @@ def c4_section(isa, torch, refine_steps=3):
     return res
 
 
+def ingest_section(isa, torch, raw, steps, warmup, cpu=True, distinct=4, frames=20):
+    """Raw ingest of a burst: `frames` PEF files (Pentax Huffman) and `frames` DNG files (256 x 256 lossless-JPEG
+    tiles, two components, predictor 1) holding `distinct` frames of the synthetic burst, cycled. Timed: one
+    isa_raw_decode call over all files, file bytes resident in HBM, outputs and workspace allocated beforehand
+    (median of `steps`). The host parse (isa_raw_open) is timed separately. Decoded frames are checked against
+    the originals. cpu_baseline: the oracle's decoders (the reference's loops), one thread per file."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import raw_writer as W
+    from imagestackalignator_amd import raw as P
+
+    n, h, w = raw.shape
+    frames_np = [raw[i].cpu().numpy().view(np.uint16).copy() for i in range(min(distinct, n))]
+    out = {}
+    for kind in ("pef", "dng"):
+        t0 = time.perf_counter()
+        if kind == "pef":
+            blobs = [W.write_pef(f) for f in frames_np]
+        else:
+            blobs = [W.write_dng(f, tile=(256, 256), comps=2) for f in frames_np]
+        enc_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        files = [P.RawFile(blobs[i % len(blobs)]) for i in range(frames)]
+        parse_ms = (time.perf_counter() - t0) * 1000.0 / frames
+        bufs = []
+        for f in files:
+            b = torch.empty(((len(f.data) + 3) // 4 * 4,), dtype=torch.uint8, device="cuda")
+            b[: len(f.data)].copy_(torch.frombuffer(bytearray(f.data), dtype=torch.uint8))
+            bufs.append(b)
+        outs = [torch.empty((h, w), dtype=torch.int16, device="cuda") for _ in files]
+        need = __import__("ctypes").c_size_t()
+        hs = (__import__("ctypes").c_void_p * frames)(*[f._h.value for f in files])
+        isa.call("isa_raw_decode_workspace", hs, frames, __import__("ctypes").byref(need))
+        ws = torch.empty((need.value,), dtype=torch.uint8, device="cuda")
+        run = lambda: P.decode_device(files, bufs, outs, workspace=ws)
+        mean_ms, med_ms, _ = timed_runs(torch, run, steps, warmup)
+        ok = all(np.array_equal(outs[i].cpu().numpy().view(np.uint16), frames_np[i]) for i in range(len(frames_np)))
+        walks, redec = __import__("ctypes").c_int(), __import__("ctypes").c_int()
+        isa.call("isa_raw_decode_stats", __import__("ctypes").byref(walks), __import__("ctypes").byref(redec))
+        comp_bytes = sum(len(f.data) for f in files)
+        algo = comp_bytes + 2 * frames * w * h
+        rec = {"workload": f"{frames} {kind.upper()} files of {w}x{h} 14-bit ({len(frames_np)} distinct frames of the "
+                           f"burst, cycled)" + (", Pentax Huffman strip" if kind == "pef" else
+                                                ", 256x256 lossless-JPEG tiles, 2 components, predictor 1"),
+               "value": round(frames * (w * h / 1e6) / (med_ms / 1000.0), 1), "unit": "frames*MPix/s decoded",
+               "median_ms": round(med_ms, 3), "compressed_mb": round(comp_bytes / 1e6, 1),
+               "host_parse_ms_per_file": round(parse_ms, 3), "bit_exact": ok,
+               "algorithmic_gb": round(algo / 1e9, 3), "achieved_gbs": round(algo / med_ms / 1e6, 1),
+               "frac": round(algo / med_ms / 1e6 / PEAK_HBM_GBS, 4), "encode_s": round(enc_s, 1),
+               "sync_walks_max": walks.value, "chunk_redecodes": redec.value}
+        if cpu:
+            from oracle import raw as R
+
+            threads = max(1, min(16, len(os.sched_getaffinity(0))))
+            t0 = time.perf_counter()
+            with cf.ThreadPoolExecutor(threads) as ex:
+                list(ex.map(lambda i: R.RawFile(blobs[i % len(blobs)]), range(frames)))
+            cpu_s = time.perf_counter() - t0
+            rec["cpu_baseline"] = {"value": round(frames * (w * h / 1e6) / cpu_s, 2), "unit": "frames*MPix/s decoded",
+                                   "cores": threads, "kind": "port",
+                                   "sample": f"the same {frames} files, the oracle's decoders (the reference's "
+                                             f"symbol-by-symbol loops), one thread per file, {cpu_s:.2f} s"}
+        out[kind] = rec
+        log(f"[bench] ingest {kind}: {json.dumps(rec)}")
+        del bufs, outs, ws, files
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -522,6 +594,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 2x, c2 and c4 sections")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the differentiable-merge section (prints its JSON object)")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the raw-ingest section (PEF / DNG decode)")
+    ap.add_argument("--ingest-only", action="store_true", help="only the raw-ingest section (prints its JSON object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -553,6 +627,10 @@ def main():
                              flow_max=cfg["flow"], occluded_fraction=cfg["occ"])
     log(f"[bench] synthesising {n} x {w}x{h} burst")
     raw, truth = isa.synth_burst(scfg)
+    if args.ingest_only:
+        print(json.dumps({"ingest": ingest_section(isa, torch, raw, args.steps, args.warmup,
+                                                   cpu=not args.no_cpu_baseline)}), flush=True)
+        return
     params = isa.default_params(levels=cfg["levels"], prealign_max_shift=cfg["prealign"])
     merger = isa.BurstMerger(w, h, n, params)
     out = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
@@ -670,6 +748,9 @@ def main():
         result["mixed"] = mixed_section(isa, torch, cfg, raw, out, gt, result["psnr_vs_gt_db"], args.steps,
                                         args.warmup, names, m_pairs)
     del merger
+    if not args.no_ingest:
+        log("[bench] raw ingest")
+        result["ingest"] = ingest_section(isa, torch, raw, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
     if not args.no_extra and args.config == "full_fp32":
         # the same burst merged at 2x over the whole frame (12000 x 8000 output)
         p2 = isa.default_params(levels=cfg["levels"], super_resolution=2, prealign_max_shift=cfg["prealign"])

@@ -221,26 +221,28 @@ struct LJHead {
     int tab[20] = {};       // table ids per DC/AC index (-1: none)
 };
 
-void make_decoder(const uint8_t*& src, const uint8_t* end, std::vector<uint16_t>& out) {
+// A Huffman table as make_decoder_ref (R:PefFile/DNGFile.cs:72-92) reads it: the longest code length `max`
+// and the codes per length with their symbols. The device expands it into the 1 + 2^max lookup entries.
+struct TableSpec {
+    int max = 0;
+    uint8_t count[17] = {};
+    std::vector<uint8_t> sym;
+};
+
+void make_spec(const uint8_t*& src, const uint8_t* end, TableSpec& t) {
     if (end - src < 16) throw DataError("DHT segment too short");
     const uint8_t* count = (src += 16) - 17;
     int max;
     for (max = 16; max > 0 && count[max] == 0; --max) {
     }
-    std::vector<uint16_t> h(1 + (1u << max), 0);
-    h[0] = (uint16_t)max;
-    int k = 1;
+    t.max = max;
+    for (int len = 1; len <= 16; ++len) t.count[len] = count[len];
     for (int len = 1; len <= max; ++len)
-        for (int i = 0; i < count[len]; ++i, ++src) {
-            uint8_t sym = src < end ? *src : 0;
-            for (int j = 0; j < 1 << (max - len); ++j)
-                if (k <= 1 << max) h[k++] = (uint16_t)(len << 8 | sym);
-        }
-    out = std::move(h);
+        for (int i = 0; i < count[len]; ++i, ++src) t.sym.push_back(src < end ? *src : 0);
 }
 
-// Parses the marker segments up to SOS; tables are appended to `tables` (their offsets go to head.tab).
-LJHead ljpeg_header(const uint8_t* seg, size_t n, std::vector<uint16_t>& tables, std::vector<int>& tab_offsets) {
+// Parses the marker segments up to SOS; tables are appended to `specs` (their indices go to head.tab).
+LJHead ljpeg_header(const uint8_t* seg, size_t n, std::vector<TableSpec>& specs) {
     LJHead jh;
     for (int& t : jh.tab) t = -1;
     std::vector<uint8_t> data(0x10000, 0);
@@ -283,11 +285,10 @@ LJHead ljpeg_header(const uint8_t* seg, size_t n, std::vector<uint16_t>& tables,
                 const uint8_t* end = data.data() + len;
                 int c;
                 while (dp < end && ((c = *dp++) & -20) == 0) {
-                    std::vector<uint16_t> h;
-                    make_decoder(dp, end, h);
-                    jh.tab[c] = (int)tab_offsets.size();
-                    tab_offsets.push_back((int)tables.size());
-                    tables.insert(tables.end(), h.begin(), h.end());
+                    TableSpec t;
+                    make_spec(dp, end, t);
+                    jh.tab[c] = (int)specs.size();
+                    specs.push_back(std::move(t));
                 }
                 break;
             }
@@ -340,7 +341,8 @@ struct isa_raw_file {
     bool large = false;
     std::vector<Unit> units;
     std::vector<Seg> segs;
-    std::vector<uint16_t> tables;
+    std::vector<uint16_t> tables;    // PEF: the Pentax table
+    std::vector<TableSpec> ljtabs;   // lossless JPEG: the tables of every tile / strip
     std::vector<uint16_t> lin, bh, bv;
     bool has_lin = false;
     uint64_t n_sym = 0;
@@ -741,8 +743,7 @@ void parse_dng(isa_raw_file& f, const Reader& r, const std::vector<Dir>& ifds) {
     in.compression = comp == 7 ? ISA_RAW_LOSSLESS_JPEG : comp;
     auto add_ljpeg = [&](size_t off, size_t len, int r0, int c0, int max_rows, int max_cols) {
         if (off > f.size || len > f.size - off) throw DataError("tile or strip beyond the end of the file");
-        std::vector<int> tab_offsets;
-        LJHead jh = ljpeg_header(r.d + off, len, f.tables, tab_offsets);
+        LJHead jh = ljpeg_header(r.d + off, len, f.ljtabs);
         if (!jh.ok || jh.algo != 0xc3) return;  // the reference leaves such a tile at 0
         if (jh.sraw != 0) throw std::invalid_argument("lossless JPEG with subsampled components (sraw)");
         if (jh.clrs < 1 || jh.clrs > 4) throw std::invalid_argument("lossless JPEG with other than 1 to 4 components");
@@ -793,7 +794,7 @@ void parse_dng(isa_raw_file& f, const Reader& r, const std::vector<Dir>& ifds) {
             s.sym = f.n_sym;
             s.unit = (int)f.units.size() - 1;
             s.clrs = jh.clrs;
-            for (int c = 0; c < jh.clrs; ++c) s.tab[c] = tab_offsets[jh.tab[c]];
+            for (int c = 0; c < jh.clrs; ++c) s.tab[c] = jh.tab[c];
             f.segs.push_back(s);
             f.n_sym += s.n_sym;
             pos = next;
@@ -864,6 +865,14 @@ struct DSeg {
     int32_t pad[2];
 };
 
+struct DSpec {        // a lossless-JPEG Huffman table to expand on the device
+    int32_t dst;      // element offset of its 1 + 2^max entries in the table array
+    int32_t max;
+    int32_t sym;      // offset of its symbols in the symbol array
+    int32_t nsym;
+    uint8_t count[20];
+};
+
 struct DFile {
     const uint8_t* bytes;
     uint16_t* out;
@@ -887,6 +896,36 @@ __device__ inline uint32_t bswap32(uint32_t x) {
 
 // ---------------------------------------------------------------------------------------------------
 // Output fill: every image starts at 0 (pixels no tile or strip covers stay 0).
+// make_decoder_ref's table (R:PefFile/DNGFile.cs:72-92): entry 0 = max; entries are filled in code order, each
+// code of length len covering 2^(max - len) entries; entries past the last code stay 0. One block per table.
+__global__ void k_build_tables(const DSpec* specs, const uint8_t* syms, uint16_t* tables) {
+    const DSpec& t = specs[blockIdx.x];
+    const int max = t.max;
+    uint16_t* out = tables + t.dst;
+    const uint32_t n = 1u << max;
+    if (threadIdx.x == 0) out[0] = (uint16_t)max;
+    for (uint32_t h = threadIdx.x + 1; h <= n; h += blockDim.x) {
+        // entries before length len: sum over shorter lengths of count * 2^(max - l)
+        uint32_t cum = 0, before = 0;
+        uint16_t e = 0;
+        for (int len = 1; len <= max; ++len) {
+            const uint32_t span = (uint32_t)t.count[len] << (max - len);
+            if (h <= cum + span) {
+                const uint32_t k = before + (h - cum - 1) / (1u << (max - len));
+                e = (uint16_t)(len << 8 | (k < (uint32_t)t.nsym ? syms[t.sym + k] : 0));
+                break;
+            }
+            cum += span;
+            before += t.count[len];
+        }
+        out[h] = e;
+    }
+}
+
+__global__ void k_zero_int(int* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 __global__ void k_fill_out(const DFile* files, int n, int* status) {
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) status[threadIdx.x] = 0;
     const DFile& f = files[blockIdx.y];
@@ -950,35 +989,71 @@ __global__ __launch_bounds__(kUnstuffThreads) void k_unstuff(const DSeg* segs, c
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Bit cursor over a stream: MSB-first, 64-bit window refilled with aligned 32-bit loads.
+// Bit cursor over a stream: MSB-first 64-bit window fed from 16-byte blocks that are loaded three blocks
+// (384 bits, some 40 codes) before they are needed, so the decode rarely waits on memory.
 struct Cursor {
-    const uint8_t* base;  // 4-byte aligned origin of the word grid
+    const uint8_t* base;  // 16-byte aligned origin of the block grid
     uint64_t avail;       // readable bytes from base (PEF: the file; JPEG: the padded copy)
     uint64_t start;       // bit offset of the stream's first bit from base
     int fill_ff;          // bytes past `avail` read as 0xFF (JPEG) or 0 with an EOF check (PEF)
     uint64_t buf;
     int nb;
-    uint64_t next_word;
-    uint32_t pos;         // bits consumed from the stream start
+    uint4 q0, q1, q2, q3;  // q0: block being spliced (word j next), q1..q3 loaded ahead
+    int j;
+    uint64_t next_block;
+    uint32_t pos;          // bits consumed from the stream start
 
-    __device__ uint32_t load(uint64_t k) const {
-        const uint64_t b = k * 4;
-        if (b + 4 <= avail) return bswap32(reinterpret_cast<const uint32_t*>(base)[k]);
+    __device__ uint32_t word_at(uint64_t b) const {
         uint32_t w = 0;
         for (int i = 0; i < 4; ++i) {
-            uint32_t v = b + i < avail ? base[b + i] : (fill_ff ? 0xFFu : 0u);
+            const uint32_t v = b + i < avail ? base[b + i] : (fill_ff ? 0xFFu : 0u);
             w = w << 8 | v;
+        }
+        return w;
+    }
+    __device__ uint4 block(uint64_t k) const {
+        const uint64_t b = k * 16;
+        uint4 r;
+        if (b + 16 <= avail) {
+            r = reinterpret_cast<const uint4*>(base)[k];
+            r.x = bswap32(r.x);
+            r.y = bswap32(r.y);
+            r.z = bswap32(r.z);
+            r.w = bswap32(r.w);
+        } else {
+            r.x = word_at(b);
+            r.y = word_at(b + 4);
+            r.z = word_at(b + 8);
+            r.w = word_at(b + 12);
+        }
+        return r;
+    }
+    __device__ uint32_t take() {
+        const uint32_t w = j == 0 ? q0.x : j == 1 ? q0.y : j == 2 ? q0.z : q0.w;
+        if (++j == 4) {
+            q0 = q1;
+            q1 = q2;
+            q2 = q3;
+            q3 = block(next_block++);
+            j = 0;
         }
         return w;
     }
     __device__ void seek(uint32_t p) {
         pos = p;
         const uint64_t a = start + p;
-        const uint64_t k = a >> 5;
+        const uint64_t k = a >> 7;
+        q0 = block(k);
+        q1 = block(k + 1);
+        q2 = block(k + 2);
+        q3 = block(k + 3);
+        next_block = k + 4;
+        j = (int)((a >> 5) & 3);
         const int sh = (int)(a & 31);
-        buf = ((uint64_t)load(k) << 32 | load(k + 1)) << sh;
+        const uint64_t hi = take();
+        const uint64_t lo = take();
+        buf = (hi << 32 | lo) << sh;
         nb = 64 - sh;
-        next_word = k + 2;
     }
     __device__ uint32_t peek(int n) const { return n ? (uint32_t)(buf >> (64 - n)) : 0u; }
     __device__ void skip(int n) {
@@ -986,7 +1061,7 @@ struct Cursor {
         nb -= n;
         pos += n;
         if (nb <= 32) {
-            buf |= (uint64_t)load(next_word++) << (32 - nb);
+            buf |= (uint64_t)take() << (32 - nb);
             nb += 32;
         }
     }
@@ -994,10 +1069,14 @@ struct Cursor {
     __device__ bool past_end(int n) const { return !fill_ff && start + pos + n > avail * 8; }
 };
 
-// One difference (ljpeg_diff, R:PefFile/PEFFile.cs:73-81, R:PefFile/DNGFile.cs:270-281). Returns false at the
-// end of a PEF file. stuck: the code consumed no bits (the reference then repeats this symbol forever).
-__device__ inline bool decode_diff(Cursor& c, const uint16_t* tab, int large, int32_t& diff, bool& stuck) {
-    const int m = tab[0];
+constexpr uint32_t kChunkBits = 4096;  // chunk of a stream decoded by one lane
+constexpr uint32_t kCheckBits = 512;   // distance of a chunk's checkpoint from its start
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// One difference (ljpeg_diff, R:PefFile/PEFFile.cs:73-81, R:PefFile/DNGFile.cs:270-281) with lookahead m and the
+// table at tab[1 + code]. Returns false at the end of a PEF file. stuck: the code consumed no bits (the
+// reference then repeats this symbol forever).
+__device__ inline bool decode_diff(Cursor& c, const uint16_t* tab, int m, int large, int32_t& diff, bool& stuck) {
     int len = 0, clen = 0;
     if (m > 0) {
         if (c.past_end(m)) return false;
@@ -1015,7 +1094,7 @@ __device__ inline bool decode_diff(Cursor& c, const uint16_t* tab, int large, in
         if (len == 0) d = 0;
     } else {
         if (c.past_end(len)) return false;
-        uint32_t v = c.peek(len);
+        const uint32_t v = c.peek(len);
         c.skip(len);
         xbits = len;
         d = (int32_t)v;
@@ -1026,66 +1105,38 @@ __device__ inline bool decode_diff(Cursor& c, const uint16_t* tab, int large, in
     return true;
 }
 
-constexpr int kSyncThreads = 1024;
-constexpr int kMinChunkBits = 2048;
-constexpr int kLdsTable = 16384;  // u16 table entries staged in LDS when they fit
-
-// Per-chunk state of the synchronising decode.
-struct ChunkLds {
-    uint32_t st[kSyncThreads];
-    uint32_t ex[4][kSyncThreads];
-    uint32_t cnt[4][kSyncThreads];
-    uint32_t jbase[kSyncThreads];
-    uint32_t nwrite[kSyncThreads];
-    uint8_t stuck[kSyncThreads];  // bit p: the decode from parity p met a code that consumes nothing
-    uint8_t dirty[kSyncThreads];
-    uint8_t jpar[kSyncThreads];
-    uint8_t used[kSyncThreads];
+// State of one chunk of a stream. A chunk is decoded from its start `st` once per possible component phase p
+// at that start (streams with C components decode component k mod C at symbol k); ex / cnt: where the decode
+// passes the chunk's end and how many codes it took; ckp / ckc: the first code boundary at least kCheckBits
+// past the start and the codes before it. A chunk decoded again from another start stops as soon as it lands
+// on that boundary in the same phase: from there on both decodes are the same.
+struct Chunk {
+    uint32_t st;
+    uint32_t ex[4], cnt[4], ckp[4], ckc[4];
+    uint32_t base, nwrite;
+    int32_t seg;
+    uint32_t dec_st;  // the start the stored decodes belong to
+    uint8_t stuck, dirty, par, used, decoded, valid, pad[2];  // valid: bit p = phase p decoded from dec_st
 };
 
-__global__ __launch_bounds__(kSyncThreads) void k_huff_sync(const DSeg* segs, const DFile* files, const uint8_t* ubuf,
-                                                           const uint64_t* ulen, const uint16_t* tables,
-                                                           int32_t* diffs, int* status, int max_iter) {
-    __shared__ ChunkLds L;
-    __shared__ uint16_t tlds[kLdsTable];
-    __shared__ int any, iters;
-    const DSeg& s = segs[blockIdx.x];
-    const int t = threadIdx.x;
-    const int C = s.clrs;
-    const uint32_t N = s.n_sym;
-    if (N == 0) return;
+struct StreamPlan {    // chunks [first, first + n) of segment seg
+    int32_t seg, first, n, chunk_bits;
+};
 
-    // tables: stage in LDS when they fit; component c's table at tb + toff[c]
-    const uint16_t* tb = tables;
-    int to0 = s.tab[0], to1 = s.tab[1], to2 = s.tab[2], to3 = s.tab[3];
-    {
-        const int l0 = 1 + (1 << tables[to0]);
-        const int l1 = C > 1 ? 1 + (1 << tables[to1]) : 0;
-        const int l2 = C > 2 ? 1 + (1 << tables[to2]) : 0;
-        const int l3 = C > 3 ? 1 + (1 << tables[to3]) : 0;
-        if (l0 + l1 + l2 + l3 <= kLdsTable) {
-            for (int i = t; i < l0; i += kSyncThreads) tlds[i] = tables[to0 + i];
-            for (int i = t; i < l1; i += kSyncThreads) tlds[l0 + i] = tables[to1 + i];
-            for (int i = t; i < l2; i += kSyncThreads) tlds[l0 + l1 + i] = tables[to2 + i];
-            for (int i = t; i < l3; i += kSyncThreads) tlds[l0 + l1 + l2 + i] = tables[to3 + i];
-            tb = tlds;
-            to0 = 0;
-            to1 = l0;
-            to2 = l0 + l1;
-            to3 = l0 + l1 + l2;
-        }
-    }
+// The stream's cursor: PEF streams read the file (16-byte aligned origin), lossless-JPEG streams their
+// unstuffed copy.
+__device__ inline Cursor stream_cursor(const DSeg& s, int si, const DFile* files, const uint8_t* ubuf,
+                                       const uint64_t* ulen, uint64_t& nbits) {
     Cursor cur;
-    uint64_t nbits;
     if (s.ljpeg) {
         cur.base = ubuf + s.ubuf;
         cur.avail = s.ucap;
         cur.start = 0;
         cur.fill_ff = 1;
-        nbits = ulen[blockIdx.x] * 8;
+        nbits = ulen[si] * 8;
     } else {
         const DFile& f = files[s.file];
-        const uint64_t a0 = s.off & ~(uint64_t)3;
+        const uint64_t a0 = s.off & ~(uint64_t)15;
         cur.base = f.bytes + a0;
         cur.avail = f.size - a0;
         cur.start = (s.off - a0) * 8;
@@ -1093,101 +1144,317 @@ __global__ __launch_bounds__(kSyncThreads) void k_huff_sync(const DSeg* segs, co
         nbits = s.len * 8;
     }
     if (nbits > 0xFFFFFFF0ull) nbits = 0xFFFFFFF0ull;
-    int chunks = (int)umin64(kSyncThreads, nbits / kMinChunkBits > 1 ? nbits / kMinChunkBits : 1);
-    const uint32_t S = (uint32_t)((nbits + chunks - 1) / chunks);
-    const int large = s.large;
+    return cur;
+}
 
-    if (t < chunks) {
-        L.st[t] = (uint32_t)umin64((uint64_t)t * S, nbits);
-        L.dirty[t] = 1;
+// Decodes every dirty chunk for each start phase, up to its end (or its old checkpoint). One work-group per range
+// of at most 256 chunks of one stream; the stream's tables are read from LDS (kLds) so that a table lookup never
+// waits behind the stream blocks loaded ahead (vector loads complete in order).
+struct BlockPlan {
+    int32_t stream, first, n, pad;
+};
+
+constexpr int kLdsTable = 12288;  // u16 table entries a work-group stages
+
+template <bool kLds>
+__device__ inline const uint16_t* stage_tables(const DSeg& s, const uint16_t* tables, uint16_t* tlds, int (&to)[4]) {
+    if constexpr (!kLds) {
+        for (int c = 0; c < 4; ++c) to[c] = s.tab[c];
+        return tables;
+    } else {
+        int o = 0;
+        for (int c = 0; c < 4; ++c) {
+            to[c] = o;
+            if (c >= s.clrs) continue;
+            const int len = 1 + (1 << __ldg(tables + s.tab[c]));
+            for (int i = threadIdx.x; i < len; i += blockDim.x) tlds[o + i] = tables[s.tab[c] + i];
+            o += len;
+        }
+        __syncthreads();
+        return tlds;
     }
-    if (t == 0) iters = 0;
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_chunk_decode(Chunk* chunks, const BlockPlan* blocks, const StreamPlan* streams,
+                                                      const DSeg* segs, const DFile* files, const uint8_t* ubuf,
+                                                      const uint64_t* ulen, const uint16_t* tables, int* redecodes) {
+    __shared__ uint16_t tlds[kLds ? kLdsTable : 1];
+    const BlockPlan& bp = blocks[blockIdx.x];
+    __shared__ int any_dirty;
+    if (threadIdx.x == 0) any_dirty = 0;
     __syncthreads();
-
-    // Phase 1: decode every dirty chunk from its start for each component phase, up to its end.
-    for (;;) {
-        if (t < chunks && L.dirty[t]) {
-            const uint32_t end = t == chunks - 1 ? (uint32_t)nbits : (uint32_t)umin64((uint64_t)(t + 1) * S, nbits);
-            uint8_t stuck_bits = 0;
-            for (int p = 0; p < C; ++p) {
-                cur.seek(L.st[t]);
-                uint32_t n = 0;
-                int q = p;
-                bool stk = false;
-                while (cur.pos < end) {
-                    int32_t d;
-                    if (!decode_diff(cur, tb + (q == 0 ? to0 : q == 1 ? to1 : q == 2 ? to2 : to3), large, d, stk)) break;
-                    if (stk) break;
-                    ++n;
-                    if (++q == C) q = 0;
-                }
-                L.ex[p][t] = cur.pos;
-                L.cnt[p][t] = n;
-                if (stk) stuck_bits |= 1 << p;
-            }
-            L.stuck[t] = stuck_bits;
-            L.dirty[t] = 0;
-        }
-        __syncthreads();
-        // Walk the chain from the stream start: chunk j must start where chunk j-1 ends (for the phase
-        // it is entered with); a mismatch re-decodes chunk j. The walk stops where the stream's N
-        // differences are complete.
-        if (t == 0) {
-            int a = 0;
-            uint32_t pos = 0, base = 0;
-            int par = 0;
-            bool done = false;
-            for (int j = 0; j < chunks; ++j) {
-                L.used[j] = 0;
-                if (done) continue;
-                if (L.st[j] != pos) {
-                    L.st[j] = pos;
-                    L.dirty[j] = 1;
-                    a = 1;
-                }
-                L.used[j] = 1;
-                L.jbase[j] = base;
-                L.jpar[j] = (uint8_t)par;
-                const uint32_t n = L.cnt[par][j];
-                if ((L.stuck[j] >> par & 1) || j == chunks - 1 || base + n >= N) {
-                    L.nwrite[j] = N - base;
-                    done = true;
-                    continue;
-                }
-                L.nwrite[j] = n;
-                base += n;
-                pos = L.ex[par][j];
-                par = (int)((par + n) % (uint32_t)C);
-            }
-            ++iters;
-            any = a && iters < max_iter;
-            if (a && iters >= max_iter) raise_error(status, E_SYNC, s.file);
-        }
-        __syncthreads();
-        if (!any) break;
+    if ((int)threadIdx.x < bp.n && chunks[bp.first + threadIdx.x].dirty) any_dirty = 1;
+    __syncthreads();
+    if (!any_dirty) return;
+    const StreamPlan& sp = streams[bp.stream];
+    const DSeg& s = segs[sp.seg];
+    int to[4];
+    const uint16_t* tb = stage_tables<kLds>(s, tables, tlds, to);
+    if ((int)threadIdx.x >= bp.n) return;
+    const int g = bp.first + threadIdx.x;
+    Chunk& k = chunks[g];
+    if (!k.dirty) return;
+    if (k.decoded) atomicAdd(redecodes, 1);
+    uint64_t nbits;
+    Cursor cur = stream_cursor(s, sp.seg, files, ubuf, ulen, nbits);
+    const int j = g - sp.first;
+    const int C = s.clrs;
+    const uint32_t cb = (uint32_t)sp.chunk_bits;
+    const uint32_t end = j == sp.n - 1 ? (uint32_t)nbits : (uint32_t)umin64((uint64_t)(j + 1) * cb, nbits);
+    const uint32_t st = (uint32_t)umin64(k.st, nbits);
+    const int large = s.large;
+    const int m0 = tb[to[0]], m1 = C > 1 ? tb[to[1]] : 0, m2 = C > 2 ? tb[to[2]] : 0, m3 = C > 3 ? tb[to[3]] : 0;
+    const int to0 = to[0], to1 = to[1], to2 = to[2], to3 = to[3];
+    // Snapshot of the previous decodes (one per entry phase): a new decode that lands on one of their
+    // checkpoints in the same component phase follows that decode from there on.
+    const bool had = k.decoded;
+    uint32_t o_ex[4], o_cnt[4], o_ckp[4], o_ckc[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const bool live = had && p < C;
+        o_ex[p] = live ? k.ex[p] : 0;
+        o_cnt[p] = live ? k.cnt[p] : 0;
+        o_ckp[p] = live ? k.ckp[p] : kNone;
+        o_ckc[p] = live ? k.ckc[p] : 0;
     }
-    if (status[0] == E_SYNC) return;
-
-    // Phase 2: every used chunk writes its differences.
-    if (t < chunks && L.used[t]) {
-        cur.seek(L.st[t]);
-        int q = L.jpar[t];
-        const uint32_t n = L.nwrite[t];
-        int32_t* out = diffs + s.diff + L.jbase[t];
-        int32_t d = 0;
+    const uint8_t o_stuck = k.stuck;
+    uint8_t stuck_bits = 0;
+    for (int p = 0; p < C; ++p) {
+        cur.seek(st);
+        uint32_t n = 0, ckp = kNone, ckc = 0;
+        uint32_t lo = umin64(umin64(o_ckp[0], o_ckp[1]), umin64(o_ckp[2], o_ckp[3]));
+        int q = p;
+        int from = -1;  // the old phase this decode joined
         bool stk = false;
-        uint32_t k = 0;
-        for (; k < n; ++k) {
-            if (!decode_diff(cur, tb + (q == 0 ? to0 : q == 1 ? to1 : q == 2 ? to2 : to3), large, d, stk)) {
-                raise_error(status, E_EOF, s.file);
-                return;
+        while (cur.pos < end) {
+            if (cur.pos >= lo) {
+                uint32_t nlo = kNone;
+#pragma unroll
+                for (int pp = 0; pp < 4; ++pp) {
+                    if (from < 0 && o_ckp[pp] == cur.pos && (p + n) % (uint32_t)C == (pp + o_ckc[pp]) % (uint32_t)C)
+                        from = pp;
+                    if (o_ckp[pp] > cur.pos && o_ckp[pp] < nlo) nlo = o_ckp[pp];
+                }
+                if (from >= 0) break;
+                lo = nlo;
             }
-            out[k] = d;
+            if (ckp == kNone && cur.pos >= st + kCheckBits) {
+                ckp = cur.pos;
+                ckc = n;
+            }
+            const int tq = q == 0 ? to0 : q == 1 ? to1 : q == 2 ? to2 : to3;
+            const int m = q == 0 ? m0 : q == 1 ? m1 : q == 2 ? m2 : m3;
+            int32_t d;
+            if (!decode_diff(cur, tb + tq, m, large, d, stk)) break;
             if (stk) break;
+            ++n;
             if (++q == C) q = 0;
         }
-        for (++k; k < n && stk; ++k) out[k] = d;  // the reference repeats a code that consumes nothing
+        if (from >= 0) {  // the rest is the old decode's (stuck bit included)
+            const uint32_t fe = from == 0 ? o_ex[0] : from == 1 ? o_ex[1] : from == 2 ? o_ex[2] : o_ex[3];
+            const uint32_t fc = from == 0 ? o_cnt[0] : from == 1 ? o_cnt[1] : from == 2 ? o_cnt[2] : o_cnt[3];
+            const uint32_t fk = from == 0 ? o_ckc[0] : from == 1 ? o_ckc[1] : from == 2 ? o_ckc[2] : o_ckc[3];
+            k.ex[p] = fe;
+            k.cnt[p] = n + (fc - fk);
+            k.ckp[p] = cur.pos;
+            k.ckc[p] = n;
+            if (o_stuck >> from & 1) stuck_bits |= 1 << p;
+        } else {
+            k.ex[p] = cur.pos;
+            k.cnt[p] = n;
+            k.ckp[p] = ckp;
+            k.ckc[p] = ckc;
+            if (stk) stuck_bits |= 1 << p;
+        }
     }
+    k.stuck = stuck_bits;
+    k.valid = (uint8_t)((1 << C) - 1);
+    k.dec_st = k.st;
+    k.decoded = 1;
+    k.dirty = 0;
+}
+
+// Chain walk of a stream with C > 1 components (one thread per stream): chunk j must start where chunk j - 1
+// ends for the phase it is entered with; a mismatch marks chunk j dirty with the new start. Also the base,
+// entry phase and number of codes each chunk writes; the walk stops where the stream's N codes are complete.
+__global__ void k_walk_serial(Chunk* chunks, const StreamPlan* streams, const int* list, int n_list,
+                              const DSeg* segs, int* any_dirty) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_list) return;
+    const StreamPlan& sp = streams[list[g]];
+    const DSeg& s = segs[sp.seg];
+    const int C = s.clrs;
+    const uint32_t N = s.n_sym;
+    uint32_t pos = 0, base = 0;
+    int par = 0;
+    bool done = false, a = false;
+    for (int j = 0; j < sp.n; ++j) {
+        Chunk& k = chunks[sp.first + j];
+        k.used = 0;
+        if (done) continue;
+        if (k.st != pos) {
+            k.st = pos;
+            k.dirty = 1;
+            a = true;
+        } else if (k.dec_st != k.st || !(k.valid >> par & 1)) {  // this phase is not decoded from here yet
+            k.dirty = 1;
+            a = true;
+        }
+        k.used = 1;
+        k.base = base;
+        k.par = (uint8_t)par;
+        const uint32_t n = k.cnt[par];
+        if ((k.stuck >> par & 1) || j == sp.n - 1 || base + n >= N) {
+            k.nwrite = N - base;
+            done = true;
+            continue;
+        }
+        k.nwrite = n;
+        base += n;
+        pos = k.ex[par];
+        par = (int)((par + n) % (uint32_t)C);
+    }
+    if (a) atomicAdd(any_dirty, 1);
+}
+
+// Chain check of a single-table stream, one thread per chunk: chunk j starts where chunk j - 1 ends.
+__global__ __launch_bounds__(256) void k_walk_parallel(Chunk* chunks, const int* par_chunks, int n,
+                                                       const int* chunk_stream, const StreamPlan* streams,
+                                                       int* any_dirty) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = par_chunks[i];
+    const StreamPlan& sp = streams[chunk_stream[g]];
+    const uint32_t want = g == sp.first ? 0u : chunks[g - 1].ex[0];
+    Chunk& k = chunks[g];
+    if (k.st != want) {
+        k.st = want;
+        k.dirty = 1;
+        atomicAdd(any_dirty, 1);
+    }
+}
+
+// Bases of a single-table stream once its chunks agree: an exclusive scan of the counts; the chunk that
+// completes the N codes (or meets a code that consumes nothing, or is the last) writes the remainder.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_scan_parallel(Chunk* chunks, const StreamPlan* streams,
+                                                                const int* list, const DSeg* segs) {
+    const StreamPlan& sp = streams[list[blockIdx.x]];
+    const uint32_t N = segs[sp.seg].n_sym;
+    const int t = threadIdx.x;
+    __shared__ uint64_t ssum[kScanThreads];
+    __shared__ int first_end;
+    const int per = (sp.n + kScanThreads - 1) / kScanThreads;
+    const int j0 = t * per, j1 = imin(sp.n, j0 + per);
+    uint64_t acc = 0;
+    for (int j = j0; j < j1; ++j) acc += chunks[sp.first + j].cnt[0];
+    ssum[t] = acc;
+    if (t == 0) first_end = sp.n - 1;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1) {
+        const uint64_t v = t >= o ? ssum[t - o] : 0;
+        __syncthreads();
+        ssum[t] += v;
+        __syncthreads();
+    }
+    uint64_t base = t > 0 ? ssum[t - 1] : 0;
+    for (int j = j0; j < j1; ++j) {
+        Chunk& k = chunks[sp.first + j];
+        if ((k.stuck & 1) || base + k.cnt[0] >= N) atomicMin(&first_end, j);
+        k.base = (uint32_t)umin64(base, N);
+        base += k.cnt[0];
+    }
+    __syncthreads();
+    const int e = first_end;
+    for (int j = j0; j < j1; ++j) {
+        Chunk& k = chunks[sp.first + j];
+        k.par = 0;
+        k.used = j <= e;
+        k.nwrite = j < e ? k.cnt[0] : j == e ? N - k.base : 0;
+    }
+}
+
+// Every used chunk writes its codes' differences from its true start and entry phase.
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_chunk_write(const Chunk* chunks, const BlockPlan* blocks,
+                                                     const StreamPlan* streams, const DSeg* segs, const DFile* files,
+                                                     const uint8_t* ubuf, const uint64_t* ulen,
+                                                     const uint16_t* tables, uint16_t* diffs, int* status) {
+    __shared__ uint16_t tlds[kLds ? kLdsTable : 1];
+    const BlockPlan& bp = blocks[blockIdx.x];
+    const StreamPlan& sp = streams[bp.stream];
+    const DSeg& s = segs[sp.seg];
+    int to[4];
+    const uint16_t* tb = stage_tables<kLds>(s, tables, tlds, to);
+    if ((int)threadIdx.x >= bp.n) return;
+    const Chunk& k = chunks[bp.first + threadIdx.x];
+    if (!k.used || k.nwrite == 0) return;
+    uint64_t nbits;
+    Cursor cur = stream_cursor(s, sp.seg, files, ubuf, ulen, nbits);
+    const int C = s.clrs;
+    const int large = s.large;
+    cur.seek(k.st);
+    int q = k.par;
+    const uint32_t n = k.nwrite;
+    // Differences are kept modulo 2^16: every later step (running sums, predictors on stored samples) reduces
+    // modulo 2^16 before a value is used. Eight at a time go out as one 16-byte store.
+    const uint64_t o0 = s.diff + k.base;
+    uint16_t* out = diffs + o0;
+    const uint32_t head = (uint32_t)((8 - (o0 & 7)) & 7);
+    uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    int32_t d = 0;
+    bool stk = false;
+    uint32_t i = 0;
+    auto put = [&](uint32_t idx, uint16_t v) {
+        if (idx < head || idx + (8 - ((idx - head) & 7)) > n) {
+            out[idx] = v;  // unaligned head and the tail that does not fill a 16-byte group
+            return;
+        }
+        const uint32_t w = (idx - head) & 7;
+        const uint32_t hi = (uint32_t)v << 16;
+        switch (w) {
+            case 0: p0 = v; break;
+            case 1: p0 |= hi; break;
+            case 2: p1 = v; break;
+            case 3: p1 |= hi; break;
+            case 4: p2 = v; break;
+            case 5: p2 |= hi; break;
+            case 6: p3 = v; break;
+            default: *reinterpret_cast<uint4*>(out + idx - 7) = make_uint4(p0, p1, p2, p3 | hi); break;
+        }
+    };
+    for (; i < n; ++i) {
+        const int tq = q == 0 ? to[0] : q == 1 ? to[1] : q == 2 ? to[2] : to[3];
+        if (!decode_diff(cur, tb + tq, tb[tq], large, d, stk)) {
+            raise_error(status, E_EOF, s.file);
+            return;
+        }
+        put(i, (uint16_t)d);
+        if (stk) break;
+        if (++q == C) q = 0;
+    }
+    for (++i; i < n && stk; ++i) put(i, (uint16_t)d);  // the reference repeats a code that consumes nothing
+}
+
+// Chunk set-up: guessed starts every chunk_bits bits, everything dirty.
+__global__ __launch_bounds__(256) void k_chunk_init(Chunk* chunks, int n_chunks, const StreamPlan* streams,
+                                                    const int* chunk_stream) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_chunks) return;
+    const StreamPlan& sp = streams[chunk_stream[g]];
+    Chunk& k = chunks[g];
+    k.st = (uint32_t)(g - sp.first) * (uint32_t)sp.chunk_bits;
+    k.seg = sp.seg;
+    k.stuck = 0;
+    k.dirty = 1;
+    k.par = 0;
+    k.used = 0;
+    k.decoded = 0;
+    k.valid = 0;
+    k.dec_st = 0;
+    k.base = 0;
+    k.nwrite = 0;
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -1198,7 +1465,7 @@ __global__ __launch_bounds__(kSyncThreads) void k_huff_sync(const DSeg* segs, co
 constexpr int kChainThreads = 1024;
 
 __global__ __launch_bounds__(kChainThreads) void k_row_starts(const DUnit* units, const int* lin_units,
-                                                             const int32_t* diffs, int32_t* vchain) {
+                                                             const uint16_t* diffs, int32_t* vchain) {
     const DUnit& u = units[lin_units[blockIdx.x]];
     const int t = threadIdx.x;
     const int C = u.clrs;
@@ -1264,66 +1531,63 @@ __device__ inline uint16_t level_transform(const DUnit& u, const uint16_t* lin, 
     return (uint16_t)(o > 0 ? o : 0);
 }
 
-constexpr int kRowThreads = 256;
+constexpr int kRowWaves = 4;  // rows per 256-thread block, one wave per row
 
-// One row of a linear unit per block: each thread a contiguous run of the row.
-__global__ __launch_bounds__(kRowThreads) void k_rows(const DUnit* units, const int* row_unit, const int* row_index,
-                                                      const int32_t* diffs, const int32_t* vchain, const DFile* files,
-                                                      const uint16_t* lin, const uint16_t* bh, const uint16_t* bv,
-                                                      int* status) {
-    const DUnit& u = units[row_unit[blockIdx.x]];
-    const int r = row_index[blockIdx.x];
-    const int t = threadIdx.x;
+__device__ inline void row_scan(const DUnit& u, int r, int lane, const uint16_t* diffs, const int32_t* vchain,
+                                const DFile* files, const uint16_t* lin, const uint16_t* bh, const uint16_t* bv,
+                                int* status);
+
+// One row of a linear unit per wave. The row is read in rounds of R = 64 - 64 mod C samples, lane l taking
+// sample k R + l, whose column phase l mod C is then the same in every round; a strided scan over the lanes of
+// each phase and a carry per phase give the running sums. Sample x < C starts its phase at the row start value.
+__global__ __launch_bounds__(64 * kRowWaves) void k_rows(const DUnit* units, const int* row_unit,
+                                                        const int* row_index, int n_rows, const uint16_t* diffs,
+                                                        const int32_t* vchain, const DFile* files,
+                                                        const uint16_t* lin, const uint16_t* bh, const uint16_t* bv,
+                                                        int* status) {
+    const int lane = threadIdx.x & 63;
+    for (int g = blockIdx.x * kRowWaves + (threadIdx.x >> 6); g < n_rows; g += gridDim.x * kRowWaves)
+        row_scan(units[row_unit[g]], row_index[g], lane, diffs, vchain, files, lin, bh, bv, status);
+}
+
+__device__ inline void row_scan(const DUnit& u, int r, int lane, const uint16_t* diffs, const int32_t* vchain,
+                                const DFile* files, const uint16_t* lin, const uint16_t* bh, const uint16_t* bv,
+                                int* status) {
     const int C = u.clrs;
     const int n = u.cols;
-    const int per = (n + kRowThreads - 1) / kRowThreads;
-    const int x0 = t * per, x1 = imin(n, x0 + per);
-    const int32_t* d = diffs + u.sym + (uint64_t)r * n;
-    __shared__ int32_t sc[4][kRowThreads];
-    int32_t part[4] = {0, 0, 0, 0};
-    for (int x = imax(x0, C); x < x1; ++x) {
-        const int c = x % C;
-        part[c] = (int32_t)((uint32_t)part[c] + (uint32_t)d[x]);
-    }
-    for (int c = 0; c < C; ++c) sc[c][t] = part[c];
-    __syncthreads();
-    for (int o = 1; o < kRowThreads; o <<= 1) {
-        int32_t v[4];
-        for (int c = 0; c < C; ++c) v[c] = t >= o ? sc[c][t - o] : 0;
-        __syncthreads();
-        for (int c = 0; c < C; ++c) sc[c][t] = (int32_t)((uint32_t)sc[c][t] + (uint32_t)v[c]);
-        __syncthreads();
-    }
-    int32_t run[4];
-    for (int c = 0; c < C; ++c) {
-        const int32_t ex = t > 0 ? sc[c][t - 1] : 0;
-        run[c] = (int32_t)((uint32_t)vchain[u.vchain + (uint64_t)r * C + c] + (uint32_t)ex);
-    }
+    const int R = 64 - 64 % C;
+    const int c = lane % C;
+    const uint16_t* d = diffs + u.sym + (uint64_t)r * n;
     const DFile& f = files[u.file];
     const int y = u.r0 + r;
     uint16_t* orow = reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(f.out) + (size_t)y * f.pitch);
-    for (int x = x0; x < x1; ++x) {
-        const int c = x % C;
-        int32_t v;
-        if (x < C) {
-            v = run[c];
-        } else {
-            run[c] = (int32_t)((uint32_t)run[c] + (uint32_t)d[x]);
-            v = run[c];
+    const bool lj = u.kind == U_LJPEG;
+    const int32_t start = lane < C && lane < n ? vchain[u.vchain + (uint64_t)r * C + lane] : 0;
+    uint32_t carry = 0;
+    bool bad = false;
+    for (int x0 = 0; x0 < n; x0 += R) {
+        const int x = x0 + lane;
+        const bool in = lane < R && x < n;
+        uint32_t v = !in ? 0u : x < C ? (uint32_t)start : (uint32_t)d[x];
+        for (int o = C; o < R; o <<= 1) {
+            const uint32_t w = __shfl_up(v, o, 64);
+            if (lane >= o) v += w;
         }
-        const uint16_t v16 = (uint16_t)v;
-        if (u.kind == U_LJPEG && (v16 >> (u.bits & 31)) != 0) {
-            raise_error(status, E_RANGE, u.file);
-            return;
+        v += carry;
+        carry = __shfl(v, R - C + c, 64);
+        if (in) {
+            const uint16_t v16 = (uint16_t)v;
+            if (lj && (v16 >> (u.bits & 31)) != 0) bad = true;
+            const int xo = u.c0 + x;
+            if (xo < f.width && y < f.height)
+                orow[xo] = u.transform ? level_transform(u, lin, bh, bv, v16, xo, y) : v16;
         }
-        const int xo = u.c0 + x;
-        if (xo < f.width && y < f.height)
-            orow[xo] = u.transform ? level_transform(u, lin, bh, bv, v16, xo, y) : v16;
     }
+    if (bad) raise_error(status, E_RANGE, u.file);
 }
 
 // Lossless-JPEG predictors 2..7 (ljpeg_row, R:PefFile/DNGFile.cs:199-222): one lane per (unit, component).
-__global__ void k_rows_nonlinear(const DUnit* units, const int* nl_units, int n_nl, const int32_t* diffs,
+__global__ void k_rows_nonlinear(const DUnit* units, const int* nl_units, int n_nl, const uint16_t* diffs,
                                  uint16_t* vals, const DFile* files, const uint16_t* lin, const uint16_t* bh,
                                  const uint16_t* bv, int* status) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1411,12 +1675,29 @@ __global__ void k_uncompressed(const DUnit* units, const int* un_units, const ui
 // =====================================================================================================
 
 struct Layout {
-    size_t files, units, segs, tables, lin, bh, bv, lists, status, ulen, ubuf, diffs, vchain, vals, total;
+    size_t files, units, segs, tables, lin, bh, bv, lists, status, ulen, ubuf, diffs, vchain, vals;
+    size_t chunks, chunk_stream, streams, walk_lists, dirty, total;
     size_t n_units = 0, n_segs = 0, n_tables = 0, n_lin = 0, n_bh = 0, n_bv = 0, n_lists = 0;
-    size_t ubuf_bytes = 0, n_diffs = 0, n_vchain = 0, n_vals = 0;
+    size_t ubuf_bytes = 0, n_diffs = 0, n_vchain = 0, n_vals = 0, n_chunks = 0, n_specs = 0, n_spec_syms = 0;
+    size_t specs = 0, spec_syms = 0, blocks = 0;
 };
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Chunking of a stream: single-table streams in chunks of kChunkBits (chained by a parallel check and scan);
+// streams that alternate tables in at most 2048 chunks (chained by one thread). The stream's bits are bounded
+// by its stored bytes (a lossless-JPEG stream loses its stuffing bytes on the device).
+void chunk_plan(const Seg& s, int& chunk_bits, int& n_chunks) {
+    const uint64_t bits = std::max<uint64_t>(1, (uint64_t)s.len * 8);
+    uint64_t cb = kChunkBits;
+    if (bits <= 256ull * kChunkBits)
+        cb = std::max<uint64_t>(2048, (bits + 255) / 256);  // a small stream fills one work-group
+    else if (s.clrs > 1)
+        cb = std::max<uint64_t>(cb, (bits + 2047) / 2048);
+    cb = std::min<uint64_t>(cb, 1u << 30);
+    chunk_bits = (int)cb;
+    n_chunks = (int)std::max<uint64_t>(1, (bits + cb - 1) / cb);
+}
 
 Layout layout_of(const isa_raw_file* const* fs, int n) {
     Layout L{};
@@ -1426,12 +1707,21 @@ Layout layout_of(const isa_raw_file* const* fs, int n) {
         L.n_units += f.units.size();
         L.n_segs += f.segs.size();
         L.n_tables += f.tables.size();
+        for (const TableSpec& t : f.ljtabs) {
+            L.n_tables += 1 + ((size_t)1 << t.max);
+            L.n_specs += 1;
+            L.n_spec_syms += t.sym.size();
+        }
         L.n_lin += f.lin.size();
         L.n_bh += f.bh.size();
         L.n_bv += f.bv.size();
         L.n_diffs += f.n_sym;
-        for (const Seg& s : f.segs)
+        for (const Seg& s : f.segs) {
             if (f.units[s.unit].kind == U_LJPEG) L.ubuf_bytes += align_up(s.len + 16, 16);
+            int cb, nc;
+            chunk_plan(s, cb, nc);
+            L.n_chunks += nc;
+        }
         for (const Unit& u : f.units) {
             if (u.kind == U_PEF_HUFF || (u.kind == U_LJPEG && u.psv == 1)) {
                 L.n_vchain += (size_t)u.rows * u.clrs;
@@ -1441,7 +1731,7 @@ Layout layout_of(const isa_raw_file* const* fs, int n) {
         }
     }
     // index lists: ljpeg segments, linear units, row->unit and row->index, nonlinear units, uncompressed units
-    L.n_lists = L.n_segs + L.n_units * 3 + 2 * rows + 2;
+    L.n_lists = 2 * L.n_segs + L.n_units * 3 + 2 * rows + 2;
     size_t o = 0;
     auto take = [&](size_t bytes) {
         size_t at = o;
@@ -1459,12 +1749,23 @@ Layout layout_of(const isa_raw_file* const* fs, int n) {
     L.status = take(16);
     L.ulen = take(8 * L.n_segs + 8);
     L.ubuf = take(L.ubuf_bytes + 16);
-    L.diffs = take(4 * L.n_diffs + 16);
+    L.diffs = take(2 * L.n_diffs + 64);
     L.vchain = take(4 * L.n_vchain + 16);
+    L.chunks = take(sizeof(Chunk) * L.n_chunks + 16);
+    L.chunk_stream = take(4 * L.n_chunks + 16);
+    L.streams = take(sizeof(StreamPlan) * L.n_segs + 16);
+    L.walk_lists = take(4 * (2 * L.n_segs + L.n_chunks) + 64);
+    L.blocks = take(sizeof(BlockPlan) * (L.n_chunks + L.n_segs) + 16);
+    L.dirty = take(16);
+    L.specs = take(sizeof(DSpec) * L.n_specs + 16);
+    L.spec_syms = take(L.n_spec_syms + 16);
     L.vals = take(2 * L.n_vals + 16);
     L.total = o;
     return L;
 }
+
+thread_local int g_stats[2] = {0, 0};  // the last decode on this thread: chain walks, chunk re-decodes
+thread_local int g_walks = 0;
 
 int open_file(const void* file, size_t size, int format, isa_raw_file** out) {
     ISA_REQUIRE(file && out, "isa_raw_open: null argument");
@@ -1530,6 +1831,13 @@ int isa_raw_get_info(const isa_raw_file* f, isa_raw_info* info) {
 
 void isa_raw_close(isa_raw_file* f) { delete f; }
 
+int isa_raw_decode_stats(int* max_sync_walks, int* chunk_redecodes) {
+    ISA_REQUIRE(max_sync_walks && chunk_redecodes, "isa_raw_decode_stats: null argument");
+    *max_sync_walks = g_stats[0];
+    *chunk_redecodes = g_stats[1];
+    return ISA_OK;
+}
+
 int isa_raw_decode_workspace(const isa_raw_file* const* files, int n, size_t* bytes) {
     ISA_REQUIRE(files && bytes && n > 0, "isa_raw_decode_workspace: bad arguments");
     for (int i = 0; i < n; ++i) ISA_REQUIRE(files[i], "isa_raw_decode_workspace: file %d is null", i);
@@ -1544,7 +1852,7 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
         ISA_REQUIRE(files[i] && file_dev[i] && raw_dev[i], "isa_raw_decode: file %d has a null pointer", i);
         ISA_REQUIRE(raw_pitch[i] >= 2 * files[i]->info.width && raw_pitch[i] % 2 == 0,
                     "isa_raw_decode: pitch of file %d", i);
-        ISA_REQUIRE(reinterpret_cast<uintptr_t>(file_dev[i]) % 4 == 0, "isa_raw_decode: file %d not 4-byte aligned", i);
+        ISA_REQUIRE(reinterpret_cast<uintptr_t>(file_dev[i]) % 16 == 0, "isa_raw_decode: file %d not 16-byte aligned", i);
     }
     const Layout L = layout_of(files, n);
     ISA_REQUIRE(workspace_dev && workspace_bytes >= L.total, "isa_raw_decode: workspace %zu < %zu bytes",
@@ -1558,14 +1866,35 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
     std::vector<DSeg> hs;
     std::vector<uint16_t> ht, hl, hbh, hbv;
     std::vector<int> lj_segs, lin_units, row_unit, row_index, nl_units, un_units;
+    std::vector<StreamPlan> hsp;
+    std::vector<int> hcs, serial_streams, par_chunks, scan_streams;
+    std::vector<BlockPlan> blk_lds, blk_glb;
     std::vector<uint64_t> un_first;
+    std::vector<DSpec> hspec;
+    std::vector<uint8_t> hsyms;
     uint64_t sym_base = 0, vchain_base = 0, vals_base = 0, ubuf_base = 0, un_total = 0;
     int max_rows = 0;
+    // table array: the host-built (Pentax) tables of every file first, then the expanded lossless-JPEG tables
+    size_t host_tab = 0;
+    for (int i = 0; i < n; ++i) host_tab += files[i]->tables.size();
+    size_t lj_tab = host_tab;
     for (int i = 0; i < n; ++i) {
         const isa_raw_file& f = *files[i];
         hf[i] = DFile{static_cast<const uint8_t*>(file_dev[i]), static_cast<uint16_t*>(raw_dev[i]), f.size,
                       raw_pitch[i], f.info.width, f.info.height, 0};
         const int unit0 = (int)hu.size(), tab0 = (int)ht.size();
+        const int spec0 = (int)hspec.size();
+        for (const TableSpec& t : f.ljtabs) {
+            DSpec d{};
+            d.dst = (int32_t)lj_tab;
+            d.max = t.max;
+            d.sym = (int32_t)hsyms.size();
+            d.nsym = (int32_t)t.sym.size();
+            for (int l = 0; l <= 16; ++l) d.count[l] = t.count[l];
+            hsyms.insert(hsyms.end(), t.sym.begin(), t.sym.end());
+            hspec.push_back(d);
+            lj_tab += 1 + ((size_t)1 << t.max);
+        }
         const int lin0 = (int)hl.size(), bh0 = (int)hbh.size(), bv0 = (int)hbv.size();
         for (const Unit& u : f.units) {
             DUnit d{};
@@ -1621,7 +1950,28 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
             d.clrs = sg.clrs;
             d.large = f.info.format == ISA_RAW_DNG && f.large ? 1 : 0;
             d.ljpeg = f.units[sg.unit].kind == U_LJPEG ? 1 : 0;
-            for (int c = 0; c < 4; ++c) d.tab[c] = tab0 + sg.tab[c];
+            for (int c = 0; c < 4; ++c)
+                d.tab[c] = d.ljpeg ? (c < sg.clrs ? hspec[spec0 + sg.tab[c]].dst : 0) : tab0 + sg.tab[c];
+            {
+                StreamPlan sp{};
+                sp.seg = (int)hs.size();
+                sp.first = (int)hcs.size();
+                chunk_plan(sg, sp.chunk_bits, sp.n);
+                const int si = (int)hsp.size();
+                for (int c = 0; c < sp.n; ++c) {
+                    if (sg.clrs == 1) par_chunks.push_back((int)hcs.size());
+                    hcs.push_back(si);
+                }
+                (sg.clrs == 1 ? scan_streams : serial_streams).push_back(si);
+                hsp.push_back(sp);
+                int entries = 0;
+                for (int c = 0; c < sg.clrs; ++c)
+                    entries += f.units[sg.unit].kind == U_LJPEG ? 1 + (1 << f.ljtabs.at(sg.tab[c]).max)
+                                                               : (int)f.tables.size();
+                for (int b0 = 0; b0 < sp.n; b0 += 256)
+                    (entries <= kLdsTable ? blk_lds : blk_glb)
+                        .push_back(BlockPlan{si, sp.first + b0, std::min(256, sp.n - b0), 0});
+            }
             if (d.ljpeg) {
                 d.ubuf = ubuf_base;
                 d.ucap = align_up(sg.len + 16, 16);
@@ -1645,6 +1995,11 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
     };
     const size_t o_lj = put_list(lj_segs), o_lin = put_list(lin_units), o_ru = put_list(row_unit),
                  o_ri = put_list(row_index), o_nl = put_list(nl_units), o_un = put_list(un_units);
+    std::vector<int> walk;
+    walk.insert(walk.end(), serial_streams.begin(), serial_streams.end());
+    walk.insert(walk.end(), scan_streams.begin(), scan_streams.end());
+    walk.insert(walk.end(), par_chunks.begin(), par_chunks.end());
+    ISA_REQUIRE(hcs.size() == L.n_chunks && hsp.size() == L.n_segs, "isa_raw_decode: internal chunk plan");
     // un_first as a separate u64 array after the int lists
     std::vector<uint8_t> list_bytes(lists.size() * 4 + 8 + un_first.size() * 8);
     memcpy(list_bytes.data(), lists.data(), lists.size() * 4);
@@ -1661,10 +2016,18 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
     ISA_TRY(up(L.units, hu.data(), sizeof(DUnit) * hu.size()));
     ISA_TRY(up(L.segs, hs.data(), sizeof(DSeg) * hs.size()));
     ISA_TRY(up(L.tables, ht.data(), 2 * ht.size()));
+    ISA_TRY(up(L.specs, hspec.data(), sizeof(DSpec) * hspec.size()));
+    ISA_TRY(up(L.spec_syms, hsyms.data(), hsyms.size()));
     ISA_TRY(up(L.lin, hl.data(), 2 * hl.size()));
     ISA_TRY(up(L.bh, hbh.data(), 2 * hbh.size()));
     ISA_TRY(up(L.bv, hbv.data(), 2 * hbv.size()));
     ISA_TRY(up(L.lists, list_bytes.data(), list_bytes.size()));
+    ISA_TRY(up(L.chunk_stream, hcs.data(), 4 * hcs.size()));
+    ISA_TRY(up(L.streams, hsp.data(), sizeof(StreamPlan) * hsp.size()));
+    ISA_TRY(up(L.walk_lists, walk.data(), 4 * walk.size()));
+    std::vector<BlockPlan> blks(blk_lds);
+    blks.insert(blks.end(), blk_glb.begin(), blk_glb.end());
+    ISA_TRY(up(L.blocks, blks.data(), sizeof(BlockPlan) * blks.size()));
 
     const DFile* dF = reinterpret_cast<const DFile*>(ws + L.files);
     const DUnit* dU = reinterpret_cast<const DUnit*>(ws + L.units);
@@ -1678,7 +2041,7 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
     int* dStatus = reinterpret_cast<int*>(ws + L.status);
     uint64_t* dUlen = reinterpret_cast<uint64_t*>(ws + L.ulen);
     uint8_t* dUbuf = ws + L.ubuf;
-    int32_t* dDiff = reinterpret_cast<int32_t*>(ws + L.diffs);
+    uint16_t* dDiff = reinterpret_cast<uint16_t*>(ws + L.diffs);
     int32_t* dChain = reinterpret_cast<int32_t*>(ws + L.vchain);
     uint16_t* dVals = reinterpret_cast<uint16_t*>(ws + L.vals);
 
@@ -1686,21 +2049,75 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
     for (int i = 0; i < n; ++i) max_h = std::max(max_h, files[i]->info.height);
     k_fill_out<<<dim3(std::max(1, std::min(1024, max_h)), n), 256, 0, st>>>(dF, n, dStatus);
     ISA_TRY(isa::check_launch("k_fill_out"));
+    if (!hspec.empty()) {
+        k_build_tables<<<(unsigned)hspec.size(), 256, 0, st>>>(reinterpret_cast<const DSpec*>(ws + L.specs),
+                                                               ws + L.spec_syms, reinterpret_cast<uint16_t*>(ws + L.tables));
+        ISA_TRY(isa::check_launch("k_build_tables"));
+    }
     if (!lj_segs.empty()) {
         k_unstuff<<<(unsigned)lj_segs.size(), kUnstuffThreads, 0, st>>>(dS, dLists + o_lj, dF, dUbuf, dUlen);
         ISA_TRY(isa::check_launch("k_unstuff"));
     }
+    // Chunked decode of the entropy-coded streams: decode the dirty chunks, walk the chains, until every
+    // chunk starts where its predecessor ends (ulen, by segment, comes from k_unstuff).
     if (!hs.empty()) {
-        // ulen is indexed by segment; k_unstuff writes it at the segment's index
-        k_huff_sync<<<(unsigned)hs.size(), kSyncThreads, 0, st>>>(dS, dF, dUbuf, dUlen, dT, dDiff, dStatus,
-                                                                  kSyncThreads + 2);
-        ISA_TRY(isa::check_launch("k_huff_sync"));
+        Chunk* dC = reinterpret_cast<Chunk*>(ws + L.chunks);
+        const int* dCS = reinterpret_cast<const int*>(ws + L.chunk_stream);
+        const StreamPlan* dSP = reinterpret_cast<const StreamPlan*>(ws + L.streams);
+        const int* dW = reinterpret_cast<const int*>(ws + L.walk_lists);
+        int* dDirty = reinterpret_cast<int*>(ws + L.dirty);
+        const BlockPlan* dB = reinterpret_cast<const BlockPlan*>(ws + L.blocks);
+        const int nc = (int)hcs.size(), nser = (int)serial_streams.size(), nscan = (int)scan_streams.size();
+        const int npar = (int)par_chunks.size();
+        const unsigned cgrid = (unsigned)((nc + 255) / 256);
+        k_chunk_init<<<cgrid, 256, 0, st>>>(dC, nc, dSP, dCS);
+        ISA_TRY(isa::check_launch("k_chunk_init"));
+        int max_chunks = 1;
+        for (const StreamPlan& sp : hsp) max_chunks = std::max(max_chunks, sp.n);
+        int walks = 0;
+        for (;;) {
+            if (!blk_lds.empty())
+                k_chunk_decode<true><<<(unsigned)blk_lds.size(), 256, 0, st>>>(dC, dB, dSP, dS, dF, dUbuf, dUlen, dT,
+                                                                               dStatus + 3);
+            if (!blk_glb.empty())
+                k_chunk_decode<false><<<(unsigned)blk_glb.size(), 256, 0, st>>>(dC, dB + blk_lds.size(), dSP, dS, dF,
+                                                                                dUbuf, dUlen, dT, dStatus + 3);
+            ISA_TRY(isa::check_launch("k_chunk_decode"));
+            k_zero_int<<<1, 64, 0, st>>>(dDirty, 4);
+            if (nser > 0) k_walk_serial<<<(unsigned)((nser + 63) / 64), 64, 0, st>>>(dC, dSP, dW, nser, dS, dDirty);
+            if (npar > 0)
+                k_walk_parallel<<<(unsigned)((npar + 255) / 256), 256, 0, st>>>(dC, dW + nser + nscan, npar, dCS,
+                                                                                 dSP, dDirty);
+            ISA_TRY(isa::check_launch("chunk walk"));
+            int dirty = 0;
+            ISA_TRY(isa::check_hip(hipMemcpyAsync(&dirty, dDirty, 4, hipMemcpyDeviceToHost, st), "raw walk read"));
+            ISA_TRY(isa::check_hip(hipStreamSynchronize(st), "raw walk"));
+            ++walks;
+            if (dirty == 0) break;
+            if (walks > max_chunks + 2) {
+                isa::set_error("isa_raw_decode: the chunks of a stream did not synchronise");
+                return ISA_ERR_DATA;
+            }
+        }
+        g_walks = walks;
+        if (nscan > 0) {
+            k_scan_parallel<<<(unsigned)nscan, kScanThreads, 0, st>>>(dC, dSP, dW + nser, dS);
+            ISA_TRY(isa::check_launch("k_scan_parallel"));
+        }
+        if (!blk_lds.empty())
+            k_chunk_write<true><<<(unsigned)blk_lds.size(), 256, 0, st>>>(dC, dB, dSP, dS, dF, dUbuf, dUlen, dT,
+                                                                          dDiff, dStatus);
+        if (!blk_glb.empty())
+            k_chunk_write<false><<<(unsigned)blk_glb.size(), 256, 0, st>>>(dC, dB + blk_lds.size(), dSP, dS, dF,
+                                                                           dUbuf, dUlen, dT, dDiff, dStatus);
+        ISA_TRY(isa::check_launch("k_chunk_write"));
     }
     if (!lin_units.empty()) {
         k_row_starts<<<(unsigned)lin_units.size(), kChainThreads, 0, st>>>(dU, dLists + o_lin, dDiff, dChain);
         ISA_TRY(isa::check_launch("k_row_starts"));
-        k_rows<<<(unsigned)row_unit.size(), kRowThreads, 0, st>>>(dU, dLists + o_ru, dLists + o_ri, dDiff, dChain, dF,
-                                                                  dL, dBH, dBV, dStatus);
+        const int n_rows = (int)row_unit.size();
+        k_rows<<<(unsigned)std::min((n_rows + kRowWaves - 1) / kRowWaves, 4096), 64 * kRowWaves, 0, st>>>(
+            dU, dLists + o_ru, dLists + o_ri, n_rows, dDiff, dChain, dF, dL, dBH, dBV, dStatus);
         ISA_TRY(isa::check_launch("k_rows"));
     }
     if (!nl_units.empty()) {
@@ -1718,6 +2135,8 @@ int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* f
     int status[4] = {0, 0, 0, 0};
     ISA_TRY(isa::check_hip(hipMemcpyAsync(status, dStatus, 16, hipMemcpyDeviceToHost, st), "raw status read"));
     ISA_TRY(isa::check_hip(hipStreamSynchronize(st), "isa_raw_decode"));
+    g_stats[0] = g_walks;
+    g_stats[1] = status[3];
     if (status[0] != 0) {
         const char* what = status[0] == E_EOF     ? "the stream ends before the image is complete"
                            : status[0] == E_RANGE ? "a decoded value exceeds the stream's bit depth"

@@ -539,6 +539,10 @@ int isa_raw_decode_workspace(const isa_raw_file* const* files, int n, size_t* by
  * per stream, or per restart interval), then the predictors are undone by scans. */
 int isa_raw_decode(const isa_raw_file* const* files, int n, const void* const* file_dev, void* const* raw_dev,
                    const int* raw_pitch, void* workspace_dev, size_t workspace_bytes, isa_stream_t stream);
+/* Of the last isa_raw_decode on this thread: the most chain walks one stream needed before its chunks agreed
+ * (1 when every guessed chunk start was right) and the number of chunk re-decodes. No counterpart in the
+ * reference (its decode is sequential). */
+int isa_raw_decode_stats(int* max_sync_walks, int* chunk_redecodes);
 
 /* ------------------------------------------------------------------------
  * Synthetic bursts (benchmark / test input; not part of the reference).
