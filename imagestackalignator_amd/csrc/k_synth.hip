@@ -73,6 +73,16 @@ __device__ void frame_position(const SceneDev& s, int f, float x, float y, float
     py = y + fr[1] + fy;
 }
 
+// Frames other than 0 carry occluding patches: a fraction occ_frac of the 32 x 32 pixel tiles of the frame's
+// pixel grid get a random offset in [-0.3, 0.3] on every channel.
+__device__ float occluded(const SceneDev& s, int f, int x, int y, float v) {
+    if (f == 0 || s.occ_frac <= 0.0f) return v;
+    const int tx = x >> 5, ty = y >> 5;
+    const uint64_t hsh = mix64(s.seed ^ mix64(0x0CC1ull + (uint64_t)f * 1000003ull + (uint64_t)ty * 7919ull + tx));
+    if (u01(hsh) < s.occ_frac) v = fminf(fmaxf(v + (float)(u01(mix64(hsh)) - 0.5) * 0.6f, 0.0f), 1.0f);
+    return v;
+}
+
 __global__ void k_synth(SceneDev s, uint16_t* raw, int f) {
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= s.w || y >= s.h) return;
@@ -82,11 +92,7 @@ __global__ void k_synth(SceneDev s, uint16_t* raw, int f) {
     scene_rgb(s, px, py, rgb);
     int c = (y & 1) == 0 ? ((x & 1) == 0 ? 0 : 1) : ((x & 1) == 0 ? 1 : 2);  // RGGB
     float v = rgb[c];
-    if (f != 0 && s.occ_frac > 0.0f) {
-        int tx = x >> 5, ty = y >> 5;
-        uint64_t hsh = mix64(s.seed ^ mix64(0x0CC1ull + (uint64_t)f * 1000003ull + (uint64_t)ty * 7919ull + tx));
-        if (u01(hsh) < s.occ_frac) v = fminf(fmaxf(v + (float)(u01(mix64(hsh)) - 0.5) * 0.6f, 0.0f), 1.0f);
-    }
+    v = occluded(s, f, x, y, v);
     float dn = v * s.white;
     uint64_t base = mix64(s.seed ^ 0xA5A5ull) + ((uint64_t)f * s.h + y) * (uint64_t)s.w + x;
     double u1 = u01(mix64(base * 2 + 1)), u2 = u01(mix64(base * 2 + 2));
@@ -102,9 +108,13 @@ __global__ void k_ground_truth(SceneDev s, int f, float* out, int pitch, int ow,
     int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= ow || y >= oh) return;
     float px, py;
-    frame_position(s, f, (float)x * scale + ox, (float)y * scale + oy, px, py);
+    const float fx = (float)x * scale + ox, fy = (float)y * scale + oy;
+    frame_position(s, f, fx, fy, px, py);
     float rgb[3];
     scene_rgb(s, px, py, rgb);
+    // the frame's occluders, at the pixel nearest to the sample (a merge onto this frame keeps them)
+    const int qx = (int)floorf(fx + 0.5f), qy = (int)floorf(fy + 0.5f);
+    for (int c = 0; c < 3; ++c) rgb[c] = occluded(s, f, qx < 0 ? 0 : qx, qy < 0 ? 0 : qy, rgb[c]);
     float* o = row_ptr(out, pitch, y) + x * 3;
     o[0] = rgb[0];
     o[1] = rgb[1];

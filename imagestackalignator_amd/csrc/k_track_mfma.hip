@@ -497,6 +497,11 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
     // ---- 6. findMinimum per pair, when the intervals decide the integer minimum and the threshold
     // test as the exact search would; otherwise the item is queued for the exact search. Sixteen lanes
     // per pair (aligned groups within a wave) scan the lags and combine by shuffles.
+    // Undecided items are collected per block and appended with one atomic on the global count (one per
+    // item serialised the appends on that address).
+    __shared__ int q_n, q_base, q_pair[kPairsPerChunk];
+    if (tid == 0) q_n = 0;
+    __syncthreads();
     {
         static_assert(kPairsPerChunk * 16 == NT, "sixteen lanes per pair");
         const int jj = tid >> 4, l = tid & 15;
@@ -553,12 +558,17 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
                 sh[0] = psx + cx;
                 sh[1] = psy + cy;
             } else {
-                // the queue holds one slot per (pair, tile) of the level, so q stays in range
-                const int q = atomicAdd(xp.count, 1);
-                xp.queue[2 * q] = pp;
-                xp.queue[2 * q + 1] = tile_idx;
+                q_pair[atomicAdd(&q_n, 1)] = pp;
             }
         }
+    }
+    __syncthreads();
+    // the queue holds one slot per (pair, tile) of the level, so every slot stays in range
+    if (tid == 0 && q_n > 0) q_base = atomicAdd(xp.count, q_n);
+    __syncthreads();
+    if (tid < q_n) {
+        xp.queue[2 * (q_base + tid)] = q_pair[tid];
+        xp.queue[2 * (q_base + tid) + 1] = tile_idx;
     }
 }
 

@@ -562,8 +562,9 @@ void isa_default_synth_config(isa_synth_config* c);
 /* Renders the burst (frames x height x width u16, RGGB) on the device. frame_shift_out (host, 2*frames, may be
  * NULL) receives each frame's global shift. */
 int isa_synth_burst(const isa_synth_config* c, uint16_t* raw_dev, float* frame_shift_out, isa_stream_t stream);
-/* Noise-free RGB rendering of frame `frame`'s scene geometry on an out_w x out_h grid covering the frame with
- * sample spacing `scale` (1 = pixel grid, 0.5 = 2x) and origin offset (ox, oy) in frame pixels. */
+/* Noise-free RGB rendering of frame `frame`'s scene geometry, with that frame's occluding patches, on an
+ * out_w x out_h grid covering the frame with sample spacing `scale` (1 = pixel grid, 0.5 = 2x) and origin
+ * offset (ox, oy) in frame pixels. */
 int isa_synth_ground_truth(const isa_synth_config* c, int frame, float* out3_dev, int out_pitch, int out_w, int out_h,
                            float scale, float ox, float oy, isa_stream_t stream);
 
