@@ -1096,7 +1096,7 @@ int launch_track_mfma(int precision, const float* const* frames, const unsigned*
                       int* decisions, int* queue, int* count, hipStream_t s);
 int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax, hipStream_t s);
 int vbox_run_rows(int h, int T);
-int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s);
+int split_planes(int precision, const float* src, int P, int w, int h, unsigned* dst, hipStream_t s);
 }  // namespace isa
 
 namespace isa {
@@ -1517,8 +1517,8 @@ int isa_track_xc_ex(const float* img_track_dev, const float* img_ref_dev, int im
     ISA_TRY(isa::check_hip(hipMemcpy2DAsync(dn, row, preshift2_dev, preshift_pitch, row, cy, hipMemcpyDeviceToDevice, s),
                            "isa_track: copy"));
     for (int f = 0; f < 2 && split; ++f)
-        ISA_TRY(isa::split_planes(xc_precision, f ? img_track_dev : img_ref_dev, (unsigned*)h.xplanes[f],
-                                  ((size_t)img_pitch / 4) * height, s));
+        ISA_TRY(isa::split_planes(xc_precision, f ? img_track_dev : img_ref_dev, img_pitch / 4, width, height,
+                                  (unsigned*)h.xplanes[f], s));
     if (mfma)
         ISA_TRY(isa::vbox_energies(img_track_dev, img_pitch, width, height, tile_size, (float*)h.vbox[1],
                                    (float*)h.vrun[1], s));

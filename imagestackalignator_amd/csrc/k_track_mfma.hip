@@ -7,15 +7,17 @@
 // with s = (iy, ix) over the (2M+1)^2 lags and j over the pairs (their track blocks sit at each
 // pair's own pre-shift). The product runs on 16x16 MFMA tiles: lags on the rows, up to 16 pairs on
 // the columns, the track-block pixels as the reduction dimension. Operands:
-//   ISA_XC_BF16X3: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); hi.hi + hi.lo + lo.hi.
-//   ISA_XC_FP16X3: the same split in fp16; the reference interior is scaled by a power of two that
-//                  brings its maximum into [0.5, 1) (per tile), the track images are split as is.
-//   ISA_XC_FP32:   fp32 operands (16x16x4 MFMA).
-// All accumulate in fp32. The track images are split once per level into packed (hi | lo << 16)
-// planes, so the product loop only loads and repacks. The window energies sum r^2 (FP64) and
-// sum t(. + s)^2 (column sums in fp32, totals in FP64) are formed on the vector units, and
-// D(s) = sum r^2 + sum t(. + s)^2 - 2 C(s) feeds findMinimum as in the exact search.
+//   ISA_XC_BF16X3: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); hi.hi + hi.lo + lo.hi on
+//                  v_mfma_f32_16x16x32_bf16, whose 32 reduction slots take 16 columns of two track rows;
+//   ISA_XC_FP16X3: the same split in fp16 (v_mfma_f32_16x16x32_f16);
+//   ISA_XC_FP32:   fp32 operands (v_mfma_f32_16x16x4f32, one track row per step).
+// All accumulate in fp32. The track images are split once per level into planes of 16-bit hi and lo
+// parts, and the reference interior is staged in LDS as the same two planes, so the product loop only
+// loads. The window energies sum r^2 (FP64, from the fp32 pixels) and sum t(. + s)^2 (the vertical
+// window energies of the level image, summed and slid along each row in FP64) are formed on the vector
+// units, and D(s) = sum r^2 + sum t(. + s)^2 - 2 C(s) feeds findMinimum as in the exact search.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 #include "isa_device.h"
@@ -24,15 +26,25 @@
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));  // 16 bytes at a 4-byte aligned address
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// 16 and 8 bytes at 2-byte aligned addresses (global and LDS accesses need no natural alignment here)
+typedef unsigned u32x4a2 __attribute__((ext_vector_type(4), aligned(2)));
+typedef unsigned u32x2a2 __attribute__((ext_vector_type(2), aligned(2)));
+typedef unsigned u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+// global-memory views (plain loads, not flat ones: they wait on the vector-memory counter only)
+typedef __attribute__((address_space(1))) const unsigned gu32;
+typedef __attribute__((address_space(1))) const float gf32;
+typedef __attribute__((address_space(1))) const u32x4a4 gu32x4;
+typedef __attribute__((address_space(1))) const f32x4a4 gf32x4;
 
 constexpr int kPairsPerChunk = 16;  // MFMA columns
-constexpr int kWaves = 4;           // waves per block; wave w reduces over track rows uy = w mod 4
 constexpr int kMaxNB = 5;           // track-block width up to 80 (16-column blocks)
+constexpr int kVRun = 64;           // window starts per vertical slide of k_vbox_sq
+constexpr int kCB = 64;             // columns per block of the slide maxima
 
 __device__ __forceinline__ unsigned bf16_rn(float x) {
     unsigned u = __float_as_uint(x);
@@ -41,152 +53,210 @@ __device__ __forceinline__ unsigned bf16_rn(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(unsigned h) { return __uint_as_float(h << 16); }
 
-// x = hi + lo packed as (hi | lo << 16)
-template <int PREC> __device__ __forceinline__ unsigned split_pack(float x) {
+// x = hi + lo (+ the remainder), each part as its 16-bit pattern
+template <int PREC> __device__ __forceinline__ void split2(float x, unsigned& hi, unsigned& lo) {
     if (PREC == ISA_XC_BF16X3) {
-        unsigned hi = bf16_rn(x);
-        unsigned lo = bf16_rn(x - bf16_to_f(hi));
-        return hi | (lo << 16);
+        hi = bf16_rn(x);
+        lo = bf16_rn(x - bf16_to_f(hi));
     } else {
-        _Float16 h = (_Float16)x;
-        _Float16 l = (_Float16)(x - (float)h);
-        return (unsigned)__builtin_bit_cast(unsigned short, h) | ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
+        const _Float16 h = (_Float16)x;
+        const _Float16 l = (_Float16)(x - (float)h);
+        hi = __builtin_bit_cast(unsigned short, h);
+        lo = __builtin_bit_cast(unsigned short, l);
     }
 }
 
-// four packed values -> (hi x4, lo x4) as two 64-bit operand halves; one byte permute per dword
-// (v_perm_b32: bytes 0-3 of the 64-bit source are the second operand, 4-7 the first)
-__device__ __forceinline__ void unpack4(unsigned x0, unsigned x1, unsigned x2, unsigned x3, u32x2& hi, u32x2& lo) {
-    hi = u32x2{__builtin_amdgcn_perm(x1, x0, 0x05040100u), __builtin_amdgcn_perm(x3, x2, 0x05040100u)};
-    lo = u32x2{__builtin_amdgcn_perm(x1, x0, 0x07060302u), __builtin_amdgcn_perm(x3, x2, 0x07060302u)};
-}
-
-template <int PREC>
-__device__ __forceinline__ f32x4 mfma16(u32x2 a, u32x2 b, f32x4 c) {
+template <int PREC> __device__ __forceinline__ f32x4 mfma32(u32x4 a, u32x4 b, f32x4 c) {
     if (PREC == ISA_XC_BF16X3)
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(bf16x4, a), __builtin_bit_cast(bf16x4, b), c,
-                                                        0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
     else
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, a), __builtin_bit_cast(f16x4, b), c, 0,
-                                                     0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                      0, 0);
 }
 
-// four elements per thread (16-byte accesses when both arrays are 16-byte aligned, else one by one)
-template <int PREC> __global__ void k_split_planes(const float* src, unsigned* dst, size_t n, bool vec) {
-    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-    if (vec && i + 4 <= n) {
-        const float4 v = *(const float4*)(src + i);
-        *(uint4*)(dst + i) = make_uint4(split_pack<PREC>(v.x), split_pack<PREC>(v.y), split_pack<PREC>(v.z),
-                                        split_pack<PREC>(v.w));
+// Packed split planes of a level image (row pitch P floats): word y P + x holds hi | lo << 16 of pixel
+// (x, y). Four pixels per thread.
+template <int PREC> __global__ void k_split_rows(const float* src, int P, int w, unsigned* dst) {
+    const int y = blockIdx.y, x = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (x >= w) return;
+    const float* s = src + (size_t)y * P + x;
+    unsigned* d = dst + (size_t)y * P + x;
+    if (x + 4 <= w) {
+        const f32x4a4 v = *(const f32x4a4*)s;
+        u32x4a4 o;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            unsigned h, l;
+            split2<PREC>(v[e], h, l);
+            o[e] = h | (l << 16);
+        }
+        *(u32x4a4*)d = o;
         return;
     }
-    for (size_t k = i; k < i + 4 && k < n; ++k) dst[k] = split_pack<PREC>(src[k]);
+    for (int e = 0; x + e < w; ++e) {
+        unsigned h, l;
+        split2<PREC>(s[e], h, l);
+        d[e] = h | (l << 16);
+    }
 }
 
 // Vertical window energies V(y0, x) = sum_{y = y0}^{y0 + T - 1} src(y, x)^2 for y0 in [0, h - T]:
-// one thread per column and run of kVRun window starts (a T-term sum, then slid in FP64). runmax gets
-// the largest energy of each (run, column), rounded up: the squares are exact in FP64, so a window's
-// error before its rounding to float is at most (T + 2 kVRun) 2^-53 times that maximum.
-constexpr int kVRun = 64;
+// one thread per column and run of kVRun window starts (a T-term sum, then slid in FP64), rows loaded
+// sixteen at a time. runmax gets, per run and block of kCB columns (one wave), the largest energy any
+// of its columns was slid through, rounded up: the squares are exact in FP64, so a window's error
+// before its rounding to float is at most (T + 2 kVRun) 2^-53 times that maximum. Blocks of 256 threads.
 __global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax) {
+    constexpr int kB = 16;  // rows per batch of loads
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y0 = blockIdx.y * kVRun, nv = h - T + 1;
-    if (x >= w || y0 >= nv) return;
-    auto at = [&](int y) { return (double)row_ptr(src, pitch, min(y, h - 1))[x]; };
-    double e = 0.0, emax = 0.0;
-    for (int yb = y0; yb < y0 + T; yb += 8) {  // loads of a batch issued together
-        double v[8];
+    double emax = 0.0;
+    if (x < w) {
+        auto at = [&](int y) { return row_ptr(src, pitch, min(y, h - 1))[x]; };
+        double e = 0.0;
+        for (int yb = y0; yb < y0 + T; yb += kB) {
+            float v[kB];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = yb + k < y0 + T ? at(yb + k) : 0.0;
+            for (int k = 0; k < kB; ++k) v[k] = yb + k < y0 + T ? at(yb + k) : 0.0f;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) e += v[k] * v[k];
-    }
-    const int y1 = min(y0 + kVRun, nv);
-    for (int yb = y0; yb < y1; yb += 8) {
-        double a[8], b[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            a[k] = at(yb + k);
-            b[k] = at(yb + k + T);  // rows past the image are clamped and never used
+            for (int k = 0; k < kB; ++k) e += (double)v[k] * v[k];
         }
+        const int y1 = min(y0 + kVRun, nv);
+        for (int yb = y0; yb < y1; yb += kB) {
+            float a[kB], b[kB];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int y = yb + k;
-            if (y >= y1) break;
-            dst[(size_t)y * w + x] = (float)e;
-            emax = fmax(emax, e);
-            if (y + T < h) e = (e - a[k] * a[k]) + b[k] * b[k];
+            for (int k = 0; k < kB; ++k) {
+                a[k] = at(yb + k);
+                b[k] = at(yb + k + T);  // rows past the image are clamped and never used
+            }
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const int y = yb + k;
+                if (y >= y1) break;
+                dst[(size_t)y * w + x] = (float)e;
+                emax = fmax(emax, e);
+                if (y + T < h) e = (e - (double)a[k] * a[k]) + (double)b[k] * b[k];
+            }
         }
     }
-    runmax[(size_t)blockIdx.y * w + x] = (float)(emax * (1.0 + 0x1p-20));
+    float m = (float)(emax * (1.0 + 0x1p-20));
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && x < w) runmax[(size_t)blockIdx.y * ((w + kCB - 1) / kCB) + x / kCB] = m;
 }
 
-// LDS: the reference plane is live until the product loop ends; the partial products, C, the
-// column energies and D reuse its bytes afterwards. misc (R2, scale, wave sums of r^2) is separate.
+// Waves per block, and waves per SIMD the register allocation must allow (the second launch bound):
+// eight-wave blocks, two per CU, where the accumulators leave room for four waves per SIMD; four-wave
+// blocks with two waves per SIMD above.
+__host__ __device__ constexpr int mfma_waves(int M) { return M <= 4 ? 8 : 4; }
+template <int M> struct MfmaCfg {
+    static constexpr int waves = mfma_waves(M);
+    static constexpr int waves_per_simd = M <= 4 ? 4 : 2;
+};
+
+// LDS of one block. Region 0 holds the reference planes until the product loop ends, then the partial
+// products of half the waves and, after them, C; once C is formed, TT, the row bounds, D and W take the
+// bytes before C. The column energies of the track blocks (and the slide maxima they come with) have a
+// region of their own: they are copied in while the products run. The split planes hold pair words:
+// word (yy, xx) = element xx in its low half and element xx + 1 in its high half, so that any run of
+// elements from any column is read as aligned words.
 struct MfmaLayout {
-    int RW, RH;        // padded reference plane: RH rows x RW columns (+ one zero row)
-    int Bp;            // track-block width rounded up to 16
-    size_t off_plane;  // reference plane, 4 bytes per element (packed hi|lo, or fp32)
-    size_t off_red;    // running sum of the waves' partial products [MB][256] (after the product loop)
-    size_t off_c;      // C [chunk][S*S]                                  (after the product loop)
-    size_t off_col;    // column energies [chunk][B][S] (float); aliases red (after C is formed)
-    size_t off_d;      // D [chunk][S*S]                                  (after the product loop)
-    size_t off_w;      // certificate half-widths W [chunk][S*S] (float)   (after the product loop)
-    size_t off_misc;   // R2 (double), scale, per-wave sums of r^2 (double), per-pair energy maxima
+    int S, B, Bp, NB, NP;  // lags per axis, track-block width, its 16-multiple, 16-column blocks, row pairs
+    int RH, RW, CO;        // planes: RH + 1 rows x RW words; interior at row S - 1, column CO
+    int PE;                // words of one plane
+    size_t off_plane;      // hi plane, then lo plane (pair words); the fp32 plane is the first of them
+    size_t off_red;        // partial products of waves W/2 .. W-1 [W/2][MB][256] floats
+    size_t off_tt;         // window energies TT [chunk][S*S] doubles
+    size_t off_eb;         // per (pair, lag row): bound of every window energy of the row, slide-error maximum
+    size_t off_d, off_w;   // D, W [chunk][S*S] floats
+    size_t off_c;          // C [chunk][S*S] floats
+    size_t off_col;        // column energies [chunk][S][Bq] floats, Bq = B rounded up to 4
+    size_t off_vr;         // slide maxima of the vertical window energies a lag row reads [chunk][S][4] floats
     size_t bytes;
 };
 
 __host__ __device__ inline MfmaLayout mfma_layout(int T, int M) {
     MfmaLayout L;
-    const int S = 2 * M + 1, B = T + 2 * M, MB = (S * S + 15) / 16;
+    const int S = 2 * M + 1, B = T + 2 * M, SS = S * S, MB = (SS + 15) / 16;
+    L.S = S;
+    L.B = B;
     L.Bp = (B + 15) / 16 * 16;
+    L.NB = L.Bp / 16;
+    L.NP = (B + 1) / 2;
     L.RH = B + S - 1;
-    L.RW = L.Bp + S - 1;  // reads reach column Bp - 1 + S - 1
+    L.CO = (S - 1 + 7) / 8 * 8;
+    L.RW = L.Bp + L.CO;
+    L.PE = (L.RH + 1) * L.RW;
     auto al = [](size_t v) { return (v + 15) / 16 * 16; };
     L.off_plane = 0;
-    const size_t plane = al((size_t)(L.RH + 1) * L.RW * 4);
+    L.off_red = 0;
     size_t o = 0;
-    // the running sum of the waves' partial products and, after it, the column energies share bytes
-    L.off_red = o;
-    L.off_col = o;
-    o += std::max(al((size_t)MB * 256 * 4), al((size_t)kPairsPerChunk * B * S * 4));
-    L.off_c = o;
-    o += al((size_t)kPairsPerChunk * S * S * 4);
+    L.off_tt = o;
+    o += al((size_t)kPairsPerChunk * SS * 8);
+    L.off_eb = o;
+    o += al((size_t)2 * kPairsPerChunk * S * 4);
     L.off_d = o;
-    o += al((size_t)kPairsPerChunk * S * S * 4);
+    o += al((size_t)kPairsPerChunk * SS * 4);
     L.off_w = o;
-    o += al((size_t)kPairsPerChunk * S * S * 4);
-    L.off_misc = o > plane ? o : plane;
-    L.bytes = L.off_misc + 128;
+    o += al((size_t)kPairsPerChunk * SS * 4);
+    const size_t red = al((size_t)(mfma_waves(M) / 2) * MB * 256 * 4);
+    L.off_c = o > red ? o : red;
+    o = L.off_c + al((size_t)kPairsPerChunk * SS * 4);
+    const size_t planes = 2 * al((size_t)L.PE * 4);
+    o = o > planes ? o : planes;
+    L.off_col = o;
+    o += al((size_t)kPairsPerChunk * S * ((B + 3) & ~3) * 4);
+    L.off_vr = o;
+    o += al((size_t)kPairsPerChunk * S * 4 * 4);
+    L.bytes = o;
     return L;
 }
 
 struct XPlanes {
-    const unsigned* const* frames;  // per frame: packed split plane, same pixel pitch as the level image
-    const float* const* vbox;       // per frame: vertical T-window energies, (h - T + 1) x w
-    const float* const* vrun;       // per frame: run maxima of vbox, ceil((h - T + 1) / kVRun) x w
-    int* queue;                     // items left to the exact search: (pair, tile) int pairs
-    int* count;                     // number of queued items
+    const unsigned* const* frames;  // per frame: packed split planes (k_split_rows), same pitch as the level image
+    const float* const* vbox;             // per frame: vertical T-window energies, (h - T + 1) x w
+    const float* const* vrun;             // per frame: their slide maxima, runs x ceil(w / kCB)
+    int* queue;                           // items left to the exact search: (pair, tile) int pairs
+    int* count;                           // number of queued items
 };
 
-// chunk c of the level: up to 16 pair indices sharing one reference frame (-1 = empty slot)
-// waves per SIMD the register allocation must allow: 4 for the 25-lag search, 3 above
-template <int M> struct MfmaWaves { static constexpr int v = M <= 2 ? 4 : 3; };
+// hi (sel 0x05040100) or lo (0x07060302) halves of two packed words, as one word of two elements
+__device__ __forceinline__ unsigned halves(unsigned w1, unsigned w0, unsigned sel) {
+    return __builtin_amdgcn_perm(w1, w0, sel);
+}
 
+// chunk c of the level: up to 16 pair indices sharing one reference frame (-1 = empty slot)
 template <int PREC, int M>
-__global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(TrackArgs a, XPlanes xp, const int* __restrict__ chunk_pairs,
-                                                            int n_chunks, int nblk) {
-    constexpr int S = 2 * M + 1, SS = S * S, MB = (SS + 15) / 16;
-    constexpr int NT = kWaves * 64;
+__global__ __launch_bounds__(MfmaCfg<M>::waves * 64, MfmaCfg<M>::waves_per_simd) void k_track_mfma(
+    TrackArgs a, XPlanes xp, const int* __restrict__ chunk_pairs, int n_chunks, int nblk) {
+    constexpr int S = 2 * M + 1, SS = S * S, MB = (SS + 15) / 16, RO = S - 1;
+    constexpr int W = MfmaCfg<M>::waves, NT = W * 64;
+    constexpr bool SPLIT = PREC != ISA_XC_FP32;
+    // fp32 summation chains of the split products, as the registers allow: up to 32 lags, a chain per
+    // track row pair for the hi.hi products and one chain for the products with a lo part; up to 96
+    // lags, a chain per row pair for all three; beyond, one chain for all
+    constexpr bool kFresh = MB <= 6;
+    constexpr bool kSepLo = MB <= 2;
+    constexpr int kRing = 4;  // units (row pair, column block) of track operands in flight per wave
     extern __shared__ __align__(16) char smem[];
-    const int T = a.tile, B = T + 2 * M;
+    __shared__ double s_r2w[W];
+    __shared__ double s_R2;
+    __shared__ unsigned s_rmax_edge[kPairsPerChunk];
+    __shared__ int s_bx[kPairsPerChunk], s_by[kPairsPerChunk], s_in[kPairsPerChunk];
+    __shared__ const float* s_vb[kPairsPerChunk];  // vertical window energies of the pair's block, first row
+    __shared__ const float* s_vr[kPairsPerChunk];  // slide maxima of its frame
+    __shared__ float* s_sh[kPairsPerChunk];        // the pair's shift of this tile, and its value
+    __shared__ float s_ps[kPairsPerChunk][2];
+    __shared__ int s_any_edge;                     // some pair's block leaves the image
+    __shared__ int q_n, q_base, q_pair[kPairsPerChunk];
+    const int T = a.tile;
     const MfmaLayout L = mfma_layout(T, M);
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int B = L.B, RW = L.RW, CO = L.CO, NB = L.NB, NP = L.NP, Bq = (B + 3) & ~3;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // XCD-aware order (as in the exact search): each XCD gets a contiguous run of (tile, chunk)
     const int per = (int)(gridDim.x / 8);
     const int blk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (blk >= nblk) return;
-    if (threadIdx.x < kPairsPerChunk) ((unsigned*)(smem + mfma_layout(a.tile, M).off_misc + 64))[threadIdx.x] = 0u;
     const int tile_idx = blk / n_chunks, chunk = blk - tile_idx * n_chunks;
     const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
     const int* cp = chunk_pairs + (size_t)chunk * kPairsPerChunk;
@@ -196,233 +266,381 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
                    0.0f, 0.0f, rox, roy);
     const int rx0 = tx * T + M + rox, ry0 = ty * T + M + roy;
     const float* ia = a.frames[fa];
-    float* misc = (float*)(smem + L.off_misc);
-    double* R2s = (double*)misc;
-    float* scale_s = misc + 2;
-    float* red = (float*)(smem + L.off_red);
-    auto rpix = [&](int y, int x) {
-        return row_ptr(ia, a.img_pitch, clampi(ry0 + y, 0, a.h - 1))[clampi(rx0 + x, 0, a.w - 1)];
-    };
-    // ---- 0. power-of-two scale of the reference interior (fp16 only)
-    float sc = 1.0f;
-    if (PREC == ISA_XC_FP16X3) {
-        float rmax = 0.0f;
-        for (int yb = wave; yb < T; yb += kWaves * 8) {
-            float v[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int y = yb + (k >> 1) * kWaves, x = lane + 64 * (k & 1);
-                v[k] = (y < T && x < T) ? rpix(y, x) : 0.0f;
-            }
-#pragma unroll
-            for (int k = 0; k < 16; ++k) rmax = fmaxf(rmax, fabsf(v[k]));
-        }
-        red[tid] = rmax;
-        __syncthreads();
-        if (tid == 0) {
-            float mx = 0.0f;
-            for (int i = 0; i < NT; ++i) mx = fmaxf(mx, red[i]);
-            float f = 1.0f;
-            if (mx > 0.0f && isfinite(mx)) {
-                int e;
-                frexpf(mx, &e);  // mx = m * 2^e, m in [0.5, 1)
-                f = ldexpf(1.0f, -e);
-            }
-            *scale_s = f;
-        }
-        __syncthreads();
-        sc = *scale_s;
-    }
-    // ---- 1. zero-padded reference plane P[yy][xx] = r(yy - (S-1), xx - (S-1)), and sum r^2
-    const int RW = L.RW, RH = L.RH;
-    unsigned* pu = (unsigned*)(smem + L.off_plane);
-    float* pf = (float*)(smem + L.off_plane);
-    double r2 = 0.0;
-    constexpr int kFill = 8;
-    for (int yb = wave; yb < RH + 1; yb += kWaves * kFill) {
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            const int xx = lane + 64 * half, x = xx - (S - 1);
-            float v[kFill];
-#pragma unroll
-            for (int k = 0; k < kFill; ++k) {
-                const int yy = yb + k * kWaves, y = yy - (S - 1);
-                v[k] = (xx < RW && yy < RH && y >= 0 && y < T && x >= 0 && x < T) ? rpix(y, x) : 0.0f;
-            }
-#pragma unroll
-            for (int k = 0; k < kFill; ++k) {
-                const int yy = yb + k * kWaves;
-                if (xx >= RW || yy >= RH + 1) continue;
-                if (PREC == ISA_XC_FP32) pf[yy * RW + xx] = v[k];
-                else pu[yy * RW + xx] = split_pack<PREC>(v[k] * sc);
-                r2 = fma((double)v[k], (double)v[k], r2);  // v = 0 outside the interior
-            }
-        }
-    }
-    // block sum of r^2: a butterfly within each wave, then the four wave sums in order
-    for (int off = 32; off >= 1; off >>= 1) r2 += __shfl_xor(r2, off);
-    double* r2buf = (double*)(smem + L.off_misc + 16);
-    if (lane == 0) r2buf[wave] = r2;
+    const size_t P = (size_t)a.img_pitch / sizeof(float);  // pixel pitch, shared by the split planes
+    // per pair: origin of its track block and whether the block lies inside the image
+    if (tid == 0) s_any_edge = 0;
     __syncthreads();
-    if (tid == 0) *R2s = ((r2buf[0] + r2buf[1]) + r2buf[2]) + r2buf[3];
+    if (tid < kPairsPerChunk) {
+        const int pp = cp[tid];
+        int ox = 0, oy = 0;
+        float* sh = nullptr;
+        float psx = 0.0f, psy = 0.0f;
+        if (pp >= 0) {
+            const int fb = a.pb[pp];
+            sh = a.shifts + (size_t)a.pk[pp] * a.meas_stride + (size_t)tile_idx * 2;
+            psx = sh[0];
+            psy = sh[1];
+            tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2],
+                           a.base[fb * 4 + 3], psx, psy, ox, oy);
+        }
+        const int bx0 = tx * T + ox, by0 = ty * T + oy;
+        const bool in = pp >= 0 && bx0 >= 0 && by0 >= 0 && bx0 + B <= a.w && by0 + B <= a.h;
+        s_bx[tid] = bx0;
+        s_by[tid] = by0;
+        s_in[tid] = in;
+        s_vb[tid] = in ? xp.vbox[a.pb[pp]] + (size_t)by0 * a.w + bx0 : nullptr;
+        s_vr[tid] = in ? xp.vrun[a.pb[pp]] : nullptr;
+        s_sh[tid] = sh;
+        s_ps[tid][0] = psx;
+        s_ps[tid][1] = psy;
+        s_rmax_edge[tid] = 0u;
+        if (pp >= 0 && !in) s_any_edge = 1;
+    }
+    if (tid == 0) q_n = 0;
+    __syncthreads();
+    // ---- 0. column energies of the track blocks inside the image: the S rows of B vertical window
+    // energies each, copied into LDS by direct loads (one wave per lag row), and the slide maxima of
+    // the column blocks each row reads. They complete while the products run.
+    float* colE = (float*)(smem + L.off_col);
+    float* vr_s = (float*)(smem + L.off_vr);
+    const int ncb = (a.w + kCB - 1) / kCB;
+    typedef __attribute__((address_space(3))) void lds_void;
+    typedef __attribute__((address_space(1))) const void glob_void;
+    {
+        // lane l < 16 holds pair l's values; a lag row reads them with lane reads
+        const int lp = lane & (kPairsPerChunk - 1);
+        const int l_in = s_in[lp], l_bx = s_bx[lp], l_by = s_by[lp];
+        const unsigned long long l_vb = (unsigned long long)s_vb[lp], l_vr = (unsigned long long)s_vr[lp];
+        for (int r = wave; r < kPairsPerChunk * S; r += W) {
+            const int jj = r / S, iy = r - (r / S) * S;
+            if (!__builtin_amdgcn_readlane(l_in, jj)) continue;
+            const unsigned long long vb = ((unsigned long long)__builtin_amdgcn_readlane((unsigned)(l_vb >> 32), jj) << 32) |
+                                          (unsigned)__builtin_amdgcn_readlane((unsigned)l_vb, jj);
+            gf32* vr = (gf32*)vb + (size_t)iy * a.w;
+            // B / 4 lanes copy 16 bytes each, the last B % 4 columns one by one
+            if (lane < B / 4)
+                __builtin_amdgcn_global_load_lds((glob_void*)(vr + 4 * lane), (lds_void*)(colE + r * Bq), 16, 0, 0);
+            if (lane < (B & 3))
+                __builtin_amdgcn_global_load_lds((glob_void*)(vr + (B & ~3) + lane), (lds_void*)(colE + r * Bq + (B & ~3)),
+                                                 4, 0, 0);
+            const int bx0 = __builtin_amdgcn_readlane(l_bx, jj), y = __builtin_amdgcn_readlane(l_by, jj) + iy;
+            const int cb0 = bx0 / kCB, ncr = (bx0 + B - 1) / kCB - cb0 + 1;  // at most 3 (B <= 80)
+            const unsigned long long vrr = ((unsigned long long)__builtin_amdgcn_readlane((unsigned)(l_vr >> 32), jj) << 32) |
+                                           (unsigned)__builtin_amdgcn_readlane((unsigned)l_vr, jj);
+            if (lane < ncr)
+                __builtin_amdgcn_global_load_lds((glob_void*)((gf32*)vrr + (size_t)(y / kVRun) * ncb + cb0 + lane),
+                                                 (lds_void*)(vr_s + r * 4), 4, 0, 0);
+        }
+    }
+    // ---- 1. zero-padded reference planes P[yy][xx] = r(yy - (S-1), xx - CO), and sum r^2 (called by
+    // the product loops once their first track operands are in flight)
+    auto fill_planes = [&]() {
+    unsigned* hiW = (unsigned*)(smem + L.off_plane);
+    unsigned* loW = hiW + L.PE;
+    float* pf = (float*)(smem + L.off_plane);
+    const bool ref_in = rx0 >= 0 && ry0 >= 0 && rx0 + T <= a.w && ry0 + T <= a.h;
+    double r2 = 0.0;
+    if (SPLIT && ref_in && (T & 7) == 0) {
+        // eight words per item, from elements x .. x + 8 (x = xx - CO): CO and T are multiples of 8, so
+        // elements x .. x + 7 lie wholly inside the interior or wholly outside; inside, the split parts
+        // come from the frame's packed planes
+        gu32* xr = (gu32*)xp.frames[fa];
+        const int G = RW / 8, items = (L.RH + 1) * G;
+        for (int c = tid; c < items; c += NT) {
+            const int yy = c / G, xx = (c - yy * G) * 8, y = yy - RO, x = xx - CO;
+            unsigned pk[9];  // packed elements x .. x + 8
+#pragma unroll
+            for (int k = 0; k < 9; ++k) pk[k] = 0u;
+            if (y >= 0 && y < T) {
+                gu32* row = xr + (size_t)(ry0 + y) * P + rx0;
+                if (x >= 0 && x < T) {
+                    const u32x4 u0 = *(gu32x4*)(row + x), u1 = *(gu32x4*)(row + x + 4);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        pk[k] = u0[k];
+                        pk[4 + k] = u1[k];
+                    }
+                    gf32* fr = (gf32*)ia + (size_t)(ry0 + y) * P + rx0 + x;
+                    const f32x4 u = *(gf32x4*)fr, v = *(gf32x4*)(fr + 4);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) r2 = fma((double)u[k], (double)u[k], r2);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) r2 = fma((double)v[k], (double)v[k], r2);
+                }
+                if (x + 8 >= 0 && x + 8 < T) pk[8] = row[x + 8];
+            }
+            u32x4 h0, h1, l0, l1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                h0[k] = __builtin_amdgcn_perm(pk[k + 1], pk[k], 0x05040100u);
+                h1[k] = __builtin_amdgcn_perm(pk[k + 5], pk[k + 4], 0x05040100u);
+                l0[k] = __builtin_amdgcn_perm(pk[k + 1], pk[k], 0x07060302u);
+                l1[k] = __builtin_amdgcn_perm(pk[k + 5], pk[k + 4], 0x07060302u);
+            }
+            *(u32x4*)(hiW + yy * RW + xx) = h0;
+            *(u32x4*)(hiW + yy * RW + xx + 4) = h1;
+            *(u32x4*)(loW + yy * RW + xx) = l0;
+            *(u32x4*)(loW + yy * RW + xx + 4) = l1;
+        }
+    } else if (!SPLIT && ref_in && (T & 3) == 0) {
+        const int G = RW / 4, items = (L.RH + 1) * G;
+        for (int c = tid; c < items; c += NT) {
+            const int yy = c / G, xx = (c - yy * G) * 4, y = yy - RO, x = xx - CO;
+            f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (y >= 0 && y < T && x >= 0 && x < T) {
+                v = *(gf32x4*)((gf32*)ia + (size_t)(ry0 + y) * P + rx0 + x);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r2 = fma((double)v[k], (double)v[k], r2);
+            }
+            *(f32x4*)(pf + yy * RW + xx) = v;
+        }
+    } else {
+        // element by element, the reference's rows and columns clamped to the image
+        for (int yy = wave; yy <= L.RH; yy += W) {
+            const int y = yy - RO;
+            const float* row = row_ptr(ia, a.img_pitch, clampi(ry0 + y, 0, a.h - 1));
+            auto val = [&](int xx) {
+                const int x = xx - CO;
+                return (y >= 0 && y < T && x >= 0 && x < T) ? row[clampi(rx0 + x, 0, a.w - 1)] : 0.0f;
+            };
+            for (int xx = lane; xx < RW; xx += 64) {
+                const float v = val(xx);
+                if (SPLIT) {
+                    unsigned h0, l0, h1, l1;
+                    split2<PREC>(v, h0, l0);
+                    split2<PREC>(val(xx + 1), h1, l1);
+                    hiW[yy * RW + xx] = h0 | (h1 << 16);
+                    loW[yy * RW + xx] = l0 | (l1 << 16);
+                } else {
+                    pf[yy * RW + xx] = v;
+                }
+                r2 = fma((double)v, (double)v, r2);  // v = 0 outside the interior
+            }
+        }
+    }
+    // block sum of r^2: a butterfly within each wave, then the wave sums in order
+    for (int off = 32; off >= 1; off >>= 1) r2 += __shfl_xor(r2, off);
+    if (lane == 0) s_r2w[wave] = r2;
+    __syncthreads();
+    if (tid == 0) {
+        double t = s_r2w[0];
+        for (int w = 1; w < W; ++w) t += s_r2w[w];
+        s_R2 = t;
+    }
+    };
+    const unsigned* hiW = (const unsigned*)(smem + L.off_plane);
+    const float* pf = (const float*)(smem + L.off_plane);
     // ---- 2. cross terms on MFMA: lane (column j = lane % 16, quad q = lane / 16)
     const int j = lane & 15, q = lane >> 4;
     const int pj = cp[j];
-    int tox = 0, toy = 0;
-    if (pj >= 0) {
-        const int fb = a.pb[pj];
-        const float* sh = a.shifts + (size_t)a.pk[pj] * a.meas_stride + (size_t)tile_idx * 2;
-        tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2],
-                       a.base[fb * 4 + 3], sh[0], sh[1], tox, toy);
-    }
     const int fbj = pj >= 0 ? a.pb[pj] : fa;
-    const int tx0 = tx * T + tox, ty0 = ty * T + toy;
-    const size_t ipitch = (size_t)a.img_pitch / sizeof(float);  // the packed planes share the pixel pitch
-    const float* bimg = a.frames[fbj];
-    const unsigned* bx = PREC == ISA_XC_FP32 ? nullptr : xp.frames[fbj];
-    // LDS offset of this lane's A operand for each row block: row s = mb*16 + (lane % 16)
-    int aoff[MB];
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-        const int s = mb * 16 + j;
-        aoff[mb] = s < SS ? (S - 1 - s / S) * RW + (S - 1 - (s - (s / S) * S)) : -1;  // -1: padding row (zeros)
-    }
-    const int zero_row = RH * RW;
-    f32x4 acc[MB];
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) acc[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int NB = L.Bp / 16;  // <= kMaxNB (checked by the launcher)
+    const int tx0 = s_bx[j], ty0 = s_by[j];
     const bool cols_in = tx0 >= 0 && tx0 + L.Bp <= a.w;
-    int colx[kMaxNB][4];  // image column of each operand element of this lane
+    f32x4 tot[MB];
 #pragma unroll
-    for (int nb = 0; nb < kMaxNB; ++nb)
+    for (int mb = 0; mb < MB; ++mb) tot[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (SPLIT) {
+        // unit (row pair p, column block nb): reduction slot 8 q + 4 k + e <-> track pixel (row 2 p + k,
+        // column 16 nb + 4 q + e), k in {0, 1}, e in 0..3; lane (j, q) holds B[slot][pair j] and
+        // A[lag 16 mb + j][slot], so that the four lanes of a pair load 64 contiguous bytes of a packed
+        // row per instruction. Padding lags (16 mb + j >= S*S) read lag 0's operands: their rows of C are
+        // never used. Wave w takes a contiguous run of the NP * NB units.
+        int aoff[MB];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int ux = 16 * nb + 4 * q + e;  // a lane's four operand words: consecutive columns
-            colx[nb][e] = cols_in ? tx0 + ux : clampi(tx0 + ux, 0, a.w - 1);
+        for (int mb = 0; mb < MB; ++mb) {
+            const int s = mb * 16 + j < SS ? mb * 16 + j : 0, iy = s / S, ix = s - (s / S) * S;
+            aoff[mb] = (RO - iy) * RW + CO - ix + 4 * q;
         }
-    // operand words of one track row (packed split or fp32 bits); the wave's next row is fetched
-    // while the current one is multiplied
-    unsigned cur[kMaxNB][4], nxt[kMaxNB][4];
-    // (block inside the image: a lane's four words are consecutive columns, one 16-byte load at a
-    // 4-byte aligned address)
-    const unsigned* bw = PREC == ISA_XC_FP32 ? (const unsigned*)bimg : bx;
-    auto fetch_row = [&](int uy) {
-        const size_t ro = (size_t)clampi(ty0 + min(uy, B - 1), 0, a.h - 1) * ipitch;
+        gu32* bpl = (gu32*)xp.frames[fbj];
+        const int cx0 = tx0 + 4 * q;
+        const int NU = NP * NB, UPW = (NU + W - 1) / W;
+        const int u0 = min(wave * UPW, NU), u1 = min(u0 + UPW, NU);
+        f32x4 xs[MB], hh[MB];  // products with a lo part; hi.hi products (a chain per row pair)
 #pragma unroll
-        for (int nb = 0; nb < kMaxNB; ++nb) {
-            if (cols_in) {
-                u32x4a4 v = {0u, 0u, 0u, 0u};
-                if (pj >= 0 && nb < NB) v = *(const u32x4a4*)(bw + ro + colx[nb][0]);
-                nxt[nb][0] = v.x;
-                nxt[nb][1] = v.y;
-                nxt[nb][2] = v.z;
-                nxt[nb][3] = v.w;
-                continue;
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                unsigned v = 0;
-                if (pj >= 0 && nb < NB)
-                    v = PREC == ISA_XC_FP32 ? __float_as_uint(bimg[ro + colx[nb][e]]) : bx[ro + colx[nb][e]];
-                nxt[nb][e] = v;
-            }
+        for (int mb = 0; mb < MB; ++mb) {
+            xs[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            hh[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
-    };
-    fetch_row(wave);
-    for (int uy = wave; uy < B; uy += kWaves) {
+        // the loop in two versions: every pair's block columns inside the image (16-byte loads), or
+        // some at the edge (clamped columns, element by element); the choice is the same for all waves
+        auto products = [&](auto edge_t) {
+            constexpr bool EDGE = decltype(edge_t)::value;
+            u32x4 r0[kRing], r1[kRing];  // packed words of the unit's two rows
+            // operands of unit (p, nb) (zero for an empty pair slot); rows past the block repeat its last
+            // row and meet zero rows of A, columns past it meet zero columns of A
+            auto fetch = [&](int p, int nb, int slot) {
+                u32x4 w0 = {0u, 0u, 0u, 0u}, w1 = {0u, 0u, 0u, 0u};
+                if (pj >= 0) {
+                    gu32* row0 = bpl + (size_t)clampi(ty0 + min(2 * p, B - 1), 0, a.h - 1) * P;
+                    gu32* row1 = bpl + (size_t)clampi(ty0 + min(2 * p + 1, B - 1), 0, a.h - 1) * P;
+                    if (!EDGE) {
+                        w0 = *(gu32x4*)(row0 + cx0 + 16 * nb);
+                        w1 = *(gu32x4*)(row1 + cx0 + 16 * nb);
+                    } else {
+                        int x0 = cx0 + 16 * nb;
+                        asm volatile("" : "+v"(x0));  // keeps the clamped columns from being hoisted and held
 #pragma unroll
-        for (int nb = 0; nb < kMaxNB; ++nb)
+                        for (int e = 0; e < 4; ++e) {
+                            const int x = clampi(x0 + e, 0, a.w - 1);
+                            w0[e] = row0[x];
+                            w1[e] = row1[x];
+                        }
+                    }
+                }
+                r0[slot] = w0;
+                r1[slot] = w1;
+            };
+            // (p, nb) of the unit being multiplied and of the next one to fetch
+            int fp = u0 / NB, fnb = u0 - fp * NB;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) cur[nb][e] = nxt[nb][e];
-        if (uy + kWaves < B) fetch_row(uy + kWaves);
+            for (int k = 0; k < kRing; ++k)
+                if (u0 + k < u1) {
+                    fetch(fp, fnb, k);
+                    if (++fnb == NB) fnb = 0, ++fp;
+                }
+            fill_planes();
+            int cpr = u0 / NB, cnb = u0 - cpr * NB;
+            for (int u = u0; u < u1; u += kRing) {
 #pragma unroll
-        for (int nb = 0; nb < kMaxNB; ++nb) {
-            if (nb >= NB) break;
-            const int ux0 = 16 * nb;
-            if (PREC == ISA_XC_FP32) {
-                // product kk takes column ux0 + 4 q + kk of every lane quad q (the K slices of the four
-                // products cover the 16 columns once)
-                u32x4a4 av[MB];
+                for (int k = 0; k < kRing; ++k) {
+                    if (u + k >= u1) break;
+                    const u32x4 w0 = r0[k], w1 = r1[k];
+                    const u32x4 ch = {halves(w0.y, w0.x, 0x05040100u), halves(w0.w, w0.z, 0x05040100u),
+                                      halves(w1.y, w1.x, 0x05040100u), halves(w1.w, w1.z, 0x05040100u)};
+                    const u32x4 cl = {halves(w0.y, w0.x, 0x07060302u), halves(w0.w, w0.z, 0x07060302u),
+                                      halves(w1.y, w1.x, 0x07060302u), halves(w1.w, w1.z, 0x07060302u)};
+                    if (u + k + kRing < u1) {
+                        fetch(fp, fnb, k);
+                        if (++fnb == NB) fnb = 0, ++fp;
+                    }
+                    const unsigned* arow = hiW + 2 * cpr * RW + 16 * cnb;
 #pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
-                    av[mb] = *(const u32x4a4*)(pu + (aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux0 + 4 * q);
+                    for (int mb = 0; mb < MB; ++mb) {
+                        const unsigned* ap = arow + aoff[mb];
+                        const u32x4 ah = {ap[0], ap[2], ap[RW], ap[RW + 2]};
+                        const u32x4 al = {ap[L.PE], ap[L.PE + 2], ap[L.PE + RW], ap[L.PE + RW + 2]};
+                        f32x4& hacc = kFresh ? hh[mb] : tot[mb];
+                        f32x4& lacc = kSepLo ? xs[mb] : hacc;
+                        hacc = mfma32<PREC>(ah, ch, hacc);
+                        lacc = mfma32<PREC>(al, ch, lacc);
+                        lacc = mfma32<PREC>(ah, cl, lacc);
+                    }
+                    if (++cnb == NB) {
+                        cnb = 0;
+                        ++cpr;
+                        if (kFresh)
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const float bv = __uint_as_float(cur[nb][kk]);
+                            for (int mb = 0; mb < MB; ++mb) {
+                                tot[mb] += hh[mb];
+                                hh[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                            }
+                    }
+                }
+            }
+        };
+        if (__all(cols_in)) products(std::false_type{});
+        else products(std::true_type{});
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            if (kFresh) tot[mb] += hh[mb];  // the run's last row pair, when it ends inside one
+            if (kSepLo) tot[mb] += xs[mb];
+        }
+    } else {
+        // fp32: one track row per step; product kk takes column 16 nb + 4 q + kk in slot q, of A and B
+        int aoff[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            const int s = mb * 16 + j < SS ? mb * 16 + j : 0, iy = s / S, ix = s - (s / S) * S;
+            aoff[mb] = (RO - iy) * RW + CO - ix + 4 * q;
+        }
+        gf32* bimg = (gf32*)a.frames[fbj];
+        float cur[kMaxNB][4], nxt[kMaxNB][4];
+        auto fetch_row = [&](int uy) {
+            gf32* row = bimg + (size_t)clampi(ty0 + min(uy, B - 1), 0, a.h - 1) * P;
+#pragma unroll
+            for (int nb = 0; nb < kMaxNB; ++nb) {
+                f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (pj >= 0 && nb < NB) {
+                    if (cols_in) {
+                        v = *(gf32x4*)(row + tx0 + 16 * nb + 4 * q);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) v[e] = row[clampi(tx0 + 16 * nb + 4 * q + e, 0, a.w - 1)];
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) nxt[nb][e] = v[e];
+            }
+        };
+        if (wave < B) fetch_row(wave);
+        fill_planes();
+        for (int uy = wave; uy < B; uy += W) {
+#pragma unroll
+            for (int nb = 0; nb < kMaxNB; ++nb)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) cur[nb][e] = nxt[nb][e];
+            if (uy + W < B) fetch_row(uy + W);
+            f32x4 racc[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) racc[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int nb = 0; nb < kMaxNB; ++nb) {
+                if (nb >= NB) break;
+                f32x4 av[MB];
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) av[mb] = *(const f32x4a4*)(pf + aoff[mb] + uy * RW + 16 * nb);
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
                     for (int mb = 0; mb < MB; ++mb)
-                        acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(av[mb][kk]), bv, acc[mb], 0, 0, 0);
-                }
-            } else {
-                u32x2 bh, bl;
-                unpack4(cur[nb][0], cur[nb][1], cur[nb][2], cur[nb][3], bh, bl);
-                // all A words of this column block first, so the LDS reads overlap the products
-                u32x4a4 aw[MB];
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
-                    aw[mb] = *(const u32x4a4*)(pu + (aoff[mb] >= 0 ? aoff[mb] + uy * RW : zero_row) + ux0 + 4 * q);
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb) {
-                    u32x2 ah, al;
-                    unpack4(aw[mb][0], aw[mb][1], aw[mb][2], aw[mb][3], ah, al);
-                    acc[mb] = mfma16<PREC>(al, bh, acc[mb]);
-                    acc[mb] = mfma16<PREC>(ah, bl, acc[mb]);
-                    acc[mb] = mfma16<PREC>(ah, bh, acc[mb]);
-                }
+                        racc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb][kk], cur[nb][kk], racc[mb], 0, 0, 0);
             }
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) tot[mb] += racc[mb];
         }
     }
-    // ---- 3. combine the waves' partial products in the fixed order ((w0 + w1) + w2) + w3: wave w adds
-    // its own to the running sum in LDS, one wave after the other; lane holds C[4q+i][j]
+    // ---- 3. combine the waves' partial products in a fixed tree: waves W/2 .. W-1 park theirs in LDS,
+    // wave w < W/2 adds wave w + W/2's to its own, and C = ((c0 + c1) + c2) + ... over those W/2 sums;
+    // lane (j, q) holds C[4q+i][j] of each row block
+    constexpr int HW = W / 2;
+    float* red = (float*)(smem + L.off_red);
     float* Cm = (float*)(smem + L.off_c);
-    const float inv_sc = 1.0f / sc;
-    __syncthreads();  // every wave is done with the reference plane, whose bytes are reused below
-    for (int w = 0; w < kWaves; ++w) {
-        if (wave == w) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's direct loads into LDS have landed
+    __syncthreads();  // every wave is done with the reference planes, whose bytes are reused below
+    if (wave >= HW)
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int e = mb * 256 + (4 * q + i) * 16 + j, s = mb * 16 + 4 * q + i;
-                    const float v = w == 0 ? acc[mb][i] : red[e] + acc[mb][i];
-                    if (w + 1 < kWaves) red[e] = v;
-                    else if (s < SS) Cm[j * SS + s] = v * inv_sc;
-                }
-        }
-        __syncthreads();  // the last one also ends the reads of red, whose bytes the column energies reuse
+            for (int i = 0; i < 4; ++i) red[((wave - HW) * MB + mb) * 256 + (4 * q + i) * 16 + j] = tot[mb][i];
+    __syncthreads();
+    if (wave < HW)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float* e = red + (wave * MB + mb) * 256 + (4 * q + i) * 16 + j;
+                *e = tot[mb][i] + *e;
+            }
+    __syncthreads();
+    for (int e = tid; e < MB * 256; e += NT) {
+        const int s = (e >> 8) * 16 + ((e >> 4) & 15), jc = e & 15;
+        float v = red[e];
+        for (int w = 1; w < HW; ++w) v += red[w * MB * 256 + e];
+        if (s < SS) Cm[jc * SS + s] = v;
     }
-    // ---- 4. window energies of each pair's track block: per column, the S row windows (a T-row
-    // sum, then slid one row at a time), all loads of a column issued in batches of 8. rmax[jj] gets
-    // the largest window energy any entry of pair jj was slid through (for the certificate).
-    float* colE = (float*)(smem + L.off_col);
-    unsigned* rmax = (unsigned*)(smem + L.off_misc + 64);
-    for (int task = tid; task < kPairsPerChunk * B; task += NT) {
-        const int jj = task / B, c = task - jj * B;
-        const int pp = cp[jj];
-        if (pp < 0) continue;
-        const int fb = a.pb[pp];
-        const float* sh = a.shifts + (size_t)a.pk[pp] * a.meas_stride + (size_t)tile_idx * 2;
-        int ox, oy;
-        tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2],
-                       a.base[fb * 4 + 3], sh[0], sh[1], ox, oy);
-        const int bx0 = tx * T + ox, by0 = ty * T + oy;
-        float* ce = colE + ((size_t)jj * B + c) * S;
-        float rm;
-        if (bx0 >= 0 && by0 >= 0 && bx0 + B <= a.w && by0 + B <= a.h) {
-            // block inside the image: the S row windows of this column are entries of the
-            // vertical window-energy image
-            const float* vb = xp.vbox[fb] + (size_t)by0 * a.w + bx0 + c;
-#pragma unroll
-            for (int k = 0; k < S; ++k) ce[k] = vb[(size_t)k * a.w];
-            const float* vr = xp.vrun[fb] + bx0 + c;
-            rm = fmaxf(vr[(size_t)(by0 / kVRun) * a.w], vr[(size_t)((by0 + S - 1) / kVRun) * a.w]);
-        } else {
-            // the exact search's column energies (FP64 chain, then slid), rows and columns clamped
-            const float* img = a.frames[fb];
-            const int gx = clampi(bx0 + c, 0, a.w - 1), gy0 = by0;
+    __syncthreads();
+    // ---- 4. column energies of the track blocks that leave the image (rows and columns clamped, as
+    // the exact search reads them): per column, the S row windows (a T-row FP64 sum, then slid one row
+    // at a time), loads issued in batches of 8; s_rmax_edge gets the block's largest column energy.
+    if (s_any_edge) {
+        for (int task = tid; task < kPairsPerChunk * B; task += NT) {
+            const int jj = task / B, c = task - jj * B;
+            const int pp = cp[jj];
+            if (pp < 0 || s_in[jj]) continue;
+            const float* img = a.frames[a.pb[pp]];
+            const int gx = clampi(s_bx[jj] + c, 0, a.w - 1), gy0 = s_by[jj];
             auto px = [&](int y) { return (double)row_ptr(img, a.img_pitch, clampi(gy0 + y, 0, a.h - 1))[gx]; };
-            double e = 0.0, eb = 0.0;
+            double e = 0.0;
             for (int y0 = 0; y0 < T; y0 += 8) {
                 double v[8];
 #pragma unroll
@@ -430,8 +648,9 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
 #pragma unroll
                 for (int k = 0; k < 8; ++k) e = fma(v[k], v[k], e);
             }
+            float* ce = colE + (size_t)jj * S * Bq + c;
             ce[0] = (float)e;
-            eb = e;
+            double eb = e;
             double u[S - 1], v[S - 1];
 #pragma unroll
             for (int k = 0; k + 1 < S; ++k) {
@@ -441,71 +660,120 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
 #pragma unroll
             for (int k = 0; k + 1 < S; ++k) {
                 e = (e - u[k] * u[k]) + v[k] * v[k];
-                eb += v[k] * v[k];  // energy of the whole column block bounds every window
-                ce[k + 1] = (float)e;
+                eb += v[k] * v[k];  // energy of the whole column of the block bounds every window
+                ce[(size_t)(k + 1) * Bq] = (float)e;
             }
-            rm = (float)(eb * (1.0 + 0x1p-20));
+            atomicMax(&s_rmax_edge[jj], __float_as_uint((float)(eb * (1.0 + 0x1p-20))));  // non-negative floats order as their bits
         }
-        atomicMax(&rmax[jj], __float_as_uint(rm));  // non-negative floats order as their bits
+        __syncthreads();
+    }
+    // ---- 5. window energies TT(iy, ix) = sum over the T columns ix .. ix + T - 1 of the column energies
+    // of lag row iy: four lanes per row sum a quarter of the first window each (FP64), combined as
+    // (q0 + q1) + (q2 + q3), then lane 0 slides the window one column at a time.
+    double* TT = (double*)(smem + L.off_tt);
+    float* EB = (float*)(smem + L.off_eb);  // [pair][iy]: sum of the row's B column energies, rounded up
+    float* RM = EB + kPairsPerChunk * S;    // [pair][iy]: slide maximum of its column energies
+    {
+        const int sub = tid & 3, Q = (T + 3) / 4;
+        const int c_beg = min(sub * Q, T), c_end = min(c_beg + Q, T);
+        for (int r0 = 0; r0 < kPairsPerChunk * S; r0 += NT / 4) {  // the same trip count in every lane
+            const int r = r0 + (tid >> 2);
+            const int jj = r / S, iy = r - (r / S) * S;
+            const bool on = r < kPairsPerChunk * S && cp[jj] >= 0;
+            const float* cr = colE + (size_t)r * Bq;
+            double part = 0.0;
+            if (on)
+                for (int c = c_beg; c < c_end; ++c) part += (double)cr[c];
+            part += __shfl_xor(part, 1);  // q0 + q1 (q2 + q3): the same sum in both lanes
+            part += __shfl_xor(part, 2);
+            if (on && sub == 0) {
+                double* tt = TT + jj * SS + iy * S;
+                double t = part, rest = 0.0;
+                tt[0] = t;
+                for (int c = T; c < B; ++c) rest += (double)cr[c];
+                for (int ix = 1; ix < S; ++ix) {
+                    t = (t - (double)cr[ix - 1]) + (double)cr[ix - 1 + T];
+                    tt[ix] = t;
+                }
+                EB[jj * S + iy] = (float)((part + rest) * (1.0 + 0x1p-20));
+                float rm = __uint_as_float(s_rmax_edge[jj]);  // blocks at the edge
+                if (s_in[jj]) {
+                    const int bx0 = s_bx[jj], ncr = (bx0 + B - 1) / kCB - bx0 / kCB + 1;
+                    for (int k = 0; k < ncr; ++k) rm = fmaxf(rm, vr_s[r * 4 + k]);
+                }
+                RM[jj * S + iy] = rm;
+            }
+        }
     }
     __syncthreads();
-    // ---- 5. D(s) = sum r^2 + sum_x colE[x + ix][iy] - 2 C(s), and the half-width W(s) of an interval
-    // holding both D(s) and the exact search's value of it:
+    // ---- 6. D(s) = sum r^2 + TT(s) - 2 C(s), and the half-width W(s) of an interval holding both D(s)
+    // and the exact search's value of it:
     //  - cross terms: each product of the split operands differs from r t by at most c_split |r t|
-    //    (plus, in fp16, terms from the subnormal spacing); a wave's accumulator receives hgt - 3
-    //    products (or fewer, whatever order and grouping the matrix cores use inside an
-    //    instruction) and the four waves' sums are added in fp32, so any summation tree has height
-    //    <= hgt and the fp32 error is at most gamma(hgt) = hgt u / (1 - hgt u) (u = 2^-23: every
-    //    rounding faithful) times the sum of |products|; sum |r(u - s) t(u)| <= sqrt(R2 TT(s));
-    //  - window energies: rounding to float (2^-24), the FP64 slide (T + 2 kVRun) 2^-53 rmax per
-    //    column and the FP64 sum over the T columns;
+    //    (plus, in fp16, terms from the subnormal spacing). Whatever order and grouping the matrix cores
+    //    use inside an instruction, a product passes through at most K roundings there (K products per
+    //    instruction) and through every later instruction of its accumulator's chain; the chains of the
+    //    hi.hi products restart at every track row pair (row, in fp32) and are added into the wave's
+    //    total in fp32, the products with a lo part (at most 2^-8 |r t| in bf16, 2^-11 in fp16) keep one
+    //    chain, and the wave totals are added in fp32 (a tree of depth W/2). With h the longest path, the fp32 error is
+    //    at most gamma(h) = h u / (1 - h u) (u = 2^-23: every rounding faithful) times the sum of
+    //    |products| of the chain; sum |r(u - s) t(u)| <= sqrt(R2 TT(s));
+    //  - window energies: rounding of the column energies to float, their FP64 slide, bounded by
+    //    (T + 2 kVRun) 2^-53 times the slide maxima, and the FP64 sum and slide along the row, bounded
+    //    by (T + 2 S) 2^-53 times the sum of the row's column energies;
+    //  - sum r^2: the FP64 sums of the T^2 squares in two orders;
     //  - the FP64 evaluation of R2 + TT - 2C on both sides and the rounding of D to float on both.
     float* Dm = (float*)(smem + L.off_d);
     float* Wm = (float*)(smem + L.off_w);
-    const double R2 = *R2s;
-    constexpr int kProd = PREC == ISA_XC_FP32 ? 4 : 16;   // products per matrix instruction
-    constexpr int kPerBlk = PREC == ISA_XC_FP32 ? 4 : 3;  // instructions per 16-column block and row
-    const int hgt = kProd * ((B + kWaves - 1) / kWaves) * NB * kPerBlk + (kWaves - 1) + (PREC == ISA_XC_FP32 ? 1 : 0);
-    const float gam = (float)(hgt * 0x1p-23 / (1.0 - hgt * 0x1p-23));
+    const double R2 = s_R2;
+    // longest paths of a product through the fp32 roundings: h_main for the hi.hi products (all
+    // products when kSepLo is false), h_lo for the products with a lo part in their own chain. A wave's
+    // run of UPW units spans at most ceil(UPW / NB) + 1 row pairs; its total then receives the lo chain
+    // and passes through at most W/2 additions of the wave combine.
+    const int UPW = (NP * NB + W - 1) / W, NRw = (B + W - 1) / W, NRP = (UPW + NB - 1) / NB + 1;
+    const int h_main = !SPLIT ? 16 * NB + NRw + HW + 2
+                              : (kFresh ? (kSepLo ? 32 : 96) * NB + NRP + HW + 1 : 96 * UPW + HW);
+    const int h_lo = 64 * UPW + HW + 1;
+    auto gam = [](int hgt) { return (float)(hgt * 0x1p-23 / (1.0 - hgt * 0x1p-23)); };
+    const float lo_rel = PREC == ISA_XC_BF16X3 ? 0x1p-8f : 0x1p-11f;
     const float csplit = PREC == ISA_XC_BF16X3 ? 3.1f * 0x1p-16f : (PREC == ISA_XC_FP16X3 ? 3.1f * 0x1p-22f : 0.0f);
     const float eta = 0x1p-24f;  // fp16: bound on the absolute error of a split from the subnormal spacing
     // W is evaluated in fp32: every quantity below is rounded up by a relative 2^-20 or more, which
     // covers the roundings of the fp32 evaluation (a few 2^-24 each)
     const float sr2 = fmaxf((float)(sqrt(R2) * (1.0 + 0x1p-20)), R2 > 0.0 ? 1.17549435e-38f : 0.0f);
     const float r2f = (float)(R2 * (1.0 + 0x1p-20));
-    const float kslide = (float)((double)T * (T + 2 * kVRun) * 0x1p-53 * 1.01);
-    const float crel = (csplit + 1.03f * gam) * (1.0f + 0x1p-18f);
+    const float kvT = (float)((double)T * (T + 2 * kVRun) * 0x1p-53 * 1.01);
+    const float kh = (float)((T + 2 * S) * 0x1p-52);
+    const float er2 = (float)((double)T * T * 0x1p-52) * r2f;
+    const float crel = (csplit + (SPLIT && !kSepLo ? 1.03f + 2.06f * lo_rel : 1.03f) * gam(h_main) +
+                        (SPLIT && kSepLo ? 2.06f * lo_rel * gam(h_lo) : 0.0f)) *
+                       (1.0f + 0x1p-18f);
     for (int task = tid; task < kPairsPerChunk * SS; task += NT) {
         const int jj = task / SS, s = task - jj * SS;
         if (cp[jj] < 0) continue;
-        const int iy = s / S, ix = s - iy * S;
-        const float* ce = colE + ((size_t)jj * B + ix) * S + iy;
-        double tt = 0.0;
-        for (int x = 0; x < T; ++x) tt += (double)ce[(size_t)x * S];
+        const int iy = s / S;
+        const double tt = TT[jj * SS + s];
         const double dd = R2 + tt - 2.0 * (double)Cm[jj * SS + s];
         Dm[jj * SS + s] = (float)dd;
-        const float ttf = (float)tt * (1.0f + 0x1p-20f), rm = __uint_as_float(rmax[jj]);
-        const float ett = 0x1p-23f * ttf + kslide * rm + (float)T * 0x1p-52f * ttf;
+        const float ttf = (float)tt * (1.0f + 0x1p-20f), ebf = EB[jj * S + iy], rm = RM[jj * S + iy];
+        const float ett = 0x1p-23f * ttf + kvT * rm + kh * ebf;
         const float ttu = ttf + ett;
-        const float P = sr2 * sqrtf(ttu) * (1.0f + 0x1p-20f);
-        float ec = crel * P;
-        if (PREC == ISA_XC_FP16X3) ec += (1.1f * eta * T * (sc * sr2 + sqrtf(ttu)) + 3.0f * eta * eta * T * T) / sc;
-        const float wd = 1.06f * (2.0f * ec + ett + 0x1p-44f * (r2f + ttf + 2.0f * P) + 0x1p-22f * fabsf((float)dd)) + 1e-30f;
+        const float Pq = sr2 * sqrtf(ttu) * (1.0f + 0x1p-20f);
+        float ec = crel * Pq;
+        if (PREC == ISA_XC_FP16X3) ec += 1.1f * eta * T * (sr2 + sqrtf(ttu)) + 3.0f * eta * eta * T * T;
+        const float wd =
+            1.06f * (2.0f * ec + ett + er2 + 0x1p-44f * (r2f + ttf + 2.0f * Pq) + 0x1p-22f * fabsf((float)dd)) + 1e-30f;
         Wm[jj * SS + s] = wd;
     }
     __syncthreads();
-    // ---- 6. findMinimum per pair, when the intervals decide the integer minimum and the threshold
+    // ---- 7. findMinimum per pair, when the intervals decide the integer minimum and the threshold
     // test as the exact search would; otherwise the item is queued for the exact search. Sixteen lanes
     // per pair (aligned groups within a wave) scan the lags and combine by shuffles.
     // Undecided items are collected per block and appended with one atomic on the global count (one per
     // item serialised the appends on that address).
-    __shared__ int q_n, q_base, q_pair[kPairsPerChunk];
-    if (tid == 0) q_n = 0;
-    __syncthreads();
     {
-        static_assert(kPairsPerChunk * 16 == NT, "sixteen lanes per pair");
-        const int jj = tid >> 4, l = tid & 15;
-        const int pp = cp[jj];
+        static_assert(kPairsPerChunk * 16 <= NT, "sixteen lanes per pair");
+        const int jj = (tid >> 4) & (kPairsPerChunk - 1), l = tid & 15;
+        const int pp = tid < kPairsPerChunk * 16 ? cp[jj] : -1;
         const float* D = Dm + jj * SS;
         const float* W = Wm + jj * SS;
         unsigned long long key = ~0ull;
@@ -541,7 +809,7 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
             hmax = fmax(hmax, __shfl_xor(hmax, o, 16));
         }
         if (pp >= 0 && l == 0) {
-            float* sh = a.shifts + (size_t)a.pk[pp] * a.meas_stride + (size_t)tile_idx * 2;
+            float* sh = s_sh[jj];
             if (a.shift_images)
 #pragma unroll 1
                 for (int s = 0; s < SS; ++s) a.shift_images[((size_t)pp * a.cx * a.cy + tile_idx) * SS + s] = D[s];
@@ -554,9 +822,8 @@ __global__ __launch_bounds__(kWaves * 64, MfmaWaves<M>::v) void k_track_mfma(Tra
                 float cx, cy;
                 const int dec = find_min_fit(D, M, mi, mn, mx, a.threshold, cx, cy);
                 if (a.decisions) a.decisions[(size_t)pp * a.cx * a.cy + tile_idx] = dec;
-                const float psx = sh[0], psy = sh[1];
-                sh[0] = psx + cx;
-                sh[1] = psy + cy;
+                sh[0] = s_ps[jj][0] + cx;
+                sh[1] = s_ps[jj][1] + cy;
             } else {
                 q_pair[atomicAdd(&q_n, 1)] = pp;
             }
@@ -599,11 +866,14 @@ int track_chunks(const int* pa, int m, int n, int* out_chunks) {
 
 int track_chunks_capacity(int m, int n) { return m / kPairsPerChunk + n + 1; }
 
+// LDS the kernel declares statically, next to its dynamic layout
+constexpr size_t kMfmaStaticLds = 1024;
+
 template <int PREC, int M>
 static int launch_mfma_pm(const TrackArgs& a, XPlanes xp, const int* chunks, int n_chunks, hipStream_t s) {
     const MfmaLayout L = mfma_layout(a.tile, M);
     const int blocks = a.cx * a.cy * n_chunks;
-    k_track_mfma<PREC, M><<<8 * div_up(blocks, 8), kWaves * 64, L.bytes, s>>>(a, xp, chunks, n_chunks, blocks);
+    k_track_mfma<PREC, M><<<8 * div_up(blocks, 8), MfmaCfg<M>::waves * 64, L.bytes, s>>>(a, xp, chunks, n_chunks, blocks);
     return check_launch("track_mfma");
 }
 
@@ -623,18 +893,19 @@ static int launch_mfma_p(const TrackArgs& a, XPlanes xp, int M, const int* chunk
     return ISA_ERR_UNSUPPORTED;
 }
 
-// Packed (hi | lo << 16) split planes of count pixels (nothing to do for ISA_XC_FP32).
-int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s) {
-    const unsigned blocks = (unsigned)((count + 1023) / 1024);  // four elements per thread
-    const bool vec = ((size_t)src & 15) == 0 && ((size_t)dst & 15) == 0;
-    if (precision == ISA_XC_BF16X3) k_split_planes<ISA_XC_BF16X3><<<blocks, 256, 0, s>>>(src, dst, count, vec);
-    else if (precision == ISA_XC_FP16X3) k_split_planes<ISA_XC_FP16X3><<<blocks, 256, 0, s>>>(src, dst, count, vec);
+// Packed split planes of a w x h level image of row pitch P floats into dst (h x P 32-bit words, word
+// y P + x = hi | lo << 16 of pixel (x, y)); nothing to do for ISA_XC_FP32.
+int split_planes(int precision, const float* src, int P, int w, int h, unsigned* dst, hipStream_t s) {
+    if (w <= 0 || h <= 0) return ISA_OK;
+    const dim3 grid((unsigned)div_up(w, 1024), (unsigned)h);  // four pixels per thread
+    if (precision == ISA_XC_BF16X3) k_split_rows<ISA_XC_BF16X3><<<grid, 256, 0, s>>>(src, P, w, dst);
+    else if (precision == ISA_XC_FP16X3) k_split_rows<ISA_XC_FP16X3><<<grid, 256, 0, s>>>(src, P, w, dst);
     else return ISA_OK;
     return check_launch("split_planes");
 }
 
-// Vertical T-window energies of a level image (count of window starts: h - T + 1 rows of w).
-// runmax: vbox_run_rows(h, T) x w floats.
+// Vertical T-window energies of a level image (count of window starts: h - T + 1 rows of w) and their
+// slide maxima (vbox_run_rows(h, T) x ceil(w / 64) floats, fewer than the same rows of w).
 int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax, hipStream_t s) {
     if (h < T) return ISA_OK;
     k_vbox_sq<<<dim3(div_up(w, 256), div_up(h - T + 1, kVRun)), 256, 0, s>>>(src, pitch, w, h, T, dst, runmax);
@@ -645,11 +916,11 @@ int vbox_run_rows(int h, int T) { return h < T ? 0 : div_up(h - T + 1, kVRun); }
 
 bool track_mfma_supported(int tile, int M) {
     const MfmaLayout L = mfma_layout(tile, M);
-    return M >= 1 && M <= 8 && L.Bp / 16 <= kMaxNB && L.bytes <= 160 * 1024;
+    return M >= 1 && M <= 8 && L.Bp / 16 <= kMaxNB && L.bytes + kMfmaStaticLds <= 160 * 1024;
 }
 
-// xplanes: per frame, the packed split plane of the level image (same pixel pitch; unused for fp32);
-// vbox, vrun: per frame, vbox_energies of the level image (dense, row length w) and its run maxima.
+// xplanes: per frame, the split planes of the level image (split_planes; unused for fp32);
+// vbox, vrun: per frame, vbox_energies of the level image (dense, row length w) and its slide maxima.
 // Items whose integer minimum and threshold test the error bounds cannot decide are appended to
 // queue (2 ints each; room for every (pair, tile) of the level) and counted in *count, which the
 // caller clears; launch_track_queue then searches them exactly.

@@ -58,7 +58,7 @@ int launch_track_mfma(int precision, const float* const* frames, const unsigned*
                       int* decisions, int* queue, int* count, hipStream_t s);
 int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax, hipStream_t s);
 int vbox_run_rows(int h, int T);
-int split_planes(int precision, const float* src, unsigned* dst, size_t count, hipStream_t s);
+int split_planes(int precision, const float* src, int P, int w, int h, unsigned* dst, hipStream_t s);
 bool track_mfma_supported(int tile, int M);
 int build_tap_tables(void* tables, int w, int h, int quant, hipStream_t s);
 int launch_subsample3(const uint16_t* raw, float* out, float max_val, int hw, int hh, int out_pitch, const int cfa[4],
@@ -511,6 +511,8 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
     // the minimizer's first iteration (all rows active) is the same for every tile: factor it here
     const bool have_factor = isa::minimize_full_factor(rows.data(), n, m, (double*)(th + tl.lfull));
     const int n_chunks = isa::track_chunks(pa, m, n, (int*)(th + tl.chunks));
+    std::vector<char> tracked(n, 0);  // frames some pair tracks (the matrix-core search reads their window energies)
+    for (int p = 0; p < m; ++p) tracked[pb[p]] = 1;
     float* base = (float*)(th + tl.base);
     const float** fr = (const float**)(th + tl.frames);
     const unsigned** xfr = (const unsigned**)(th + tl.xframes);
@@ -603,14 +605,16 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
         if (M->xplanes)
             for (int f = 0; f < n; ++f)
                 ISA_TRY(isa::split_planes(P.xc_precision,
-                                          l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np,
-                                          M->xplanes + M->xplane_off[i] + (size_t)f * lw * lh, (size_t)lw * lh, s));
+                                          l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np, lw, lw,
+                                          lh, M->xplanes + M->xplane_off[i] + (size_t)f * lw * lh, s));
         if (M->vbox)
-            for (int f = 0; f < n; ++f)
+            for (int f = 0; f < n; ++f) {
+                if (!tracked[f]) continue;
                 ISA_TRY(isa::vbox_energies(l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np,
                                            lw * (int)sizeof(float), lw, lh, P.tile_size[i],
                                            M->vbox + M->vbox_off[i] + (size_t)f * M->vbox_len[i],
                                            M->vrun + M->vrun_off[i] + (size_t)f * M->vrun_len[i], s));
+            }
         (void)hipEventRecord(M->lev_ev[i][1], s);
         if (P.xc_precision == ISA_XC_FP64) {
             ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
