@@ -132,10 +132,11 @@ __global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, floa
 #pragma unroll
             for (int k = 0; k < kB; ++k) {
                 const int y = yb + k;
-                if (y >= y1) break;
-                dst[(size_t)y * w + x] = (float)e;
-                emax = fmax(emax, e);
-                if (y + T < h) e = (e - (double)a[k] * a[k]) + (double)b[k] * b[k];
+                if (y < y1) {
+                    dst[(size_t)y * w + x] = (float)e;
+                    emax = fmax(emax, e);
+                    if (y + T < h) e = (e - (double)a[k] * a[k]) + (double)b[k] * b[k];
+                }
             }
         }
     }
