@@ -293,23 +293,30 @@ __device__ __forceinline__ void track_cols_group(const TrackArgs& a, int pairs, 
     const int item = blk * ppb + pp;
     const bool live = item < items_total;
     const int it = live ? item : items_total - 1;
+    // the item's pair and tile, and its shift (re-derived after the row loop rather than held in
+    // registers across it)
+    auto decode = [&](int& p, int& tile_idx) {
+        if (queue) {
+            p = queue[2 * it];
+            tile_idx = queue[2 * it + 1];
+        } else {
+            tile_idx = it / pairs;
+            p = it - tile_idx * pairs;
+        }
+    };
+    auto shift_of = [&](int p, int tile_idx) {
+        return a.shifts + (size_t)a.pk[p] * a.meas_stride + (size_t)tile_idx * 2;
+    };
     int tile_idx, p;
-    if (queue) {
-        p = queue[2 * it];
-        tile_idx = queue[2 * it + 1];
-    } else {
-        tile_idx = it / pairs;
-        p = it - tile_idx * pairs;
-    }
+    decode(p, tile_idx);
     const int ty = tile_idx / a.cx, tx = tile_idx - ty * a.cx;
-    const int fa = a.pa[p], fb = a.pb[p], k = a.pk[p];
-    float* sh = a.shifts + (size_t)k * a.meas_stride + (size_t)tile_idx * 2;
-    const float psx = sh[0], psy = sh[1];
+    const int fa = a.pa[p], fb = a.pb[p];
+    const float* sh0 = shift_of(p, tile_idx);
     int rox, roy, tox, toy;
     tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fa * 4], a.base[fa * 4 + 1], a.base[fa * 4 + 2], a.base[fa * 4 + 3],
                    0.0f, 0.0f, rox, roy);
     tile_offset_sc(tx, ty, T, a.w, a.h, a.base[fb * 4], a.base[fb * 4 + 1], a.base[fb * 4 + 2], a.base[fb * 4 + 3],
-                   psx, psy, tox, toy);
+                   sh0[0], sh0[1], tox, toy);
     const float* ia = a.frames[fa];
     const float* ib = a.frames[fb];
     char* mine = smem + (size_t)pp * item_bytes;
@@ -477,9 +484,13 @@ __device__ __forceinline__ void track_cols_group(const TrackArgs& a, int pairs, 
     }
     __syncthreads();
     if (!live) return;
+    asm volatile("" ::: "memory");  // the item's pair, tile and shift are loaded again, not kept from before
+    decode(p, tile_idx);
+    float* sh = shift_of(p, tile_idx);
     if (a.shift_images)
         for (int s = lane; s < S * S; s += lpi) a.shift_images[((size_t)p * a.cx * a.cy + tile_idx) * S * S + s] = Dm[s];
     if (lane == 0) {
+        const float psx = sh[0], psy = sh[1];
         const unsigned long long km = *kmin;
         const int mi = km == ~0ull ? -1 : (int)(unsigned)km;
         const float mn = mi < 0 ? 3.402823466e+38f : Dm[mi];
