@@ -101,58 +101,71 @@ template <int PREC> __global__ void k_split_rows(const float* src, int P, int w,
     }
 }
 
-// Vertical window energies V(y0, x) = sum_{y = y0}^{y0 + T - 1} src(y, x)^2 for y0 in [0, h - T]:
-// one thread per column and run of kVRun window starts (a T-term sum, then slid in FP64), rows loaded
-// sixteen at a time. runmax gets, per run and block of kCB columns (one wave), the largest energy any
-// of its columns was slid through, rounded up: the squares are exact in FP64, so a window's error
-// before its rounding to float is at most (T + 2 kVRun) 2^-53 times that maximum. Blocks of 256 threads.
-__global__ void k_vbox_sq(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax) {
-    constexpr int kB = 16;  // rows per batch of loads
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y0 = blockIdx.y * kVRun, nv = h - T + 1;
-    double emax = 0.0;
-    if (x < w) {
-        auto at = [&](int y) { return row_ptr(src, pitch, min(y, h - 1))[x]; };
-        double e = 0.0;
-        for (int yb = y0; yb < y0 + T; yb += kB) {
-            float v[kB];
-#pragma unroll
-            for (int k = 0; k < kB; ++k) v[k] = yb + k < y0 + T ? at(yb + k) : 0.0f;
-#pragma unroll
-            for (int k = 0; k < kB; ++k) e += (double)v[k] * v[k];
+// Vertical window energies V(y0, x) = sum_{y = y0}^{y0 + T - 1} src(y, x)^2 for y0 in [0, h - T].
+// A block takes kCB columns and a run of kVRun window starts: it loads the kVRun + T - 1 rows the run
+// reads into LDS (16-byte loads, all in flight together), then four threads per column take a quarter
+// of the run each: a T-term FP64 sum, then slid one row at a time. runmax gets, per run and block of
+// kCB columns (one block), the largest energy any of its columns was slid through, rounded up: the
+// squares are exact in FP64, so a window's error before its rounding to float is at most
+// (T + 2 kVRun) 2^-53 times that maximum. Blocks of 256 threads; T <= kVTMax.
+constexpr int kVTMax = 128;
+__global__ __launch_bounds__(256) void k_vbox_sq(const float* src, int pitch, int w, int h, int T, float* dst,
+                                                 float* runmax) {
+    static_assert(kCB == 64 && kVRun == 64, "one column per lane, four quarters of the run per column");
+    extern __shared__ __align__(16) float Lds[];  // (kVRun + T - 1) rows of kCB columns
+    auto L = reinterpret_cast<float(*)[kCB]>(Lds);
+    __shared__ double wmax[4];
+    const int tid = threadIdx.x, c = tid & 63, quarter = tid >> 6;
+    const int x0 = blockIdx.x * kCB, y0 = blockIdx.y * kVRun, nv = h - T + 1;
+    const int rows = min(kVRun + T - 1, h - y0);  // rows the run reads (those past the image are never used)
+    const int P = pitch / (int)sizeof(float);
+    // rows y0 .. y0 + rows - 1, columns x0 .. x0 + 63: sixteen 16-byte pieces per row
+    const bool full = x0 + kCB <= w;
+    for (int i = tid; i < rows * 16; i += 256) {
+        const int r = i >> 4, k = (i & 15) * 4;
+        const float* g = src + (size_t)(y0 + r) * P + x0 + k;
+        f32x4a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (full) {
+            v = *(const f32x4a4*)g;
+        } else {
+            for (int e = 0; e < 4; ++e)
+                if (x0 + k + e < w) v[e] = g[e];
         }
-        const int y1 = min(y0 + kVRun, nv);
-        for (int yb = y0; yb < y1; yb += kB) {
-            float a[kB], b[kB];
-#pragma unroll
-            for (int k = 0; k < kB; ++k) {
-                a[k] = at(yb + k);
-                b[k] = at(yb + k + T);  // rows past the image are clamped and never used
-            }
-#pragma unroll
-            for (int k = 0; k < kB; ++k) {
-                const int y = yb + k;
-                if (y < y1) {
-                    dst[(size_t)y * w + x] = (float)e;
-                    emax = fmax(emax, e);
-                    if (y + T < h) e = (e - (double)a[k] * a[k]) + (double)b[k] * b[k];
-                }
+        *(f32x4a4*)&L[r][k] = v;
+    }
+    __syncthreads();
+    double emax = 0.0;
+    const int s0 = y0 + quarter * (kVRun / 4), s1 = min(s0 + kVRun / 4, nv);  // the quarter's window starts
+    if (x0 + c < w && s0 < s1) {
+        const int l0 = s0 - y0;
+        double e = 0.0;
+        for (int k = 0; k < T; ++k) {
+            const double v = (double)L[l0 + k][c];
+            e += v * v;
+        }
+        for (int y = s0; y < s1; ++y) {
+            dst[(size_t)y * w + x0 + c] = (float)e;
+            emax = fmax(emax, e);
+            if (y + T < h && y + 1 < s1) {
+                const double a = (double)L[y - y0][c], b = (double)L[y - y0 + T][c];
+                e = (e - a * a) + b * b;
             }
         }
     }
-    float m = (float)(emax * (1.0 + 0x1p-20));
-    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0 && x < w) runmax[(size_t)blockIdx.y * ((w + kCB - 1) / kCB) + x / kCB] = m;
+    for (int off = 32; off >= 1; off >>= 1) emax = fmax(emax, __shfl_xor(emax, off));
+    if (c == 0) wmax[quarter] = emax;
+    __syncthreads();
+    if (tid == 0) {
+        const double m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        runmax[(size_t)blockIdx.y * ((w + kCB - 1) / kCB) + blockIdx.x] = (float)(m * (1.0 + 0x1p-20));
+    }
 }
 
-// Waves per block, and waves per SIMD the register allocation must allow (the second launch bound):
-// eight-wave blocks, two per CU, where the accumulators leave room for four waves per SIMD; four-wave
-// blocks with two waves per SIMD above.
-__host__ __device__ constexpr int mfma_waves(int M) { return M <= 4 ? 8 : 4; }
-template <int M> struct MfmaCfg {
-    static constexpr int waves = mfma_waves(M);
-    static constexpr int waves_per_simd = M <= 4 ? 4 : 2;
-};
+// Waves per block, and waves per SIMD the register allocation must allow (the second launch bound:
+// four up to 96 lags, two above).
+// eight-wave blocks need enough (row pair, column block) units per wave: tiles below 64 take four
+// (measured: levels 2-8 of the full workload 8.8 -> 7.8 ms with four; level 1 7.0 -> 8.6 ms)
+__host__ __device__ constexpr int mfma_waves(int T, int M) { return M <= 4 && T >= 64 ? 8 : 4; }
 
 // LDS of one block. Region 0 holds the reference planes until the product loop ends, then the partial
 // products of half the waves and, after them, C; once C is formed, TT, the row bounds, D and W take the
@@ -175,7 +188,7 @@ struct MfmaLayout {
     size_t bytes;
 };
 
-__host__ __device__ inline MfmaLayout mfma_layout(int T, int M) {
+__host__ __device__ inline MfmaLayout mfma_layout(int T, int M, int W) {
     MfmaLayout L;
     const int S = 2 * M + 1, B = T + 2 * M, SS = S * S, MB = (SS + 15) / 16;
     L.S = S;
@@ -199,7 +212,7 @@ __host__ __device__ inline MfmaLayout mfma_layout(int T, int M) {
     o += al((size_t)kPairsPerChunk * SS * 4);
     L.off_w = o;
     o += al((size_t)kPairsPerChunk * SS * 4);
-    const size_t red = al((size_t)(mfma_waves(M) / 2) * MB * 256 * 4);
+    const size_t red = al((size_t)(W / 2) * MB * 256 * 4);
     L.off_c = o > red ? o : red;
     o = L.off_c + al((size_t)kPairsPerChunk * SS * 4);
     const size_t planes = 2 * al((size_t)L.PE * 4);
@@ -226,11 +239,11 @@ __device__ __forceinline__ unsigned halves(unsigned w1, unsigned w0, unsigned se
 }
 
 // chunk c of the level: up to 16 pair indices sharing one reference frame (-1 = empty slot)
-template <int PREC, int M>
-__global__ __launch_bounds__(MfmaCfg<M>::waves * 64, MfmaCfg<M>::waves_per_simd) void k_track_mfma(
+template <int PREC, int M, int W>
+__global__ __launch_bounds__(W * 64, (M <= 4 ? 4 : 2)) void k_track_mfma(
     TrackArgs a, XPlanes xp, const int* __restrict__ chunk_pairs, int n_chunks, int nblk) {
     constexpr int S = 2 * M + 1, SS = S * S, MB = (SS + 15) / 16, RO = S - 1;
-    constexpr int W = MfmaCfg<M>::waves, NT = W * 64;
+    constexpr int NT = W * 64;
     constexpr bool SPLIT = PREC != ISA_XC_FP32;
     // fp32 summation chains of the split products, as the registers allow: up to 32 lags, a chain per
     // track row pair for the hi.hi products and one chain for the products with a lo part; up to 96
@@ -250,7 +263,7 @@ __global__ __launch_bounds__(MfmaCfg<M>::waves * 64, MfmaCfg<M>::waves_per_simd)
     __shared__ int s_any_edge;                     // some pair's block leaves the image
     __shared__ int q_n, q_base, q_pair[kPairsPerChunk];
     const int T = a.tile;
-    const MfmaLayout L = mfma_layout(T, M);
+    const MfmaLayout L = mfma_layout(T, M, W);
     const int B = L.B, RW = L.RW, CO = L.CO, NB = L.NB, NP = L.NP, Bq = (B + 3) & ~3;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -776,7 +789,7 @@ __global__ __launch_bounds__(MfmaCfg<M>::waves * 64, MfmaCfg<M>::waves_per_simd)
         const int jj = (tid >> 4) & (kPairsPerChunk - 1), l = tid & 15;
         const int pp = tid < kPairsPerChunk * 16 ? cp[jj] : -1;
         const float* D = Dm + jj * SS;
-        const float* W = Wm + jj * SS;
+        const float* Wd = Wm + jj * SS;
         unsigned long long key = ~0ull;
         float mx = -3.402823466e+38f;
         int fin = 1;
@@ -785,7 +798,7 @@ __global__ __launch_bounds__(MfmaCfg<M>::waves * 64, MfmaCfg<M>::waves_per_simd)
                 const float v = D[s];
                 key = min(key, argmin_key(v, s));  // first index of the smallest value, as find_min_core
                 mx = fmaxf(mx, v);
-                fin &= isfinite(v) && isfinite(W[s]);
+                fin &= isfinite(v) && isfinite(Wd[s]);
             }
         for (int o = 8; o >= 1; o >>= 1) {
             key = min(key, (unsigned long long)__shfl_xor((long long)key, o, 16));
@@ -794,14 +807,14 @@ __global__ __launch_bounds__(MfmaCfg<M>::waves * 64, MfmaCfg<M>::waves_per_simd)
         }
         const int mi = key == ~0ull ? -1 : (int)(unsigned)key;
         const float mn = mi >= 0 ? D[mi] : 3.402823466e+38f;
-        const double hi = mi >= 0 ? (double)mn + W[mi] : 0.0, lo = mi >= 0 ? (double)mn - W[mi] : 0.0;
+        const double hi = mi >= 0 ? (double)mn + Wd[mi] : 0.0, lo = mi >= 0 ? (double)mn - Wd[mi] : 0.0;
         // every other lag lies strictly above the minimum's interval
         double lother = 1e300, lmax = -1e300, hmax = -1e300;
         if (pp >= 0 && mi >= 0)
             for (int s = l; s < SS; s += 16) {
-                const double lv = (double)D[s] - W[s];
+                const double lv = (double)D[s] - Wd[s];
                 lmax = fmax(lmax, lv);
-                hmax = fmax(hmax, (double)D[s] + W[s]);
+                hmax = fmax(hmax, (double)D[s] + Wd[s]);
                 if (s != mi) lother = fmin(lother, lv);
             }
         for (int o = 8; o >= 1; o >>= 1) {
@@ -870,12 +883,20 @@ int track_chunks_capacity(int m, int n) { return m / kPairsPerChunk + n + 1; }
 // LDS the kernel declares statically, next to its dynamic layout
 constexpr size_t kMfmaStaticLds = 1024;
 
+template <int PREC, int M, int W>
+static int launch_mfma_pmw(const TrackArgs& a, XPlanes xp, const int* chunks, int n_chunks, hipStream_t s) {
+    const MfmaLayout L = mfma_layout(a.tile, M, W);
+    const int blocks = a.cx * a.cy * n_chunks;
+    k_track_mfma<PREC, M, W><<<8 * div_up(blocks, 8), W * 64, L.bytes, s>>>(a, xp, chunks, n_chunks, blocks);
+    return check_launch("track_mfma");
+}
+
 template <int PREC, int M>
 static int launch_mfma_pm(const TrackArgs& a, XPlanes xp, const int* chunks, int n_chunks, hipStream_t s) {
-    const MfmaLayout L = mfma_layout(a.tile, M);
-    const int blocks = a.cx * a.cy * n_chunks;
-    k_track_mfma<PREC, M><<<8 * div_up(blocks, 8), MfmaCfg<M>::waves * 64, L.bytes, s>>>(a, xp, chunks, n_chunks, blocks);
-    return check_launch("track_mfma");
+    if constexpr (M <= 4) {
+        if (mfma_waves(a.tile, M) == 8) return launch_mfma_pmw<PREC, M, 8>(a, xp, chunks, n_chunks, s);
+    }
+    return launch_mfma_pmw<PREC, M, 4>(a, xp, chunks, n_chunks, s);
 }
 
 template <int PREC>
@@ -909,14 +930,19 @@ int split_planes(int precision, const float* src, int P, int w, int h, unsigned*
 // slide maxima (vbox_run_rows(h, T) x ceil(w / 64) floats, fewer than the same rows of w).
 int vbox_energies(const float* src, int pitch, int w, int h, int T, float* dst, float* runmax, hipStream_t s) {
     if (h < T) return ISA_OK;
-    k_vbox_sq<<<dim3(div_up(w, 256), div_up(h - T + 1, kVRun)), 256, 0, s>>>(src, pitch, w, h, T, dst, runmax);
+    if (T > kVTMax) {
+        set_error("track (mfma): tile %d above %d", T, kVTMax);
+        return ISA_ERR_UNSUPPORTED;
+    }
+    const size_t lds = (size_t)(kVRun + T - 1) * kCB * sizeof(float);
+    k_vbox_sq<<<dim3(div_up(w, kCB), div_up(h - T + 1, kVRun)), 256, lds, s>>>(src, pitch, w, h, T, dst, runmax);
     return check_launch("vbox_energies");
 }
 
 int vbox_run_rows(int h, int T) { return h < T ? 0 : div_up(h - T + 1, kVRun); }
 
 bool track_mfma_supported(int tile, int M) {
-    const MfmaLayout L = mfma_layout(tile, M);
+    const MfmaLayout L = mfma_layout(tile, M, mfma_waves(tile, M));
     return M >= 1 && M <= 8 && L.Bp / 16 <= kMaxNB && L.bytes + kMfmaStaticLds <= 160 * 1024;
 }
 
