@@ -262,6 +262,23 @@ __device__ __forceinline__ bool lk_solve_fp64(float fa, float fb, float fd, floa
     return true;
 }
 
+// FP32 solve of a window that lk_ill has passed (det > ill_ratio * trace^2 > 0 in exact products): G is
+// invertible, so the reference's closed-form pseudo-inverse V diag(1 / sigma) U^T is G^-1 = adj(G) / det,
+// formed here without the SVD's square roots and half angle. The gate is the same sigma1 = lambda1 of the
+// symmetric G; det and the adjugate products take one rounding each (fma), the reciprocal and the square
+// root are the hardware ones (about 1 ulp). NaN -> 0 as in the reference.
+__device__ __forceinline__ bool lk_solve_wc(float a, float b, float d, float b0, float b1, float min_det, float& u0,
+                                            float& u1) {
+    const float e = a - d;
+    const float lam1 = 0.5f * ((a + d) + __builtin_amdgcn_sqrtf(e * e + 4.0f * (b * b)));
+    if (lam1 < min_det) return false;
+    const float r = __builtin_amdgcn_rcpf(fmaf(a, d, -(b * b)));
+    const float v0 = fmaf(d, b0, -(b * b1)) * r, v1 = fmaf(a, b1, -(b * b0)) * r;
+    u0 = isnan(v0) ? 0.0f : v0;
+    u1 = isnan(v1) ? 0.0f : v1;
+    return true;
+}
+
 template <bool EXACT>
 __device__ __forceinline__ bool lk_pinv(float a, float b, float d, float min_det, float inv[4]) {
     float c = b;
@@ -425,8 +442,10 @@ __global__ __launch_bounds__(LK_BX* LK_BY) void k_lk_step_box(float* flow, int f
 //  C. the five window sums as separable box sums: per region row, sums along x over segments of 8
 //     output columns (from a register window of the row's derivatives), then per output column sums
 //     along y over 4 output rows, both by window_sums (no running subtraction);
-//  D. the closed-form solve and update, reading flow_in and writing flow_out (the halo of the
-//     warp reads neighbouring blocks' flow, so the flow is double-buffered across iterations).
+//  D. the solve and update, reading flow_in and writing flow_out (the halo of the warp reads
+//     neighbouring blocks' flow, so the flow is double-buffered across iterations). With an ill ratio
+//     set, windows that pass the FP64 conditioning test take the FP32 adjugate inverse (lk_solve_wc) and
+//     the others the FP64 solve; without one, every window takes the closed-form pseudo-inverse.
 constexpr int LKF_BX = 64, LKF_BY = 32, LKF_RPT = 4;  // 512 threads, 4 output rows each
 constexpr int LKF_SEG = 8;                            // output columns per horizontal task
 
@@ -746,9 +765,12 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         float f0 = fi[0], f1 = fi[1];
         float inv[4];
         if (px >= HW && px < w - HW && py >= HW && py < h - HW) {
-            if (ill_ratio > 0.0 && lk_ill(V[o][0], V[o][1], V[o][2], ill_ratio)) {
+            if (ill_ratio > 0.0) {
                 float u0, u1;
-                if (lk_solve_fp64(V[o][0], V[o][1], V[o][2], V[o][3], V[o][4], min_det, u0, u1)) {
+                const bool ok = lk_ill(V[o][0], V[o][1], V[o][2], ill_ratio)
+                                    ? lk_solve_fp64(V[o][0], V[o][1], V[o][2], V[o][3], V[o][4], min_det, u0, u1)
+                                    : lk_solve_wc(V[o][0], V[o][1], V[o][2], V[o][3], V[o][4], min_det, u0, u1);
+                if (ok) {
                     f0 += u0;
                     f1 += u1;
                 }
