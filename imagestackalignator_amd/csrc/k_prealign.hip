@@ -11,7 +11,8 @@
 // resolution (the second window centred on the first's maximum: a maximum that stays put is the largest
 // of its own 5 x 5 neighbourhood). Sums are FP64 per thread and reduced across blocks in a fixed order,
 // so the result is deterministic; nothing is allocated and the host never waits (the window centre of
-// each level is read on the device).
+// each level is read on the device). Up to PA_BATCH tracked frames share every launch: their lag tiles
+// are blocks of one grid and each frame keeps its own partial sums, maxima and window centres.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -32,26 +33,36 @@ constexpr int PA_MAX_LEVELS = 6;               // levels 1, 2, 4, 8, 16, 32
 constexpr int PA_COARSE_RADIUS = 8;            // lags each way at the coarsest level (before the slack)
 constexpr int PA_MIN_SIDE = 64;                // smallest side of the coarsest level
 constexpr int PA_MIN_ROWS = 32;                // smallest row chunk
+constexpr int PA_BATCH = 8;                    // tracked frames per launch
+constexpr int PA_BSTRIDE = 2 * PA_MAX_LEVELS;  // ints of one frame's maxima (x, y per level)
+
+struct FrameSet {
+    const float* p[PA_BATCH];  // per frame of the batch: its image at the level searched
+};
 
 struct LagWindow {
     int tiles;       // lag tiles per row and column (odd); lag tile (tx, ty) covers 5 x 5 lags
     int radius;      // the window is centre + [-radius, radius]^2, radius = (5 tiles - 1) / 2
 };
 
-// Partial c(s) over one (column block, row chunk) of the image for the 25 lags of lag tile blockIdx.z:
-// s = centre + o with o in the tile, centre = scale * (*centre_in) (the previous level's maximum), or 0.
+// Partial c(s) over one (column block, row chunk) of the image for the 25 lags of one lag tile of one
+// frame of the batch (blockIdx.z = frame * tiles + tile): s = centre + o with o in the tile, centre =
+// scale * (the frame's previous maximum, centre_in + frame * PA_BSTRIDE), or 0.
 // Thread x walks the rows of its column: each b row is loaded once (5 neighbouring columns) and meets the
 // 5 a rows of the tile's row lags, held in a ring of registers (zero outside the chunk), so a pixel costs
 // 6 loads for 25 FP64 products (float products are exact in FP64). Lags are reduced modulo the image
 // size first, so one wrap per coordinate suffices.
-__global__ __launch_bounds__(PA_THREADS) void k_cc_rows(const float* __restrict__ a, const float* __restrict__ b,
-                                                        int pitch, int w, int h, const int* centre_in, int scale,
-                                                        LagWindow win, int rows, double* partial) {
-    const int tile = blockIdx.z, ty = tile / win.tiles, tx = tile - ty * win.tiles;
+__global__ __launch_bounds__(PA_THREADS) void k_cc_rows(const float* __restrict__ a, FrameSet bs, int pitch, int w,
+                                                        int h, const int* centre_in, int scale, LagWindow win,
+                                                        int rows, double* partial, size_t part_stride) {
+    const int tiles = win.tiles * win.tiles, fb = blockIdx.z / tiles;
+    const int tile = blockIdx.z - fb * tiles, ty = tile / win.tiles, tx = tile - ty * win.tiles;
+    const float* __restrict__ b = bs.p[fb];
+    partial += fb * part_stride;
     int cx = 0, cy = 0;
     if (centre_in) {
-        cx = centre_in[0] * scale;
-        cy = centre_in[1] * scale;
+        cx = centre_in[fb * PA_BSTRIDE] * scale;
+        cy = centre_in[fb * PA_BSTRIDE + 1] * scale;
     }
     int sx0 = (cx - win.radius + PA_K * tx) % w, sy0 = (cy - win.radius + PA_K * ty) % h;
     if (sx0 < 0) sx0 += w;
@@ -141,11 +152,19 @@ __global__ __launch_bounds__(PA_THREADS) void k_cc_rows(const float* __restrict_
 // lag x). Each lag is summed by a group of G lanes (a power of two up to 64, so that a lane sums a few
 // chunks): lane j of the group sums chunks j, j + G, ... in order, the group is combined by a fixed
 // butterfly and its first lane's sum is the value. Writes the maximum's lag to best (2 ints, absolute lag)
-// and its value. partial is lag-major: [tile][lag][chunk].
+// and its value. partial is lag-major: [tile][lag][chunk]. One block per frame of the batch (blockIdx.x):
+// its partial sums at partial + frame * part_stride, its centre, maximum and value at frame * PA_BSTRIDE
+// (ints) and frame * PA_MAX_LEVELS (doubles).
 constexpr int PA_ARGMAX_THREADS = 1024;
-__global__ __launch_bounds__(PA_ARGMAX_THREADS) void k_cc_argmax(const double* partial, int chunks, int group,
-                                                                 const int* centre_in, int scale, LagWindow win,
-                                                                 int* best, double* best_value) {
+__global__ __launch_bounds__(PA_ARGMAX_THREADS) void k_cc_argmax(const double* partial, size_t part_stride,
+                                                                 int chunks, int group, const int* centre_in,
+                                                                 int scale, LagWindow win, int* best,
+                                                                 double* best_value) {
+    const int fb = blockIdx.x;
+    partial += fb * part_stride;
+    if (centre_in) centre_in += fb * PA_BSTRIDE;
+    best += fb * PA_BSTRIDE;
+    best_value += fb * PA_MAX_LEVELS;
     constexpr int waves = PA_ARGMAX_THREADS / 64;
     const int n = win.tiles * win.tiles * PA_LAGS;
     const int side = win.tiles * PA_K;
@@ -213,10 +232,15 @@ int argmax_group(int chunks) {
     return g;
 }
 
-// Final result of one frame: the full-resolution maximum s wrapped into (-W/2, W/2] x (-H/2, H/2] as
-// the reference does, returned negated (R:PEFStudioDX/PreAlignment.cs:248-258), with angle 0 and the
-// correlation value.
-__global__ void k_cc_result(const int* best, const double* value, int w, int h, float* out4) {
+// Final result of each frame of the batch (thread t): the full-resolution maximum s wrapped into
+// (-W/2, W/2] x (-H/2, H/2] as the reference does, returned negated (R:PEFStudioDX/PreAlignment.cs:248-258),
+// with angle 0 and the correlation value.
+__global__ void k_cc_result(const int* best, const double* value, int w, int h, float* out4, int nb) {
+    const int t = threadIdx.x;
+    if (t >= nb) return;
+    best += t * PA_BSTRIDE;
+    value += t * PA_MAX_LEVELS;
+    out4 += 4 * t;
     int x = best[0] % w, y = best[1] % h;
     if (x < 0) x += w;
     if (y < 0) y += h;
@@ -239,7 +263,8 @@ struct Plan {
     size_t pyr_bytes;                 // one pyramid (levels 2, 4, ...)
     LagWindow coarse;
     int rows[PA_MAX_LEVELS], row_chunks[PA_MAX_LEVELS];  // row chunk of the correlation blocks, and their count
-    size_t partial_bytes, bytes;      // workspace: two pyramids, partial sums, maxima
+    size_t partial_bytes, bytes;      // workspace: the reference's and PA_BATCH tracked frames' pyramids,
+                                      // partial sums, maxima (partial_bytes: one frame's partial sums)
 };
 
 size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -290,8 +315,8 @@ bool make_plan(int w, int h, int max_shift, Plan& P) {
         part = std::max(part, (size_t)tiles * cols * P.row_chunks[i] * PA_LAGS * sizeof(double));
     }
     P.partial_bytes = al256(part);
-    P.bytes = 2 * P.pyr_bytes + P.partial_bytes + al256((size_t)PA_MAX_LEVELS * 2 * sizeof(int)) +
-              al256(PA_MAX_LEVELS * sizeof(double));
+    P.bytes = (1 + PA_BATCH) * P.pyr_bytes + PA_BATCH * P.partial_bytes +
+              al256((size_t)PA_BATCH * PA_BSTRIDE * sizeof(int)) + al256((size_t)PA_BATCH * PA_MAX_LEVELS * sizeof(double));
     return true;
 }
 
@@ -312,10 +337,11 @@ int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int
     ISA_REQUIRE(img_pitch % 4 == 0 && img_pitch >= w * 4, "prealign: bad image pitch");
     char* ws = static_cast<char*>(workspace);
     char* pyr_ref = ws;
-    char* pyr_trk = ws + P.pyr_bytes;
-    double* partial = (double*)(ws + 2 * P.pyr_bytes);
-    int* best = (int*)(ws + 2 * P.pyr_bytes + P.partial_bytes);
-    double* best_value = (double*)(ws + 2 * P.pyr_bytes + P.partial_bytes + al256((size_t)PA_MAX_LEVELS * 2 * sizeof(int)));
+    char* pyr_trk = ws + P.pyr_bytes;  // PA_BATCH pyramids
+    double* partial = (double*)(ws + (1 + PA_BATCH) * P.pyr_bytes);
+    const size_t part_stride = P.partial_bytes / sizeof(double);
+    int* best = (int*)((char*)partial + PA_BATCH * P.partial_bytes);
+    double* best_value = (double*)((char*)best + al256((size_t)PA_BATCH * PA_BSTRIDE * sizeof(int)));
     auto level_ptr = [&](char* pyr, const float* full, int i) -> const float* {
         return i == 0 ? full : (const float*)(pyr + P.pyr_off[i]);
     };
@@ -331,27 +357,30 @@ int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int
     if (n < 2) return ISA_OK;
     ISA_TRY(build(pyr_ref, frames[0]));
     const int lc = P.levels - 1;
-    for (int f = 1; f < n; ++f) {
-        ISA_TRY(build(pyr_trk, frames[f]));
+    for (int f0 = 1; f0 < n; f0 += PA_BATCH) {
+        const int nb = std::min(PA_BATCH, n - f0);
+        for (int j = 0; j < nb; ++j) ISA_TRY(build(pyr_trk + j * P.pyr_bytes, frames[f0 + j]));
         for (int i = lc; i >= 0; --i) {
             const LagWindow win = window_of(P, i);
             const float* a = level_ptr(pyr_ref, frames[0], i);
-            const float* b = level_ptr(pyr_trk, frames[f], i);
+            FrameSet bs;
+            for (int j = 0; j < PA_BATCH; ++j)
+                bs.p[j] = level_ptr(pyr_trk + std::min(j, nb - 1) * P.pyr_bytes, frames[f0 + std::min(j, nb - 1)], i);
             const int pitch = i == 0 ? img_pitch : P.lpitch[i];
             const int cols = div_up(P.lw[i], PA_THREADS), chunks = cols * P.row_chunks[i];
             for (int pass = 0; pass < passes_of(P, i); ++pass) {
                 // centre: none at the coarsest level, twice the coarser maximum, then this level's own
                 const int* centre = i == lc ? nullptr : (pass == 0 ? best + 2 * (i + 1) : best + 2 * i);
                 const int scale = pass == 0 ? 2 : 1;
-                k_cc_rows<<<dim3(cols, P.row_chunks[i], win.tiles * win.tiles), PA_THREADS, 0, s>>>(
-                    a, b, pitch, P.lw[i], P.lh[i], centre, scale, win, P.rows[i], partial);
+                k_cc_rows<<<dim3(cols, P.row_chunks[i], win.tiles * win.tiles * nb), PA_THREADS, 0, s>>>(
+                    a, bs, pitch, P.lw[i], P.lh[i], centre, scale, win, P.rows[i], partial, part_stride);
                 ISA_TRY(check_launch("prealign: correlation"));
-                k_cc_argmax<<<1, PA_ARGMAX_THREADS, 0, s>>>(partial, chunks, argmax_group(chunks), centre, scale, win,
-                                                            best + 2 * i, best_value + i);
+                k_cc_argmax<<<nb, PA_ARGMAX_THREADS, 0, s>>>(partial, part_stride, chunks, argmax_group(chunks),
+                                                             centre, scale, win, best + 2 * i, best_value + i);
                 ISA_TRY(check_launch("prealign: maximum"));
             }
         }
-        k_cc_result<<<1, 1, 0, s>>>(best, best_value, w, h, shifts4 + 4 * f);
+        k_cc_result<<<1, 64, 0, s>>>(best, best_value, w, h, shifts4 + 4 * f0, nb);
         ISA_TRY(check_launch("prealign: result"));
     }
     return ISA_OK;

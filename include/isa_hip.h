@@ -386,7 +386,8 @@ int isa_erode_border4(const float* in4_dev, int in_pitch, float* out4_dev, int o
  * per axis, wrapped into (-W/2, W/2] x (-H/2, H/2]. shifts4_dev (device, frames x float4): (-s.x, -s.y, angle 0,
  * correlation value) per frame, frame 0 all zero: the initialShifts the PreAlignmentStore is built from
  * (isa_merger_run's base_shift). The rotation scan is not implemented (angle 0). frames_dev: host array of device
- * pointers; workspace_dev: isa_prealign_workspace() bytes, 256-byte aligned. No allocation, no host synchronisation. */
+ * pointers; workspace_dev: isa_prealign_workspace() bytes, 256-byte aligned (box pyramids of the reference and of
+ * the eight tracked frames searched per launch, about 12 B per pixel). No allocation, no host synchronisation. */
 size_t isa_prealign_workspace(int width, int height, int max_shift);
 int isa_prealign(const float* const* frames_dev, int img_pitch, int width, int height, int frames, int max_shift,
                  float* shifts4_dev, void* workspace_dev, isa_stream_t stream);
