@@ -243,10 +243,11 @@ ColsShape cols_shape(int T, int M) {
 }
 
 // per item: TT (B S doubles), the reference energy (1 double), D (S S floats, padded to 8 bytes),
-// the argmin key (8 bytes) and the max key (4 bytes, padded to 8)
+// the argmin key (8 bytes) and the max key (4 bytes, padded to 8), then the tree buffer: the T/2
+// partial sums over x of every lag (T/2 x S S doubles)
 __host__ __device__ inline size_t cols_item_bytes(int T, int M) {
     const int S = 2 * M + 1, B = T + 2 * M;
-    return (size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7) + 16;
+    return (size_t)B * S * 8 + 8 + (((size_t)S * S * 4 + 7) & ~(size_t)7) + 16 + (size_t)(T / 2) * S * S * 8;
 }
 
 size_t cols_smem(int T, int M, int items) { return (size_t)items * cols_item_bytes(T, M); }
@@ -271,6 +272,25 @@ __device__ __forceinline__ double tree_step(double v) {
         hi = __shfl_down(hi, ST);
     }
     return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Pairwise sum of N partial sums read from LDS at v[0], v[stride], ...: (v0 + v1) + (v2 + v3), ... by
+// halves, the order of the stride-doubling tree.
+template <int I0, int NN, int N>
+__device__ __forceinline__ double tree_lds_sum(const double (&v)[N]) {
+    if constexpr (NN == 1) return v[I0];
+    else return tree_lds_sum<I0, NN / 2>(v) + tree_lds_sum<I0 + NN / 2, NN / 2>(v);
+}
+template <int N>
+__device__ __forceinline__ double tree_lds(const double* p, int stride) {
+    if constexpr (N > 8) {
+        return tree_lds<N / 2>(p, stride) + tree_lds<N / 2>(p + (N / 2) * stride, stride);
+    } else {
+        double v[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = p[i * stride];
+        return tree_lds_sum<0, N>(v);
+    }
 }
 
 // minimum waves per SIMD the register allocation must allow: M >= 2 runs faster at 3 waves (with a
@@ -323,8 +343,9 @@ __device__ __forceinline__ void track_cols_group(const TrackArgs& a, int pairs, 
     double* TT = (double*)mine;
     double* R2tot = TT + B * S;
     float* Dm = (float*)(R2tot + 1);
-    unsigned long long* kmin = (unsigned long long*)(mine + item_bytes - 16);
+    unsigned long long* kmin = (unsigned long long*)((char*)Dm + (((size_t)S * S * 4 + 7) & ~(size_t)7));
     unsigned* kmax = (unsigned*)(kmin + 1);
+    double* tbuf = (double*)(kmin + 2);
     // block origins in the level image
     const int rx0 = tx * T + M + rox, ry0 = ty * T + M + roy;  // reference interior
     const int tx0 = tx * T + tox, ty0 = ty * T + toy;          // track block
@@ -446,15 +467,14 @@ __device__ __forceinline__ void track_cols_group(const TrackArgs& a, int pairs, 
     for (int q = 0; q < RPG; ++q)
 #pragma unroll
         for (int iy = 0; iy < S; ++iy) acc[q][iy] = fma(-2.0, acc[q][iy], TT[(x + g * RPG + q) * S + iy]);
-    // 3. pairwise trees over x (the tree of ssd_core: stride 1, 2, 4, ... < T)
+    // 3. pairwise trees over x (the tree of ssd_core: stride 1, 2, 4, ... < T). r2's runs on lane
+    // shuffles. For the lags, the first level (stride 1) runs on lane pairs; the T/2 partial sums of each
+    // lag then go through LDS, where one lane per lag adds them in the same pairwise order (partial p
+    // holds x = 2p, 2p + 1), so a lag's remaining levels cost one lane instead of every lane of the item.
     auto level = [&](auto st) {
         constexpr int ST = decltype(st)::value;
         if (ST >= T) return;
         r2 = tree_step<ST>(r2);
-#pragma unroll
-        for (int q = 0; q < RPG; ++q)
-#pragma unroll
-            for (int iy = 0; iy < S; ++iy) acc[q][iy] = tree_step<ST>(acc[q][iy]);
     };
     level(std::integral_constant<int, 1>{});
     level(std::integral_constant<int, 2>{});
@@ -462,20 +482,36 @@ __device__ __forceinline__ void track_cols_group(const TrackArgs& a, int pairs, 
     level(std::integral_constant<int, 8>{});
     level(std::integral_constant<int, 16>{});
     level(std::integral_constant<int, 32>{});
+#pragma unroll
+    for (int q = 0; q < RPG; ++q)
+#pragma unroll
+        for (int iy = 0; iy < S; ++iy) acc[q][iy] = tree_step<1>(acc[q][iy]);
     if (x == 0 && g == 0) {
         *R2tot = r2;
         *kmin = ~0ull;
         *kmax = ordered_bits(-3.402823466e+38f);
     }
-    __syncthreads();
-    if (x == 0) {
-        const double r2t = *R2tot;
+    // partial p of lag t = iy S + ix at tbuf[p S S + t]
+    if ((x & 1) == 0) {
+        double* wp = tbuf + (x >> 1) * (S * S) + g * RPG;
 #pragma unroll
-        for (int q = 0; q < RPG; ++q)
+        for (int iy = 0; iy < S; ++iy)
 #pragma unroll
-            for (int iy = 0; iy < S; ++iy) Dm[iy * S + g * RPG + q] = (float)(r2t + acc[q][iy]);
+            for (int q = 0; q < RPG; ++q) wp[iy * S + q] = acc[q][iy];
     }
-    __syncthreads();
+    __syncthreads();  // partials and R2tot complete
+    for (int t = lane; t < S * S; t += lpi) {  // one pass unless the item has fewer lanes than lags
+        double total;
+        switch (T) {
+            case 4: total = tree_lds<2>(tbuf + t, S * S); break;
+            case 8: total = tree_lds<4>(tbuf + t, S * S); break;
+            case 16: total = tree_lds<8>(tbuf + t, S * S); break;
+            case 32: total = tree_lds<16>(tbuf + t, S * S); break;
+            default: total = tree_lds<32>(tbuf + t, S * S); break;
+        }
+        Dm[t] = (float)(*R2tot + total);
+    }
+    __syncthreads();  // Dm complete
     // findMinimum's scan over the S x S values, spread over the item's lanes
     for (int i = lane; i < S * S; i += lpi) {
         const float v = Dm[i];
