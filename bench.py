@@ -413,9 +413,12 @@ def mixed_section(isa, torch, cfg, raw, out_exact, gt, psnr_gt_exact, steps, war
         prods = 1 if name == "fp32" else 3
         levels = []
         for i, (ms_launch, _) in enumerate(kt["track"]):
+            if kt["exact_queue"][i][1] == 0:  # xc_levels (default: 32-pixel tiles) left this level to the exact search
+                levels.append({"level": cfg["levels"][i][0], "search": "exact", "exact_ms": round(ms_launch, 3)})
+                continue
             fl = track_flops(cfg, i, m_pairs, prods)
             tf = fl / 1e9 / ms_launch if ms_launch > 0 else 0.0
-            levels.append({"level": cfg["levels"][i][0], "mfma_ms": round(ms_launch, 3),
+            levels.append({"level": cfg["levels"][i][0], "search": "mfma", "mfma_ms": round(ms_launch, 3),
                            "exact_queue_ms": round(kt["exact_queue"][i][0], 3),
                            "mfma_tflops": round(tf, 1), "mfma_util": round(tf / PEAK_MFMA_TFLOPS[name], 4)})
         ex_items = mm.exact_items()

@@ -68,6 +68,7 @@ class IsaParams(C.Structure):
         ("xc_precision", C.c_int),
         ("lk_ill_ratio", C.c_float),
         ("prealign_max_shift", C.c_int),
+        ("xc_levels", C.c_int),
     ]
 
     def set_levels(self, levels):

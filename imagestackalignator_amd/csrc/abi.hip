@@ -106,6 +106,7 @@ void isa_default_params(isa_params* p) {
     p->xc_precision = ISA_XC_FP64;
     p->lk_ill_ratio = 0.01f;
     p->prealign_max_shift = 0;
+    p->xc_levels = -1;
 }
 
 }  // extern "C"

@@ -49,6 +49,11 @@ int lucas_kanade(float* src, const float* tgt, int img_pitch, const float* tiles
                  float rot, float min_det, int window, int quant, bool box, hipStream_t s, const void* taps,
                  float ill_ratio, hipEvent_t* iter_ev);
 size_t tap_table_bytes(int w, int h);
+// Pyramid level i of a merge with these parameters runs the matrix-core tile search (otherwise the exact one).
+inline bool xc_at(const isa_params& P, int i) {
+    if (P.xc_precision == ISA_XC_FP64) return false;
+    return P.xc_levels < 0 ? P.tile_size[i] == 32 : ((P.xc_levels >> i) & 1) != 0;
+}
 int track_chunks(const int* pa, int m, int n, int* out_chunks);
 int track_chunks_capacity(int m, int n);
 int launch_track_mfma(int precision, const float* const* frames, const unsigned* const* xplanes,
@@ -286,6 +291,7 @@ int validate(const isa_params& P, int w, int h, int n) {
                 "merger: prealign_max_shift must be in [0, min(width, height) / 2]");
     for (int i = 0; i < 4; ++i) ISA_REQUIRE(P.cfa[i] >= 0 && P.cfa[i] <= 2, "merger: CFA entries must be 0, 1 or 2");
     ISA_REQUIRE(P.xc_precision >= ISA_XC_FP64 && P.xc_precision <= ISA_XC_FP32, "merger: unknown xc_precision");
+    ISA_REQUIRE(P.xc_levels >= -1, "merger: xc_levels must be a bit mask of levels or -1");
     if (P.xc_precision != ISA_XC_FP64)
         for (int i = 0; i < P.n_levels; ++i)
             if (!isa::track_mfma_supported(P.tile_size[i], P.max_shift[i])) {
@@ -354,7 +360,7 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
         size_t tot = 0;
         for (int i = 0; i < M->L; ++i) {
             M->xplane_off[i] = tot;
-            tot += (size_t)frames * (width / P.resize_level[i]) * (height / P.resize_level[i]);
+            if (isa::xc_at(P, i)) tot += (size_t)frames * (width / P.resize_level[i]) * (height / P.resize_level[i]);
         }
         A(M->xplanes, tot);
     }
@@ -363,7 +369,7 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
         for (int i = 0; i < M->L; ++i) {
             const int lw = width / P.resize_level[i], lh = height / P.resize_level[i];
             M->vbox_off[i] = tot;
-            M->vbox_len[i] = (size_t)std::max(lh - P.tile_size[i] + 1, 1) * lw;
+            M->vbox_len[i] = isa::xc_at(P, i) ? (size_t)std::max(lh - P.tile_size[i] + 1, 1) * lw : 0;
             tot += (size_t)frames * M->vbox_len[i];
         }
         A(M->vbox, tot);
@@ -371,7 +377,7 @@ int isa_merger_create(isa_merger** out, int width, int height, int frames, const
         for (int i = 0; i < M->L; ++i) {
             const int lw = width / P.resize_level[i], lh = height / P.resize_level[i];
             M->vrun_off[i] = tot;
-            M->vrun_len[i] = (size_t)std::max(isa::vbox_run_rows(lh, P.tile_size[i]), 1) * lw;
+            M->vrun_len[i] = isa::xc_at(P, i) ? (size_t)std::max(isa::vbox_run_rows(lh, P.tile_size[i]), 1) * lw : 0;
             tot += (size_t)frames * M->vrun_len[i];
         }
         A(M->vrun, tot);
@@ -602,12 +608,13 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
             for (int f = 0; f < n; ++f)
                 ISA_TRY(isa::launch_resize_box(M->trk + (size_t)f * np, wp, M->lev[i] + (size_t)f * lw * lh,
                                                lw * (int)sizeof(float), w, h, l, s));
-        if (M->xplanes)
+        const bool xc = isa::xc_at(P, i);
+        if (M->xplanes && xc)
             for (int f = 0; f < n; ++f)
                 ISA_TRY(isa::split_planes(P.xc_precision,
                                           l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np, lw, lw,
                                           lh, M->xplanes + M->xplane_off[i] + (size_t)f * lw * lh, s));
-        if (M->vbox)
+        if (M->vbox && xc)
             for (int f = 0; f < n; ++f) {
                 if (!tracked[f]) continue;
                 ISA_TRY(isa::vbox_energies(l > 1 ? M->lev[i] + (size_t)f * lw * lh : M->trk + (size_t)f * np,
@@ -616,7 +623,7 @@ int isa_merger_run(isa_merger* M, const uint16_t* raw_dev, const float* base_shi
                                            M->vrun + M->vrun_off[i] + (size_t)f * M->vrun_len[i], s));
             }
         (void)hipEventRecord(M->lev_ev[i][1], s);
-        if (P.xc_precision == ISA_XC_FP64) {
+        if (!xc) {
             ISA_TRY(isa::launch_track_level(d_frames + (size_t)i * n, lw * (int)sizeof(float), lw, lh, P.tile_size[i],
                                             P.max_shift[i], M->cx[i], M->cy[i], d_pa, d_pb, d_pk, m,
                                             d_base + (size_t)i * n * 4, M->shifts, mstride, P.matching_threshold,
@@ -791,7 +798,7 @@ int isa_merger_kernel_ms(isa_merger* M, int kernel, int level, float* ms, int* l
         for (int i = 0; i < M->L; ++i) {
             if (level >= 0 && i != level) continue;
             tot += kernel == 1 ? el(M->lev_ev[i][1], M->lev_ev[i][4]) : el(M->lev_ev[i][4], M->lev_ev[i][2]);
-            cnt += kernel == 1 || M->P.xc_precision != ISA_XC_FP64 ? 1 : 0;
+            cnt += kernel == 1 || isa::xc_at(M->P, i) ? 1 : 0;
         }
     } else {  // accumulation of every frame
         tot = el(M->ev[ST_ACCUM], M->ev[ST_ACCUM + 1]);
@@ -806,7 +813,7 @@ int isa_merger_exact_items(isa_merger* M, int level, int* exact_items, int* item
     ISA_REQUIRE(M && exact_items && items && level >= 0 && level < M->L, "isa_merger_exact_items: bad arguments");
     if (!M->have_times) return ISA_ERR_ARG;
     *items = M->m_last * M->cx[level] * M->cy[level];
-    if (!M->xc_count) {
+    if (!M->xc_count || !isa::xc_at(M->P, level)) {
         *exact_items = *items;  // the exact search ran on every item
         return ISA_OK;
     }

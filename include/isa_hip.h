@@ -125,6 +125,11 @@ typedef struct isa_params {
     int prealign_max_shift;           /* 0 (SkipPreAlignment): base shifts from the caller; > 0: when the caller
                                          passes none, ComputePreAlignment's translation search (isa_prealign) within
                                          this radius in pixels sets them */
+    int xc_levels;                    /* with xc_precision other than ISA_XC_FP64: bit i set = pyramid level i runs
+                                         the matrix-core search, the other levels the exact one; -1 (default): the
+                                         levels with 32-pixel tiles, where the matrix-core search is the faster one
+                                         (with 64- and 16-pixel tiles the exact search is as fast or faster). No
+                                         counterpart in the reference. */
 } isa_params;
 
 void isa_default_params(isa_params* p);

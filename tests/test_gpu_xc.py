@@ -112,19 +112,25 @@ def _psnr(x, y):
     return 99.0 if mse == 0 else 10 * np.log10(1.0 / mse)
 
 
+@pytest.mark.parametrize("xc_levels", [-1, 7])
 @pytest.mark.parametrize("prec", list(XC))
-def test_merge_xc_against_exact_and_ground_truth(isa, gpu, prec):
+def test_merge_xc_against_exact_and_ground_truth(isa, gpu, prec, xc_levels):
+    """xc_levels -1 (default): the 32-pixel levels on the matrix cores, the 16-pixel level exact; 7: every
+    level on the matrix cores."""
     w, h, n = 320, 256, 5
     cfg = isa.default_synth(seed=91, width=w, height=h, frames=n, global_shift_max=6.0, occluded_fraction=0.03)
     raw_t, _ = isa.synth_burst(cfg)
     levels = [(1, 32, 2), (2, 32, 4), (4, 16, 4)]
+    on_mfma = [True, True, xc_levels == 7]
     res = {}
     for name, code in [("exact", 0), (prec, XC[prec])]:
-        m = isa.BurstMerger(w, h, n, isa.default_params(levels=levels, xc_precision=code))
+        m = isa.BurstMerger(w, h, n, isa.default_params(levels=levels, xc_precision=code, xc_levels=xc_levels))
         res[name] = (host(m.run(raw_t)).copy(), host(m.buffer_tensor(2, 0, 2)).copy(), m.reference_index,
                      m.exact_items())
         for level, (n_exact, items) in enumerate(res[name][3]):
             assert 0 <= n_exact <= items and (code != 0 or n_exact == items), (level, n_exact, items)
+            if code != 0:  # a level on the matrix cores certifies most items; the others are all searched exactly
+                assert (n_exact < items) == on_mfma[level], (level, n_exact, items)
     gt = host(isa.ground_truth(cfg, res["exact"][2], w, h))
     sl = (slice(4, -4), slice(4, -4))
     out_x, out_e = res[prec][0][sl], res["exact"][0][sl]
