@@ -516,6 +516,15 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     const int x0 = blockIdx.x * LKF_BX, y0 = blockIdx.y * LKF_BY;
     const int wx0 = x0 - HW - 3, wy0 = y0 - HW - 3;
     const double iw = 1.0 / (double)w, ih = 1.0 / (double)h;
+    // flow_in at this thread's output pixels (read by phase D), loaded first so that it has arrived by then
+    // (clamped to the image; rows and columns past it are not written)
+    float2 fpre[LKF_RPT];
+    {
+        const int pxp = min(x0 + tid % LKF_BX, w - 1), oyp = y0 + (tid / LKF_BX) * LKF_RPT;
+#pragma unroll
+        for (int o = 0; o < LKF_RPT; ++o)
+            fpre[o] = *(const float2*)(row_ptr(flow_in, flow_pitch, min(oyp + o, h - 1)) + pxp * 2);
+    }
     if (tid == 0) {
         sm.mixed = 0;
         sm.ident = 1;
@@ -761,8 +770,7 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
     for (int o = 0; o < LKF_RPT; ++o) {
         const int py = y0 + oy0 + o;
         if (py >= h) break;
-        const float* fi = row_ptr(flow_in, flow_pitch, py) + px * 2;
-        float f0 = fi[0], f1 = fi[1];
+        float f0 = fpre[o].x, f1 = fpre[o].y;
         float inv[4];
         if (px >= HW && px < w - HW && py >= HW && py < h - HW) {
             if (ill_ratio > 0.0) {
