@@ -525,25 +525,19 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         for (int o = 0; o < LKF_RPT; ++o)
             fpre[o] = *(const float2*)(row_ptr(flow_in, flow_pitch, min(oyp + o, h - 1)) + pxp * 2);
     }
-    if (tid == 0) {
-        sm.mixed = 0;
-        sm.ident = 1;
-    }
-    __syncthreads();
-    // tap tables of the block's derivative region, translated to warped-tile coordinates
-    for (int i = tid; i < (DW + DH) * 5; i += NT) {
+    // this thread's entries of the tap tables of the block's derivative region, loaded before the first
+    // barrier; the flow texels (WarpingKernel reads the flow by point sampling at ((x + 0.5) / w,
+    // (y + 0.5) / h)) go to LDS meanwhile
+    constexpr int NTAP = ((DW + DH) * 5 + NT - 1) / NT;
+    Tap tpre[NTAP];
+#pragma unroll
+    for (int j = 0; j < NTAP; ++j) {
+        const int i = min(tid + j * NT, (DW + DH) * 5 - 1);
         const bool is_c = i < DW * 5;
         const int e = is_c ? i : i - DW * 5, pix = e / 5, k = e - pix * 5;
-        const int g = (is_c ? x0 : y0) - HW + pix, n = is_c ? w : h, o = is_c ? wx0 : wy0, lim = is_c ? WC : WR;
-        const Tap t = (is_c ? col : row)[(size_t)clampi(g, 0, n - 1) * 5 + k];
-        LTap l = {clampi(t.i0 - o, 0, lim - 1), clampi(t.i1 - o, 0, lim - 1), t.a, 0};
-        l.p = t.a == 0.0f ? l.c0 : l.c1;
-        if (t.a != 0.0f && t.a != 1.0f) sm.mixed = 1;
-        if (l.p != pix + k + 1) sm.ident = 0;  // tap k of region pixel pix is warped-tile index pix + 1 + k
-        if (is_c) sm.u.ab.Tc[pix][k] = l;
-        else sm.u.ab.Tr[pix][k] = l;
+        const int g = (is_c ? x0 : y0) - HW + pix, n = is_c ? w : h;
+        tpre[j] = (is_c ? col : row)[(size_t)clampi(g, 0, n - 1) * 5 + k];
     }
-    // flow texels: WarpingKernel reads the flow by point sampling at ((x + 0.5) / w, (y + 0.5) / h)
     for (int i = tid; i < WC + WR; i += NT) {
         const bool is_c = i < WC;
         const int g = is_c ? clampi(wx0 + i, 0, w - 1) : clampi(wy0 + i - WC, 0, h - 1);
@@ -552,6 +546,27 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         const int t = tex_addr((int)floorf(u * (float)n), n, ISA_ADDR_CLAMP);
         if (is_c) sm.u.ab.fx[i] = t;
         else sm.u.ab.fy[i - WC] = t;
+    }
+    if (tid == 0) {
+        sm.mixed = 0;
+        sm.ident = 1;
+    }
+    __syncthreads();
+    // the tap tables, translated to warped-tile coordinates
+#pragma unroll
+    for (int j = 0; j < NTAP; ++j) {
+        const int i = tid + j * NT;
+        if (i >= (DW + DH) * 5) break;
+        const Tap t = tpre[j];
+        const bool is_c = i < DW * 5;
+        const int e = is_c ? i : i - DW * 5, pix = e / 5, k = e - pix * 5;
+        const int o = is_c ? wx0 : wy0, lim = is_c ? WC : WR;
+        LTap l = {clampi(t.i0 - o, 0, lim - 1), clampi(t.i1 - o, 0, lim - 1), t.a, 0};
+        l.p = t.a == 0.0f ? l.c0 : l.c1;
+        if (t.a != 0.0f && t.a != 1.0f) sm.mixed = 1;
+        if (l.p != pix + k + 1) sm.ident = 0;  // tap k of region pixel pix is warped-tile index pix + 1 + k
+        if (is_c) sm.u.ab.Tc[pix][k] = l;
+        else sm.u.ab.Tr[pix][k] = l;
     }
     __syncthreads();
     // A. warped source and target. All flow loads first, then the texel coordinates of every element.
