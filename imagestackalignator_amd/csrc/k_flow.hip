@@ -568,7 +568,8 @@ __global__ __launch_bounds__(LKF_BX*(LKF_BY / LKF_RPT)) void k_lk_fused(
         if (is_c) sm.u.ab.Tc[pix][k] = l;
         else sm.u.ab.Tr[pix][k] = l;
     }
-    __syncthreads();
+    // (phase A reads only the flow texels, complete at the barrier above; the tap tables and flags are read
+    // after the barrier that ends phase A)
     // A. warped source and target. All flow loads first, then the texel coordinates of every element.
     // When the 2x2 footprint of every element of the wave lies inside the image (the common case)
     // the texel indices are the coordinates themselves and all gathers are issued together from
