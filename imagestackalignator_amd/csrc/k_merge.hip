@@ -404,12 +404,18 @@ __global__ __launch_bounds__(256) void k_accumulate_frames_wide(float* sum, floa
 #pragma unroll
         for (int px = -2; px <= 2; ++px) wt[(py + 2) * 5 + px + 2] = kernel_weight(px, py, k[0], k[1], k[2]);
     const bool in_xy = x >= 2 && x <= w - 3 && y >= 2 && y <= h - 3;
+    // each frame's flow at the pixel is loaded one frame ahead, so the sample rows it addresses do not wait
+    // for a load issued in the same iteration
+    float2 fnext = make_float2(0.0f, 0.0f);
+    if (fa.nframes > 0 && fa.flow[0]) fnext = *(const float2*)(row_ptr(fa.flow[0], flow_pitch, y) + x * 2);
     for (int f = 0; f < fa.nframes; ++f) {
         int sx = 0, sy = 0;
+        const float2 fcur = fnext;
+        if (f + 1 < fa.nframes && fa.flow[f + 1])
+            fnext = *(const float2*)(row_ptr(fa.flow[f + 1], flow_pitch, y) + x * 2);
         if (fa.flow[f]) {
-            const float* fl = row_ptr(fa.flow[f], flow_pitch, y) + x * 2;
-            sx = (int)roundf(fl[0]);
-            sy = (int)roundf(fl[1]);
+            sx = (int)roundf(fcur.x);
+            sy = (int)roundf(fcur.y);
         }
         const int qx = x + sx, qy = y + sy;
         if (in_xy && qx >= 2 && qx <= w - 3 && qy >= 2 && qy <= h - 3) {
@@ -625,10 +631,12 @@ int launch_accumulate_frames(float* sum, float* wsum, const float* kern, int w, 
     // raw, mask, flow: host arrays of device pointers, frames in accumulation order; launches of up to
     // ACC_MAXF frames run in order on the stream, each continuing the sums of the previous one
     const AccArgs a = make_acc(white, black, cfa);
-    // wide form: raw frames 4-byte aligned with an even width, mask rows 16-byte aligned
-    bool wide = w % 2 == 0 && mask_pitch % 16 == 0;
+    // wide form: raw frames 4-byte aligned with an even width, mask rows 16-byte aligned, flow pixels (float2)
+    // 8-byte aligned
+    bool wide = w % 2 == 0 && mask_pitch % 16 == 0 && flow_pitch % 8 == 0;
     for (int f = 0; f < nframes && wide; ++f)
-        wide = reinterpret_cast<uintptr_t>(raw[f]) % 4 == 0 && reinterpret_cast<uintptr_t>(mask[f]) % 16 == 0;
+        wide = reinterpret_cast<uintptr_t>(raw[f]) % 4 == 0 && reinterpret_cast<uintptr_t>(mask[f]) % 16 == 0 &&
+               reinterpret_cast<uintptr_t>(flow[f]) % 8 == 0;
     auto kern_fn = a.white_int ? k_accumulate_frames<true> : k_accumulate_frames<false>;
     auto wide_fn = a.white_int ? k_accumulate_frames_wide<true> : k_accumulate_frames_wide<false>;
     for (int f0 = 0; f0 < nframes; f0 += ACC_MAXF) {
