@@ -45,6 +45,22 @@ struct LagWindow {
     int radius;      // the window is centre + [-radius, radius]^2, radius = (5 tiles - 1) / 2
 };
 
+// 2 x 2 box means of every frame of a batch (frame blockIdx.z: in.p[z] -> out + z * out_stride floats): the
+// sums (rows in order, columns in order, from 0) and the division of k_resize_box's 2 x 2 box, so the
+// values are identical to its.
+__global__ void k_box2_batch(FrameSet in, int in_pitch, float* out, size_t out_stride, int out_pitch, int ow, int oh) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= ow || y >= oh) return;
+    const float* r0 = row_ptr(in.p[blockIdx.z], in_pitch, 2 * y) + 2 * x;
+    const float* r1 = row_ptr(in.p[blockIdx.z], in_pitch, 2 * y + 1) + 2 * x;
+    float s = 0.0f;
+    s += r0[0];
+    s += r0[1];
+    s += r1[0];
+    s += r1[1];
+    row_ptr(out + blockIdx.z * out_stride, out_pitch, y)[x] = s / 4.0f;
+}
+
 // Partial c(s) over one (column block, row chunk) of the image for the 25 lags of one lag tile of one
 // frame of the batch (blockIdx.z = frame * tiles + tile): s = centre + o with o in the tile, centre =
 // scale * (the frame's previous maximum, centre_in + frame * PA_BSTRIDE), or 0.
@@ -359,7 +375,18 @@ int prealign(const float* const* frames, int img_pitch, int w, int h, int n, int
     const int lc = P.levels - 1;
     for (int f0 = 1; f0 < n; f0 += PA_BATCH) {
         const int nb = std::min(PA_BATCH, n - f0);
-        for (int j = 0; j < nb; ++j) ISA_TRY(build(pyr_trk + j * P.pyr_bytes, frames[f0 + j]));
+        // the batch's pyramids, one launch per level for all its frames
+        for (int i = 1; i < P.levels; ++i) {
+            FrameSet src;
+            for (int j = 0; j < PA_BATCH; ++j) {
+                const int jj = std::min(j, nb - 1);
+                src.p[j] = level_ptr(pyr_trk + jj * P.pyr_bytes, frames[f0 + jj], i - 1);
+            }
+            k_box2_batch<<<dim3(div_up(P.lw[i], 32), div_up(P.lh[i], 8), nb), dim3(32, 8), 0, s>>>(
+                src, i == 1 ? img_pitch : P.lpitch[i - 1], (float*)(pyr_trk + P.pyr_off[i]), P.pyr_bytes / sizeof(float),
+                P.lpitch[i], P.lw[i], P.lh[i]);
+            ISA_TRY(check_launch("prealign: pyramid"));
+        }
         for (int i = lc; i >= 0; --i) {
             const LagWindow win = window_of(P, i);
             const float* a = level_ptr(pyr_ref, frames[0], i);
